@@ -1,0 +1,108 @@
+"""ctypes binding of the HIP library (athad_cond_amd/libathad_hip.so).
+
+The library is the product: every operator runs on the GPU through the C-ABI declared in
+include/athad_hip.h. There is no CPU fallback; loading fails loudly when the library has
+not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libathad_hip.so")
+
+FIELDS = [
+    "t", "u", "v", "w", "c", "cloud", "ice", "gr", "co2", "tke", "dis",
+    "tn", "un", "vn", "wn", "cn", "cloudn", "icen", "grn", "co2n", "tken", "disn",
+    "p_dyn", "p_stat", "r_humid", "r_dry", "t_eq", "nue", "prod", "tke_source",
+    "dis_source", "aux_u", "aux_v", "aux_w", "S_c_c",
+]
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+FIELDS2D = ["vel_star", "t_top_init"]
+PROGNOSTIC = FIELDS[:11]
+
+FP64, MIXED = 0, 1
+LEX, JACOBI = 0, 1
+
+KERNEL_ID = {"rk4": 0, "rk4_plane": 1, "level_mean": 2, "poisson": 3, "satadj": 4, "damp": 5,
+             "densities": 6, "turb": 7, "bc": 8, "solid": 9, "polar": 10, "radial": 11,
+             "store": 12, "init": 13}
+
+# Exported entry points (include/athad_hip.h)
+EXPORTS = [
+    "athad_create", "athad_destroy", "athad_last_error", "athad_footprint_bytes", "athad_init_state",
+    "athad_set_field", "athad_get_field", "athad_set_field2d", "athad_get_field2d", "athad_upload_rows",
+    "athad_download_rows", "athad_set_iter", "athad_get_iter", "athad_poisson_sweep", "athad_satadj",
+    "athad_damp_moist", "athad_densities", "athad_turbulence", "athad_boundary", "athad_rk4",
+    "athad_filters", "athad_store", "athad_step", "athad_run", "athad_sync", "athad_profile",
+    "athad_profile_result", "athad_version",
+]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("im", ctypes.c_int), ("jm", ctypes.c_int), ("km", ctypes.c_int)]
+
+
+class MemberParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "t_surf_equator", "t_surf_pole", "r_air", "x_H2O", "x_CO2", "x_N2",
+        "c_0", "co2_0", "c_h2o_dry_top")]
+
+
+def default_params() -> MemberParams:
+    """cli/config_cond.xml values."""
+    return MemberParams(513.15, 503.15, 40.8, 0.5578, 0.4109, 0.0313, 0.3464, 0.6233, 0.00423)
+
+
+_lib = None
+
+
+def lib():
+    """Loads the library; raises when it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"athad_cond_amd: HIP library not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.athad_create.argtypes = [i, ctypes.POINTER(Grid), ctypes.POINTER(MemberParams), i, i,
+                                   ctypes.POINTER(vp)]
+        L.athad_destroy.argtypes = [vp]
+        L.athad_last_error.restype = ctypes.c_char_p
+        L.athad_last_error.argtypes = [vp]
+        L.athad_footprint_bytes.restype = ctypes.c_int64
+        L.athad_footprint_bytes.argtypes = [ctypes.POINTER(Grid), i]
+        L.athad_init_state.argtypes = [vp, ctypes.c_uint64]
+        for name in ("athad_set_field", "athad_get_field", "athad_set_field2d", "athad_get_field2d"):
+            getattr(L, name).argtypes = [vp, i, i, dp]
+        L.athad_upload_rows.argtypes = [vp, i, i, ctypes.POINTER(dp)]
+        L.athad_download_rows.argtypes = [vp, i, i, ctypes.POINTER(dp)]
+        L.athad_set_iter.argtypes = [vp, i, i, i]
+        L.athad_get_iter.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i)]
+        for name in ("athad_poisson_sweep", "athad_satadj", "athad_damp_moist", "athad_turbulence",
+                     "athad_boundary", "athad_rk4", "athad_filters", "athad_store", "athad_step",
+                     "athad_sync"):
+            getattr(L, name).argtypes = [vp]
+        L.athad_densities.argtypes = [vp, i, i]
+        L.athad_run.argtypes = [vp, i, i]
+        L.athad_profile.argtypes = [vp, i, i]
+        L.athad_profile_result.argtypes = [vp, i, ctypes.POINTER(d), ctypes.POINTER(i)]
+        L.athad_version.restype = ctypes.c_char_p
+        for name in EXPORTS:
+            if name not in ("athad_last_error", "athad_footprint_bytes", "athad_version"):
+                getattr(L, name).restype = i
+        _lib = L
+    return _lib
+
+
+class AthadError(RuntimeError):
+    pass
+
+
+def check(status, ctx=None):
+    if status != 0:
+        msg = lib().athad_last_error(ctx)
+        raise AthadError(f"athad status {status}: {msg.decode() if msg else ''}")

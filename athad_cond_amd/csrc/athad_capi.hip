@@ -1,0 +1,784 @@
+// C-ABI of the ATHAD_COND HIP library: context lifetime, state transfer, the per-operator
+// entry points and the scoped iteration (include/athad_hip.h).
+//
+// Host-side grid and constant setup restates cAtmosphereModel's (cAtmosphereModel.cpp:44-60,
+// 327-399; cAtmosphereModel.h:441-469) so device tables match the reference bit for bit.
+#include "../../include/athad_hip.h"
+#include "athad_device.h"
+#include "athad_kernels.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace athad;
+
+namespace {
+
+std::string g_create_error;
+
+// ---- configuration values (cli/config_cond.xml) ----------------------------------------
+struct Config {
+    double omega = 0.000317, g = 9.8066, r_Earth = 6370.001, t_0 = 273.15, t_skin = 254.0;
+    double dt_visc = 4e-05, L_atm = 6287.5, zeta = 3.0, sc_WaterVapour = 0.61, sc_CO2 = 0.96;
+    double pr_turb = 0.9, abl_height = 10000.0, u_0 = 8.0, cloud_cap = 0.05, t_max_phys = 2000.0;
+    double t_00 = 236.15, cp_l = 1349.0, co2_scale = 1.0;
+};
+
+athad_member_params default_member() {
+    athad_member_params p;
+    p.t_surf_equator = 513.15; p.t_surf_pole = 503.15; p.r_air = 40.8;
+    p.x_H2O = 0.5578; p.x_CO2 = 0.4109; p.x_N2 = 0.0313;
+    p.c_0 = 0.3464; p.co2_0 = 0.6233; p.c_h2o_dry_top = 0.00423;
+    return p;
+}
+
+// AtmMixture::resolve and M_nonwater (MixtureAtm.h:82-109, :152-217), host copies.
+struct Comp { double M_mean, R_mix, q_CO2, q_bg, M_bg, R_bg; };
+constexpr double hRstar = 8.314462618, hMH2O = 0.018015, hMCO2 = 0.044010, hMN2 = 0.028014;
+Comp resolve(double xh, double xc, double xn) {
+    Comp C;
+    const double mh = xh * hMH2O, mc = xc * hMCO2, mb = xn * hMN2;
+    C.M_mean = mh + mc + mb;
+    C.R_mix = hRstar / C.M_mean;
+    C.q_CO2 = mc / C.M_mean;
+    C.q_bg = mb / C.M_mean;
+    C.M_bg = (xn > 0.0) ? (mb / xn) : hMN2;
+    C.R_bg = hRstar / C.M_bg;
+    return C;
+}
+double M_nonwater_h(double c, double co2, double Mb) {
+    double qv = std::min(std::max(c, 0.0), 1.0), qc = std::min(std::max(co2, 0.0), 1.0);
+    const double s = qv + qc;
+    if (s > 1.0) { qv /= s; qc /= s; }
+    const double qb = std::max(0.0, 1.0 - qv - qc);
+    const double t = qc + qb;
+    if (!(t > 0.0)) return Mb;
+    const double inv = (qc / t) / hMCO2 + (qb / t) / Mb;
+    return (inv > 0.0) ? 1.0 / inv : Mb;
+}
+
+struct MemberHost {
+    athad_grid g{};
+    athad_member_params p{};
+    Consts K{};
+    size_t n = 0;
+    void* field[NFIELD] = {};
+    void* alt[NPROG] = {};
+    void* tmp[NTMP] = {};
+    double *vel_star = nullptr, *t_top = nullptr, *t_ref = nullptr, *tables = nullptr, *scratch = nullptr;
+    int* counters = nullptr;
+    int* passes = nullptr;
+    int iter_n = 0, total = 0;     // host mirror of the device counters
+    // table offsets inside 'tables'
+    size_t o_rad = 0, o_sint = 0, o_cost = 0, o_sinr = 0, o_hgt = 0, o_nuef = 0, o_step = 0;
+};
+
+}  // namespace
+
+struct athad_ctx {
+    int nmem = 0, precision = ATHAD_FP64, order = ATHAD_ORDER_JACOBI;
+    std::vector<MemberHost> mem;
+    MemberDesc* d_md[2] = {nullptr, nullptr};
+    std::vector<MemberDesc> h_md[2];
+    int parity = 0;
+    hipStream_t stream = nullptr;
+    Batch batch;
+    std::string err;
+    // profiling
+    bool prof_on[KID_COUNT] = {};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[KID_COUNT];
+    double prof_ms[KID_COUNT] = {};
+    int prof_n[KID_COUNT] = {};
+    bool capturing = false;
+    // 4-iteration graph
+    hipGraphExec_t graph = nullptr;
+    int graph_phase = -1, graph_parity = -1;
+    std::vector<double> staging;   // host staging for conversions
+};
+
+namespace {
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                 \
+            return ATHAD_ERR_DEVICE;                                                        \
+        }                                                                                   \
+    } while (0)
+
+size_t elem_size(const athad_ctx* c) { return c->precision == ATHAD_MIXED ? 4 : 8; }
+size_t field_bytes(const athad_ctx* c, const MemberHost& m, int f) {
+    return m.n * ((f == F_P_STAT) ? 8 : elem_size(c));
+}
+
+void fill_consts(MemberHost& m, const Config& cfg, int order) {
+    Consts& K = m.K;
+    const int im = m.g.im, jm = m.g.jm, km = m.g.km;
+    K.im = im; K.jm = jm; K.km = km;
+    K.n = (long long)im * jm * km;
+    const double pi180 = 180.0 / M_PI;
+    K.dr = 1.0 / (double)(im - 1);
+    K.dthe = (180.0 / (double)(jm - 1)) / pi180;
+    K.dphi = (360.0 / (double)(km - 1)) / pi180;
+    K.inv_2dr = 1.0 / (2.0 * K.dr); K.inv_2dthe = 1.0 / (2.0 * K.dthe); K.inv_2dphi = 1.0 / (2.0 * K.dphi);
+    K.inv_dr2 = 1.0 / (K.dr * K.dr); K.inv_dthe2 = 1.0 / (K.dthe * K.dthe); K.inv_dphi2 = 1.0 / (K.dphi * K.dphi);
+    K.rad0 = 1.0;
+    double rlast = 1.0;
+    for (int l = 1; l < im; l++) rlast = rlast + K.dr;
+    const double span = rlast - 1.0;
+    K.shell_len = (span > 0.0) ? (std::exp(cfg.zeta * span) - 1.0) * cfg.L_atm / span : cfg.L_atm;
+    K.metric_r0 = (K.shell_len > 0.0) ? (cfg.r_Earth * 1.0e3 / K.shell_len) : 0.0;
+    K.omega = cfg.omega; K.g = cfg.g; K.t_0 = cfg.t_0; K.t_skin = cfg.t_skin; K.r_air = m.p.r_air;
+    K.dt = cfg.dt_visc; K.L_atm = cfg.L_atm; K.u_0 = cfg.u_0; K.t_00 = cfg.t_00; K.cloud_cap = cfg.cloud_cap;
+    K.t_max_phys = cfg.t_max_phys; K.c_h2o_dry_top = m.p.c_h2o_dry_top; K.pr_turb = cfg.pr_turb;
+    K.abl_height = cfg.abl_height; K.sc_wv = cfg.sc_WaterVapour; K.sc_co2 = cfg.sc_CO2;
+    const Comp C = resolve(m.p.x_H2O, m.p.x_CO2, m.p.x_N2);
+    K.R_mix = C.R_mix; K.M_bg = C.M_bg; K.R_bg = C.R_bg;
+    K.p_sl_factor = 1e-2 * m.p.r_air * C.R_mix;
+    K.force_nd = cfg.omega * K.shell_len / cfg.u_0;
+    K.re_turb = 0.4 * 0.1 / 1.5e-5;
+    K.coeff_energy_p = cfg.u_0 * cfg.u_0 / (cfg.cp_l * cfg.t_0);
+    K.diff_prec_re_inv = 1.0 / (cfg.sc_WaterVapour * K.re_turb);
+    K.diff_co2_re_inv = 1.0 / (cfg.sc_CO2 * K.re_turb);
+    K.nue_max = 1000.0 / (cfg.u_0 * cfg.L_atm);
+    K.nue_air_nd = 1.5e-5 / (cfg.u_0 * cfg.L_atm);
+    K.tke_max_nd = 1000.0 / (cfg.u_0 * cfg.u_0);
+    K.vel_max_nd = 100.0 / cfg.u_0;
+    K.sqrt_cnue = std::sqrt(0.028);
+    K.hs_k_a = 1.0 / (4.0 * 86400.0);
+    K.hs_k_s = 1.0 / (0.4 * 86400.0);
+    K.drag_coeff = (1.0 / 86400.0) * cfg.L_atm / cfg.u_0 * cfg.dt_visc;
+    K.evap_r = std::exp(-1.0);
+    K.evap_w_norm = (1.0 - K.evap_r) / (1.0 - std::pow(K.evap_r, 3 + 1));
+    K.t_surf_eq = m.p.t_surf_equator; K.t_surf_pole = m.p.t_surf_pole;
+    K.init_M_nw = M_nonwater_h(m.p.c_0, m.p.co2_0, C.M_bg);
+    K.init_f_co2 = (C.q_CO2 + C.q_bg > 0.0) ? C.q_CO2 / (C.q_CO2 + C.q_bg) : m.p.co2_0 * cfg.co2_scale;
+    K.co2_scale = cfg.co2_scale;
+    K.order = order;
+    K.sym = 0;
+}
+
+// 1-D tables: rad, sin/cos, layer heights, ABL taper, SatAdj dt_dim, polar passes.
+void build_tables(MemberHost& m, const Config& cfg, std::vector<double>& tab, std::vector<int>& passes) {
+    const int im = m.g.im, jm = m.g.jm;
+    const Consts& K = m.K;
+    std::vector<double> rad(im), the(jm);
+    rad[0] = 1.0;
+    for (int l = 1; l < im; l++) rad[l] = rad[l - 1] + K.dr;
+    the[0] = 0.0;
+    for (int l = 1; l < jm; l++) the[l] = the[l - 1] + K.dthe;
+    const float h = (float)cfg.L_atm;
+    std::vector<double> hgt(im);
+    for (int i = 0; i < im; i++) hgt[i] = (double)(float)((std::exp(cfg.zeta * (rad[i] - rad[0])) - 1) * h);
+    m.o_rad = 0; m.o_sint = im; m.o_cost = m.o_sint + jm; m.o_sinr = m.o_cost + jm; m.o_hgt = m.o_sinr + jm;
+    m.o_nuef = m.o_hgt + im; m.o_step = m.o_nuef + im;
+    tab.assign(m.o_step + im, 0.0);
+    for (int i = 0; i < im; i++) tab[m.o_rad + i] = rad[i];
+    for (int j = 0; j < jm; j++) {
+        double s = std::sin(the[j]);
+        tab[m.o_sinr + j] = s;
+        tab[m.o_sint + j] = (s < 0.55) ? 0.55 : s;
+        tab[m.o_cost + j] = std::cos(the[j]);
+    }
+    for (int i = 0; i < im; i++) tab[m.o_hgt + i] = hgt[i];
+    // clamp_nue taper (TurbulenceAtm.h:271-297)
+    int i_abl = 0;
+    while (i_abl < im - 1 && hgt[i_abl] <= cfg.abl_height) i_abl++;
+    const double taper_dz = std::max(1.0, hgt[std::min(i_abl, im - 1)] - hgt[std::max(i_abl - 1, 0)]);
+    for (int i = 0; i < im; i++) {
+        const double z_agl = hgt[i] - hgt[0];
+        double f = (cfg.abl_height + taper_dz - z_agl) / taper_dz;
+        f = (f < 0.0) ? 0.0 : (1.0 < f) ? 1.0 : f;
+        tab[m.o_nuef + i] = f;
+    }
+    // SaturationAdjustment.h:56-60,99
+    for (int i = 0; i + 1 < im; i++) tab[m.o_step + i] = (double)(float)((float)hgt[i + 1] - (float)hgt[i]) / 1.6;
+    tab[m.o_step + im - 1] = 1.0;
+    // polar_zonal_filter pass counts (lib/Utils.cpp:557-571)
+    passes.assign(jm, 0);
+    const double sin_ref = std::cos(30.0 * (M_PI / 180.0));
+    for (int j = 0; j < jm; j++) {
+        const double sj = std::max(std::sin(the[j]), 1.0e-6);
+        if (sj >= sin_ref) continue;
+        const double ratio = sin_ref / sj;
+        int p = (int)std::lround(3.0 * (ratio * ratio - 1.0));
+        passes[j] = std::min(std::max(p, 0), 12);
+    }
+    // wall omega (TurbulenceAtm.h:110-143, SST branch)
+    const double y1s = std::max(hgt[1], 1.0e-6);
+    m.K.om_wall = (60.0 * 1.5e-5 / (0.0333 * y1s * y1s)) * (cfg.L_atm / cfg.u_0);
+}
+
+void make_desc(const athad_ctx* c, const MemberHost& m, int parity, MemberDesc& d) {
+    std::memset(&d, 0, sizeof(d));
+    d.K = m.K;
+    for (int f = 0; f < NFIELD; f++) d.f[f] = m.field[f];
+    for (int f = 0; f < NPROG; f++) d.alt[f] = m.alt[f];
+    if (parity) for (int f = 0; f < NPROG; f++) { d.f[f] = m.alt[f]; d.alt[f] = m.field[f]; }
+    for (int q = 0; q < NTMP; q++) d.tmp[q] = m.tmp[q];
+    d.vel_star = m.vel_star; d.t_top = m.t_top; d.t_ref = m.t_ref;
+    d.rad = m.tables + m.o_rad; d.sint = m.tables + m.o_sint; d.cost = m.tables + m.o_cost;
+    d.sinr = m.tables + m.o_sinr; d.hgt = m.tables + m.o_hgt; d.nue_f = m.tables + m.o_nuef;
+    d.step_dim = m.tables + m.o_step; d.passes = m.passes; d.counters = m.counters; d.scratch = m.scratch;
+    (void)c;
+}
+
+athad_status upload_descs(athad_ctx* c) {
+    for (int p = 0; p < 2; p++) {
+        c->h_md[p].resize(c->nmem);
+        for (int q = 0; q < c->nmem; q++) make_desc(c, c->mem[q], p, c->h_md[p][q]);
+        HIPCHK(c, hipMemcpyAsync(c->d_md[p], c->h_md[p].data(), sizeof(MemberDesc) * c->nmem,
+                                 hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ATHAD_OK;
+}
+
+void free_ctx(athad_ctx* c) {
+    if (!c) return;
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    for (auto& m : c->mem) {
+        for (auto p : m.field) if (p) hipFree(p);
+        for (auto p : m.alt) if (p) hipFree(p);
+        for (auto p : m.tmp) if (p) hipFree(p);
+        for (void* p : {(void*)m.vel_star, (void*)m.t_top, (void*)m.t_ref, (void*)m.tables, (void*)m.scratch,
+                        (void*)m.counters, (void*)m.passes})
+            if (p) hipFree(p);
+    }
+    for (int p = 0; p < 2; p++) if (c->d_md[p]) hipFree(c->d_md[p]);
+    for (int k = 0; k < KID_COUNT; k++)
+        for (auto& e : c->prof_ev[k]) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+Batch& cur(athad_ctx* c) {
+    c->batch.d_md = c->d_md[c->parity];
+    return c->batch;
+}
+
+// Kernel timing bracket.
+struct Prof {
+    athad_ctx* c; int kid; hipEvent_t e1 = nullptr, e2 = nullptr;
+    Prof(athad_ctx* cc, int k) : c(cc), kid(k) {
+        if (!c->capturing && c->prof_on[kid]) {
+            hipEventCreate(&e1); hipEventCreate(&e2);
+            hipEventRecord(e1, c->stream);
+        }
+    }
+    ~Prof() {
+        if (e1) { hipEventRecord(e2, c->stream); c->prof_ev[kid].push_back({e1, e2}); }
+    }
+};
+
+athad_status check_member(athad_ctx* c, int member, int field, int nf) {
+    if (!c) return ATHAD_ERR_ARG;
+    if (member < 0 || member >= c->nmem) { c->err = "member index out of range"; return ATHAD_ERR_ARG; }
+    if (field < 0 || field >= nf) { c->err = "field id out of range"; return ATHAD_ERR_ARG; }
+    return ATHAD_OK;
+}
+
+// Device buffer currently holding a field of a member (Jacobi parity aware).
+void* field_ptr(athad_ctx* c, int member, int f) {
+    MemberHost& m = c->mem[member];
+    if (f < NPROG && c->parity) return m.alt[f];
+    return m.field[f];
+}
+
+athad_status after_launch(athad_ctx* c) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { c->err = std::string("kernel launch: ") + hipGetErrorString(e); return ATHAD_ERR_DEVICE; }
+    return ATHAD_OK;
+}
+
+// ---- the operators (all members, no cadence gate unless given) -----------------------
+const int kMoistSrc[4] = {F_T, F_ICE, F_C, F_CLOUD};
+
+void op_damp_moist(athad_ctx* c, int gate) {
+    Prof p(c, KID_DAMP);
+    Batch& b = cur(c);
+    const int t0[4] = {NFIELD + 0, NFIELD + 1, NFIELD + 2, NFIELD + 3};
+    const int t1[4] = {NFIELD + 4, NFIELD + 5, NFIELD + 6, NFIELD + 7};
+    launch_damp(b, 0, kMoistSrc, t0, 4, 0, gate);
+    launch_damp(b, 1, t0, t1, 4, 0, gate);
+    launch_damp(b, 2, t1, kMoistSrc, 4, 1, gate);
+}
+
+void op_turbulence(athad_ctx* c, int gate) {
+    Batch& b = cur(c);
+    {
+        Prof p(c, KID_TURB);
+        launch_vel_star(b, gate);
+        launch_turb_sources(b, gate);
+    }
+    Prof p(c, KID_DAMP);
+    const int t0[6] = {NFIELD + 0, NFIELD + 1, NFIELD + 2, NFIELD + 3, NFIELD + 4, NFIELD + 5};
+    const int t1[6] = {NFIELD + 6, NFIELD + 7, NFIELD + 8, NFIELD + 9, NFIELD + 10, NFIELD + 11};
+    const int dst[6] = {F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC};
+    launch_damp(b, 0, t0, t1, 6, 0, gate);
+    launch_damp(b, 1, t1, t0, 6, 0, gate);
+    launch_damp(b, 2, t0, dst, 6, 0, gate);
+}
+
+void op_moist_block(athad_ctx* c, int gate) {
+    {
+        Prof p(c, KID_SATADJ);
+        launch_satadj(cur(c), gate);
+    }
+    op_damp_moist(c, gate);
+    {
+        Prof p(c, KID_DENSITIES);
+        launch_densities(cur(c), 0, 1, gate);
+    }
+    op_turbulence(c, gate);
+}
+
+void op_bc(athad_ctx* c, bool solid) {
+    {
+        Prof p(c, KID_BC);
+        launch_bc(cur(c));
+    }
+    if (solid) {
+        Prof p(c, KID_SOLID);
+        launch_solid_clamp(cur(c));
+    }
+}
+
+int max_plane(const athad_ctx* c) {
+    int d = 0;
+    for (auto& m : c->mem) d = std::max(d, (m.g.im - 2) + (m.g.jm - 2) + (m.g.km - 2));
+    return d;
+}
+
+// RK4 in the context's order. fused: the Jacobi kernel applies the solid-ground clamps on
+// read and fuses the store of the unfiltered fields (only valid inside a full step).
+void op_rk4(athad_ctx* c, bool fused) {
+    {
+        Prof p(c, KID_LEVEL_MEAN);
+        launch_level_mean(cur(c));
+    }
+    if (c->order == ATHAD_ORDER_LEX) {
+        Prof p(c, KID_RK4_PLANE);
+        const int dmax = max_plane(c);
+        for (int d = 3; d <= dmax; d++) launch_rk4_lex(cur(c), d);
+    } else {
+        {
+            Prof p(c, KID_RK4);
+            launch_rk4_jacobi(cur(c), fused);
+        }
+        c->parity ^= 1;
+    }
+}
+
+void op_filters(athad_ctx* c, int store) {
+    {
+        Prof p(c, KID_POLAR);
+        launch_polar(cur(c));
+    }
+    Prof p(c, KID_RADIAL);
+    launch_radial(cur(c), store);
+}
+
+// One scoped iteration for all members. When every member is at the same phase the
+// cadence is decided on the host; otherwise the gated kernels decide per member.
+void op_step(athad_ctx* c) {
+    launch_advance(cur(c));
+    bool same = true;
+    for (auto& m : c->mem) same = same && ((m.iter_n + 1) % 4 == (c->mem[0].iter_n + 1) % 4);
+    const int it = c->mem[0].iter_n + 1;
+    if (same) {
+        if (it % 4 == 0) { Prof p(c, KID_POISSON); launch_poisson(cur(c), 0); }
+        if (it % 2 == 0) op_moist_block(c, 0);
+    } else {
+        { Prof p(c, KID_POISSON); launch_poisson(cur(c), 4); }
+        op_moist_block(c, 2);
+    }
+    const bool lex = (c->order == ATHAD_ORDER_LEX);
+    op_bc(c, lex);
+    op_rk4(c, !lex);
+    op_filters(c, 1);
+    if (lex) {
+        Prof p(c, KID_STORE);
+        launch_store(cur(c), 0x7FF & ~0xE);   // all but u, v, w (stored by the radial filter)
+    }
+    for (auto& m : c->mem) { m.iter_n++; m.total++; }
+}
+
+}  // namespace
+
+// =======================================================================================
+// C-ABI
+// =======================================================================================
+extern "C" {
+
+const char* athad_version(void) { return "athad_hip 0.1 (gfx950)"; }
+
+int64_t athad_footprint_bytes(const athad_grid* g, int precision) {
+    if (!g) return -1;
+    const int64_t n = (int64_t)g->im * g->jm * g->km;
+    const int64_t es = (precision == ATHAD_MIXED) ? 4 : 8;
+    return n * (es * (NFIELD - 1 + NPROG + NTMP) + 8) + (int64_t)g->jm * g->km * 16 + 65536;
+}
+
+athad_status athad_create(int n_members, const athad_grid* grids, const athad_member_params* params,
+                          int precision, int order, athad_ctx** out) {
+    g_create_error.clear();
+    if (!out || !grids || n_members <= 0) { g_create_error = "bad arguments"; return ATHAD_ERR_ARG; }
+    *out = nullptr;
+    if (precision != ATHAD_FP64 && precision != ATHAD_MIXED) { g_create_error = "bad precision"; return ATHAD_ERR_ARG; }
+    if (order != ATHAD_ORDER_LEX && order != ATHAD_ORDER_JACOBI) { g_create_error = "bad order"; return ATHAD_ERR_ARG; }
+    for (int q = 0; q < n_members; q++) {
+        const athad_grid& g = grids[q];
+        if (g.im < 6 || g.jm < 6 || g.km < 6 || g.im > 1024 || g.jm > 8192 || g.km > 16384 || (g.jm % 2) == 0) {
+            g_create_error = "unsupported grid (need im,jm,km >= 6, jm odd)";
+            return ATHAD_ERR_SHAPE;
+        }
+    }
+    athad_ctx* c = new (std::nothrow) athad_ctx();
+    if (!c) return ATHAD_ERR_ALLOC;
+    c->nmem = n_members;
+    c->precision = precision;
+    c->order = order;
+    c->mem.resize(n_members);
+    Config cfg;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { g_create_error = hipGetErrorString(e); free_ctx(c); return ATHAD_ERR_DEVICE; }
+    for (int q = 0; q < n_members; q++) {
+        MemberHost& m = c->mem[q];
+        m.g = grids[q];
+        m.p = params ? params[q] : default_member();
+        m.n = (size_t)m.g.im * m.g.jm * m.g.km;
+        fill_consts(m, cfg, order);
+        std::vector<double> tab;
+        std::vector<int> passes;
+        build_tables(m, cfg, tab, passes);
+        auto alloc = [&](void** p, size_t bytes) {
+            if (hipMalloc(p, bytes) != hipSuccess) { *p = nullptr; return false; }
+            return true;
+        };
+        bool ok = true;
+        for (int f = 0; f < NFIELD && ok; f++) ok = alloc(&m.field[f], field_bytes(c, m, f));
+        for (int f = 0; f < NPROG && ok; f++) ok = alloc(&m.alt[f], m.n * elem_size(c));
+        for (int f = 0; f < NTMP && ok; f++) ok = alloc(&m.tmp[f], m.n * elem_size(c));
+        const size_t n2 = (size_t)m.g.jm * m.g.km;
+        ok = ok && alloc((void**)&m.vel_star, n2 * 8) && alloc((void**)&m.t_top, n2 * 8) &&
+             alloc((void**)&m.t_ref, m.g.im * 8) && alloc((void**)&m.tables, tab.size() * 8) &&
+             alloc((void**)&m.scratch, 4096 * 8) && alloc((void**)&m.counters, 64) &&
+             alloc((void**)&m.passes, passes.size() * 4);
+        if (!ok) { g_create_error = "device allocation failed"; free_ctx(c); return ATHAD_ERR_ALLOC; }
+        for (int f = 0; f < NFIELD; f++) hipMemsetAsync(m.field[f], 0, field_bytes(c, m, f), c->stream);
+        for (int f = 0; f < NPROG; f++) hipMemsetAsync(m.alt[f], 0, m.n * elem_size(c), c->stream);
+        hipMemsetAsync(m.vel_star, 0, n2 * 8, c->stream);
+        hipMemsetAsync(m.t_top, 0, n2 * 8, c->stream);
+        hipMemsetAsync(m.counters, 0, 64, c->stream);
+        hipMemcpyAsync(m.tables, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, c->stream);
+        hipMemcpyAsync(m.passes, passes.data(), passes.size() * 4, hipMemcpyHostToDevice, c->stream);
+        hipStreamSynchronize(c->stream);
+        c->batch.max_n = std::max(c->batch.max_n, (long long)m.n);
+        c->batch.max_im = std::max(c->batch.max_im, m.g.im);
+        c->batch.max_jm = std::max(c->batch.max_jm, m.g.jm);
+        c->batch.max_km = std::max(c->batch.max_km, m.g.km);
+    }
+    for (int p = 0; p < 2; p++) {
+        if (hipMalloc((void**)&c->d_md[p], sizeof(MemberDesc) * n_members) != hipSuccess) {
+            g_create_error = "device allocation failed"; free_ctx(c); return ATHAD_ERR_ALLOC;
+        }
+    }
+    c->batch.nmem = n_members;
+    c->batch.stream = c->stream;
+    c->batch.fp32 = (precision == ATHAD_MIXED);
+    if (upload_descs(c) != ATHAD_OK) { g_create_error = c->err; free_ctx(c); return ATHAD_ERR_DEVICE; }
+    *out = c;
+    return ATHAD_OK;
+}
+
+athad_status athad_destroy(athad_ctx* ctx) {
+    if (!ctx) return ATHAD_ERR_ARG;
+    hipStreamSynchronize(ctx->stream);
+    free_ctx(ctx);
+    return ATHAD_OK;
+}
+
+const char* athad_last_error(const athad_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+athad_status athad_sync(athad_ctx* ctx) {
+    if (!ctx) return ATHAD_ERR_ARG;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return ATHAD_OK;
+}
+
+athad_status athad_set_iter(athad_ctx* ctx, int member, int iter_n, int total) {
+    athad_status s = check_member(ctx, member, 0, 1);
+    if (s) return s;
+    ctx->mem[member].iter_n = iter_n;
+    ctx->mem[member].total = total;
+    int h[2] = {iter_n, total};
+    HIPCHK(ctx, hipMemcpyAsync(ctx->mem[member].counters, h, 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return ATHAD_OK;
+}
+
+athad_status athad_get_iter(athad_ctx* ctx, int member, int* iter_n, int* total) {
+    athad_status s = check_member(ctx, member, 0, 1);
+    if (s) return s;
+    int h[2];
+    HIPCHK(ctx, hipMemcpyAsync(h, ctx->mem[member].counters, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (iter_n) *iter_n = h[0];
+    if (total) *total = h[1];
+    return ATHAD_OK;
+}
+
+athad_status athad_set_field(athad_ctx* ctx, int member, int field, const double* src) {
+    athad_status s = check_member(ctx, member, field, NFIELD);
+    if (s) return s;
+    if (!src) return ATHAD_ERR_ARG;
+    MemberHost& m = ctx->mem[member];
+    void* dst = field_ptr(ctx, member, field);
+    if (field == F_P_STAT || ctx->precision == ATHAD_FP64) {
+        HIPCHK(ctx, hipMemcpyAsync(dst, src, m.n * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    } else {
+        std::vector<float> tmp(m.n);
+        for (size_t q = 0; q < m.n; q++) tmp[q] = (float)src[q];
+        HIPCHK(ctx, hipMemcpyAsync(dst, tmp.data(), m.n * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return ATHAD_OK;
+}
+
+athad_status athad_get_field(athad_ctx* ctx, int member, int field, double* dst) {
+    athad_status s = check_member(ctx, member, field, NFIELD);
+    if (s) return s;
+    if (!dst) return ATHAD_ERR_ARG;
+    MemberHost& m = ctx->mem[member];
+    const void* src = field_ptr(ctx, member, field);
+    if (field == F_P_STAT || ctx->precision == ATHAD_FP64) {
+        HIPCHK(ctx, hipMemcpyAsync(dst, src, m.n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    } else {
+        std::vector<float> tmp(m.n);
+        HIPCHK(ctx, hipMemcpyAsync(tmp.data(), src, m.n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        for (size_t q = 0; q < m.n; q++) dst[q] = (double)tmp[q];
+    }
+    return ATHAD_OK;
+}
+
+athad_status athad_set_field2d(athad_ctx* ctx, int member, int field, const double* src) {
+    athad_status s = check_member(ctx, member, field, ATHAD_NFIELD2D);
+    if (s) return s;
+    if (!src) return ATHAD_ERR_ARG;
+    MemberHost& m = ctx->mem[member];
+    double* dst = (field == ATHAD_VEL_STAR) ? m.vel_star : m.t_top;
+    HIPCHK(ctx, hipMemcpyAsync(dst, src, (size_t)m.g.jm * m.g.km * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return ATHAD_OK;
+}
+
+athad_status athad_get_field2d(athad_ctx* ctx, int member, int field, double* dst) {
+    athad_status s = check_member(ctx, member, field, ATHAD_NFIELD2D);
+    if (s) return s;
+    if (!dst) return ATHAD_ERR_ARG;
+    MemberHost& m = ctx->mem[member];
+    const double* src = (field == ATHAD_VEL_STAR) ? m.vel_star : m.t_top;
+    HIPCHK(ctx, hipMemcpyAsync(dst, src, (size_t)m.g.jm * m.g.km * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return ATHAD_OK;
+}
+
+athad_status athad_upload_rows(athad_ctx* ctx, int member, int field, double* const* rows) {
+    athad_status s = check_member(ctx, member, field, NFIELD);
+    if (s) return s;
+    if (!rows) return ATHAD_ERR_ARG;
+    MemberHost& m = ctx->mem[member];
+    std::vector<double> flat(m.n);
+    const int nrow = m.g.im * m.g.jm;
+    for (int r = 0; r < nrow; r++) {
+        if (!rows[r]) { ctx->err = "null row pointer"; return ATHAD_ERR_ARG; }
+        std::memcpy(&flat[(size_t)r * m.g.km], rows[r], (size_t)m.g.km * 8);
+    }
+    return athad_set_field(ctx, member, field, flat.data());
+}
+
+athad_status athad_download_rows(athad_ctx* ctx, int member, int field, double* const* rows) {
+    athad_status s = check_member(ctx, member, field, NFIELD);
+    if (s) return s;
+    if (!rows) return ATHAD_ERR_ARG;
+    MemberHost& m = ctx->mem[member];
+    std::vector<double> flat(m.n);
+    s = athad_get_field(ctx, member, field, flat.data());
+    if (s) return s;
+    const int nrow = m.g.im * m.g.jm;
+    for (int r = 0; r < nrow; r++) {
+        if (!rows[r]) { ctx->err = "null row pointer"; return ATHAD_ERR_ARG; }
+        std::memcpy(rows[r], &flat[(size_t)r * m.g.km], (size_t)m.g.km * 8);
+    }
+    return ATHAD_OK;
+}
+
+athad_status athad_init_state(athad_ctx* c, uint64_t seed) {
+    if (!c) return ATHAD_ERR_ARG;
+    // member seeds
+    for (int q = 0; q < c->nmem; q++) c->mem[q].K.seed = seed + (uint64_t)q;
+    c->parity = 0;
+    athad_status s = upload_descs(c);
+    if (s) return s;
+    Batch& b = cur(c);
+    // resetArrays: all fields zero except t (1) — the init kernels below overwrite every
+    // field they read; the rest are cleared here.
+    for (auto& m : c->mem) {
+        for (int f = 0; f < NFIELD; f++) HIPCHK(c, hipMemsetAsync(m.field[f], 0, field_bytes(c, m, f), c->stream));
+        HIPCHK(c, hipMemsetAsync(m.counters, 0, 64, c->stream));
+        m.iter_n = 0; m.total = 0;
+    }
+    launch_init_column(b);
+    {   // damp_wiggles(t) (cAtmosphereModel.cpp:528)
+        const int src[1] = {F_T}, t0[1] = {NFIELD + 0}, t1[1] = {NFIELD + 1};
+        launch_damp(b, 0, src, t0, 1, 0, 0);
+        launch_damp(b, 1, t0, t1, 1, 0, 0);
+        launch_damp(b, 2, t1, src, 1, 0, 0);
+    }
+    launch_copy_lid(b);
+    launch_init_water(b);
+    for (int pass = 0; pass < 2; pass++) {
+        launch_init_co2(b);
+        launch_densities(b, 1, 0, 0);
+    }
+    launch_init_rest(b);
+    op_turbulence(c, 0);
+    op_bc(c, true);
+    launch_copy_field(b, F_T_EQ, F_T);
+    launch_store(b, 0x7FF);
+    if (after_launch(c)) return ATHAD_ERR_DEVICE;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ATHAD_OK;
+}
+
+athad_status athad_poisson_sweep(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    { Prof p(c, KID_POISSON); launch_poisson(cur(c), 0); }
+    return after_launch(c);
+}
+athad_status athad_satadj(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    { Prof p(c, KID_SATADJ); launch_satadj(cur(c), 0); }
+    return after_launch(c);
+}
+athad_status athad_damp_moist(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    op_damp_moist(c, 0);
+    return after_launch(c);
+}
+athad_status athad_densities(athad_ctx* c, int set_water, int with_evap) {
+    if (!c) return ATHAD_ERR_ARG;
+    { Prof p(c, KID_DENSITIES); launch_densities(cur(c), set_water ? 1 : 0, with_evap ? 1 : 0, 0); }
+    return after_launch(c);
+}
+athad_status athad_turbulence(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    op_turbulence(c, 0);
+    return after_launch(c);
+}
+athad_status athad_boundary(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    op_bc(c, true);
+    return after_launch(c);
+}
+athad_status athad_rk4(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    op_rk4(c, false);
+    return after_launch(c);
+}
+athad_status athad_filters(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    op_filters(c, 0);
+    return after_launch(c);
+}
+athad_status athad_store(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    { Prof p(c, KID_STORE); launch_store(cur(c), 0x7FF); }
+    return after_launch(c);
+}
+
+athad_status athad_step(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    op_step(c);
+    return after_launch(c);
+}
+
+athad_status athad_run(athad_ctx* c, int n_steps, int use_graph) {
+    if (!c || n_steps < 0) return ATHAD_ERR_ARG;
+    if (!use_graph) {
+        for (int q = 0; q < n_steps; q++) op_step(c);
+        return after_launch(c);
+    }
+    int done = 0;
+    // align to the graph's phase with plain steps
+    while (done < n_steps) {
+        const int phase = c->mem[0].iter_n % 4;
+        if (c->graph && phase == c->graph_phase && c->parity == c->graph_parity && n_steps - done >= 4) {
+            HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+            for (auto& m : c->mem) { m.iter_n += 4; m.total += 4; }
+            done += 4;
+            continue;
+        }
+        if (!c->graph && n_steps - done >= 4) {
+            // capture four iterations
+            const int ph = c->mem[0].iter_n % 4, par = c->parity;
+            std::vector<MemberHost> saved = c->mem;
+            hipGraph_t g;
+            c->capturing = true;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (int q = 0; q < 4; q++) op_step(c);
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            c->capturing = false;
+            if (e != hipSuccess) { c->err = std::string("graph capture: ") + hipGetErrorString(e); return ATHAD_ERR_DEVICE; }
+            e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) { c->err = std::string("graph instantiate: ") + hipGetErrorString(e); return ATHAD_ERR_DEVICE; }
+            // capture recorded but did not execute the launches: restore the host mirror
+            for (int q = 0; q < c->nmem; q++) { c->mem[q].iter_n = saved[q].iter_n; c->mem[q].total = saved[q].total; }
+            c->parity = par;
+            c->graph_phase = ph;
+            c->graph_parity = par;
+            continue;
+        }
+        op_step(c);
+        done++;
+    }
+    return after_launch(c);
+}
+
+athad_status athad_profile(athad_ctx* c, int kid, int enable) {
+    if (!c || kid < 0 || kid >= KID_COUNT) return ATHAD_ERR_ARG;
+    c->prof_on[kid] = enable != 0;
+    if (enable) {
+        for (auto& e : c->prof_ev[kid]) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+        c->prof_ev[kid].clear();
+    }
+    return ATHAD_OK;
+}
+
+athad_status athad_profile_result(athad_ctx* c, int kid, double* total_ms, int* launches) {
+    if (!c || kid < 0 || kid >= KID_COUNT) return ATHAD_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (auto& e : c->prof_ev[kid]) {
+        float ms = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&ms, e.first, e.second));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = (int)c->prof_ev[kid].size();
+    return ATHAD_OK;
+}
+
+}  // extern "C"

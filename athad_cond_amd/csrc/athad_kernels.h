@@ -1,0 +1,47 @@
+// Host-side launch wrappers for the ATHAD_COND kernels (defined in athad_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "athad_device.h"
+
+namespace athad {
+
+// One batch of members that share a stream and a storage precision.
+struct Batch {
+    MemberDesc* d_md = nullptr;   // device array [nmem]
+    int nmem = 0;
+    long long max_n = 0;          // largest im*jm*km in the batch
+    int max_im = 0, max_jm = 0, max_km = 0;
+    hipStream_t stream = nullptr;
+    bool fp32 = false;
+};
+
+// Kernel ids for per-kernel timing.
+enum KernelId {
+    KID_RK4 = 0, KID_RK4_PLANE, KID_LEVEL_MEAN, KID_POISSON, KID_SATADJ, KID_DAMP, KID_DENSITIES,
+    KID_TURB, KID_BC, KID_SOLID, KID_POLAR, KID_RADIAL, KID_STORE, KID_INIT, KID_COUNT
+};
+
+void launch_advance(const Batch& b);
+void launch_level_mean(const Batch& b);
+void launch_rk4_jacobi(const Batch& b, bool fused);
+void launch_rk4_lex(const Batch& b, int plane);
+void launch_poisson(const Batch& b, int gate);
+void launch_satadj(const Batch& b, int gate);
+// pass 0,1,2 over the fields of one job
+void launch_damp(const Batch& b, int axis, const int* src, const int* dst, int nf, int moist_clamp, int gate);
+void launch_densities(const Batch& b, int set_water, int evaporate, int gate);
+void launch_vel_star(const Batch& b, int gate);
+void launch_turb_sources(const Batch& b, int gate);
+void launch_bc(const Batch& b);
+void launch_solid_clamp(const Batch& b);
+void launch_polar(const Batch& b);
+void launch_radial(const Batch& b, int store);
+void launch_store(const Batch& b, int mask);
+void launch_copy_field(const Batch& b, int dst, int src);
+void launch_init_column(const Batch& b);
+void launch_init_water(const Batch& b);
+void launch_init_co2(const Batch& b);
+void launch_copy_lid(const Batch& b);
+void launch_init_rest(const Batch& b);
+
+}  // namespace athad

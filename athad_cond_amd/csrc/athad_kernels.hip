@@ -1,0 +1,1451 @@
+// HIP kernels for one ATHAD_COND time step on gfx950 (MI355X).
+//
+// Memory-bound stencils: fields are SoA, [i][j][k] with k (longitude) fastest, so a
+// wavefront of 64 lanes reads 64 consecutive longitudes of one (i,j) row. Every kernel
+// takes a device array of MemberDesc and uses blockIdx.z as the member, so the same
+// launches serve one grid, an ensemble and the serving batch. Storage type S is double
+// (FP64 drop-in) or float (mixed form); p_stat is always double, and the saturation,
+// column and Newton arithmetic is always double.
+//
+// Each kernel cites the reference code it restates.
+#include "athad_device.h"
+#include "athad_kernels.h"
+
+namespace athad {
+
+template <typename S> ATHAD_DEV S* fp(const MemberDesc& M, int f) { return (S*)M.f[f]; }
+ATHAD_DEV double* pstat(const MemberDesc& M) { return (double*)M.f[F_P_STAT]; }
+
+// 1-D cell index of this thread for a kernel launched over a member's n cells with the
+// XCD-aware block remap; -1 when out of range.
+ATHAD_DEV long long cell_id(const Consts& K) {
+    const long long nb = (K.n + blockDim.x - 1) / blockDim.x;
+    const long long b = xcd_block(blockIdx.x, gridDim.x, nb);
+    if (b < 0) return -1;
+    const long long id = b * blockDim.x + threadIdx.x;
+    return (id < K.n) ? id : -1;
+}
+
+ATHAD_DEV void decode(const Consts& K, long long id, int& i, int& j, int& k) {
+    k = (int)(id % K.km);
+    const long long ij = id / K.km;
+    j = (int)(ij % K.jm);
+    i = (int)(ij / K.jm);
+}
+
+// Cadence gate: a kernel launched with gate g > 0 does nothing for members whose iteration
+// counter is not a multiple of g (Poisson every 4th, moist block every 2nd iteration,
+// cAtmosphereModel.cpp:1156-1185), so members at different phases can share launches.
+ATHAD_DEV bool gated_off(const MemberDesc& M, int gate) {
+    return gate > 0 && (M.counters[0] % gate) != 0;
+}
+
+// ------------------------------------------------------------------------------------
+// Step counter (device resident so a captured graph can replay steps)
+// ------------------------------------------------------------------------------------
+__global__ void k_advance(MemberDesc* md) {
+    if (threadIdx.x == 0) {
+        int* c = md[blockIdx.z].counters;
+        c[0] += 1;
+        c[1] += 1;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Level-mean temperature (RungeKutta_Atm_Turb.cpp:24-41). One block per (level, member);
+// fixed reduction tree, so the result is deterministic run to run.
+// ------------------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(256) void k_level_mean(const MemberDesc* md) {
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const int i = blockIdx.x;
+    if (i >= K.im) return;
+    const S* t = fp<S>(M, F_T);
+    __shared__ double ss[256], sw[256];
+    double sum = 0.0, wsum = 0.0;
+    const int njk = K.jm * K.km;
+    for (int q = threadIdx.x; q < njk; q += blockDim.x) {
+        const int j = q / K.km;
+        const double w = M.sinr[j];
+        sum += w * (double)t[(size_t)i * njk + q];
+        wsum += w;
+    }
+    ss[threadIdx.x] = sum;
+    sw[threadIdx.x] = wsum;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) { ss[threadIdx.x] += ss[threadIdx.x + s]; sw[threadIdx.x] += sw[threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) M.t_ref[i] = (sw[0] > 0.0) ? ss[0] / sw[0] : 1.0;
+}
+
+// ------------------------------------------------------------------------------------
+// RHS of one cell (RHS_Atm_Turb.cpp:31-1080, land-free central stencils) and the RK4
+// stage loop (RungeKutta_Atm_Turb.cpp:137-263). C is the arithmetic type.
+// Neighbours are held in registers: nb[f][d], d = i+1, i-1, j+1, j-1, k+1, k-1.
+// ------------------------------------------------------------------------------------
+enum { FT = 0, FU, FV, FW, FC, FCL, FI, FG, FCO, FK, FD };
+
+template <typename C> struct Geo {
+    C exp_rm, exp_2_rm, sinthe2, costhe, inv_rm, inv_rm2, inv_rmsinthe, inv_rm2sinthe, inv_rm2sinthe2,
+      cos_rm2sin, two_over_rm_exp, v_metric, inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2,
+      inv_dr, inv_dthe, inv_dphi;
+    C surf_drag, d_close, hs_kT;
+};
+
+// Solid-ground clamps of bcSolidGround passes 4-5 (BC_Atm.h:475-548) as a value transform,
+// used when the Jacobi kernel applies them on read instead of in a separate pass.
+template <typename C> ATHAD_DEV C solid_clamp(int f, C v, C tke_max, C vel_max) {
+    switch (f) {
+        case FC: case FCL: case FI: return (v < C(0)) ? C(0) : v;
+        case FK: { C x = (v < C(0)) ? C(0) : v; if (!finite_bits(x)) return C(0);
+                   return (x < C(0)) ? C(0) : (x > tke_max) ? tke_max : x; }
+        case FD: { C x = (v < C(0)) ? C(0) : v; if (!finite_bits(x)) return C(1.0e-10);
+                   return (x < C(1.0e-10)) ? C(1.0e-10) : (x > C(1.0e6)) ? C(1.0e6) : x; }
+        case FU: case FV: case FW: if (!finite_bits(v)) return C(0);
+                 return (v < -vel_max) ? -vel_max : (v > vel_max) ? vel_max : v;
+        default: return v;
+    }
+}
+template <typename C> ATHAD_DEV C solid_clamp_n(int f, C v, C tke_max, C vel_max) {   // tken, disn, un, vn, wn
+    switch (f) {
+        case FK: if (!finite_bits(v)) return C(0); return (v < C(0)) ? C(0) : (v > tke_max) ? tke_max : v;
+        case FD: if (!finite_bits(v)) return C(1.0e-10);
+                 return (v < C(1.0e-10)) ? C(1.0e-10) : (v > C(1.0e6)) ? C(1.0e6) : v;
+        case FU: case FV: case FW: if (!finite_bits(v)) return C(0);
+                 return (v < -vel_max) ? -vel_max : (v > vel_max) ? vel_max : v;
+        default: return v;
+    }
+}
+
+template <typename C> ATHAD_DEV C minmod(C a, C b) {
+    return (a * b <= C(0)) ? C(0) : (fabs(a) < fabs(b) ? a : b);
+}
+
+template <typename C>
+struct RhsIn {
+    C nb[NPROG][6];
+    C dpdr, dpdthe, dpdphi;    // raw central differences of p_dyn
+    C nue, p_ratio_bl, t_eq, t_ref, buoy;   // p_ratio_bl: Held-Suarez boundary-layer weight
+};
+
+template <typename C>
+ATHAD_DEV void rhs_eval(const Consts& K, const Geo<C>& g, const RhsIn<C>& in, const C* cen, C* rhs, C* aux,
+                        C& prod, C& tsrc, C& dsrc) {
+    C d1r[NPROG], d2r[NPROG], d1t[NPROG], d2t[NPROG], d1p[NPROG], d2p[NPROG];
+#pragma unroll
+    for (int f = 0; f < NPROG; f++) {
+        const C c0 = cen[f];
+        d1r[f] = (in.nb[f][0] - in.nb[f][1]) * g.inv_2dr;
+        d2r[f] = (in.nb[f][0] - C(2) * c0 + in.nb[f][1]) * g.inv_dr2;
+        d1t[f] = (in.nb[f][2] - in.nb[f][3]) * g.inv_2dthe;
+        d2t[f] = (in.nb[f][2] - C(2) * c0 + in.nb[f][3]) * g.inv_dthe2;
+        d1p[f] = (in.nb[f][4] - in.nb[f][5]) * g.inv_2dphi;
+        d2p[f] = (in.nb[f][4] - C(2) * c0 + in.nb[f][5]) * g.inv_dphi2;
+    }
+    const C u = cen[FU], v = cen[FV], w = cen[FW], tke = cen[FK], dis = cen[FD];
+    const C nue_max = C(K.nue_max);
+    // prod refresh :528-561
+    {
+        const C dudr_s = d1r[FU] * g.exp_rm, dvdr_s = d1r[FV] * g.exp_rm, dwdr_s = d1r[FW] * g.exp_rm;
+        const C dudthe_s = d1t[FU] * g.inv_rm, dvdthe_s = d1t[FV] * g.inv_rm, dwdthe_s = d1t[FW] * g.inv_rm;
+        const C dudphi_s = d1p[FU] * g.inv_rmsinthe, dvdphi_s = d1p[FV] * g.inv_rmsinthe,
+                dwdphi_s = d1p[FW] * g.inv_rmsinthe;
+        const C dis_here = tmax(dis, C(1.0e-10));
+        const C tke_here = tmax(tke, C(0));
+        C cnue = tke_here / dis_here;
+        cnue = tmin(cnue, nue_max);
+        const C der = C(0.66667) * (dudr_s + dvdthe_s + dwdphi_s);
+        prod = tmax(C(0),
+              (cnue * (C(2) * dudr_s - der) - C(0.66667) * tke) * dudr_s
+            + (cnue * (dudthe_s + dvdr_s)) * dudthe_s
+            + (cnue * (dudphi_s + dwdr_s)) * dudphi_s
+            + (cnue * (C(2) * dvdthe_s - der) - C(0.66667) * tke) * dvdthe_s
+            + (cnue * (dvdr_s + dudthe_s)) * dvdr_s
+            + (cnue * (dvdphi_s + dwdthe_s)) * dvdphi_s
+            + (cnue * (C(2) * dwdphi_s - der) - C(0.66667) * tke) * dwdphi_s
+            + (cnue * (dwdr_s + dudphi_s)) * dwdr_s
+            + (cnue * (dwdthe_s + dvdphi_s)) * dwdthe_s);
+    }
+    // k-omega SST diffusion coefficients and sources :633-721
+    C dif_t, dif_vel, dif_tke, dif_dis;
+    {
+        const C sig_k1 = C(1.176), sig_k2 = C(1.0), sig_w1 = C(2.0), sig_w2 = C(1.168), bet_star = C(0.09);
+        const C dc = g.d_close;
+        const C cross = d1r[FK] * d1r[FD] + d1t[FK] * g.inv_rm * d1t[FD] * g.inv_rm
+                      + d1p[FK] * g.inv_rmsinthe * d1p[FD] * g.inv_rmsinthe;
+        const C CD_kw = tmax((C(2) * sig_w2) / dis * cross, C(1.0e-20));
+        const C arg1 = tmin(tmax(sqrt(tke) / (bet_star * dis * dc), (C(500) * C(K.nue_air_nd)) / (dc * dc * dis)),
+                            (C(4) * sig_w2 * tke) / (CD_kw * dc * dc));
+        const C a2 = arg1 * arg1;
+        const C F1 = tanh(a2 * a2);
+        const C inv_re = C(1) / C(K.re_turb);
+        const C nue = in.nue;
+        dif_t = (inv_re + nue) / C(K.pr_turb);
+        dif_vel = inv_re + nue;
+        dif_tke = inv_re + nue / (F1 * sig_k1 + (C(1) - F1) * sig_k2);
+        dif_dis = inv_re + nue / (F1 * sig_w1 + (C(1) - F1) * sig_w2);
+        const C bet1 = C(0.0333), bet2 = C(0.0368), gam1 = C(0.413), gam2 = C(0.2);
+        const C tke_s = tmax(tke, C(1.0e-10)), dis_s = tmax(dis, C(1.0e-10));
+        const C P_k = tmin(prod, C(20) * bet_star * tke_s * dis_s);
+        const C Y_k = bet_star * tke_s * dis_s;
+        const C P_w = (F1 * gam1 + (C(1) - F1) * gam2) * P_k * dis_s / tke_s;
+        const C Y_w = (F1 * bet1 + (C(1) - F1) * bet2) * dis_s * dis_s;
+        const C D_w = C(2) * (C(1) - F1) * sig_w2 / dis_s * cross;
+        const C dsm = C(20) * dis_s * dis_s, tsm = C(20) * tke_s * dis_s;
+        tsrc = tclamp(P_k - Y_k, -tsm, tsm);
+        dsrc = tclamp(P_w - Y_w + D_w, -dsm, dsm);
+    }
+    const C u_exp = u * g.exp_rm, v_invrm = v * g.inv_rm, w_invrs = w * g.inv_rmsinthe;
+    const C pressure_t = C(K.coeff_energy_p) * (u_exp * in.dpdr * g.inv_2dr + v_invrm * in.dpdthe * g.inv_2dthe
+                                               + w_invrs * in.dpdphi * g.inv_2dphi);
+    C tr[NPROG];
+#pragma unroll
+    for (int f = 0; f < NPROG; f++) {
+        C gr_, gt_, gp_;
+        if (f <= FI) {
+            const C c0 = cen[f];
+            gr_ = minmod(c0 - in.nb[f][1], in.nb[f][0] - c0) * g.inv_dr;
+            gt_ = minmod(c0 - in.nb[f][3], in.nb[f][2] - c0) * g.inv_dthe;
+            gp_ = minmod(c0 - in.nb[f][5], in.nb[f][4] - c0) * g.inv_dphi;
+        } else { gr_ = d1r[f]; gt_ = d1t[f]; gp_ = d1p[f]; }
+        tr[f] = u_exp * gr_ + v_invrm * gt_ + w_invrs * gp_;
+    }
+    auto lap = [&](int f) {
+        return d2r[f] * g.exp_2_rm + d1r[f] * g.two_over_rm_exp + d2t[f] * g.inv_rm2 + d1t[f] * g.cos_rm2sin
+             + d2p[f] * g.inv_rm2sinthe2;
+    };
+    const C diffusion_u = (d2r[FU] * g.exp_2_rm + C(2) * u * g.inv_rm2 + d2t[FU] * g.inv_rm2
+        + C(4) * d1r[FU] * g.inv_rm * g.exp_rm + d1t[FU] * g.cos_rm2sin + d2p[FU] * g.inv_rm2sinthe2) * dif_vel;
+    const C diffusion_v = (d2r[FV] * g.exp_2_rm + d1r[FV] * g.two_over_rm_exp + d2t[FV] * g.inv_rm2
+        + d1t[FV] * g.cos_rm2sin - g.v_metric * v + d2p[FV] * g.inv_rm2sinthe2 + C(2) * d1t[FU] * g.inv_rm2
+        - d1p[FW] * C(2) * g.costhe * g.inv_rm2sinthe2) * dif_vel;
+    const C diffusion_w = (d2r[FW] * g.exp_2_rm + d1r[FW] * g.two_over_rm_exp + d2t[FW] * g.inv_rm2
+        + d1t[FW] * g.cos_rm2sin - g.v_metric * w + d2p[FW] * g.inv_rm2sinthe2 + C(2) * d1p[FU] * g.inv_rm2sinthe
+        + d1p[FV] * C(2) * g.costhe * g.inv_rm2sinthe2) * dif_vel;
+    const C dpdr_exp = in.dpdr * g.inv_2dr * g.exp_rm;
+    const C dpdthe_invrm = in.dpdthe * g.inv_2dthe * g.inv_rm;
+    const C dpdphi_invrs = in.dpdphi * g.inv_2dphi * g.inv_rmsinthe;
+    C rt = pressure_t - tr[FT] + lap(FT) * dif_t;
+    rt -= g.hs_kT * (cen[FT] - in.t_eq);
+    rhs[FT] = rt;
+    rhs[FU] = -dpdr_exp - tr[FU] + diffusion_u + in.buoy * (cen[FT] - in.t_ref);
+    const C force_nd = C(K.force_nd);
+    rhs[FV] = -dpdthe_invrm - tr[FV] + diffusion_v + force_nd * (C(2) * g.costhe * w) - g.surf_drag * v;
+    rhs[FW] = -dpdphi_invrs - tr[FW] + diffusion_w + force_nd * (C(-2) * (g.costhe * v)) - g.surf_drag * w;
+    const C dpr = C(K.diff_prec_re_inv), dco = C(K.diff_co2_re_inv);
+    rhs[FC] = -tr[FC] + lap(FC) * dpr;
+    rhs[FCL] = -tr[FCL] + lap(FCL) * dpr;
+    rhs[FI] = -tr[FI] + lap(FI) * dpr;
+    rhs[FG] = -tr[FG] + lap(FG) * dpr;
+    rhs[FCO] = -tr[FCO] + lap(FCO) * dco;
+    rhs[FK] = -tr[FK] + lap(FK) * dif_tke + tsrc;
+    rhs[FD] = -tr[FD] + lap(FD) * dif_dis + dsrc;
+    aux[0] = rhs[FU] + dpdr_exp;
+    aux[1] = rhs[FV] + dpdthe_invrm;
+    aux[2] = rhs[FW] + dpdphi_invrs;
+}
+
+// Geometry of one (i,j) column and the per-cell SST wall distance and Held-Suarez rate.
+template <typename C>
+ATHAD_DEV void make_geo(const MemberDesc& M, int i, int j, int k, double p_stat_c, double p_surf, Geo<C>& g) {
+    const Consts& K = M.K;
+    const double rm = M.rad[i];
+    const double exp_rm = 1.0 / (rm + 1.0);
+    const double sinthe = M.sint[j];
+    const double sinthe2 = sinthe * sinthe;
+    const double costhe = M.cost[j];
+    const double rmet = (K.metric_r0 > 0.0) ? (K.metric_r0 + (rm - K.rad0)) : rm;
+    const double inv_rm = 1.0 / rmet;
+    const double inv_rm2 = 1.0 / (rmet * rmet);
+    const double inv_rm2sinthe = inv_rm2 / sinthe;
+    g.exp_rm = C(exp_rm);
+    g.exp_2_rm = C(exp_rm * exp_rm);
+    g.sinthe2 = C(sinthe2);
+    g.costhe = C(costhe);
+    g.inv_rm = C(inv_rm);
+    g.inv_rm2 = C(inv_rm2);
+    g.inv_rmsinthe = C(1.0 / (rmet * sinthe));
+    g.inv_rm2sinthe = C(inv_rm2sinthe);
+    g.inv_rm2sinthe2 = C(inv_rm2 / sinthe2);
+    g.cos_rm2sin = C(costhe * inv_rm2sinthe);
+    g.two_over_rm_exp = C(2.0 * inv_rm * exp_rm);
+    g.v_metric = C((1.0 + costhe * costhe / sinthe2) * inv_rm2);
+    g.inv_2dr = C(K.inv_2dr); g.inv_2dthe = C(K.inv_2dthe); g.inv_2dphi = C(K.inv_2dphi);
+    g.inv_dr2 = C(K.inv_dr2); g.inv_dthe2 = C(K.inv_dthe2); g.inv_dphi2 = C(K.inv_dphi2);
+    g.inv_dr = C(2.0 * K.inv_2dr); g.inv_dthe = C(2.0 * K.inv_2dthe); g.inv_dphi = C(2.0 * K.inv_2dphi);
+    double dp = 1.0 - (double)i / 5.0;
+    if (dp < 0.0) dp = 0.0;
+    if (dp > 1.0) dp = 1.0;
+    g.surf_drag = C(K.drag_coeff * dp);
+    // SST wall distance: min(i, j, k) (RHS_Atm_Turb.cpp:640-671 evaluated without land);
+    // the symmetric variant measures j from the nearer pole.
+    double dc = (double)i;
+    const double dj = K.sym ? (double)min(j, K.jm - 1 - j) : (double)j;
+    if (dj <= dc) dc = dj;
+    if ((double)k <= dc) dc = (double)k;
+    g.d_close = C(dc);
+    // Held-Suarez :976-992
+    const double sigma = (p_surf > 0.0) ? p_stat_c / p_surf : 0.0;
+    double bl = (sigma - 0.7) / (1.0 - 0.7);
+    if (bl < 0.0) bl = 0.0;
+    const double kT = K.hs_k_a + (K.hs_k_s - K.hs_k_a) * bl * (sinthe2 * sinthe2);
+    g.hs_kT = C(kT * K.L_atm / K.u_0);
+}
+
+// Loads the six neighbours of every prognostic field (optionally through the solid-ground
+// clamp) and the p_dyn differences.
+template <typename S, typename C, bool CLAMP>
+ATHAD_DEV void load_nb(const MemberDesc& M, const S* const* X, size_t id, RhsIn<C>& in) {
+    const Consts& K = M.K;
+    const size_t si = (size_t)K.jm * K.km, sj = K.km;
+    const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
+#pragma unroll
+    for (int f = 0; f < NPROG; f++) {
+        const S* x = X[f];
+        C v0 = C(x[id + si]), v1 = C(x[id - si]), v2 = C(x[id + sj]), v3 = C(x[id - sj]), v4 = C(x[id + 1]),
+          v5 = C(x[id - 1]);
+        if (CLAMP) {
+            v0 = solid_clamp(f, v0, tkm, vm); v1 = solid_clamp(f, v1, tkm, vm); v2 = solid_clamp(f, v2, tkm, vm);
+            v3 = solid_clamp(f, v3, tkm, vm); v4 = solid_clamp(f, v4, tkm, vm); v5 = solid_clamp(f, v5, tkm, vm);
+        }
+        in.nb[f][0] = v0; in.nb[f][1] = v1; in.nb[f][2] = v2; in.nb[f][3] = v3; in.nb[f][4] = v4; in.nb[f][5] = v5;
+    }
+    const S* p = fp<S>(M, F_P_DYN);
+    in.dpdr = C(p[id + si]) - C(p[id - si]);
+    in.dpdthe = C(p[id + sj]) - C(p[id - sj]);
+    in.dpdphi = C(p[id + 1]) - C(p[id - 1]);
+}
+
+// The four stages of one cell. base[] = *n values, cen[] = start values (updated in
+// place to the final values). Returns aux/prod/sources of stage 4.
+template <typename C>
+ATHAD_DEV void rk4_cell(const Consts& K, const Geo<C>& g, const RhsIn<C>& in, const C* base, C* cen, C* aux,
+                        C& prod, C& ts, C& ds) {
+    const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0), tkm = C(K.tke_max_nd);
+    C acc[NPROG], r[NPROG];
+#pragma unroll 1
+    for (int st = 0; st < 4; st++) {
+        rhs_eval(K, g, in, cen, r, aux, prod, ts, ds);
+#pragma unroll
+        for (int f = 0; f < NPROG; f++) {
+            C val;
+            if (st == 0) { acc[f] = r[f]; val = base[f] + r[f] * half_dt; }
+            else if (st == 1) { acc[f] = acc[f] + C(2) * r[f]; val = base[f] + r[f] * half_dt; }
+            else if (st == 2) { acc[f] = acc[f] + C(2) * r[f]; val = base[f] + r[f] * dt; }
+            else { acc[f] = acc[f] + r[f]; val = base[f] + acc[f] * dt_sixth; }
+            if (f == FC || f == FCL || f == FI || f == FG) val = tmax(C(0), val);
+            else if (f == FK) { if (!finite_bits(val)) val = C(0); else val = (val < C(0)) ? C(0) : (val > tkm) ? tkm : val; }
+            else if (f == FD) val = tmax(C(1.0e-10), val);
+            cen[f] = val;
+        }
+    }
+}
+
+constexpr int kProgF[NPROG] = {F_T, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2, F_TKE, F_DIS};
+constexpr int kProgN[NPROG] = {F_TN, F_UN, F_VN, F_WN, F_CN, F_CLOUDN, F_ICEN, F_GRN, F_CO2N, F_TKEN, F_DISN};
+
+// Jacobi-order RK4 over the whole grid. Interior cells integrate four stages against the
+// start-of-RK4 neighbourhood read from f[] and write alt[]; boundary cells are copied. The
+// solid-ground clamps are applied on read (FUSED=1), and the store of the eight fields that
+// no later filter touches is fused (xn = new value).
+template <typename S, bool FUSED>
+__global__ __launch_bounds__(256) void k_rk4_jacobi(const MemberDesc* md) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const long long id = cell_id(K);
+    if (id < 0) return;
+    int i, j, k;
+    decode(K, id, i, j, k);
+    const S* X[NPROG];
+#pragma unroll
+    for (int f = 0; f < NPROG; f++) X[f] = fp<S>(M, kProgF[f]);
+    const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
+    const bool interior = (i > 0 && i < K.im - 1 && j > 0 && j < K.jm - 1 && k > 0 && k < K.km - 1);
+    if (!interior) {
+#pragma unroll
+        for (int f = 0; f < NPROG; f++) {
+            C v = C(X[f][id]);
+            if (FUSED) v = solid_clamp(f, v, tkm, vm);
+            ((S*)M.alt[f])[id] = S(v);
+            if (FUSED && f != FU && f != FV && f != FW) fp<S>(M, kProgN[f])[id] = S(v);
+        }
+        if (FUSED) {
+            S* nue = fp<S>(M, F_NUE);
+            const C nv = C(nue[id]);
+            nue[id] = S(finite_bits(nv) ? tclamp(nv, C(0), C(K.nue_max)) : C(0));
+            S* pr = fp<S>(M, F_PROD);
+            const C pv = C(pr[id]);
+            pr[id] = S(finite_bits(pv) ? tclamp(pv, C(0), C(1.0e4)) : C(0));
+            S* ts = fp<S>(M, F_TKE_SRC);
+            const C tv = C(ts[id]);
+            ts[id] = S(finite_bits(tv) ? tclamp(tv, C(-1.0e4), C(1.0e4)) : C(-1.0e4));
+            S* dsr = fp<S>(M, F_DIS_SRC);
+            const C dv = C(dsr[id]);
+            dsr[id] = S(finite_bits(dv) ? tclamp(dv, C(-1.0e4), C(1.0e4)) : C(-1.0e4));
+        }
+        return;
+    }
+    RhsIn<C> in;
+    load_nb<S, C, FUSED>(M, X, id, in);
+    const double* ps = pstat(M);
+    Geo<C> g;
+    make_geo<C>(M, i, j, k, ps[id], ps[ix3(K, 0, j, k)], g);
+    C cen[NPROG], base[NPROG];
+#pragma unroll
+    for (int f = 0; f < NPROG; f++) {
+        C c0 = C(X[f][id]);
+        C b0 = C(fp<S>(M, kProgN[f])[id]);
+        if (FUSED) { c0 = solid_clamp(f, c0, tkm, vm); b0 = solid_clamp_n(f, b0, tkm, vm); }
+        cen[f] = c0;
+        base[f] = b0;
+    }
+    {
+        C nv = C(fp<S>(M, F_NUE)[id]);
+        if (FUSED) nv = finite_bits(nv) ? tclamp(nv, C(0), C(K.nue_max)) : C(0);
+        in.nue = nv;
+    }
+    in.t_eq = C(fp<S>(M, F_T_EQ)[id]);
+    in.t_ref = C(M.t_ref[i]);
+    {
+        const int total = M.counters[1];
+        const double ramp = fmin(1.0, (double)total / 300.0);
+        in.buoy = C(ramp * 1.0 * K.g * K.dt / K.u_0);
+    }
+    C aux[3], prod, ts, ds;
+    rk4_cell<C>(K, g, in, base, cen, aux, prod, ts, ds);
+#pragma unroll
+    for (int f = 0; f < NPROG; f++) {
+        ((S*)M.alt[f])[id] = S(cen[f]);
+        if (FUSED && f != FU && f != FV && f != FW) fp<S>(M, kProgN[f])[id] = S(cen[f]);
+    }
+    fp<S>(M, F_AUX_U)[id] = S(aux[0]);
+    fp<S>(M, F_AUX_V)[id] = S(aux[1]);
+    fp<S>(M, F_AUX_W)[id] = S(aux[2]);
+    fp<S>(M, F_PROD)[id] = S(prod);
+    fp<S>(M, F_TKE_SRC)[id] = S(ts);
+    fp<S>(M, F_DIS_SRC)[id] = S(ds);
+}
+
+// Lexicographic-order RK4 restricted to the hyperplane i+j+k = d (in place). Cells of one
+// plane only read neighbours on planes d-1 (final) and d+1 (not yet updated), which is
+// exactly what the reference's single-threaded sweep sees.
+template <typename S>
+__global__ __launch_bounds__(256) void k_rk4_lex_plane(const MemberDesc* md, int d) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const int nj = K.jm - 2;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)(K.im - 2) * nj) return;
+    const int i = 1 + (int)(q / nj), j = 1 + (int)(q % nj);
+    const int k = d - i - j;
+    if (k < 1 || k > K.km - 2) return;
+    const size_t id = ix3(K, i, j, k);
+    S* X[NPROG];
+#pragma unroll
+    for (int f = 0; f < NPROG; f++) X[f] = fp<S>(M, kProgF[f]);
+    RhsIn<C> in;
+    load_nb<S, C, false>(M, (const S* const*)X, id, in);
+    const double* ps = pstat(M);
+    Geo<C> g;
+    make_geo<C>(M, i, j, k, ps[id], ps[ix3(K, 0, j, k)], g);
+    C cen[NPROG], base[NPROG];
+#pragma unroll
+    for (int f = 0; f < NPROG; f++) { cen[f] = C(X[f][id]); base[f] = C(fp<S>(M, kProgN[f])[id]); }
+    in.nue = C(fp<S>(M, F_NUE)[id]);
+    in.t_eq = C(fp<S>(M, F_T_EQ)[id]);
+    in.t_ref = C(M.t_ref[i]);
+    {
+        const int total = M.counters[1];
+        const double ramp = fmin(1.0, (double)total / 300.0);
+        in.buoy = C(ramp * 1.0 * K.g * K.dt / K.u_0);
+    }
+    C aux[3], prod, ts, ds;
+    rk4_cell<C>(K, g, in, base, cen, aux, prod, ts, ds);
+#pragma unroll
+    for (int f = 0; f < NPROG; f++) X[f][id] = S(cen[f]);
+    fp<S>(M, F_AUX_U)[id] = S(aux[0]);
+    fp<S>(M, F_AUX_V)[id] = S(aux[1]);
+    fp<S>(M, F_AUX_W)[id] = S(aux[2]);
+    fp<S>(M, F_PROD)[id] = S(prod);
+    fp<S>(M, F_TKE_SRC)[id] = S(ts);
+    fp<S>(M, F_DIS_SRC)[id] = S(ds);
+}
+
+// ------------------------------------------------------------------------------------
+// Poisson red-black sweep (PressureSolverAtm.h:106-121, :225-486, :504-594)
+// ------------------------------------------------------------------------------------
+template <typename S>
+__global__ void k_aux_bc(const MemberDesc* md, int gate) {
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)(K.jm - 2) * (K.km - 2)) return;
+    const int j = 1 + (int)(q / (K.km - 2)), k = 1 + (int)(q % (K.km - 2));
+    for (int f = F_AUX_U; f <= F_AUX_W; f++) {
+        S* a = fp<S>(M, f);
+        a[ix3(K, 0, j, k)] = a[ix3(K, 1, j, k)];
+        a[ix3(K, K.im - 1, j, k)] = a[ix3(K, K.im - 2, j, k)];
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_poisson_colour(const MemberDesc* md, int colour, int gate) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const long long id = cell_id(K);
+    if (id < 0) return;
+    int i, j, k;
+    decode(K, id, i, j, k);
+    if (i < 1 || i > K.im - 2 || j < 1 || j > K.jm - 2 || k < 1 || k > K.km - 2) return;
+    if (((i + j + k) & 1) != colour) return;
+    const double rm = M.rad[i];
+    const double exp_rm = 1.0 / (rm + 1.0);
+    const double exp_2_rm = exp_rm * exp_rm;
+    const double sinthe = M.sint[j];
+    const double sinthe2 = sinthe * sinthe;
+    const double rmet = (K.metric_r0 > 0.0) ? (K.metric_r0 + (rm - K.rad0)) : rm;
+    const double inv_rm = 1.0 / rmet;
+    const double inv_rm2 = 1.0 / (rmet * rmet);
+    const double inv_rmsinthe = 1.0 / (rmet * sinthe);
+    const double m_phi = inv_rm2 / sinthe2;
+    const double denom = 2.0 * exp_2_rm * K.inv_dr2 + 2.0 * inv_rm2 * K.inv_dthe2 + 2.0 * m_phi * K.inv_dphi2;
+    const C inv_denom = C(1.0 / denom);
+    const C num1 = C(exp_2_rm * K.inv_dr2), num2 = C(inv_rm2 * K.inv_dthe2), num3 = C(m_phi * K.inv_dphi2);
+    const size_t si = (size_t)K.jm * K.km, sj = K.km;
+    const S *au = fp<S>(M, F_AUX_U), *av = fp<S>(M, F_AUX_V), *aw = fp<S>(M, F_AUX_W);
+    S* p = fp<S>(M, F_P_DYN);
+    const C du_dr = (C(au[id + si]) - C(au[id - si])) * C(K.inv_2dr);
+    const C dv_dthe = (C(av[id + sj]) - C(av[id - sj])) * C(K.inv_2dthe);
+    const C dw_dphi = (C(aw[id + 1]) - C(aw[id - 1])) * C(K.inv_2dphi);
+    C div_src = du_dr * C(exp_rm) + dv_dthe * C(inv_rm) + dw_dphi * C(inv_rmsinthe);
+    const C src_max = C(denom * 2.0);
+    if (!finite_bits(div_src)) div_src = C(0);
+    else if (div_src > src_max) div_src = src_max;
+    else if (div_src < -src_max) div_src = -src_max;
+    p[id] = ((C(p[id + si]) + C(p[id - si])) * num1 + (C(p[id + sj]) + C(p[id - sj])) * num2
+           + (C(p[id + 1]) + C(p[id - 1])) * num3 - div_src) * inv_denom;
+}
+
+// Boundary values of p_dyn, composed: radial copy, then pole copy, then the seam average
+// (PressureSolverAtm.h:504-552). Only boundary cells are written and only interior cells
+// are read, so one launch applies all three in the reference's order.
+template <typename S>
+__global__ void k_p_bc(const MemberDesc* md, int gate) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const long long id = cell_id(K);
+    if (id < 0) return;
+    int i, j, k;
+    decode(K, id, i, j, k);
+    const bool bi = (i == 0 || i == K.im - 1), bj = (j == 0 || j == K.jm - 1), bk = (k == 0 || k == K.km - 1);
+    if (!(bi || bj || bk)) return;
+    const int ci = min(max(i, 1), K.im - 2), cj = min(max(j, 1), K.jm - 2);
+    S* p = fp<S>(M, F_P_DYN);
+    C v;
+    if (bk) v = (C(p[ix3(K, ci, cj, 1)]) + C(p[ix3(K, ci, cj, K.km - 2)])) / C(2);
+    else v = C(p[ix3(K, ci, cj, k)]);
+    p[id] = S(v);
+}
+
+template <typename S>
+__global__ void k_p_ceiling(const MemberDesc* md, int gate) {
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const long long id = cell_id(M.K);
+    if (id < 0) return;
+    const S c = S((M.counters[1] > 300) ? 3.0 : 10.0);
+    S* p = fp<S>(M, F_P_DYN);
+    const S v = p[id];
+    if (!finite_bits(v)) p[id] = S(0);
+    else if (v > c) p[id] = c;
+    else if (v < -c) p[id] = -c;
+}
+
+// ------------------------------------------------------------------------------------
+// SaturationAdjustment (SaturationAdjustment.h:62-451; IceSchemeCommon.h:121-190) and the
+// cap on S_c_c (cAtmosphereModel.cpp:1278-1295). Pointwise, FP64 arithmetic.
+// ------------------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate) {
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const long long id = cell_id(K);
+    if (id < 0) return;
+    int i, j, k;
+    decode(K, id, i, j, k);
+    S *pt = fp<S>(M, F_T), *pc = fp<S>(M, F_C), *pcl = fp<S>(M, F_CLOUD), *pi = fp<S>(M, F_ICE), *pg = fp<S>(M, F_GR);
+    const double co2 = (double)fp<S>(M, F_CO2)[id];
+    const double p_local = pstat(M)[id];
+    double t = pt[id], c = pc[id], cloud = pcl[id], ice = pi[id], gr = pg[id];
+    double scc = 0.0;
+    const double t_0 = K.t_0, t_00 = K.t_00, Mb = K.M_bg;
+    const double inv_t_0 = 1.0 / t_0;
+    if (i >= 1 && i <= K.im - 2) {
+        // adjustSaturation :91-284
+        const double t_range_inv = 1.0 / (t_0 - t_00);
+        const double qv_old = tmax(0.0, c), qc_old = tmax(0.0, cloud), qi_old = tmax(0.0, ice);
+        double T = t * t_0;
+        if (T < kTcrit) {
+            const double M_other = th_M_nonwater(c, co2, Mb);
+            const double qs0 = th_q_sat(th_p_auto(T), p_local, M_other);
+            const double alpha = 1.0 / (1.0 + exp(-(T - t_00) / 5.0));
+            if ((qv_old > qs0 && alpha > 0.01) || (qv_old < qs0 && (qc_old > 1e-12 || qi_old > 1e-12))) {
+                double qvb = qv_old, qcb = qc_old, qib = qi_old, qv_hyp = qvb;
+                const double T_orig = t * t_0;
+                for (int it = 1; it <= 20; it++) {
+                    const double CND = tmax(0.0, tmin(1.0, (T - t_00) * t_range_inv));
+                    const double DEP = 1.0 - CND;
+                    double dqv = qv_hyp - qvb;
+                    if (dqv > 0) { const double me = qcb + qib; if (dqv > me) dqv = me; }
+                    const double dcnd = dqv * CND, ddep = dqv * DEP;
+                    qvb += dqv;
+                    qcb = tmax(0.0, qcb - dcnd);
+                    qib = tmax(0.0, qib - ddep);
+                    const double cp_loc = th_cp_of(qvb, co2, T, Mb);
+                    T -= (th_L_vap(T) * dcnd + th_L_sub(T) * ddep) / cp_loc;
+                    const double Es = th_p_sat(T), Ei = th_p_subl(T);
+                    const double qs = th_q_sat(Es, p_local, M_other), qi = th_q_sat(Ei, p_local, M_other);
+                    const double qsum = qcb + qib;
+                    const double target = (qsum > 1e-12) ? (qcb * qs + qib * qi) / qsum : ((T >= t_0) ? qs : qi);
+                    const double cp_g = th_cp_of(qvb, co2, T, Mb);
+                    const double Lc = th_L_vap(T), Ld = th_L_sub(T);
+                    const double gain = CND * (Lc / cp_g) * th_dqdT(Es, Lc, T, p_local, M_other)
+                                      + DEP * (Ld / cp_g) * th_dqdT(Ei, Ld, T, p_local, M_other);
+                    const double om = 1.0 / (1.0 + gain);
+                    qv_hyp = qvb + om * (target - qvb);
+                    if (fabs(qvb / qv_hyp - 1.0) <= 1.0e-6) break;
+                }
+                qcb = tmax(0.0, qcb);
+                qib = tmax(0.0, qib);
+                if (T > kTcrit) T = kTcrit;
+                if (!isnan(T) && !isnan(qvb)) {
+                    scc = alpha * (qcb - qc_old) / M.step_dim[i];
+                    c = qv_old + alpha * (qvb - qv_old);
+                    cloud = qc_old + alpha * (qcb - qc_old);
+                    ice = qi_old + alpha * (qib - qi_old);
+                    t = (T_orig + alpha * (T - T_orig)) * inv_t_0;
+                }
+                if (T < t_00) { cloud = 0.0; ice = 0.0; }
+            }
+        }
+    }
+    // clampAndFade :303-451
+    {
+        if (c < 0.0) c = 0.0;
+        if (cloud < 0.0) cloud = 0.0;
+        if (ice < 0.0) ice = 0.0;
+        const double c_max = tmax(0.0, 1.0 - co2);
+        if (c > c_max) c = c_max;
+        double Td = t * t_0;
+        if (Td > K.t_max_phys) { Td = K.t_max_phys; t = K.t_max_phys * inv_t_0; }
+        const double M_o = th_M_nonwater(c, co2, Mb);
+        const double q_s = th_q_sat_at(Td, p_local, M_o);
+        if (Td >= kTcrit || q_s >= 1.0) {
+            const double q_cond = tmax(0.0, cloud) + tmax(0.0, ice) + tmax(0.0, gr);
+            if (q_cond > 0.0) {
+                const double cm = tmax(0.0, 1.0 - co2);
+                const double c_new = tmin(cm, c + q_cond);
+                const double q_used = tmax(0.0, c_new - c);
+                c = c_new; cloud = 0.0; ice = 0.0; gr = 0.0;
+                if (Td < kTcrit) {
+                    const double cp_loc = th_cp_of(c, co2, Td, Mb);
+                    if (cp_loc > 0.0) {
+                        double T_new = Td - th_L_vap(Td) * q_used / cp_loc;
+                        if (T_new < 1.0) T_new = 1.0;
+                        t = T_new / t_0;
+                    }
+                }
+            }
+            scc = 0.0;
+        } else {
+            const double qs = q_s;
+            if (c > qs) {
+                const double excess = c - qs;
+                c = qs;
+                if (Td >= t_00) { cloud += excess; Td += th_L_vap(Td) * excess / th_cp_of(c, co2, Td, Mb); }
+                else { ice += excess; Td += th_L_sub(Td) * excess / th_cp_of(c, co2, Td, Mb); }
+                if (Td > K.t_max_phys) Td = K.t_max_phys;
+                t = Td * inv_t_0;
+            }
+            if (cloud > K.cloud_cap) cloud = K.cloud_cap;
+            if (ice > K.cloud_cap) ice = K.cloud_cap;
+            const double al = 1.0 / (1.0 + exp(-(Td - t_00) / 5.0));
+            c *= al; cloud *= al; ice *= al;
+        }
+    }
+    if (!finite_bits(scc)) scc = 0.0;
+    else if (scc > 1.0e-3) scc = 1.0e-3;
+    else if (scc < -1.0e-3) scc = -1.0e-3;
+    pt[id] = S(t); pc[id] = S(c); pcl[id] = S(cloud); pi[id] = S(ice); pg[id] = S(gr);
+    fp<S>(M, F_S_C_C)[id] = S(scc);
+}
+
+// ------------------------------------------------------------------------------------
+// damp_wiggles passes (lib/Utils.cpp:429-532): k periodic, j clamped, i clamped, each from
+// the previous pass's complete output. A job names up to six (src, dst) pairs, where an
+// index < NFIELD is a field and NFIELD + q is scratch buffer q. The i pass can finish with
+// the moist clamp of cAtmosphereModel.cpp:1307-1399.
+// ------------------------------------------------------------------------------------
+struct DampJob { int nf; int src[6]; int dst[6]; int moist_clamp; };
+
+template <typename S> ATHAD_DEV S* job_ptr(const MemberDesc& M, int q) {
+    return (q < NFIELD) ? (S*)M.f[q] : (S*)M.tmp[q - NFIELD];
+}
+
+template <typename S, int AXIS>
+__global__ __launch_bounds__(256) void k_damp_pass(const MemberDesc* md, DampJob job, int gate) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const long long id = cell_id(K);
+    if (id < 0) return;
+    int i, j, k;
+    decode(K, id, i, j, k);
+    size_t a, b;
+    if (AXIS == 0) {   // k, periodic
+        const int km1 = (k - 1 + K.km) % K.km, kp1 = (k + 1) % K.km;
+        a = id - k + km1; b = id - k + kp1;
+    } else if (AXIS == 1) {   // j, clamped
+        a = ix3(K, i, max(j - 1, 0), k); b = ix3(K, i, min(j + 1, K.jm - 1), k);
+    } else {   // i, clamped
+        a = ix3(K, max(i - 1, 0), j, k); b = ix3(K, min(i + 1, K.im - 1), j, k);
+    }
+    C out[6];
+    for (int q = 0; q < job.nf; q++) {
+        const S* s = job_ptr<S>(M, job.src[q]);
+        const C c0 = C(s[id]);
+        out[q] = c0 + C(0.25) * (C(s[a]) - C(2) * c0 + C(s[b]));
+    }
+    if (AXIS == 2 && job.moist_clamp) {
+        // job order t, ice, c, cloud; co2 from the field
+        const C co2 = C(fp<S>(M, F_CO2)[id]);
+        C cv = out[2], cld = out[3], ic = out[1];
+        if (!finite_bits(cv)) cv = C(0);
+        if (!finite_bits(cld)) cld = C(0);
+        if (!finite_bits(ic)) ic = C(0);
+        if (cv < C(0)) cv = C(0);
+        if (cld < C(0)) cld = C(0);
+        if (ic < C(0)) ic = C(0);
+        const C cv_max = tmax(C(0), C(1) - co2);
+        if (cv > cv_max) cv = cv_max;
+        const C sum = cld + ic;
+        const C cap = C(K.cloud_cap);
+        if (sum > cap) { const C sc = cap / sum; cld *= sc; ic *= sc; }
+        if (cv < C(0)) cv = C(0);
+        if (cv > cv_max) cv = cv_max;
+        out[2] = cv; out[3] = cld; out[1] = ic;
+    }
+    for (int q = 0; q < job.nf; q++) job_ptr<S>(M, job.dst[q])[id] = S(out[q]);
+}
+
+// ------------------------------------------------------------------------------------
+// Column kernel: ThermoAtm::densities (ThermoAtm.h:1436-1614) followed by the sea-surface
+// saturation boundary of waterVapourEvaporation (ThermoAtm.h:115-258). One thread per
+// (j,k) column, serial upward in i; FP64 arithmetic. set_water: initial profile mode.
+// ------------------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(128) void k_densities(const MemberDesc* md, int set_water, int evaporate, int gate) {
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)K.jm * K.km) return;
+    const int j = (int)(q / K.km), k = (int)(q % K.km);
+    S *t = fp<S>(M, F_T), *c = fp<S>(M, F_C), *co2 = fp<S>(M, F_CO2), *cl = fp<S>(M, F_CLOUD), *ice = fp<S>(M, F_ICE),
+      *rh = fp<S>(M, F_R_HUMID), *rd = fp<S>(M, F_R_DRY);
+    double* ps = pstat(M);
+    const double t_0 = K.t_0, Mb = K.M_bg, Rb = K.R_bg;
+    const size_t si = (size_t)K.jm * K.km;
+    const size_t base = (size_t)j * K.km + k;
+    double T_prev = tmax(180.0, (double)t[base] * t_0);
+    double p_prev = K.p_sl_factor * T_prev;
+    double q_sat_min = 1.0;
+    double qv_prev = 0.0, qc_prev = 0.0;
+    for (int i = 0; i < K.im; i++) {
+        const size_t id = base + (size_t)i * si;
+        double qv = c[id];
+        const double qc = co2[id];
+        const double R_loc = th_R_of(qv, qc, Rb);
+        double Ti, pi;
+        if (i == 0) { Ti = T_prev; pi = p_prev; }
+        else {
+            const double dz = M.hgt[i] - M.hgt[i - 1];
+            const double cp_loc = th_cp_of(qv_prev, qc_prev, T_prev, Mb);
+            const double M_nw = th_M_nonwater(qv_prev, qc_prev, Mb);
+            const double q_s = th_q_sat_at(T_prev, p_prev, M_nw);
+            const double gamma = (qv_prev >= 0.99 * q_s) ? th_moist_lapse(T_prev, p_prev, M_nw, cp_loc, R_loc, K.g)
+                                                         : K.g / cp_loc;
+            const double T_ad = T_prev - gamma * dz;
+            Ti = tmax(K.t_skin, T_ad);
+            const double T_mean = 0.5 * (T_prev + Ti);
+            pi = p_prev * exp(-K.g * dz / (R_loc * T_mean));
+        }
+        t[id] = S(Ti / t_0);
+        ps[id] = pi;
+        if (set_water && i > 0) {
+            const double q_s_i = th_q_sat_at(Ti, pi, th_M_nonwater(qv, qc, Mb));
+            if (q_s_i < q_sat_min) q_sat_min = q_s_i;
+            qv = tmax(q_sat_min, K.c_h2o_dry_top);
+            c[id] = S(qv);
+        }
+        const double wf = tmax(0.5, 1.0 - (double)cl[id] - (double)ice[id]);
+        rh[id] = S(1e2 * pi / (R_loc * Ti * wf));
+        rd[id] = S(1e2 * pi / (th_R_of(0.0, qc, Rb) * Ti));
+        // the next level reads this level's stored water (the reference reads c[i-1] back)
+        qv_prev = (double)c[id];
+        qc_prev = qc;
+        T_prev = Ti; p_prev = pi;
+    }
+    M.t_top[base] = (double)t[base + (size_t)(K.im - 1) * si];
+    if (!evaporate) return;
+    // waterVapourEvaporation, sea branch
+    const double t0s = (double)t[base] * t_0;
+    if (t0s >= kTcrit) return;
+    const double T_s = tmax(180.0, t0s);
+    const double c0 = c[base];
+    const double q_sea = th_q_sat(th_p_auto(T_s), ps[base], th_M_nonwater(c0, (double)co2[base], Mb));
+    c[base] = S(c0 + (q_sea - c0) * K.evap_w_norm);
+    double rp = 1.0;
+    for (int i = 1; i <= 3 && i < K.im; i++) {
+        const size_t id = base + (size_t)i * si;
+        rp = pow(K.evap_r, (double)i);
+        const double weight = rp * K.evap_w_norm;
+        const double T_i = tmax(180.0, (double)t[id] * t_0);
+        const double ci = c[id];
+        const double q_s_i = th_q_sat_at(T_i, ps[id], th_M_nonwater(ci, (double)co2[id], Mb));
+        c[id] = S(tmin(ci + (q_sea - c0) * weight, q_s_i));
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// TurbulenceAtm::run (TurbulenceAtm.h:62-79): compute_vel_star, compute_sources with the
+// k-omega SST closure, clamp_nue. compute_sources updates tke/dis in place while sweeping
+// (j,k,i) in order; here every cell reads its neighbours' raw values and applies, for the
+// neighbours a serial sweep would already have visited (i-1, j-1, k-1 inside the interior),
+// the same idempotent floors that visit applied, which reproduces the serial sweep exactly
+// without ordering. Outputs go to scratch q0..q5 = tke, dis, nue, prod, tke_src, dis_src.
+// ------------------------------------------------------------------------------------
+template <typename S>
+__global__ void k_vel_star(const MemberDesc* md, int gate) {
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)K.jm * K.km) return;
+    const int isurf = min(1, K.im - 1);
+    const size_t id = (size_t)isurf * K.jm * K.km + q;
+    const double vs = (double)fp<S>(M, F_V)[id], ws = (double)fp<S>(M, F_W)[id];
+    const double U = K.u_0 * sqrt(vs * vs + ws * ws);
+    M.vel_star[q] = tmax(0.05, sqrt(0.002) * U);
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* md, int gate) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const long long id = cell_id(K);
+    if (id < 0) return;
+    int i, j, k;
+    decode(K, id, i, j, k);
+    const S *ptk = fp<S>(M, F_TKE), *pds = fp<S>(M, F_DIS);
+    S *otk = (S*)M.tmp[0], *ods = (S*)M.tmp[1], *onu = (S*)M.tmp[2], *opr = (S*)M.tmp[3], *ots = (S*)M.tmp[4],
+      *odss = (S*)M.tmp[5];
+    const C nue_max = C(K.nue_max);
+    const C f_abl = C(M.nue_f[i]);
+    const bool interior = (i > 0 && i < K.im - 1 && j > 0 && j < K.jm - 1 && k > 0 && k < K.km - 1);
+    if (!interior) {
+        otk[id] = ptk[id]; ods[id] = pds[id];
+        const C nv = C(fp<S>(M, F_NUE)[id]);
+        onu[id] = S(tclamp(nv, C(0), nue_max) * f_abl);
+        opr[id] = fp<S>(M, F_PROD)[id];
+        ots[id] = fp<S>(M, F_TKE_SRC)[id];
+        odss[id] = fp<S>(M, F_DIS_SRC)[id];
+        return;
+    }
+    const size_t si = (size_t)K.jm * K.km, sj = K.km;
+    const C u0 = C(K.u_0);
+    auto kbg = [&](int jj, int kk) {
+        const C vs = C(M.vel_star[(size_t)jj * K.km + kk]);
+        return C(0.01) * vs * vs / (C(K.sqrt_cnue) * u0 * u0);
+    };
+    const C dis_min = C(1.0e-10);
+    // visited neighbours: floors of the visit (max(0,.) then max(k_bg,.); dis max(dis_min,.))
+    auto tke_vis = [&](size_t q, int jj, int kk) { return tmax(kbg(jj, kk), tmax(C(0), C(ptk[q]))); };
+    auto dis_vis = [&](size_t q) { return tmax(dis_min, C(pds[q])); };
+    const C tk = tmax(C(0), C(ptk[id]));
+    const C ds = tmax(dis_min, C(pds[id]));
+    const C tk_im1 = (i - 1 >= 1) ? tke_vis(id - si, j, k) : C(ptk[id - si]);
+    const C ds_im1 = (i - 1 >= 1) ? dis_vis(id - si) : C(pds[id - si]);
+    const C tk_jm1 = (j - 1 >= 1) ? tke_vis(id - sj, j - 1, k) : C(ptk[id - sj]);
+    const C ds_jm1 = (j - 1 >= 1) ? dis_vis(id - sj) : C(pds[id - sj]);
+    const C tk_km1 = (k - 1 >= 1) ? tke_vis(id - 1, j, k - 1) : C(ptk[id - 1]);
+    const C ds_km1 = (k - 1 >= 1) ? dis_vis(id - 1) : C(pds[id - 1]);
+    const C tk_ip1 = C(ptk[id + si]), ds_ip1 = C(pds[id + si]);
+    const C tk_jp1 = C(ptk[id + sj]), ds_jp1 = C(pds[id + sj]);
+    const C tk_kp1 = C(ptk[id + 1]), ds_kp1 = C(pds[id + 1]);
+
+    const double rm = M.rad[i];
+    const double exp_rm_d = 1.0 / (rm + 1.0);
+    double sinthe = M.sinr[j];
+    if (sinthe == 0.0) sinthe = 1.0e-5;
+    const double rmsinthe = rm * sinthe;
+    const double rmet = (K.metric_r0 > 0.0) ? (K.metric_r0 + (rm - K.rad0)) : rm;
+    const C exp_rm = C(exp_rm_d), inv_2dr = C(K.inv_2dr);
+    const C inv_rm2dthe = C(K.inv_2dthe / rmet), inv_rms2dphi = C(K.inv_2dphi / (rmet * sinthe));
+    const S *u = fp<S>(M, F_U), *v = fp<S>(M, F_V), *w = fp<S>(M, F_W);
+    const C dudr = (C(u[id + si]) - C(u[id - si])) * inv_2dr * exp_rm;
+    const C dvdr = (C(v[id + si]) - C(v[id - si])) * inv_2dr * exp_rm;
+    const C dwdr = (C(w[id + si]) - C(w[id - si])) * inv_2dr * exp_rm;
+    const C dudthe = (C(u[id + sj]) - C(u[id - sj])) * inv_rm2dthe;
+    const C dvdthe = (C(v[id + sj]) - C(v[id - sj])) * inv_rm2dthe;
+    const C dwdthe = (C(w[id + sj]) - C(w[id - sj])) * inv_rm2dthe;
+    const C dudphi = (C(u[id + 1]) - C(u[id - 1])) * inv_rms2dphi;
+    const C dvdphi = (C(v[id + 1]) - C(v[id - 1])) * inv_rms2dphi;
+    const C dwdphi = (C(w[id + 1]) - C(w[id - 1])) * inv_rms2dphi;
+    const C dtkedr = (tk_ip1 - tk_im1) * inv_2dr * exp_rm;
+    const C ddisdr = (ds_ip1 - ds_im1) * inv_2dr * exp_rm;
+    C cnue = tmax(tk, C(0)) / tmax(ds, dis_min);
+    cnue = tmin(cnue, nue_max);
+    const C der = C(0.66667) * (dudr + dvdthe + dwdphi);
+    const C prod = tmax(C(0),
+          (cnue * (C(2) * dudr - der) - C(0.66667) * tk) * dudr
+        + (cnue * (dudthe + dvdr)) * dudthe
+        + (cnue * (dudphi + dwdr)) * dudphi
+        + (cnue * (C(2) * dvdthe - der) - C(0.66667) * tk) * dvdthe
+        + (cnue * (dvdr + dudthe)) * dvdr
+        + (cnue * (dvdphi + dwdthe)) * dvdphi
+        + (cnue * (C(2) * dwdphi - der) - C(0.66667) * tk) * dwdphi
+        + (cnue * (dwdr + dudphi)) * dwdr
+        + (cnue * (dwdthe + dvdphi)) * dwdthe);
+    const C W12 = dudthe - dvdr, W13 = dudphi - dwdr, W23 = dvdphi - dwdthe;
+    const C Omega = sqrt(W12 * W12 + W13 * W13 + W23 * W23);
+    // compute_k_omega_SST (grid rm in the theta/phi gradients, TurbulenceAtm.h:722-725)
+    const C a1 = C(0.31), bet_star = C(0.09), bet1 = C(0.0333), bet2 = C(0.0368), gam1 = C(0.413), gam2 = C(0.2);
+    const C sig_w2 = C(1.168);
+    const C y_star = C(tmax(M.hgt[i] - M.hgt[0], 1.0e-6) / K.L_atm);
+    const C rm2dthe = C(rm * 2.0 * K.dthe), rms2dphi = C(rmsinthe * 2.0 * K.dphi);
+    const C dtkedthe = (tk_jp1 - tk_jm1) / rm2dthe, ddisdthe = (ds_jp1 - ds_jm1) / rm2dthe;
+    const C dtkedphi = (tk_kp1 - tk_km1) / rms2dphi, ddisdphi = (ds_kp1 - ds_km1) / rms2dphi;
+    const C cross = dtkedr * ddisdr + dtkedthe * ddisdthe + dtkedphi * ddisdphi;
+    const C CD_kw = tmax(C(2) * sig_w2 / tmax(ds, C(1.0e-20)) * cross, C(1.0e-20));
+    const C dis_safe = tmax(ds, C(1.0e-20));
+    const C nu_nd = C(K.nue_air_nd);
+    const C arg1 = tmin(tmax(sqrt(tk) / (bet_star * dis_safe * y_star), C(500) * nu_nd / (y_star * y_star * dis_safe)),
+                        C(4) * sig_w2 * tk / (CD_kw * y_star * y_star));
+    const C arg2 = tmax(C(2) * sqrt(tk) / (bet_star * dis_safe * y_star), C(500) * nu_nd / (y_star * y_star * dis_safe));
+    const C a1sq = arg1 * arg1;
+    const C F1 = tanh(a1sq * a1sq);
+    const C F2 = tanh(arg2 * arg2);
+    C nue = a1 * tk / tmax(a1 * dis_safe, Omega * F2);
+    const C tke_safe = tmax(tk, C(1.0e-20));
+    const C P_k = tmin(prod, C(20) * bet_star * tke_safe * dis_safe);
+    const C Y_k = bet_star * tke_safe * dis_safe;
+    const C P_w = (F1 * gam1 + (C(1) - F1) * gam2) * P_k * dis_safe / tke_safe;
+    const C Y_w = (F1 * bet1 + (C(1) - F1) * bet2) * dis_safe * dis_safe;
+    const C D_w = C(2) * (C(1) - F1) * sig_w2 / dis_safe * cross;
+    C tsrc = P_k - Y_k, dsrc = P_w - Y_w + D_w;
+    const C tk_out = tmax(kbg(j, k), tk);
+    const C ds_out = tmax(dis_min, ds);
+    nue = tclamp(nue, C(0), nue_max);
+    const C dsm = C(20) * ds_out * ds_out;
+    dsrc = tclamp(dsrc, -dsm, dsm);
+    const C tsm = C(20) * tk_out * ds_out;
+    tsrc = tclamp(tsrc, -tsm, tsm);
+    otk[id] = S(tk_out);
+    ods[id] = S(ds_out);
+    onu[id] = S(tclamp(nue, C(0), nue_max) * f_abl);
+    opr[id] = S(prod);
+    ots[id] = S(tsrc);
+    odss[id] = S(dsrc);
+}
+
+// ------------------------------------------------------------------------------------
+// Boundary conditions (every iteration): bcRadius + apply_wall_bc per (j,k) column,
+// bcTheta per (i,k), bcPhi + seam smoothing per (i,j), bcSolidGround clamps per cell.
+// BC_Atm.h:556-917, :23-551; TurbulenceAtm.h:109-147.
+// ------------------------------------------------------------------------------------
+template <typename S>
+__global__ void k_bc_radial(const MemberDesc* md) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)K.jm * K.km) return;
+    const size_t si = (size_t)K.jm * K.km;
+    const int iml = K.im - 1;
+    auto at = [&](int i) { return (size_t)i * si + (size_t)q; };
+    fp<S>(M, F_T)[at(iml)] = S(M.t_top[q]);
+    {   // hydrostatic fields: cubic bottom, log-linear lid
+        double* p = pstat(M);
+        p[at(0)] = p[at(3)] - 3.0 * p[at(2)] + 3.0 * p[at(1)];
+        const double x1 = p[at(iml - 1)], x2 = p[at(iml - 2)];
+        p[at(iml)] = (x1 > 0.0 && x2 > 0.0) ? (x1 * x1 / x2) : x1;
+        for (int f : {F_R_HUMID, F_R_DRY}) {
+            S* x = fp<S>(M, f);
+            x[at(0)] = S(C(x[at(3)]) - C(3) * C(x[at(2)]) + C(3) * C(x[at(1)]));
+            const C y1 = C(x[at(iml - 1)]), y2 = C(x[at(iml - 2)]);
+            x[at(iml)] = S((y1 > C(0) && y2 > C(0)) ? (y1 * y1 / y2) : y1);
+        }
+    }
+    for (int f : {F_U, F_V, F_W}) {
+        S* x = fp<S>(M, f);
+        x[at(0)] = x[at(1)];
+        x[at(iml)] = x[at(iml - 1)];
+    }
+    for (int f : {F_V, F_W, F_VN, F_WN}) fp<S>(M, f)[at(iml)] = S(0);
+    for (int f : {F_V, F_W, F_VN, F_WN}) { S* x = fp<S>(M, f); x[at(iml - 1)] = S(C(x[at(iml - 1)]) * C(1.0 / 3.0)); }
+    for (int f : {F_V, F_W, F_VN, F_WN}) { S* x = fp<S>(M, f); x[at(iml - 2)] = S(C(x[at(iml - 2)]) * C(2.0 / 3.0)); }
+    fp<S>(M, F_U)[at(iml)] = S(0); fp<S>(M, F_UN)[at(iml)] = S(0);
+    fp<S>(M, F_U)[at(0)] = S(0); fp<S>(M, F_UN)[at(0)] = S(0);
+    for (int f : {F_CLOUD, F_ICE, F_GR, F_CO2, F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC}) {
+        S* x = fp<S>(M, f);
+        x[at(0)] = S(C(x[at(3)]) - C(3) * C(x[at(2)]) + C(3) * C(x[at(1)]));
+        x[at(iml)] = x[at(iml - 1)];
+    }
+    // apply_wall_bc (k-omega SST)
+    const double vs = M.vel_star[q];
+    fp<S>(M, F_TKE)[at(0)] = S(vs * vs / (K.sqrt_cnue * K.u_0 * K.u_0));
+    fp<S>(M, F_NUE)[at(0)] = S(0);
+    fp<S>(M, F_DIS)[at(0)] = S(K.om_wall);
+}
+
+constexpr int kThetaF[18] = {F_T, F_P_STAT, F_R_HUMID, F_R_DRY, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2,
+                             F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC};
+
+template <typename S>
+__global__ void k_bc_theta(const MemberDesc* md) {
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)K.im * K.km) return;
+    const int i = (int)(q / K.km), k = (int)(q % K.km);
+    const size_t a0 = ix3(K, i, 0, k), a1 = ix3(K, i, 1, k), b0 = ix3(K, i, K.jm - 1, k), b1 = ix3(K, i, K.jm - 2, k);
+#pragma unroll
+    for (int n = 0; n < 18; n++) {
+        const int f = kThetaF[n];
+        if (f == F_P_STAT) { double* x = pstat(M); x[a0] = x[a1]; x[b0] = x[b1]; }
+        else { S* x = fp<S>(M, f); x[a0] = x[a1]; x[b0] = x[b1]; }
+    }
+}
+
+constexpr int kPhiAvg[12] = {F_T, F_P_STAT, F_R_HUMID, F_R_DRY, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2};
+constexpr int kPhiExt[6] = {F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC};
+constexpr int kSeam[6] = {F_U, F_UN, F_V, F_VN, F_W, F_WN};
+
+template <typename S>
+__global__ void k_bc_phi(const MemberDesc* md) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)K.im * K.jm) return;
+    const size_t r = (size_t)q * K.km;
+    const int km = K.km;
+#pragma unroll
+    for (int n = 0; n < 12; n++) {
+        const int f = kPhiAvg[n];
+        if (f == F_P_STAT) {
+            double* x = pstat(M) + r;
+            x[0] = x[km - 1] = (x[1] + x[km - 2]) * 0.5;
+        } else {
+            S* x = fp<S>(M, f) + r;
+            const C v = (C(x[1]) + C(x[km - 2])) * C(0.5);
+            x[0] = S(v); x[km - 1] = S(v);
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < 6; n++) {
+        S* x = fp<S>(M, kPhiExt[n]) + r;
+        x[0] = x[1];
+        x[km - 1] = x[km - 2];
+    }
+#pragma unroll
+    for (int n = 0; n < 6; n++) {
+        S* x = fp<S>(M, kSeam[n]) + r;
+        for (int pass = 0; pass < 2; pass++) {
+            const C a = C(x[km - 2]), b = C(x[0]), c1 = C(x[1]), wa = C(x[km - 3]), ec = C(x[2]);
+            x[km - 2] = S(a + C(0.25) * (wa - C(2) * a + b));
+            const C nb = b + C(0.25) * (a - C(2) * b + c1);
+            x[0] = S(nb); x[km - 1] = S(nb);
+            x[1] = S(c1 + C(0.25) * (b - C(2) * c1 + ec));
+        }
+    }
+}
+
+// bcSolidGround passes 4-5 (used by the lexicographic form and the initial state; the
+// Jacobi form applies the same clamps inside k_rk4_jacobi).
+template <typename S>
+__global__ void k_solid_clamp(const MemberDesc* md) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const long long id = cell_id(K);
+    if (id < 0) return;
+    const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
+    for (int f = 0; f < NPROG; f++) {
+        if (f == FT || f == FG || f == FCO) continue;
+        S* x = fp<S>(M, kProgF[f]);
+        x[id] = S(solid_clamp(f, C(x[id]), tkm, vm));
+    }
+    for (int f : {FU, FV, FW, FK, FD}) {
+        S* x = fp<S>(M, kProgN[f]);
+        x[id] = S(solid_clamp_n(f, C(x[id]), tkm, vm));
+    }
+    {
+        S* x = fp<S>(M, F_NUE); const C v = C(x[id]);
+        x[id] = S(finite_bits(v) ? tclamp(v, C(0), C(K.nue_max)) : C(0));
+    }
+    {
+        S* x = fp<S>(M, F_PROD); const C v = C(x[id]);
+        x[id] = S(finite_bits(v) ? tclamp(v, C(0), C(1.0e4)) : C(0));
+    }
+    for (int f : {F_TKE_SRC, F_DIS_SRC}) {
+        S* x = fp<S>(M, f); const C v = C(x[id]);
+        x[id] = S(finite_bits(v) ? tclamp(v, C(-1.0e4), C(1.0e4)) : C(-1.0e4));
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Polar zonal filter (lib/Utils.cpp:542-608): per (i,j) row, p_j periodic 1-2-1 passes
+// along k, staged in LDS. One block per row.
+// ------------------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(256) void k_polar(const MemberDesc* md, int fbits) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const int row = blockIdx.x;
+    if (row >= K.im * K.jm) return;
+    const int j = row % K.jm;
+    const int np = M.passes[j];
+    if (np == 0) return;
+    extern __shared__ unsigned char smem_raw[];
+    C* buf0 = (C*)smem_raw;
+    C* buf1 = buf0 + K.km;
+    const int km = K.km;
+    for (int f = F_U; f <= F_W; f++) {
+        if (!(fbits & (1 << (f - F_U)))) continue;
+        S* x = fp<S>(M, f) + (size_t)row * km;
+        C* a = buf0;
+        C* b = buf1;
+        for (int k = threadIdx.x; k < km; k += blockDim.x) a[k] = C(x[k]);
+        __syncthreads();
+        for (int p = 0; p < np; p++) {
+            for (int k = threadIdx.x; k < km; k += blockDim.x) {
+                const int km1 = (k == 0) ? km - 1 : k - 1, kp1 = (k == km - 1) ? 0 : k + 1;
+                b[k] = a[k] + C(0.25) * (a[km1] - C(2) * a[k] + a[kp1]);
+            }
+            __syncthreads();
+            C* t = a; a = b; b = t;
+        }
+        for (int k = threadIdx.x; k < km; k += blockDim.x) x[k] = S(a[k]);
+        __syncthreads();
+    }
+}
+
+// Radial Shapiro filters (lib/Utils.cpp:831-933): two passes along i of u (1-2-1) and of
+// v, w (4th order, 1-2-1 next to the wall/lid), one thread per (j,k) column with the column
+// in LDS; optionally the store xn <- x of UtilsAtm.h:25-49 for u, v, w.
+template <typename S>
+__global__ __launch_bounds__(64) void k_radial_store(const MemberDesc* md, int store) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)K.jm * K.km) return;
+    extern __shared__ unsigned char smem_raw[];
+    C* col = (C*)smem_raw + (size_t)threadIdx.x * K.im;
+    const size_t si = (size_t)K.jm * K.km;
+    const int im = K.im;
+    for (int f = F_U; f <= F_W; f++) {
+        S* x = fp<S>(M, f) + q;
+        const bool ho = (f != F_U);
+        for (int i = 0; i < im; i++) col[i] = C(x[(size_t)i * si]);
+        for (int pass = 0; pass < 2; pass++) {
+            C o1 = col[0], o2 = C(0);   // old values at i-1 and i-2
+            for (int i = 1; i < im - 1; i++) {
+                const C c0 = col[i];
+                C nv;
+                if (ho && i >= 2 && i + 2 <= im - 1) {
+                    const C d4 = col[i + 2] - C(4) * col[i + 1] + C(6) * c0 - C(4) * o1 + o2;
+                    nv = c0 - C(1.0 / 16.0) * d4;
+                } else {
+                    nv = c0 + C(0.25) * (o1 - C(2) * c0 + col[i + 1]);
+                }
+                o2 = o1;
+                o1 = c0;
+                col[i] = nv;
+            }
+        }
+        S* xn = fp<S>(M, f - F_U + F_UN) + q;
+        for (int i = 0; i < im; i++) {
+            const S v = S(col[i]);
+            x[(size_t)i * si] = v;
+            if (store) xn[(size_t)i * si] = v;
+        }
+    }
+}
+
+// UtilsAtm::storeIntermediateData3D (UtilsAtm.h:25-49): xn <- x for the fields selected by mask.
+template <typename S>
+__global__ void k_store(const MemberDesc* md, int mask) {
+    const MemberDesc& M = md[blockIdx.z];
+    const long long id = cell_id(M.K);
+    if (id < 0) return;
+    for (int f = 0; f < NPROG; f++)
+        if (mask & (1 << f)) fp<S>(M, kProgN[f])[id] = fp<S>(M, kProgF[f])[id];
+}
+
+// ------------------------------------------------------------------------------------
+// Initial state on the device (the recipe the oracle's init follows):
+// surface parabola + dry column (InitValues_Atm.cpp:493-632), initWaterWapour (:843-905),
+// co2Atmosphere (ThermoAtm.h:873-913), seeded N-S mirrored velocity perturbation.
+// ------------------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(128) void k_init_column(const MemberDesc* md) {
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const double t_eq_K = K.t_surf_eq, t_pole_K = K.t_surf_pole;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)K.jm * K.km) return;
+    const int j = (int)(q / K.km);
+    const size_t si = (size_t)K.jm * K.km;
+    S *t = fp<S>(M, F_T), *rh = fp<S>(M, F_R_HUMID), *rd = fp<S>(M, F_R_DRY);
+    double* ps = pstat(M);
+    const double d_j_half = 0.5 * (K.jm - 1);
+    const double ratio = (double)j / d_j_half;
+    const double T_s = (t_pole_K - t_eq_K) * (ratio * ratio - 2.0 * ratio) + t_pole_K;
+    const double tu = (T_s * (1.0 / K.t_0)) * K.t_0;
+    double T_prev = tmax(100.0, tu);
+    double p_prev = 1e-2 * (K.r_air * K.R_mix * tu);
+    const double R_loc = th_R_of(0.0, 0.0, K.R_bg);
+    for (int i = 0; i < K.im; i++) {
+        double Tc, pv;
+        if (i == 0) { Tc = T_prev; pv = p_prev; }
+        else {
+            const double dz = (double)(float)((float)M.hgt[i] - (float)M.hgt[i - 1]);
+            const double cp_loc = th_cp_of(0.0, 0.0, T_prev, K.M_bg);
+            const double T_ad = T_prev - (K.g / cp_loc) * dz;
+            Tc = tmax(K.t_skin, T_ad);
+            const double Tm = 0.5 * (T_prev + Tc);
+            pv = p_prev * exp(-K.g * dz / (R_loc * Tm));
+        }
+        const size_t id = (size_t)i * si + q;
+        t[id] = S(Tc * (1.0 / K.t_0));
+        ps[id] = pv;
+        rd[id] = S(pv * 100.0 / (th_R_of(0.0, 0.0, K.R_bg) * Tc));
+        rh[id] = S(pv * 100.0 / (R_loc * Tc * 1.0));
+        T_prev = Tc; p_prev = pv;
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(128) void k_init_water(const MemberDesc* md) {
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const double M_nw = K.init_M_nw;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)K.jm * K.km) return;
+    const size_t si = (size_t)K.jm * K.km;
+    S *t = fp<S>(M, F_T), *c = fp<S>(M, F_C);
+    const double* ps = pstat(M);
+    double q_sat_min = 1.0;
+    for (int i = 0; i < K.im; i++) {
+        const size_t id = (size_t)i * si + q;
+        const double q_s = th_q_sat_at((double)t[id] * K.t_0, ps[id], M_nw);
+        if (q_s < q_sat_min) q_sat_min = q_s;
+        c[id] = S(tmax(q_sat_min, K.c_h2o_dry_top));
+    }
+}
+
+template <typename S>
+__global__ void k_init_co2(const MemberDesc* md) {
+    const MemberDesc& M = md[blockIdx.z];
+    const long long id = cell_id(M.K);
+    if (id < 0) return;
+    fp<S>(M, F_CO2)[id] = S((1.0 - (double)fp<S>(M, F_C)[id]) * M.K.init_f_co2 * M.K.co2_scale);
+}
+
+template <typename S>
+__global__ void k_copy_lid(const MemberDesc* md) {
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)K.jm * K.km) return;
+    M.t_top[q] = (double)fp<S>(M, F_T)[(size_t)(K.im - 1) * K.jm * K.km + q];
+}
+
+ATHAD_DEV uint64_t splitmix64(uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+template <typename S>
+__global__ void k_init_rest(const MemberDesc* md) {
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const unsigned long long seed = K.seed;
+    const long long id = cell_id(K);
+    if (id < 0) return;
+    int i, j, k;
+    decode(K, id, i, j, k);
+    for (int f : {F_CLOUD, F_ICE, F_GR, F_P_DYN, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC, F_AUX_U, F_AUX_V, F_AUX_W, F_S_C_C})
+        fp<S>(M, f)[id] = S(0);
+    const double amp = 1.0e-3 / K.u_0;
+    const int jeq = (K.jm - 1) / 2;
+    const int jj = (j <= jeq) ? j : (K.jm - 1 - j);
+    for (int f = 0; f < 3; f++) {
+        const uint64_t ctr = ((((uint64_t)f * K.im + i) * K.jm + jj) * K.km + k);
+        const uint64_t r = splitmix64(seed * 0xD1B54A32D192ED03ULL + ctr);
+        double val = (2.0 * ((double)(r >> 11) * (1.0 / 9007199254740992.0)) - 1.0) * amp;
+        if (f == 1) {
+            if (j == jeq) val = 0.0;
+            else if (j > jeq) val = -val;
+        }
+        fp<S>(M, F_U + f)[id] = S(val);
+    }
+    fp<S>(M, F_TKE)[id] = S(1.0e-4 / (K.u_0 * K.u_0));
+    fp<S>(M, F_DIS)[id] = S(1.0 * K.L_atm / K.u_0);
+}
+
+template <typename S>
+__global__ void k_copy_field(const MemberDesc* md, int dst, int src) {
+    const MemberDesc& M = md[blockIdx.z];
+    const long long id = cell_id(M.K);
+    if (id < 0) return;
+    fp<S>(M, dst)[id] = fp<S>(M, src)[id];
+}
+
+
+// ------------------------------------------------------------------------------------
+// Host launch wrappers
+// ------------------------------------------------------------------------------------
+namespace {
+inline unsigned cells_grid(const Batch& b, int bs) {
+    const long long nb = (b.max_n + bs - 1) / bs;
+    return (unsigned)(((nb + 7) / 8) * 8);   // multiple of 8 for the XCD remap
+}
+inline unsigned cols_grid(const Batch& b, int bs) {
+    return (unsigned)(((long long)b.max_jm * b.max_km + bs - 1) / bs);
+}
+}  // namespace
+
+#define ATHAD_DISPATCH(kern, grid, block, shm, ...)                                              \
+    do {                                                                                          \
+        if (b.fp32) hipLaunchKernelGGL(kern<float>, grid, block, shm, b.stream, __VA_ARGS__);     \
+        else hipLaunchKernelGGL(kern<double>, grid, block, shm, b.stream, __VA_ARGS__);           \
+    } while (0)
+
+void launch_advance(const Batch& b) {
+    hipLaunchKernelGGL(k_advance, dim3(1, 1, b.nmem), dim3(64), 0, b.stream, b.d_md);
+}
+void launch_level_mean(const Batch& b) {
+    ATHAD_DISPATCH(k_level_mean, dim3(b.max_im, 1, b.nmem), dim3(256), 0, b.d_md);
+}
+void launch_rk4_jacobi(const Batch& b, bool fused) {
+    const dim3 g(cells_grid(b, 256), 1, b.nmem);
+    if (b.fp32) {
+        if (fused) hipLaunchKernelGGL((k_rk4_jacobi<float, true>), g, dim3(256), 0, b.stream, b.d_md);
+        else hipLaunchKernelGGL((k_rk4_jacobi<float, false>), g, dim3(256), 0, b.stream, b.d_md);
+    } else {
+        if (fused) hipLaunchKernelGGL((k_rk4_jacobi<double, true>), g, dim3(256), 0, b.stream, b.d_md);
+        else hipLaunchKernelGGL((k_rk4_jacobi<double, false>), g, dim3(256), 0, b.stream, b.d_md);
+    }
+}
+void launch_rk4_lex(const Batch& b, int plane) {
+    const long long nq = (long long)(b.max_im - 2) * (b.max_jm - 2);
+    ATHAD_DISPATCH(k_rk4_lex_plane, dim3((unsigned)((nq + 255) / 256), 1, b.nmem), dim3(256), 0, b.d_md, plane);
+}
+void launch_poisson(const Batch& b, int gate) {
+    const unsigned na = (unsigned)(((long long)(b.max_jm - 2) * (b.max_km - 2) + 255) / 256);
+    ATHAD_DISPATCH(k_aux_bc, dim3(na, 1, b.nmem), dim3(256), 0, b.d_md, gate);
+    ATHAD_DISPATCH(k_poisson_colour, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, 0, gate);
+    ATHAD_DISPATCH(k_poisson_colour, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, 1, gate);
+    ATHAD_DISPATCH(k_p_bc, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate);
+    ATHAD_DISPATCH(k_p_ceiling, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate);
+}
+void launch_satadj(const Batch& b, int gate) {
+    ATHAD_DISPATCH(k_satadj, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate);
+}
+void launch_damp(const Batch& b, int axis, const int* src, const int* dst, int nf, int moist_clamp, int gate) {
+    DampJob job{};
+    job.nf = nf;
+    for (int q = 0; q < nf; q++) { job.src[q] = src[q]; job.dst[q] = dst[q]; }
+    job.moist_clamp = moist_clamp;
+    const dim3 g(cells_grid(b, 256), 1, b.nmem);
+    if (b.fp32) {
+        if (axis == 0) hipLaunchKernelGGL((k_damp_pass<float, 0>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
+        else if (axis == 1) hipLaunchKernelGGL((k_damp_pass<float, 1>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
+        else hipLaunchKernelGGL((k_damp_pass<float, 2>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
+    } else {
+        if (axis == 0) hipLaunchKernelGGL((k_damp_pass<double, 0>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
+        else if (axis == 1) hipLaunchKernelGGL((k_damp_pass<double, 1>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
+        else hipLaunchKernelGGL((k_damp_pass<double, 2>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
+    }
+}
+void launch_densities(const Batch& b, int set_water, int evaporate, int gate) {
+    ATHAD_DISPATCH(k_densities, dim3(cols_grid(b, 128), 1, b.nmem), dim3(128), 0, b.d_md, set_water, evaporate, gate);
+}
+void launch_vel_star(const Batch& b, int gate) {
+    ATHAD_DISPATCH(k_vel_star, dim3(cols_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate);
+}
+void launch_turb_sources(const Batch& b, int gate) {
+    ATHAD_DISPATCH(k_turb_sources, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate);
+}
+void launch_bc(const Batch& b) {
+    ATHAD_DISPATCH(k_bc_radial, dim3(cols_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
+    const unsigned nt = (unsigned)(((long long)b.max_im * b.max_km + 255) / 256);
+    ATHAD_DISPATCH(k_bc_theta, dim3(nt, 1, b.nmem), dim3(256), 0, b.d_md);
+    const unsigned np = (unsigned)(((long long)b.max_im * b.max_jm + 255) / 256);
+    ATHAD_DISPATCH(k_bc_phi, dim3(np, 1, b.nmem), dim3(256), 0, b.d_md);
+}
+void launch_solid_clamp(const Batch& b) {
+    ATHAD_DISPATCH(k_solid_clamp, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
+}
+void launch_polar(const Batch& b) {
+    const size_t shm = 2 * (size_t)b.max_km * (b.fp32 ? 4 : 8);
+    ATHAD_DISPATCH(k_polar, dim3((unsigned)(b.max_im * b.max_jm), 1, b.nmem), dim3(256), shm, b.d_md, 7);
+}
+void launch_radial(const Batch& b, int store) {
+    const size_t shm = 64 * (size_t)b.max_im * (b.fp32 ? 4 : 8);
+    ATHAD_DISPATCH(k_radial_store, dim3(cols_grid(b, 64), 1, b.nmem), dim3(64), shm, b.d_md, store);
+}
+void launch_store(const Batch& b, int mask) {
+    ATHAD_DISPATCH(k_store, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, mask);
+}
+void launch_copy_field(const Batch& b, int dst, int src) {
+    ATHAD_DISPATCH(k_copy_field, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, dst, src);
+}
+void launch_init_column(const Batch& b) {
+    ATHAD_DISPATCH(k_init_column, dim3(cols_grid(b, 128), 1, b.nmem), dim3(128), 0, b.d_md);
+}
+void launch_init_water(const Batch& b) {
+    ATHAD_DISPATCH(k_init_water, dim3(cols_grid(b, 128), 1, b.nmem), dim3(128), 0, b.d_md);
+}
+void launch_init_co2(const Batch& b) {
+    ATHAD_DISPATCH(k_init_co2, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
+}
+void launch_copy_lid(const Batch& b) {
+    ATHAD_DISPATCH(k_copy_lid, dim3(cols_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
+}
+void launch_init_rest(const Batch& b) {
+    ATHAD_DISPATCH(k_init_rest, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
+}
+
+}  // namespace athad

@@ -1,0 +1,149 @@
+"""Host-side mirror of the reference's model object for the scoped time step.
+
+``Integration`` owns one device context holding one or more members (grids). Its
+methods are the operators that ``cAtmosphereModel::run_3D_loop``
+(atmosphere/cAtmosphereModel.cpp:744-1728) calls in the scoped step, in the same order
+and with the same names where the reference has them; ``step()`` is one iteration.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Integration:
+    def __init__(self, grids: Sequence[tuple] | tuple, params: Iterable | None = None,
+                 precision: str = "fp64", order: str = "jacobi"):
+        if isinstance(grids, tuple) and len(grids) == 3 and isinstance(grids[0], int):
+            grids = [grids]
+        self.grids = [tuple(int(x) for x in g) for g in grids]
+        n = len(self.grids)
+        garr = (L.Grid * n)(*[L.Grid(*g) for g in self.grids])
+        if params is None:
+            parr = None
+        else:
+            params = list(params)
+            parr = (L.MemberParams * n)(*params)
+        self.precision = {"fp64": L.FP64, "mixed": L.MIXED}[precision]
+        self.order = {"lex": L.LEX, "jacobi": L.JACOBI}[order]
+        h = ctypes.c_void_p()
+        st = L.lib().athad_create(n, garr, parr, self.precision, self.order, ctypes.byref(h))
+        L.check(st, None)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().athad_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- state ------------------------------------------------------------------------
+    def _chk(self, st):
+        L.check(st, self.h)
+
+    def init_state(self, seed: int = 1234):
+        self._chk(L.lib().athad_init_state(self.h, seed))
+
+    def get(self, name: str, member: int = 0) -> np.ndarray:
+        g = self.grids[member]
+        if name in L.FIELDS2D:
+            out = np.empty(g[1:], dtype=np.float64)
+            self._chk(L.lib().athad_get_field2d(self.h, member, L.FIELDS2D.index(name),
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+            return out
+        out = np.empty(g, dtype=np.float64)
+        self._chk(L.lib().athad_get_field(self.h, member, L.FIELD_ID[name],
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def set(self, name: str, value: np.ndarray, member: int = 0):
+        a = np.ascontiguousarray(value, dtype=np.float64)
+        ptr = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        if name in L.FIELDS2D:
+            self._chk(L.lib().athad_set_field2d(self.h, member, L.FIELDS2D.index(name), ptr))
+        else:
+            self._chk(L.lib().athad_set_field(self.h, member, L.FIELD_ID[name], ptr))
+
+    def load(self, state: dict, member: int = 0):
+        for n in L.FIELDS + L.FIELDS2D:
+            if n in state:
+                self.set(n, state[n], member)
+
+    def snapshot(self, member: int = 0, names=None) -> dict:
+        return {n: self.get(n, member) for n in (names or (L.FIELDS + L.FIELDS2D))}
+
+    def set_iter(self, iter_n: int, total: int, member: int | None = None):
+        ms = range(len(self.grids)) if member is None else [member]
+        for m in ms:
+            self._chk(L.lib().athad_set_iter(self.h, m, iter_n, total))
+
+    def get_iter(self, member: int = 0):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._chk(L.lib().athad_get_iter(self.h, member, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    # ---- operators (run_3D_loop call sites) ------------------------------------------
+    def poisson_sweep(self):
+        self._chk(L.lib().athad_poisson_sweep(self.h))
+
+    def saturation_adjustment(self):
+        self._chk(L.lib().athad_satadj(self.h))
+
+    def damp_moist(self):
+        self._chk(L.lib().athad_damp_moist(self.h))
+
+    def densities(self, set_water_profile=False, with_evaporation=False):
+        self._chk(L.lib().athad_densities(self.h, int(set_water_profile), int(with_evaporation)))
+
+    def turbulence(self):
+        self._chk(L.lib().athad_turbulence(self.h))
+
+    def boundary(self):
+        self._chk(L.lib().athad_boundary(self.h))
+
+    def rk4(self):
+        self._chk(L.lib().athad_rk4(self.h))
+
+    def filters(self):
+        self._chk(L.lib().athad_filters(self.h))
+
+    def store(self):
+        self._chk(L.lib().athad_store(self.h))
+
+    def step(self):
+        self._chk(L.lib().athad_step(self.h))
+
+    def run(self, n: int, graph: bool = False):
+        self._chk(L.lib().athad_run(self.h, n, int(graph)))
+
+    def sync(self):
+        self._chk(L.lib().athad_sync(self.h))
+
+    # ---- measurement -------------------------------------------------------------------
+    def profile(self, kernel: str, enable: bool = True):
+        self._chk(L.lib().athad_profile(self.h, L.KERNEL_ID[kernel], int(enable)))
+
+    def profile_result(self, kernel: str):
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        self._chk(L.lib().athad_profile_result(self.h, L.KERNEL_ID[kernel], ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+
+def footprint_bytes(grid: tuple, precision: str = "fp64") -> int:
+    g = L.Grid(*grid)
+    return int(L.lib().athad_footprint_bytes(ctypes.byref(g), {"fp64": L.FP64, "mixed": L.MIXED}[precision]))

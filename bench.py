@@ -1,0 +1,128 @@
+"""Benchmark of the ATHAD_COND time step on one MI355X.
+
+Workload: the FP64 drop-in at the full grid 41 x 181 x 361 (2,678,981 cells) from the
+nominal initial state, every scoped operator live (Poisson every 4th iteration, moist block
+every 2nd). A step is one scoped iteration of all cells; value = cell-updates/s with the
+state resident in HBM. The same line reports the mixed-precision form (with its error
+against the FP64 form over the same steps), the lexicographic parity form, the roofline of
+the dominant kernel (RK4) from HIP events on the launch stream, and the CPU oracle timed on
+this host's cores on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+GRID = (41, 181, 361)
+METRIC = ("RK4 cell-updates/s at 41×181×361, mixed vs FP64 at stated error; p99 latency")
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per cell of the dominant kernel (k_rk4_jacobi, FP64): reads 11 current
+# fields + 11 stored (*n) fields + p_dyn, nue, p_stat, t_eq = 26 words; writes 11 new
+# fields + aux_u/v/w + prod, tke_source, dis_source + the 8 fused stores = 25 words.
+RK4_WORDS = 26 + 25
+# Whole scoped step, compulsory traffic per cell-update (FP64): 84 words.
+STEP_WORDS = 84
+
+
+def run_form(A, precision, order, steps, warmup, seed=1234, profile=True):
+    n = GRID[0] * GRID[1] * GRID[2]
+    with A.Integration(GRID, precision=precision, order=order) as integ:
+        integ.init_state(seed)
+        integ.run(warmup)
+        integ.sync()
+        if profile:
+            integ.profile("rk4", True)
+        t0 = time.perf_counter()
+        integ.run(steps)
+        integ.sync()
+        dt = time.perf_counter() - t0
+        rk4_ms, rk4_n = integ.profile_result("rk4") if profile else (0.0, 0)
+        fields = {name: integ.get(name) for name in A.PROGNOSTIC}
+    return {"ms_per_step": dt / steps * 1e3, "cells_per_s": n * steps / dt, "rk4_ms": rk4_ms,
+            "rk4_launches": rk4_n, "fields": fields}
+
+
+def cpu_baseline(seconds_budget=20.0):
+    """The oracle (clean-room CPU restatement of the same step) on this host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    st = O.OracleState(*GRID, seed=1234)
+    st.run(1, "jacobi")
+    n = GRID[0] * GRID[1] * GRID[2]
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        st.step("jacobi")
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > seconds_budget or steps >= 16:
+            break
+    return {"value": n * steps / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} scoped steps of 41x181x361 (Jacobi order, FP64, OpenMP) after 1 warmup step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the mixed and lex forms")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
+                                   "(see DESIGN.md), --gpus must be 1"}))
+        sys.exit(2)
+    import numpy as np
+    import athad_cond_amd as A
+
+    n = GRID[0] * GRID[1] * GRID[2]
+    fp64 = run_form(A, "fp64", "jacobi", args.steps, args.warmup)
+    avg_rk4_s = fp64["rk4_ms"] / max(fp64["rk4_launches"], 1) * 1e-3
+    achieved = RK4_WORDS * 8 * n / avg_rk4_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": fp64["cells_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fp64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (nominal initial state, seeded mirrored perturbation)",
+        "config": {"workload": "FP64 drop-in, Jacobi order, full scoped step at 41x181x361",
+                   "grid": list(GRID), "order": "jacobi", "precision": "fp64", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_rk4_jacobi<double>", "avg_launch_ms": avg_rk4_s * 1e3,
+                     "bytes_per_cell": RK4_WORDS * 8},
+        "step_hbm_frac": STEP_WORDS * 8 * fp64["cells_per_s"] / (HBM_PEAK_GBS * 1e9),
+    }
+    if not args.no_extra:
+        mixed = run_form(A, "mixed", "jacobi", args.steps, args.warmup, profile=False)
+        err = {k: float(np.sqrt(np.sum((mixed["fields"][k] - fp64["fields"][k]) ** 2))
+                        / max(np.sqrt(np.sum(fp64["fields"][k] ** 2)), 1e-300)) for k in A.PROGNOSTIC}
+        out["mixed"] = {"cells_per_s": mixed["cells_per_s"], "ms_per_step": mixed["ms_per_step"],
+                        "speedup_vs_fp64": mixed["cells_per_s"] / fp64["cells_per_s"],
+                        "rel_l2_vs_fp64": err, "max_rel_l2": max(err.values())}
+        lex_steps = max(4, min(args.steps, 12))
+        lex = run_form(A, "fp64", "lex", lex_steps, 4, profile=False)
+        out["lex"] = {"cells_per_s": lex["cells_per_s"], "ms_per_step": lex["ms_per_step"], "steps": lex_steps}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,154 @@
+/*
+ * athad_hip.h — C-ABI of the MI355X (gfx950) implementation of the ATHAD_COND time step.
+ *
+ * The reference has no C interface for this path: cAtmosphereModel::run_3D_loop
+ * (atmosphere/cAtmosphereModel.cpp:744-1728) calls five operators synchronously, each a
+ * friend class built on the model, plus the boundary/filter glue between them. This header
+ * exports one entry point per call site (cited on each declaration) and one for the whole
+ * scoped iteration, over device-resident state owned by an opaque context. Host arrays stay
+ * owned by the caller; plain C types only; every function returns an athad_status and never
+ * throws or aborts across the boundary.
+ *
+ * State layout on the device: one contiguous [i][j][k] array per field (k fastest), FP64 in
+ * the drop-in form, FP32 (p_stat FP64) in the mixed form. A context holds one or more
+ * members (independent integrations, possibly of different grid sizes) that share a HIP
+ * stream; every operator advances all of them.
+ */
+#ifndef ATHAD_HIP_H
+#define ATHAD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int athad_status;
+#define ATHAD_OK            0
+#define ATHAD_ERR_ARG       1   /* bad argument (null pointer, unknown field, bad member) */
+#define ATHAD_ERR_DEVICE    2   /* HIP runtime error (message in athad_last_error) */
+#define ATHAD_ERR_ALLOC     3   /* device allocation failed */
+#define ATHAD_ERR_SHAPE     4   /* grid shape not supported or mismatched */
+#define ATHAD_ERR_STATE     5   /* call not valid in the context's current state */
+
+/* Storage precision */
+#define ATHAD_FP64   0          /* drop-in form: FP64 storage and arithmetic */
+#define ATHAD_MIXED  1          /* FP32 storage/stencils, FP64 thermodynamics and p_stat */
+
+/* RK4 update order (RungeKutta_Atm_Turb.cpp:99-264 sweeps in place) */
+#define ATHAD_ORDER_LEX     0   /* the reference's sweep at OMP_NUM_THREADS=1 (parity form) */
+#define ATHAD_ORDER_JACOBI  1   /* start-of-step neighbourhood (throughput form) */
+
+/* Field ids: the cAtmosphereModel Array members of the scoped step
+ * (atmosphere/cAtmosphereModel.h:535-662). */
+enum athad_field {
+    ATHAD_T = 0, ATHAD_U, ATHAD_V, ATHAD_W, ATHAD_C, ATHAD_CLOUD, ATHAD_ICE, ATHAD_GR, ATHAD_CO2,
+    ATHAD_TKE, ATHAD_DIS,
+    ATHAD_TN, ATHAD_UN, ATHAD_VN, ATHAD_WN, ATHAD_CN, ATHAD_CLOUDN, ATHAD_ICEN, ATHAD_GRN,
+    ATHAD_CO2N, ATHAD_TKEN, ATHAD_DISN,
+    ATHAD_P_DYN, ATHAD_P_STAT, ATHAD_R_HUMID, ATHAD_R_DRY, ATHAD_T_EQ, ATHAD_NUE, ATHAD_PROD,
+    ATHAD_TKE_SOURCE, ATHAD_DIS_SOURCE, ATHAD_AUX_U, ATHAD_AUX_V, ATHAD_AUX_W, ATHAD_S_C_C,
+    ATHAD_NFIELD
+};
+/* 2-D (jm x km) fields: Array_2D vel_star (cAtmosphereModel.h:533) and the lid snapshot
+ * t_top_init (cAtmosphereModel.h:400). */
+enum athad_field2d { ATHAD_VEL_STAR = 0, ATHAD_T_TOP_INIT = 1, ATHAD_NFIELD2D = 2 };
+
+typedef struct athad_grid {
+    int im, jm, km;             /* radial levels, latitudes (pole to pole), longitudes (seam duplicated) */
+} athad_grid;
+
+/* Per-member surface state and composition; the remaining parameters are the values of
+ * cli/config_cond.xml. A NULL pointer where one is accepted selects that file's values. */
+typedef struct athad_member_params {
+    double t_surf_equator, t_surf_pole;   /* [K] */
+    double r_air;                         /* [kg/m3] reference density at the sea surface */
+    double x_H2O, x_CO2, x_N2;            /* surface mole fractions */
+    double c_0, co2_0, c_h2o_dry_top;     /* [kg/kg] */
+} athad_member_params;
+
+typedef struct athad_ctx athad_ctx;
+
+/* --- lifetime ------------------------------------------------------------------------ */
+
+/* Replaces constructing cAtmosphereModel + LoadConfig (cAtmosphereModel.cpp:74-127) for the
+ * device state of n_members integrations. grids[n_members]; params may be NULL. */
+athad_status athad_create(int n_members, const athad_grid* grids, const athad_member_params* params,
+                          int precision, int order, athad_ctx** out);
+athad_status athad_destroy(athad_ctx* ctx);
+/* Last error message of this context (or of the last failed athad_create when ctx is NULL). */
+const char* athad_last_error(const athad_ctx* ctx);
+/* Device bytes held by a context of this shape (admission control). */
+int64_t athad_footprint_bytes(const athad_grid* grid, int precision);
+
+/* --- state transfer ---------------------------------------------------------------- */
+
+/* Nominal initial state for every member (init recipe of RunTimeSlice's thermodynamic
+ * chain, cAtmosphereModel.cpp:527-568, plus the benchmark's velocity/turbulence fields);
+ * member m uses seed + m. */
+athad_status athad_init_state(athad_ctx* ctx, uint64_t seed);
+/* Contiguous FP64 host arrays [im][jm][km] (jm x km for 2-D fields). */
+athad_status athad_set_field(athad_ctx* ctx, int member, int field, const double* src);
+athad_status athad_get_field(athad_ctx* ctx, int member, int field, double* dst);
+athad_status athad_set_field2d(athad_ctx* ctx, int member, int field, const double* src);
+athad_status athad_get_field2d(athad_ctx* ctx, int member, int field, double* dst);
+/* Row-pointer form matching the reference Array (double*** x, one km-row per (i,j),
+ * lib/Array.cpp:62-71): rows[i*jm + j] points at row (i,j). */
+athad_status athad_upload_rows(athad_ctx* ctx, int member, int field, double* const* rows);
+athad_status athad_download_rows(athad_ctx* ctx, int member, int field, double* const* rows);
+/* iteration counters iter_n / total_iter_count (cAtmosphereModel.h:311-316) */
+athad_status athad_set_iter(athad_ctx* ctx, int member, int iter_n, int total_iter_count);
+athad_status athad_get_iter(athad_ctx* ctx, int member, int* iter_n, int* total_iter_count);
+
+/* --- the five call sites of run_3D_loop and their glue ------------------------------ */
+
+/* PressureSolverAtm(*this).run()  (cAtmosphereModel.cpp:1158; PressureSolverAtm.h:27-669):
+ * one red-black sweep of p_dyn with its boundaries and ceiling. */
+athad_status athad_poisson_sweep(athad_ctx* ctx);
+/* SaturationAdjustment(*this).run()  (cAtmosphereModel.cpp:1218; SaturationAdjustment.h:25-45),
+ * including the S_c_c cap of cAtmosphereModel.cpp:1278-1295. */
+athad_status athad_satadj(athad_ctx* ctx);
+/* damp_wiggles of t, ice, c, cloud and the microphysics clamp / vapour ceiling
+ * (cAtmosphereModel.cpp:1225-1228, 1307-1399). */
+athad_status athad_damp_moist(athad_ctx* ctx);
+/* ThermoAtm(*this).densities(set_water)  (cAtmosphereModel.cpp:1418; ThermoAtm.h:1436-1614);
+ * with_evaporation also applies waterVapourEvaporation (:1421; ThermoAtm.h:115-258). */
+athad_status athad_densities(athad_ctx* ctx, int set_water_profile, int with_evaporation);
+/* TurbulenceAtm(*this).run() and the damp_wiggles of its six fields
+ * (cAtmosphereModel.cpp:1426-1433; TurbulenceAtm.h:62-79). */
+athad_status athad_turbulence(athad_ctx* ctx);
+/* bcRadius, apply_wall_bc, bcTheta, bcPhi, bcScalarSurfSur, bcSolidGround
+ * (cAtmosphereModel.cpp:1440-1446). */
+athad_status athad_boundary(athad_ctx* ctx);
+/* solveRungeKutta_Atmosphere_Turb()  (cAtmosphereModel.cpp:1509; RungeKutta_Atm_Turb.cpp:44-273)
+ * in the context's update order. */
+athad_status athad_rk4(athad_ctx* ctx);
+/* polar_zonal_filter(u,v,w) and the radial Shapiro filters (cAtmosphereModel.cpp:1524-1575). */
+athad_status athad_filters(athad_ctx* ctx);
+/* UtilsAtm(*this).storeIntermediateData3D(1.0)  (cAtmosphereModel.cpp:1647). */
+athad_status athad_store(athad_ctx* ctx);
+
+/* One scoped iteration of run_3D_loop (cAtmosphereModel.cpp:1103-1647 minus the rows
+ * outside the path: ice scheme, moist convection, radiation, diagnostics, output), with
+ * the cadences of the reference (Poisson every 4th, moist block every 2nd iteration).
+ * Advances iter_n and total_iter_count. */
+athad_status athad_step(athad_ctx* ctx);
+/* n_steps iterations; use_graph replays a captured 4-iteration hipGraph. */
+athad_status athad_run(athad_ctx* ctx, int n_steps, int use_graph);
+/* Waits for the context's stream. */
+athad_status athad_sync(athad_ctx* ctx);
+
+/* --- measurement --------------------------------------------------------------------- */
+
+/* Per-kernel timing with HIP events on the context's stream: kernel ids as in
+ * athad_cond_amd/csrc/athad_kernels.h (0 = RK4). Collected while enabled. */
+athad_status athad_profile(athad_ctx* ctx, int kernel_id, int enable);
+athad_status athad_profile_result(athad_ctx* ctx, int kernel_id, double* total_ms, int* launches);
+
+/* Library version string. */
+const char* athad_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ATHAD_HIP_H */

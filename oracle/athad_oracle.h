@@ -1,0 +1,101 @@
+// CPU oracle for the ATHAD_COND time step.
+//
+// TEST INFRASTRUCTURE ONLY. This is a clean-room restatement of the reference's scoped
+// time step (RK4 of RHS_Atm_Turb, PressureSolverAtm red-black sweep, TurbulenceAtm k-omega
+// SST, SaturationAdjustment + SaturationH2O/AtmMixture, ThermoAtm::densities and
+// waterVapourEvaporation, BC_Atm, the damp_wiggles / polar / radial filters and the
+// storeIntermediateData3D copy). It exists to check the HIP product and to serve as the
+// CPU baseline; nothing in the product library links, loads or calls it.
+//
+// Grid-generic (any im x jm x km with a uniform 180/(jm-1) and 360/(km-1) degree spacing),
+// contiguous [i][j][k] storage with k fastest. Two RK4 update orders:
+//   ORDER_LEX     the reference's in-place sweep at OMP_NUM_THREADS=1
+//                 (RungeKutta_Atm_Turb.cpp:99-264 with one thread)
+//   ORDER_JACOBI  every cell integrates its four stages against the start-of-RK4
+//                 neighbourhood (order-free; the GPU throughput form)
+#pragma once
+#include <cstdint>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Field ids. Shared numbering with include/athad_hip.h.
+enum orc_field {
+    ORC_T = 0, ORC_U, ORC_V, ORC_W, ORC_C, ORC_CLOUD, ORC_ICE, ORC_GR, ORC_CO2, ORC_TKE, ORC_DIS,
+    ORC_TN, ORC_UN, ORC_VN, ORC_WN, ORC_CN, ORC_CLOUDN, ORC_ICEN, ORC_GRN, ORC_CO2N, ORC_TKEN, ORC_DISN,
+    ORC_P_DYN, ORC_P_STAT, ORC_R_HUMID, ORC_R_DRY, ORC_T_EQ, ORC_NUE, ORC_PROD, ORC_TKE_SRC,
+    ORC_DIS_SRC, ORC_AUX_U, ORC_AUX_V, ORC_AUX_W, ORC_S_C_C,
+    ORC_NFIELD
+};
+// 2-D (jm x km) fields
+enum orc_field2d { ORC2_VEL_STAR = 0, ORC2_T_TOP_INIT = 1, ORC2_NFIELD = 2 };
+
+enum orc_order { ORC_ORDER_LEX = 0, ORC_ORDER_JACOBI = 1 };
+
+// Operators that can be called one at a time (unit parity tests).
+enum orc_op {
+    ORC_OP_POISSON = 0, ORC_OP_SATADJ, ORC_OP_DAMP_MOIST, ORC_OP_MOIST_CLAMP, ORC_OP_DENSITIES,
+    ORC_OP_EVAPORATION, ORC_OP_TURBULENCE, ORC_OP_DAMP_TURB, ORC_OP_BC, ORC_OP_RK4,
+    ORC_OP_POLAR, ORC_OP_RADIAL, ORC_OP_STORE, ORC_OP_LEVEL_MEAN,
+    ORC_NOP
+};
+
+// Member parameters that vary across the ensemble (surface composition and state).
+typedef struct orc_member {
+    double t_surf_equator, t_surf_pole;  // [K]
+    double r_air;                        // [kg/m3] reference density at the sea surface
+    double x_H2O, x_CO2, x_N2;           // surface mole fractions
+    double c_0, co2_0, c_h2o_dry_top;    // [kg/kg]
+} orc_member;
+
+typedef struct orc_state orc_state;
+
+// Creates the nominal initial state (the init recipe documented in DESIGN.md).
+// member == NULL selects config_cond.xml's values.
+orc_state* orc_create(int im, int jm, int km, uint64_t seed, const orc_member* member);
+void       orc_destroy(orc_state* s);
+// Blank state of the right size with the grid/parameters set but all fields zero (for
+// loading a state from elsewhere).
+orc_state* orc_create_blank(int im, int jm, int km, const orc_member* member);
+// The state right after the thermodynamic init chain (surface parabola, dry column,
+// damp_wiggles(t), initWaterWapour, two co2Atmosphere + densities(true) passes).
+orc_state* orc_create_thermo(int im, int jm, int km, const orc_member* member);
+
+double* orc_field_ptr(orc_state* s, int field);
+double* orc_field2d_ptr(orc_state* s, int field);
+int     orc_total_iter(const orc_state* s);
+void    orc_set_iter(orc_state* s, int iter_n, int total_iter_count);
+double  orc_scalar(const orc_state* s, const char* name);
+double  orc_level_mean(const orc_state* s, int i);
+
+// One scoped time step (iter_n and total_iter_count advance by one).
+void orc_step(orc_state* s, int order);
+// n steps.
+void orc_run(orc_state* s, int n, int order);
+// One operator, with the cadence flags of the step it would run in. For ORC_OP_RK4 the
+// ramps are taken from the current total_iter_count.
+void orc_op(orc_state* s, int op, int order);
+
+// Individual thermodynamic functions (known-answer tests).
+double orc_p_sat(double T);
+double orc_p_subl(double T);
+double orc_p_auto(double T);
+double orc_latent_heat(double T);
+double orc_q_sat(double E, double p, double M_other);
+double orc_dqsat_dT(double T, double p, double M_other);
+double orc_moist_lapse(double T, double p, double M_other, double cp, double R_mix, double g);
+double orc_cp_of(double c, double co2, double T, double M_bg);
+double orc_R_of(double c, double co2, double R_bg);
+double orc_M_nonwater(double c, double co2, double M_bg);
+// Resolved composition: out[0..6] = M_mean, R_mix, q_H2O, q_CO2, q_bg, M_bg, R_bg.
+void   orc_resolve(double x_H2O, double x_CO2, double x_N2, double* out);
+// Per-cell saturation adjustment on arrays of n cells (the adjustSaturation + clampAndFade
+// pair at one level index i_level of the grid's layer heights).
+void   orc_satadj_cells(int n, double* t, double* c, double* cloud, double* ice, double* gr,
+                        const double* co2, const double* p_stat, double* s_c_c,
+                        double dt_dim, int interior);
+
+#ifdef __cplusplus
+}
+#endif

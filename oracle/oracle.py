@@ -1,0 +1,193 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and by the
+``cpu_baseline`` leg of bench.py. The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import struct
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+FIELDS = [
+    "t", "u", "v", "w", "c", "cloud", "ice", "gr", "co2", "tke", "dis",
+    "tn", "un", "vn", "wn", "cn", "cloudn", "icen", "grn", "co2n", "tken", "disn",
+    "p_dyn", "p_stat", "r_humid", "r_dry", "t_eq", "nue", "prod", "tke_source",
+    "dis_source", "aux_u", "aux_v", "aux_w", "S_c_c",
+]
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+FIELDS2D = ["vel_star", "t_top_init"]
+OPS = ["poisson", "satadj", "damp_moist", "moist_clamp", "densities", "evaporation",
+       "turbulence", "damp_turb", "bc", "rk4", "polar", "radial", "store", "level_mean"]
+ORDER = {"lex": 0, "jacobi": 1}
+
+
+class Member(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "t_surf_equator", "t_surf_pole", "r_air", "x_H2O", "x_CO2", "x_N2",
+        "c_0", "co2_0", "c_h2o_dry_top")]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {LIB_PATH} (run make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, d, i, u64 = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_uint64
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [i, i, i, u64, ctypes.POINTER(Member)]
+        L.orc_create_thermo.restype = vp
+        L.orc_create_thermo.argtypes = [i, i, i, ctypes.POINTER(Member)]
+        L.orc_create_blank.restype = vp
+        L.orc_create_blank.argtypes = [i, i, i, ctypes.POINTER(Member)]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_field_ptr.restype = dp
+        L.orc_field_ptr.argtypes = [vp, i]
+        L.orc_field2d_ptr.restype = dp
+        L.orc_field2d_ptr.argtypes = [vp, i]
+        L.orc_total_iter.restype = i
+        L.orc_total_iter.argtypes = [vp]
+        L.orc_set_iter.argtypes = [vp, i, i]
+        L.orc_scalar.restype = d
+        L.orc_scalar.argtypes = [vp, ctypes.c_char_p]
+        L.orc_level_mean.restype = d
+        L.orc_level_mean.argtypes = [vp, i]
+        L.orc_step.argtypes = [vp, i]
+        L.orc_run.argtypes = [vp, i, i]
+        L.orc_op.argtypes = [vp, i, i]
+        for name in ("orc_p_sat", "orc_p_subl", "orc_p_auto", "orc_latent_heat"):
+            getattr(L, name).restype = d
+            getattr(L, name).argtypes = [d]
+        L.orc_q_sat.restype = d
+        L.orc_q_sat.argtypes = [d, d, d]
+        L.orc_dqsat_dT.restype = d
+        L.orc_dqsat_dT.argtypes = [d, d, d]
+        L.orc_moist_lapse.restype = d
+        L.orc_moist_lapse.argtypes = [d, d, d, d, d, d]
+        L.orc_cp_of.restype = d
+        L.orc_cp_of.argtypes = [d, d, d, d]
+        L.orc_R_of.restype = d
+        L.orc_R_of.argtypes = [d, d, d]
+        L.orc_M_nonwater.restype = d
+        L.orc_M_nonwater.argtypes = [d, d, d]
+        L.orc_resolve.argtypes = [d, d, d, dp]
+        L.orc_satadj_cells.argtypes = [i, dp, dp, dp, dp, dp, dp, dp, dp, d, i]
+        _lib = L
+    return _lib
+
+
+def resolve(x_h2o, x_co2, x_n2):
+    out = (ctypes.c_double * 7)()
+    lib().orc_resolve(x_h2o, x_co2, x_n2, out)
+    keys = ["M_mean", "R_mix", "q_H2O", "q_CO2", "q_bg", "M_bg", "R_bg"]
+    return dict(zip(keys, list(out)))
+
+
+class OracleState:
+    """One grid's state in the oracle."""
+
+    def __init__(self, im, jm, km, seed=1234, member=None, blank=False, thermo_only=False):
+        L = lib()
+        m = ctypes.byref(member) if member is not None else None
+        if thermo_only:
+            self.h = L.orc_create_thermo(im, jm, km, m)
+        elif blank:
+            self.h = L.orc_create_blank(im, jm, km, m)
+        else:
+            self.h = L.orc_create(im, jm, km, seed, m)
+        if not self.h:
+            raise ValueError("oracle rejected the grid")
+        self.shape = (im, jm, km)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def field(self, name) -> np.ndarray:
+        """A live (writable) view of one field."""
+        p = lib().orc_field_ptr(self.h, FIELD_ID[name])
+        return np.ctypeslib.as_array(p, shape=self.shape)
+
+    def field2d(self, name) -> np.ndarray:
+        p = lib().orc_field2d_ptr(self.h, FIELDS2D.index(name))
+        return np.ctypeslib.as_array(p, shape=self.shape[1:])
+
+    def snapshot(self) -> dict:
+        d = {n: self.field(n).copy() for n in FIELDS}
+        for n in FIELDS2D:
+            d[n] = self.field2d(n).copy()
+        return d
+
+    def load(self, d: dict):
+        for n in FIELDS:
+            if n in d:
+                self.field(n)[...] = d[n]
+        for n in FIELDS2D:
+            if n in d:
+                self.field2d(n)[...] = d[n]
+
+    @property
+    def total_iter(self):
+        return lib().orc_total_iter(self.h)
+
+    def set_iter(self, iter_n, total):
+        lib().orc_set_iter(self.h, iter_n, total)
+
+    def scalar(self, name):
+        return lib().orc_scalar(self.h, name.encode())
+
+    def step(self, order="lex"):
+        lib().orc_step(self.h, ORDER[order])
+
+    def run(self, n, order="lex"):
+        lib().orc_run(self.h, n, ORDER[order])
+
+    def op(self, name, order="lex"):
+        lib().orc_op(self.h, OPS.index(name), ORDER[order])
+
+    def level_mean(self):
+        return np.array([lib().orc_level_mean(self.h, i) for i in range(self.shape[0])])
+
+    # Raw state file shared with oracle/ref_harness.cpp
+    def save_bin(self, path, iter_n=0):
+        im, jm, km = self.shape
+        with open(path, "wb") as f:
+            f.write(struct.pack("<5i", im, jm, km, iter_n, self.total_iter))
+            for n in FIELDS:
+                f.write(np.ascontiguousarray(self.field(n), dtype=np.float64).tobytes())
+            for n in FIELDS2D:
+                f.write(np.ascontiguousarray(self.field2d(n), dtype=np.float64).tobytes())
+
+
+def read_bin(path):
+    with open(path, "rb") as f:
+        im, jm, km, iter_n, total = struct.unpack("<5i", f.read(20))
+        n = im * jm * km
+        d = {}
+        for name in FIELDS:
+            d[name] = np.frombuffer(f.read(n * 8), dtype=np.float64).reshape(im, jm, km).copy()
+        for name in FIELDS2D:
+            d[name] = np.frombuffer(f.read(jm * km * 8), dtype=np.float64).reshape(jm, km).copy()
+    d["_iter_n"], d["_total"] = iter_n, total
+    return d
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.sqrt(np.sum(b * b))
+    num = np.sqrt(np.sum((a - b) ** 2))
+    if den == 0.0:
+        return float(num)
+    return float(num / den)

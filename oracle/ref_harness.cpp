@@ -1,0 +1,255 @@
+// Drives the reference's own operators (compiled from /root/reference sources by
+// oracle/Makefile.ref) on a state written by the oracle, and writes the result back, so the
+// oracle restatement can be checked against the reference. TEST INFRASTRUCTURE ONLY.
+//
+// Access to cAtmosphereModel's private members goes through the friend declaration the
+// reference already carries for its offline test driver (cAtmosphereModel.h:67).
+//
+// usage: ref_harness <in.bin> <out.bin> <mode> [n]
+//   mode = steps            n scoped iterations (the IN rows of run_3D_loop)
+//   mode = op:<name>        one operator: poisson satadj damp_moist moist_clamp densities
+//                           evaporation turbulence damp_turb bc rk4 polar radial store
+//   mode = init             the reference's thermodynamic init chain on a blank state
+// State file: int32 im,jm,km,iter_n,total_iter; then 35 fields of im*jm*km doubles
+// (athad_oracle.h order); then vel_star and t_top_init (jm*km doubles each).
+#include "cAtmosphereModel.h"
+#include "PressureSolverAtm.h"
+#include "SaturationAdjustment.h"
+#include "ThermoAtm.h"
+#include "UtilsAtm.h"
+#include "BC_Atm.h"
+#include "TurbulenceAtm.h"
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+class RadiationSelfTest {
+public:
+    static std::vector<Array*> fields(cAtmosphereModel& m) {
+        return {&m.t, &m.u, &m.v, &m.w, &m.c, &m.cloud, &m.ice, &m.gr, &m.co2, &m.tke, &m.dis,
+                &m.tn, &m.un, &m.vn, &m.wn, &m.cn, &m.cloudn, &m.icen, &m.grn, &m.co2n, &m.tken, &m.disn,
+                &m.p_dyn, &m.p_stat, &m.r_humid, &m.r_dry, &m.t_eq, &m.nue, &m.prod, &m.tke_source,
+                &m.dis_source, &m.aux_u, &m.aux_v, &m.aux_w, &m.S_c_c};
+    }
+
+    static void setup(cAtmosphereModel& m) {
+        m.initComposition();
+        m.m_current_time = m.m_time_list.insert(0.0f).first;   // time slice Ma = 0
+        UtilsAtm(m).resetArrays();
+        m.dt = m.dt_visc;
+        m.turb_model = "k_omega_SST";
+        m.use_turbulence_model = true;
+        m.use_k_epsilon_turbulence_model = false;
+        m.use_k_omega_turbulence_model = false;
+        m.use_k_omega_SST_turbulence_model = true;
+        m.initGridCoordinates();
+        m.initMetricRadius();
+        m.init_layer_heights();
+        m.re_turb = 0.4 * 0.1 / 1.5e-5;
+        m.diffusion_ramp = 1.0;
+        m.inviscid_phase = false;
+        m.vbudget_capture = false;
+        m.wbudget_capture = false;
+        m.t_top_init.assign(m.jm, std::vector<double>(m.km, 0.0));
+    }
+
+    static bool load(cAtmosphereModel& m, const char* fn) {
+        std::ifstream f(fn, std::ios::binary);
+        int32_t hdr[5];
+        f.read((char*)hdr, sizeof(hdr));
+        if (!f || hdr[0] != m.im || hdr[1] != m.jm || hdr[2] != m.km) {
+            std::fprintf(stderr, "state %s does not match the compiled grid %dx%dx%d\n", fn, m.im, m.jm, m.km);
+            return false;
+        }
+        m.iter_n = hdr[3];
+        m.total_iter_count = hdr[4];
+        std::vector<double> buf((size_t)m.im * m.jm * m.km);
+        for (Array* a : fields(m)) {
+            f.read((char*)buf.data(), buf.size() * 8);
+            size_t q = 0;
+            for (int i = 0; i < m.im; i++)
+                for (int j = 0; j < m.jm; j++)
+                    for (int k = 0; k < m.km; k++) a->x[i][j][k] = buf[q++];
+        }
+        std::vector<double> b2((size_t)m.jm * m.km);
+        f.read((char*)b2.data(), b2.size() * 8);
+        for (int j = 0; j < m.jm; j++)
+            for (int k = 0; k < m.km; k++) m.vel_star.y[j][k] = b2[(size_t)j * m.km + k];
+        f.read((char*)b2.data(), b2.size() * 8);
+        for (int j = 0; j < m.jm; j++)
+            for (int k = 0; k < m.km; k++) m.t_top_init[j][k] = b2[(size_t)j * m.km + k];
+        return (bool)f;
+    }
+
+    static void save(cAtmosphereModel& m, const char* fn) {
+        std::ofstream f(fn, std::ios::binary);
+        int32_t hdr[5] = {m.im, m.jm, m.km, m.iter_n, m.total_iter_count};
+        f.write((const char*)hdr, sizeof(hdr));
+        std::vector<double> buf((size_t)m.im * m.jm * m.km);
+        for (Array* a : fields(m)) {
+            size_t q = 0;
+            for (int i = 0; i < m.im; i++)
+                for (int j = 0; j < m.jm; j++)
+                    for (int k = 0; k < m.km; k++) buf[q++] = a->x[i][j][k];
+            f.write((const char*)buf.data(), buf.size() * 8);
+        }
+        std::vector<double> b2((size_t)m.jm * m.km);
+        for (int j = 0; j < m.jm; j++)
+            for (int k = 0; k < m.km; k++) b2[(size_t)j * m.km + k] = m.vel_star.y[j][k];
+        f.write((const char*)b2.data(), b2.size() * 8);
+        for (int j = 0; j < m.jm; j++)
+            for (int k = 0; k < m.km; k++) b2[(size_t)j * m.km + k] = m.t_top_init[j][k];
+        f.write((const char*)b2.data(), b2.size() * 8);
+    }
+
+    // The loop glue of run_3D_loop that is written inline there (cap_S, the microphysics
+    // clamp and the water-vapour ceiling, cAtmosphereModel.cpp:1278-1399), repeated here
+    // because a lambda inside run_3D_loop cannot be called from outside.
+    static void moist_clamp(cAtmosphereModel& m) {
+        constexpr double S_max = 1.0e-3;
+        Array* caps[] = {&m.S_v, &m.S_c, &m.S_i, &m.S_r, &m.S_s, &m.S_g, &m.S_c_c};
+        for (Array* S : caps)
+            for (int i = 0; i < m.im; i++)
+                for (int j = 0; j < m.jm; j++)
+                    for (int k = 0; k < m.km; k++) {
+                        double v = S->x[i][j][k];
+                        if (!AtomUtils::is_finite_safe(v)) S->x[i][j][k] = 0.0;
+                        else if (v > S_max) S->x[i][j][k] = S_max;
+                        else if (v < -S_max) S->x[i][j][k] = -S_max;
+                    }
+        for (int i = 0; i < m.im; i++)
+            for (int j = 0; j < m.jm; j++)
+                for (int k = 0; k < m.km; k++) {
+                    double cv = m.c.x[i][j][k], cld = m.cloud.x[i][j][k], ic = m.ice.x[i][j][k];
+                    if (!AtomUtils::is_finite_safe(cv)) cv = 0.0;
+                    if (!AtomUtils::is_finite_safe(cld)) cld = 0.0;
+                    if (!AtomUtils::is_finite_safe(ic)) ic = 0.0;
+                    if (cv < 0.0) cv = 0.0;
+                    if (cld < 0.0) cld = 0.0;
+                    if (ic < 0.0) ic = 0.0;
+                    const double cv_max = std::max(0.0, 1.0 - m.co2.x[i][j][k]);
+                    if (cv > cv_max) cv = cv_max;
+                    const double sum = cld + ic;
+                    if (sum > m.cloud_cap) { const double sc = m.cloud_cap / sum; cld *= sc; ic *= sc; }
+                    m.c.x[i][j][k] = cv; m.cloud.x[i][j][k] = cld; m.ice.x[i][j][k] = ic;
+                }
+        for (int i = 0; i < m.im; i++)
+            for (int j = 0; j < m.jm; j++)
+                for (int k = 0; k < m.km; k++) {
+                    if (m.c.x[i][j][k] < 0.0) m.c.x[i][j][k] = 0.0;
+                    const double cc = std::max(0.0, 1.0 - m.co2.x[i][j][k]);
+                    if (m.c.x[i][j][k] > cc) m.c.x[i][j][k] = cc;
+                }
+    }
+
+    static void damp(cAtmosphereModel& m, Array& a) { AtomUtils::damp_wiggles(a, &m.i_topography, true, true, true); }
+
+    static void op(cAtmosphereModel& m, const std::string& name) {
+        m.buoyancy_ramp = std::min(1.0, (double)m.total_iter_count / 300.0);
+        if (name == "poisson") PressureSolverAtm(m).run();
+        else if (name == "satadj") SaturationAdjustment(m).run();
+        else if (name == "damp_moist") { damp(m, m.t); damp(m, m.ice); damp(m, m.c); damp(m, m.cloud); }
+        else if (name == "moist_clamp") moist_clamp(m);
+        else if (name == "densities") ThermoAtm(m).densities();
+        else if (name == "evaporation") ThermoAtm(m).waterVapourEvaporation();
+        else if (name == "turbulence") TurbulenceAtm(m).run();
+        else if (name == "damp_turb") {
+            damp(m, m.tke); damp(m, m.dis); damp(m, m.nue); damp(m, m.prod); damp(m, m.tke_source); damp(m, m.dis_source);
+        } else if (name == "bc") {
+            BC_Atm(m).bcRadius();
+            TurbulenceAtm(m).apply_wall_bc();
+            BC_Atm(m).bcTheta();
+            BC_Atm(m).bcPhi();
+            BC_Atm(m).bcScalarSurfSur();
+            BC_Atm(m).bcSolidGround();
+        } else if (name == "rk4") m.solveRungeKutta_Atmosphere_Turb();
+        else if (name == "polar") {
+            AtomUtils::polar_zonal_filter(m.u, m.the.z, &m.i_topography, 30.0, 12, 3.0);
+            AtomUtils::polar_zonal_filter(m.v, m.the.z, &m.i_topography, 30.0, 12, 3.0);
+            AtomUtils::polar_zonal_filter(m.w, m.the.z, &m.i_topography, 30.0, 12, 3.0);
+        } else if (name == "radial") {
+            AtomUtils::orographic_shapiro_filter(m.u, m.i_topography, 2, 3, 3);
+            AtomUtils::orographic_shapiro_filter(m.v, m.i_topography, 2, 3, 3);
+            AtomUtils::orographic_shapiro_filter(m.w, m.i_topography, 2, 3, 3);
+            AtomUtils::radial_shapiro_filter(m.u, m.i_topography, 2, 1.0);
+            AtomUtils::radial_shapiro_filter_ho(m.v, m.i_topography, 2, 1.0);
+            AtomUtils::radial_shapiro_filter_ho(m.w, m.i_topography, 2, 1.0);
+            AtomUtils::orographic_radial_shapiro_filter(m.t, m.i_topography, 2, 10, 2);
+            AtomUtils::orographic_radial_shapiro_filter(m.c, m.i_topography, 2, 10, 2);
+            AtomUtils::orographic_radial_shapiro_filter(m.cloud, m.i_topography, 2, 10, 2);
+            AtomUtils::orographic_radial_shapiro_filter(m.ice, m.i_topography, 2, 10, 2);
+            AtomUtils::steep_massif_field_smoothing(m.u, m.i_topography, 6, 3, 30, 1);
+            AtomUtils::steep_massif_field_smoothing(m.v, m.i_topography, 6, 3, 30, 1);
+            AtomUtils::steep_massif_field_smoothing(m.w, m.i_topography, 6, 3, 30, 1);
+            AtomUtils::steep_massif_field_smoothing(m.p_dyn, m.i_topography, 6, 3, 30, 1);
+            AtomUtils::coastal_velocity_sponge(m.u, m.i_topography, 3.0, 1.0, 12);
+            AtomUtils::coastal_velocity_sponge(m.v, m.i_topography, 3.0, 1.0, 12);
+            AtomUtils::coastal_velocity_sponge(m.w, m.i_topography, 3.0, 1.0, 12);
+        } else if (name == "store") UtilsAtm(m).storeIntermediateData3D(1.0);
+        else std::fprintf(stderr, "unknown op %s\n", name.c_str());
+    }
+
+    // One iteration of run_3D_loop restricted to the scoped rows (cAtmosphereModel.cpp:1103-1647).
+    static void step(cAtmosphereModel& m) {
+        m.iter_n++;
+        m.total_iter_count++;
+        m.diffusion_ramp = 1.0;
+        m.dt = m.dt_visc;
+        if (m.iter_n % 4 == 0) op(m, "poisson");
+        if (m.iter_n % 2 == 0) {
+            op(m, "satadj");
+            op(m, "damp_moist");
+            op(m, "moist_clamp");
+            op(m, "densities");
+            op(m, "evaporation");
+            op(m, "turbulence");
+            op(m, "damp_turb");
+        }
+        op(m, "bc");
+        op(m, "rk4");
+        op(m, "polar");
+        op(m, "radial");
+        op(m, "store");
+    }
+
+    // RunTimeSlice's thermodynamic init chain, cAtmosphereModel.cpp:527-568 (without
+    // initCloudIce; condensate stays zero as resetArrays left it).
+    static void init_chain(cAtmosphereModel& m) {
+        m.initTemperatureData(0);
+        AtomUtils::damp_wiggles(m.t, &m.i_topography, true, true, true);
+        for (int j = 0; j < m.jm; j++)
+            for (int k = 0; k < m.km; k++) m.t_top_init[j][k] = m.t.x[m.im - 1][j][k];
+        m.initWaterWapour();
+        for (int pass = 0; pass < 2; pass++) {
+            ThermoAtm(m).co2Atmosphere();
+            ThermoAtm(m).densities(true);
+        }
+    }
+};
+
+int main(int argc, char** argv) {
+    if (argc < 4) {
+        std::fprintf(stderr, "usage: %s in.bin out.bin mode [n]\n", argv[0]);
+        return 2;
+    }
+    std::ofstream devnull("/dev/null");
+    std::streambuf* old = std::cout.rdbuf(devnull.rdbuf());
+    cAtmosphereModel* m = new cAtmosphereModel();
+    RadiationSelfTest::setup(*m);
+    const std::string mode = argv[3];
+    if (mode != "init" && !RadiationSelfTest::load(*m, argv[1])) return 1;
+    if (mode == "steps") {
+        const int n = (argc > 4) ? std::atoi(argv[4]) : 1;
+        for (int q = 0; q < n; q++) RadiationSelfTest::step(*m);
+    } else if (mode.rfind("op:", 0) == 0) {
+        RadiationSelfTest::op(*m, mode.substr(3));
+    } else if (mode == "init") {
+        RadiationSelfTest::init_chain(*m);
+    }
+    RadiationSelfTest::save(*m, argv[2]);
+    std::cout.rdbuf(old);
+    return 0;
+}

@@ -1,0 +1,114 @@
+"""Pins the CPU oracle against the reference's own code (oracle/_ref/ref_harness).
+
+TEST INFRASTRUCTURE ONLY. Runs in the build container, where /root/reference exists; the
+reference harness is compiled at the reference's built-in grid (61 x 181 x 361), so every
+comparison runs there. For each operator the oracle and the reference start from the same
+state (written by the oracle) and the per-field relative L2 difference is reported; then
+whole scoped steps are compared (lex order = the reference at OMP_NUM_THREADS=1).
+
+    python oracle/validate_ref.py [--steps N] [--out report.json]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import oracle as O  # noqa: E402
+
+HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+GRID = (61, 181, 361)
+ORDER_FREE = {"poisson", "satadj", "damp_moist", "moist_clamp", "densities", "evaporation",
+              "damp_turb", "bc", "polar", "radial", "store"}
+
+
+def run_ref(state_in, mode, n=None, threads=1, workdir=None):
+    out = os.path.join(workdir, "ref_out.bin")
+    cmd = [HARNESS, state_in, out, mode] + ([str(n)] if n is not None else [])
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    subprocess.run(cmd, check=True, env=env, cwd=workdir, stdout=subprocess.DEVNULL)
+    return O.read_bin(out)
+
+
+def compare(ora: dict, ref: dict, names=None):
+    res = {}
+    for n in names or O.FIELDS + O.FIELDS2D:
+        a, b = ora[n], ref[n]
+        res[n] = {"rel_l2": O.rel_l2(a, b), "max_abs": float(np.max(np.abs(a - b)))}
+    return res
+
+
+def worst(res, skip=("S_c_c",)):
+    return max(v["rel_l2"] for k, v in res.items() if k not in skip)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--out", default=os.path.join(HERE, "_ref", "validation.json"))
+    ap.add_argument("--spin", type=int, default=6, help="oracle steps before the per-operator checks")
+    args = ap.parse_args()
+    if not os.path.exists(HARNESS):
+        sys.exit("reference harness not built: make -C oracle -f Makefile.ref")
+    report = {"grid": GRID}
+    wd = tempfile.mkdtemp(prefix="athad_val_")
+    im, jm, km = GRID
+
+    # 1. thermodynamic init chain
+    t0 = time.time()
+    st = O.OracleState(im, jm, km, thermo_only=True)
+    ora = st.snapshot()
+    blank = os.path.join(wd, "blank.bin")
+    st.save_bin(blank)
+    ref = run_ref(blank, "init", threads=8, workdir=wd)
+    names = ["t", "p_stat", "r_humid", "r_dry", "c", "co2", "t_top_init"]
+    report["init"] = compare(ora, ref, names)
+    print(f"init chain: worst rel L2 {worst(report['init']):.3e}  ({time.time()-t0:.1f}s)", flush=True)
+    del st
+
+    # 2. per-operator checks on a spun-up state
+    st = O.OracleState(im, jm, km, seed=1234)
+    st.run(args.spin, "lex")
+    base = os.path.join(wd, "base.bin")
+    iter_n = args.spin
+    report["ops"] = {}
+    for op in ["poisson", "satadj", "damp_moist", "moist_clamp", "densities", "evaporation",
+               "turbulence", "damp_turb", "bc", "rk4", "polar", "radial", "store"]:
+        t0 = time.time()
+        s2 = O.OracleState(im, jm, km, blank=True)
+        s2.load(st.snapshot())
+        s2.set_iter(iter_n, st.total_iter)
+        s2.save_bin(base, iter_n=iter_n)
+        s2.op(op, "lex")
+        ref = run_ref(base, "op:" + op, threads=8 if op in ORDER_FREE else 1, workdir=wd)
+        report["ops"][op] = compare(s2.snapshot(), ref)
+        print(f"op {op:12s}: worst rel L2 {worst(report['ops'][op]):.3e}  ({time.time()-t0:.1f}s)", flush=True)
+        del s2
+
+    # 3. whole scoped steps from the initial state (lex = 1 reference thread)
+    t0 = time.time()
+    st = O.OracleState(im, jm, km, seed=1234)
+    init = os.path.join(wd, "init.bin")
+    st.save_bin(init)
+    ref = run_ref(init, "steps", n=args.steps, threads=1, workdir=wd)
+    st.run(args.steps, "lex")
+    report["steps"] = {"n": args.steps, "fields": compare(st.snapshot(), ref)}
+    print(f"{args.steps} steps: worst rel L2 {worst(report['steps']['fields']):.3e}  ({time.time()-t0:.1f}s)",
+          flush=True)
+    for k, v in report["steps"]["fields"].items():
+        print(f"   {k:12s} {v['rel_l2']:.3e}  max {v['max_abs']:.3e}")
+    with open(args.out, "w") as f:
+        json.dump(report, f, indent=1)
+    print("report:", args.out)
+
+
+if __name__ == "__main__":
+    main()

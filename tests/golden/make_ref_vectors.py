@@ -1,0 +1,98 @@
+"""Writes the golden vectors that pin the oracle, from the reference's own compiled code.
+
+Runs only where the reference sources exist (the build container): it drives
+oracle/_ref/ref_harness (built by oracle/Makefile.ref from the reference's sources at their
+built-in grid 61 x 181 x 361) and stores, per check, the reference's output at a fixed
+sample of cells plus per-field sums and L2 norms. The inputs are regenerated by the oracle
+in the tests from the same seed, so only outputs are stored.
+
+    python tests/golden/make_ref_vectors.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle as O  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+GRID = (61, 181, 361)
+SEED = 1234
+SPIN = 2          # Jacobi-order oracle steps that make the operator inputs
+NSAMPLE = 512
+OPS = ["poisson", "satadj", "damp_moist", "moist_clamp", "densities", "evaporation", "turbulence",
+       "damp_turb", "bc", "rk4", "polar", "radial"]
+ORDER_FREE = {"poisson", "satadj", "damp_moist", "moist_clamp", "densities", "evaporation",
+              "damp_turb", "bc", "polar", "radial"}
+
+
+def sample_index(n, seed=99):
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(n, size=NSAMPLE, replace=False))
+
+
+def run_ref(state_in, mode, wd, n=None, threads=1):
+    out = os.path.join(wd, "out.bin")
+    cmd = [HARNESS, state_in, out, mode] + ([str(n)] if n is not None else [])
+    subprocess.run(cmd, check=True, cwd=wd, stdout=subprocess.DEVNULL,
+                   env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
+    return O.read_bin(out)
+
+
+def pack(prefix, d, idx, store):
+    for n in O.FIELDS:
+        a = d[n].reshape(-1)
+        store[f"{prefix}/{n}/sample"] = a[idx]
+        store[f"{prefix}/{n}/sum"] = np.array(np.sum(a))
+        store[f"{prefix}/{n}/l2"] = np.array(np.sqrt(np.sum(a * a)))
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference harness first: make -C oracle -f Makefile.ref")
+    wd = tempfile.mkdtemp(prefix="athad_golden_")
+    n = GRID[0] * GRID[1] * GRID[2]
+    idx = sample_index(n)
+    out = {"grid": np.array(GRID), "seed": np.array(SEED), "spin": np.array(SPIN), "index": idx}
+
+    # 1. thermodynamic init chain: whole columns at five latitudes
+    st = O.OracleState(*GRID, thermo_only=True)
+    blank = os.path.join(wd, "blank.bin")
+    st.save_bin(blank)
+    ref = run_ref(blank, "init", wd, threads=8)
+    for name in ["t", "p_stat", "r_humid", "r_dry", "c", "co2"]:
+        out[f"init/{name}"] = ref[name][:, [0, 45, 90, 135, 180], 0]
+    out["init/t_top_init"] = ref["t_top_init"][:, 0]
+    del st
+
+    # 2. two whole scoped steps from the initial state, lex order (1 reference thread)
+    st = O.OracleState(*GRID, seed=SEED)
+    init = os.path.join(wd, "init.bin")
+    st.save_bin(init)
+    ref = run_ref(init, "steps", wd, n=2, threads=1)
+    pack("steps2", ref, idx, out)
+    print("steps done", flush=True)
+
+    # 3. one operator at a time on the Jacobi-spun-up state
+    st.run(SPIN, "jacobi")
+    base = os.path.join(wd, "base.bin")
+    st.save_bin(base, iter_n=SPIN)
+    for op in OPS:
+        ref = run_ref(base, "op:" + op, wd, threads=8 if op in ORDER_FREE else 1)
+        pack(f"op/{op}", ref, idx, out)
+        print("op", op, flush=True)
+
+    path = os.path.join(HERE, "ref_vectors.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()
