@@ -468,7 +468,7 @@ athad_status athad_create(int n_members, const athad_grid* grids, const athad_me
         const size_t n2 = (size_t)m.g.jm * m.g.km;
         ok = ok && alloc((void**)&m.vel_star, n2 * 8) && alloc((void**)&m.t_top, n2 * 8) &&
              alloc((void**)&m.t_ref, m.g.im * 8) && alloc((void**)&m.tables, tab.size() * 8) &&
-             alloc((void**)&m.scratch, 4096 * 8) && alloc((void**)&m.counters, 64) &&
+             alloc((void**)&m.scratch, (size_t)std::max(8192, 2 * 32 * m.g.im) * 8) && alloc((void**)&m.counters, 64) &&
              alloc((void**)&m.passes, passes.size() * 4);
         if (!ok) { g_create_error = "device allocation failed"; free_ctx(c); return ATHAD_ERR_ALLOC; }
         for (int f = 0; f < NFIELD; f++) hipMemsetAsync(m.field[f], 0, field_bytes(c, m, f), c->stream);

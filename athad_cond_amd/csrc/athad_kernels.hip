@@ -55,30 +55,50 @@ __global__ void k_advance(MemberDesc* md) {
 // Level-mean temperature (RungeKutta_Atm_Turb.cpp:24-41). One block per (level, member);
 // fixed reduction tree, so the result is deterministic run to run.
 // ------------------------------------------------------------------------------------
+constexpr int kLevelChunks = 32;
+
+// pass 1: block (chunk c, level i) sums a fixed contiguous chunk of the level in a fixed tree
 template <typename S>
 __global__ __launch_bounds__(256) void k_level_mean(const MemberDesc* md) {
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
-    const int i = blockIdx.x;
+    const int i = blockIdx.y, c = blockIdx.x;
     if (i >= K.im) return;
     const S* t = fp<S>(M, F_T);
     __shared__ double ss[256], sw[256];
-    double sum = 0.0, wsum = 0.0;
     const int njk = K.jm * K.km;
-    for (int q = threadIdx.x; q < njk; q += blockDim.x) {
-        const int j = q / K.km;
-        const double w = M.sinr[j];
+    const int per = (njk + kLevelChunks - 1) / kLevelChunks;
+    const int q0 = c * per, q1 = min(njk, q0 + per);
+    double sum = 0.0, wsum = 0.0;
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        const double w = M.sinr[q / K.km];
         sum += w * (double)t[(size_t)i * njk + q];
         wsum += w;
     }
     ss[threadIdx.x] = sum;
     sw[threadIdx.x] = wsum;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) { ss[threadIdx.x] += ss[threadIdx.x + s]; sw[threadIdx.x] += sw[threadIdx.x + s]; }
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) { ss[threadIdx.x] += ss[threadIdx.x + st]; sw[threadIdx.x] += sw[threadIdx.x + st]; }
         __syncthreads();
     }
-    if (threadIdx.x == 0) M.t_ref[i] = (sw[0] > 0.0) ? ss[0] / sw[0] : 1.0;
+    if (threadIdx.x == 0) {
+        M.scratch[2 * (i * kLevelChunks + c)] = ss[0];
+        M.scratch[2 * (i * kLevelChunks + c) + 1] = sw[0];
+    }
+}
+
+// pass 2: ordered sum of the chunks
+__global__ void k_level_mean_final(const MemberDesc* md) {
+    const MemberDesc& M = md[blockIdx.z];
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    if (i >= M.K.im) return;
+    double sum = 0.0, wsum = 0.0;
+    for (int c = 0; c < kLevelChunks; c++) {
+        sum += M.scratch[2 * (i * kLevelChunks + c)];
+        wsum += M.scratch[2 * (i * kLevelChunks + c) + 1];
+    }
+    M.t_ref[i] = (wsum > 0.0) ? sum / wsum : 1.0;
 }
 
 // ------------------------------------------------------------------------------------
@@ -575,6 +595,49 @@ __global__ void k_p_ceiling(const MemberDesc* md, int gate) {
 // SaturationAdjustment (SaturationAdjustment.h:62-451; IceSchemeCommon.h:121-190) and the
 // cap on S_c_c (cAtmosphereModel.cpp:1278-1295). Pointwise, FP64 arithmetic.
 // ------------------------------------------------------------------------------------
+// Newton-loop helpers of the saturation adjustment: the same functions as th_* with the
+// shared sub-expressions evaluated once (results agree with th_* to a few ulp).
+ATHAD_DEV void sat_species_cp(double T, double Mb, double& cpH, double& cpC, double& cpB) {
+    const double t = tmin(tmax(T, 298.0), 2000.0) * 1.0e-3;
+    const double t2 = t * t, t3 = t2 * t, it2 = 1.0 / t2;
+    cpH = (30.09200 + 6.832514 * t + 6.793435 * t2 + -2.534480 * t3 + 0.082139 * it2) * (1.0 / kMH2O);
+    cpC = (24.99735 + 55.18696 * t + -33.69137 * t2 + 7.948387 * t3 + -0.136638 * it2) * (1.0 / kMCO2);
+    cpB = (28.98641 + 1.853978 * t + -9.647459 * t2 + 16.63537 * t3 + 0.000117 * it2) / Mb;
+}
+ATHAD_DEV double sat_mix_cp(double c, double co2, double cpH, double cpC, double cpB) {
+    double qv, qc, qb;
+    th_split(c, co2, qv, qc, qb);
+    return qv * cpH + qc * cpC + qb * cpB;
+}
+ATHAD_DEV double sat_L_vap(double T) {
+    if (T >= kTcrit) return 0.0;
+    return 2.501e6 * exp(0.38 * log((kTcrit - T) / (kTcrit - 273.15)));
+}
+ATHAD_DEV double sat_p_subl(double T) {
+    if (T <= 0.0) return 0.0;
+    const double th = T / kTtriple;
+    const double lth = log(th);
+    const double s = -0.212144006e2 * exp(0.333333333e-2 * lth) + 0.273203819e2 * exp(0.120666667e1 * lth)
+                   + -0.610598130e1 * exp(0.170333333e1 * lth);
+    return kPtriple * exp(s / th);
+}
+// exact q_sat with its mole fraction x and denominator returned for the derivative factor
+ATHAD_DEV double sat_q(double E, double p, double Mo, double& x, double& den) {
+    x = 0.0; den = 0.0;
+    if (E == kNoSat) return 1.0;
+    if (!(p > 0.0)) return 0.0;
+    if (E >= p) return 1.0;
+    x = E / p;
+    const double num = x * kMH2O;
+    den = num + (1.0 - x) * Mo;
+    return (den > 0.0) ? num / den : 1.0;
+}
+ATHAD_DEV double sat_A_from(double E, double p, double Mo, double x, double den) {
+    if (E == kNoSat || !(p > 0.0) || E >= p) return 0.0;
+    if (!(den > 0.0)) return 0.0;
+    return x * kMH2O * Mo / (den * den);
+}
+
 template <typename S>
 __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate) {
     const MemberDesc& M = md[blockIdx.z];
@@ -603,6 +666,12 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate) 
             if ((qv_old > qs0 && alpha > 0.01) || (qv_old < qs0 && (qc_old > 1e-12 || qi_old > 1e-12))) {
                 double qvb = qv_old, qcb = qc_old, qib = qi_old, qv_hyp = qvb;
                 const double T_orig = t * t_0;
+                // Per-temperature quantities are evaluated once per Newton temperature and
+                // reused: the species heat capacities and L(T) of iteration k+1's update are
+                // those computed at the end of iteration k (same T).
+                double cpH, cpC, cpB, Lv;
+                sat_species_cp(T, Mb, cpH, cpC, cpB);
+                Lv = sat_L_vap(T);
                 for (int it = 1; it <= 20; it++) {
                     const double CND = tmax(0.0, tmin(1.0, (T - t_00) * t_range_inv));
                     const double DEP = 1.0 - CND;
@@ -612,16 +681,24 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate) 
                     qvb += dqv;
                     qcb = tmax(0.0, qcb - dcnd);
                     qib = tmax(0.0, qib - ddep);
-                    const double cp_loc = th_cp_of(qvb, co2, T, Mb);
-                    T -= (th_L_vap(T) * dcnd + th_L_sub(T) * ddep) / cp_loc;
-                    const double Es = th_p_sat(T), Ei = th_p_subl(T);
-                    const double qs = th_q_sat(Es, p_local, M_other), qi = th_q_sat(Ei, p_local, M_other);
+                    const double cp_loc = sat_mix_cp(qvb, co2, cpH, cpC, cpB);
+                    T -= (Lv * dcnd + (Lv + 3.337e5) * ddep) / cp_loc;
+                    // new temperature
+                    sat_species_cp(T, Mb, cpH, cpC, cpB);
+                    Lv = sat_L_vap(T);
+                    const double Es = th_p_sat(T), Ei = sat_p_subl(T);
+                    double xs, dens, xi, deni;
+                    const double qs = sat_q(Es, p_local, M_other, xs, dens);
+                    const double qi = sat_q(Ei, p_local, M_other, xi, deni);
                     const double qsum = qcb + qib;
                     const double target = (qsum > 1e-12) ? (qcb * qs + qib * qi) / qsum : ((T >= t_0) ? qs : qi);
-                    const double cp_g = th_cp_of(qvb, co2, T, Mb);
-                    const double Lc = th_L_vap(T), Ld = th_L_sub(T);
-                    const double gain = CND * (Lc / cp_g) * th_dqdT(Es, Lc, T, p_local, M_other)
-                                      + DEP * (Ld / cp_g) * th_dqdT(Ei, Ld, T, p_local, M_other);
+                    const double cp_g = sat_mix_cp(qvb, co2, cpH, cpC, cpB);
+                    const double Lc = Lv, Ld = Lv + 3.337e5;
+                    const double inv_RT2 = (T > 0.0) ? 1.0 / (kRH2O * T * T) : 0.0;
+                    const double Ac = sat_A_from(Es, p_local, M_other, xs, dens);
+                    const double Ad = sat_A_from(Ei, p_local, M_other, xi, deni);
+                    const double inv_cp = 1.0 / cp_g;
+                    const double gain = CND * (Lc * inv_cp) * (Ac * Lc * inv_RT2) + DEP * (Ld * inv_cp) * (Ad * Ld * inv_RT2);
                     const double om = 1.0 / (1.0 + gain);
                     qv_hyp = qvb + om * (target - qvb);
                     if (fabs(qvb / qv_hyp - 1.0) <= 1.0e-6) break;
@@ -1356,7 +1433,8 @@ void launch_advance(const Batch& b) {
     hipLaunchKernelGGL(k_advance, dim3(1, 1, b.nmem), dim3(64), 0, b.stream, b.d_md);
 }
 void launch_level_mean(const Batch& b) {
-    ATHAD_DISPATCH(k_level_mean, dim3(b.max_im, 1, b.nmem), dim3(256), 0, b.d_md);
+    ATHAD_DISPATCH(k_level_mean, dim3(kLevelChunks, b.max_im, b.nmem), dim3(256), 0, b.d_md);
+    hipLaunchKernelGGL(k_level_mean_final, dim3((b.max_im + 63) / 64, 1, b.nmem), dim3(64), 0, b.stream, b.d_md);
 }
 void launch_rk4_jacobi(const Batch& b, bool fused) {
     const dim3 g(cells_grid(b, 256), 1, b.nmem);
