@@ -32,22 +32,55 @@ RK4_WORDS = 26 + 25
 STEP_WORDS = 84
 
 
-def run_form(A, precision, order, steps, warmup, seed=1234, profile=True):
+def run_form(A, precision, order, steps, warmup, seed=1234, profile_steps=0, graph=True):
+    """Times `steps` scoped iterations (hipGraph replay of the 4-iteration cycle when `graph`),
+    then, if asked, `profile_steps` eager iterations with HIP events around each RK4 launch on
+    the library's own stream."""
     n = GRID[0] * GRID[1] * GRID[2]
     with A.Integration(GRID, precision=precision, order=order) as integ:
         integ.init_state(seed)
-        integ.run(warmup)
+        integ.run(warmup, graph=graph)
         integ.sync()
-        if profile:
-            integ.profile("rk4", True)
         t0 = time.perf_counter()
-        integ.run(steps)
+        integ.run(steps, graph=graph)
         integ.sync()
         dt = time.perf_counter() - t0
-        rk4_ms, rk4_n = integ.profile_result("rk4") if profile else (0.0, 0)
         fields = {name: integ.get(name) for name in A.PROGNOSTIC}
+        rk4_ms, rk4_n = 0.0, 0
+        if profile_steps:
+            integ.profile("rk4", True)
+            integ.run(profile_steps)
+            integ.sync()
+            rk4_ms, rk4_n = integ.profile_result("rk4")
+            integ.profile("rk4", False)
     return {"ms_per_step": dt / steps * 1e3, "cells_per_s": n * steps / dt, "rk4_ms": rk4_ms,
             "rk4_launches": rk4_n, "fields": fields}
+
+
+def measured_traffic():
+    """HBM bytes per k_rk4_jacobi<double> launch at this grid, as recorded by the FETCH_SIZE and
+    WRITE_SIZE counter runs (scaled by the copy-kernel calibration) in the committed profile
+    summary; None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_rk4_jacobi_fp64_41x181x361_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def calibrate_copy(mib=512, reps=5):
+    """A plain device copy of known size (reads and writes `mib` MiB per launch), the
+    reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    x = torch.ones(mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read_per_launch": x.numel() * 8,
+                      "bytes_written_per_launch": y.numel() * 8, "launches": reps}))
 
 
 def cpu_baseline(seconds_budget=20.0):
@@ -76,18 +109,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed and lex forms")
+    ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of the hipGraph")
+    ap.add_argument("--calibrate-copy", action="store_true", help="only run a plain copy of known size")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
-                                   "(see DESIGN.md), --gpus must be 1"}))
+                                   "(members or latitude bands would shard across GPUs), --gpus must be 1"}))
         sys.exit(2)
     import numpy as np
     import athad_cond_amd as A
 
     n = GRID[0] * GRID[1] * GRID[2]
-    fp64 = run_form(A, "fp64", "jacobi", args.steps, args.warmup)
+    graph = not args.no_graph
+    fp64 = run_form(A, "fp64", "jacobi", args.steps, args.warmup, profile_steps=20, graph=graph)
     avg_rk4_s = fp64["rk4_ms"] / max(fp64["rk4_launches"], 1) * 1e-3
     achieved = RK4_WORDS * 8 * n / avg_rk4_s / 1e9
+    traffic = measured_traffic()
     out = {
         "metric": METRIC,
         "value": fp64["cells_per_s"],
@@ -102,22 +142,23 @@ def main():
         "dtype": "f64",
         "data": "synthetic (nominal initial state, seeded mirrored perturbation)",
         "config": {"workload": "FP64 drop-in, Jacobi order, full scoped step at 41x181x361",
-                   "grid": list(GRID), "order": "jacobi", "precision": "fp64", "parallelism": "single GPU"},
+                   "grid": list(GRID), "order": "jacobi", "precision": "fp64", "parallelism": "single GPU",
+                   "launch": "hipGraph of 4 iterations" if graph else "eager"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_rk4_jacobi<double>", "avg_launch_ms": avg_rk4_s * 1e3,
-                     "bytes_per_cell": RK4_WORDS * 8},
+                     "bytes_per_cell": RK4_WORDS * 8, "algorithmic_bytes_per_launch": RK4_WORDS * 8 * n},
         "step_hbm_frac": STEP_WORDS * 8 * fp64["cells_per_s"] / (HBM_PEAK_GBS * 1e9),
     }
     if not args.no_extra:
-        mixed = run_form(A, "mixed", "jacobi", args.steps, args.warmup, profile=False)
+        mixed = run_form(A, "mixed", "jacobi", args.steps, args.warmup, graph=graph)
         err = {k: float(np.sqrt(np.sum((mixed["fields"][k] - fp64["fields"][k]) ** 2))
                         / max(np.sqrt(np.sum(fp64["fields"][k] ** 2)), 1e-300)) for k in A.PROGNOSTIC}
         out["mixed"] = {"cells_per_s": mixed["cells_per_s"], "ms_per_step": mixed["ms_per_step"],
                         "speedup_vs_fp64": mixed["cells_per_s"] / fp64["cells_per_s"],
                         "rel_l2_vs_fp64": err, "max_rel_l2": max(err.values())}
         lex_steps = max(4, min(args.steps, 12))
-        lex = run_form(A, "fp64", "lex", lex_steps, 4, profile=False)
+        lex = run_form(A, "fp64", "lex", lex_steps, 4, graph=graph)
         out["lex"] = {"cells_per_s": lex["cells_per_s"], "ms_per_step": lex["ms_per_step"], "steps": lex_steps}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

@@ -35,7 +35,9 @@ typedef int athad_status;
 #define ATHAD_FP64   0          /* drop-in form: FP64 storage and arithmetic */
 #define ATHAD_MIXED  1          /* FP32 storage/stencils, FP64 thermodynamics and p_stat */
 
-/* RK4 update order (RungeKutta_Atm_Turb.cpp:99-264 sweeps in place) */
+/* RK4 update order. RungeKutta_Atm_Turb.cpp:99-264 sweeps in place, so a cell's neighbours are
+ * read at whatever time level the sweep has left them; the two orders below are the
+ * reproducible choices. */
 #define ATHAD_ORDER_LEX     0   /* the reference's sweep at OMP_NUM_THREADS=1 (parity form) */
 #define ATHAD_ORDER_JACOBI  1   /* start-of-step neighbourhood (throughput form) */
 

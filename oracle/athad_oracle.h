@@ -51,7 +51,9 @@ typedef struct orc_member {
 
 typedef struct orc_state orc_state;
 
-// Creates the nominal initial state (the init recipe documented in DESIGN.md).
+// Creates the nominal initial state: RunTimeSlice's thermodynamic init chain
+// (cAtmosphereModel.cpp:527-568), then u,v,w = +-1e-3 m/s seeded and mirrored N-S, k = 1e-4 m2/s2,
+// omega = 1 1/s, one turbulence update and its damp_wiggles (:612-622), boundaries, t_eq = t, store.
 // member == NULL selects config_cond.xml's values.
 orc_state* orc_create(int im, int jm, int km, uint64_t seed, const orc_member* member);
 void       orc_destroy(orc_state* s);
