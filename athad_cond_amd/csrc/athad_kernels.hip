@@ -26,11 +26,15 @@ ATHAD_DEV long long cell_id(const Consts& K) {
     return (id < K.n) ? id : -1;
 }
 
+// (i,j,k) of a cell index; a member has fewer than 2^31 cells (checked at creation), so
+// 32-bit unsigned division suffices.
 ATHAD_DEV void decode(const Consts& K, long long id, int& i, int& j, int& k) {
-    k = (int)(id % K.km);
-    const long long ij = id / K.km;
-    j = (int)(ij % K.jm);
-    i = (int)(ij / K.jm);
+    const unsigned u = (unsigned)id, km = (unsigned)K.km, jm = (unsigned)K.jm;
+    const unsigned ij = u / km;
+    k = (int)(u - ij * km);
+    const unsigned ii = ij / jm;
+    j = (int)(ij - ii * jm);
+    i = (int)ii;
 }
 
 // Cadence gate: a kernel launched with gate g > 0 does nothing for members whose iteration
@@ -104,16 +108,16 @@ __global__ void k_level_mean_final(const MemberDesc* md) {
 // ------------------------------------------------------------------------------------
 // RHS of one cell (RHS_Atm_Turb.cpp:31-1080, land-free central stencils) and the RK4
 // stage loop (RungeKutta_Atm_Turb.cpp:137-263). C is the arithmetic type.
-// Neighbours are held in registers: nb[f][d], d = i+1, i-1, j+1, j-1, k+1, k-1.
+//
+// Over the four stages only the cell's own values change; its neighbours do not (they are
+// the start-of-RK4 values in the Jacobi order and values on other hyperplanes in the lex
+// order). Everything built from neighbours alone -- central gradients, the neighbour sums
+// of every Laplacian and of the vector diffusion, the velocity-gradient invariants of the
+// production term, the SST cross-diffusion product, the pressure gradient -- is computed
+// once per cell (prep_cell); each stage (rhs_stage) only adds the centre-value terms, the
+// minmod slopes and the SST closure. The RK4 base and accumulator live in LDS.
 // ------------------------------------------------------------------------------------
 enum { FT = 0, FU, FV, FW, FC, FCL, FI, FG, FCO, FK, FD };
-
-template <typename C> struct Geo {
-    C exp_rm, exp_2_rm, sinthe2, costhe, inv_rm, inv_rm2, inv_rmsinthe, inv_rm2sinthe, inv_rm2sinthe2,
-      cos_rm2sin, two_over_rm_exp, v_metric, inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2,
-      inv_dr, inv_dthe, inv_dphi;
-    C surf_drag, d_close, hs_kT;
-};
 
 // Solid-ground clamps of bcSolidGround passes 4-5 (BC_Atm.h:475-548) as a value transform,
 // used when the Jacobi kernel applies them on read instead of in a separate pass.
@@ -144,236 +148,385 @@ template <typename C> ATHAD_DEV C minmod(C a, C b) {
     return (a * b <= C(0)) ? C(0) : (fabs(a) < fabs(b) ? a : b);
 }
 
-template <typename C>
-struct RhsIn {
-    C nb[NPROG][6];
-    C dpdr, dpdthe, dpdphi;    // raw central differences of p_dyn
-    C nue, p_ratio_bl, t_eq, t_ref, buoy;   // p_ratio_bl: Held-Suarez boundary-layer weight
+// Reciprocal: the hardware estimate refined by Newton steps to within about 1 ulp. Used for
+// operands bounded away from zero and infinity (dis >= 1e-10, CD_kw >= 1e-20, sigma blends >= 1).
+ATHAD_DEV double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+ATHAD_DEV float rcp_nr(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    const float e = fmaf(-x, r, 1.0f);
+    return fmaf(r, e, r);
+}
+
+// Global-memory access through address space 1 (pointers read from a MemberDesc are
+// otherwise generic and compile to flat instructions).
+template <typename T> ATHAD_DEV T gld(const T* p, long long i) {
+    return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+template <typename T> ATHAD_DEV void gst(T* p, long long i, T v) {
+    ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+
+// Per-cell metric (cAtmosphereModel.h:170-178, 282-284; RHS_Atm_Turb.cpp:86-130), the
+// Held-Suarez rate (:976-992), the Rayleigh drag factor and the Coriolis factor.
+template <typename C> struct CellGeo {
+    C aR, aT, aP;          // advection metric / grid step, for the minmod slopes
+    C cexp, cirm, cirs;    // advection metric exp_rm, 1/rm, 1/(rm sin theta)
+    C wr, wt, wp;          // Laplacian weights of the neighbour sums (r, theta, phi)
+    C two_over, crs, Lc2;  // first-derivative Laplacian weights; centre coefficient
+    C cu, cvw, cirm2, cirm2s, c2cos, hs_kT, drag, cor;   // dynamics group only
 };
 
 template <typename C>
-ATHAD_DEV void rhs_eval(const Consts& K, const Geo<C>& g, const RhsIn<C>& in, const C* cen, C* rhs, C* aux,
-                        C& prod, C& tsrc, C& dsrc) {
-    C d1r[NPROG], d2r[NPROG], d1t[NPROG], d2t[NPROG], d1p[NPROG], d2p[NPROG];
-#pragma unroll
-    for (int f = 0; f < NPROG; f++) {
-        const C c0 = cen[f];
-        d1r[f] = (in.nb[f][0] - in.nb[f][1]) * g.inv_2dr;
-        d2r[f] = (in.nb[f][0] - C(2) * c0 + in.nb[f][1]) * g.inv_dr2;
-        d1t[f] = (in.nb[f][2] - in.nb[f][3]) * g.inv_2dthe;
-        d2t[f] = (in.nb[f][2] - C(2) * c0 + in.nb[f][3]) * g.inv_dthe2;
-        d1p[f] = (in.nb[f][4] - in.nb[f][5]) * g.inv_2dphi;
-        d2p[f] = (in.nb[f][4] - C(2) * c0 + in.nb[f][5]) * g.inv_dphi2;
-    }
-    const C u = cen[FU], v = cen[FV], w = cen[FW], tke = cen[FK], dis = cen[FD];
-    const C nue_max = C(K.nue_max);
-    // prod refresh :528-561
-    {
-        const C dudr_s = d1r[FU] * g.exp_rm, dvdr_s = d1r[FV] * g.exp_rm, dwdr_s = d1r[FW] * g.exp_rm;
-        const C dudthe_s = d1t[FU] * g.inv_rm, dvdthe_s = d1t[FV] * g.inv_rm, dwdthe_s = d1t[FW] * g.inv_rm;
-        const C dudphi_s = d1p[FU] * g.inv_rmsinthe, dvdphi_s = d1p[FV] * g.inv_rmsinthe,
-                dwdphi_s = d1p[FW] * g.inv_rmsinthe;
-        const C dis_here = tmax(dis, C(1.0e-10));
-        const C tke_here = tmax(tke, C(0));
-        C cnue = tke_here / dis_here;
-        cnue = tmin(cnue, nue_max);
-        const C der = C(0.66667) * (dudr_s + dvdthe_s + dwdphi_s);
-        prod = tmax(C(0),
-              (cnue * (C(2) * dudr_s - der) - C(0.66667) * tke) * dudr_s
-            + (cnue * (dudthe_s + dvdr_s)) * dudthe_s
-            + (cnue * (dudphi_s + dwdr_s)) * dudphi_s
-            + (cnue * (C(2) * dvdthe_s - der) - C(0.66667) * tke) * dvdthe_s
-            + (cnue * (dvdr_s + dudthe_s)) * dvdr_s
-            + (cnue * (dvdphi_s + dwdthe_s)) * dvdphi_s
-            + (cnue * (C(2) * dwdphi_s - der) - C(0.66667) * tke) * dwdphi_s
-            + (cnue * (dwdr_s + dudphi_s)) * dwdr_s
-            + (cnue * (dwdthe_s + dvdphi_s)) * dwdthe_s);
-    }
-    // k-omega SST diffusion coefficients and sources :633-721
-    C dif_t, dif_vel, dif_tke, dif_dis;
-    {
-        const C sig_k1 = C(1.176), sig_k2 = C(1.0), sig_w1 = C(2.0), sig_w2 = C(1.168), bet_star = C(0.09);
-        const C dc = g.d_close;
-        const C cross = d1r[FK] * d1r[FD] + d1t[FK] * g.inv_rm * d1t[FD] * g.inv_rm
-                      + d1p[FK] * g.inv_rmsinthe * d1p[FD] * g.inv_rmsinthe;
-        const C CD_kw = tmax((C(2) * sig_w2) / dis * cross, C(1.0e-20));
-        const C arg1 = tmin(tmax(sqrt(tke) / (bet_star * dis * dc), (C(500) * C(K.nue_air_nd)) / (dc * dc * dis)),
-                            (C(4) * sig_w2 * tke) / (CD_kw * dc * dc));
-        const C a2 = arg1 * arg1;
-        const C F1 = tanh(a2 * a2);
-        const C inv_re = C(1) / C(K.re_turb);
-        const C nue = in.nue;
-        dif_t = (inv_re + nue) / C(K.pr_turb);
-        dif_vel = inv_re + nue;
-        dif_tke = inv_re + nue / (F1 * sig_k1 + (C(1) - F1) * sig_k2);
-        dif_dis = inv_re + nue / (F1 * sig_w1 + (C(1) - F1) * sig_w2);
-        const C bet1 = C(0.0333), bet2 = C(0.0368), gam1 = C(0.413), gam2 = C(0.2);
-        const C tke_s = tmax(tke, C(1.0e-10)), dis_s = tmax(dis, C(1.0e-10));
-        const C P_k = tmin(prod, C(20) * bet_star * tke_s * dis_s);
-        const C Y_k = bet_star * tke_s * dis_s;
-        const C P_w = (F1 * gam1 + (C(1) - F1) * gam2) * P_k * dis_s / tke_s;
-        const C Y_w = (F1 * bet1 + (C(1) - F1) * bet2) * dis_s * dis_s;
-        const C D_w = C(2) * (C(1) - F1) * sig_w2 / dis_s * cross;
-        const C dsm = C(20) * dis_s * dis_s, tsm = C(20) * tke_s * dis_s;
-        tsrc = tclamp(P_k - Y_k, -tsm, tsm);
-        dsrc = tclamp(P_w - Y_w + D_w, -dsm, dsm);
-    }
-    const C u_exp = u * g.exp_rm, v_invrm = v * g.inv_rm, w_invrs = w * g.inv_rmsinthe;
-    const C pressure_t = C(K.coeff_energy_p) * (u_exp * in.dpdr * g.inv_2dr + v_invrm * in.dpdthe * g.inv_2dthe
-                                               + w_invrs * in.dpdphi * g.inv_2dphi);
-    C tr[NPROG];
-#pragma unroll
-    for (int f = 0; f < NPROG; f++) {
-        C gr_, gt_, gp_;
-        if (f <= FI) {
-            const C c0 = cen[f];
-            gr_ = minmod(c0 - in.nb[f][1], in.nb[f][0] - c0) * g.inv_dr;
-            gt_ = minmod(c0 - in.nb[f][3], in.nb[f][2] - c0) * g.inv_dthe;
-            gp_ = minmod(c0 - in.nb[f][5], in.nb[f][4] - c0) * g.inv_dphi;
-        } else { gr_ = d1r[f]; gt_ = d1t[f]; gp_ = d1p[f]; }
-        tr[f] = u_exp * gr_ + v_invrm * gt_ + w_invrs * gp_;
-    }
-    auto lap = [&](int f) {
-        return d2r[f] * g.exp_2_rm + d1r[f] * g.two_over_rm_exp + d2t[f] * g.inv_rm2 + d1t[f] * g.cos_rm2sin
-             + d2p[f] * g.inv_rm2sinthe2;
-    };
-    const C diffusion_u = (d2r[FU] * g.exp_2_rm + C(2) * u * g.inv_rm2 + d2t[FU] * g.inv_rm2
-        + C(4) * d1r[FU] * g.inv_rm * g.exp_rm + d1t[FU] * g.cos_rm2sin + d2p[FU] * g.inv_rm2sinthe2) * dif_vel;
-    const C diffusion_v = (d2r[FV] * g.exp_2_rm + d1r[FV] * g.two_over_rm_exp + d2t[FV] * g.inv_rm2
-        + d1t[FV] * g.cos_rm2sin - g.v_metric * v + d2p[FV] * g.inv_rm2sinthe2 + C(2) * d1t[FU] * g.inv_rm2
-        - d1p[FW] * C(2) * g.costhe * g.inv_rm2sinthe2) * dif_vel;
-    const C diffusion_w = (d2r[FW] * g.exp_2_rm + d1r[FW] * g.two_over_rm_exp + d2t[FW] * g.inv_rm2
-        + d1t[FW] * g.cos_rm2sin - g.v_metric * w + d2p[FW] * g.inv_rm2sinthe2 + C(2) * d1p[FU] * g.inv_rm2sinthe
-        + d1p[FV] * C(2) * g.costhe * g.inv_rm2sinthe2) * dif_vel;
-    const C dpdr_exp = in.dpdr * g.inv_2dr * g.exp_rm;
-    const C dpdthe_invrm = in.dpdthe * g.inv_2dthe * g.inv_rm;
-    const C dpdphi_invrs = in.dpdphi * g.inv_2dphi * g.inv_rmsinthe;
-    C rt = pressure_t - tr[FT] + lap(FT) * dif_t;
-    rt -= g.hs_kT * (cen[FT] - in.t_eq);
-    rhs[FT] = rt;
-    rhs[FU] = -dpdr_exp - tr[FU] + diffusion_u + in.buoy * (cen[FT] - in.t_ref);
-    const C force_nd = C(K.force_nd);
-    rhs[FV] = -dpdthe_invrm - tr[FV] + diffusion_v + force_nd * (C(2) * g.costhe * w) - g.surf_drag * v;
-    rhs[FW] = -dpdphi_invrs - tr[FW] + diffusion_w + force_nd * (C(-2) * (g.costhe * v)) - g.surf_drag * w;
-    const C dpr = C(K.diff_prec_re_inv), dco = C(K.diff_co2_re_inv);
-    rhs[FC] = -tr[FC] + lap(FC) * dpr;
-    rhs[FCL] = -tr[FCL] + lap(FCL) * dpr;
-    rhs[FI] = -tr[FI] + lap(FI) * dpr;
-    rhs[FG] = -tr[FG] + lap(FG) * dpr;
-    rhs[FCO] = -tr[FCO] + lap(FCO) * dco;
-    rhs[FK] = -tr[FK] + lap(FK) * dif_tke + tsrc;
-    rhs[FD] = -tr[FD] + lap(FD) * dif_dis + dsrc;
-    aux[0] = rhs[FU] + dpdr_exp;
-    aux[1] = rhs[FV] + dpdthe_invrm;
-    aux[2] = rhs[FW] + dpdphi_invrs;
-}
-
-// Geometry of one (i,j) column and the per-cell SST wall distance and Held-Suarez rate.
-template <typename C>
-ATHAD_DEV void make_geo(const MemberDesc& M, int i, int j, int k, double p_stat_c, double p_surf, Geo<C>& g) {
+ATHAD_DEV void cell_geo(const MemberDesc& M, long long id, int i, int j, int k, CellGeo<C>& g) {
     const Consts& K = M.K;
     const double rm = M.rad[i];
     const double exp_rm = 1.0 / (rm + 1.0);
-    const double sinthe = M.sint[j];
+    const double sinthe = M.sint[j], costhe = M.cost[j];
     const double sinthe2 = sinthe * sinthe;
-    const double costhe = M.cost[j];
     const double rmet = (K.metric_r0 > 0.0) ? (K.metric_r0 + (rm - K.rad0)) : rm;
     const double inv_rm = 1.0 / rmet;
     const double inv_rm2 = 1.0 / (rmet * rmet);
+    const double inv_rms = 1.0 / (rmet * sinthe);
     const double inv_rm2sinthe = inv_rm2 / sinthe;
-    g.exp_rm = C(exp_rm);
-    g.exp_2_rm = C(exp_rm * exp_rm);
-    g.sinthe2 = C(sinthe2);
-    g.costhe = C(costhe);
-    g.inv_rm = C(inv_rm);
-    g.inv_rm2 = C(inv_rm2);
-    g.inv_rmsinthe = C(1.0 / (rmet * sinthe));
-    g.inv_rm2sinthe = C(inv_rm2sinthe);
-    g.inv_rm2sinthe2 = C(inv_rm2 / sinthe2);
-    g.cos_rm2sin = C(costhe * inv_rm2sinthe);
-    g.two_over_rm_exp = C(2.0 * inv_rm * exp_rm);
-    g.v_metric = C((1.0 + costhe * costhe / sinthe2) * inv_rm2);
-    g.inv_2dr = C(K.inv_2dr); g.inv_2dthe = C(K.inv_2dthe); g.inv_2dphi = C(K.inv_2dphi);
-    g.inv_dr2 = C(K.inv_dr2); g.inv_dthe2 = C(K.inv_dthe2); g.inv_dphi2 = C(K.inv_dphi2);
-    g.inv_dr = C(2.0 * K.inv_2dr); g.inv_dthe = C(2.0 * K.inv_2dthe); g.inv_dphi = C(2.0 * K.inv_2dphi);
-    double dp = 1.0 - (double)i / 5.0;
-    if (dp < 0.0) dp = 0.0;
-    if (dp > 1.0) dp = 1.0;
-    g.surf_drag = C(K.drag_coeff * dp);
-    // SST wall distance: min(i, j, k) (RHS_Atm_Turb.cpp:640-671 evaluated without land);
-    // the symmetric variant measures j from the nearer pole.
-    double dc = (double)i;
-    const double dj = K.sym ? (double)min(j, K.jm - 1 - j) : (double)j;
-    if (dj <= dc) dc = dj;
-    if ((double)k <= dc) dc = (double)k;
-    g.d_close = C(dc);
-    // Held-Suarez :976-992
-    const double sigma = (p_surf > 0.0) ? p_stat_c / p_surf : 0.0;
+    const double inv_rm2s2 = inv_rm2 / sinthe2;
+    const double v_metric = (1.0 + costhe * costhe / sinthe2) * inv_rm2;
+    const double wr = K.inv_dr2 * (exp_rm * exp_rm), wt = K.inv_dthe2 * inv_rm2, wp = K.inv_dphi2 * inv_rm2s2;
+    const double Lc2 = 2.0 * (wr + wt + wp);
+    g.aR = C(exp_rm * 2.0 * K.inv_2dr);
+    g.aT = C(inv_rm * 2.0 * K.inv_2dthe);
+    g.aP = C(inv_rms * 2.0 * K.inv_2dphi);
+    g.cexp = C(exp_rm); g.cirm = C(inv_rm); g.cirs = C(inv_rms);
+    g.wr = C(wr); g.wt = C(wt); g.wp = C(wp);
+    g.two_over = C(2.0 * inv_rm * exp_rm);
+    g.crs = C(costhe * inv_rm2sinthe);
+    g.Lc2 = C(Lc2);
+    g.cu = C(2.0 * inv_rm2 - Lc2);
+    g.cvw = C(-v_metric - Lc2);
+    g.cirm2 = C(inv_rm2); g.cirm2s = C(inv_rm2sinthe); g.c2cos = C(2.0 * costhe * inv_rm2s2);
+    const double* ps = pstat(M);
+    const double p_surf = gld(ps, (long long)ix3(K, 0, j, k));
+    const double sigma = (p_surf > 0.0) ? gld(ps, id) / p_surf : 0.0;
     double bl = (sigma - 0.7) / (1.0 - 0.7);
     if (bl < 0.0) bl = 0.0;
     const double kT = K.hs_k_a + (K.hs_k_s - K.hs_k_a) * bl * (sinthe2 * sinthe2);
     g.hs_kT = C(kT * K.L_atm / K.u_0);
+    double dp = 1.0 - (double)i / 5.0;
+    if (dp < 0.0) dp = 0.0;
+    if (dp > 1.0) dp = 1.0;
+    g.drag = C(K.drag_coeff * dp);
+    g.cor = C(K.force_nd * 2.0 * costhe);
 }
 
-// Loads the six neighbours of every prognostic field (optionally through the solid-ground
-// clamp) and the p_dyn differences.
+// The six neighbours (i+1, i-1, j+1, j-1, k+1, k-1) of field f at id, optionally through the
+// solid-ground clamp.
 template <typename S, typename C, bool CLAMP>
-ATHAD_DEV void load_nb(const MemberDesc& M, const S* const* X, size_t id, RhsIn<C>& in) {
-    const Consts& K = M.K;
-    const size_t si = (size_t)K.jm * K.km, sj = K.km;
-    const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
+ATHAD_DEV void load6(const S* x, long long id, long long si, long long sj, int f, C tkm, C vm, C* v) {
+    v[0] = C(gld(x, id + si)); v[1] = C(gld(x, id - si)); v[2] = C(gld(x, id + sj));
+    v[3] = C(gld(x, id - sj)); v[4] = C(gld(x, id + 1)); v[5] = C(gld(x, id - 1));
+    if (CLAMP) {
 #pragma unroll
-    for (int f = 0; f < NPROG; f++) {
-        const S* x = X[f];
-        C v0 = C(x[id + si]), v1 = C(x[id - si]), v2 = C(x[id + sj]), v3 = C(x[id - sj]), v4 = C(x[id + 1]),
-          v5 = C(x[id - 1]);
-        if (CLAMP) {
-            v0 = solid_clamp(f, v0, tkm, vm); v1 = solid_clamp(f, v1, tkm, vm); v2 = solid_clamp(f, v2, tkm, vm);
-            v3 = solid_clamp(f, v3, tkm, vm); v4 = solid_clamp(f, v4, tkm, vm); v5 = solid_clamp(f, v5, tkm, vm);
-        }
-        in.nb[f][0] = v0; in.nb[f][1] = v1; in.nb[f][2] = v2; in.nb[f][3] = v3; in.nb[f][4] = v4; in.nb[f][5] = v5;
+        for (int d = 0; d < 6; d++) v[d] = solid_clamp(f, v[d], tkm, vm);
     }
-    const S* p = fp<S>(M, F_P_DYN);
-    in.dpdr = C(p[id + si]) - C(p[id - si]);
-    in.dpdthe = C(p[id + sj]) - C(p[id - sj]);
-    in.dpdphi = C(p[id + 1]) - C(p[id - 1]);
 }
 
-// The four stages of one cell. base[] = *n values, cen[] = start values (updated in
-// place to the final values). Returns aux/prod/sources of stage 4.
-template <typename C>
-ATHAD_DEV void rk4_cell(const Consts& K, const Geo<C>& g, const RhsIn<C>& in, const C* base, C* cen, C* aux,
-                        C& prod, C& ts, C& ds) {
-    const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0), tkm = C(K.tke_max_nd);
-    C acc[NPROG], r[NPROG];
+template <typename C> ATHAD_DEV C minmod3(const C* nb, C c0, C ua, C va, C wa) {
+    return ua * minmod(c0 - nb[1], nb[0] - c0) + va * minmod(c0 - nb[3], nb[2] - c0)
+         + wa * minmod(c0 - nb[5], nb[4] - c0);
+}
+
+template <typename C> ATHAD_DEV C buoyancy(const MemberDesc& M) {
+    const int total = M.counters[1];
+    const double ramp = fmin(1.0, (double)total / 300.0);
+    return C(ramp * 1.0 * M.K.g * M.K.dt / M.K.u_0);
+}
+
+// RK4 stage update of one field (RungeKutta_Atm_Turb.cpp:167-263): b = base (*n value) and
+// a = k1+2k2+2k3 accumulator, both in this thread's LDS slots.
+template <typename C> ATHAD_DEV C stage_update(int st, C r, C* b, C* a, C dt, C half_dt, C dt_sixth) {
+    const C bv = *b;
+    if (st == 0) { *a = r; return bv + r * half_dt; }
+    if (st == 1) { *a = *a + C(2) * r; return bv + r * half_dt; }
+    if (st == 2) { *a = *a + C(2) * r; return bv + r * dt; }
+    return bv + (*a + r) * dt_sixth;
+}
+
+constexpr int LB = 256;   // LDS slot stride = threads per block of the RK4 kernels
+
+// RK4 of the dynamics group T, u, v, w (RHS_Atm_Turb.cpp rhs_t/rhs_u/rhs_v/rhs_w). Their
+// tendencies involve no other prognostic field, so the group integrates first and records
+// the velocity entering each stage in rec[] for the scalar groups. cen[] = start values in,
+// final values out. A, B: velocity-gradient invariants of the production term.
+template <typename S, typename C, bool CLAMP>
+ATHAD_DEV void rk4_dyn(const MemberDesc& M, const S* const* X, long long id, int i, const CellGeo<C>& g,
+                       C* ba, C* rec, C* cen, C* aux, C& A, C& B) {
+    const Consts& K = M.K;
+    const long long si = (long long)K.jm * K.km, sj = K.km;
+    const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
+    const C i2r = C(K.inv_2dr), i2t = C(K.inv_2dthe), i2p = C(K.inv_2dphi);
+    C nb[4][6], g1r[4], g1t[4], g1p[4], Sr[4], St[4], Sp[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        load6<S, C, CLAMP>(X[f], id, si, sj, f, tkm, vm, nb[f]);
+        g1r[f] = (nb[f][0] - nb[f][1]) * i2r;
+        g1t[f] = (nb[f][2] - nb[f][3]) * i2t;
+        g1p[f] = (nb[f][4] - nb[f][5]) * i2p;
+        Sr[f] = nb[f][0] + nb[f][1]; St[f] = nb[f][2] + nb[f][3]; Sp[f] = nb[f][4] + nb[f][5];
+    }
+    const C L0T = Sr[FT] * g.wr + g1r[FT] * g.two_over + St[FT] * g.wt + g1t[FT] * g.crs + Sp[FT] * g.wp;
+    // vector diffusion of u, v, w without their centre terms
+    const C D0u = Sr[FU] * g.wr + St[FU] * g.wt + C(4) * g1r[FU] * g.cirm * g.cexp + g1t[FU] * g.crs + Sp[FU] * g.wp;
+    const C D0v = Sr[FV] * g.wr + g1r[FV] * g.two_over + St[FV] * g.wt + g1t[FV] * g.crs + Sp[FV] * g.wp
+                + C(2) * g1t[FU] * g.cirm2 - g1p[FW] * g.c2cos;
+    const C D0w = Sr[FW] * g.wr + g1r[FW] * g.two_over + St[FW] * g.wt + g1t[FW] * g.crs + Sp[FW] * g.wp
+                + C(2) * g1p[FU] * g.cirm2s + g1p[FV] * g.c2cos;
+    // production invariants (prod refresh RHS_Atm_Turb.cpp:528-561):
+    // prod = max(0, cnue*A - 0.66667*tke*B)
+    {
+        const C a1 = g1r[FU] * g.cexp, a2 = g1r[FV] * g.cexp, a3 = g1r[FW] * g.cexp;
+        const C b1 = g1t[FU] * g.cirm, b2 = g1t[FV] * g.cirm, b3 = g1t[FW] * g.cirm;
+        const C c1 = g1p[FU] * g.cirs, c2 = g1p[FV] * g.cirs, c3 = g1p[FW] * g.cirs;
+        const C der = C(0.66667) * (a1 + b2 + c3);
+        const C s12 = b1 + a2, s13 = c1 + a3, s23 = c2 + b3;
+        A = (C(2) * a1 - der) * a1 + (C(2) * b2 - der) * b2 + (C(2) * c3 - der) * c3 + s12 * s12 + s13 * s13
+          + s23 * s23;
+        B = a1 + b2 + c3;
+    }
+    C PR, PT, PP;
+    {
+        const S* p = fp<S>(M, F_P_DYN);
+        PR = (C(gld(p, id + si)) - C(gld(p, id - si))) * i2r * g.cexp;
+        PT = (C(gld(p, id + sj)) - C(gld(p, id - sj))) * i2t * g.cirm;
+        PP = (C(gld(p, id + 1)) - C(gld(p, id - 1))) * i2p * g.cirs;
+    }
+    C dif_t, dif_vel;
+    {
+        C nv = C(gld(fp<S>(M, F_NUE), id));
+        if (CLAMP) nv = finite_bits(nv) ? tclamp(nv, C(0), C(K.nue_max)) : C(0);
+        const C inv_re = C(1.0 / K.re_turb);
+        dif_vel = inv_re + nv;
+        dif_t = (inv_re + nv) * C(1.0 / K.pr_turb);
+    }
+    const C t_eq = C(gld(fp<S>(M, F_T_EQ), id));
+    const C t_ref = C(M.t_ref[i]);
+    const C buoy = buoyancy<C>(M);
+    const C ce = C(K.coeff_energy_p);
+    const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
 #pragma unroll 1
     for (int st = 0; st < 4; st++) {
-        rhs_eval(K, g, in, cen, r, aux, prod, ts, ds);
+        const C T = cen[FT], u = cen[FU], v = cen[FV], w = cen[FW];
+        rec[(3 * st + 0) * LB] = u; rec[(3 * st + 1) * LB] = v; rec[(3 * st + 2) * LB] = w;
+        const C ua = u * g.aR, va = v * g.aT, wa = w * g.aP;
+        const C rT = ce * (u * PR + v * PT + w * PP) - minmod3(nb[FT], T, ua, va, wa) + (L0T - g.Lc2 * T) * dif_t
+                   - g.hs_kT * (T - t_eq);
+        const C rU = -PR - minmod3(nb[FU], u, ua, va, wa) + (D0u + g.cu * u) * dif_vel + buoy * (T - t_ref);
+        const C rV = -PT - minmod3(nb[FV], v, ua, va, wa) + (D0v + g.cvw * v) * dif_vel + g.cor * w - g.drag * v;
+        const C rW = -PP - minmod3(nb[FW], w, ua, va, wa) + (D0w + g.cvw * w) * dif_vel - g.cor * v - g.drag * w;
+        if (st == 3) { aux[0] = rU + PR; aux[1] = rV + PT; aux[2] = rW + PP; }
+        cen[FT] = stage_update(st, rT, ba + 0 * LB, ba + 4 * LB, dt, half_dt, dt_sixth);
+        cen[FU] = stage_update(st, rU, ba + 1 * LB, ba + 5 * LB, dt, half_dt, dt_sixth);
+        cen[FV] = stage_update(st, rV, ba + 2 * LB, ba + 6 * LB, dt, half_dt, dt_sixth);
+        cen[FW] = stage_update(st, rW, ba + 3 * LB, ba + 7 * LB, dt, half_dt, dt_sixth);
+    }
+}
+
+// RK4 of the moisture group c, cloud, ice (minmod advection), gr, co2 (central), advected by
+// the recorded stage velocities; cen[0..4] = start values in, final values out.
+template <typename S, typename C, bool CLAMP>
+ATHAD_DEV void rk4_moist(const MemberDesc& M, const S* const* X, long long id, const CellGeo<C>& g, C* ba,
+                         const C* rec, C* cen) {
+    const Consts& K = M.K;
+    const long long si = (long long)K.jm * K.km, sj = K.km;
+    const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
+    const C i2r = C(K.inv_2dr), i2t = C(K.inv_2dthe), i2p = C(K.inv_2dphi);
+    C nb[3][6], L0[5], gc[2][3];
 #pragma unroll
-        for (int f = 0; f < NPROG; f++) {
-            C val;
-            if (st == 0) { acc[f] = r[f]; val = base[f] + r[f] * half_dt; }
-            else if (st == 1) { acc[f] = acc[f] + C(2) * r[f]; val = base[f] + r[f] * half_dt; }
-            else if (st == 2) { acc[f] = acc[f] + C(2) * r[f]; val = base[f] + r[f] * dt; }
-            else { acc[f] = acc[f] + r[f]; val = base[f] + acc[f] * dt_sixth; }
-            if (f == FC || f == FCL || f == FI || f == FG) val = tmax(C(0), val);
-            else if (f == FK) { if (!finite_bits(val)) val = C(0); else val = (val < C(0)) ? C(0) : (val > tkm) ? tkm : val; }
-            else if (f == FD) val = tmax(C(1.0e-10), val);
-            cen[f] = val;
+    for (int q = 0; q < 5; q++) {
+        const int f = FC + q;
+        C v[6];
+        load6<S, C, CLAMP>(X[f], id, si, sj, f, tkm, vm, v);
+        const C d1r = (v[0] - v[1]) * i2r, d1t = (v[2] - v[3]) * i2t, d1p = (v[4] - v[5]) * i2p;
+        L0[q] = (v[0] + v[1]) * g.wr + d1r * g.two_over + (v[2] + v[3]) * g.wt + d1t * g.crs + (v[4] + v[5]) * g.wp;
+        if (q < 3) {
+#pragma unroll
+            for (int d = 0; d < 6; d++) nb[q][d] = v[d];
+        } else {
+            gc[q - 3][0] = d1r * g.cexp; gc[q - 3][1] = d1t * g.cirm; gc[q - 3][2] = d1p * g.cirs;
         }
+    }
+    const C dpr = C(K.diff_prec_re_inv), dco = C(K.diff_co2_re_inv);
+    const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
+#pragma unroll 1
+    for (int st = 0; st < 4; st++) {
+        const C u = rec[(3 * st + 0) * LB], v = rec[(3 * st + 1) * LB], w = rec[(3 * st + 2) * LB];
+        const C ua = u * g.aR, va = v * g.aT, wa = w * g.aP;
+        C r[5];
+#pragma unroll
+        for (int q = 0; q < 3; q++) r[q] = -minmod3(nb[q], cen[q], ua, va, wa) + (L0[q] - g.Lc2 * cen[q]) * dpr;
+        r[3] = -(u * gc[0][0] + v * gc[0][1] + w * gc[0][2]) + (L0[3] - g.Lc2 * cen[3]) * dpr;
+        r[4] = -(u * gc[1][0] + v * gc[1][1] + w * gc[1][2]) + (L0[4] - g.Lc2 * cen[4]) * dco;
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            const C val = stage_update(st, r[q], ba + q * LB, ba + (5 + q) * LB, dt, half_dt, dt_sixth);
+            cen[q] = (q < 4) ? tmax(C(0), val) : val;
+        }
+    }
+}
+
+// RK4 of the turbulence group tke, dis with the k-omega SST closure (RHS_Atm_Turb.cpp:633-721),
+// advected by the recorded stage velocities. cen[0..1] = start values in, final values out;
+// prod and the sources are those of the fourth stage.
+template <typename S, typename C, bool CLAMP>
+ATHAD_DEV void rk4_turb(const MemberDesc& M, const S* const* X, long long id, int i, int j, int k,
+                        const CellGeo<C>& g, C* ba, const C* rec, C A, C B, C* cen, C& prod, C& tsrc, C& dsrc) {
+    const Consts& K = M.K;
+    const long long si = (long long)K.jm * K.km, sj = K.km;
+    const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
+    const C i2r = C(K.inv_2dr), i2t = C(K.inv_2dthe), i2p = C(K.inv_2dphi);
+    C L0[2], gc[2][3], d1[2][3];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int f = FK + q;
+        C v[6];
+        load6<S, C, CLAMP>(X[f], id, si, sj, f, tkm, vm, v);
+        d1[q][0] = (v[0] - v[1]) * i2r; d1[q][1] = (v[2] - v[3]) * i2t; d1[q][2] = (v[4] - v[5]) * i2p;
+        L0[q] = (v[0] + v[1]) * g.wr + d1[q][0] * g.two_over + (v[2] + v[3]) * g.wt + d1[q][1] * g.crs
+              + (v[4] + v[5]) * g.wp;
+        gc[q][0] = d1[q][0] * g.cexp; gc[q][1] = d1[q][1] * g.cirm; gc[q][2] = d1[q][2] * g.cirs;
+    }
+    const C cross = d1[0][0] * d1[1][0] + (d1[0][1] * g.cirm) * (d1[1][1] * g.cirm)
+                  + (d1[0][2] * g.cirs) * (d1[1][2] * g.cirs);
+    C nue = C(gld(fp<S>(M, F_NUE), id));
+    if (CLAMP) nue = finite_bits(nue) ? tclamp(nue, C(0), C(K.nue_max)) : C(0);
+    // SST wall distance min(i, j, k) (RHS_Atm_Turb.cpp:640-671 without land); the symmetric
+    // variant measures j from the nearer pole.
+    C K1, K2, K3;
+    {
+        double dc = (double)i;
+        const double dj = K.sym ? (double)min(j, K.jm - 1 - j) : (double)j;
+        if (dj <= dc) dc = dj;
+        if ((double)k <= dc) dc = (double)k;
+        K1 = C(500.0 * K.nue_air_nd / (dc * dc));
+        K2 = C(1.0 / (0.09 * dc));
+        K3 = C(4.0 * 1.168 / (dc * dc));
+    }
+    const C sig_k1 = C(1.176), sig_k2 = C(1.0), sig_w1 = C(2.0), sig_w2 = C(1.168), bet_star = C(0.09);
+    const C bet1 = C(0.0333), bet2 = C(0.0368), gam1 = C(0.413), gam2 = C(0.2);
+    const C inv_re = C(1.0 / K.re_turb);
+    const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
+#pragma unroll 1
+    for (int st = 0; st < 4; st++) {
+        const C u = rec[(3 * st + 0) * LB], v = rec[(3 * st + 1) * LB], w = rec[(3 * st + 2) * LB];
+        const C tke = cen[0], dis = cen[1];
+        // Stage inputs satisfy dis >= 1e-10 and tke >= 0 (start values pass the solid-ground
+        // clamp, stage outputs are floored), so max(dis, 1e-10) == dis and one reciprocal
+        // serves every 1/dis.
+        const C dis_s = tmax(dis, C(1.0e-10));
+        const C inv_dis = rcp_nr(dis_s);
+        const C cnue = tmin(tmax(tke, C(0)) * inv_dis, C(K.nue_max));
+        prod = tmax(C(0), cnue * A - C(0.66667) * tke * B);
+        const C CD_kw = tmax(C(2) * sig_w2 * inv_dis * cross, C(1.0e-20));
+        const C arg1 = tmin(tmax(sqrt(tke) * inv_dis * K2, K1 * inv_dis), tke * K3 * rcp_nr(CD_kw));
+        const C a2 = arg1 * arg1;
+        const C F1 = tanh(a2 * a2);
+        const C dif_tke = inv_re + nue * rcp_nr(F1 * sig_k1 + (C(1) - F1) * sig_k2);
+        const C dif_dis = inv_re + nue * rcp_nr(F1 * sig_w1 + (C(1) - F1) * sig_w2);
+        const C tke_s = tmax(tke, C(1.0e-10));
+        const C P_k = tmin(prod, C(20) * bet_star * tke_s * dis_s);
+        const C Y_k = bet_star * tke_s * dis_s;
+        const C P_w = (F1 * gam1 + (C(1) - F1) * gam2) * P_k * dis_s * rcp_nr(tke_s);
+        const C Y_w = (F1 * bet1 + (C(1) - F1) * bet2) * dis_s * dis_s;
+        const C D_w = C(2) * (C(1) - F1) * sig_w2 * inv_dis * cross;
+        const C dsm = C(20) * dis_s * dis_s, tsm = C(20) * tke_s * dis_s;
+        tsrc = tclamp(P_k - Y_k, -tsm, tsm);
+        dsrc = tclamp(P_w - Y_w + D_w, -dsm, dsm);
+        const C rk = -(u * gc[0][0] + v * gc[0][1] + w * gc[0][2]) + (L0[0] - g.Lc2 * tke) * dif_tke + tsrc;
+        const C rd = -(u * gc[1][0] + v * gc[1][1] + w * gc[1][2]) + (L0[1] - g.Lc2 * dis) * dif_dis + dsrc;
+        C vk = stage_update(st, rk, ba + 0 * LB, ba + 2 * LB, dt, half_dt, dt_sixth);
+        vk = finite_bits(vk) ? ((vk < C(0)) ? C(0) : (vk > tkm) ? tkm : vk) : C(0);
+        const C vd = stage_update(st, rd, ba + 1 * LB, ba + 3 * LB, dt, half_dt, dt_sixth);
+        cen[0] = vk;
+        cen[1] = tmax(C(1.0e-10), vd);
     }
 }
 
 constexpr int kProgF[NPROG] = {F_T, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2, F_TKE, F_DIS};
 constexpr int kProgN[NPROG] = {F_TN, F_UN, F_VN, F_WN, F_CN, F_CLOUDN, F_ICEN, F_GRN, F_CO2N, F_TKEN, F_DISN};
 
+// Waves per SIMD the RK4 kernels are compiled for (register budget 512 / waves).
+#ifndef ATHAD_RK4_WAVES_F64
+#define ATHAD_RK4_WAVES_F64 3
+#endif
+#ifndef ATHAD_RK4_WAVES_F32
+#define ATHAD_RK4_WAVES_F32 4
+#endif
+template <typename S> constexpr int rk4_waves() { return sizeof(S) == 8 ? ATHAD_RK4_WAVES_F64 : ATHAD_RK4_WAVES_F32; }
+
+// Start values (optionally clamped) into cen[] and the bases into the LDS slots of a group.
+template <typename S, typename C, bool CLAMP>
+ATHAD_DEV void group_start(const MemberDesc& M, const S* const* X, long long id, int f0, int nf, C* cen, C* ba) {
+    const C tkm = C(M.K.tke_max_nd), vm = C(M.K.vel_max_nd);
+#pragma unroll
+    for (int q = 0; q < nf; q++) {
+        const int f = f0 + q;
+        C c0 = C(gld(X[f], id));
+        C b0 = C(gld(fp<S>(M, kProgN[f]), id));
+        if (CLAMP) { c0 = solid_clamp(f, c0, tkm, vm); b0 = solid_clamp_n(f, b0, tkm, vm); }
+        cen[q] = c0;
+        ba[q * LB] = b0;
+    }
+}
+
+// RK4 of one interior cell, group by group: dynamics, moisture, turbulence. out(f, value)
+// writes a final value; the aux / prod / source outputs are written here.
+template <typename S, typename C, bool CLAMP, typename Out>
+ATHAD_DEV void rk4_cell(const MemberDesc& M, const S* const* X, long long id, int i, int j, int k, C* ba, C* rec,
+                        Out out) {
+    CellGeo<C> g;
+    cell_geo<C>(M, id, i, j, k, g);
+    C A, B;
+    {
+        C cen[4], aux[3];
+        group_start<S, C, CLAMP>(M, X, id, FT, 4, cen, ba);
+        rk4_dyn<S, C, CLAMP>(M, X, id, i, g, ba, rec, cen, aux, A, B);
+#pragma unroll
+        for (int q = 0; q < 4; q++) out(FT + q, cen[q]);
+        gst(fp<S>(M, F_AUX_U), id, S(aux[0]));
+        gst(fp<S>(M, F_AUX_V), id, S(aux[1]));
+        gst(fp<S>(M, F_AUX_W), id, S(aux[2]));
+    }
+    {
+        C cen[5];
+        group_start<S, C, CLAMP>(M, X, id, FC, 5, cen, ba);
+        rk4_moist<S, C, CLAMP>(M, X, id, g, ba, rec, cen);
+#pragma unroll
+        for (int q = 0; q < 5; q++) out(FC + q, cen[q]);
+    }
+    {
+        C cen[2], prod, ts, ds;
+        group_start<S, C, CLAMP>(M, X, id, FK, 2, cen, ba);
+        rk4_turb<S, C, CLAMP>(M, X, id, i, j, k, g, ba, rec, A, B, cen, prod, ts, ds);
+        out(FK, cen[0]);
+        out(FD, cen[1]);
+        gst(fp<S>(M, F_PROD), id, S(prod));
+        gst(fp<S>(M, F_TKE_SRC), id, S(ts));
+        gst(fp<S>(M, F_DIS_SRC), id, S(ds));
+    }
+}
+
 // Jacobi-order RK4 over the whole grid. Interior cells integrate four stages against the
 // start-of-RK4 neighbourhood read from f[] and write alt[]; boundary cells are copied. The
 // solid-ground clamps are applied on read (FUSED=1), and the store of the eight fields that
 // no later filter touches is fused (xn = new value).
 template <typename S, bool FUSED>
-__global__ __launch_bounds__(256) void k_rk4_jacobi(const MemberDesc* md) {
+__global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>()))) void k_rk4_jacobi(const MemberDesc* md) {
     using C = S;
+    __shared__ C s_ba[10 * LB], s_rec[12 * LB];
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const long long id = cell_id(K);
@@ -388,112 +541,54 @@ __global__ __launch_bounds__(256) void k_rk4_jacobi(const MemberDesc* md) {
     if (!interior) {
 #pragma unroll
         for (int f = 0; f < NPROG; f++) {
-            C v = C(X[f][id]);
+            C v = C(gld(X[f], id));
             if (FUSED) v = solid_clamp(f, v, tkm, vm);
-            ((S*)M.alt[f])[id] = S(v);
-            if (FUSED && f != FU && f != FV && f != FW) fp<S>(M, kProgN[f])[id] = S(v);
+            gst((S*)M.alt[f], id, S(v));
+            if (FUSED && f != FU && f != FV && f != FW) gst(fp<S>(M, kProgN[f]), id, S(v));
         }
         if (FUSED) {
             S* nue = fp<S>(M, F_NUE);
-            const C nv = C(nue[id]);
-            nue[id] = S(finite_bits(nv) ? tclamp(nv, C(0), C(K.nue_max)) : C(0));
+            const C nv = C(gld(nue, id));
+            gst(nue, id, S(finite_bits(nv) ? tclamp(nv, C(0), C(K.nue_max)) : C(0)));
             S* pr = fp<S>(M, F_PROD);
-            const C pv = C(pr[id]);
-            pr[id] = S(finite_bits(pv) ? tclamp(pv, C(0), C(1.0e4)) : C(0));
+            const C pv = C(gld(pr, id));
+            gst(pr, id, S(finite_bits(pv) ? tclamp(pv, C(0), C(1.0e4)) : C(0)));
             S* ts = fp<S>(M, F_TKE_SRC);
-            const C tv = C(ts[id]);
-            ts[id] = S(finite_bits(tv) ? tclamp(tv, C(-1.0e4), C(1.0e4)) : C(-1.0e4));
+            const C tv = C(gld(ts, id));
+            gst(ts, id, S(finite_bits(tv) ? tclamp(tv, C(-1.0e4), C(1.0e4)) : C(-1.0e4)));
             S* dsr = fp<S>(M, F_DIS_SRC);
-            const C dv = C(dsr[id]);
-            dsr[id] = S(finite_bits(dv) ? tclamp(dv, C(-1.0e4), C(1.0e4)) : C(-1.0e4));
+            const C dv = C(gld(dsr, id));
+            gst(dsr, id, S(finite_bits(dv) ? tclamp(dv, C(-1.0e4), C(1.0e4)) : C(-1.0e4)));
         }
         return;
     }
-    RhsIn<C> in;
-    load_nb<S, C, FUSED>(M, X, id, in);
-    const double* ps = pstat(M);
-    Geo<C> g;
-    make_geo<C>(M, i, j, k, ps[id], ps[ix3(K, 0, j, k)], g);
-    C cen[NPROG], base[NPROG];
-#pragma unroll
-    for (int f = 0; f < NPROG; f++) {
-        C c0 = C(X[f][id]);
-        C b0 = C(fp<S>(M, kProgN[f])[id]);
-        if (FUSED) { c0 = solid_clamp(f, c0, tkm, vm); b0 = solid_clamp_n(f, b0, tkm, vm); }
-        cen[f] = c0;
-        base[f] = b0;
-    }
-    {
-        C nv = C(fp<S>(M, F_NUE)[id]);
-        if (FUSED) nv = finite_bits(nv) ? tclamp(nv, C(0), C(K.nue_max)) : C(0);
-        in.nue = nv;
-    }
-    in.t_eq = C(fp<S>(M, F_T_EQ)[id]);
-    in.t_ref = C(M.t_ref[i]);
-    {
-        const int total = M.counters[1];
-        const double ramp = fmin(1.0, (double)total / 300.0);
-        in.buoy = C(ramp * 1.0 * K.g * K.dt / K.u_0);
-    }
-    C aux[3], prod, ts, ds;
-    rk4_cell<C>(K, g, in, base, cen, aux, prod, ts, ds);
-#pragma unroll
-    for (int f = 0; f < NPROG; f++) {
-        ((S*)M.alt[f])[id] = S(cen[f]);
-        if (FUSED && f != FU && f != FV && f != FW) fp<S>(M, kProgN[f])[id] = S(cen[f]);
-    }
-    fp<S>(M, F_AUX_U)[id] = S(aux[0]);
-    fp<S>(M, F_AUX_V)[id] = S(aux[1]);
-    fp<S>(M, F_AUX_W)[id] = S(aux[2]);
-    fp<S>(M, F_PROD)[id] = S(prod);
-    fp<S>(M, F_TKE_SRC)[id] = S(ts);
-    fp<S>(M, F_DIS_SRC)[id] = S(ds);
+    rk4_cell<S, C, FUSED>(M, X, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x, [&](int f, C v) {
+        gst((S*)M.alt[f], id, S(v));
+        if (FUSED && f != FU && f != FV && f != FW) gst(fp<S>(M, kProgN[f]), id, S(v));
+    });
 }
 
 // Lexicographic-order RK4 restricted to the hyperplane i+j+k = d (in place). Cells of one
 // plane only read neighbours on planes d-1 (final) and d+1 (not yet updated), which is
 // exactly what the reference's single-threaded sweep sees.
 template <typename S>
-__global__ __launch_bounds__(256) void k_rk4_lex_plane(const MemberDesc* md, int d) {
+__global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>()))) void k_rk4_lex_plane(const MemberDesc* md, int d) {
     using C = S;
+    __shared__ C s_ba[10 * LB], s_rec[12 * LB];
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const int nj = K.jm - 2;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (long long)(K.im - 2) * nj) return;
-    const int i = 1 + (int)(q / nj), j = 1 + (int)(q % nj);
+    const long long qd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (qd >= (long long)(K.im - 2) * nj) return;
+    const int i = 1 + (int)(qd / nj), j = 1 + (int)(qd % nj);
     const int k = d - i - j;
     if (k < 1 || k > K.km - 2) return;
-    const size_t id = ix3(K, i, j, k);
+    const long long id = (long long)ix3(K, i, j, k);
     S* X[NPROG];
 #pragma unroll
     for (int f = 0; f < NPROG; f++) X[f] = fp<S>(M, kProgF[f]);
-    RhsIn<C> in;
-    load_nb<S, C, false>(M, (const S* const*)X, id, in);
-    const double* ps = pstat(M);
-    Geo<C> g;
-    make_geo<C>(M, i, j, k, ps[id], ps[ix3(K, 0, j, k)], g);
-    C cen[NPROG], base[NPROG];
-#pragma unroll
-    for (int f = 0; f < NPROG; f++) { cen[f] = C(X[f][id]); base[f] = C(fp<S>(M, kProgN[f])[id]); }
-    in.nue = C(fp<S>(M, F_NUE)[id]);
-    in.t_eq = C(fp<S>(M, F_T_EQ)[id]);
-    in.t_ref = C(M.t_ref[i]);
-    {
-        const int total = M.counters[1];
-        const double ramp = fmin(1.0, (double)total / 300.0);
-        in.buoy = C(ramp * 1.0 * K.g * K.dt / K.u_0);
-    }
-    C aux[3], prod, ts, ds;
-    rk4_cell<C>(K, g, in, base, cen, aux, prod, ts, ds);
-#pragma unroll
-    for (int f = 0; f < NPROG; f++) X[f][id] = S(cen[f]);
-    fp<S>(M, F_AUX_U)[id] = S(aux[0]);
-    fp<S>(M, F_AUX_V)[id] = S(aux[1]);
-    fp<S>(M, F_AUX_W)[id] = S(aux[2]);
-    fp<S>(M, F_PROD)[id] = S(prod);
-    fp<S>(M, F_TKE_SRC)[id] = S(ts);
-    fp<S>(M, F_DIS_SRC)[id] = S(ds);
+    rk4_cell<S, C, false>(M, (const S* const*)X, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
+                          [&](int f, C v) { gst(X[f], id, S(v)); });
 }
 
 // ------------------------------------------------------------------------------------
