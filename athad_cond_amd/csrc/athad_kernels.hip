@@ -690,47 +690,107 @@ __global__ void k_p_ceiling(const MemberDesc* md, int gate) {
 // SaturationAdjustment (SaturationAdjustment.h:62-451; IceSchemeCommon.h:121-190) and the
 // cap on S_c_c (cAtmosphereModel.cpp:1278-1295). Pointwise, FP64 arithmetic.
 // ------------------------------------------------------------------------------------
-// Newton-loop helpers of the saturation adjustment: the same functions as th_* with the
-// shared sub-expressions evaluated once (results agree with th_* to a few ulp).
-ATHAD_DEV void sat_species_cp(double T, double Mb, double& cpH, double& cpC, double& cpB) {
-    const double t = tmin(tmax(T, 298.0), 2000.0) * 1.0e-3;
-    const double t2 = t * t, t3 = t2 * t, it2 = 1.0 / t2;
-    cpH = (30.09200 + 6.832514 * t + 6.793435 * t2 + -2.534480 * t3 + 0.082139 * it2) * (1.0 / kMH2O);
-    cpC = (24.99735 + 55.18696 * t + -33.69137 * t2 + 7.948387 * t3 + -0.136638 * it2) * (1.0 / kMCO2);
-    cpB = (28.98641 + 1.853978 * t + -9.647459 * t2 + 16.63537 * t3 + 0.000117 * it2) / Mb;
+// Newton-loop helpers of the saturation adjustment, templated on the Newton arithmetic type N
+// (double in the drop-in form, float in the mixed form). They restate th_* with shared
+// sub-expressions evaluated once and divisions by positive bounded operands done through
+// refined reciprocals; the saturation vapour pressures, q_sat and its derivative factor are
+// always FP64.
+template <typename N> ATHAD_DEV void sat_species_cp(N T, N Mb, N& cpH, N& cpC, N& cpB) {
+    const N t = tmin(tmax(T, N(298)), N(2000)) * N(1.0e-3);
+    const N t2 = t * t, t3 = t2 * t, it2 = rcp_nr(t2);
+    cpH = (N(30.09200) + N(6.832514) * t + N(6.793435) * t2 + N(-2.534480) * t3 + N(0.082139) * it2) * N(1.0 / kMH2O);
+    cpC = (N(24.99735) + N(55.18696) * t + N(-33.69137) * t2 + N(7.948387) * t3 + N(-0.136638) * it2) * N(1.0 / kMCO2);
+    cpB = (N(28.98641) + N(1.853978) * t + N(-9.647459) * t2 + N(16.63537) * t3 + N(0.000117) * it2) / Mb;
 }
-ATHAD_DEV double sat_mix_cp(double c, double co2, double cpH, double cpC, double cpB) {
-    double qv, qc, qb;
-    th_split(c, co2, qv, qc, qb);
+template <typename N> ATHAD_DEV N sat_mix_cp(N c, N co2, N cpH, N cpC, N cpB) {
+    N qv = tmin(tmax(c, N(0)), N(1)), qc = tmin(tmax(co2, N(0)), N(1));
+    const N s = qv + qc;
+    if (s > N(1)) { qv /= s; qc /= s; }
+    const N qb = tmax(N(0), N(1) - qv - qc);
     return qv * cpH + qc * cpC + qb * cpB;
 }
-ATHAD_DEV double sat_L_vap(double T) {
-    if (T >= kTcrit) return 0.0;
-    return 2.501e6 * exp(0.38 * log((kTcrit - T) / (kTcrit - 273.15)));
+template <typename N> ATHAD_DEV N sat_L_vap(N T) {
+    if (T >= N(kTcrit)) return N(0);
+    return N(2.501e6) * exp(N(0.38) * log((N(kTcrit) - T) * N(1.0 / (kTcrit - 273.15))));
+}
+ATHAD_DEV double sat_p_sat(double T) {
+    if (T >= kTcrit) return kNoSat;
+    if (T <= 0.0) return 0.0;
+    const double th = 1.0 - T * (1.0 / kTcrit);
+    const double sq = sqrt(th);
+    const double t3 = th * th * th;
+    const double t4 = t3 * th;
+    const double s = -7.85951783 * th + 1.84408259 * (th * sq) + -11.7866497 * t3
+                   + 22.6807411 * (t3 * sq) + -15.9618719 * t4 + 1.80122502 * (t4 * t3 * sq);
+    return kPcrit * exp((kTcrit * rcp_nr(T)) * s);
 }
 ATHAD_DEV double sat_p_subl(double T) {
     if (T <= 0.0) return 0.0;
-    const double th = T / kTtriple;
+    const double th = T * (1.0 / kTtriple);
     const double lth = log(th);
     const double s = -0.212144006e2 * exp(0.333333333e-2 * lth) + 0.273203819e2 * exp(0.120666667e1 * lth)
                    + -0.610598130e1 * exp(0.170333333e1 * lth);
-    return kPtriple * exp(s / th);
+    return kPtriple * exp(s * rcp_nr(th));
 }
-// exact q_sat with its mole fraction x and denominator returned for the derivative factor
-ATHAD_DEV double sat_q(double E, double p, double Mo, double& x, double& den) {
-    x = 0.0; den = 0.0;
-    if (E == kNoSat) return 1.0;
-    if (!(p > 0.0)) return 0.0;
-    if (E >= p) return 1.0;
-    x = E / p;
+// exact q_sat (MixtureAtm.h:152-217) and the factor A of dq_sat/dT = A L / (R_v T^2)
+ATHAD_DEV void sat_qA(double E, double p, double inv_p, double Mo, double& q, double& A) {
+    A = 0.0;
+    if (E == kNoSat) { q = 1.0; return; }
+    if (!(p > 0.0)) { q = 0.0; return; }
+    if (E >= p) { q = 1.0; return; }
+    const double x = E * inv_p;
     const double num = x * kMH2O;
-    den = num + (1.0 - x) * Mo;
-    return (den > 0.0) ? num / den : 1.0;
+    const double den = num + (1.0 - x) * Mo;
+    if (!(den > 0.0)) { q = 1.0; return; }
+    const double rd = rcp_nr(den);
+    q = num * rd;
+    A = num * Mo * rd * rd;
 }
-ATHAD_DEV double sat_A_from(double E, double p, double Mo, double x, double den) {
-    if (E == kNoSat || !(p > 0.0) || E >= p) return 0.0;
-    if (!(den > 0.0)) return 0.0;
-    return x * kMH2O * Mo / (den * den);
+
+// Damped Newton iteration of adjustSaturation (SaturationAdjustment.h:177-254) in type N.
+// The ice branch (sublimation pressure, its q_sat and derivative) is evaluated only when it
+// can contribute: deposition fraction > 0, ice present, or the no-condensate target is ice;
+// otherwise every term it feeds is multiplied by an exact zero. Likewise the liquid branch.
+template <typename N>
+ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi_io, double co2d, double p_local,
+                          double M_other, double Mb_d, double t_00d, double t_0d) {
+    const N t_00 = N(t_00d), t_0 = N(t_0d), t_range_inv = N(1.0 / (t_0d - t_00d)), co2 = N(co2d), Mb = N(Mb_d);
+    const double inv_p = (p_local > 0.0) ? 1.0 / p_local : 0.0;
+    N T = N(T_io), qvb = N(qv_io), qcb = N(qc_io), qib = N(qi_io), qv_hyp = qvb;
+    // species heat capacities and L(T) of iteration k+1's update are those of iteration k's
+    // new temperature
+    N cpH, cpC, cpB;
+    sat_species_cp(T, Mb, cpH, cpC, cpB);
+    N Lv = sat_L_vap(T);
+    for (int it = 1; it <= 20; it++) {
+        const N CND = tmax(N(0), tmin(N(1), (T - t_00) * t_range_inv));
+        const N DEP = N(1) - CND;
+        N dqv = qv_hyp - qvb;
+        if (dqv > N(0)) { const N me = qcb + qib; if (dqv > me) dqv = me; }
+        const N dcnd = dqv * CND, ddep = dqv * DEP;
+        qvb += dqv;
+        qcb = tmax(N(0), qcb - dcnd);
+        qib = tmax(N(0), qib - ddep);
+        T -= (Lv * dcnd + (Lv + N(3.337e5)) * ddep) * rcp_nr(sat_mix_cp(qvb, co2, cpH, cpC, cpB));
+        sat_species_cp(T, Mb, cpH, cpC, cpB);
+        Lv = sat_L_vap(T);
+        const N qsum = qcb + qib;
+        const bool hot = (T >= t_0), some = (qsum > N(1e-12));
+        const bool need_l = (CND > N(0)) || (qcb > N(0)) || (!some && hot);
+        const bool need_i = (DEP > N(0)) || (qib > N(0)) || (!some && !hot);
+        double qs = 0.0, Ac = 0.0, qi = 0.0, Ad = 0.0;
+        if (need_l) sat_qA(sat_p_sat((double)T), p_local, inv_p, M_other, qs, Ac);
+        if (need_i) sat_qA(sat_p_subl((double)T), p_local, inv_p, M_other, qi, Ad);
+        const N target = some ? (qcb * N(qs) + qib * N(qi)) * rcp_nr(qsum) : (hot ? N(qs) : N(qi));
+        const N inv_cp = rcp_nr(sat_mix_cp(qvb, co2, cpH, cpC, cpB));
+        const N inv_RT2 = (T > N(0)) ? rcp_nr(N(kRH2O) * T * T) : N(0);
+        const N Lc = Lv, Ld = Lv + N(3.337e5);
+        const N gain = CND * (Lc * inv_cp) * (N(Ac) * Lc * inv_RT2) + DEP * (Ld * inv_cp) * (N(Ad) * Ld * inv_RT2);
+        const N om = rcp_nr(N(1) + gain);
+        qv_hyp = qvb + om * (target - qvb);
+        if (fabs(qvb / qv_hyp - N(1)) <= N(1.0e-6)) break;
+    }
+    T_io = (double)T; qv_io = (double)qvb; qc_io = (double)qcb; qi_io = (double)qib;
 }
 
 template <typename S>
@@ -759,45 +819,9 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate) 
             const double qs0 = th_q_sat(th_p_auto(T), p_local, M_other);
             const double alpha = 1.0 / (1.0 + exp(-(T - t_00) / 5.0));
             if ((qv_old > qs0 && alpha > 0.01) || (qv_old < qs0 && (qc_old > 1e-12 || qi_old > 1e-12))) {
-                double qvb = qv_old, qcb = qc_old, qib = qi_old, qv_hyp = qvb;
+                double qvb = qv_old, qcb = qc_old, qib = qi_old;
                 const double T_orig = t * t_0;
-                // Per-temperature quantities are evaluated once per Newton temperature and
-                // reused: the species heat capacities and L(T) of iteration k+1's update are
-                // those computed at the end of iteration k (same T).
-                double cpH, cpC, cpB, Lv;
-                sat_species_cp(T, Mb, cpH, cpC, cpB);
-                Lv = sat_L_vap(T);
-                for (int it = 1; it <= 20; it++) {
-                    const double CND = tmax(0.0, tmin(1.0, (T - t_00) * t_range_inv));
-                    const double DEP = 1.0 - CND;
-                    double dqv = qv_hyp - qvb;
-                    if (dqv > 0) { const double me = qcb + qib; if (dqv > me) dqv = me; }
-                    const double dcnd = dqv * CND, ddep = dqv * DEP;
-                    qvb += dqv;
-                    qcb = tmax(0.0, qcb - dcnd);
-                    qib = tmax(0.0, qib - ddep);
-                    const double cp_loc = sat_mix_cp(qvb, co2, cpH, cpC, cpB);
-                    T -= (Lv * dcnd + (Lv + 3.337e5) * ddep) / cp_loc;
-                    // new temperature
-                    sat_species_cp(T, Mb, cpH, cpC, cpB);
-                    Lv = sat_L_vap(T);
-                    const double Es = th_p_sat(T), Ei = sat_p_subl(T);
-                    double xs, dens, xi, deni;
-                    const double qs = sat_q(Es, p_local, M_other, xs, dens);
-                    const double qi = sat_q(Ei, p_local, M_other, xi, deni);
-                    const double qsum = qcb + qib;
-                    const double target = (qsum > 1e-12) ? (qcb * qs + qib * qi) / qsum : ((T >= t_0) ? qs : qi);
-                    const double cp_g = sat_mix_cp(qvb, co2, cpH, cpC, cpB);
-                    const double Lc = Lv, Ld = Lv + 3.337e5;
-                    const double inv_RT2 = (T > 0.0) ? 1.0 / (kRH2O * T * T) : 0.0;
-                    const double Ac = sat_A_from(Es, p_local, M_other, xs, dens);
-                    const double Ad = sat_A_from(Ei, p_local, M_other, xi, deni);
-                    const double inv_cp = 1.0 / cp_g;
-                    const double gain = CND * (Lc * inv_cp) * (Ac * Lc * inv_RT2) + DEP * (Ld * inv_cp) * (Ad * Ld * inv_RT2);
-                    const double om = 1.0 / (1.0 + gain);
-                    qv_hyp = qvb + om * (target - qvb);
-                    if (fabs(qvb / qv_hyp - 1.0) <= 1.0e-6) break;
-                }
+                sat_newton<S>(T, qvb, qcb, qib, co2, p_local, M_other, Mb, t_00, t_0);
                 qcb = tmax(0.0, qcb);
                 qib = tmax(0.0, qib);
                 if (T > kTcrit) T = kTcrit;
