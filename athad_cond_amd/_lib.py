@@ -36,7 +36,7 @@ EXPORTS = [
     "athad_download_rows", "athad_set_iter", "athad_get_iter", "athad_poisson_sweep", "athad_satadj",
     "athad_damp_moist", "athad_densities", "athad_turbulence", "athad_boundary", "athad_rk4",
     "athad_filters", "athad_store", "athad_step", "athad_run", "athad_sync", "athad_profile",
-    "athad_profile_result", "athad_version",
+    "athad_profile_result", "athad_version", "athad_dew_point", "athad_member_from_surface",
 ]
 
 
@@ -91,8 +91,11 @@ def lib():
         L.athad_profile.argtypes = [vp, i, i]
         L.athad_profile_result.argtypes = [vp, i, ctypes.POINTER(d), ctypes.POINTER(i)]
         L.athad_version.restype = ctypes.c_char_p
+        L.athad_dew_point.restype = d
+        L.athad_dew_point.argtypes = [d]
+        L.athad_member_from_surface.argtypes = [d, d, d, d, ctypes.POINTER(MemberParams)]
         for name in EXPORTS:
-            if name not in ("athad_last_error", "athad_footprint_bytes", "athad_version"):
+            if name not in ("athad_last_error", "athad_footprint_bytes", "athad_version", "athad_dew_point"):
                 getattr(L, name).restype = i
         _lib = L
     return _lib

@@ -37,13 +37,14 @@ athad_member_params default_member() {
 }
 
 // AtmMixture::resolve and M_nonwater (MixtureAtm.h:82-109, :152-217), host copies.
-struct Comp { double M_mean, R_mix, q_CO2, q_bg, M_bg, R_bg; };
+struct Comp { double M_mean, R_mix, q_H2O, q_CO2, q_bg, M_bg, R_bg; };
 constexpr double hRstar = 8.314462618, hMH2O = 0.018015, hMCO2 = 0.044010, hMN2 = 0.028014;
 Comp resolve(double xh, double xc, double xn) {
     Comp C;
     const double mh = xh * hMH2O, mc = xc * hMCO2, mb = xn * hMN2;
     C.M_mean = mh + mc + mb;
     C.R_mix = hRstar / C.M_mean;
+    C.q_H2O = mh / C.M_mean;
     C.q_CO2 = mc / C.M_mean;
     C.q_bg = mb / C.M_mean;
     C.M_bg = (xn > 0.0) ? (mb / xn) : hMN2;
@@ -507,6 +508,43 @@ athad_status athad_destroy(athad_ctx* ctx) {
 
 const char* athad_last_error(const athad_ctx* ctx) {
     return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+double athad_dew_point(double e_hPa) {
+    if (!(e_hPa > 0.0)) return 0.0;
+    if (e_hPa >= kPcrit) return kTcrit;
+    double lo = 150.0, hi = kTcrit;
+    if (th_p_auto(lo) >= e_hPa) return lo;
+    for (int it = 0; it < 60; it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (th_p_auto(mid) < e_hPa) lo = mid; else hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+athad_status athad_member_from_surface(double p_surf_bar, double t_sea, double x_h2o_dry, double x_co2_dry,
+                                       athad_member_params* out) {
+    if (!out || !(p_surf_bar > 0.0) || !(t_sea > 0.0) || !(x_h2o_dry >= 0.0) || !(x_co2_dry >= 0.0) ||
+        x_h2o_dry + x_co2_dry > 1.0)
+        return ATHAD_ERR_ARG;
+    const double e_sea = th_p_sat(t_sea);                     // [hPa]
+    const double p_hpa = p_surf_bar * 1000.0;
+    if (!(e_sea > 0.0) || e_sea >= p_hpa) return ATHAD_ERR_ARG;   // sea would boil
+    const double x_n2_dry = 1.0 - x_h2o_dry - x_co2_dry;
+    const double x_h2o = e_sea / p_hpa;                        // saturated over the sea
+    const double rest = 1.0 - x_h2o, dry_non_h2o = x_co2_dry + x_n2_dry;
+    const double x_co2 = (dry_non_h2o > 0.0) ? rest * x_co2_dry / dry_non_h2o : 0.0;
+    const double x_n2 = (dry_non_h2o > 0.0) ? rest * x_n2_dry / dry_non_h2o : 0.0;
+    const Comp S = resolve(x_h2o, x_co2, x_n2);
+    const Comp D = resolve(x_h2o_dry, x_co2_dry, x_n2_dry);
+    out->t_surf_equator = t_sea;
+    out->t_surf_pole = t_sea - 10.0;
+    out->r_air = p_surf_bar * 1.0e5 / (S.R_mix * t_sea);
+    out->x_H2O = x_h2o; out->x_CO2 = x_co2; out->x_N2 = x_n2;
+    out->c_0 = S.q_H2O;
+    out->co2_0 = S.q_CO2;
+    out->c_h2o_dry_top = D.q_H2O;
+    return ATHAD_OK;
 }
 
 athad_status athad_sync(athad_ctx* ctx) {

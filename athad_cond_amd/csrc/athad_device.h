@@ -67,12 +67,13 @@ struct MemberDesc {
 };
 
 #define ATHAD_DEV __device__ __forceinline__
+#define ATHAD_HD __host__ __device__ __forceinline__
 
 ATHAD_DEV size_t ix3(const Consts& K, int i, int j, int k) { return ((size_t)i * K.jm + j) * K.km + k; }
 
-template <typename T> ATHAD_DEV T tmax(T a, T b) { return (a < b) ? b : a; }       // std::max
-template <typename T> ATHAD_DEV T tmin(T a, T b) { return (b < a) ? b : a; }       // std::min
-template <typename T> ATHAD_DEV T tclamp(T v, T lo, T hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
+template <typename T> ATHAD_HD T tmax(T a, T b) { return (a < b) ? b : a; }       // std::max
+template <typename T> ATHAD_HD T tmin(T a, T b) { return (b < a) ? b : a; }       // std::min
+template <typename T> ATHAD_HD T tclamp(T v, T lo, T hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
 
 ATHAD_DEV bool finite_bits(double v) {
     return (__double_as_longlong(v) & 0x7FF0000000000000LL) != 0x7FF0000000000000LL;
@@ -98,7 +99,7 @@ constexpr double kRstar = 8.314462618, kMH2O = 0.018015, kMCO2 = 0.044010;
 #define kRH2O (kRstar / kMH2O)
 #define kRCO2 (kRstar / kMCO2)
 
-ATHAD_DEV double th_p_sat(double T) {
+ATHAD_HD double th_p_sat(double T) {
     if (T >= kTcrit) return kNoSat;
     if (T <= 0.0) return 0.0;
     const double th = 1.0 - T / kTcrit;
@@ -109,24 +110,24 @@ ATHAD_DEV double th_p_sat(double T) {
                    + 22.6807411 * (t3 * sq) + -15.9618719 * t4 + 1.80122502 * (t4 * t3 * sq);
     return kPcrit * exp((kTcrit / T) * s);
 }
-ATHAD_DEV double th_p_subl(double T) {
+ATHAD_HD double th_p_subl(double T) {
     if (T <= 0.0) return 0.0;
     const double th = T / kTtriple;
     const double s = -0.212144006e2 * pow(th, 0.333333333e-2) + 0.273203819e2 * pow(th, 0.120666667e1)
                    + -0.610598130e1 * pow(th, 0.170333333e1);
     return kPtriple * exp(s / th);
 }
-ATHAD_DEV double th_p_auto(double T) {
+ATHAD_HD double th_p_auto(double T) {
     if (T >= kTcrit) return kNoSat;
     return (T >= kTtriple) ? th_p_sat(T) : th_p_subl(T);
 }
-ATHAD_DEV double th_L_vap(double T) {
+ATHAD_HD double th_L_vap(double T) {
     if (T >= kTcrit) return 0.0;
     return 2.501e6 * pow((kTcrit - T) / (kTcrit - 273.15), 0.38);
 }
-ATHAD_DEV double th_L_sub(double T) { return th_L_vap(T) + 3.337e5; }
-ATHAD_DEV double th_L_auto(double T) { return (T >= kTtriple) ? th_L_vap(T) : th_L_sub(T); }
-ATHAD_DEV double th_q_sat(double E, double p, double Mo) {
+ATHAD_HD double th_L_sub(double T) { return th_L_vap(T) + 3.337e5; }
+ATHAD_HD double th_L_auto(double T) { return (T >= kTtriple) ? th_L_vap(T) : th_L_sub(T); }
+ATHAD_HD double th_q_sat(double E, double p, double Mo) {
     if (E == kNoSat) return 1.0;
     if (!(p > 0.0)) return 0.0;
     if (E >= p) return 1.0;
@@ -135,18 +136,18 @@ ATHAD_DEV double th_q_sat(double E, double p, double Mo) {
     const double den = num + (1.0 - x) * Mo;
     return (den > 0.0) ? num / den : 1.0;
 }
-ATHAD_DEV double th_sat_A(double E, double p, double Mo) {
+ATHAD_HD double th_sat_A(double E, double p, double Mo) {
     if (E == kNoSat || !(p > 0.0) || E >= p) return 0.0;
     const double x = E / p;
     const double den = x * kMH2O + (1.0 - x) * Mo;
     if (!(den > 0.0)) return 0.0;
     return x * kMH2O * Mo / (den * den);
 }
-ATHAD_DEV double th_dqdT(double E, double L, double T, double p, double Mo) {
+ATHAD_HD double th_dqdT(double E, double L, double T, double p, double Mo) {
     if (!(T > 0.0)) return 0.0;
     return th_sat_A(E, p, Mo) * L / (kRH2O * T * T);
 }
-ATHAD_DEV double th_moist_lapse(double T, double p, double Mo, double cp, double Rm, double g) {
+ATHAD_HD double th_moist_lapse(double T, double p, double Mo, double cp, double Rm, double g) {
     if (!(cp > 0.0) || !(T > 0.0) || !(Rm > 0.0)) return 0.0;
     const double A = th_sat_A(th_p_auto(T), p, Mo);
     if (!(A > 0.0)) return g / cp;
@@ -155,31 +156,31 @@ ATHAD_DEV double th_moist_lapse(double T, double p, double Mo, double cp, double
     const double den = cp + L * L * A / (kRH2O * T * T);
     return (den > 0.0) ? g * num / den : g / cp;
 }
-ATHAD_DEV double th_q_sat_at(double T, double p, double Mo) { return th_q_sat(th_p_auto(T), p, Mo); }
+ATHAD_HD double th_q_sat_at(double T, double p, double Mo) { return th_q_sat(th_p_auto(T), p, Mo); }
 
-ATHAD_DEV double th_shomate(double T, double A, double B, double C, double D, double E) {
+ATHAD_HD double th_shomate(double T, double A, double B, double C, double D, double E) {
     const double t = tmin(tmax(T, 298.0), 2000.0) * 1.0e-3;
     const double t2 = t * t;
     return A + B * t + C * t2 + D * t2 * t + E / t2;
 }
-ATHAD_DEV void th_split(double c, double co2, double& qv, double& qc, double& qb) {
+ATHAD_HD void th_split(double c, double co2, double& qv, double& qc, double& qb) {
     qv = tmin(tmax(c, 0.0), 1.0);
     qc = tmin(tmax(co2, 0.0), 1.0);
     const double s = qv + qc;
     if (s > 1.0) { qv /= s; qc /= s; }
     qb = tmax(0.0, 1.0 - qv - qc);
 }
-ATHAD_DEV double th_R_of(double c, double co2, double Rb) {
+ATHAD_HD double th_R_of(double c, double co2, double Rb) {
     double qv, qc, qb; th_split(c, co2, qv, qc, qb);
     return qv * kRH2O + qc * kRCO2 + qb * Rb;
 }
-ATHAD_DEV double th_cp_of(double c, double co2, double T, double Mb) {
+ATHAD_HD double th_cp_of(double c, double co2, double T, double Mb) {
     double qv, qc, qb; th_split(c, co2, qv, qc, qb);
     return qv * (th_shomate(T, 30.09200, 6.832514, 6.793435, -2.534480, 0.082139) / kMH2O)
          + qc * (th_shomate(T, 24.99735, 55.18696, -33.69137, 7.948387, -0.136638) / kMCO2)
          + qb * (th_shomate(T, 28.98641, 1.853978, -9.647459, 16.63537, 0.000117) / Mb);
 }
-ATHAD_DEV double th_M_nonwater(double c, double co2, double Mb) {
+ATHAD_HD double th_M_nonwater(double c, double co2, double Mb) {
     double qv, qc, qb; th_split(c, co2, qv, qc, qb);
     const double s = qc + qb;
     if (!(s > 0.0)) return Mb;

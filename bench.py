@@ -160,6 +160,9 @@ def main():
         lex_steps = max(4, min(args.steps, 12))
         lex = run_form(A, "fp64", "lex", lex_steps, 4, graph=graph)
         out["lex"] = {"cells_per_s": lex["cells_per_s"], "ms_per_step": lex["ms_per_step"], "steps": lex_steps}
+        from athad_cond_amd.ensemble import run_ensemble
+        out["ensemble"] = {prec: run_ensemble(32, (41, 91, 181), steps=40, precision=prec)
+                           for prec in ("fp64", "mixed")}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -80,6 +80,16 @@ athad_status athad_create(int n_members, const athad_grid* grids, const athad_me
 athad_status athad_destroy(athad_ctx* ctx);
 /* Last error message of this context (or of the last failed athad_create when ctx is NULL). */
 const char* athad_last_error(const athad_ctx* ctx);
+/* Boiling temperature [K] of water at vapour pressure e [hPa]: bisection on the saturation
+ * curve, as SaturationH2O::dewPoint (atmosphere/SaturationH2O.h:189-202). Host only. */
+double athad_dew_point(double e_hPa);
+/* Member parameters of an integration from its surface state: sea saturated at t_sea under
+ * p_surf, CO2 : N2 at the dry ratio, r_air from the ideal gas at the surface, dry-top
+ * vapour mass fraction from the dry composition; the pole is 10 K colder than the equator.
+ * As test/cond_column_selftest.cpp:86-145 derives the config values. Host only; returns
+ * ATHAD_ERR_ARG if the sea would boil. */
+athad_status athad_member_from_surface(double p_surf_bar, double t_sea, double x_h2o_dry, double x_co2_dry,
+                                       athad_member_params* out);
 /* Device bytes held by a context of this shape (admission control). */
 int64_t athad_footprint_bytes(const athad_grid* grid, int precision);
 
