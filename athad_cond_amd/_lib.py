@@ -23,6 +23,7 @@ FIELDS2D = ["vel_star", "t_top_init"]
 PROGNOSTIC = FIELDS[:11]
 
 FP64, MIXED = 0, 1
+OK, NOT_READY = 0, 6
 LEX, JACOBI = 0, 1
 
 KERNEL_ID = {"rk4": 0, "rk4_plane": 1, "level_mean": 2, "poisson": 3, "satadj": 4, "damp": 5,
@@ -37,6 +38,7 @@ EXPORTS = [
     "athad_damp_moist", "athad_densities", "athad_turbulence", "athad_boundary", "athad_rk4",
     "athad_filters", "athad_store", "athad_step", "athad_run", "athad_sync", "athad_profile",
     "athad_profile_result", "athad_version", "athad_dew_point", "athad_member_from_surface",
+    "athad_query", "athad_device_memory",
 ]
 
 
@@ -91,6 +93,8 @@ def lib():
         L.athad_profile.argtypes = [vp, i, i]
         L.athad_profile_result.argtypes = [vp, i, ctypes.POINTER(d), ctypes.POINTER(i)]
         L.athad_version.restype = ctypes.c_char_p
+        L.athad_query.argtypes = [vp]
+        L.athad_device_memory.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
         L.athad_dew_point.restype = d
         L.athad_dew_point.argtypes = [d]
         L.athad_member_from_surface.argtypes = [d, d, d, d, ctypes.POINTER(MemberParams)]

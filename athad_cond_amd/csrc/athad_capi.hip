@@ -547,6 +547,23 @@ athad_status athad_member_from_surface(double p_surf_bar, double t_sea, double x
     return ATHAD_OK;
 }
 
+athad_status athad_query(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipSuccess) return ATHAD_OK;
+    if (e == hipErrorNotReady) return ATHAD_NOT_READY;
+    c->err = std::string("hipStreamQuery: ") + hipGetErrorString(e);
+    return ATHAD_ERR_DEVICE;
+}
+
+athad_status athad_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return ATHAD_ERR_DEVICE;
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
+    return ATHAD_OK;
+}
+
 athad_status athad_sync(athad_ctx* ctx) {
     if (!ctx) return ATHAD_ERR_ARG;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

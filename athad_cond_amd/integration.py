@@ -131,6 +131,14 @@ class Integration:
     def run(self, n: int, graph: bool = False):
         self._chk(L.lib().athad_run(self.h, n, int(graph)))
 
+    def busy(self) -> bool:
+        """True while work of this context is still in flight (non-blocking)."""
+        st = L.lib().athad_query(self.h)
+        if st == L.NOT_READY:
+            return True
+        self._chk(st)
+        return False
+
     def sync(self):
         self._chk(L.lib().athad_sync(self.h))
 

@@ -30,6 +30,7 @@ typedef int athad_status;
 #define ATHAD_ERR_ALLOC     3   /* device allocation failed */
 #define ATHAD_ERR_SHAPE     4   /* grid shape not supported or mismatched */
 #define ATHAD_ERR_STATE     5   /* call not valid in the context's current state */
+#define ATHAD_NOT_READY     6   /* athad_query: work still in flight (not an error) */
 
 /* Storage precision */
 #define ATHAD_FP64   0          /* drop-in form: FP64 storage and arithmetic */
@@ -147,6 +148,11 @@ athad_status athad_store(athad_ctx* ctx);
 athad_status athad_step(athad_ctx* ctx);
 /* n_steps iterations; use_graph replays a captured 4-iteration hipGraph. */
 athad_status athad_run(athad_ctx* ctx, int n_steps, int use_graph);
+/* Non-blocking: ATHAD_OK when all work of the context has finished, ATHAD_NOT_READY while
+ * some is in flight. For host loops that keep several contexts busy at once (serving). */
+athad_status athad_query(athad_ctx* ctx);
+/* Free and total device memory of the current device (admission control). */
+athad_status athad_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 /* Waits for the context's stream. */
 athad_status athad_sync(athad_ctx* ctx);
 
