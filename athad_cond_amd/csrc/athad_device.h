@@ -147,14 +147,18 @@ ATHAD_HD double th_dqdT(double E, double L, double T, double p, double Mo) {
     if (!(T > 0.0)) return 0.0;
     return th_sat_A(E, p, Mo) * L / (kRH2O * T * T);
 }
-ATHAD_HD double th_moist_lapse(double T, double p, double Mo, double cp, double Rm, double g) {
+// moist-adiabatic lapse rate with the saturation pressure E = p_auto(T) supplied
+ATHAD_HD double th_moist_lapse_E(double T, double p, double Mo, double cp, double Rm, double g, double E) {
     if (!(cp > 0.0) || !(T > 0.0) || !(Rm > 0.0)) return 0.0;
-    const double A = th_sat_A(th_p_auto(T), p, Mo);
+    const double A = th_sat_A(E, p, Mo);
     if (!(A > 0.0)) return g / cp;
     const double L = th_L_auto(T);
     const double num = 1.0 + L * A / (Rm * T);
     const double den = cp + L * L * A / (kRH2O * T * T);
     return (den > 0.0) ? g * num / den : g / cp;
+}
+ATHAD_HD double th_moist_lapse(double T, double p, double Mo, double cp, double Rm, double g) {
+    return th_moist_lapse_E(T, p, Mo, cp, Rm, g, th_p_auto(T));
 }
 ATHAD_HD double th_q_sat_at(double T, double p, double Mo) { return th_q_sat(th_p_auto(T), p, Mo); }
 

@@ -969,10 +969,17 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* md, int set
     double p_prev = K.p_sl_factor * T_prev;
     double q_sat_min = 1.0;
     double qv_prev = 0.0, qc_prev = 0.0;
+    // this level's inputs; the next level's are loaded before this level's stores so their
+    // latency overlaps the column arithmetic
+    double qv = gld(c, (long long)base), qc = gld(co2, (long long)base);
+    double wcl = gld(cl, (long long)base), wic = gld(ice, (long long)base);
     for (int i = 0; i < K.im; i++) {
-        const size_t id = base + (size_t)i * si;
-        double qv = c[id];
-        const double qc = co2[id];
+        const long long id = (long long)base + (long long)i * (long long)si;
+        double qv_n = 0.0, qc_n = 0.0, cl_n = 0.0, ic_n = 0.0;
+        if (i + 1 < K.im) {
+            qv_n = gld(c, id + (long long)si); qc_n = gld(co2, id + (long long)si);
+            cl_n = gld(cl, id + (long long)si); ic_n = gld(ice, id + (long long)si);
+        }
         const double R_loc = th_R_of(qv, qc, Rb);
         double Ti, pi;
         if (i == 0) { Ti = T_prev; pi = p_prev; }
@@ -980,29 +987,31 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* md, int set
             const double dz = M.hgt[i] - M.hgt[i - 1];
             const double cp_loc = th_cp_of(qv_prev, qc_prev, T_prev, Mb);
             const double M_nw = th_M_nonwater(qv_prev, qc_prev, Mb);
-            const double q_s = th_q_sat_at(T_prev, p_prev, M_nw);
-            const double gamma = (qv_prev >= 0.99 * q_s) ? th_moist_lapse(T_prev, p_prev, M_nw, cp_loc, R_loc, K.g)
-                                                         : K.g / cp_loc;
+            const double E = th_p_auto(T_prev);
+            const double q_s = th_q_sat(E, p_prev, M_nw);
+            const double gamma = (qv_prev >= 0.99 * q_s)
+                ? th_moist_lapse_E(T_prev, p_prev, M_nw, cp_loc, R_loc, K.g, E) : K.g / cp_loc;
             const double T_ad = T_prev - gamma * dz;
             Ti = tmax(K.t_skin, T_ad);
             const double T_mean = 0.5 * (T_prev + Ti);
             pi = p_prev * exp(-K.g * dz / (R_loc * T_mean));
         }
-        t[id] = S(Ti / t_0);
-        ps[id] = pi;
+        gst(t, id, S(Ti / t_0));
+        gst(ps, id, pi);
         if (set_water && i > 0) {
             const double q_s_i = th_q_sat_at(Ti, pi, th_M_nonwater(qv, qc, Mb));
             if (q_s_i < q_sat_min) q_sat_min = q_s_i;
             qv = tmax(q_sat_min, K.c_h2o_dry_top);
-            c[id] = S(qv);
+            gst(c, id, S(qv));
         }
-        const double wf = tmax(0.5, 1.0 - (double)cl[id] - (double)ice[id]);
-        rh[id] = S(1e2 * pi / (R_loc * Ti * wf));
-        rd[id] = S(1e2 * pi / (th_R_of(0.0, qc, Rb) * Ti));
+        const double wf = tmax(0.5, 1.0 - (double)S(wcl) - (double)S(wic));
+        gst(rh, id, S(1e2 * pi / (R_loc * Ti * wf)));
+        gst(rd, id, S(1e2 * pi / (th_R_of(0.0, qc, Rb) * Ti)));
         // the next level reads this level's stored water (the reference reads c[i-1] back)
-        qv_prev = (double)c[id];
+        qv_prev = (double)S(qv);
         qc_prev = qc;
         T_prev = Ti; p_prev = pi;
+        qv = qv_n; qc = qc_n; wcl = cl_n; wic = ic_n;
     }
     M.t_top[base] = (double)t[base + (size_t)(K.im - 1) * si];
     if (!evaporate) return;
@@ -1174,6 +1183,10 @@ __global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* md, int 
 // bcTheta per (i,k), bcPhi + seam smoothing per (i,j), bcSolidGround clamps per cell.
 // BC_Atm.h:556-917, :23-551; TurbulenceAtm.h:109-147.
 // ------------------------------------------------------------------------------------
+// The boundary kernels load every value they need before storing any (the field pointers
+// may alias as far as the compiler knows, so interleaved loads and stores would serialise
+// on memory round trips); the final values equal those of the reference's sequence of
+// assignments.
 template <typename S>
 __global__ void k_bc_radial(const MemberDesc* md) {
     using C = S;
@@ -1181,47 +1194,74 @@ __global__ void k_bc_radial(const MemberDesc* md) {
     const Consts& K = M.K;
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (long long)K.jm * K.km) return;
-    const size_t si = (size_t)K.jm * K.km;
+    const long long si = (long long)K.jm * K.km;
     const int iml = K.im - 1;
-    auto at = [&](int i) { return (size_t)i * si + (size_t)q; };
-    fp<S>(M, F_T)[at(iml)] = S(M.t_top[q]);
-    {   // hydrostatic fields: cubic bottom, log-linear lid
-        double* p = pstat(M);
-        p[at(0)] = p[at(3)] - 3.0 * p[at(2)] + 3.0 * p[at(1)];
-        const double x1 = p[at(iml - 1)], x2 = p[at(iml - 2)];
-        p[at(iml)] = (x1 > 0.0 && x2 > 0.0) ? (x1 * x1 / x2) : x1;
-        for (int f : {F_R_HUMID, F_R_DRY}) {
-            S* x = fp<S>(M, f);
-            x[at(0)] = S(C(x[at(3)]) - C(3) * C(x[at(2)]) + C(3) * C(x[at(1)]));
-            const C y1 = C(x[at(iml - 1)]), y2 = C(x[at(iml - 2)]);
-            x[at(iml)] = S((y1 > C(0) && y2 > C(0)) ? (y1 * y1 / y2) : y1);
-        }
+    auto at = [&](int i) { return (long long)i * si + q; };
+    // bcRadius (BC_Atm.h:556-745): cubic bottom / log-linear lid for the hydrostatic fields
+    double* p = pstat(M);
+    const double p1 = gld(p, at(1)), p2 = gld(p, at(2)), p3 = gld(p, at(3));
+    const double pl1 = gld(p, at(iml - 1)), pl2 = gld(p, at(iml - 2));
+    S* rh = fp<S>(M, F_R_HUMID);
+    S* rd = fp<S>(M, F_R_DRY);
+    C h[5], d[5];
+    h[0] = C(gld(rh, at(1))); h[1] = C(gld(rh, at(2))); h[2] = C(gld(rh, at(3)));
+    h[3] = C(gld(rh, at(iml - 1))); h[4] = C(gld(rh, at(iml - 2)));
+    d[0] = C(gld(rd, at(1))); d[1] = C(gld(rd, at(2))); d[2] = C(gld(rd, at(3)));
+    d[3] = C(gld(rd, at(iml - 1))); d[4] = C(gld(rd, at(iml - 2)));
+    // velocities: v, w copied at the bottom, zero at the lid, tapered x1/3, x2/3 below it
+    C vw[4][3];   // v, w, vn, wn at i = 1, iml-1, iml-2
+    const int fvw[4] = {F_V, F_W, F_VN, F_WN};
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const S* x = fp<S>(M, fvw[n]);
+        vw[n][0] = (n < 2) ? C(gld(x, at(1))) : C(0);
+        vw[n][1] = C(gld(x, at(iml - 1)));
+        vw[n][2] = C(gld(x, at(iml - 2)));
     }
-    for (int f : {F_U, F_V, F_W}) {
-        S* x = fp<S>(M, f);
-        x[at(0)] = x[at(1)];
-        x[at(iml)] = x[at(iml - 1)];
+    // scalars: cubic bottom, copied lid
+    constexpr int kRad[10] = {F_CLOUD, F_ICE, F_GR, F_CO2, F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC};
+    C sc[10][4];
+#pragma unroll
+    for (int n = 0; n < 10; n++) {
+        const S* x = fp<S>(M, kRad[n]);
+        sc[n][0] = C(gld(x, at(1))); sc[n][1] = C(gld(x, at(2))); sc[n][2] = C(gld(x, at(3)));
+        sc[n][3] = C(gld(x, at(iml - 1)));
     }
-    for (int f : {F_V, F_W, F_VN, F_WN}) fp<S>(M, f)[at(iml)] = S(0);
-    for (int f : {F_V, F_W, F_VN, F_WN}) { S* x = fp<S>(M, f); x[at(iml - 1)] = S(C(x[at(iml - 1)]) * C(1.0 / 3.0)); }
-    for (int f : {F_V, F_W, F_VN, F_WN}) { S* x = fp<S>(M, f); x[at(iml - 2)] = S(C(x[at(iml - 2)]) * C(2.0 / 3.0)); }
-    fp<S>(M, F_U)[at(iml)] = S(0); fp<S>(M, F_UN)[at(iml)] = S(0);
-    fp<S>(M, F_U)[at(0)] = S(0); fp<S>(M, F_UN)[at(0)] = S(0);
-    for (int f : {F_CLOUD, F_ICE, F_GR, F_CO2, F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC}) {
-        S* x = fp<S>(M, f);
-        x[at(0)] = S(C(x[at(3)]) - C(3) * C(x[at(2)]) + C(3) * C(x[at(1)]));
-        x[at(iml)] = x[at(iml - 1)];
-    }
-    // apply_wall_bc (k-omega SST)
     const double vs = M.vel_star[q];
-    fp<S>(M, F_TKE)[at(0)] = S(vs * vs / (K.sqrt_cnue * K.u_0 * K.u_0));
-    fp<S>(M, F_NUE)[at(0)] = S(0);
-    fp<S>(M, F_DIS)[at(0)] = S(K.om_wall);
+    // stores
+    gst(fp<S>(M, F_T), at(iml), S(M.t_top[q]));
+    gst(p, at(0), p3 - 3.0 * p2 + 3.0 * p1);
+    gst(p, at(iml), (pl1 > 0.0 && pl2 > 0.0) ? (pl1 * pl1 / pl2) : pl1);
+    gst(rh, at(0), S(h[2] - C(3) * h[1] + C(3) * h[0]));
+    gst(rh, at(iml), S((h[3] > C(0) && h[4] > C(0)) ? (h[3] * h[3] / h[4]) : h[3]));
+    gst(rd, at(0), S(d[2] - C(3) * d[1] + C(3) * d[0]));
+    gst(rd, at(iml), S((d[3] > C(0) && d[4] > C(0)) ? (d[3] * d[3] / d[4]) : d[3]));
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        S* x = fp<S>(M, fvw[n]);
+        if (n < 2) gst(x, at(0), S(vw[n][0]));
+        gst(x, at(iml), S(0));
+        gst(x, at(iml - 1), S(vw[n][1] * C(1.0 / 3.0)));
+        gst(x, at(iml - 2), S(vw[n][2] * C(2.0 / 3.0)));
+    }
+    for (int f : {F_U, F_UN}) { gst(fp<S>(M, f), at(0), S(0)); gst(fp<S>(M, f), at(iml), S(0)); }
+#pragma unroll
+    for (int n = 0; n < 10; n++) {
+        S* x = fp<S>(M, kRad[n]);
+        C bottom = sc[n][2] - C(3) * sc[n][1] + C(3) * sc[n][0];
+        // apply_wall_bc (TurbulenceAtm.h:109-147, k-omega SST)
+        if (kRad[n] == F_TKE) bottom = C(vs * vs / (K.sqrt_cnue * K.u_0 * K.u_0));
+        if (kRad[n] == F_NUE) bottom = C(0);
+        if (kRad[n] == F_DIS) bottom = C(K.om_wall);
+        gst(x, at(0), S(bottom));
+        gst(x, at(iml), S(sc[n][3]));
+    }
 }
 
 constexpr int kThetaF[18] = {F_T, F_P_STAT, F_R_HUMID, F_R_DRY, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2,
                              F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC};
 
+// bcTheta (BC_Atm.h:746-917): the pole rows copy their neighbours.
 template <typename S>
 __global__ void k_bc_theta(const MemberDesc* md) {
     const MemberDesc& M = md[blockIdx.z];
@@ -1229,19 +1269,44 @@ __global__ void k_bc_theta(const MemberDesc* md) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (long long)K.im * K.km) return;
     const int i = (int)(q / K.km), k = (int)(q % K.km);
-    const size_t a0 = ix3(K, i, 0, k), a1 = ix3(K, i, 1, k), b0 = ix3(K, i, K.jm - 1, k), b1 = ix3(K, i, K.jm - 2, k);
+    const long long a0 = ix3(K, i, 0, k), a1 = ix3(K, i, 1, k), b0 = ix3(K, i, K.jm - 1, k), b1 = ix3(K, i, K.jm - 2, k);
+    double pa = gld(pstat(M), a1), pb = gld(pstat(M), b1);
+    S va[17], vb[17];
+    int m = 0;
 #pragma unroll
     for (int n = 0; n < 18; n++) {
-        const int f = kThetaF[n];
-        if (f == F_P_STAT) { double* x = pstat(M); x[a0] = x[a1]; x[b0] = x[b1]; }
-        else { S* x = fp<S>(M, f); x[a0] = x[a1]; x[b0] = x[b1]; }
+        if (kThetaF[n] == F_P_STAT) continue;
+        const S* x = fp<S>(M, kThetaF[n]);
+        va[m] = gld(x, a1); vb[m] = gld(x, b1); m++;
+    }
+    gst(pstat(M), a0, pa); gst(pstat(M), b0, pb);
+    m = 0;
+#pragma unroll
+    for (int n = 0; n < 18; n++) {
+        if (kThetaF[n] == F_P_STAT) continue;
+        S* x = fp<S>(M, kThetaF[n]);
+        gst(x, a0, va[m]); gst(x, b0, vb[m]); m++;
     }
 }
 
-constexpr int kPhiAvg[12] = {F_T, F_P_STAT, F_R_HUMID, F_R_DRY, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2};
+constexpr int kPhiAvg[11] = {F_T, F_R_HUMID, F_R_DRY, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2};
 constexpr int kPhiExt[6] = {F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC};
-constexpr int kSeam[6] = {F_U, F_UN, F_V, F_VN, F_W, F_WN};
+constexpr int kSeamN[3] = {F_UN, F_VN, F_WN};
 
+// Two 1-2-1 passes across the periodic seam (BC_Atm.h:1026-1130): e = x[km-3], a = x[km-2],
+// b = x[0] = x[km-1], c = x[1], f = x[2].
+template <typename C> ATHAD_DEV void seam_passes(C e, C& a, C& b, C& c, C f) {
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+        const C a1 = a + C(0.25) * (e - C(2) * a + b);
+        const C b1 = b + C(0.25) * (a - C(2) * b + c);
+        const C c1 = c + C(0.25) * (b - C(2) * c + f);
+        a = a1; b = b1; c = c1;
+    }
+}
+
+// bcPhi (BC_Atm.h:918-1025): seam columns averaged (or copied for the turbulence fields),
+// then the seam passes on u, un, v, vn, w, wn.
 template <typename S>
 __global__ void k_bc_phi(const MemberDesc* md) {
     using C = S;
@@ -1249,36 +1314,57 @@ __global__ void k_bc_phi(const MemberDesc* md) {
     const Consts& K = M.K;
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (long long)K.im * K.jm) return;
-    const size_t r = (size_t)q * K.km;
+    const long long r = q * K.km;
     const int km = K.km;
+    double* ps = pstat(M);
+    const double pavg = (gld(ps, r + 1) + gld(ps, r + km - 2)) * 0.5;
+    C avg[11], sea[3][4];   // sea: x[km-3], x[km-2], x[1], x[2] of u, v, w
 #pragma unroll
-    for (int n = 0; n < 12; n++) {
-        const int f = kPhiAvg[n];
-        if (f == F_P_STAT) {
-            double* x = pstat(M) + r;
-            x[0] = x[km - 1] = (x[1] + x[km - 2]) * 0.5;
+    for (int n = 0; n < 11; n++) {
+        const S* x = fp<S>(M, kPhiAvg[n]);
+        const C x1 = C(gld(x, r + 1)), x2 = C(gld(x, r + km - 2));
+        avg[n] = (x1 + x2) * C(0.5);
+        if (n >= 3 && n <= 5) {
+            sea[n - 3][0] = C(gld(x, r + km - 3)); sea[n - 3][1] = x2; sea[n - 3][2] = x1;
+            sea[n - 3][3] = C(gld(x, r + 2));
+        }
+    }
+    C tx[6][2];
+#pragma unroll
+    for (int n = 0; n < 6; n++) {
+        const S* x = fp<S>(M, kPhiExt[n]);
+        tx[n][0] = C(gld(x, r + 1)); tx[n][1] = C(gld(x, r + km - 2));
+    }
+    C sn[3][5];   // un, vn, wn: x[km-3], x[km-2], x[0], x[1], x[2]
+#pragma unroll
+    for (int n = 0; n < 3; n++) {
+        const S* x = fp<S>(M, kSeamN[n]);
+        sn[n][0] = C(gld(x, r + km - 3)); sn[n][1] = C(gld(x, r + km - 2)); sn[n][2] = C(gld(x, r));
+        sn[n][3] = C(gld(x, r + 1)); sn[n][4] = C(gld(x, r + 2));
+    }
+    gst(ps, r, pavg); gst(ps, r + km - 1, pavg);
+#pragma unroll
+    for (int n = 0; n < 11; n++) {
+        S* x = fp<S>(M, kPhiAvg[n]);
+        if (n >= 3 && n <= 5) {   // u, v, w: averaged seam, then the seam passes
+            C a = sea[n - 3][1], b = avg[n], c = sea[n - 3][2];
+            seam_passes(sea[n - 3][0], a, b, c, sea[n - 3][3]);
+            gst(x, r + km - 2, S(a)); gst(x, r, S(b)); gst(x, r + km - 1, S(b)); gst(x, r + 1, S(c));
         } else {
-            S* x = fp<S>(M, f) + r;
-            const C v = (C(x[1]) + C(x[km - 2])) * C(0.5);
-            x[0] = S(v); x[km - 1] = S(v);
+            gst(x, r, S(avg[n])); gst(x, r + km - 1, S(avg[n]));
         }
     }
 #pragma unroll
     for (int n = 0; n < 6; n++) {
-        S* x = fp<S>(M, kPhiExt[n]) + r;
-        x[0] = x[1];
-        x[km - 1] = x[km - 2];
+        S* x = fp<S>(M, kPhiExt[n]);
+        gst(x, r, S(tx[n][0])); gst(x, r + km - 1, S(tx[n][1]));
     }
 #pragma unroll
-    for (int n = 0; n < 6; n++) {
-        S* x = fp<S>(M, kSeam[n]) + r;
-        for (int pass = 0; pass < 2; pass++) {
-            const C a = C(x[km - 2]), b = C(x[0]), c1 = C(x[1]), wa = C(x[km - 3]), ec = C(x[2]);
-            x[km - 2] = S(a + C(0.25) * (wa - C(2) * a + b));
-            const C nb = b + C(0.25) * (a - C(2) * b + c1);
-            x[0] = S(nb); x[km - 1] = S(nb);
-            x[1] = S(c1 + C(0.25) * (b - C(2) * c1 + ec));
-        }
+    for (int n = 0; n < 3; n++) {
+        S* x = fp<S>(M, kSeamN[n]);
+        C a = sn[n][1], b = sn[n][2], c = sn[n][3];
+        seam_passes(sn[n][0], a, b, c, sn[n][4]);
+        gst(x, r + km - 2, S(a)); gst(x, r, S(b)); gst(x, r + km - 1, S(b)); gst(x, r + 1, S(c));
     }
 }
 
