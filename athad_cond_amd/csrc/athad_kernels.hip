@@ -144,8 +144,11 @@ template <typename C> ATHAD_DEV C solid_clamp_n(int f, C v, C tke_max, C vel_max
     }
 }
 
+// minmod(a, b) = 0 if a b <= 0, else the argument of smaller magnitude, evaluated as
+// max(0, min(a, b)) + min(0, max(a, b)) (equal for finite arguments whose product does not
+// underflow to zero).
 template <typename C> ATHAD_DEV C minmod(C a, C b) {
-    return (a * b <= C(0)) ? C(0) : (fabs(a) < fabs(b) ? a : b);
+    return fmax(C(0), fmin(a, b)) + fmin(C(0), fmax(a, b));
 }
 
 // Reciprocal: the hardware estimate refined by Newton steps to within about 1 ulp. Used for
@@ -762,6 +765,7 @@ ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi
     N cpH, cpC, cpB;
     sat_species_cp(T, Mb, cpH, cpC, cpB);
     N Lv = sat_L_vap(T);
+    N T_prev = T, qv_prev = qvb, qc_prev = qcb, qi_prev = qib;
     for (int it = 1; it <= 20; it++) {
         const N CND = tmax(N(0), tmin(N(1), (T - t_00) * t_range_inv));
         const N DEP = N(1) - CND;
@@ -787,8 +791,14 @@ ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi
         const N Lc = Lv, Ld = Lv + N(3.337e5);
         const N gain = CND * (Lc * inv_cp) * (N(Ac) * Lc * inv_RT2) + DEP * (Ld * inv_cp) * (N(Ad) * Ld * inv_RT2);
         const N om = rcp_nr(N(1) + gain);
+        const N hyp_prev = qv_hyp;
         qv_hyp = qvb + om * (target - qvb);
         if (fabs(qvb / qv_hyp - N(1)) <= N(1.0e-6)) break;
+        // The iteration is a deterministic map of (T, qvb, qcb, qib, qv_hyp): once an
+        // iteration leaves that state unchanged, every later one does too and the break test
+        // above keeps failing, so the remaining iterations are skipped with identical results.
+        if (T == T_prev && qvb == qv_prev && qcb == qc_prev && qib == qi_prev && qv_hyp == hyp_prev) break;
+        T_prev = T; qv_prev = qvb; qc_prev = qcb; qi_prev = qib;
     }
     T_io = (double)T; qv_io = (double)qvb; qc_io = (double)qcb; qi_io = (double)qib;
 }
