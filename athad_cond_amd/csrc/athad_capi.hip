@@ -10,7 +10,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -99,6 +102,8 @@ struct athad_ctx {
     hipGraphExec_t graph = nullptr;
     int graph_phase = -1, graph_parity = -1;
     std::vector<double> staging;   // host staging for conversions
+    // fused damp_wiggles tile: shape index and levels per block (autotuned at creation)
+    int tile_shape = 0, tile_ich = 1 << 20;
 };
 
 namespace {
@@ -300,14 +305,15 @@ athad_status after_launch(athad_ctx* c) {
 // ---- the operators (all members, no cadence gate unless given) -----------------------
 const int kMoistSrc[4] = {F_T, F_ICE, F_C, F_CLOUD};
 
-void op_damp_moist(athad_ctx* c, int gate) {
+// damp_wiggles of t, ice, c, cloud plus the microphysics clamp, from scratch 6..9 (where the
+// saturation adjustment of the same moist block left them) into the fields; from_fields
+// first stages the current fields there (the standalone operator).
+void op_damp_moist(athad_ctx* c, int gate, bool from_fields) {
     Prof p(c, KID_DAMP);
     Batch& b = cur(c);
-    const int t0[4] = {NFIELD + 0, NFIELD + 1, NFIELD + 2, NFIELD + 3};
-    const int t1[4] = {NFIELD + 4, NFIELD + 5, NFIELD + 6, NFIELD + 7};
-    launch_damp(b, 0, kMoistSrc, t0, 4, 0, gate);
-    launch_damp(b, 1, t0, t1, 4, 0, gate);
-    launch_damp(b, 2, t1, kMoistSrc, 4, 1, gate);
+    const int tmp[4] = {NFIELD + 6, NFIELD + 7, NFIELD + 8, NFIELD + 9};
+    if (from_fields) launch_copy_job(b, kMoistSrc, tmp, 4);
+    launch_damp_tile(b, c->tile_shape, c->tile_ich, tmp, kMoistSrc, 4, 1, gate);
 }
 
 void op_turbulence(athad_ctx* c, int gate) {
@@ -319,24 +325,68 @@ void op_turbulence(athad_ctx* c, int gate) {
     }
     Prof p(c, KID_DAMP);
     const int t0[6] = {NFIELD + 0, NFIELD + 1, NFIELD + 2, NFIELD + 3, NFIELD + 4, NFIELD + 5};
-    const int t1[6] = {NFIELD + 6, NFIELD + 7, NFIELD + 8, NFIELD + 9, NFIELD + 10, NFIELD + 11};
     const int dst[6] = {F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC};
-    launch_damp(b, 0, t0, t1, 6, 0, gate);
-    launch_damp(b, 1, t1, t0, 6, 0, gate);
-    launch_damp(b, 2, t0, dst, 6, 0, gate);
+    launch_damp_tile(b, c->tile_shape, c->tile_ich, t0, dst, 6, 0, gate);
 }
 
 void op_moist_block(athad_ctx* c, int gate) {
     {
         Prof p(c, KID_SATADJ);
-        launch_satadj(cur(c), gate);
+        launch_satadj(cur(c), gate, 1);
     }
-    op_damp_moist(c, gate);
+    op_damp_moist(c, gate, false);
     {
         Prof p(c, KID_DENSITIES);
         launch_densities(cur(c), 0, 1, gate);
     }
     op_turbulence(c, gate);
+}
+
+// Chooses the damp tile shape and levels per block for this context's grids by timing
+// every candidate on scratch buffers (results do not depend on the choice). Cached per
+// grid shape, member count and precision.
+athad_status tune_damp(athad_ctx* c) {
+    struct Key { int prec, n, im, jm, km; bool operator<(const Key& o) const {
+        return std::tie(prec, n, im, jm, km) < std::tie(o.prec, o.n, o.im, o.jm, o.km); } };
+    static std::mutex mu;
+    static std::map<Key, std::pair<int, int>> cache;
+    Batch& b = cur(c);
+    const Key key{c->precision, c->nmem, b.max_im, b.max_jm, b.max_km};
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) { c->tile_shape = it->second.first; c->tile_ich = it->second.second; return ATHAD_OK; }
+    }
+    for (auto& m : c->mem)
+        for (int q = 0; q < NTMP; q++) HIPCHK(c, hipMemsetAsync(m.tmp[q], 0, m.n * elem_size(c), c->stream));
+    const int src[6] = {NFIELD + 0, NFIELD + 1, NFIELD + 2, NFIELD + 3, NFIELD + 4, NFIELD + 5};
+    const int dst[6] = {NFIELD + 6, NFIELD + 7, NFIELD + 8, NFIELD + 9, NFIELD + 10, NFIELD + 11};
+    const int ichs[3] = {1 << 20, 16, 8};
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    float best = 1e30f;
+    int bs = 0, bi = ichs[0];
+    for (int shape = 0; shape < damp_tile_shapes(); shape++) {
+        for (int ich : ichs) {
+            if (ich < (1 << 20) && ich >= b.max_im) continue;
+            launch_damp_tile(b, shape, ich, src, dst, 6, 0, 0);   // warm
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+            for (int r = 0; r < 3; r++) launch_damp_tile(b, shape, ich, src, dst, 6, 0, 0);
+            HIPCHK(c, hipEventRecord(e1, c->stream));
+            HIPCHK(c, hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; bs = shape; bi = ich; }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->tile_shape = bs;
+    c->tile_ich = bi;
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = {bs, bi};
+    return ATHAD_OK;
 }
 
 void op_bc(athad_ctx* c, bool solid) {
@@ -495,6 +545,7 @@ athad_status athad_create(int n_members, const athad_grid* grids, const athad_me
     c->batch.stream = c->stream;
     c->batch.fp32 = (precision == ATHAD_MIXED);
     if (upload_descs(c) != ATHAD_OK) { g_create_error = c->err; free_ctx(c); return ATHAD_ERR_DEVICE; }
+    if (tune_damp(c) != ATHAD_OK) { g_create_error = c->err; free_ctx(c); return ATHAD_ERR_DEVICE; }
     *out = c;
     return ATHAD_OK;
 }
@@ -561,6 +612,24 @@ athad_status athad_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
     if (hipMemGetInfo(&f, &t) != hipSuccess) return ATHAD_ERR_DEVICE;
     if (free_bytes) *free_bytes = (int64_t)f;
     if (total_bytes) *total_bytes = (int64_t)t;
+    return ATHAD_OK;
+}
+
+athad_status athad_damp_tile(athad_ctx* c, int* tj, int* tk, int* levels) {
+    if (!c) return ATHAD_ERR_ARG;
+    int a = 0, b = 0;
+    damp_tile_shape(c->tile_shape, &a, &b);
+    if (tj) *tj = a;
+    if (tk) *tk = b;
+    if (levels) *levels = std::min(c->tile_ich, c->batch.max_im);
+    return ATHAD_OK;
+}
+
+athad_status athad_set_damp_tile(athad_ctx* c, int shape, int levels) {
+    if (!c || shape < 0 || shape >= damp_tile_shapes() || levels < 1) return ATHAD_ERR_ARG;
+    c->tile_shape = shape;
+    c->tile_ich = levels;
+    if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
     return ATHAD_OK;
 }
 
@@ -725,12 +794,12 @@ athad_status athad_poisson_sweep(athad_ctx* c) {
 }
 athad_status athad_satadj(athad_ctx* c) {
     if (!c) return ATHAD_ERR_ARG;
-    { Prof p(c, KID_SATADJ); launch_satadj(cur(c), 0); }
+    { Prof p(c, KID_SATADJ); launch_satadj(cur(c), 0, 0); }
     return after_launch(c);
 }
 athad_status athad_damp_moist(athad_ctx* c) {
     if (!c) return ATHAD_ERR_ARG;
-    op_damp_moist(c, 0);
+    op_damp_moist(c, 0, true);
     return after_launch(c);
 }
 athad_status athad_densities(athad_ctx* c, int set_water, int with_evap) {

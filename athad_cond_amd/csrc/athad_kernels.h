@@ -26,9 +26,16 @@ void launch_level_mean(const Batch& b);
 void launch_rk4_jacobi(const Batch& b, bool fused);
 void launch_rk4_lex(const Batch& b, int plane);
 void launch_poisson(const Batch& b, int gate);
-void launch_satadj(const Batch& b, int gate);
+void launch_satadj(const Batch& b, int gate, int to_tmp);
 // pass 0,1,2 over the fields of one job
 void launch_damp(const Batch& b, int axis, const int* src, const int* dst, int nf, int moist_clamp, int gate);
+// Fused three-pass damp_wiggles over nf = 4 or 6 buffers with tile shape `shape` (one of
+// damp_tile_shapes()) and ich levels per block; src and dst must differ.
+int damp_tile_shapes();
+void damp_tile_shape(int shape, int* tj, int* tk);
+void launch_damp_tile(const Batch& b, int shape, int ich, const int* src, const int* dst, int nf, int moist_clamp,
+                      int gate);
+void launch_copy_job(const Batch& b, const int* src, const int* dst, int nf);
 void launch_densities(const Batch& b, int set_water, int evaporate, int gate);
 void launch_vel_star(const Batch& b, int gate);
 void launch_turb_sources(const Batch& b, int gate);

@@ -804,7 +804,7 @@ ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi
 }
 
 template <typename S>
-__global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate) {
+__global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate, int to_tmp) {
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const Consts& K = M.K;
@@ -893,7 +893,12 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate) 
     if (!finite_bits(scc)) scc = 0.0;
     else if (scc > 1.0e-3) scc = 1.0e-3;
     else if (scc < -1.0e-3) scc = -1.0e-3;
-    pt[id] = S(t); pc[id] = S(c); pcl[id] = S(cloud); pi[id] = S(ice); pg[id] = S(gr);
+    if (to_tmp) {   // scratch 6..9 = t, ice, c, cloud for the fused damping that follows
+        ((S*)M.tmp[6])[id] = S(t); ((S*)M.tmp[7])[id] = S(ice); ((S*)M.tmp[8])[id] = S(c); ((S*)M.tmp[9])[id] = S(cloud);
+    } else {
+        pt[id] = S(t); pc[id] = S(c); pcl[id] = S(cloud); pi[id] = S(ice);
+    }
+    pg[id] = S(gr);
     fp<S>(M, F_S_C_C)[id] = S(scc);
 }
 
@@ -954,6 +959,111 @@ __global__ __launch_bounds__(256) void k_damp_pass(const MemberDesc* md, DampJob
         out[2] = cv; out[3] = cld; out[1] = ic;
     }
     for (int q = 0; q < job.nf; q++) job_ptr<S>(M, job.dst[q])[id] = S(out[q]);
+}
+
+// The same three passes fused: a block owns a TJ x TK (latitude x longitude) tile and marches
+// it upward through levels [i0, i0 + ich). For each level it stages the tile plus a one-cell
+// halo of every field in LDS (j clamped, k periodic, exactly the neighbours the separate
+// passes read), forms the k pass and then the j pass at its own cell, and keeps the j-pass
+// results of levels i-1, i, i+1 in registers for the i pass. Reads src, writes dst (distinct
+// buffers: neighbouring tiles still read src). Results equal the separate passes bit for bit
+// for every tile shape, so the shape is free for an autotuner to choose.
+template <typename S, int TJ, int TK, int NF>
+__global__ __launch_bounds__(TJ * TK) void k_damp_tile(const MemberDesc* md, DampJob job, int ich, int gate) {
+    using C = S;
+    constexpr int W = TK + 2, SL = (TJ + 2) * W, NT = TJ * TK;
+    __shared__ C slab[NF * SL];
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const int ntk = (K.km + TK - 1) / TK, ntj = (K.jm + TJ - 1) / TJ, nch = (K.im + ich - 1) / ich;
+    const long long bb = xcd_block(blockIdx.x, gridDim.x, (long long)ntk * ntj * nch);
+    if (bb < 0) return;
+    int b = (int)bb;
+    const int tk = b % ntk; b /= ntk;
+    const int tj = b % ntj;
+    const int ch = b / ntj;
+    const int j0 = tj * TJ, k0 = tk * TK;
+    const int i0 = ch * ich, i1 = min(K.im, i0 + ich);
+    const int lj = threadIdx.x / TK, lk = threadIdx.x % TK;
+    const int j = j0 + lj, k = k0 + lk;
+    const bool own = (j < K.jm) && (k < K.km);
+    const S* src[NF];
+    S* dst[NF];
+#pragma unroll
+    for (int q = 0; q < NF; q++) { src[q] = job_ptr<S>(M, job.src[q]); dst[q] = job_ptr<S>(M, job.dst[q]); }
+    // j-pass result at level il (clamped to the grid) for this thread's cell, every field
+    auto level = [&](int il, C* out) {
+        const int ii = min(max(il, 0), K.im - 1);
+        __syncthreads();
+        for (int e = threadIdx.x; e < SL; e += NT) {
+            const int r = e / W, qk = e - r * W;
+            const int jj = min(max(j0 - 1 + r, 0), K.jm - 1);
+            const int kk = (k0 - 1 + qk + K.km) % K.km;
+            const long long g = ((long long)ii * K.jm + jj) * K.km + kk;
+#pragma unroll
+            for (int q = 0; q < NF; q++) slab[q * SL + e] = C(src[q][g]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+            const C* sb = slab + q * SL + lk;
+            C kp[3];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const C* row = sb + (lj + r) * W;
+                const C c0 = row[1];
+                kp[r] = c0 + C(0.25) * (row[0] - C(2) * c0 + row[2]);
+            }
+            out[q] = kp[1] + C(0.25) * (kp[0] - C(2) * kp[1] + kp[2]);
+        }
+    };
+    C jm_[NF], jc[NF], jp[NF];
+    level(i0 - 1, jm_);
+    level(i0, jc);
+    for (int i = i0; i < i1; i++) {
+        level(i + 1, jp);
+        C out[NF];
+#pragma unroll
+        for (int q = 0; q < NF; q++) out[q] = jc[q] + C(0.25) * (jm_[q] - C(2) * jc[q] + jp[q]);
+        if (NF == 4 && job.moist_clamp) {
+            // job order t, ice, c, cloud; the microphysics clamp and vapour ceiling
+            // (cAtmosphereModel.cpp:1307-1399)
+            const long long id = ((long long)i * K.jm + min(j, K.jm - 1)) * K.km + min(k, K.km - 1);
+            const C co2 = C(fp<S>(M, F_CO2)[id]);
+            C cv = out[2], cld = out[3], ic = out[1];
+            if (!finite_bits(cv)) cv = C(0);
+            if (!finite_bits(cld)) cld = C(0);
+            if (!finite_bits(ic)) ic = C(0);
+            if (cv < C(0)) cv = C(0);
+            if (cld < C(0)) cld = C(0);
+            if (ic < C(0)) ic = C(0);
+            const C cv_max = tmax(C(0), C(1) - co2);
+            if (cv > cv_max) cv = cv_max;
+            const C sum = cld + ic;
+            const C cap = C(K.cloud_cap);
+            if (sum > cap) { const C sc = cap / sum; cld *= sc; ic *= sc; }
+            if (cv < C(0)) cv = C(0);
+            if (cv > cv_max) cv = cv_max;
+            out[2] = cv; out[3] = cld; out[1] = ic;
+        }
+        if (own) {
+            const long long id = ((long long)i * K.jm + j) * K.km + k;
+#pragma unroll
+            for (int q = 0; q < NF; q++) dst[q][id] = S(out[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < NF; q++) { jm_[q] = jc[q]; jc[q] = jp[q]; }
+    }
+}
+
+// Copies up to six buffers (src[q] -> dst[q]).
+template <typename S>
+__global__ void k_copy_job(const MemberDesc* md, DampJob job) {
+    const MemberDesc& M = md[blockIdx.z];
+    const long long id = cell_id(M.K);
+    if (id < 0) return;
+    for (int q = 0; q < job.nf; q++) job_ptr<S>(M, job.dst[q])[id] = job_ptr<S>(M, job.src[q])[id];
 }
 
 // ------------------------------------------------------------------------------------
@@ -1673,8 +1783,8 @@ void launch_poisson(const Batch& b, int gate) {
     ATHAD_DISPATCH(k_p_bc, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate);
     ATHAD_DISPATCH(k_p_ceiling, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate);
 }
-void launch_satadj(const Batch& b, int gate) {
-    ATHAD_DISPATCH(k_satadj, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate);
+void launch_satadj(const Batch& b, int gate, int to_tmp) {
+    ATHAD_DISPATCH(k_satadj, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate, to_tmp);
 }
 void launch_damp(const Batch& b, int axis, const int* src, const int* dst, int nf, int moist_clamp, int gate) {
     DampJob job{};
@@ -1691,6 +1801,55 @@ void launch_damp(const Batch& b, int axis, const int* src, const int* dst, int n
         else if (axis == 1) hipLaunchKernelGGL((k_damp_pass<double, 1>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
         else hipLaunchKernelGGL((k_damp_pass<double, 2>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
     }
+}
+namespace {
+struct TileShape { int tj, tk; };
+constexpr TileShape kTiles[] = {{2, 64}, {4, 64}, {8, 64}, {1, 128}, {2, 128}, {4, 128}, {8, 32}, {16, 32}};
+constexpr int kNTiles = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
+
+template <typename S, int TJ, int TK>
+void damp_tile_launch(const Batch& b, const DampJob& job, int ich, int gate) {
+    const long long ntk = (b.max_km + TK - 1) / TK, ntj = (b.max_jm + TJ - 1) / TJ;
+    const long long nch = (b.max_im + ich - 1) / ich;
+    const unsigned nb = (unsigned)(((ntk * ntj * nch + 7) / 8) * 8);
+    if (job.nf == 4) hipLaunchKernelGGL((k_damp_tile<S, TJ, TK, 4>), dim3(nb, 1, b.nmem), dim3(TJ * TK), 0, b.stream, b.d_md, job, ich, gate);
+    else hipLaunchKernelGGL((k_damp_tile<S, TJ, TK, 6>), dim3(nb, 1, b.nmem), dim3(TJ * TK), 0, b.stream, b.d_md, job, ich, gate);
+}
+template <typename S>
+void damp_tile_dispatch(const Batch& b, int shape, const DampJob& job, int ich, int gate) {
+    switch (shape) {
+        case 0: damp_tile_launch<S, 2, 64>(b, job, ich, gate); break;
+        case 1: damp_tile_launch<S, 4, 64>(b, job, ich, gate); break;
+        case 2: damp_tile_launch<S, 8, 64>(b, job, ich, gate); break;
+        case 3: damp_tile_launch<S, 1, 128>(b, job, ich, gate); break;
+        case 4: damp_tile_launch<S, 2, 128>(b, job, ich, gate); break;
+        case 5: damp_tile_launch<S, 4, 128>(b, job, ich, gate); break;
+        case 6: damp_tile_launch<S, 8, 32>(b, job, ich, gate); break;
+        default: damp_tile_launch<S, 16, 32>(b, job, ich, gate); break;
+    }
+}
+}  // namespace
+
+int damp_tile_shapes() { return kNTiles; }
+void damp_tile_shape(int shape, int* tj, int* tk) {
+    const TileShape t = kTiles[(shape >= 0 && shape < kNTiles) ? shape : 0];
+    *tj = t.tj; *tk = t.tk;
+}
+void launch_damp_tile(const Batch& b, int shape, int ich, const int* src, const int* dst, int nf, int moist_clamp,
+                      int gate) {
+    DampJob job{};
+    job.nf = nf;
+    for (int q = 0; q < nf; q++) { job.src[q] = src[q]; job.dst[q] = dst[q]; }
+    job.moist_clamp = moist_clamp;
+    if (!b.d_md || ich < 1) return;
+    if (b.fp32) damp_tile_dispatch<float>(b, shape, job, ich, gate);
+    else damp_tile_dispatch<double>(b, shape, job, ich, gate);
+}
+void launch_copy_job(const Batch& b, const int* src, const int* dst, int nf) {
+    DampJob job{};
+    job.nf = nf;
+    for (int q = 0; q < nf; q++) { job.src[q] = src[q]; job.dst[q] = dst[q]; }
+    ATHAD_DISPATCH(k_copy_job, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, job);
 }
 void launch_densities(const Batch& b, int set_water, int evaporate, int gate) {
     ATHAD_DISPATCH(k_densities, dim3(cols_grid(b, 128), 1, b.nmem), dim3(128), 0, b.d_md, set_water, evaporate, gate);

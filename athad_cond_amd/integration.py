@@ -131,6 +131,15 @@ class Integration:
     def run(self, n: int, graph: bool = False):
         self._chk(L.lib().athad_run(self.h, n, int(graph)))
 
+    def damp_tile(self):
+        """(tj, tk, levels) of the fused damping tile chosen for this context."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._chk(L.lib().athad_damp_tile(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_damp_tile(self, shape: int, levels: int):
+        self._chk(L.lib().athad_set_damp_tile(self.h, shape, levels))
+
     def busy(self) -> bool:
         """True while work of this context is still in flight (non-blocking)."""
         st = L.lib().athad_query(self.h)

@@ -153,6 +153,13 @@ athad_status athad_run(athad_ctx* ctx, int n_steps, int use_graph);
 athad_status athad_query(athad_ctx* ctx);
 /* Free and total device memory of the current device (admission control). */
 athad_status athad_device_memory(int64_t* free_bytes, int64_t* total_bytes);
+/* The tile of the fused damp_wiggles kernel (tj latitudes x tk longitudes, `levels` radial
+ * levels per block) chosen for this context's grids by timing every candidate at creation.
+ * No counterpart in the reference (tiling is a device choice; results do not depend on it). */
+athad_status athad_damp_tile(athad_ctx* ctx, int* tj, int* tk, int* levels);
+/* Forces tile candidate `shape` (0 .. n-1, n fixed by the library; ATHAD_ERR_ARG beyond) and
+ * `levels` per block; for tests that check every candidate gives the same result. */
+athad_status athad_set_damp_tile(athad_ctx* ctx, int shape, int levels);
 /* Waits for the context's stream. */
 athad_status athad_sync(athad_ctx* ctx);
 

@@ -143,3 +143,30 @@ def test_lex_100_steps(oracle):
         integ.run(100)
         w = compare(integ, ref, PARITY_FIELDS, TOL, "C1 lex 100")
         print({k: f"{v:.2e}" for k, v in w.items()})
+
+
+def test_damp_tiles_give_identical_results(oracle):
+    """Every tile shape and level chunk of the fused damping gives the same bits, and the
+    result matches the oracle's three separate passes plus the microphysics clamp."""
+    print("case damp tiles", flush=True)
+    grid = (21, 91, 181)
+    ref = oracle.OracleState(*grid, seed=5)
+    ref.run(6, "lex")
+    state = ref.snapshot()
+    for name in ["damp_moist", "moist_clamp"]:
+        ref.op(name, "lex")
+    first = None
+    with A.Integration(grid, order="lex") as integ:
+        print("autotuned tile", integ.damp_tile(), flush=True)
+        for shape in range(8):
+            for levels in (1 << 20, 8, 3):
+                integ.load(state)
+                integ.set_iter(6, ref.total_iter)
+                integ.set_damp_tile(shape, levels)
+                integ.damp_moist()
+                out = {n: integ.get(n) for n in ["t", "ice", "c", "cloud"]}
+                if first is None:
+                    first = out
+                    compare(integ, ref, ["t", "ice", "c", "cloud"], 1e-12, "damp tile")
+                for n in out:
+                    assert np.array_equal(out[n], first[n]), (shape, levels, n)
