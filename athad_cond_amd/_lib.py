@@ -24,7 +24,7 @@ PROGNOSTIC = FIELDS[:11]
 
 FP64, MIXED = 0, 1
 OK, NOT_READY = 0, 6
-LEX, JACOBI = 0, 1
+LEX, JACOBI, SYMMETRIC = 0, 1, 2
 
 KERNEL_ID = {"rk4": 0, "rk4_plane": 1, "level_mean": 2, "poisson": 3, "satadj": 4, "damp": 5,
              "densities": 6, "turb": 7, "bc": 8, "solid": 9, "polar": 10, "radial": 11,

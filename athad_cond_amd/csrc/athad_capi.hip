@@ -165,8 +165,8 @@ void fill_consts(MemberHost& m, const Config& cfg, int order) {
     K.init_M_nw = M_nonwater_h(m.p.c_0, m.p.co2_0, C.M_bg);
     K.init_f_co2 = (C.q_CO2 + C.q_bg > 0.0) ? C.q_CO2 / (C.q_CO2 + C.q_bg) : m.p.co2_0 * cfg.co2_scale;
     K.co2_scale = cfg.co2_scale;
-    K.order = order;
-    K.sym = 0;
+    K.order = (order == ATHAD_ORDER_SYMMETRIC) ? ATHAD_ORDER_JACOBI : order;
+    K.sym = (order == ATHAD_ORDER_SYMMETRIC) ? 1 : 0;
 }
 
 // 1-D tables: rad, sin/cos, layer heights, ABL taper, SatAdj dt_dim, polar passes.
@@ -178,6 +178,10 @@ void build_tables(MemberHost& m, const Config& cfg, std::vector<double>& tab, st
     for (int l = 1; l < im; l++) rad[l] = rad[l - 1] + K.dr;
     the[0] = 0.0;
     for (int l = 1; l < jm; l++) the[l] = the[l - 1] + K.dthe;
+    // the symmetric form mirrors every latitude table about the equator row jm/2, where
+    // cos(theta) is exactly zero
+    const int jeq = (jm - 1) / 2;
+    auto src_row = [&](int j) { return (K.sym && j > jeq) ? jm - 1 - j : j; };
     const float h = (float)cfg.L_atm;
     std::vector<double> hgt(im);
     for (int i = 0; i < im; i++) hgt[i] = (double)(float)((std::exp(cfg.zeta * (rad[i] - rad[0])) - 1) * h);
@@ -186,10 +190,13 @@ void build_tables(MemberHost& m, const Config& cfg, std::vector<double>& tab, st
     tab.assign(m.o_step + im, 0.0);
     for (int i = 0; i < im; i++) tab[m.o_rad + i] = rad[i];
     for (int j = 0; j < jm; j++) {
-        double s = std::sin(the[j]);
+        const int jr = src_row(j);
+        double s = std::sin(the[jr]);
         tab[m.o_sinr + j] = s;
         tab[m.o_sint + j] = (s < 0.55) ? 0.55 : s;
-        tab[m.o_cost + j] = std::cos(the[j]);
+        double cth = std::cos(the[jr]);
+        if (K.sym) cth = (j == jeq) ? 0.0 : (j > jeq ? -cth : cth);
+        tab[m.o_cost + j] = cth;
     }
     for (int i = 0; i < im; i++) tab[m.o_hgt + i] = hgt[i];
     // clamp_nue taper (TurbulenceAtm.h:271-297)
@@ -209,7 +216,7 @@ void build_tables(MemberHost& m, const Config& cfg, std::vector<double>& tab, st
     passes.assign(jm, 0);
     const double sin_ref = std::cos(30.0 * (M_PI / 180.0));
     for (int j = 0; j < jm; j++) {
-        const double sj = std::max(std::sin(the[j]), 1.0e-6);
+        const double sj = std::max(std::sin(the[src_row(j)]), 1.0e-6);
         if (sj >= sin_ref) continue;
         const double ratio = sin_ref / sj;
         int p = (int)std::lround(3.0 * (ratio * ratio - 1.0));
@@ -482,7 +489,10 @@ athad_status athad_create(int n_members, const athad_grid* grids, const athad_me
     if (!out || !grids || n_members <= 0) { g_create_error = "bad arguments"; return ATHAD_ERR_ARG; }
     *out = nullptr;
     if (precision != ATHAD_FP64 && precision != ATHAD_MIXED) { g_create_error = "bad precision"; return ATHAD_ERR_ARG; }
-    if (order != ATHAD_ORDER_LEX && order != ATHAD_ORDER_JACOBI) { g_create_error = "bad order"; return ATHAD_ERR_ARG; }
+    if (order != ATHAD_ORDER_LEX && order != ATHAD_ORDER_JACOBI && order != ATHAD_ORDER_SYMMETRIC) {
+        g_create_error = "bad order";
+        return ATHAD_ERR_ARG;
+    }
     for (int q = 0; q < n_members; q++) {
         const athad_grid& g = grids[q];
         if (g.im < 6 || g.jm < 6 || g.km < 6 || g.im > 1024 || g.jm > 8192 || g.km > 16384 || (g.jm % 2) == 0 ||

@@ -937,7 +937,9 @@ __global__ __launch_bounds__(256) void k_damp_pass(const MemberDesc* md, DampJob
     for (int q = 0; q < job.nf; q++) {
         const S* s = job_ptr<S>(M, job.src[q]);
         const C c0 = C(s[id]);
-        out[q] = c0 + C(0.25) * (C(s[a]) - C(2) * c0 + C(s[b]));
+        // the symmetric form adds the two j neighbours first so mirrored cells round alike
+        out[q] = (AXIS == 1 && K.sym) ? c0 + C(0.25) * ((C(s[a]) + C(s[b])) - C(2) * c0)
+                                      : c0 + C(0.25) * (C(s[a]) - C(2) * c0 + C(s[b]));
     }
     if (AXIS == 2 && job.moist_clamp) {
         // job order t, ice, c, cloud; co2 from the field
@@ -1015,7 +1017,8 @@ __global__ __launch_bounds__(TJ * TK) void k_damp_tile(const MemberDesc* md, Dam
                 const C c0 = row[1];
                 kp[r] = c0 + C(0.25) * (row[0] - C(2) * c0 + row[2]);
             }
-            out[q] = kp[1] + C(0.25) * (kp[0] - C(2) * kp[1] + kp[2]);
+            out[q] = K.sym ? kp[1] + C(0.25) * ((kp[0] + kp[2]) - C(2) * kp[1])
+                           : kp[1] + C(0.25) * (kp[0] - C(2) * kp[1] + kp[2]);
         }
     };
     C jm_[NF], jc[NF], jp[NF];
@@ -1220,7 +1223,11 @@ __global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* md, int 
     const C tk_km1 = (k - 1 >= 1) ? tke_vis(id - 1, j, k - 1) : C(ptk[id - 1]);
     const C ds_km1 = (k - 1 >= 1) ? dis_vis(id - 1) : C(pds[id - 1]);
     const C tk_ip1 = C(ptk[id + si]), ds_ip1 = C(pds[id + si]);
-    const C tk_jp1 = C(ptk[id + sj]), ds_jp1 = C(pds[id + sj]);
+    // the symmetric form floors both latitude neighbours (a serial sweep's visit order is
+    // not mirror symmetric)
+    const bool fl_jp = K.sym && (j + 1 <= K.jm - 2);
+    const C tk_jp1 = fl_jp ? tke_vis(id + sj, j + 1, k) : C(ptk[id + sj]);
+    const C ds_jp1 = fl_jp ? dis_vis(id + sj) : C(pds[id + sj]);
     const C tk_kp1 = C(ptk[id + 1]), ds_kp1 = C(pds[id + 1]);
 
     const double rm = M.rad[i];
@@ -1629,7 +1636,7 @@ __global__ __launch_bounds__(128) void k_init_column(const MemberDesc* md) {
     S *t = fp<S>(M, F_T), *rh = fp<S>(M, F_R_HUMID), *rd = fp<S>(M, F_R_DRY);
     double* ps = pstat(M);
     const double d_j_half = 0.5 * (K.jm - 1);
-    const double ratio = (double)j / d_j_half;
+    const double ratio = (double)(K.sym ? min(j, K.jm - 1 - j) : j) / d_j_half;
     const double T_s = (t_pole_K - t_eq_K) * (ratio * ratio - 2.0 * ratio) + t_pole_K;
     const double tu = (T_s * (1.0 / K.t_0)) * K.t_0;
     double T_prev = tmax(100.0, tu);

@@ -29,7 +29,7 @@ class Integration:
             params = list(params)
             parr = (L.MemberParams * n)(*params)
         self.precision = {"fp64": L.FP64, "mixed": L.MIXED}[precision]
-        self.order = {"lex": L.LEX, "jacobi": L.JACOBI}[order]
+        self.order = {"lex": L.LEX, "jacobi": L.JACOBI, "symmetric": L.SYMMETRIC}[order]
         h = ctypes.c_void_p()
         st = L.lib().athad_create(n, garr, parr, self.precision, self.order, ctypes.byref(h))
         L.check(st, None)

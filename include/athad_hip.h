@@ -41,6 +41,12 @@ typedef int athad_status;
  * reproducible choices. */
 #define ATHAD_ORDER_LEX     0   /* the reference's sweep at OMP_NUM_THREADS=1 (parity form) */
 #define ATHAD_ORDER_JACOBI  1   /* start-of-step neighbourhood (throughput form) */
+#define ATHAD_ORDER_SYMMETRIC 2 /* Jacobi order with north-south mirror symmetry kept to the last
+                                   bit: mirrored latitude tables (cos theta = 0 at the equator),
+                                   latitude neighbours summed before weighting, SST wall distance
+                                   from the nearer pole, turbulence floors on both latitude
+                                   neighbours. Not in the reference, whose sweep order and wall
+                                   distance break the mirror at rounding level and near the poles. */
 
 /* Field ids: the cAtmosphereModel Array members of the scoped step
  * (atmosphere/cAtmosphereModel.h:535-662). */
