@@ -1542,27 +1542,37 @@ __global__ __launch_bounds__(256) void k_polar(const MemberDesc* md, int fbits) 
     const int j = row % K.jm;
     const int np = M.passes[j];
     if (np == 0) return;
+    // the selected fields of u, v, w pass together: 2 buffers x 3 fields x km in LDS
     extern __shared__ unsigned char smem_raw[];
-    C* buf0 = (C*)smem_raw;
-    C* buf1 = buf0 + K.km;
     const int km = K.km;
-    for (int f = F_U; f <= F_W; f++) {
-        if (!(fbits & (1 << (f - F_U)))) continue;
-        S* x = fp<S>(M, f) + (size_t)row * km;
-        C* a = buf0;
-        C* b = buf1;
-        for (int k = threadIdx.x; k < km; k += blockDim.x) a[k] = C(x[k]);
-        __syncthreads();
-        for (int p = 0; p < np; p++) {
-            for (int k = threadIdx.x; k < km; k += blockDim.x) {
-                const int km1 = (k == 0) ? km - 1 : k - 1, kp1 = (k == km - 1) ? 0 : k + 1;
-                b[k] = a[k] + C(0.25) * (a[km1] - C(2) * a[k] + a[kp1]);
+    C* a = (C*)smem_raw;
+    C* b = a + 3 * km;
+    bool on[3];
+#pragma unroll
+    for (int f = 0; f < 3; f++) on[f] = (fbits >> f) & 1;
+    for (int k = threadIdx.x; k < km; k += blockDim.x) {
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+            if (on[f]) a[f * km + k] = C(fp<S>(M, F_U + f)[(size_t)row * km + k]);
+    }
+    __syncthreads();
+    for (int p = 0; p < np; p++) {
+        for (int k = threadIdx.x; k < km; k += blockDim.x) {
+            const int km1 = (k == 0) ? km - 1 : k - 1, kp1 = (k == km - 1) ? 0 : k + 1;
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                if (!on[f]) continue;
+                const C* r = a + f * km;
+                b[f * km + k] = r[k] + C(0.25) * (r[km1] - C(2) * r[k] + r[kp1]);
             }
-            __syncthreads();
-            C* t = a; a = b; b = t;
         }
-        for (int k = threadIdx.x; k < km; k += blockDim.x) x[k] = S(a[k]);
         __syncthreads();
+        C* t = a; a = b; b = t;
+    }
+    for (int k = threadIdx.x; k < km; k += blockDim.x) {
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+            if (on[f]) fp<S>(M, F_U + f)[(size_t)row * km + k] = S(a[f * km + k]);
     }
 }
 
@@ -1607,6 +1617,67 @@ __global__ __launch_bounds__(64) void k_radial_store(const MemberDesc* md, int s
             if (store) xn[(size_t)i * si] = v;
         }
     }
+}
+
+// The same filters on a tile: a block holds RK consecutive longitudes x all levels of one
+// latitude row of u, v, w in LDS. The reference sweeps each pass in place but keeps the old
+// values of levels i-1, i-2, so every level of a pass depends only on the previous pass and
+// all levels are computed at once (two LDS buffers). Results equal the column kernel's.
+template <typename S, int RK>
+__global__ __launch_bounds__(256) void k_radial_tile(const MemberDesc* md, int store) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const int nkc = (K.km + RK - 1) / RK;
+    if ((int)blockIdx.x >= K.jm * nkc) return;
+    const int j = blockIdx.x / nkc, k0 = (blockIdx.x % nkc) * RK;
+    const int im = K.im, n = 3 * im * RK;
+    extern __shared__ unsigned char smem_raw[];
+    C* a = (C*)smem_raw;
+    C* b = a + n;
+    const size_t si = (size_t)K.jm * K.km;
+    const size_t base = (size_t)j * K.km + k0;
+    // element (f, i, kk) at (f * im + i) * RK + kk; thread t owns column kk = t % RK and
+    // levels i = t / RK, + NR, + 2 NR, ...
+    constexpr int NR = 256 / RK;
+    const int kk = threadIdx.x % RK, r0 = threadIdx.x / RK;
+    const bool in = (k0 + kk < K.km);
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+        for (int i = r0; i < im; i += NR)
+            if (in) a[(f * im + i) * RK + kk] = C(fp<S>(M, F_U + f)[base + (size_t)i * si + kk]);
+    __syncthreads();
+    for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            const C* col = a + (size_t)f * im * RK + kk;
+            for (int i = r0; i < im; i += NR) {
+                const C c0 = col[i * RK];
+                C nv = c0;
+                if (i >= 1 && i <= im - 2) {
+                    const C o1 = col[(i - 1) * RK];
+                    if (f != 0 && i >= 2 && i + 2 <= im - 1) {
+                        const C d4 = col[(i + 2) * RK] - C(4) * col[(i + 1) * RK] + C(6) * c0 - C(4) * o1 + col[(i - 2) * RK];
+                        nv = c0 - C(1.0 / 16.0) * d4;
+                    } else {
+                        nv = c0 + C(0.25) * (o1 - C(2) * c0 + col[(i + 1) * RK]);
+                    }
+                }
+                b[(f * im + i) * RK + kk] = nv;
+            }
+        }
+        __syncthreads();
+        C* t = a; a = b; b = t;
+    }
+    if (!in) return;
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+        for (int i = r0; i < im; i += NR) {
+            const S v = S(a[(f * im + i) * RK + kk]);
+            const size_t g = base + (size_t)i * si + kk;
+            fp<S>(M, F_U + f)[g] = v;
+            if (store) fp<S>(M, F_UN + f)[g] = v;
+        }
 }
 
 // UtilsAtm::storeIntermediateData3D (UtilsAtm.h:25-49): xn <- x for the fields selected by mask.
@@ -1878,10 +1949,18 @@ void launch_solid_clamp(const Batch& b) {
     ATHAD_DISPATCH(k_solid_clamp, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
 }
 void launch_polar(const Batch& b) {
-    const size_t shm = 2 * (size_t)b.max_km * (b.fp32 ? 4 : 8);
+    const size_t shm = 6 * (size_t)b.max_km * (b.fp32 ? 4 : 8);
     ATHAD_DISPATCH(k_polar, dim3((unsigned)(b.max_im * b.max_jm), 1, b.nmem), dim3(256), shm, b.d_md, 7);
 }
 void launch_radial(const Batch& b, int store) {
+    constexpr int RK = 16;
+    const size_t tile = 2 * 3 * (size_t)b.max_im * RK * (b.fp32 ? 4 : 8);
+    if (tile <= 64 * 1024) {
+        const unsigned nb = (unsigned)(b.max_jm * ((b.max_km + RK - 1) / RK));
+        if (b.fp32) hipLaunchKernelGGL((k_radial_tile<float, RK>), dim3(nb, 1, b.nmem), dim3(256), tile, b.stream, b.d_md, store);
+        else hipLaunchKernelGGL((k_radial_tile<double, RK>), dim3(nb, 1, b.nmem), dim3(256), tile, b.stream, b.d_md, store);
+        return;
+    }
     const size_t shm = 64 * (size_t)b.max_im * (b.fp32 ? 4 : 8);
     ATHAD_DISPATCH(k_radial_store, dim3(cols_grid(b, 64), 1, b.nmem), dim3(64), shm, b.d_md, store);
 }
