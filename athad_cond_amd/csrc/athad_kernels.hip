@@ -166,6 +166,14 @@ ATHAD_DEV float rcp_nr(float x) {
     return fmaf(r, e, r);
 }
 
+// tanh(x) for x >= 0 with the tiny-argument case short-cut: below 1e-9 (double) or 1e-4
+// (float) the correctly rounded tanh(x) is x itself (x^3/3 is below half an ulp of x).
+template <typename C> ATHAD_DEV C tanh_pos(C x) {
+    C r = x;
+    if (x >= C(sizeof(C) == 8 ? 1.0e-9 : 1.0e-4)) r = tanh(x);   // a branch: skipped when no lane needs it
+    return r;
+}
+
 // Global-memory access through address space 1 (pointers read from a MemberDesc are
 // otherwise generic and compile to flat instructions).
 template <typename T> ATHAD_DEV T gld(const T* p, long long i) {
@@ -436,7 +444,7 @@ ATHAD_DEV void rk4_turb(const MemberDesc& M, const S* const* X, long long id, in
         const C CD_kw = tmax(C(2) * sig_w2 * inv_dis * cross, C(1.0e-20));
         const C arg1 = tmin(tmax(sqrt(tke) * inv_dis * K2, K1 * inv_dis), tke * K3 * rcp_nr(CD_kw));
         const C a2 = arg1 * arg1;
-        const C F1 = tanh(a2 * a2);
+        const C F1 = tanh_pos(a2 * a2);
         const C dif_tke = inv_re + nue * rcp_nr(F1 * sig_k1 + (C(1) - F1) * sig_k2);
         const C dif_dis = inv_re + nue * rcp_nr(F1 * sig_w1 + (C(1) - F1) * sig_w2);
         const C tke_s = tmax(tke, C(1.0e-10));
@@ -1280,8 +1288,8 @@ __global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* md, int 
                         C(4) * sig_w2 * tk / (CD_kw * y_star * y_star));
     const C arg2 = tmax(C(2) * sqrt(tk) / (bet_star * dis_safe * y_star), C(500) * nu_nd / (y_star * y_star * dis_safe));
     const C a1sq = arg1 * arg1;
-    const C F1 = tanh(a1sq * a1sq);
-    const C F2 = tanh(arg2 * arg2);
+    const C F1 = tanh_pos(a1sq * a1sq);
+    const C F2 = tanh_pos(arg2 * arg2);
     C nue = a1 * tk / tmax(a1 * dis_safe, Omega * F2);
     const C tke_safe = tmax(tk, C(1.0e-20));
     const C P_k = tmin(prod, C(20) * bet_star * tke_safe * dis_safe);
@@ -1388,40 +1396,38 @@ __global__ void k_bc_radial(const MemberDesc* md) {
 constexpr int kThetaF[18] = {F_T, F_P_STAT, F_R_HUMID, F_R_DRY, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2,
                              F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC};
 
-// bcTheta (BC_Atm.h:746-917): the pole rows copy their neighbours.
+// bcTheta (BC_Atm.h:746-917): the pole rows copy their neighbours. One thread per
+// (field, level, longitude).
 template <typename S>
 __global__ void k_bc_theta(const MemberDesc* md) {
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
+    const long long nik = (long long)K.im * K.km;
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (long long)K.im * K.km) return;
-    const int i = (int)(q / K.km), k = (int)(q % K.km);
+    if (q >= 18 * nik) return;
+    const int n = (int)(q / nik);
+    const long long r = q - n * nik;
+    const int i = (int)(r / K.km), k = (int)(r - (long long)i * K.km);
     const long long a0 = ix3(K, i, 0, k), a1 = ix3(K, i, 1, k), b0 = ix3(K, i, K.jm - 1, k), b1 = ix3(K, i, K.jm - 2, k);
-    double pa = gld(pstat(M), a1), pb = gld(pstat(M), b1);
-    S va[17], vb[17];
-    int m = 0;
-#pragma unroll
-    for (int n = 0; n < 18; n++) {
-        if (kThetaF[n] == F_P_STAT) continue;
-        const S* x = fp<S>(M, kThetaF[n]);
-        va[m] = gld(x, a1); vb[m] = gld(x, b1); m++;
-    }
-    gst(pstat(M), a0, pa); gst(pstat(M), b0, pb);
-    m = 0;
-#pragma unroll
-    for (int n = 0; n < 18; n++) {
-        if (kThetaF[n] == F_P_STAT) continue;
+    if (kThetaF[n] == F_P_STAT) {
+        double* x = pstat(M);
+        const double va = gld(x, a1), vb = gld(x, b1);
+        gst(x, a0, va); gst(x, b0, vb);
+    } else {
         S* x = fp<S>(M, kThetaF[n]);
-        gst(x, a0, va[m]); gst(x, b0, vb[m]); m++;
+        const S va = gld(x, a1), vb = gld(x, b1);
+        gst(x, a0, va); gst(x, b0, vb);
     }
 }
 
-constexpr int kPhiAvg[11] = {F_T, F_R_HUMID, F_R_DRY, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2};
-constexpr int kPhiExt[6] = {F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC};
-constexpr int kSeamN[3] = {F_UN, F_VN, F_WN};
+// bcPhi (BC_Atm.h:918-1025): seam columns averaged (t, p_stat, r_humid, r_dry, u, v, w, c,
+// cloud, ice, gr, co2) or copied (the six turbulence fields), then two 1-2-1 passes across the
+// periodic seam on u, un, v, vn, w, wn (BC_Atm.h:1026-1130). One thread per (field, row); u, v,
+// w do their averaging and seam passes in the same thread.
+constexpr int kPhiF[21] = {F_T, F_P_STAT, F_R_HUMID, F_R_DRY, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2,
+                           F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC, F_UN, F_VN, F_WN};
 
-// Two 1-2-1 passes across the periodic seam (BC_Atm.h:1026-1130): e = x[km-3], a = x[km-2],
-// b = x[0] = x[km-1], c = x[1], f = x[2].
+// e = x[km-3], a = x[km-2], b = x[0] = x[km-1], c = x[1], f = x[2]
 template <typename C> ATHAD_DEV void seam_passes(C e, C& a, C& b, C& c, C f) {
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
@@ -1432,65 +1438,44 @@ template <typename C> ATHAD_DEV void seam_passes(C e, C& a, C& b, C& c, C f) {
     }
 }
 
-// bcPhi (BC_Atm.h:918-1025): seam columns averaged (or copied for the turbulence fields),
-// then the seam passes on u, un, v, vn, w, wn.
 template <typename S>
 __global__ void k_bc_phi(const MemberDesc* md) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
+    const long long nrow = (long long)K.im * K.jm;
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (long long)K.im * K.jm) return;
-    const long long r = q * K.km;
+    if (q >= 21 * nrow) return;
+    const int n = (int)(q / nrow);
+    const long long r = (q - n * nrow) * K.km;
     const int km = K.km;
-    double* ps = pstat(M);
-    const double pavg = (gld(ps, r + 1) + gld(ps, r + km - 2)) * 0.5;
-    C avg[11], sea[3][4];   // sea: x[km-3], x[km-2], x[1], x[2] of u, v, w
-#pragma unroll
-    for (int n = 0; n < 11; n++) {
-        const S* x = fp<S>(M, kPhiAvg[n]);
+    const int f = kPhiF[n];
+    if (f == F_P_STAT) {
+        double* ps = pstat(M);
+        const double v = (gld(ps, r + 1) + gld(ps, r + km - 2)) * 0.5;
+        gst(ps, r, v); gst(ps, r + km - 1, v);
+        return;
+    }
+    S* x = fp<S>(M, f);
+    if (n < 12) {   // averaged seam
         const C x1 = C(gld(x, r + 1)), x2 = C(gld(x, r + km - 2));
-        avg[n] = (x1 + x2) * C(0.5);
-        if (n >= 3 && n <= 5) {
-            sea[n - 3][0] = C(gld(x, r + km - 3)); sea[n - 3][1] = x2; sea[n - 3][2] = x1;
-            sea[n - 3][3] = C(gld(x, r + 2));
-        }
-    }
-    C tx[6][2];
-#pragma unroll
-    for (int n = 0; n < 6; n++) {
-        const S* x = fp<S>(M, kPhiExt[n]);
-        tx[n][0] = C(gld(x, r + 1)); tx[n][1] = C(gld(x, r + km - 2));
-    }
-    C sn[3][5];   // un, vn, wn: x[km-3], x[km-2], x[0], x[1], x[2]
-#pragma unroll
-    for (int n = 0; n < 3; n++) {
-        const S* x = fp<S>(M, kSeamN[n]);
-        sn[n][0] = C(gld(x, r + km - 3)); sn[n][1] = C(gld(x, r + km - 2)); sn[n][2] = C(gld(x, r));
-        sn[n][3] = C(gld(x, r + 1)); sn[n][4] = C(gld(x, r + 2));
-    }
-    gst(ps, r, pavg); gst(ps, r + km - 1, pavg);
-#pragma unroll
-    for (int n = 0; n < 11; n++) {
-        S* x = fp<S>(M, kPhiAvg[n]);
-        if (n >= 3 && n <= 5) {   // u, v, w: averaged seam, then the seam passes
-            C a = sea[n - 3][1], b = avg[n], c = sea[n - 3][2];
-            seam_passes(sea[n - 3][0], a, b, c, sea[n - 3][3]);
+        const C avg = (x1 + x2) * C(0.5);
+        if (f == F_U || f == F_V || f == F_W) {
+            const C e = C(gld(x, r + km - 3)), ff = C(gld(x, r + 2));
+            C a = x2, b = avg, c = x1;
+            seam_passes(e, a, b, c, ff);
             gst(x, r + km - 2, S(a)); gst(x, r, S(b)); gst(x, r + km - 1, S(b)); gst(x, r + 1, S(c));
         } else {
-            gst(x, r, S(avg[n])); gst(x, r + km - 1, S(avg[n]));
+            gst(x, r, S(avg)); gst(x, r + km - 1, S(avg));
         }
-    }
-#pragma unroll
-    for (int n = 0; n < 6; n++) {
-        S* x = fp<S>(M, kPhiExt[n]);
-        gst(x, r, S(tx[n][0])); gst(x, r + km - 1, S(tx[n][1]));
-    }
-#pragma unroll
-    for (int n = 0; n < 3; n++) {
-        S* x = fp<S>(M, kSeamN[n]);
-        C a = sn[n][1], b = sn[n][2], c = sn[n][3];
-        seam_passes(sn[n][0], a, b, c, sn[n][4]);
+    } else if (n < 18) {   // turbulence fields: copied
+        const S v0 = gld(x, r + 1), v1 = gld(x, r + km - 2);
+        gst(x, r, v0); gst(x, r + km - 1, v1);
+    } else {   // un, vn, wn: seam passes only
+        const C e = C(gld(x, r + km - 3)), a0 = C(gld(x, r + km - 2)), b0 = C(gld(x, r)), c0 = C(gld(x, r + 1)),
+                ff = C(gld(x, r + 2));
+        C a = a0, b = b0, c = c0;
+        seam_passes(e, a, b, c, ff);
         gst(x, r + km - 2, S(a)); gst(x, r, S(b)); gst(x, r + km - 1, S(b)); gst(x, r + 1, S(c));
     }
 }
@@ -1940,9 +1925,9 @@ void launch_turb_sources(const Batch& b, int gate) {
 }
 void launch_bc(const Batch& b) {
     ATHAD_DISPATCH(k_bc_radial, dim3(cols_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
-    const unsigned nt = (unsigned)(((long long)b.max_im * b.max_km + 255) / 256);
+    const unsigned nt = (unsigned)((18LL * b.max_im * b.max_km + 255) / 256);
     ATHAD_DISPATCH(k_bc_theta, dim3(nt, 1, b.nmem), dim3(256), 0, b.d_md);
-    const unsigned np = (unsigned)(((long long)b.max_im * b.max_jm + 255) / 256);
+    const unsigned np = (unsigned)((21LL * b.max_im * b.max_jm + 255) / 256);
     ATHAD_DISPATCH(k_bc_phi, dim3(np, 1, b.nmem), dim3(256), 0, b.d_md);
 }
 void launch_solid_clamp(const Batch& b) {
