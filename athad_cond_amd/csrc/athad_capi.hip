@@ -90,6 +90,10 @@ struct athad_ctx {
     std::vector<MemberDesc> h_md[2];
     int parity = 0;
     hipStream_t stream = nullptr;
+    // second stream: the turbulence update runs beside the moist thermodynamics (disjoint
+    // fields), forked and joined with events (also inside a captured graph)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     Batch batch;
     std::string err;
     // profiling
@@ -266,6 +270,10 @@ void free_ctx(athad_ctx* c) {
     for (int p = 0; p < 2; p++) if (c->d_md[p]) hipFree(c->d_md[p]);
     for (int k = 0; k < KID_COUNT; k++)
         for (auto& e : c->prof_ev[k]) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->stream2) hipStreamDestroy(c->stream2);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -277,15 +285,15 @@ Batch& cur(athad_ctx* c) {
 
 // Kernel timing bracket.
 struct Prof {
-    athad_ctx* c; int kid; hipEvent_t e1 = nullptr, e2 = nullptr;
-    Prof(athad_ctx* cc, int k) : c(cc), kid(k) {
+    athad_ctx* c; int kid; hipStream_t s; hipEvent_t e1 = nullptr, e2 = nullptr;
+    Prof(athad_ctx* cc, int k, hipStream_t st = nullptr) : c(cc), kid(k), s(st ? st : cc->stream) {
         if (!c->capturing && c->prof_on[kid]) {
             hipEventCreate(&e1); hipEventCreate(&e2);
-            hipEventRecord(e1, c->stream);
+            hipEventRecord(e1, s);
         }
     }
     ~Prof() {
-        if (e1) { hipEventRecord(e2, c->stream); c->prof_ev[kid].push_back({e1, e2}); }
+        if (e1) { hipEventRecord(e2, s); c->prof_ev[kid].push_back({e1, e2}); }
     }
 };
 
@@ -323,20 +331,31 @@ void op_damp_moist(athad_ctx* c, int gate, bool from_fields) {
     launch_damp_tile(b, c->tile_shape, c->tile_ich, tmp, kMoistSrc, 4, 1, gate);
 }
 
-void op_turbulence(athad_ctx* c, int gate) {
-    Batch& b = cur(c);
+// TurbulenceAtm::run and the damping of its six fields, on stream st (the context's own
+// stream when null).
+void op_turbulence(athad_ctx* c, int gate, hipStream_t st = nullptr) {
+    Batch b = cur(c);
+    if (st) b.stream = st;
     {
-        Prof p(c, KID_TURB);
+        Prof p(c, KID_TURB, st);
         launch_vel_star(b, gate);
         launch_turb_sources(b, gate);
     }
-    Prof p(c, KID_DAMP);
+    Prof p(c, KID_DAMP, st);
     const int t0[6] = {NFIELD + 0, NFIELD + 1, NFIELD + 2, NFIELD + 3, NFIELD + 4, NFIELD + 5};
     const int dst[6] = {F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC};
     launch_damp_tile(b, c->tile_shape, c->tile_ich, t0, dst, 6, 0, gate);
 }
 
+// The moist block of run_3D_loop (cAtmosphereModel.cpp:1185-1433). The turbulence update
+// reads u, v, w, tke, dis and writes only the six turbulence fields and its scratch 0..5;
+// SatAdj, the moist damping and densities read and write the thermodynamic fields and
+// scratch 6..9. The two sets are disjoint, so their order is immaterial and the turbulence
+// runs on the second stream beside the others.
 void op_moist_block(athad_ctx* c, int gate) {
+    hipEventRecord(c->ev_fork, c->stream);
+    hipStreamWaitEvent(c->stream2, c->ev_fork, 0);
+    op_turbulence(c, gate, c->stream2);
     {
         Prof p(c, KID_SATADJ);
         launch_satadj(cur(c), gate, 1);
@@ -346,7 +365,8 @@ void op_moist_block(athad_ctx* c, int gate) {
         Prof p(c, KID_DENSITIES);
         launch_densities(cur(c), 0, 1, gate);
     }
-    op_turbulence(c, gate);
+    hipEventRecord(c->ev_join, c->stream2);
+    hipStreamWaitEvent(c->stream, c->ev_join, 0);
 }
 
 // Chooses the damp tile shape and levels per block for this context's grids by timing
@@ -509,6 +529,9 @@ athad_status athad_create(int n_members, const athad_grid* grids, const athad_me
     c->mem.resize(n_members);
     Config cfg;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) { g_create_error = hipGetErrorString(e); free_ctx(c); return ATHAD_ERR_DEVICE; }
     for (int q = 0; q < n_members; q++) {
         MemberHost& m = c->mem[q];
