@@ -39,6 +39,7 @@ EXPORTS = [
     "athad_filters", "athad_store", "athad_step", "athad_run", "athad_sync", "athad_profile",
     "athad_profile_result", "athad_version", "athad_dew_point", "athad_member_from_surface",
     "athad_query", "athad_device_memory", "athad_damp_tile", "athad_set_damp_tile",
+    "athad_field_stats", "athad_set_member_params",
 ]
 
 
@@ -94,6 +95,8 @@ def lib():
         L.athad_profile_result.argtypes = [vp, i, ctypes.POINTER(d), ctypes.POINTER(i)]
         L.athad_version.restype = ctypes.c_char_p
         L.athad_query.argtypes = [vp]
+        L.athad_field_stats.argtypes = [vp, i, i, dp, dp]
+        L.athad_set_member_params.argtypes = [vp, i, ctypes.POINTER(MemberParams)]
         L.athad_damp_tile.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
         L.athad_set_damp_tile.argtypes = [vp, i, i]
         L.athad_device_memory.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]

@@ -553,7 +553,7 @@ athad_status athad_create(int n_members, const athad_grid* grids, const athad_me
         const size_t n2 = (size_t)m.g.jm * m.g.km;
         ok = ok && alloc((void**)&m.vel_star, n2 * 8) && alloc((void**)&m.t_top, n2 * 8) &&
              alloc((void**)&m.t_ref, m.g.im * 8) && alloc((void**)&m.tables, tab.size() * 8) &&
-             alloc((void**)&m.scratch, (size_t)std::max(8192, 2 * 32 * m.g.im) * 8) && alloc((void**)&m.counters, 64) &&
+             alloc((void**)&m.scratch, (size_t)std::max(8192, 2 * 32 * m.g.im + 1024) * 8) && alloc((void**)&m.counters, 64) &&
              alloc((void**)&m.passes, passes.size() * 4);
         if (!ok) { g_create_error = "device allocation failed"; free_ctx(c); return ATHAD_ERR_ALLOC; }
         for (int f = 0; f < NFIELD; f++) hipMemsetAsync(m.field[f], 0, field_bytes(c, m, f), c->stream);
@@ -646,6 +646,34 @@ athad_status athad_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
     if (free_bytes) *free_bytes = (int64_t)f;
     if (total_bytes) *total_bytes = (int64_t)t;
     return ATHAD_OK;
+}
+
+athad_status athad_field_stats(athad_ctx* c, int member, int field, double* sum, double* sumsq) {
+    athad_status s = check_member(c, member, field, NFIELD);
+    if (s) return s;
+    launch_field_stats(cur(c), member, field);
+    double r[2] = {0.0, 0.0};
+    const MemberHost& m = c->mem[member];
+    HIPCHK(c, hipMemcpyAsync(r, m.scratch + field_stats_offset(m.g.im), sizeof(r), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (sum) *sum = r[0];
+    if (sumsq) *sumsq = r[1];
+    return ATHAD_OK;
+}
+
+athad_status athad_set_member_params(athad_ctx* c, int member, const athad_member_params* p) {
+    if (!c || !p || member < 0 || member >= c->nmem) return ATHAD_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    MemberHost& m = c->mem[member];
+    m.p = *p;
+    const Config cfg;
+    fill_consts(m, cfg, c->order);
+    std::vector<double> tab;
+    std::vector<int> passes;
+    build_tables(m, cfg, tab, passes);
+    HIPCHK(c, hipMemcpy(m.tables, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m.passes, passes.data(), passes.size() * 4, hipMemcpyHostToDevice));
+    return upload_descs(c);
 }
 
 athad_status athad_damp_tile(athad_ctx* c, int* tj, int* tk, int* levels) {

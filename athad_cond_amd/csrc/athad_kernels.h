@@ -23,6 +23,10 @@ enum KernelId {
 
 void launch_advance(const Batch& b);
 void launch_level_mean(const Batch& b);
+// sum and sum of squares of one field of one member, in a fixed order, left at
+// scratch[field_stats_offset(im)], [+1]
+void launch_field_stats(const Batch& b, int member, int field);
+int field_stats_offset(int im);
 void launch_rk4_jacobi(const Batch& b, bool fused);
 void launch_rk4_lex(const Batch& b, int plane);
 void launch_poisson(const Batch& b, int gate);

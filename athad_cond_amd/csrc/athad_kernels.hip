@@ -117,6 +117,50 @@ __global__ void k_level_mean_final(const MemberDesc* md) {
 // once per cell (prep_cell); each stage (rhs_stage) only adds the centre-value terms, the
 // minmod slopes and the SST closure. The RK4 base and accumulator live in LDS.
 // ------------------------------------------------------------------------------------
+// Sum and sum of squares of one field of member `member` in a fixed order (kStatChunks
+// contiguous chunks, each reduced by a fixed tree, then the chunks in order), so equal
+// fields give equal bits wherever and whenever they are computed. Results at
+// scratch[kStatBase(im) ...].
+constexpr int kStatChunks = 256;
+ATHAD_DEV int stat_base(const Consts& K) { return 2 * kLevelChunks * K.im; }
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_field_stats(const MemberDesc* md, int member, int field) {
+    const MemberDesc& M = md[member];
+    const Consts& K = M.K;
+    const long long n = K.n;
+    const long long per = (n + kStatChunks - 1) / kStatChunks;
+    const long long q0 = (long long)blockIdx.x * per, q1 = min(n, q0 + per);
+    const bool dbl = (field == F_P_STAT) || sizeof(S) == 8;
+    __shared__ double ss[256], sq[256];
+    double a = 0.0, b = 0.0;
+    for (long long q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        const double v = dbl ? ((const double*)M.f[field])[q] : (double)((const float*)M.f[field])[q];
+        a += v;
+        b += v * v;
+    }
+    ss[threadIdx.x] = a;
+    sq[threadIdx.x] = b;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) { ss[threadIdx.x] += ss[threadIdx.x + st]; sq[threadIdx.x] += sq[threadIdx.x + st]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        M.scratch[stat_base(K) + 2 * blockIdx.x] = ss[0];
+        M.scratch[stat_base(K) + 2 * blockIdx.x + 1] = sq[0];
+    }
+}
+
+__global__ void k_field_stats_final(const MemberDesc* md, int member) {
+    const MemberDesc& M = md[member];
+    double a = 0.0, b = 0.0;
+    const double* r = M.scratch + stat_base(M.K);
+    for (int c = 0; c < kStatChunks; c++) { a += r[2 * c]; b += r[2 * c + 1]; }
+    M.scratch[stat_base(M.K) + 2 * kStatChunks] = a;
+    M.scratch[stat_base(M.K) + 2 * kStatChunks + 1] = b;
+}
+
 enum { FT = 0, FU, FV, FW, FC, FCL, FI, FG, FCO, FK, FD };
 
 // Solid-ground clamps of bcSolidGround passes 4-5 (BC_Atm.h:475-548) as a value transform,
@@ -1820,6 +1864,11 @@ inline unsigned cols_grid(const Batch& b, int bs) {
 void launch_advance(const Batch& b) {
     hipLaunchKernelGGL(k_advance, dim3(1, 1, b.nmem), dim3(64), 0, b.stream, b.d_md);
 }
+void launch_field_stats(const Batch& b, int member, int field) {
+    ATHAD_DISPATCH(k_field_stats, dim3(kStatChunks), dim3(256), 0, b.d_md, member, field);
+    hipLaunchKernelGGL(k_field_stats_final, dim3(1), dim3(1), 0, b.stream, b.d_md, member);
+}
+int field_stats_offset(int im) { return 2 * kLevelChunks * im + 2 * kStatChunks; }
 void launch_level_mean(const Batch& b) {
     ATHAD_DISPATCH(k_level_mean, dim3(kLevelChunks, b.max_im, b.nmem), dim3(256), 0, b.d_md);
     hipLaunchKernelGGL(k_level_mean_final, dim3((b.max_im + 63) / 64, 1, b.nmem), dim3(64), 0, b.stream, b.d_md);

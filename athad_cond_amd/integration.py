@@ -131,6 +131,16 @@ class Integration:
     def run(self, n: int, graph: bool = False):
         self._chk(L.lib().athad_run(self.h, n, int(graph)))
 
+    def field_stats(self, name: str, member: int = 0):
+        """(sum, sum of squares) of a field, reduced on the device in a fixed order."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._chk(L.lib().athad_field_stats(self.h, member, L.FIELD_ID[name], ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def set_member_params(self, params, member: int = 0):
+        """New surface state / composition for a member (reuse of a context); then init_state."""
+        self._chk(L.lib().athad_set_member_params(self.h, member, ctypes.byref(params)))
+
     def damp_tile(self):
         """(tj, tk, levels) of the fused damping tile chosen for this context."""
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

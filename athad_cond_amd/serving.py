@@ -71,9 +71,16 @@ def device_memory() -> tuple[int, int]:
 
 
 class Server:
-    """Continuous-batching loop over per-request contexts."""
+    """Continuous-batching loop over per-request contexts.
 
-    def __init__(self, hbm_budget: int | None = None, max_slots: int = 8, chunk: int = 4,
+    Finished contexts go back to a pool keyed by (grid, precision) and are reused for later
+    requests of the same shape (new member parameters, fresh initial state), so the loop does
+    no device allocation or free while requests run; pooled memory counts against the budget
+    and idle pooled contexts are released when an admission needs the room. A request's result
+    is the (sum, sum of squares) of each prognostic field, reduced on the device in a fixed
+    order; keep_fields also downloads the fields."""
+
+    def __init__(self, hbm_budget: int | None = None, max_slots: int = 4, chunk: int = 4,
                  keep_fields: bool = False):
         if hbm_budget is None:
             free, _ = device_memory()
@@ -82,50 +89,77 @@ class Server:
         self.max_slots = max_slots
         self.chunk = chunk
         self.keep_fields = keep_fields
+        self.pool: dict = collections.defaultdict(list)
+        self.used = 0                       # bytes held by active and pooled contexts
         self.peak_active = 0
         self.peak_bytes = 0
+        self.reused = 0
 
-    def _start(self, r: Request, now: float):
-        ctx = Integration(r.grid, [member_params(r.state)], precision=r.precision)
+    def close(self):
+        for lst in self.pool.values():
+            for ctx in lst:
+                ctx.close()
+        self.pool.clear()
+        self.used = 0
+
+    def _evict_for(self, need: int) -> bool:
+        for key in list(self.pool):
+            while self.pool[key] and self.used + need > self.hbm_budget:
+                self.pool[key].pop().close()
+                self.used -= footprint_bytes(key[0], key[1])
+        return self.used + need <= self.hbm_budget
+
+    def _admit(self, r: Request, now: float):
+        """A slot for r, or None if the device memory is not there yet."""
+        key = (r.grid, r.precision)
+        if self.pool[key]:
+            ctx = self.pool[key].pop()
+            ctx.set_member_params(member_params(r.state))
+            self.reused += 1
+        else:
+            need = footprint_bytes(r.grid, r.precision)
+            if self.used + need > self.hbm_budget and not self._evict_for(need):
+                return None
+            ctx = Integration(r.grid, [member_params(r.state)], precision=r.precision)
+            self.used += need
         ctx.init_state(r.seed)
         r.admitted = now
-        return {"req": r, "ctx": ctx, "done": 0, "bytes": footprint_bytes(r.grid, r.precision)}
+        return {"req": r, "ctx": ctx, "done": 0}
 
     def _finish(self, slot, now: float) -> bool:
         """Collects a finished request; False if it has to run again in FP64."""
         r, ctx = slot["req"], slot["ctx"]
-        fields = {n: ctx.get(n) for n in L.PROGNOSTIC}
-        ctx.close()
-        finite = all(np.isfinite(v).all() for v in fields.values())
+        stats = {n: ctx.field_stats(n) for n in L.PROGNOSTIC}
+        fields = {n: ctx.get(n) for n in L.PROGNOSTIC} if self.keep_fields else None
+        self.pool[(r.grid, r.precision)].append(ctx)
+        finite = all(np.isfinite(a) and np.isfinite(b) for a, b in stats.values())
         if not finite and r.precision == "mixed":
             r.precision = "fp64"
             r.fallback = True
             return False
         r.finished = now
-        r.result = {n: (float(v.sum()), float(np.sqrt((v * v).sum()))) for n, v in fields.items()}
-        if self.keep_fields:
+        r.result = dict(stats)
+        if fields is not None:
             r.result["fields"] = fields
         return True
 
     def run(self, requests: list[Request]) -> dict:
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         active = []
-        used = 0
         t0 = time.perf_counter()
         while pending or active:
             now = time.perf_counter() - t0
             # admit arrived requests while a slot and device memory are free
             while pending and pending[0].arrival <= now and len(active) < self.max_slots:
-                need = footprint_bytes(pending[0].grid, pending[0].precision)
-                if used + need > self.hbm_budget:
+                slot = self._admit(pending[0], now)
+                if slot is None:
                     if not active:
-                        raise RuntimeError(f"request {pending[0].rid} needs {need} B, budget {self.hbm_budget} B")
+                        raise RuntimeError(f"request {pending[0].rid} does not fit the budget {self.hbm_budget} B")
                     break
-                slot = self._start(pending.popleft(), now)
-                used += slot["bytes"]
+                pending.popleft()
                 active.append(slot)
             self.peak_active = max(self.peak_active, len(active))
-            self.peak_bytes = max(self.peak_bytes, used)
+            self.peak_bytes = max(self.peak_bytes, self.used)
             # keep every admitted request's stream fed; collect the finished ones
             progressed = False
             for slot in list(active):
@@ -133,10 +167,8 @@ class Server:
                     continue
                 r = slot["req"]
                 if slot["done"] >= r.steps:
-                    now = time.perf_counter() - t0
-                    used -= slot["bytes"]
                     active.remove(slot)
-                    if not self._finish(slot, now):
+                    if not self._finish(slot, time.perf_counter() - t0):
                         pending.appendleft(r)          # FP64 fallback, ahead of later arrivals
                     progressed = True
                     continue
@@ -157,4 +189,5 @@ class Server:
         return {"requests": len(requests), "makespan_s": makespan,
                 "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
                 "cells_per_s": cells / makespan, "peak_active": self.peak_active,
-                "peak_hbm_bytes": self.peak_bytes, "fallbacks": sum(r.fallback for r in requests)}
+                "peak_hbm_bytes": self.peak_bytes, "contexts_reused": self.reused,
+                "fallbacks": sum(r.fallback for r in requests)}

@@ -165,7 +165,9 @@ def main():
                            for prec in ("fp64", "mixed")}
         from athad_cond_amd import serving
         reqs = serving.poisson_requests(24, rate_per_s=4.0, seed=99, steps=50)
-        out["serving"] = serving.Server(max_slots=8, chunk=4).run(reqs)
+        server = serving.Server(max_slots=4, chunk=4)
+        out["serving"] = server.run(reqs)
+        server.close()
         out["serving"]["arrivals"] = "Poisson, mean 4 req/s, seed 99; 50 steps each; grids 21x91x181, 41x181x361, 61x361x721; mixed with FP64 fallback"
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

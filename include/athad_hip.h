@@ -115,6 +115,14 @@ athad_status athad_get_field2d(athad_ctx* ctx, int member, int field, double* ds
  * lib/Array.cpp:62-71): rows[i*jm + j] points at row (i,j). */
 athad_status athad_upload_rows(athad_ctx* ctx, int member, int field, double* const* rows);
 athad_status athad_download_rows(athad_ctx* ctx, int member, int field, double* const* rows);
+/* Sum and sum of squares of one field of one member, reduced on the device in a fixed order
+ * (equal fields give equal bits). No counterpart in the reference; for result digests that
+ * avoid downloading whole fields. */
+athad_status athad_field_stats(athad_ctx* ctx, int member, int field, double* sum, double* sumsq);
+/* Replaces a member's surface state and composition (the constants and tables derived from
+ * them) so a context can be reused for a new integration of the same grid; follow with
+ * athad_init_state. No counterpart in the reference (one model per process there). */
+athad_status athad_set_member_params(athad_ctx* ctx, int member, const athad_member_params* params);
 /* iteration counters iter_n / total_iter_count (cAtmosphereModel.h:311-316) */
 athad_status athad_set_iter(athad_ctx* ctx, int member, int iter_n, int total_iter_count);
 athad_status athad_get_iter(athad_ctx* ctx, int member, int* iter_n, int* total_iter_count);

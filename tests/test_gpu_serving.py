@@ -14,15 +14,21 @@ GRIDS = [(11, 21, 41), (21, 91, 181), (15, 31, 61)]
 
 def test_same_result_alone_and_in_mix():
     print("case serving alone vs mix", flush=True)
-    reqs = S.poisson_requests(6, rate_per_s=500.0, seed=3, steps=10, grids=GRIDS)
-    stats = S.Server(max_slots=6, chunk=2, keep_fields=True).run(reqs)
+    reqs = S.poisson_requests(9, rate_per_s=500.0, seed=3, steps=10, grids=GRIDS)
+    server = S.Server(max_slots=4, chunk=2, keep_fields=True)
+    stats = server.run(reqs)
     print(stats, flush=True)
     assert stats["peak_active"] > 1
-    for r in reqs[:3]:
+    assert stats["contexts_reused"] > 0          # later requests ran in pooled contexts
+    for r in reqs[:4]:
         alone = S.Request(r.rid, 0.0, r.grid, r.state, r.steps, "mixed", r.seed)
-        S.Server(max_slots=1, keep_fields=True).run([alone])
+        one = S.Server(max_slots=1, keep_fields=True)
+        one.run([alone])
+        one.close()
         for n, v in alone.result["fields"].items():
             assert np.array_equal(v, r.result["fields"][n]), (r.rid, n)
+            assert alone.result[n] == r.result[n], (r.rid, n)   # device digests, bit for bit
+    server.close()
 
 
 def test_admission_respects_budget():
@@ -31,6 +37,7 @@ def test_admission_respects_budget():
     one = footprint_bytes((21, 91, 181), "mixed")
     server = S.Server(hbm_budget=2 * one + 1, max_slots=8, chunk=2)
     stats = server.run(reqs)
+    server.close()
     print(stats, flush=True)
     assert stats["peak_active"] == 2
     assert stats["peak_hbm_bytes"] <= 2 * one + 1
