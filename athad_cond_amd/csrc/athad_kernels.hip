@@ -817,12 +817,17 @@ ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi
     N cpH, cpC, cpB;
     sat_species_cp(T, Mb, cpH, cpC, cpB);
     N Lv = sat_L_vap(T);
-    N T_prev = T, qv_prev = qvb, qc_prev = qcb, qi_prev = qib;
+    N T_prev = T, qv_prev = qvb, qc_prev = qcb, qi_prev = qib, CND_prev = N(-1);
     for (int it = 1; it <= 20; it++) {
         const N CND = tmax(N(0), tmin(N(1), (T - t_00) * t_range_inv));
         const N DEP = N(1) - CND;
         N dqv = qv_hyp - qvb;
         if (dqv > N(0)) { const N me = qcb + qib; if (dqv > me) dqv = me; }
+        // A zero step with the same phase split as the previous iteration leaves T, qvb, qcb,
+        // qib unchanged and recomputes the same qv_hyp from the same inputs: the state is
+        // already the fixed point the remaining iterations would keep.
+        if (dqv == N(0) && CND == CND_prev) break;
+        CND_prev = CND;
         const N dcnd = dqv * CND, ddep = dqv * DEP;
         qvb += dqv;
         qcb = tmax(N(0), qcb - dcnd);
