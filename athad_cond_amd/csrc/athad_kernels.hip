@@ -303,12 +303,26 @@ template <typename C> ATHAD_DEV C buoyancy(const MemberDesc& M) {
 
 // RK4 stage update of one field (RungeKutta_Atm_Turb.cpp:167-263): b = base (*n value) and
 // a = k1+2k2+2k3 accumulator, both in this thread's LDS slots.
+// Stage st adds r * kr to the accumulator (kr = 1, 2, 2, 1; the first stage starts it from
+// zero) and returns b + r * cdt (cdt = dt/2, dt/2, dt), or in the last stage
+// b + (k1 + 2 k2 + 2 k3 + k4) dt/6. The FP32 kernel uses the branch-free form (wave-uniform
+// coefficients); the FP64 kernel keeps per-stage branches, which need fewer registers there
+// (measured: the branch-free FP64 kernel spills).
 template <typename C> ATHAD_DEV C stage_update(int st, C r, C* b, C* a, C dt, C half_dt, C dt_sixth) {
-    const C bv = *b;
-    if (st == 0) { *a = r; return bv + r * half_dt; }
-    if (st == 1) { *a = *a + C(2) * r; return bv + r * half_dt; }
-    if (st == 2) { *a = *a + C(2) * r; return bv + r * dt; }
-    return bv + (*a + r) * dt_sixth;
+    if constexpr (sizeof(C) == 4) {
+        const C kr = (st == 1 || st == 2) ? C(2) : C(1);
+        const C cdt = (st == 2) ? dt : half_dt;
+        const C acc = ((st == 0) ? C(0) : *a) + r * kr;
+        *a = acc;
+        const C bv = *b;
+        return (st == 3) ? bv + acc * dt_sixth : bv + r * cdt;
+    } else {
+        const C bv = *b;
+        if (st == 0) { *a = r; return bv + r * half_dt; }
+        if (st == 1) { *a = *a + C(2) * r; return bv + r * half_dt; }
+        if (st == 2) { *a = *a + C(2) * r; return bv + r * dt; }
+        return bv + (*a + r) * dt_sixth;
+    }
 }
 
 constexpr int LB = 256;   // LDS slot stride = threads per block of the RK4 kernels
