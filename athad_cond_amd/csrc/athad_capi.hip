@@ -94,6 +94,10 @@ struct athad_ctx {
     // fields), forked and joined with events (also inside a captured graph)
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // the filters of an iteration may run on the second stream beside the next iteration's
+    // Poisson sweep and moist thermodynamics; s2_pending: not yet joined into `stream`
+    hipEvent_t ev_ffork = nullptr, ev_fjoin = nullptr;
+    bool s2_pending = false;
     Batch batch;
     std::string err;
     // profiling
@@ -273,6 +277,8 @@ void free_ctx(athad_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->ev_ffork) hipEventDestroy(c->ev_ffork);
+    if (c->ev_fjoin) hipEventDestroy(c->ev_fjoin);
     if (c->stream2) hipStreamDestroy(c->stream2);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -367,6 +373,15 @@ void op_moist_block(athad_ctx* c, int gate) {
     }
     hipEventRecord(c->ev_join, c->stream2);
     hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    c->s2_pending = false;
+}
+
+// Makes the context's stream wait for work left on the second stream.
+void join_s2(athad_ctx* c) {
+    if (!c->s2_pending) return;
+    hipEventRecord(c->ev_fjoin, c->stream2);
+    hipStreamWaitEvent(c->stream, c->ev_fjoin, 0);
+    c->s2_pending = false;
 }
 
 // Chooses the damp tile shape and levels per block for this context's grids by timing
@@ -453,13 +468,17 @@ void op_rk4(athad_ctx* c, bool fused) {
     }
 }
 
-void op_filters(athad_ctx* c, int store) {
+// polar_zonal_filter and the radial Shapiro filters of u, v, w (+ their store), on stream st
+// (the context's own when null).
+void op_filters(athad_ctx* c, int store, hipStream_t st = nullptr) {
+    Batch b = cur(c);
+    if (st) b.stream = st;
     {
-        Prof p(c, KID_POLAR);
-        launch_polar(cur(c));
+        Prof p(c, KID_POLAR, st);
+        launch_polar(b);
     }
-    Prof p(c, KID_RADIAL);
-    launch_radial(cur(c), store);
+    Prof p(c, KID_RADIAL, st);
+    launch_radial(b, store);
 }
 
 // One scoped iteration for all members. When every member is at the same phase the
@@ -477,9 +496,21 @@ void op_step(athad_ctx* c) {
         op_moist_block(c, 2);
     }
     const bool lex = (c->order == ATHAD_ORDER_LEX);
+    join_s2(c);   // the previous iteration's filters, if no moist block joined them already
     op_bc(c, lex);
     op_rk4(c, !lex);
-    op_filters(c, 1);
+    if (!lex) {
+        // The filters read and write only u, v, w and un, vn, wn. The next iteration's
+        // Poisson sweep (aux, p_dyn) and moist thermodynamics (t, c, cloud, ice, gr, co2,
+        // p_stat, r_humid, r_dry) touch none of them, and its turbulence update, which reads
+        // u, v, w, is queued behind them on the same stream; the boundaries join first.
+        hipEventRecord(c->ev_ffork, c->stream);
+        hipStreamWaitEvent(c->stream2, c->ev_ffork, 0);
+        op_filters(c, 1, c->stream2);
+        c->s2_pending = true;
+    } else {
+        op_filters(c, 1);
+    }
     if (lex) {
         Prof p(c, KID_STORE);
         launch_store(cur(c), 0x7FF & ~0xE);   // all but u, v, w (stored by the radial filter)
@@ -532,6 +563,8 @@ athad_status athad_create(int n_members, const athad_grid* grids, const athad_me
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_ffork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fjoin, hipEventDisableTiming);
     if (e != hipSuccess) { g_create_error = hipGetErrorString(e); free_ctx(c); return ATHAD_ERR_DEVICE; }
     for (int q = 0; q < n_members; q++) {
         MemberHost& m = c->mem[q];
@@ -897,6 +930,7 @@ athad_status athad_store(athad_ctx* c) {
 athad_status athad_step(athad_ctx* c) {
     if (!c) return ATHAD_ERR_ARG;
     op_step(c);
+    join_s2(c);
     return after_launch(c);
 }
 
@@ -904,6 +938,7 @@ athad_status athad_run(athad_ctx* c, int n_steps, int use_graph) {
     if (!c || n_steps < 0) return ATHAD_ERR_ARG;
     if (!use_graph) {
         for (int q = 0; q < n_steps; q++) op_step(c);
+        join_s2(c);
         return after_launch(c);
     }
     int done = 0;
@@ -911,6 +946,7 @@ athad_status athad_run(athad_ctx* c, int n_steps, int use_graph) {
     while (done < n_steps) {
         const int phase = c->mem[0].iter_n % 4;
         if (c->graph && phase == c->graph_phase && c->parity == c->graph_parity && n_steps - done >= 4) {
+            join_s2(c);
             HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
             for (auto& m : c->mem) { m.iter_n += 4; m.total += 4; }
             done += 4;
@@ -921,9 +957,11 @@ athad_status athad_run(athad_ctx* c, int n_steps, int use_graph) {
             const int ph = c->mem[0].iter_n % 4, par = c->parity;
             std::vector<MemberHost> saved = c->mem;
             hipGraph_t g;
+            join_s2(c);
             c->capturing = true;
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             for (int q = 0; q < 4; q++) op_step(c);
+            join_s2(c);
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             c->capturing = false;
             if (e != hipSuccess) { c->err = std::string("graph capture: ") + hipGetErrorString(e); return ATHAD_ERR_DEVICE; }
@@ -940,6 +978,7 @@ athad_status athad_run(athad_ctx* c, int n_steps, int use_graph) {
         op_step(c);
         done++;
     }
+    join_s2(c);
     return after_launch(c);
 }
 
