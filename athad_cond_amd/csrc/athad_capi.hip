@@ -358,9 +358,17 @@ void op_turbulence(athad_ctx* c, int gate, hipStream_t st = nullptr) {
 // SatAdj, the moist damping and densities read and write the thermodynamic fields and
 // scratch 6..9. The two sets are disjoint, so their order is immaterial and the turbulence
 // runs on the second stream beside the others.
-void op_moist_block(athad_ctx* c, int gate) {
+void op_moist_block(athad_ctx* c, int gate, bool poisson_beside = false) {
     hipEventRecord(c->ev_fork, c->stream);
     hipStreamWaitEvent(c->stream2, c->ev_fork, 0);
+    if (poisson_beside) {
+        // PressureSolverAtm reads aux_u/v/w and p_dyn and writes p_dyn: none of the moist
+        // block's fields; after the fork it also sees this iteration's counters
+        Batch b2 = cur(c);
+        b2.stream = c->stream2;
+        Prof p(c, KID_POISSON, c->stream2);
+        launch_poisson(b2, 0);
+    }
     op_turbulence(c, gate, c->stream2);
     {
         Prof p(c, KID_SATADJ);
@@ -489,8 +497,8 @@ void op_step(athad_ctx* c) {
     for (auto& m : c->mem) same = same && ((m.iter_n + 1) % 4 == (c->mem[0].iter_n + 1) % 4);
     const int it = c->mem[0].iter_n + 1;
     if (same) {
-        if (it % 4 == 0) { Prof p(c, KID_POISSON); launch_poisson(cur(c), 0); }
-        if (it % 2 == 0) op_moist_block(c, 0);
+        // every 4th iteration is also a moist iteration: the sweep runs beside it
+        if (it % 2 == 0) op_moist_block(c, 0, it % 4 == 0);
     } else {
         { Prof p(c, KID_POISSON); launch_poisson(cur(c), 4); }
         op_moist_block(c, 2);
