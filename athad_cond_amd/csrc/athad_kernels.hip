@@ -1145,6 +1145,47 @@ __global__ void k_copy_job(const MemberDesc* md, DampJob job) {
 // saturation boundary of waterVapourEvaporation (ThermoAtm.h:115-258). One thread per
 // (j,k) column, serial upward in i; FP64 arithmetic. set_water: initial profile mode.
 // ------------------------------------------------------------------------------------
+// Column thermodynamics of k_densities: th_R_of, th_cp_of, th_M_nonwater, th_q_sat and
+// th_moist_lapse_E restated with shared sub-expressions (one split, one 1/t^2 for the three
+// Shomate fits, reciprocals of repeated denominators) and refined reciprocals, which shortens
+// the serial per-level chain of the column recursion; results agree with th_* to a few ulp.
+struct MixQ { double qv, qc, qb; };
+ATHAD_DEV MixQ dens_split(double c, double co2) {
+    double qv = tmin(tmax(c, 0.0), 1.0), qc = tmin(tmax(co2, 0.0), 1.0);
+    const double s = qv + qc;
+    if (s > 1.0) { qv /= s; qc /= s; }
+    return {qv, qc, tmax(0.0, 1.0 - qv - qc)};
+}
+ATHAD_DEV double dens_R(const MixQ& m, double Rb) { return m.qv * kRH2O + m.qc * kRCO2 + m.qb * Rb; }
+ATHAD_DEV double dens_cp(const MixQ& m, double T, double inv_Mb) {
+    const double t = tmin(tmax(T, 298.0), 2000.0) * 1.0e-3;
+    const double t2 = t * t, t3 = t2 * t, it2 = rcp_nr(t2);
+    const double cpH = (30.09200 + 6.832514 * t + 6.793435 * t2 + -2.534480 * t3 + 0.082139 * it2) * (1.0 / kMH2O);
+    const double cpC = (24.99735 + 55.18696 * t + -33.69137 * t2 + 7.948387 * t3 + -0.136638 * it2) * (1.0 / kMCO2);
+    const double cpB = (28.98641 + 1.853978 * t + -9.647459 * t2 + 16.63537 * t3 + 0.000117 * it2) * inv_Mb;
+    return m.qv * cpH + m.qc * cpC + m.qb * cpB;
+}
+ATHAD_DEV double dens_Mnw(const MixQ& m, double Mb, double inv_Mb) {
+    const double s = m.qc + m.qb;
+    if (!(s > 0.0)) return Mb;
+    const double is = rcp_nr(s);
+    const double inv = (m.qc * is) * (1.0 / kMCO2) + (m.qb * is) * inv_Mb;
+    return (inv > 0.0) ? rcp_nr(inv) : Mb;
+}
+ATHAD_DEV double dens_p_auto(double T) {
+    if (T >= kTcrit) return kNoSat;
+    return (T >= kTtriple) ? sat_p_sat(T) : sat_p_subl(T);
+}
+// moist lapse rate given q_sat's factor A at (T, p)
+ATHAD_DEV double dens_lapse(double T, double A, double cp, double Rm, double g) {
+    if (!(cp > 0.0) || !(T > 0.0) || !(Rm > 0.0)) return 0.0;
+    if (!(A > 0.0)) return g * rcp_nr(cp);
+    const double L = (T >= kTtriple) ? sat_L_vap(T) : sat_L_vap(T) + 3.337e5;
+    const double num = 1.0 + L * A * rcp_nr(Rm * T);
+    const double den = cp + L * L * A * rcp_nr(kRH2O * T * T);
+    return (den > 0.0) ? g * num * rcp_nr(den) : g * rcp_nr(cp);
+}
+
 template <typename S>
 __global__ __launch_bounds__(128) void k_densities(const MemberDesc* md, int set_water, int evaporate, int gate) {
     const MemberDesc& M = md[blockIdx.z];
@@ -1156,7 +1197,7 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* md, int set
     S *t = fp<S>(M, F_T), *c = fp<S>(M, F_C), *co2 = fp<S>(M, F_CO2), *cl = fp<S>(M, F_CLOUD), *ice = fp<S>(M, F_ICE),
       *rh = fp<S>(M, F_R_HUMID), *rd = fp<S>(M, F_R_DRY);
     double* ps = pstat(M);
-    const double t_0 = K.t_0, Mb = K.M_bg, Rb = K.R_bg;
+    const double t_0 = K.t_0, Mb = K.M_bg, Rb = K.R_bg, inv_Mb = 1.0 / K.M_bg;
     const size_t si = (size_t)K.jm * K.km;
     const size_t base = (size_t)j * K.km + k;
     double T_prev = tmax(180.0, (double)t[base] * t_0);
@@ -1174,21 +1215,22 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* md, int set
             qv_n = gld(c, id + (long long)si); qc_n = gld(co2, id + (long long)si);
             cl_n = gld(cl, id + (long long)si); ic_n = gld(ice, id + (long long)si);
         }
-        const double R_loc = th_R_of(qv, qc, Rb);
+        const double R_loc = dens_R(dens_split(qv, qc), Rb);
         double Ti, pi;
         if (i == 0) { Ti = T_prev; pi = p_prev; }
         else {
             const double dz = M.hgt[i] - M.hgt[i - 1];
-            const double cp_loc = th_cp_of(qv_prev, qc_prev, T_prev, Mb);
-            const double M_nw = th_M_nonwater(qv_prev, qc_prev, Mb);
-            const double E = th_p_auto(T_prev);
-            const double q_s = th_q_sat(E, p_prev, M_nw);
-            const double gamma = (qv_prev >= 0.99 * q_s)
-                ? th_moist_lapse_E(T_prev, p_prev, M_nw, cp_loc, R_loc, K.g, E) : K.g / cp_loc;
+            const MixQ mp = dens_split(qv_prev, qc_prev);
+            const double cp_loc = dens_cp(mp, T_prev, inv_Mb);
+            const double M_nw = dens_Mnw(mp, Mb, inv_Mb);
+            double q_s, A;
+            sat_qA(dens_p_auto(T_prev), p_prev, (p_prev > 0.0) ? rcp_nr(p_prev) : 0.0, M_nw, q_s, A);
+            const double gamma = (qv_prev >= 0.99 * q_s) ? dens_lapse(T_prev, A, cp_loc, R_loc, K.g)
+                                                         : K.g * rcp_nr(cp_loc);
             const double T_ad = T_prev - gamma * dz;
             Ti = tmax(K.t_skin, T_ad);
             const double T_mean = 0.5 * (T_prev + Ti);
-            pi = p_prev * exp(-K.g * dz / (R_loc * T_mean));
+            pi = p_prev * exp(-K.g * dz * rcp_nr(R_loc * T_mean));
         }
         gst(t, id, S(Ti / t_0));
         gst(ps, id, pi);
@@ -1199,8 +1241,8 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* md, int set
             gst(c, id, S(qv));
         }
         const double wf = tmax(0.5, 1.0 - (double)S(wcl) - (double)S(wic));
-        gst(rh, id, S(1e2 * pi / (R_loc * Ti * wf)));
-        gst(rd, id, S(1e2 * pi / (th_R_of(0.0, qc, Rb) * Ti)));
+        gst(rh, id, S(1e2 * pi * rcp_nr(R_loc * Ti * wf)));
+        gst(rd, id, S(1e2 * pi * rcp_nr(dens_R(dens_split(0.0, qc), Rb) * Ti)));
         // the next level reads this level's stored water (the reference reads c[i-1] back)
         qv_prev = (double)S(qv);
         qc_prev = qc;
