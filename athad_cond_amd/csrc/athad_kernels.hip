@@ -816,6 +816,47 @@ ATHAD_DEV void sat_qA(double E, double p, double inv_p, double Mo, double& q, do
     A = num * Mo * rd * rd;
 }
 
+// Column and cell thermodynamics (k_densities, the SatAdj entry and clamp): th_R_of, th_cp_of, th_M_nonwater, th_q_sat and
+// th_moist_lapse_E restated with shared sub-expressions (one split, one 1/t^2 for the three
+// Shomate fits, reciprocals of repeated denominators) and refined reciprocals, which shortens
+// the serial per-level chain of the column recursion; results agree with th_* to a few ulp.
+struct MixQ { double qv, qc, qb; };
+ATHAD_DEV MixQ dens_split(double c, double co2) {
+    double qv = tmin(tmax(c, 0.0), 1.0), qc = tmin(tmax(co2, 0.0), 1.0);
+    const double s = qv + qc;
+    if (s > 1.0) { qv /= s; qc /= s; }
+    return {qv, qc, tmax(0.0, 1.0 - qv - qc)};
+}
+ATHAD_DEV double dens_R(const MixQ& m, double Rb) { return m.qv * kRH2O + m.qc * kRCO2 + m.qb * Rb; }
+ATHAD_DEV double dens_cp(const MixQ& m, double T, double inv_Mb) {
+    const double t = tmin(tmax(T, 298.0), 2000.0) * 1.0e-3;
+    const double t2 = t * t, t3 = t2 * t, it2 = rcp_nr(t2);
+    const double cpH = (30.09200 + 6.832514 * t + 6.793435 * t2 + -2.534480 * t3 + 0.082139 * it2) * (1.0 / kMH2O);
+    const double cpC = (24.99735 + 55.18696 * t + -33.69137 * t2 + 7.948387 * t3 + -0.136638 * it2) * (1.0 / kMCO2);
+    const double cpB = (28.98641 + 1.853978 * t + -9.647459 * t2 + 16.63537 * t3 + 0.000117 * it2) * inv_Mb;
+    return m.qv * cpH + m.qc * cpC + m.qb * cpB;
+}
+ATHAD_DEV double dens_Mnw(const MixQ& m, double Mb, double inv_Mb) {
+    const double s = m.qc + m.qb;
+    if (!(s > 0.0)) return Mb;
+    const double is = rcp_nr(s);
+    const double inv = (m.qc * is) * (1.0 / kMCO2) + (m.qb * is) * inv_Mb;
+    return (inv > 0.0) ? rcp_nr(inv) : Mb;
+}
+ATHAD_DEV double dens_p_auto(double T) {
+    if (T >= kTcrit) return kNoSat;
+    return (T >= kTtriple) ? sat_p_sat(T) : sat_p_subl(T);
+}
+// moist lapse rate given q_sat's factor A at (T, p)
+ATHAD_DEV double dens_lapse(double T, double A, double cp, double Rm, double g) {
+    if (!(cp > 0.0) || !(T > 0.0) || !(Rm > 0.0)) return 0.0;
+    if (!(A > 0.0)) return g * rcp_nr(cp);
+    const double L = (T >= kTtriple) ? sat_L_vap(T) : sat_L_vap(T) + 3.337e5;
+    const double num = 1.0 + L * A * rcp_nr(Rm * T);
+    const double den = cp + L * L * A * rcp_nr(kRH2O * T * T);
+    return (den > 0.0) ? g * num * rcp_nr(den) : g * rcp_nr(cp);
+}
+
 // Damped Newton iteration of adjustSaturation (SaturationAdjustment.h:177-254) in type N.
 // The ice branch (sublimation pressure, its q_sat and derivative) is evaluated only when it
 // can contribute: deposition fraction > 0, ice present, or the no-condensate target is ice;
@@ -888,17 +929,19 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate, 
     const double p_local = pstat(M)[id];
     double t = pt[id], c = pc[id], cloud = pcl[id], ice = pi[id], gr = pg[id];
     double scc = 0.0;
-    const double t_0 = K.t_0, t_00 = K.t_00, Mb = K.M_bg;
+    const double t_0 = K.t_0, t_00 = K.t_00, Mb = K.M_bg, inv_Mb = 1.0 / K.M_bg;
     const double inv_t_0 = 1.0 / t_0;
+    const double inv_p = (p_local > 0.0) ? rcp_nr(p_local) : 0.0;
     if (i >= 1 && i <= K.im - 2) {
         // adjustSaturation :91-284
         const double t_range_inv = 1.0 / (t_0 - t_00);
         const double qv_old = tmax(0.0, c), qc_old = tmax(0.0, cloud), qi_old = tmax(0.0, ice);
         double T = t * t_0;
         if (T < kTcrit) {
-            const double M_other = th_M_nonwater(c, co2, Mb);
-            const double qs0 = th_q_sat(th_p_auto(T), p_local, M_other);
-            const double alpha = 1.0 / (1.0 + exp(-(T - t_00) / 5.0));
+            const double M_other = dens_Mnw(dens_split(c, co2), Mb, inv_Mb);
+            double qs0, A0;
+            sat_qA(dens_p_auto(T), p_local, inv_p, M_other, qs0, A0);
+            const double alpha = rcp_nr(1.0 + exp(-(T - t_00) * 0.2));
             if ((qv_old > qs0 && alpha > 0.01) || (qv_old < qs0 && (qc_old > 1e-12 || qi_old > 1e-12))) {
                 double qvb = qv_old, qcb = qc_old, qib = qi_old;
                 const double T_orig = t * t_0;
@@ -926,8 +969,9 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate, 
         if (c > c_max) c = c_max;
         double Td = t * t_0;
         if (Td > K.t_max_phys) { Td = K.t_max_phys; t = K.t_max_phys * inv_t_0; }
-        const double M_o = th_M_nonwater(c, co2, Mb);
-        const double q_s = th_q_sat_at(Td, p_local, M_o);
+        const double M_o = dens_Mnw(dens_split(c, co2), Mb, inv_Mb);
+        double q_s, A_s;
+        sat_qA(dens_p_auto(Td), p_local, inv_p, M_o, q_s, A_s);
         if (Td >= kTcrit || q_s >= 1.0) {
             const double q_cond = tmax(0.0, cloud) + tmax(0.0, ice) + tmax(0.0, gr);
             if (q_cond > 0.0) {
@@ -936,9 +980,9 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate, 
                 const double q_used = tmax(0.0, c_new - c);
                 c = c_new; cloud = 0.0; ice = 0.0; gr = 0.0;
                 if (Td < kTcrit) {
-                    const double cp_loc = th_cp_of(c, co2, Td, Mb);
+                    const double cp_loc = dens_cp(dens_split(c, co2), Td, inv_Mb);
                     if (cp_loc > 0.0) {
-                        double T_new = Td - th_L_vap(Td) * q_used / cp_loc;
+                        double T_new = Td - sat_L_vap(Td) * q_used * rcp_nr(cp_loc);
                         if (T_new < 1.0) T_new = 1.0;
                         t = T_new / t_0;
                     }
@@ -950,14 +994,15 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate, 
             if (c > qs) {
                 const double excess = c - qs;
                 c = qs;
-                if (Td >= t_00) { cloud += excess; Td += th_L_vap(Td) * excess / th_cp_of(c, co2, Td, Mb); }
-                else { ice += excess; Td += th_L_sub(Td) * excess / th_cp_of(c, co2, Td, Mb); }
+                const double cpx = dens_cp(dens_split(c, co2), Td, inv_Mb);
+                if (Td >= t_00) { cloud += excess; Td += sat_L_vap(Td) * excess * rcp_nr(cpx); }
+                else { ice += excess; Td += (sat_L_vap(Td) + 3.337e5) * excess * rcp_nr(cpx); }
                 if (Td > K.t_max_phys) Td = K.t_max_phys;
                 t = Td * inv_t_0;
             }
             if (cloud > K.cloud_cap) cloud = K.cloud_cap;
             if (ice > K.cloud_cap) ice = K.cloud_cap;
-            const double al = 1.0 / (1.0 + exp(-(Td - t_00) / 5.0));
+            const double al = rcp_nr(1.0 + exp(-(Td - t_00) * 0.2));
             c *= al; cloud *= al; ice *= al;
         }
     }
@@ -1145,47 +1190,6 @@ __global__ void k_copy_job(const MemberDesc* md, DampJob job) {
 // saturation boundary of waterVapourEvaporation (ThermoAtm.h:115-258). One thread per
 // (j,k) column, serial upward in i; FP64 arithmetic. set_water: initial profile mode.
 // ------------------------------------------------------------------------------------
-// Column thermodynamics of k_densities: th_R_of, th_cp_of, th_M_nonwater, th_q_sat and
-// th_moist_lapse_E restated with shared sub-expressions (one split, one 1/t^2 for the three
-// Shomate fits, reciprocals of repeated denominators) and refined reciprocals, which shortens
-// the serial per-level chain of the column recursion; results agree with th_* to a few ulp.
-struct MixQ { double qv, qc, qb; };
-ATHAD_DEV MixQ dens_split(double c, double co2) {
-    double qv = tmin(tmax(c, 0.0), 1.0), qc = tmin(tmax(co2, 0.0), 1.0);
-    const double s = qv + qc;
-    if (s > 1.0) { qv /= s; qc /= s; }
-    return {qv, qc, tmax(0.0, 1.0 - qv - qc)};
-}
-ATHAD_DEV double dens_R(const MixQ& m, double Rb) { return m.qv * kRH2O + m.qc * kRCO2 + m.qb * Rb; }
-ATHAD_DEV double dens_cp(const MixQ& m, double T, double inv_Mb) {
-    const double t = tmin(tmax(T, 298.0), 2000.0) * 1.0e-3;
-    const double t2 = t * t, t3 = t2 * t, it2 = rcp_nr(t2);
-    const double cpH = (30.09200 + 6.832514 * t + 6.793435 * t2 + -2.534480 * t3 + 0.082139 * it2) * (1.0 / kMH2O);
-    const double cpC = (24.99735 + 55.18696 * t + -33.69137 * t2 + 7.948387 * t3 + -0.136638 * it2) * (1.0 / kMCO2);
-    const double cpB = (28.98641 + 1.853978 * t + -9.647459 * t2 + 16.63537 * t3 + 0.000117 * it2) * inv_Mb;
-    return m.qv * cpH + m.qc * cpC + m.qb * cpB;
-}
-ATHAD_DEV double dens_Mnw(const MixQ& m, double Mb, double inv_Mb) {
-    const double s = m.qc + m.qb;
-    if (!(s > 0.0)) return Mb;
-    const double is = rcp_nr(s);
-    const double inv = (m.qc * is) * (1.0 / kMCO2) + (m.qb * is) * inv_Mb;
-    return (inv > 0.0) ? rcp_nr(inv) : Mb;
-}
-ATHAD_DEV double dens_p_auto(double T) {
-    if (T >= kTcrit) return kNoSat;
-    return (T >= kTtriple) ? sat_p_sat(T) : sat_p_subl(T);
-}
-// moist lapse rate given q_sat's factor A at (T, p)
-ATHAD_DEV double dens_lapse(double T, double A, double cp, double Rm, double g) {
-    if (!(cp > 0.0) || !(T > 0.0) || !(Rm > 0.0)) return 0.0;
-    if (!(A > 0.0)) return g * rcp_nr(cp);
-    const double L = (T >= kTtriple) ? sat_L_vap(T) : sat_L_vap(T) + 3.337e5;
-    const double num = 1.0 + L * A * rcp_nr(Rm * T);
-    const double den = cp + L * L * A * rcp_nr(kRH2O * T * T);
-    return (den > 0.0) ? g * num * rcp_nr(den) : g * rcp_nr(cp);
-}
-
 template <typename S>
 __global__ __launch_bounds__(128) void k_densities(const MemberDesc* md, int set_water, int evaporate, int gate) {
     const MemberDesc& M = md[blockIdx.z];
