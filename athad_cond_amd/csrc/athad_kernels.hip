@@ -406,9 +406,9 @@ ATHAD_DEV void rk4_dyn(const MemberDesc& M, const S* const* X, long long id, int
 
 // RK4 of the moisture group c, cloud, ice (minmod advection), gr, co2 (central), advected by
 // the recorded stage velocities; cen[0..4] = start values in, final values out.
-template <typename S, typename C, bool CLAMP>
+template <typename S, typename C, bool CLAMP, typename Vel>
 ATHAD_DEV void rk4_moist(const MemberDesc& M, const S* const* X, long long id, const CellGeo<C>& g, C* ba,
-                         const C* rec, C* cen) {
+                         Vel vel, C* cen) {
     const Consts& K = M.K;
     const long long si = (long long)K.jm * K.km, sj = K.km;
     const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
@@ -432,7 +432,8 @@ ATHAD_DEV void rk4_moist(const MemberDesc& M, const S* const* X, long long id, c
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
 #pragma unroll 1
     for (int st = 0; st < 4; st++) {
-        const C u = rec[(3 * st + 0) * LB], v = rec[(3 * st + 1) * LB], w = rec[(3 * st + 2) * LB];
+        C u, v, w;
+        vel(st, u, v, w);
         const C ua = u * g.aR, va = v * g.aT, wa = w * g.aP;
         C r[5];
 #pragma unroll
@@ -450,9 +451,9 @@ ATHAD_DEV void rk4_moist(const MemberDesc& M, const S* const* X, long long id, c
 // RK4 of the turbulence group tke, dis with the k-omega SST closure (RHS_Atm_Turb.cpp:633-721),
 // advected by the recorded stage velocities. cen[0..1] = start values in, final values out;
 // prod and the sources are those of the fourth stage.
-template <typename S, typename C, bool CLAMP>
+template <typename S, typename C, bool CLAMP, typename Vel>
 ATHAD_DEV void rk4_turb(const MemberDesc& M, const S* const* X, long long id, int i, int j, int k,
-                        const CellGeo<C>& g, C* ba, const C* rec, C A, C B, C* cen, C& prod, C& tsrc, C& dsrc) {
+                        const CellGeo<C>& g, C* ba, Vel vel, C A, C B, C* cen, C& prod, C& tsrc, C& dsrc) {
     const Consts& K = M.K;
     const long long si = (long long)K.jm * K.km, sj = K.km;
     const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
@@ -490,7 +491,8 @@ ATHAD_DEV void rk4_turb(const MemberDesc& M, const S* const* X, long long id, in
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
 #pragma unroll 1
     for (int st = 0; st < 4; st++) {
-        const C u = rec[(3 * st + 0) * LB], v = rec[(3 * st + 1) * LB], w = rec[(3 * st + 2) * LB];
+        C u, v, w;
+        vel(st, u, v, w);
         const C tke = cen[0], dis = cen[1];
         // Stage inputs satisfy dis >= 1e-10 and tke >= 0 (start values pass the solid-ground
         // clamp, stage outputs are floored), so max(dis, 1e-10) == dis and one reciprocal
@@ -569,17 +571,20 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const S* const* X, long long id, in
         gst(fp<S>(M, F_AUX_V), id, S(aux[1]));
         gst(fp<S>(M, F_AUX_W), id, S(aux[2]));
     }
+    auto vel = [&](int st, C& u, C& v, C& w) {
+        u = rec[(3 * st + 0) * LB]; v = rec[(3 * st + 1) * LB]; w = rec[(3 * st + 2) * LB];
+    };
     {
         C cen[5];
         group_start<S, C, CLAMP>(M, X, id, FC, 5, cen, ba);
-        rk4_moist<S, C, CLAMP>(M, X, id, g, ba, rec, cen);
+        rk4_moist<S, C, CLAMP>(M, X, id, g, ba, vel, cen);
 #pragma unroll
         for (int q = 0; q < 5; q++) out(FC + q, cen[q]);
     }
     {
         C cen[2], prod, ts, ds;
         group_start<S, C, CLAMP>(M, X, id, FK, 2, cen, ba);
-        rk4_turb<S, C, CLAMP>(M, X, id, i, j, k, g, ba, rec, A, B, cen, prod, ts, ds);
+        rk4_turb<S, C, CLAMP>(M, X, id, i, j, k, g, ba, vel, A, B, cen, prod, ts, ds);
         out(FK, cen[0]);
         out(FD, cen[1]);
         gst(fp<S>(M, F_PROD), id, S(prod));
@@ -1940,6 +1945,7 @@ void launch_level_mean(const Batch& b) {
 }
 void launch_rk4_jacobi(const Batch& b, bool fused) {
     const dim3 g(cells_grid(b, 256), 1, b.nmem);
+
     if (b.fp32) {
         if (fused) hipLaunchKernelGGL((k_rk4_jacobi<float, true>), g, dim3(256), 0, b.stream, b.d_md);
         else hipLaunchKernelGGL((k_rk4_jacobi<float, false>), g, dim3(256), 0, b.stream, b.d_md);
