@@ -170,3 +170,44 @@ def test_damp_tiles_give_identical_results(oracle):
                     compare(integ, ref, ["t", "ice", "c", "cloud"], 1e-12, "damp tile")
                 for n in out:
                     assert np.array_equal(out[n], first[n]), (shape, levels, n)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_step_run_and_graph_agree(precision):
+    """Iterations issued one call at a time, as one eager run and as graph replays give the same
+    bits (the library overlaps work of consecutive iterations on two streams and joins before
+    every call returns)."""
+    print(f"case step/run/graph {precision}", flush=True)
+    grid = (15, 31, 61)
+    outs = []
+    for mode in ("step", "eager", "graph"):
+        with A.Integration(grid, precision=precision) as integ:
+            integ.init_state(21)
+            if mode == "step":
+                for _ in range(11):
+                    integ.step()
+            else:
+                integ.run(11, graph=(mode == "graph"))
+            assert integ.get_iter() == (11, 11)
+            outs.append({n: integ.get(n) for n in A.PROGNOSTIC + ["p_dyn", "nue", "un", "vn", "wn"]})
+    for n in outs[0]:
+        assert np.array_equal(outs[0][n], outs[1][n]), ("step vs eager", n)
+        assert np.array_equal(outs[0][n], outs[2][n]), ("step vs graph", n)
+
+
+def test_capi_errors_on_a_live_context():
+    import ctypes
+    L = A._lib.lib()
+    with A.Integration(SMALL) as integ:
+        h = integ.h
+        d = ctypes.c_double()
+        assert L.athad_field_stats(h, 5, 0, ctypes.byref(d), ctypes.byref(d)) == 1      # member
+        assert L.athad_field_stats(h, 0, 99, ctypes.byref(d), ctypes.byref(d)) == 1     # field
+        assert L.athad_set_member_params(h, 3, None) == 1
+        assert L.athad_set_damp_tile(h, 99, 8) == 1
+        assert L.athad_get_field(h, 0, -1, None) == 1
+        assert b"out of range" in L.athad_last_error(h)
+        integ.init_state(1)
+        s, q = integ.field_stats("t")
+        t = integ.get("t")
+        assert abs(s - t.sum()) <= 1e-12 * abs(t.sum()) and abs(q - (t * t).sum()) <= 1e-12 * (t * t).sum()
