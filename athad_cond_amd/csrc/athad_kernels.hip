@@ -7,6 +7,11 @@
 // (FP64 drop-in) or float (mixed form); p_stat is always double, and the saturation,
 // column and Newton arithmetic is always double.
 //
+// Kernels take the descriptor array as `const MemberDesc* __restrict__`: no kernel writes
+// it, and without the qualifier the compiler must assume that a field store may change a
+// descriptor, so it reloads descriptor fields after every store with vector loads whose
+// wait also drains every load in flight.
+//
 // Each kernel cites the reference code it restates.
 #include "athad_device.h"
 #include "athad_kernels.h"
@@ -23,6 +28,20 @@ ATHAD_DEV long long cell_id(const Consts& K) {
     const long long b = xcd_block(blockIdx.x, gridDim.x, nb);
     if (b < 0) return -1;
     const long long id = b * blockDim.x + threadIdx.x;
+    return (id < K.n) ? id : -1;
+}
+
+// 1-D cell index for a pointwise kernel whose cost per cell varies strongly with the level
+// (the saturation adjustment: mixed-phase cells of the upper levels run all 20 Newton
+// iterations, most others stop after 4 or 5). No XCD remap: the hardware deals consecutive
+// blocks round-robin over the XCDs, so every XCD gets an equal share of every level, where
+// the remap would hand whole levels, and so all of the slow ones, to one or two XCDs. The
+// blocks run from the top level down, so the slow cells start first and the cheap lower
+// levels fill in behind them. -1 when out of range.
+ATHAD_DEV long long cell_id_top_down(const Consts& K) {
+    const long long nb = (K.n + blockDim.x - 1) / blockDim.x;
+    if ((long long)blockIdx.x >= nb) return -1;
+    const long long id = (nb - 1 - (long long)blockIdx.x) * blockDim.x + threadIdx.x;
     return (id < K.n) ? id : -1;
 }
 
@@ -63,7 +82,7 @@ constexpr int kLevelChunks = 32;
 
 // pass 1: block (chunk c, level i) sums a fixed contiguous chunk of the level in a fixed tree
 template <typename S>
-__global__ __launch_bounds__(256) void k_level_mean(const MemberDesc* md) {
+__global__ __launch_bounds__(256) void k_level_mean(const MemberDesc* __restrict__ md) {
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const int i = blockIdx.y, c = blockIdx.x;
@@ -93,7 +112,7 @@ __global__ __launch_bounds__(256) void k_level_mean(const MemberDesc* md) {
 }
 
 // pass 2: ordered sum of the chunks
-__global__ void k_level_mean_final(const MemberDesc* md) {
+__global__ void k_level_mean_final(const MemberDesc* __restrict__ md) {
     const MemberDesc& M = md[blockIdx.z];
     const int i = threadIdx.x + blockIdx.x * blockDim.x;
     if (i >= M.K.im) return;
@@ -125,7 +144,7 @@ constexpr int kStatChunks = 256;
 ATHAD_DEV int stat_base(const Consts& K) { return 2 * kLevelChunks * K.im; }
 
 template <typename S>
-__global__ __launch_bounds__(256) void k_field_stats(const MemberDesc* md, int member, int field) {
+__global__ __launch_bounds__(256) void k_field_stats(const MemberDesc* __restrict__ md, int member, int field) {
     const MemberDesc& M = md[member];
     const Consts& K = M.K;
     const long long n = K.n;
@@ -152,7 +171,7 @@ __global__ __launch_bounds__(256) void k_field_stats(const MemberDesc* md, int m
     }
 }
 
-__global__ void k_field_stats_final(const MemberDesc* md, int member) {
+__global__ void k_field_stats_final(const MemberDesc* __restrict__ md, int member) {
     const MemberDesc& M = md[member];
     double a = 0.0, b = 0.0;
     const double* r = M.scratch + stat_base(M.K);
@@ -598,7 +617,7 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const S* const* X, long long id, in
 // solid-ground clamps are applied on read (FUSED=1), and the store of the eight fields that
 // no later filter touches is fused (xn = new value).
 template <typename S, bool FUSED>
-__global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>()))) void k_rk4_jacobi(const MemberDesc* md) {
+__global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>()))) void k_rk4_jacobi(const MemberDesc* __restrict__ md) {
     using C = S;
     __shared__ C s_ba[10 * LB], s_rec[12 * LB];
     const MemberDesc& M = md[blockIdx.z];
@@ -646,7 +665,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
 // plane only read neighbours on planes d-1 (final) and d+1 (not yet updated), which is
 // exactly what the reference's single-threaded sweep sees.
 template <typename S>
-__global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>()))) void k_rk4_lex_plane(const MemberDesc* md, int d) {
+__global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>()))) void k_rk4_lex_plane(const MemberDesc* __restrict__ md, int d) {
     using C = S;
     __shared__ C s_ba[10 * LB], s_rec[12 * LB];
     const MemberDesc& M = md[blockIdx.z];
@@ -669,7 +688,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
 // Poisson red-black sweep (PressureSolverAtm.h:106-121, :225-486, :504-594)
 // ------------------------------------------------------------------------------------
 template <typename S>
-__global__ void k_aux_bc(const MemberDesc* md, int gate) {
+__global__ void k_aux_bc(const MemberDesc* __restrict__ md, int gate) {
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const Consts& K = M.K;
@@ -684,7 +703,7 @@ __global__ void k_aux_bc(const MemberDesc* md, int gate) {
 }
 
 template <typename S>
-__global__ __launch_bounds__(256) void k_poisson_colour(const MemberDesc* md, int colour, int gate) {
+__global__ __launch_bounds__(256) void k_poisson_colour(const MemberDesc* __restrict__ md, int colour, int gate) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
@@ -727,7 +746,7 @@ __global__ __launch_bounds__(256) void k_poisson_colour(const MemberDesc* md, in
 // (PressureSolverAtm.h:504-552). Only boundary cells are written and only interior cells
 // are read, so one launch applies all three in the reference's order.
 template <typename S>
-__global__ void k_p_bc(const MemberDesc* md, int gate) {
+__global__ void k_p_bc(const MemberDesc* __restrict__ md, int gate) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
@@ -747,7 +766,7 @@ __global__ void k_p_bc(const MemberDesc* md, int gate) {
 }
 
 template <typename S>
-__global__ void k_p_ceiling(const MemberDesc* md, int gate) {
+__global__ void k_p_ceiling(const MemberDesc* __restrict__ md, int gate) {
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const long long id = cell_id(M.K);
@@ -921,11 +940,11 @@ ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi
 }
 
 template <typename S>
-__global__ __launch_bounds__(256) void k_satadj(const MemberDesc* md, int gate, int to_tmp) {
+__global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ md, int gate, int to_tmp) {
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const Consts& K = M.K;
-    const long long id = cell_id(K);
+    const long long id = cell_id_top_down(K);
     if (id < 0) return;
     int i, j, k;
     decode(K, id, i, j, k);
@@ -1036,7 +1055,7 @@ template <typename S> ATHAD_DEV S* job_ptr(const MemberDesc& M, int q) {
 }
 
 template <typename S, int AXIS>
-__global__ __launch_bounds__(256) void k_damp_pass(const MemberDesc* md, DampJob job, int gate) {
+__global__ __launch_bounds__(256) void k_damp_pass(const MemberDesc* __restrict__ md, DampJob job, int gate) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
@@ -1092,7 +1111,7 @@ __global__ __launch_bounds__(256) void k_damp_pass(const MemberDesc* md, DampJob
 // buffers: neighbouring tiles still read src). Results equal the separate passes bit for bit
 // for every tile shape, so the shape is free for an autotuner to choose.
 template <typename S, int TJ, int TK, int NF>
-__global__ __launch_bounds__(TJ * TK) void k_damp_tile(const MemberDesc* md, DampJob job, int ich, int gate) {
+__global__ __launch_bounds__(TJ * TK) void k_damp_tile(const MemberDesc* __restrict__ md, DampJob job, int ich, int gate) {
     using C = S;
     constexpr int W = TK + 2, SL = (TJ + 2) * W, NT = TJ * TK;
     __shared__ C slab[NF * SL];
@@ -1183,7 +1202,7 @@ __global__ __launch_bounds__(TJ * TK) void k_damp_tile(const MemberDesc* md, Dam
 
 // Copies up to six buffers (src[q] -> dst[q]).
 template <typename S>
-__global__ void k_copy_job(const MemberDesc* md, DampJob job) {
+__global__ void k_copy_job(const MemberDesc* __restrict__ md, DampJob job) {
     const MemberDesc& M = md[blockIdx.z];
     const long long id = cell_id(M.K);
     if (id < 0) return;
@@ -1196,7 +1215,7 @@ __global__ void k_copy_job(const MemberDesc* md, DampJob job) {
 // (j,k) column, serial upward in i; FP64 arithmetic. set_water: initial profile mode.
 // ------------------------------------------------------------------------------------
 template <typename S>
-__global__ __launch_bounds__(128) void k_densities(const MemberDesc* md, int set_water, int evaporate, int gate) {
+__global__ __launch_bounds__(128) void k_densities(const MemberDesc* __restrict__ md, int set_water, int evaporate, int gate) {
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const Consts& K = M.K;
@@ -1288,7 +1307,7 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* md, int set
 // without ordering. Outputs go to scratch q0..q5 = tke, dis, nue, prod, tke_src, dis_src.
 // ------------------------------------------------------------------------------------
 template <typename S>
-__global__ void k_vel_star(const MemberDesc* md, int gate) {
+__global__ void k_vel_star(const MemberDesc* __restrict__ md, int gate) {
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const Consts& K = M.K;
@@ -1302,7 +1321,7 @@ __global__ void k_vel_star(const MemberDesc* md, int gate) {
 }
 
 template <typename S>
-__global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* md, int gate) {
+__global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* __restrict__ md, int gate) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
@@ -1437,7 +1456,7 @@ __global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* md, int 
 // on memory round trips); the final values equal those of the reference's sequence of
 // assignments.
 template <typename S>
-__global__ void k_bc_radial(const MemberDesc* md) {
+__global__ void k_bc_radial(const MemberDesc* __restrict__ md) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
@@ -1513,7 +1532,7 @@ constexpr int kThetaF[18] = {F_T, F_P_STAT, F_R_HUMID, F_R_DRY, F_U, F_V, F_W, F
 // bcTheta (BC_Atm.h:746-917): the pole rows copy their neighbours. One thread per
 // (field, level, longitude).
 template <typename S>
-__global__ void k_bc_theta(const MemberDesc* md) {
+__global__ void k_bc_theta(const MemberDesc* __restrict__ md) {
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const long long nik = (long long)K.im * K.km;
@@ -1553,7 +1572,7 @@ template <typename C> ATHAD_DEV void seam_passes(C e, C& a, C& b, C& c, C f) {
 }
 
 template <typename S>
-__global__ void k_bc_phi(const MemberDesc* md) {
+__global__ void k_bc_phi(const MemberDesc* __restrict__ md) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
@@ -1597,7 +1616,7 @@ __global__ void k_bc_phi(const MemberDesc* md) {
 // bcSolidGround passes 4-5 (used by the lexicographic form and the initial state; the
 // Jacobi form applies the same clamps inside k_rk4_jacobi).
 template <typename S>
-__global__ void k_solid_clamp(const MemberDesc* md) {
+__global__ void k_solid_clamp(const MemberDesc* __restrict__ md) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
@@ -1632,7 +1651,7 @@ __global__ void k_solid_clamp(const MemberDesc* md) {
 // along k, staged in LDS. One block per row.
 // ------------------------------------------------------------------------------------
 template <typename S>
-__global__ __launch_bounds__(256) void k_polar(const MemberDesc* md, int fbits) {
+__global__ __launch_bounds__(256) void k_polar(const MemberDesc* __restrict__ md, int fbits) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
@@ -1679,7 +1698,7 @@ __global__ __launch_bounds__(256) void k_polar(const MemberDesc* md, int fbits) 
 // v, w (4th order, 1-2-1 next to the wall/lid), one thread per (j,k) column with the column
 // in LDS; optionally the store xn <- x of UtilsAtm.h:25-49 for u, v, w.
 template <typename S>
-__global__ __launch_bounds__(64) void k_radial_store(const MemberDesc* md, int store) {
+__global__ __launch_bounds__(64) void k_radial_store(const MemberDesc* __restrict__ md, int store) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
@@ -1723,7 +1742,7 @@ __global__ __launch_bounds__(64) void k_radial_store(const MemberDesc* md, int s
 // values of levels i-1, i-2, so every level of a pass depends only on the previous pass and
 // all levels are computed at once (two LDS buffers). Results equal the column kernel's.
 template <typename S, int RK>
-__global__ __launch_bounds__(256) void k_radial_tile(const MemberDesc* md, int store) {
+__global__ __launch_bounds__(256) void k_radial_tile(const MemberDesc* __restrict__ md, int store) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
@@ -1781,7 +1800,7 @@ __global__ __launch_bounds__(256) void k_radial_tile(const MemberDesc* md, int s
 
 // UtilsAtm::storeIntermediateData3D (UtilsAtm.h:25-49): xn <- x for the fields selected by mask.
 template <typename S>
-__global__ void k_store(const MemberDesc* md, int mask) {
+__global__ void k_store(const MemberDesc* __restrict__ md, int mask) {
     const MemberDesc& M = md[blockIdx.z];
     const long long id = cell_id(M.K);
     if (id < 0) return;
@@ -1795,7 +1814,7 @@ __global__ void k_store(const MemberDesc* md, int mask) {
 // co2Atmosphere (ThermoAtm.h:873-913), seeded N-S mirrored velocity perturbation.
 // ------------------------------------------------------------------------------------
 template <typename S>
-__global__ __launch_bounds__(128) void k_init_column(const MemberDesc* md) {
+__global__ __launch_bounds__(128) void k_init_column(const MemberDesc* __restrict__ md) {
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const double t_eq_K = K.t_surf_eq, t_pole_K = K.t_surf_pole;
@@ -1833,7 +1852,7 @@ __global__ __launch_bounds__(128) void k_init_column(const MemberDesc* md) {
 }
 
 template <typename S>
-__global__ __launch_bounds__(128) void k_init_water(const MemberDesc* md) {
+__global__ __launch_bounds__(128) void k_init_water(const MemberDesc* __restrict__ md) {
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const double M_nw = K.init_M_nw;
@@ -1852,7 +1871,7 @@ __global__ __launch_bounds__(128) void k_init_water(const MemberDesc* md) {
 }
 
 template <typename S>
-__global__ void k_init_co2(const MemberDesc* md) {
+__global__ void k_init_co2(const MemberDesc* __restrict__ md) {
     const MemberDesc& M = md[blockIdx.z];
     const long long id = cell_id(M.K);
     if (id < 0) return;
@@ -1860,7 +1879,7 @@ __global__ void k_init_co2(const MemberDesc* md) {
 }
 
 template <typename S>
-__global__ void k_copy_lid(const MemberDesc* md) {
+__global__ void k_copy_lid(const MemberDesc* __restrict__ md) {
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1876,7 +1895,7 @@ ATHAD_DEV uint64_t splitmix64(uint64_t x) {
 }
 
 template <typename S>
-__global__ void k_init_rest(const MemberDesc* md) {
+__global__ void k_init_rest(const MemberDesc* __restrict__ md) {
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const unsigned long long seed = K.seed;
@@ -1904,7 +1923,7 @@ __global__ void k_init_rest(const MemberDesc* md) {
 }
 
 template <typename S>
-__global__ void k_copy_field(const MemberDesc* md, int dst, int src) {
+__global__ void k_copy_field(const MemberDesc* __restrict__ md, int dst, int src) {
     const MemberDesc& M = md[blockIdx.z];
     const long long id = cell_id(M.K);
     if (id < 0) return;
