@@ -806,85 +806,90 @@ template <typename N> ATHAD_DEV N sat_L_vap(N T) {
     if (T >= N(kTcrit)) return N(0);
     return N(2.501e6) * exp(N(0.38) * log((N(kTcrit) - T) * N(1.0 / (kTcrit - 273.15))));
 }
-ATHAD_DEV double sat_p_sat(double T) {
-    if (T >= kTcrit) return kNoSat;
-    if (T <= 0.0) return 0.0;
-    const double th = 1.0 - T * (1.0 / kTcrit);
-    const double sq = sqrt(th);
-    const double t3 = th * th * th;
-    const double t4 = t3 * th;
-    const double s = -7.85951783 * th + 1.84408259 * (th * sq) + -11.7866497 * t3
-                   + 22.6807411 * (t3 * sq) + -15.9618719 * t4 + 1.80122502 * (t4 * t3 * sq);
-    return kPcrit * exp((kTcrit * rcp_nr(T)) * s);
+template <typename A> ATHAD_DEV A sat_p_sat(A T) {
+    if (T >= A(kTcrit)) return A(kNoSat);
+    if (T <= A(0)) return A(0);
+    const A th = A(1) - T * A(1.0 / kTcrit);
+    const A sq = sqrt(th);
+    const A t3 = th * th * th;
+    const A t4 = t3 * th;
+    const A s = A(-7.85951783) * th + A(1.84408259) * (th * sq) + A(-11.7866497) * t3
+              + A(22.6807411) * (t3 * sq) + A(-15.9618719) * t4 + A(1.80122502) * (t4 * t3 * sq);
+    return A(kPcrit) * exp((A(kTcrit) * rcp_nr(T)) * s);
 }
-ATHAD_DEV double sat_p_subl(double T) {
-    if (T <= 0.0) return 0.0;
-    const double th = T * (1.0 / kTtriple);
-    const double lth = log(th);
-    const double s = -0.212144006e2 * exp(0.333333333e-2 * lth) + 0.273203819e2 * exp(0.120666667e1 * lth)
-                   + -0.610598130e1 * exp(0.170333333e1 * lth);
-    return kPtriple * exp(s * rcp_nr(th));
+template <typename A> ATHAD_DEV A sat_p_subl(A T) {
+    if (T <= A(0)) return A(0);
+    const A th = T * A(1.0 / kTtriple);
+    const A lth = log(th);
+    const A s = A(-0.212144006e2) * exp(A(0.333333333e-2) * lth) + A(0.273203819e2) * exp(A(0.120666667e1) * lth)
+              + A(-0.610598130e1) * exp(A(0.170333333e1) * lth);
+    return A(kPtriple) * exp(s * rcp_nr(th));
 }
-// exact q_sat (MixtureAtm.h:152-217) and the factor A of dq_sat/dT = A L / (R_v T^2)
-ATHAD_DEV void sat_qA(double E, double p, double inv_p, double Mo, double& q, double& A) {
-    A = 0.0;
-    if (E == kNoSat) { q = 1.0; return; }
-    if (!(p > 0.0)) { q = 0.0; return; }
-    if (E >= p) { q = 1.0; return; }
-    const double x = E * inv_p;
-    const double num = x * kMH2O;
-    const double den = num + (1.0 - x) * Mo;
-    if (!(den > 0.0)) { q = 1.0; return; }
-    const double rd = rcp_nr(den);
+// exact q_sat (MixtureAtm.h:152-217) and the factor Af of dq_sat/dT = Af L / (R_v T^2)
+template <typename A> ATHAD_DEV void sat_qA(A E, A p, A inv_p, A Mo, A& q, A& Af) {
+    Af = A(0);
+    if (E == A(kNoSat)) { q = A(1); return; }
+    if (!(p > A(0))) { q = A(0); return; }
+    if (E >= p) { q = A(1); return; }
+    const A x = E * inv_p;
+    const A num = x * A(kMH2O);
+    const A den = num + (A(1) - x) * Mo;
+    if (!(den > A(0))) { q = A(1); return; }
+    const A rd = rcp_nr(den);
     q = num * rd;
-    A = num * Mo * rd * rd;
+    Af = num * Mo * rd * rd;
 }
 
 // Column and cell thermodynamics (k_densities, the SatAdj entry and clamp): th_R_of, th_cp_of, th_M_nonwater, th_q_sat and
 // th_moist_lapse_E restated with shared sub-expressions (one split, one 1/t^2 for the three
 // Shomate fits, reciprocals of repeated denominators) and refined reciprocals, which shortens
 // the serial per-level chain of the column recursion; results agree with th_* to a few ulp.
-struct MixQ { double qv, qc, qb; };
-ATHAD_DEV MixQ dens_split(double c, double co2) {
-    double qv = tmin(tmax(c, 0.0), 1.0), qc = tmin(tmax(co2, 0.0), 1.0);
-    const double s = qv + qc;
-    if (s > 1.0) { qv /= s; qc /= s; }
-    return {qv, qc, tmax(0.0, 1.0 - qv - qc)};
+// Templated on the arithmetic type A: double everywhere except the mixed form's column.
+template <typename A> struct MixQ { A qv, qc, qb; };
+template <typename A> ATHAD_DEV MixQ<A> dens_split(A c, A co2) {
+    A qv = tmin(tmax(c, A(0)), A(1)), qc = tmin(tmax(co2, A(0)), A(1));
+    const A s = qv + qc;
+    if (s > A(1)) { qv /= s; qc /= s; }
+    return {qv, qc, tmax(A(0), A(1) - qv - qc)};
 }
-ATHAD_DEV double dens_R(const MixQ& m, double Rb) { return m.qv * kRH2O + m.qc * kRCO2 + m.qb * Rb; }
-ATHAD_DEV double dens_cp(const MixQ& m, double T, double inv_Mb) {
-    const double t = tmin(tmax(T, 298.0), 2000.0) * 1.0e-3;
-    const double t2 = t * t, t3 = t2 * t, it2 = rcp_nr(t2);
-    const double cpH = (30.09200 + 6.832514 * t + 6.793435 * t2 + -2.534480 * t3 + 0.082139 * it2) * (1.0 / kMH2O);
-    const double cpC = (24.99735 + 55.18696 * t + -33.69137 * t2 + 7.948387 * t3 + -0.136638 * it2) * (1.0 / kMCO2);
-    const double cpB = (28.98641 + 1.853978 * t + -9.647459 * t2 + 16.63537 * t3 + 0.000117 * it2) * inv_Mb;
+template <typename A> ATHAD_DEV A dens_R(const MixQ<A>& m, A Rb) { return m.qv * A(kRH2O) + m.qc * A(kRCO2) + m.qb * Rb; }
+template <typename A> ATHAD_DEV A dens_cp(const MixQ<A>& m, A T, A inv_Mb) {
+    const A t = tmin(tmax(T, A(298)), A(2000)) * A(1.0e-3);
+    const A t2 = t * t, t3 = t2 * t, it2 = rcp_nr(t2);
+    const A cpH = (A(30.09200) + A(6.832514) * t + A(6.793435) * t2 + A(-2.534480) * t3 + A(0.082139) * it2) * A(1.0 / kMH2O);
+    const A cpC = (A(24.99735) + A(55.18696) * t + A(-33.69137) * t2 + A(7.948387) * t3 + A(-0.136638) * it2) * A(1.0 / kMCO2);
+    const A cpB = (A(28.98641) + A(1.853978) * t + A(-9.647459) * t2 + A(16.63537) * t3 + A(0.000117) * it2) * inv_Mb;
     return m.qv * cpH + m.qc * cpC + m.qb * cpB;
 }
-ATHAD_DEV double dens_Mnw(const MixQ& m, double Mb, double inv_Mb) {
-    const double s = m.qc + m.qb;
-    if (!(s > 0.0)) return Mb;
-    const double is = rcp_nr(s);
-    const double inv = (m.qc * is) * (1.0 / kMCO2) + (m.qb * is) * inv_Mb;
-    return (inv > 0.0) ? rcp_nr(inv) : Mb;
+template <typename A> ATHAD_DEV A dens_Mnw(const MixQ<A>& m, A Mb, A inv_Mb) {
+    const A s = m.qc + m.qb;
+    if (!(s > A(0))) return Mb;
+    const A is = rcp_nr(s);
+    const A inv = (m.qc * is) * A(1.0 / kMCO2) + (m.qb * is) * inv_Mb;
+    return (inv > A(0)) ? rcp_nr(inv) : Mb;
 }
-ATHAD_DEV double dens_p_auto(double T) {
-    if (T >= kTcrit) return kNoSat;
-    return (T >= kTtriple) ? sat_p_sat(T) : sat_p_subl(T);
+template <typename A> ATHAD_DEV A dens_p_auto(A T) {
+    if (T >= A(kTcrit)) return A(kNoSat);
+    return (T >= A(kTtriple)) ? sat_p_sat(T) : sat_p_subl(T);
 }
-// moist lapse rate given q_sat's factor A at (T, p)
-ATHAD_DEV double dens_lapse(double T, double A, double cp, double Rm, double g) {
-    if (!(cp > 0.0) || !(T > 0.0) || !(Rm > 0.0)) return 0.0;
-    if (!(A > 0.0)) return g * rcp_nr(cp);
-    const double L = (T >= kTtriple) ? sat_L_vap(T) : sat_L_vap(T) + 3.337e5;
-    const double num = 1.0 + L * A * rcp_nr(Rm * T);
-    const double den = cp + L * L * A * rcp_nr(kRH2O * T * T);
-    return (den > 0.0) ? g * num * rcp_nr(den) : g * rcp_nr(cp);
+// moist lapse rate given q_sat's factor Af at (T, p)
+template <typename A> ATHAD_DEV A dens_lapse(A T, A Af, A cp, A Rm, A g) {
+    if (!(cp > A(0)) || !(T > A(0)) || !(Rm > A(0))) return A(0);
+    if (!(Af > A(0))) return g * rcp_nr(cp);
+    const A L = (T >= A(kTtriple)) ? sat_L_vap(T) : sat_L_vap(T) + A(3.337e5);
+    const A num = A(1) + L * Af * rcp_nr(Rm * T);
+    const A den = cp + L * L * Af * rcp_nr(A(kRH2O) * T * T);
+    return (den > A(0)) ? g * num * rcp_nr(den) : g * rcp_nr(cp);
 }
 
 // Damped Newton iteration of adjustSaturation (SaturationAdjustment.h:177-254) in type N.
-// The ice branch (sublimation pressure, its q_sat and derivative) is evaluated only when it
-// can contribute: deposition fraction > 0, ice present, or the no-condensate target is ice;
-// otherwise every term it feeds is multiplied by an exact zero. Likewise the liquid branch.
+// The ice branch (sublimation pressure, its q_sat and derivative) is evaluated only when its
+// result is read: deposition fraction > 0 (the gain term), or the target uses q_ice (ice
+// present with condensate above 1e-12, or the no-condensate target when T < t_0). Otherwise
+// every term it feeds is an exact zero: a zero deposition fraction times a finite factor, or
+// zero ice times a finite q_ice. Likewise the liquid branch. (Cells that keep a trace of ice
+// below the condensate threshold, above t_0, run all 20 iterations and need only the liquid
+// branch.)
 template <typename N>
 ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi_io, double co2d, double p_local,
                           double M_other, double Mb_d, double t_00d, double t_0d) {
@@ -916,8 +921,8 @@ ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi
         Lv = sat_L_vap(T);
         const N qsum = qcb + qib;
         const bool hot = (T >= t_0), some = (qsum > N(1e-12));
-        const bool need_l = (CND > N(0)) || (qcb > N(0)) || (!some && hot);
-        const bool need_i = (DEP > N(0)) || (qib > N(0)) || (!some && !hot);
+        const bool need_l = (CND > N(0)) || (some ? (qcb > N(0)) : hot);
+        const bool need_i = (DEP > N(0)) || (some ? (qib > N(0)) : !hot);
         double qs = 0.0, Ac = 0.0, qi = 0.0, Ad = 0.0;
         if (need_l) sat_qA(sat_p_sat((double)T), p_local, inv_p, M_other, qs, Ac);
         if (need_i) sat_qA(sat_p_subl((double)T), p_local, inv_p, M_other, qi, Ad);
@@ -953,6 +958,9 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
     const double p_local = pstat(M)[id];
     double t = pt[id], c = pc[id], cloud = pcl[id], ice = pi[id], gr = pg[id];
     double scc = 0.0;
+    // the entry test's saturation pressure and fade factor, and the temperature they were taken
+    // at: the clamp below reuses them when its temperature is bitwise the same
+    double E_entry = 0.0, alpha_entry = 0.0, T_entry = __builtin_nan("");   // NaN: equal to nothing
     const double t_0 = K.t_0, t_00 = K.t_00, Mb = K.M_bg, inv_Mb = 1.0 / K.M_bg;
     const double inv_t_0 = 1.0 / t_0;
     const double inv_p = (p_local > 0.0) ? rcp_nr(p_local) : 0.0;
@@ -964,8 +972,10 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
         if (T < kTcrit) {
             const double M_other = dens_Mnw(dens_split(c, co2), Mb, inv_Mb);
             double qs0, A0;
-            sat_qA(dens_p_auto(T), p_local, inv_p, M_other, qs0, A0);
+            E_entry = dens_p_auto(T);
+            sat_qA(E_entry, p_local, inv_p, M_other, qs0, A0);
             const double alpha = rcp_nr(1.0 + exp(-(T - t_00) * 0.2));
+            T_entry = T; alpha_entry = alpha;
             if ((qv_old > qs0 && alpha > 0.01) || (qv_old < qs0 && (qc_old > 1e-12 || qi_old > 1e-12))) {
                 double qvb = qv_old, qcb = qc_old, qib = qi_old;
                 const double T_orig = t * t_0;
@@ -995,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
         if (Td > K.t_max_phys) { Td = K.t_max_phys; t = K.t_max_phys * inv_t_0; }
         const double M_o = dens_Mnw(dens_split(c, co2), Mb, inv_Mb);
         double q_s, A_s;
-        sat_qA(dens_p_auto(Td), p_local, inv_p, M_o, q_s, A_s);
+        sat_qA((Td == T_entry) ? E_entry : dens_p_auto(Td), p_local, inv_p, M_o, q_s, A_s);
         if (Td >= kTcrit || q_s >= 1.0) {
             const double q_cond = tmax(0.0, cloud) + tmax(0.0, ice) + tmax(0.0, gr);
             if (q_cond > 0.0) {
@@ -1026,7 +1036,7 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
             }
             if (cloud > K.cloud_cap) cloud = K.cloud_cap;
             if (ice > K.cloud_cap) ice = K.cloud_cap;
-            const double al = rcp_nr(1.0 + exp(-(Td - t_00) * 0.2));
+            const double al = (Td == T_entry) ? alpha_entry : rcp_nr(1.0 + exp(-(Td - t_00) * 0.2));
             c *= al; cloud *= al; ice *= al;
         }
     }
@@ -1212,10 +1222,14 @@ __global__ void k_copy_job(const MemberDesc* __restrict__ md, DampJob job) {
 // ------------------------------------------------------------------------------------
 // Column kernel: ThermoAtm::densities (ThermoAtm.h:1436-1614) followed by the sea-surface
 // saturation boundary of waterVapourEvaporation (ThermoAtm.h:115-258). One thread per
-// (j,k) column, serial upward in i; FP64 arithmetic. set_water: initial profile mode.
+// (j,k) column, serial upward in i. set_water: initial profile mode. The column arithmetic
+// is in the storage type: FP64 in the drop-in form; FP32 in the mixed form, where the
+// FP64 library exp/log on the serial per-level chain cost more than the rest of the kernel
+// (p_stat is stored in FP64 either way). The evaporation tail is FP64 in both forms.
 // ------------------------------------------------------------------------------------
 template <typename S>
 __global__ __launch_bounds__(128) void k_densities(const MemberDesc* __restrict__ md, int set_water, int evaporate, int gate) {
+    using A = S;
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const Consts& K = M.K;
@@ -1225,54 +1239,55 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* __restrict_
     S *t = fp<S>(M, F_T), *c = fp<S>(M, F_C), *co2 = fp<S>(M, F_CO2), *cl = fp<S>(M, F_CLOUD), *ice = fp<S>(M, F_ICE),
       *rh = fp<S>(M, F_R_HUMID), *rd = fp<S>(M, F_R_DRY);
     double* ps = pstat(M);
-    const double t_0 = K.t_0, Mb = K.M_bg, Rb = K.R_bg, inv_Mb = 1.0 / K.M_bg;
+    const double Mb = K.M_bg;
+    const A t_0 = A(K.t_0), Mb_a = A(K.M_bg), Rb = A(K.R_bg), inv_Mb = A(1.0 / K.M_bg), g = A(K.g);
     const size_t si = (size_t)K.jm * K.km;
     const size_t base = (size_t)j * K.km + k;
-    double T_prev = tmax(180.0, (double)t[base] * t_0);
-    double p_prev = K.p_sl_factor * T_prev;
-    double q_sat_min = 1.0;
-    double qv_prev = 0.0, qc_prev = 0.0;
+    A T_prev = tmax(A(180), A(t[base]) * t_0);
+    A p_prev = A(K.p_sl_factor) * T_prev;
+    A q_sat_min = A(1);
+    A qv_prev = A(0), qc_prev = A(0);
     // this level's inputs; the next level's are loaded before this level's stores so their
     // latency overlaps the column arithmetic
-    double qv = gld(c, (long long)base), qc = gld(co2, (long long)base);
-    double wcl = gld(cl, (long long)base), wic = gld(ice, (long long)base);
+    A qv = A(gld(c, (long long)base)), qc = A(gld(co2, (long long)base));
+    A wcl = A(gld(cl, (long long)base)), wic = A(gld(ice, (long long)base));
     for (int i = 0; i < K.im; i++) {
         const long long id = (long long)base + (long long)i * (long long)si;
-        double qv_n = 0.0, qc_n = 0.0, cl_n = 0.0, ic_n = 0.0;
+        A qv_n = A(0), qc_n = A(0), cl_n = A(0), ic_n = A(0);
         if (i + 1 < K.im) {
-            qv_n = gld(c, id + (long long)si); qc_n = gld(co2, id + (long long)si);
-            cl_n = gld(cl, id + (long long)si); ic_n = gld(ice, id + (long long)si);
+            qv_n = A(gld(c, id + (long long)si)); qc_n = A(gld(co2, id + (long long)si));
+            cl_n = A(gld(cl, id + (long long)si)); ic_n = A(gld(ice, id + (long long)si));
         }
-        const double R_loc = dens_R(dens_split(qv, qc), Rb);
-        double Ti, pi;
+        const A R_loc = dens_R(dens_split(qv, qc), Rb);
+        A Ti, pi;
         if (i == 0) { Ti = T_prev; pi = p_prev; }
         else {
-            const double dz = M.hgt[i] - M.hgt[i - 1];
-            const MixQ mp = dens_split(qv_prev, qc_prev);
-            const double cp_loc = dens_cp(mp, T_prev, inv_Mb);
-            const double M_nw = dens_Mnw(mp, Mb, inv_Mb);
-            double q_s, A;
-            sat_qA(dens_p_auto(T_prev), p_prev, (p_prev > 0.0) ? rcp_nr(p_prev) : 0.0, M_nw, q_s, A);
-            const double gamma = (qv_prev >= 0.99 * q_s) ? dens_lapse(T_prev, A, cp_loc, R_loc, K.g)
-                                                         : K.g * rcp_nr(cp_loc);
-            const double T_ad = T_prev - gamma * dz;
-            Ti = tmax(K.t_skin, T_ad);
-            const double T_mean = 0.5 * (T_prev + Ti);
-            pi = p_prev * exp(-K.g * dz * rcp_nr(R_loc * T_mean));
+            const A dz = A(M.hgt[i] - M.hgt[i - 1]);
+            const MixQ<A> mp = dens_split(qv_prev, qc_prev);
+            const A cp_loc = dens_cp(mp, T_prev, inv_Mb);
+            const A M_nw = dens_Mnw(mp, Mb_a, inv_Mb);
+            A q_s, Af;
+            sat_qA(dens_p_auto(T_prev), p_prev, (p_prev > A(0)) ? rcp_nr(p_prev) : A(0), M_nw, q_s, Af);
+            const A gamma = (qv_prev >= A(0.99) * q_s) ? dens_lapse(T_prev, Af, cp_loc, R_loc, g)
+                                                       : g * rcp_nr(cp_loc);
+            const A T_ad = T_prev - gamma * dz;
+            Ti = tmax(A(K.t_skin), T_ad);
+            const A T_mean = A(0.5) * (T_prev + Ti);
+            pi = p_prev * exp(-g * dz * rcp_nr(R_loc * T_mean));
         }
         gst(t, id, S(Ti / t_0));
-        gst(ps, id, pi);
+        gst(ps, id, (double)pi);
         if (set_water && i > 0) {
-            const double q_s_i = th_q_sat_at(Ti, pi, th_M_nonwater(qv, qc, Mb));
-            if (q_s_i < q_sat_min) q_sat_min = q_s_i;
-            qv = tmax(q_sat_min, K.c_h2o_dry_top);
+            const double q_s_i = th_q_sat_at((double)Ti, (double)pi, th_M_nonwater((double)qv, (double)qc, Mb));
+            if (A(q_s_i) < q_sat_min) q_sat_min = A(q_s_i);
+            qv = tmax(q_sat_min, A(K.c_h2o_dry_top));
             gst(c, id, S(qv));
         }
-        const double wf = tmax(0.5, 1.0 - (double)S(wcl) - (double)S(wic));
-        gst(rh, id, S(1e2 * pi * rcp_nr(R_loc * Ti * wf)));
-        gst(rd, id, S(1e2 * pi * rcp_nr(dens_R(dens_split(0.0, qc), Rb) * Ti)));
+        const A wf = tmax(A(0.5), A(1) - A(S(wcl)) - A(S(wic)));
+        gst(rh, id, S(A(1e2) * pi * rcp_nr(R_loc * Ti * wf)));
+        gst(rd, id, S(A(1e2) * pi * rcp_nr(dens_R(dens_split(A(0), qc), Rb) * Ti)));
         // the next level reads this level's stored water (the reference reads c[i-1] back)
-        qv_prev = (double)S(qv);
+        qv_prev = A(S(qv));
         qc_prev = qc;
         T_prev = Ti; p_prev = pi;
         qv = qv_n; qc = qc_n; wcl = cl_n; wic = ic_n;
@@ -1280,7 +1295,7 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* __restrict_
     M.t_top[base] = (double)t[base + (size_t)(K.im - 1) * si];
     if (!evaporate) return;
     // waterVapourEvaporation, sea branch
-    const double t0s = (double)t[base] * t_0;
+    const double t0s = (double)t[base] * K.t_0;
     if (t0s >= kTcrit) return;
     const double T_s = tmax(180.0, t0s);
     const double c0 = c[base];
@@ -1291,7 +1306,7 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* __restrict_
         const size_t id = base + (size_t)i * si;
         rp = pow(K.evap_r, (double)i);
         const double weight = rp * K.evap_w_norm;
-        const double T_i = tmax(180.0, (double)t[id] * t_0);
+        const double T_i = tmax(180.0, (double)t[id] * K.t_0);
         const double ci = c[id];
         const double q_s_i = th_q_sat_at(T_i, ps[id], th_M_nonwater(ci, (double)co2[id], Mb));
         c[id] = S(tmin(ci + (q_sea - c0) * weight, q_s_i));
