@@ -786,8 +786,8 @@ __global__ void k_p_ceiling(const MemberDesc* __restrict__ md, int gate) {
 // Newton-loop helpers of the saturation adjustment, templated on the Newton arithmetic type N
 // (double in the drop-in form, float in the mixed form). They restate th_* with shared
 // sub-expressions evaluated once and divisions by positive bounded operands done through
-// refined reciprocals; the saturation vapour pressures, q_sat and its derivative factor are
-// always FP64.
+// refined reciprocals. In the mixed form the whole loop, saturation vapour pressures and q_sat
+// included, runs in FP32; the entry test and the clamp around it stay FP64.
 template <typename N> ATHAD_DEV void sat_species_cp(N T, N Mb, N& cpH, N& cpC, N& cpB) {
     const N t = tmin(tmax(T, N(298)), N(2000)) * N(1.0e-3);
     const N t2 = t * t, t3 = t2 * t, it2 = rcp_nr(t2);
@@ -894,7 +894,8 @@ template <typename N>
 ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi_io, double co2d, double p_local,
                           double M_other, double Mb_d, double t_00d, double t_0d) {
     const N t_00 = N(t_00d), t_0 = N(t_0d), t_range_inv = N(1.0 / (t_0d - t_00d)), co2 = N(co2d), Mb = N(Mb_d);
-    const double inv_p = (p_local > 0.0) ? 1.0 / p_local : 0.0;
+    const double inv_p_d = (p_local > 0.0) ? 1.0 / p_local : 0.0;
+    const N p_n = N(p_local), inv_p = N(inv_p_d), Mo = N(M_other);
     N T = N(T_io), qvb = N(qv_io), qcb = N(qc_io), qib = N(qi_io), qv_hyp = qvb;
     // species heat capacities and L(T) of iteration k+1's update are those of iteration k's
     // new temperature
@@ -923,14 +924,14 @@ ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi
         const bool hot = (T >= t_0), some = (qsum > N(1e-12));
         const bool need_l = (CND > N(0)) || (some ? (qcb > N(0)) : hot);
         const bool need_i = (DEP > N(0)) || (some ? (qib > N(0)) : !hot);
-        double qs = 0.0, Ac = 0.0, qi = 0.0, Ad = 0.0;
-        if (need_l) sat_qA(sat_p_sat((double)T), p_local, inv_p, M_other, qs, Ac);
-        if (need_i) sat_qA(sat_p_subl((double)T), p_local, inv_p, M_other, qi, Ad);
-        const N target = some ? (qcb * N(qs) + qib * N(qi)) * rcp_nr(qsum) : (hot ? N(qs) : N(qi));
+        N qs = N(0), Ac = N(0), qi = N(0), Ad = N(0);
+        if (need_l) sat_qA(sat_p_sat(T), p_n, inv_p, Mo, qs, Ac);
+        if (need_i) sat_qA(sat_p_subl(T), p_n, inv_p, Mo, qi, Ad);
+        const N target = some ? (qcb * qs + qib * qi) * rcp_nr(qsum) : (hot ? qs : qi);
         const N inv_cp = rcp_nr(sat_mix_cp(qvb, co2, cpH, cpC, cpB));
         const N inv_RT2 = (T > N(0)) ? rcp_nr(N(kRH2O) * T * T) : N(0);
         const N Lc = Lv, Ld = Lv + N(3.337e5);
-        const N gain = CND * (Lc * inv_cp) * (N(Ac) * Lc * inv_RT2) + DEP * (Ld * inv_cp) * (N(Ad) * Ld * inv_RT2);
+        const N gain = CND * (Lc * inv_cp) * (Ac * Lc * inv_RT2) + DEP * (Ld * inv_cp) * (Ad * Ld * inv_RT2);
         const N om = rcp_nr(N(1) + gain);
         const N hyp_prev = qv_hyp;
         qv_hyp = qvb + om * (target - qvb);
