@@ -326,15 +326,26 @@ athad_status after_launch(athad_ctx* c) {
 // ---- the operators (all members, no cadence gate unless given) -----------------------
 const int kMoistSrc[4] = {F_T, F_ICE, F_C, F_CLOUD};
 
-// damp_wiggles of t, ice, c, cloud plus the microphysics clamp, from scratch 6..9 (where the
-// saturation adjustment of the same moist block left them) into the fields; from_fields
-// first stages the current fields there (the standalone operator).
-void op_damp_moist(athad_ctx* c, int gate, bool from_fields) {
+// damp_wiggles of t, ice, c, cloud plus the microphysics clamp (the standalone operator):
+// the fields are staged in scratch 6..9 and damped from there back into the fields.
+void op_damp_moist(athad_ctx* c, int gate) {
     Prof p(c, KID_DAMP);
     Batch& b = cur(c);
     const int tmp[4] = {NFIELD + 6, NFIELD + 7, NFIELD + 8, NFIELD + 9};
-    if (from_fields) launch_copy_job(b, kMoistSrc, tmp, 4);
+    launch_copy_job(b, kMoistSrc, tmp, 4);
     launch_damp_tile(b, c->tile_shape, c->tile_ich, tmp, kMoistSrc, 4, 1, gate);
+}
+
+// The moist damping inside the moist block: ice, c and cloud at every level, t at the lowest
+// level only. The densities step that follows recomputes t at every level above from the
+// damped surface value, so t damped above the surface would be discarded.
+void op_damp_moist_in_block(athad_ctx* c, int gate) {
+    Prof p(c, KID_DAMP);
+    Batch& b = cur(c);
+    const int src[3] = {NFIELD + 7, NFIELD + 8, NFIELD + 9};
+    const int dst[3] = {F_ICE, F_C, F_CLOUD};
+    launch_damp_tile(b, c->tile_shape, c->tile_ich, src, dst, 3, 1, gate);
+    launch_damp_level0(b, NFIELD + 6, F_T, gate);
 }
 
 // TurbulenceAtm::run and the damping of its six fields, on stream st (the context's own
@@ -374,7 +385,7 @@ void op_moist_block(athad_ctx* c, int gate, bool poisson_beside = false) {
         Prof p(c, KID_SATADJ);
         launch_satadj(cur(c), gate, 1);
     }
-    op_damp_moist(c, gate, false);
+    op_damp_moist_in_block(c, gate);
     {
         Prof p(c, KID_DENSITIES);
         launch_densities(cur(c), 0, 1, gate);
@@ -901,7 +912,7 @@ athad_status athad_satadj(athad_ctx* c) {
 }
 athad_status athad_damp_moist(athad_ctx* c) {
     if (!c) return ATHAD_ERR_ARG;
-    op_damp_moist(c, 0, true);
+    op_damp_moist(c, 0);
     return after_launch(c);
 }
 athad_status athad_densities(athad_ctx* c, int set_water, int with_evap) {

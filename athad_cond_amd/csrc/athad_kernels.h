@@ -33,8 +33,10 @@ void launch_poisson(const Batch& b, int gate);
 void launch_satadj(const Batch& b, int gate, int to_tmp);
 // pass 0,1,2 over the fields of one job
 void launch_damp(const Batch& b, int axis, const int* src, const int* dst, int nf, int moist_clamp, int gate);
-// Fused three-pass damp_wiggles over nf = 4 or 6 buffers with tile shape `shape` (one of
+// Fused three-pass damp_wiggles over nf = 3, 4 or 6 buffers with tile shape `shape` (one of
 // damp_tile_shapes()) and ich levels per block; src and dst must differ.
+// damp_wiggles of buffer src at level 0 only, into dst (see k_damp_level0)
+void launch_damp_level0(const Batch& b, int src, int dst, int gate);
 int damp_tile_shapes();
 void damp_tile_shape(int shape, int* tj, int* tk);
 void launch_damp_tile(const Batch& b, int shape, int ich, const int* src, const int* dst, int nf, int moist_clamp,

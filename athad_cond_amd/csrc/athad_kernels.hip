@@ -1189,12 +1189,13 @@ __global__ __launch_bounds__(TJ * TK) void k_damp_tile(const MemberDesc* __restr
         C out[NF];
 #pragma unroll
         for (int q = 0; q < NF; q++) out[q] = jc[q] + C(0.25) * (jm_[q] - C(2) * jc[q] + jp[q]);
-        if (NF == 4 && job.moist_clamp) {
-            // job order t, ice, c, cloud; the microphysics clamp and vapour ceiling
+        if ((NF == 3 || NF == 4) && job.moist_clamp) {
+            // job order [t,] ice, c, cloud; the microphysics clamp and vapour ceiling
             // (cAtmosphereModel.cpp:1307-1399)
+            constexpr int o = NF - 3;   // index of ice
             const long long id = ((long long)i * K.jm + min(j, K.jm - 1)) * K.km + min(k, K.km - 1);
             const C co2 = C(fp<S>(M, F_CO2)[id]);
-            C cv = out[2], cld = out[3], ic = out[1];
+            C cv = out[o + 1], cld = out[o + 2], ic = out[o];
             if (!finite_bits(cv)) cv = C(0);
             if (!finite_bits(cld)) cld = C(0);
             if (!finite_bits(ic)) ic = C(0);
@@ -1208,7 +1209,7 @@ __global__ __launch_bounds__(TJ * TK) void k_damp_tile(const MemberDesc* __restr
             if (sum > cap) { const C sc = cap / sum; cld *= sc; ic *= sc; }
             if (cv < C(0)) cv = C(0);
             if (cv > cv_max) cv = cv_max;
-            out[2] = cv; out[3] = cld; out[1] = ic;
+            out[o + 1] = cv; out[o + 2] = cld; out[o] = ic;
         }
         if (own) {
             const long long id = ((long long)i * K.jm + j) * K.km + k;
@@ -1218,6 +1219,37 @@ __global__ __launch_bounds__(TJ * TK) void k_damp_tile(const MemberDesc* __restr
 #pragma unroll
         for (int q = 0; q < NF; q++) { jm_[q] = jc[q]; jc[q] = jp[q]; }
     }
+}
+
+// damp_wiggles of one buffer at the lowest level only: the k and j passes on levels 0 and 1,
+// then the i pass with the level below clamped to level 0, the same operations as
+// k_damp_tile's for that level. The moist block damps the temperature this way: the
+// densities step after it overwrites every level above the surface from the surface value.
+template <typename S>
+__global__ __launch_bounds__(256) void k_damp_level0(const MemberDesc* __restrict__ md, int src, int dst, int gate) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)K.jm * K.km) return;
+    const int j = (int)(q / K.km), k = (int)(q % K.km);
+    const S* x = job_ptr<S>(M, src);
+    const int km1 = (k - 1 + K.km) % K.km, kp1 = (k + 1) % K.km;
+    auto jk = [&](int ii) {   // k pass then j pass at level ii, cell (j, k)
+        C kp[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const int jj = min(max(j - 1 + r, 0), K.jm - 1);
+            const S* row = x + ((long long)ii * K.jm + jj) * K.km;
+            const C c0 = C(row[k]);
+            kp[r] = c0 + C(0.25) * (C(row[km1]) - C(2) * c0 + C(row[kp1]));
+        }
+        return K.sym ? kp[1] + C(0.25) * ((kp[0] + kp[2]) - C(2) * kp[1])
+                     : kp[1] + C(0.25) * (kp[0] - C(2) * kp[1] + kp[2]);
+    };
+    const C jc = jk(0), jp = jk(min(1, K.im - 1));
+    job_ptr<S>(M, dst)[(long long)j * K.km + k] = S(jc + C(0.25) * (jc - C(2) * jc + jp));
 }
 
 // Copies up to six buffers (src[q] -> dst[q]).
@@ -2039,7 +2071,8 @@ void damp_tile_launch(const Batch& b, const DampJob& job, int ich, int gate) {
     const long long ntk = (b.max_km + TK - 1) / TK, ntj = (b.max_jm + TJ - 1) / TJ;
     const long long nch = (b.max_im + ich - 1) / ich;
     const unsigned nb = (unsigned)(((ntk * ntj * nch + 7) / 8) * 8);
-    if (job.nf == 4) hipLaunchKernelGGL((k_damp_tile<S, TJ, TK, 4>), dim3(nb, 1, b.nmem), dim3(TJ * TK), 0, b.stream, b.d_md, job, ich, gate);
+    if (job.nf == 3) hipLaunchKernelGGL((k_damp_tile<S, TJ, TK, 3>), dim3(nb, 1, b.nmem), dim3(TJ * TK), 0, b.stream, b.d_md, job, ich, gate);
+    else if (job.nf == 4) hipLaunchKernelGGL((k_damp_tile<S, TJ, TK, 4>), dim3(nb, 1, b.nmem), dim3(TJ * TK), 0, b.stream, b.d_md, job, ich, gate);
     else hipLaunchKernelGGL((k_damp_tile<S, TJ, TK, 6>), dim3(nb, 1, b.nmem), dim3(TJ * TK), 0, b.stream, b.d_md, job, ich, gate);
 }
 template <typename S>
@@ -2071,6 +2104,9 @@ void launch_damp_tile(const Batch& b, int shape, int ich, const int* src, const 
     if (!b.d_md || ich < 1) return;
     if (b.fp32) damp_tile_dispatch<float>(b, shape, job, ich, gate);
     else damp_tile_dispatch<double>(b, shape, job, ich, gate);
+}
+void launch_damp_level0(const Batch& b, int src, int dst, int gate) {
+    ATHAD_DISPATCH(k_damp_level0, dim3(cols_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, src, dst, gate);
 }
 void launch_copy_job(const Batch& b, const int* src, const int* dst, int nf) {
     DampJob job{};
