@@ -948,12 +948,18 @@ ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi
         // qib unchanged and recomputes the same qv_hyp from the same inputs: the state is
         // already the fixed point the remaining iterations would keep.
         if (dqv == N(0) && CND == CND_prev) break;
+        const N CND_last = CND_prev;
         CND_prev = CND;
         const N dcnd = dqv * CND, ddep = dqv * DEP;
         qvb += dqv;
         qcb = tmax(N(0), qcb - dcnd);
         qib = tmax(N(0), qib - ddep);
         T -= (Lv * dcnd + (Lv + N(3.337e5)) * ddep) * rcp_nr(sat_mix_cp(qvb, co2, cpH, cpC, cpB));
+        // An update that leaves (T, qvb, qcb, qib) as the previous iteration left them, under
+        // the previous iteration's phase split, feeds the rest of this iteration the same inputs
+        // as that iteration: qv_hyp comes out equal to its previous value, the convergence test
+        // fails again and the fixed-point test below stops the loop with this state. Stop here.
+        if (T == T_prev && qvb == qv_prev && qcb == qc_prev && qib == qi_prev && CND == CND_last) break;
         sat_species_cp(T, Mb, cpH, cpC, cpB);
         Lv = sat_L_vap(T);
         const N qsum = qcb + qib;
