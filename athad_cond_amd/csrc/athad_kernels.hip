@@ -1784,6 +1784,55 @@ __global__ __launch_bounds__(256) void k_polar(const MemberDesc* __restrict__ md
     }
 }
 
+// The same filter with the row in registers: one wave per (row, field); lane l holds the E
+// consecutive longitudes E l .. E l + E - 1 (E = ceil(km / 64)), so a pass only needs the edge
+// values of the neighbouring lanes (cross-lane moves, no barrier). Same operations as k_polar.
+template <typename S, int E>
+__global__ __launch_bounds__(256) void k_polar_wave(const MemberDesc* __restrict__ md) {
+    using C = S;
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const int lane = threadIdx.x & 63;
+    const long long pair = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (pair >= 3LL * K.im * K.jm) return;   // uniform per wave
+    const int row = (int)(pair / 3), f = (int)(pair % 3);
+    const int np = M.passes[row % K.jm];
+    if (np == 0) return;
+    const int km = K.km, last = km - 1, l_last = last / E, s_last = last % E;
+    S* x = fp<S>(M, F_U + f) + (size_t)row * km;
+    C v[E];
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+        const int e = E * lane + s;
+        v[s] = (e < km) ? C(gld(x, (long long)e)) : C(0);
+    }
+    for (int p = 0; p < np; p++) {
+        const C up = __shfl_up(v[E - 1], 1);   // lane l-1's last longitude
+        const C dn = __shfl_down(v[0], 1);     // lane l+1's first longitude
+        C lv = v[0];
+#pragma unroll
+        for (int s = 1; s < E; s++)
+            if (s == s_last) lv = v[s];
+        const C v_last = __shfl(lv, l_last);   // longitude km-1
+        const C v_first = __shfl(v[0], 0);     // longitude 0
+        C nv[E];
+#pragma unroll
+        for (int s = 0; s < E; s++) {
+            const int e = E * lane + s;
+            const C left = (s > 0) ? v[s - 1] : ((lane > 0) ? up : v_last);
+            const C right = (e == last) ? v_first : ((s < E - 1) ? v[s + 1] : dn);
+            nv[s] = v[s] + C(0.25) * (left - C(2) * v[s] + right);
+        }
+#pragma unroll
+        for (int s = 0; s < E; s++) v[s] = nv[s];
+    }
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+        const int e = E * lane + s;
+        if (e < km) gst(x, (long long)e, S(v[s]));
+    }
+}
+
 // Radial Shapiro filters (lib/Utils.cpp:831-933): two passes along i of u (1-2-1) and of
 // v, w (4th order, 1-2-1 next to the wall/lid), one thread per (j,k) column with the column
 // in LDS; optionally the store xn <- x of UtilsAtm.h:25-49 for u, v, w.
@@ -2220,7 +2269,23 @@ void launch_bc(const Batch& b) {
 void launch_solid_clamp(const Batch& b) {
     ATHAD_DISPATCH(k_solid_clamp, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
 }
+namespace {
+// Rows of up to 768 longitudes run the register kernel (one wave per row and field), longer
+// ones the LDS kernel.
+constexpr bool kPolarWave = true;
+template <int E> void polar_wave_launch(const Batch& b) {
+    const dim3 g((unsigned)((3LL * b.max_im * b.max_jm + 3) / 4), 1, b.nmem);
+    if (b.fp32) hipLaunchKernelGGL((k_polar_wave<float, E>), g, dim3(256), 0, b.stream, b.d_md);
+    else hipLaunchKernelGGL((k_polar_wave<double, E>), g, dim3(256), 0, b.stream, b.d_md);
+}
+}  // namespace
 void launch_polar(const Batch& b) {
+    if (kPolarWave && b.max_km <= 768) {
+        if (b.max_km <= 192) polar_wave_launch<3>(b);
+        else if (b.max_km <= 384) polar_wave_launch<6>(b);
+        else polar_wave_launch<12>(b);
+        return;
+    }
     const size_t shm = 6 * (size_t)b.max_km * (b.fp32 ? 4 : 8);
     ATHAD_DISPATCH(k_polar, dim3((unsigned)(b.max_im * b.max_jm), 1, b.nmem), dim3(256), shm, b.d_md, 7);
 }
