@@ -82,6 +82,24 @@ def test_operator_matches_oracle(oracle, op):
         compare(integ, ref, A.FIELDS + A.FIELDS2D, 1e-12 if op != "saturation_adjustment" else 1e-10, op)
 
 
+@pytest.mark.parametrize("grid", [(70, 11, 21), (7, 11, 801)])
+def test_filters_on_tall_columns_and_long_rows(oracle, grid):
+    """The filters keep a column (up to 64 levels) or a row (up to 768 longitudes) in
+    registers; taller columns take the LDS tile radial filter and longer rows the LDS polar
+    filter. Both paths against the oracle, on a state spun up by the oracle."""
+    print(f"case filters {grid}", flush=True)
+    ref = oracle.OracleState(*grid, seed=13)
+    ref.run(4, "lex")
+    state = ref.snapshot()
+    with A.Integration(grid, order="lex") as integ:
+        integ.load(state)
+        integ.set_iter(4, ref.total_iter)
+        integ.filters()
+        for name in ORACLE_OP["filters"]:
+            ref.op(name, "lex")
+        compare(integ, ref, A.FIELDS, 1e-12, f"filters {grid}")
+
+
 def test_graph_replay_matches_eager(oracle):
     print("case graph replay", flush=True)
     with A.Integration(SMALL) as a, A.Integration(SMALL) as b:
