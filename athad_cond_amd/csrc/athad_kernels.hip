@@ -124,18 +124,6 @@ __global__ void k_level_mean_final(const MemberDesc* __restrict__ md) {
     M.t_ref[i] = (wsum > 0.0) ? sum / wsum : 1.0;
 }
 
-// ------------------------------------------------------------------------------------
-// RHS of one cell (RHS_Atm_Turb.cpp:31-1080, land-free central stencils) and the RK4
-// stage loop (RungeKutta_Atm_Turb.cpp:137-263). C is the arithmetic type.
-//
-// Over the four stages only the cell's own values change; its neighbours do not (they are
-// the start-of-RK4 values in the Jacobi order and values on other hyperplanes in the lex
-// order). Everything built from neighbours alone -- central gradients, the neighbour sums
-// of every Laplacian and of the vector diffusion, the velocity-gradient invariants of the
-// production term, the SST cross-diffusion product, the pressure gradient -- is computed
-// once per cell (prep_cell); each stage (rhs_stage) only adds the centre-value terms, the
-// minmod slopes and the SST closure. The RK4 base and accumulator live in LDS.
-// ------------------------------------------------------------------------------------
 // Sum and sum of squares of one field of member `member` in a fixed order (kStatChunks
 // contiguous chunks, each reduced by a fixed tree, then the chunks in order), so equal
 // fields give equal bits wherever and whenever they are computed. Results at
@@ -284,6 +272,19 @@ template <typename T> ATHAD_DEV void gst(T* p, long long i, T v) {
 
 // Per-cell metric (cAtmosphereModel.h:170-178, 282-284; RHS_Atm_Turb.cpp:86-130), the
 // Held-Suarez rate (:976-992), the Rayleigh drag factor and the Coriolis factor.
+// ------------------------------------------------------------------------------------
+// RHS of one cell (RHS_Atm_Turb.cpp:31-1080, land-free central stencils) and the RK4
+// stage loop (RungeKutta_Atm_Turb.cpp:137-263). C is the arithmetic type.
+//
+// Over the four stages only the cell's own values change; its neighbours do not (they are
+// the start-of-RK4 values in the Jacobi order and values on other hyperplanes in the lex
+// order). Everything built from neighbours alone -- central gradients, the neighbour sums
+// of every Laplacian and of the vector diffusion, the velocity-gradient invariants of the
+// production term, the SST cross-diffusion product, the pressure gradient -- is computed
+// once per cell at the start of each field group (rk4_dyn, rk4_moist, rk4_turb); each stage
+// only adds the centre-value terms, the minmod slopes and the SST closure. The RK4 base and
+// accumulator live in LDS.
+// ------------------------------------------------------------------------------------
 template <typename C> struct CellGeo {
     C aR, aT, aP;          // advection metric / grid step, for the minmod slopes
     C cexp, cirm, cirs;    // advection metric exp_rm, 1/rm, 1/(rm sin theta)
