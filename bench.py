@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 GRID = (41, 181, 361)
 METRIC = ("RK4 cell-updates/s at 41×181×361, mixed vs FP64 at stated error; p99 latency")
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector, published
 # Algorithmic bytes per cell of the dominant kernel (k_rk4_jacobi, FP64): reads 11 current
 # fields + 11 stored (*n) fields + p_dyn, nue, p_stat, t_eq = 26 words; writes 11 new
 # fields + aux_u/v/w + prod, tke_source, dis_source + the 8 fused stores = 25 words.
@@ -66,6 +67,18 @@ def measured_traffic():
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def measured_flops():
+    """FP64 operations per k_rk4_jacobi<double> launch at this grid, from the SQ_INSTS_VALU_*_F64
+    counter pass recorded in the committed profile summary; None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_rk4_jacobi_fp64_41x181x361_fp64_ops.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("flop_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -150,6 +163,14 @@ def main():
                      "bytes_per_cell": RK4_WORDS * 8, "algorithmic_bytes_per_launch": RK4_WORDS * 8 * n},
         "step_hbm_frac": STEP_WORDS * 8 * fp64["cells_per_s"] / (HBM_PEAK_GBS * 1e9),
     }
+    flops = measured_flops()
+    if flops is not None:
+        # the same kernel against the FP64 vector peak: counted FP64 operations per launch over
+        # the launch time measured above
+        out["alu_roofline"] = {"bound": "fp64 vector", "achieved": flops / avg_rk4_s / 1e12,
+                               "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                               "frac": flops / avg_rk4_s / 1e12 / FP64_PEAK_TFLOPS,
+                               "kernel": "k_rk4_jacobi<double>", "flop_per_launch": flops}
     if not args.no_extra:
         mixed = run_form(A, "mixed", "jacobi", args.steps, args.warmup, graph=graph)
         err = {k: float(np.sqrt(np.sum((mixed["fields"][k] - fp64["fields"][k]) ** 2))
