@@ -4,7 +4,7 @@ The HIP library ``libathad_hip.so`` (built from ``csrc/``) holds the device stat
 operators; this package is its host-side mirror.
 """
 from ._lib import FIELDS, FIELDS2D, PROGNOSTIC, AthadError, default_params, lib  # noqa: F401
-from .integration import Integration, footprint_bytes  # noqa: F401
+from .integration import Integration, footprint_bytes, read_restart, restart_bytes, write_restart  # noqa: F401
 
 __all__ = ["Integration", "footprint_bytes", "FIELDS", "FIELDS2D", "PROGNOSTIC", "AthadError",
-           "default_params", "lib"]
+           "default_params", "lib", "read_restart", "restart_bytes", "write_restart"]

@@ -39,8 +39,17 @@ EXPORTS = [
     "athad_filters", "athad_store", "athad_step", "athad_run", "athad_sync", "athad_profile",
     "athad_profile_result", "athad_version", "athad_dew_point", "athad_member_from_surface",
     "athad_query", "athad_device_memory", "athad_damp_tile", "athad_set_damp_tile",
-    "athad_field_stats", "athad_set_member_params",
+    "athad_field_stats", "athad_set_member_params", "athad_restart_bytes", "athad_save_state_mem",
+    "athad_load_state_mem", "athad_save_state", "athad_load_state",
 ]
+
+# restart_arrays() of the reference checkpoint (atmosphere/FileIO_Atm.cpp:297-304), in file
+# order; the four precipitation arrays are not held by a context (written as zeros)
+RESTART_ARRAYS = ["t", "u", "v", "w", "c", "cloud", "ice", "gr", "co2",
+                  "tn", "un", "vn", "wn", "cn", "cloudn", "icen", "grn", "co2n",
+                  "p_dyn", "p_stat", "tke", "dis", "tken", "disn", "nue",
+                  "P_rain", "P_snow", "P_rainn", "P_snown"]
+RESTART_MAGIC = 0x41544D31
 
 
 class Grid(ctypes.Structure):
@@ -103,8 +112,15 @@ def lib():
         L.athad_dew_point.restype = d
         L.athad_dew_point.argtypes = [d]
         L.athad_member_from_surface.argtypes = [d, d, d, d, ctypes.POINTER(MemberParams)]
+        L.athad_restart_bytes.restype = ctypes.c_int64
+        L.athad_restart_bytes.argtypes = [ctypes.POINTER(Grid)]
+        L.athad_save_state_mem.argtypes = [vp, i, vp, ctypes.c_int64]
+        L.athad_load_state_mem.argtypes = [vp, i, vp, ctypes.c_int64]
+        L.athad_save_state.argtypes = [vp, i, ctypes.c_char_p]
+        L.athad_load_state.argtypes = [vp, i, ctypes.c_char_p]
         for name in EXPORTS:
-            if name not in ("athad_last_error", "athad_footprint_bytes", "athad_version", "athad_dew_point"):
+            if name not in ("athad_last_error", "athad_footprint_bytes", "athad_version", "athad_dew_point",
+                            "athad_restart_bytes"):
                 getattr(L, name).restype = i
         _lib = L
     return _lib

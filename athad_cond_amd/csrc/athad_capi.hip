@@ -862,6 +862,92 @@ athad_status athad_download_rows(athad_ctx* ctx, int member, int field, double* 
     return ATHAD_OK;
 }
 
+// ---- restart files (FileIO_Atm.cpp:293-356) --------------------------------------------
+namespace {
+constexpr int32_t kRestartMagic = 0x41544D31;   // "ATM1"
+constexpr int kRestartArrays = 29;
+// restart_arrays() (FileIO_Atm.cpp:297-304) as field ids; -1 for P_rain, P_snow, P_rainn,
+// P_snown, which the scoped step keeps at zero and the context does not hold
+constexpr int kRestartF[kRestartArrays] = {
+    F_T, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2,
+    F_TN, F_UN, F_VN, F_WN, F_CN, F_CLOUDN, F_ICEN, F_GRN, F_CO2N,
+    F_P_DYN, F_P_STAT, F_TKE, F_DIS, F_TKEN, F_DISN, F_NUE, -1, -1, -1, -1};
+int64_t restart_bytes(const athad_grid& g) {
+    return 5 * 4 + (int64_t)kRestartArrays * g.im * g.jm * g.km * 8;
+}
+}  // namespace
+
+int64_t athad_restart_bytes(const athad_grid* g) {
+    if (!g || g->im < 1 || g->jm < 1 || g->km < 1) return -1;
+    return restart_bytes(*g);
+}
+
+athad_status athad_save_state_mem(athad_ctx* ctx, int member, void* buf, int64_t size) {
+    athad_status s = check_member(ctx, member, 0, 1);
+    if (s) return s;
+    MemberHost& m = ctx->mem[member];
+    if (!buf || size < restart_bytes(m.g)) { ctx->err = "restart: buffer too small"; return ATHAD_ERR_ARG; }
+    join_s2(ctx);
+    int iter_n = 0, total = 0;
+    if ((s = athad_get_iter(ctx, member, &iter_n, &total))) return s;
+    const int32_t hdr[5] = {kRestartMagic, m.g.im, m.g.jm, m.g.km, total};
+    unsigned char* out = (unsigned char*)buf;
+    std::memcpy(out, hdr, sizeof(hdr));
+    double* arr = (double*)(out + sizeof(hdr));
+    for (int a = 0; a < kRestartArrays; a++, arr += m.n) {
+        if (kRestartF[a] < 0) { std::memset(arr, 0, m.n * 8); continue; }
+        if ((s = athad_get_field(ctx, member, kRestartF[a], arr))) return s;
+    }
+    return ATHAD_OK;
+}
+
+athad_status athad_load_state_mem(athad_ctx* ctx, int member, const void* buf, int64_t size) {
+    athad_status s = check_member(ctx, member, 0, 1);
+    if (s) return s;
+    MemberHost& m = ctx->mem[member];
+    int32_t hdr[5];
+    if (!buf || size < (int64_t)sizeof(hdr)) { ctx->err = "restart: no header"; return ATHAD_ERR_ARG; }
+    std::memcpy(hdr, buf, sizeof(hdr));
+    if (hdr[0] != kRestartMagic || hdr[1] != m.g.im || hdr[2] != m.g.jm || hdr[3] != m.g.km) {
+        ctx->err = "restart: bad header or grid mismatch";
+        return ATHAD_ERR_SHAPE;
+    }
+    if (size < restart_bytes(m.g)) { ctx->err = "restart: truncated"; return ATHAD_ERR_ARG; }
+    join_s2(ctx);
+    const double* arr = (const double*)((const unsigned char*)buf + sizeof(hdr));
+    for (int a = 0; a < kRestartArrays; a++, arr += m.n) {
+        if (kRestartF[a] < 0) continue;
+        if ((s = athad_set_field(ctx, member, kRestartF[a], arr))) return s;
+    }
+    return athad_set_iter(ctx, member, m.iter_n, hdr[4]);
+}
+
+athad_status athad_save_state(athad_ctx* ctx, int member, const char* path) {
+    athad_status s = check_member(ctx, member, 0, 1);
+    if (s) return s;
+    if (!path) return ATHAD_ERR_ARG;
+    std::vector<unsigned char> buf((size_t)restart_bytes(ctx->mem[member].g));
+    if ((s = athad_save_state_mem(ctx, member, buf.data(), (int64_t)buf.size()))) return s;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) { ctx->err = std::string("restart: cannot open ") + path; return ATHAD_ERR_ARG; }
+    const size_t w = std::fwrite(buf.data(), 1, buf.size(), f);
+    const int closed = std::fclose(f);
+    if (w != buf.size() || closed != 0) { ctx->err = std::string("restart: write failed ") + path; return ATHAD_ERR_ARG; }
+    return ATHAD_OK;
+}
+
+athad_status athad_load_state(athad_ctx* ctx, int member, const char* path) {
+    athad_status s = check_member(ctx, member, 0, 1);
+    if (s) return s;
+    if (!path) return ATHAD_ERR_ARG;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) { ctx->err = std::string("restart: no file ") + path; return ATHAD_ERR_ARG; }
+    std::vector<unsigned char> buf((size_t)restart_bytes(ctx->mem[member].g));
+    const size_t r = std::fread(buf.data(), 1, buf.size(), f);
+    std::fclose(f);
+    return athad_load_state_mem(ctx, member, buf.data(), (int64_t)r);
+}
+
 athad_status athad_init_state(athad_ctx* c, uint64_t seed) {
     if (!c) return ATHAD_ERR_ARG;
     // member seeds

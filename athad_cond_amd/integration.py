@@ -8,6 +8,7 @@ and with the same names where the reference has them; ``step()`` is one iteratio
 from __future__ import annotations
 
 import ctypes
+import os
 from typing import Iterable, Sequence
 
 import numpy as np
@@ -161,6 +162,24 @@ class Integration:
     def sync(self):
         self._chk(L.lib().athad_sync(self.h))
 
+    # ---- restart files (the reference's ATM1 checkpoint, FileIO_Atm.cpp:293-356) --------
+    def save_state(self, path: str | None = None, member: int = 0):
+        """Writes member's restart to `path`, or returns it as bytes when path is None."""
+        if path is not None:
+            self._chk(L.lib().athad_save_state(self.h, member, os.fsencode(path)))
+            return None
+        buf = ctypes.create_string_buffer(restart_bytes(self.grids[member]))
+        self._chk(L.lib().athad_save_state_mem(self.h, member, buf, len(buf)))
+        return buf.raw
+
+    def load_state(self, src, member: int = 0):
+        """Loads a restart from a path or from bytes (total iteration count from its header)."""
+        if isinstance(src, (bytes, bytearray)):
+            buf = ctypes.create_string_buffer(bytes(src), len(src))
+            self._chk(L.lib().athad_load_state_mem(self.h, member, buf, len(src)))
+        else:
+            self._chk(L.lib().athad_load_state(self.h, member, os.fsencode(src)))
+
     # ---- measurement -------------------------------------------------------------------
     def profile(self, kernel: str, enable: bool = True):
         self._chk(L.lib().athad_profile(self.h, L.KERNEL_ID[kernel], int(enable)))
@@ -169,6 +188,33 @@ class Integration:
         ms, n = ctypes.c_double(), ctypes.c_int()
         self._chk(L.lib().athad_profile_result(self.h, L.KERNEL_ID[kernel], ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+def restart_bytes(grid: tuple) -> int:
+    g = L.Grid(*grid)
+    return int(L.lib().athad_restart_bytes(ctypes.byref(g)))
+
+
+def read_restart(data: bytes):
+    """Parses an ATM1 restart: (header dict, {array name: [im][jm][km] float64 array})."""
+    hdr = np.frombuffer(data, dtype="<i4", count=5)
+    if int(hdr[0]) != L.RESTART_MAGIC:
+        raise ValueError("not an ATM1 restart")
+    im, jm, km = (int(v) for v in hdr[1:4])
+    n = im * jm * km
+    arrs = np.frombuffer(data, dtype="<f8", offset=20, count=len(L.RESTART_ARRAYS) * n)
+    return ({"im": im, "jm": jm, "km": km, "total_iter_count": int(hdr[4])},
+            {name: arrs[q * n:(q + 1) * n].reshape(im, jm, km) for q, name in enumerate(L.RESTART_ARRAYS)})
+
+
+def write_restart(header: dict, arrays: dict) -> bytes:
+    """An ATM1 restart from a header dict and arrays by name (missing arrays are zeros)."""
+    im, jm, km = header["im"], header["jm"], header["km"]
+    out = [np.array([L.RESTART_MAGIC, im, jm, km, header["total_iter_count"]], dtype="<i4").tobytes()]
+    for name in L.RESTART_ARRAYS:
+        a = arrays.get(name)
+        out.append(np.zeros((im, jm, km), "<f8").tobytes() if a is None else np.ascontiguousarray(a, "<f8").tobytes())
+    return b"".join(out)
 
 
 def footprint_bytes(grid: tuple, precision: str = "fp64") -> int:

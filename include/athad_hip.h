@@ -177,6 +177,29 @@ athad_status athad_set_damp_tile(athad_ctx* ctx, int shape, int levels);
 /* Waits for the context's stream. */
 athad_status athad_sync(athad_ctx* ctx);
 
+/* --- restart files ------------------------------------------------------------------- */
+
+/* The reference's checkpoint format (cAtmosphereModel::save_state / load_state,
+ * atmosphere/FileIO_Atm.cpp:293-356): an int32 header {0x41544D31 "ATM1", im, jm, km,
+ * total_iter_count}, then the 29 arrays of restart_arrays() (FileIO_Atm.cpp:297-304), each
+ * im*jm*km FP64 values in [i][j][k] order: t u v w c cloud ice gr co2, tn un vn wn cn cloudn
+ * icen grn co2n, p_dyn p_stat, tke dis tken disn nue, P_rain P_snow P_rainn P_snown. The four
+ * precipitation arrays are zero in the scoped step and not held by the context: they are
+ * written as zeros and skipped on load. Host byte order (little-endian on the targets). */
+/* Size in bytes of a restart of grid g. */
+int64_t athad_restart_bytes(const athad_grid* g);
+/* Writes member's restart into buf (size >= athad_restart_bytes). Replaces save_state. */
+athad_status athad_save_state_mem(athad_ctx* ctx, int member, void* buf, int64_t size);
+/* Loads a restart from buf: ATHAD_ERR_SHAPE on a bad header or a grid that differs from the
+ * member's, ATHAD_ERR_ARG when buf is shorter than the arrays. Sets the member's total
+ * iteration count from the header and leaves its cadence counter (athad_set_iter) as it was,
+ * as load_state leaves run_3D_loop's iteration to the caller. Fields outside the restart
+ * keep their values. Replaces load_state. */
+athad_status athad_load_state_mem(athad_ctx* ctx, int member, const void* buf, int64_t size);
+/* The same through a file (FileIO_Atm.cpp names it atm_restart_<Ma>Ma_<iter>.bin). */
+athad_status athad_save_state(athad_ctx* ctx, int member, const char* path);
+athad_status athad_load_state(athad_ctx* ctx, int member, const char* path);
+
 /* --- measurement --------------------------------------------------------------------- */
 
 /* Per-kernel timing with HIP events on the context's stream: kernel ids as in
