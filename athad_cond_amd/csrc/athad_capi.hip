@@ -177,6 +177,10 @@ void fill_consts(MemberHost& m, const Config& cfg, int order) {
     K.sym = (order == ATHAD_ORDER_SYMMETRIC) ? 1 : 0;
 }
 
+// Reduction scratch per member: level means, field statistics and the checkpoint diagnostics
+// (kernels' diag_offset region) with room to spare.
+int64_t scratch_doubles(int im) { return std::max<int64_t>(8192, (int64_t)diag_offset(im) + 6 * 256 + 1024); }
+
 // 1-D tables: rad, sin/cos, layer heights, ABL taper, SatAdj dt_dim, polar passes.
 void build_tables(MemberHost& m, const Config& cfg, std::vector<double>& tab, std::vector<int>& passes) {
     const int im = m.g.im, jm = m.g.jm;
@@ -550,7 +554,7 @@ int64_t athad_footprint_bytes(const athad_grid* g, int precision) {
     if (!g) return -1;
     const int64_t n = (int64_t)g->im * g->jm * g->km;
     const int64_t es = (precision == ATHAD_MIXED) ? 4 : 8;
-    return n * (es * (NFIELD - 1 + NPROG + NTMP) + 8) + (int64_t)g->jm * g->km * 16 + 65536;
+    return n * (es * (NFIELD - 1 + NPROG + NTMP) + 8) + (int64_t)g->jm * g->km * 16 + scratch_doubles(g->im) * 8;
 }
 
 athad_status athad_create(int n_members, const athad_grid* grids, const athad_member_params* params,
@@ -605,7 +609,7 @@ athad_status athad_create(int n_members, const athad_grid* grids, const athad_me
         const size_t n2 = (size_t)m.g.jm * m.g.km;
         ok = ok && alloc((void**)&m.vel_star, n2 * 8) && alloc((void**)&m.t_top, n2 * 8) &&
              alloc((void**)&m.t_ref, m.g.im * 8) && alloc((void**)&m.tables, tab.size() * 8) &&
-             alloc((void**)&m.scratch, (size_t)std::max(8192, 2 * 32 * m.g.im + 1024) * 8) && alloc((void**)&m.counters, 64) &&
+             alloc((void**)&m.scratch, (size_t)scratch_doubles(m.g.im) * 8) && alloc((void**)&m.counters, 64) &&
              alloc((void**)&m.passes, passes.size() * 4);
         if (!ok) { g_create_error = "device allocation failed"; free_ctx(c); return ATHAD_ERR_ALLOC; }
         for (int f = 0; f < NFIELD; f++) hipMemsetAsync(m.field[f], 0, field_bytes(c, m, f), c->stream);
@@ -710,6 +714,61 @@ athad_status athad_field_stats(athad_ctx* c, int member, int field, double* sum,
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (sum) *sum = r[0];
     if (sumsq) *sumsq = r[1];
+    return ATHAD_OK;
+}
+
+athad_status athad_field_minmax(athad_ctx* c, int member, int field, double* vmin, int* pos_min, double* vmax,
+                                int* pos_max) {
+    athad_status s = check_member(c, member, field, NFIELD);
+    if (s) return s;
+    join_s2(c);
+    launch_minmax(cur(c), member, field);
+    double r[4];
+    const MemberHost& m = c->mem[member];
+    HIPCHK(c, hipMemcpyAsync(r, m.scratch + diag_offset(m.g.im) + 4 * 256, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    auto ijk = [&](double q, int* pos) {
+        if (!pos) return;
+        const long long iq = (q < 0.0) ? 0 : (long long)q;   // none finite: (0, 0, 0) as the reference
+        pos[0] = (int)(iq / ((long long)m.g.jm * m.g.km));
+        pos[1] = (int)((iq / m.g.km) % m.g.jm);
+        pos[2] = (int)(iq % m.g.km);
+    };
+    if (vmin) *vmin = r[0];
+    if (vmax) *vmax = r[2];
+    ijk(r[1], pos_min);
+    ijk(r[3], pos_max);
+    return ATHAD_OK;
+}
+
+athad_status athad_residuum(athad_ctx* c, int member, double* res_max, int* pos_max, double* avg_abs,
+                            int64_t* n_finite, int64_t* n_nonfinite, int* pos_first_nonfinite) {
+    athad_status s = check_member(c, member, 0, 1);
+    if (s) return s;
+    const MemberHost& m = c->mem[member];
+    if (m.g.im < 32) { c->err = "findResiduumAtm scans levels 10..30 and needs im >= 32"; return ATHAD_ERR_SHAPE; }
+    join_s2(c);
+    launch_residuum(cur(c), member);
+    double r[6];
+    HIPCHK(c, hipMemcpyAsync(r, m.scratch + diag_offset(m.g.im) + 6 * 256, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    auto ijk = [&](double q, int* pos) {
+        if (!pos) return;
+        if (q < 0.0) { pos[0] = pos[1] = pos[2] = -1; return; }
+        const long long iq = (long long)q;
+        pos[0] = (int)(iq / ((long long)m.g.jm * m.g.km));
+        pos[1] = (int)((iq / m.g.km) % m.g.jm);
+        pos[2] = (int)(iq % m.g.km);
+    };
+    if (res_max) *res_max = r[0];
+    if (pos_max) {   // no residual above 0: (0, 0, 0), the reference's starting position
+        if (r[1] < 0.0) pos_max[0] = pos_max[1] = pos_max[2] = 0;
+        else ijk(r[1], pos_max);
+    }
+    if (avg_abs) *avg_abs = (r[3] > 0.0) ? r[2] / r[3] : 0.0;
+    if (n_finite) *n_finite = (int64_t)r[3];
+    if (n_nonfinite) *n_nonfinite = (int64_t)r[4];
+    ijk(r[5], pos_first_nonfinite);
     return ATHAD_OK;
 }
 

@@ -27,6 +27,13 @@ void launch_level_mean(const Batch& b);
 // scratch[field_stats_offset(im)], [+1]
 void launch_field_stats(const Batch& b, int member, int field);
 int field_stats_offset(int im);
+// searchMinMax_3D of one field of one member: min, its index, max, its index (indices as
+// doubles, row-major, -1 for none) at scratch[diag_offset(im)] + 4 * 256
+void launch_minmax(const Batch& b, int member, int field);
+// findResiduumAtm of one member: max, its index, sum, count, non-finite count, first
+// non-finite index at scratch[diag_offset(im)] + 6 * 256
+void launch_residuum(const Batch& b, int member);
+int diag_offset(int im);
 void launch_rk4_jacobi(const Batch& b, bool fused);
 void launch_rk4_lex(const Batch& b, int plane);
 void launch_poisson(const Batch& b, int gate);

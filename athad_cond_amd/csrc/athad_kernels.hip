@@ -13,6 +13,8 @@
 // wait also drains every load in flight.
 //
 // Each kernel cites the reference code it restates.
+#include <cfloat>
+
 #include "athad_device.h"
 #include "athad_kernels.h"
 
@@ -166,6 +168,145 @@ __global__ void k_field_stats_final(const MemberDesc* __restrict__ md, int membe
     for (int c = 0; c < kStatChunks; c++) { a += r[2 * c]; b += r[2 * c + 1]; }
     M.scratch[stat_base(M.K) + 2 * kStatChunks] = a;
     M.scratch[stat_base(M.K) + 2 * kStatChunks + 1] = b;
+}
+
+// ------------------------------------------------------------------------------------
+// Checkpoint diagnostics on the device (the reference prints them from host loops over the
+// fields): searchMinMax_3D (MinMax_Atm.cpp:51-108) and findResiduumAtm (UtilsAtm.h:70-218).
+// kStatChunks contiguous chunks, each reduced by a fixed tree, then the chunks in order, in a
+// scratch region after the field statistics. Positions travel as doubles (exact below 2^53);
+// -1 is "none".
+// ------------------------------------------------------------------------------------
+ATHAD_DEV int diag_base(const Consts& K) { return stat_base(K) + 2 * kStatChunks + 2; }
+
+// The reference scans in row-major order with strict comparisons against +-DBL_MAX starting
+// values, so among equal extremes the first occurrence, the smallest index, is kept, and a value
+// equal to the starting bound is never taken. findResiduumAtm keeps a maximum only when it
+// exceeds 0 and breaks ties by the smallest (i, j, k), which is the smallest index.
+ATHAD_DEV bool keeps_max(double v, double q, double bv, double bq) { return bq < 0.0 || v > bv || (v == bv && q < bq); }
+ATHAD_DEV bool keeps_min(double v, double q, double bv, double bq) { return bq < 0.0 || v < bv || (v == bv && q < bq); }
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_minmax(const MemberDesc* __restrict__ md, int member, int field) {
+    const MemberDesc& M = md[member];
+    const Consts& K = M.K;
+    const long long n = K.n;
+    const long long per = (n + kStatChunks - 1) / kStatChunks;
+    const long long q0 = (long long)blockIdx.x * per, q1 = min(n, q0 + per);
+    const bool dbl = (field == F_P_STAT) || sizeof(S) == 8;
+    __shared__ double s_lo[256], s_lq[256], s_hi[256], s_hq[256];
+    double lo = 0.0, lq = -1.0, hi = 0.0, hq = -1.0;
+    for (long long q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        const double v = dbl ? ((const double*)M.f[field])[q] : (double)((const float*)M.f[field])[q];
+        if (!finite_bits(v)) continue;   // searchMinMax_3D skips non-finite values
+        if (v < DBL_MAX && keeps_min(v, (double)q, lo, lq)) { lo = v; lq = (double)q; }
+        if (v > -DBL_MAX && keeps_max(v, (double)q, hi, hq)) { hi = v; hq = (double)q; }
+    }
+    s_lo[threadIdx.x] = lo; s_lq[threadIdx.x] = lq; s_hi[threadIdx.x] = hi; s_hq[threadIdx.x] = hq;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        const int t = threadIdx.x;
+        if (t < st) {
+            if (s_lq[t + st] >= 0.0 && keeps_min(s_lo[t + st], s_lq[t + st], s_lo[t], s_lq[t])) {
+                s_lo[t] = s_lo[t + st]; s_lq[t] = s_lq[t + st];
+            }
+            if (s_hq[t + st] >= 0.0 && keeps_max(s_hi[t + st], s_hq[t + st], s_hi[t], s_hq[t])) {
+                s_hi[t] = s_hi[t + st]; s_hq[t] = s_hq[t + st];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* r = M.scratch + diag_base(K) + 4 * blockIdx.x;
+        r[0] = s_lo[0]; r[1] = s_lq[0]; r[2] = s_hi[0]; r[3] = s_hq[0];
+    }
+}
+
+__global__ void k_minmax_final(const MemberDesc* __restrict__ md, int member) {
+    const MemberDesc& M = md[member];
+    const double* r = M.scratch + diag_base(M.K);
+    double lo = 0.0, lq = -1.0, hi = 0.0, hq = -1.0;
+    for (int c = 0; c < kStatChunks; c++) {
+        if (r[4 * c + 1] >= 0.0 && keeps_min(r[4 * c], r[4 * c + 1], lo, lq)) { lo = r[4 * c]; lq = r[4 * c + 1]; }
+        if (r[4 * c + 3] >= 0.0 && keeps_max(r[4 * c + 2], r[4 * c + 3], hi, hq)) { hi = r[4 * c + 2]; hq = r[4 * c + 3]; }
+    }
+    double* o = M.scratch + diag_base(M.K) + 4 * kStatChunks;
+    o[0] = lo; o[1] = lq; o[2] = hi; o[3] = hq;
+}
+
+// findResiduumAtm: over i in [10, 30], j in [1, jm-2], k in [1, km-2] the residual
+// sqrt((du/dr^2 + dv/dtheta^2 + dw/dphi^2) / 3) of the centred velocity derivatives (the metric
+// floor sin(theta) >= 0.55 as in the RHS); its maximum (ties: smallest (i,j,k)), the sum and
+// count of the finite residuals and the count and first position of the non-finite ones. The
+// reference's sum runs in its thread order; this one in a fixed order.
+template <typename S>
+__global__ __launch_bounds__(256) void k_residuum(const MemberDesc* __restrict__ md, int member) {
+    const MemberDesc& M = md[member];
+    const Consts& K = M.K;
+    const long long nj = K.jm - 2, nk = K.km - 2, nbox = 21LL * nj * nk;
+    const long long per = (nbox + kStatChunks - 1) / kStatChunks;
+    const long long b0 = (long long)blockIdx.x * per, b1 = min(nbox, b0 + per);
+    const S *u = fp<S>(M, F_U), *v = fp<S>(M, F_V), *w = fp<S>(M, F_W);
+    const long long si = (long long)K.jm * K.km, sj = K.km;
+    __shared__ double s_mx[256], s_mq[256], s_sum[256], s_cnt[256], s_nan[256], s_nq[256];
+    double mx = 0.0, mq = -1.0, sum = 0.0, cnt = 0.0, nan = 0.0, nq = -1.0;
+    for (long long b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+        const int i = 10 + (int)(b / (nj * nk));
+        const long long r = b % (nj * nk);
+        const int j = 1 + (int)(r / nk), k = 1 + (int)(r % nk);
+        const long long q = i * si + (long long)j * sj + k;
+        const double sinthe = M.sint[j];
+        const double rm = M.rad[i];
+        const double exp_rm = 1.0 / (rm + 1.0);
+        const double rmsinthe = rm * sinthe;
+        const double dudr = ((double)u[q + si] - (double)u[q - si]) / (2.0 * K.dr) * exp_rm;
+        const double dvdthe = ((double)v[q + sj] - (double)v[q - sj]) / (2.0 * rm * K.dthe);
+        const double dwdphi = ((double)w[q + 1] - (double)w[q - 1]) / (2.0 * rmsinthe * K.dphi);
+        const double res = sqrt((dudr * dudr + dvdthe * dvdthe + dwdphi * dwdphi) / 3.0);
+        if (!finite_bits(res)) {
+            if (nq < 0.0) nq = (double)q;   // this thread visits increasing q
+            nan += 1.0;
+            continue;
+        }
+        sum += res;
+        cnt += 1.0;
+        if (res > 0.0 && keeps_max(res, (double)q, mx, mq)) { mx = res; mq = (double)q; }
+    }
+    const int t = threadIdx.x;
+    s_mx[t] = mx; s_mq[t] = mq; s_sum[t] = sum; s_cnt[t] = cnt; s_nan[t] = nan; s_nq[t] = nq;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (t < st) {
+            if (s_mq[t + st] >= 0.0 && keeps_max(s_mx[t + st], s_mq[t + st], s_mx[t], s_mq[t])) {
+                s_mx[t] = s_mx[t + st]; s_mq[t] = s_mq[t + st];
+            }
+            s_sum[t] += s_sum[t + st];
+            s_cnt[t] += s_cnt[t + st];
+            s_nan[t] += s_nan[t + st];
+            if (s_nq[t + st] >= 0.0 && (s_nq[t] < 0.0 || s_nq[t + st] < s_nq[t])) s_nq[t] = s_nq[t + st];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        double* o = M.scratch + diag_base(K) + 6 * blockIdx.x;
+        o[0] = s_mx[0]; o[1] = s_mq[0]; o[2] = s_sum[0]; o[3] = s_cnt[0]; o[4] = s_nan[0]; o[5] = s_nq[0];
+    }
+}
+
+__global__ void k_residuum_final(const MemberDesc* __restrict__ md, int member) {
+    const MemberDesc& M = md[member];
+    const double* r = M.scratch + diag_base(M.K);
+    double mx = 0.0, mq = -1.0, sum = 0.0, cnt = 0.0, nan = 0.0, nq = -1.0;
+    for (int c = 0; c < kStatChunks; c++) {
+        const double* x = r + 6 * c;
+        if (x[1] >= 0.0 && keeps_max(x[0], x[1], mx, mq)) { mx = x[0]; mq = x[1]; }
+        sum += x[2];
+        cnt += x[3];
+        nan += x[4];
+        if (x[5] >= 0.0 && (nq < 0.0 || x[5] < nq)) nq = x[5];
+    }
+    double* o = M.scratch + diag_base(M.K) + 6 * kStatChunks;
+    o[0] = mx; o[1] = mq; o[2] = sum; o[3] = cnt; o[4] = nan; o[5] = nq;
 }
 
 enum { FT = 0, FU, FV, FW, FC, FCL, FI, FG, FCO, FK, FD };
@@ -2152,6 +2293,15 @@ void launch_field_stats(const Batch& b, int member, int field) {
     hipLaunchKernelGGL(k_field_stats_final, dim3(1), dim3(1), 0, b.stream, b.d_md, member);
 }
 int field_stats_offset(int im) { return 2 * kLevelChunks * im + 2 * kStatChunks; }
+int diag_offset(int im) { return 2 * kLevelChunks * im + 2 * kStatChunks + 2; }
+void launch_minmax(const Batch& b, int member, int field) {
+    ATHAD_DISPATCH(k_minmax, dim3(kStatChunks), dim3(256), 0, b.d_md, member, field);
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(1), 0, b.stream, b.d_md, member);
+}
+void launch_residuum(const Batch& b, int member) {
+    ATHAD_DISPATCH(k_residuum, dim3(kStatChunks), dim3(256), 0, b.d_md, member);
+    hipLaunchKernelGGL(k_residuum_final, dim3(1), dim3(1), 0, b.stream, b.d_md, member);
+}
 void launch_level_mean(const Batch& b) {
     ATHAD_DISPATCH(k_level_mean, dim3(kLevelChunks, b.max_im, b.nmem), dim3(256), 0, b.d_md);
     hipLaunchKernelGGL(k_level_mean_final, dim3((b.max_im + 63) / 64, 1, b.nmem), dim3(64), 0, b.stream, b.d_md);

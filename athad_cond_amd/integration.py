@@ -180,6 +180,25 @@ class Integration:
         else:
             self._chk(L.lib().athad_load_state(self.h, member, os.fsencode(src)))
 
+    # ---- checkpoint diagnostics on the device (MinMax_Atm.cpp, UtilsAtm.h) --------------
+    def field_minmax(self, name: str, member: int = 0):
+        """searchMinMax_3D: (min, (i, j, k), max, (i, j, k)) of the finite values."""
+        lo, hi = ctypes.c_double(), ctypes.c_double()
+        plo, phi = (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
+        self._chk(L.lib().athad_field_minmax(self.h, member, L.FIELD_ID[name], ctypes.byref(lo), plo,
+                                             ctypes.byref(hi), phi))
+        return lo.value, tuple(plo), hi.value, tuple(phi)
+
+    def residuum(self, member: int = 0) -> dict:
+        """findResiduumAtm: maximum residual and its (i, j, k), average, counts."""
+        mx, avg = ctypes.c_double(), ctypes.c_double()
+        pm, pn = (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
+        nf, nn = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(L.lib().athad_residuum(self.h, member, ctypes.byref(mx), pm, ctypes.byref(avg),
+                                         ctypes.byref(nf), ctypes.byref(nn), pn))
+        return {"max": mx.value, "pos_max": tuple(pm), "avg_abs": avg.value, "n_finite": nf.value,
+                "n_nonfinite": nn.value, "pos_first_nonfinite": tuple(pn)}
+
     # ---- measurement -------------------------------------------------------------------
     def profile(self, kernel: str, enable: bool = True):
         self._chk(L.lib().athad_profile(self.h, L.KERNEL_ID[kernel], int(enable)))

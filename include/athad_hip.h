@@ -200,6 +200,25 @@ athad_status athad_load_state_mem(athad_ctx* ctx, int member, const void* buf, i
 athad_status athad_save_state(athad_ctx* ctx, int member, const char* path);
 athad_status athad_load_state(athad_ctx* ctx, int member, const char* path);
 
+/* --- checkpoint diagnostics on the device --------------------------------------------- */
+
+/* searchMinMax_3D (atmosphere/MinMax_Atm.cpp:51-108) of one field without downloading it: the
+ * minimum and maximum of its finite values and the (i, j, k) of their first occurrence in
+ * [i][j][k] order (the reference's strict comparisons keep the first of equal extremes); 0 at
+ * (0, 0, 0) when no value is finite. pos_* point to 3 ints; any output may be NULL. */
+athad_status athad_field_minmax(athad_ctx* ctx, int member, int field, double* vmin, int* pos_min,
+                                double* vmax, int* pos_max);
+/* findResiduumAtm (atmosphere/UtilsAtm.h:70-218): over levels 10..30 and the interior
+ * latitudes and longitudes, the residual sqrt((du/dr^2 + dv/dtheta^2 + dw/dphi^2) / 3) of the
+ * centred velocity derivatives; its maximum and position (ties: smallest (i, j, k), as the
+ * reference's tie-break; 0 at (0, 0, 0) when no residual exceeds 0), the average of the finite residuals and their count, and the count
+ * and first (i, j, k) of the non-finite ones ({-1,-1,-1} when none). The reference sums in its
+ * thread order, this in a fixed order: the average agrees to rounding. ATHAD_ERR_SHAPE for
+ * im < 32 (the reference's scan would leave the grid). The caller keeps the previous maximum
+ * for the "declining" test, as the reference keeps residuum_old. */
+athad_status athad_residuum(athad_ctx* ctx, int member, double* res_max, int* pos_max, double* avg_abs,
+                            int64_t* n_finite, int64_t* n_nonfinite, int* pos_first_nonfinite);
+
 /* --- measurement --------------------------------------------------------------------- */
 
 /* Per-kernel timing with HIP events on the context's stream: kernel ids as in
