@@ -177,9 +177,11 @@ void fill_consts(MemberHost& m, const Config& cfg, int order) {
     K.sym = (order == ATHAD_ORDER_SYMMETRIC) ? 1 : 0;
 }
 
-// Reduction scratch per member: level means, field statistics and the checkpoint diagnostics
-// (kernels' diag_offset region) with room to spare.
-int64_t scratch_doubles(int im) { return std::max<int64_t>(8192, (int64_t)diag_offset(im) + 6 * 256 + 1024); }
+// Reduction scratch per member: level means, field statistics, the checkpoint diagnostics
+// (kernels' diag_offset region) and the streamfunction's three im x jm arrays, with room to spare.
+int64_t scratch_doubles(int im, int jm) {
+    return std::max<int64_t>(8192, (int64_t)sf_offset(im) + 3LL * im * jm + 1024);
+}
 
 // 1-D tables: rad, sin/cos, layer heights, ABL taper, SatAdj dt_dim, polar passes.
 void build_tables(MemberHost& m, const Config& cfg, std::vector<double>& tab, std::vector<int>& passes) {
@@ -554,7 +556,7 @@ int64_t athad_footprint_bytes(const athad_grid* g, int precision) {
     if (!g) return -1;
     const int64_t n = (int64_t)g->im * g->jm * g->km;
     const int64_t es = (precision == ATHAD_MIXED) ? 4 : 8;
-    return n * (es * (NFIELD - 1 + NPROG + NTMP) + 8) + (int64_t)g->jm * g->km * 16 + scratch_doubles(g->im) * 8;
+    return n * (es * (NFIELD - 1 + NPROG + NTMP) + 8) + (int64_t)g->jm * g->km * 16 + scratch_doubles(g->im, g->jm) * 8;
 }
 
 athad_status athad_create(int n_members, const athad_grid* grids, const athad_member_params* params,
@@ -609,7 +611,7 @@ athad_status athad_create(int n_members, const athad_grid* grids, const athad_me
         const size_t n2 = (size_t)m.g.jm * m.g.km;
         ok = ok && alloc((void**)&m.vel_star, n2 * 8) && alloc((void**)&m.t_top, n2 * 8) &&
              alloc((void**)&m.t_ref, m.g.im * 8) && alloc((void**)&m.tables, tab.size() * 8) &&
-             alloc((void**)&m.scratch, (size_t)scratch_doubles(m.g.im) * 8) && alloc((void**)&m.counters, 64) &&
+             alloc((void**)&m.scratch, (size_t)scratch_doubles(m.g.im, m.g.jm) * 8) && alloc((void**)&m.counters, 64) &&
              alloc((void**)&m.passes, passes.size() * 4);
         if (!ok) { g_create_error = "device allocation failed"; free_ctx(c); return ATHAD_ERR_ALLOC; }
         for (int f = 0; f < NFIELD; f++) hipMemsetAsync(m.field[f], 0, field_bytes(c, m, f), c->stream);
@@ -769,6 +771,23 @@ athad_status athad_residuum(athad_ctx* c, int member, double* res_max, int* pos_
     if (n_finite) *n_finite = (int64_t)r[3];
     if (n_nonfinite) *n_nonfinite = (int64_t)r[4];
     ijk(r[5], pos_first_nonfinite);
+    return ATHAD_OK;
+}
+
+athad_status athad_meridional_streamfunction(athad_ctx* c, int member, double* vbar, double* rho_vbar, double* psi) {
+    athad_status s = check_member(c, member, 0, 1);
+    if (s) return s;
+    const MemberHost& m = c->mem[member];
+    const Config cfg;
+    join_s2(c);
+    launch_streamfunction(cur(c), member, m.g.im, m.g.jm, 2.0 * M_PI * (cfg.r_Earth * 1000.0));
+    const size_t nij = (size_t)m.g.im * m.g.jm;
+    double* outs[3] = {vbar, rho_vbar, psi};
+    for (int a = 0; a < 3; a++)
+        if (outs[a])
+            HIPCHK(c, hipMemcpyAsync(outs[a], m.scratch + sf_offset(m.g.im) + a * nij, nij * 8,
+                                     hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return ATHAD_OK;
 }
 

@@ -34,6 +34,10 @@ void launch_minmax(const Batch& b, int member, int field);
 // non-finite index at scratch[diag_offset(im)] + 6 * 256
 void launch_residuum(const Batch& b, int member);
 int diag_offset(int im);
+// write_meridional_streamfunction's arrays of one member (grid im x jm x km): vbar, rho_vbar,
+// psi, each im x jm doubles [i][j], at scratch[sf_offset(im)]
+void launch_streamfunction(const Batch& b, int member, int im, int jm, double two_pi_a);
+int sf_offset(int im);
 void launch_rk4_jacobi(const Batch& b, bool fused);
 void launch_rk4_lex(const Batch& b, int plane);
 void launch_poisson(const Batch& b, int gate);

@@ -309,6 +309,60 @@ __global__ void k_residuum_final(const MemberDesc* __restrict__ md, int member) 
     o[0] = mx; o[1] = mq; o[2] = sum; o[3] = cnt; o[4] = nan; o[5] = nq;
 }
 
+// write_meridional_streamfunction (MinMax_Atm.cpp:173-227): zonal means of v and r_humid v
+// over the fluid cells of each (i, j) (every cell: the configuration has no land), then the
+// mass streamfunction integrated down from the lid. The sums run in k order and the integral in
+// i order, as the reference's loops, with contraction off, so the three arrays are the
+// reference's to the last bit. Arrays [i][j] at sf_base: vbar, rho_vbar, psi.
+ATHAD_DEV int sf_base(const Consts& K) { return diag_base(K) + 6 * kStatChunks + 8; }
+
+template <typename S>
+__global__ __launch_bounds__(64) void k_zonal_mean_v(const MemberDesc* __restrict__ md, int member) {
+#pragma clang fp contract(off)
+    const MemberDesc& M = md[member];
+    const Consts& K = M.K;
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;   // i * jm + j
+    if (id >= K.im * K.jm) return;
+    const long long row = (long long)id * K.km;
+    const S *v = fp<S>(M, F_V), *rh = fp<S>(M, F_R_HUMID);
+    double sum = 0.0, rsum = 0.0;
+    int n = 0;
+    for (int k = 0; k < K.km; k++) {
+        const double vv = (double)v[row + k];
+        double rho = (double)rh[row + k];
+        if (!finite_bits(vv)) continue;
+        if (!finite_bits(rho) || rho <= 0.0) rho = K.r_air;   // before the first densities()
+        sum += vv;
+        rsum += rho * vv;
+        n++;
+    }
+    double* o = M.scratch + sf_base(K);
+    o[id] = (n > 0) ? (sum / n) * K.u_0 : 0.0;
+    o[K.im * K.jm + id] = (n > 0) ? (rsum / n) * K.u_0 : 0.0;
+}
+
+// One thread per latitude: psi(im-1) = 0, psi(i) = psi(i+1) + 2 pi a sin(theta) rvm dz with the
+// layer heights' float difference as dz, as get_layer_height returns float.
+__global__ void k_streamfunction(const MemberDesc* __restrict__ md, int member, double two_pi_a) {
+#pragma clang fp contract(off)
+    const MemberDesc& M = md[member];
+    const Consts& K = M.K;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= K.jm) return;
+    const long long nij = (long long)K.im * K.jm;
+    const double* rv = M.scratch + sf_base(K) + nij;
+    double* psi = M.scratch + sf_base(K) + 2 * nij;
+    const double coeff = two_pi_a * M.sinr[j];
+    double acc = 0.0;
+    psi[(long long)(K.im - 1) * K.jm + j] = 0.0;
+    for (int i = K.im - 2; i >= 0; i--) {
+        const double dz = (double)((float)M.hgt[i + 1] - (float)M.hgt[i]);
+        const double rvm = 0.5 * (rv[(long long)i * K.jm + j] + rv[(long long)(i + 1) * K.jm + j]);
+        acc = acc + coeff * rvm * dz;
+        psi[(long long)i * K.jm + j] = acc;
+    }
+}
+
 enum { FT = 0, FU, FV, FW, FC, FCL, FI, FG, FCO, FK, FD };
 
 // Solid-ground clamps of bcSolidGround passes 4-5 (BC_Atm.h:475-548) as a value transform,
@@ -2294,6 +2348,11 @@ void launch_field_stats(const Batch& b, int member, int field) {
 }
 int field_stats_offset(int im) { return 2 * kLevelChunks * im + 2 * kStatChunks; }
 int diag_offset(int im) { return 2 * kLevelChunks * im + 2 * kStatChunks + 2; }
+int sf_offset(int im) { return diag_offset(im) + 6 * kStatChunks + 8; }
+void launch_streamfunction(const Batch& b, int member, int im, int jm, double two_pi_a) {
+    ATHAD_DISPATCH(k_zonal_mean_v, dim3((im * jm + 63) / 64), dim3(64), 0, b.d_md, member);
+    hipLaunchKernelGGL(k_streamfunction, dim3((jm + 63) / 64), dim3(64), 0, b.stream, b.d_md, member, two_pi_a);
+}
 void launch_minmax(const Batch& b, int member, int field) {
     ATHAD_DISPATCH(k_minmax, dim3(kStatChunks), dim3(256), 0, b.d_md, member, field);
     hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(1), 0, b.stream, b.d_md, member);

@@ -199,6 +199,14 @@ class Integration:
         return {"max": mx.value, "pos_max": tuple(pm), "avg_abs": avg.value, "n_finite": nf.value,
                 "n_nonfinite": nn.value, "pos_first_nonfinite": tuple(pn)}
 
+    def meridional_streamfunction(self, member: int = 0) -> dict:
+        """write_meridional_streamfunction's arrays: vbar, rho_vbar, psi, each (im, jm)."""
+        im, jm, _ = self.grids[member]
+        out = {k: np.empty((im, jm)) for k in ("vbar", "rho_vbar", "psi")}
+        ptr = [a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in out.values()]
+        self._chk(L.lib().athad_meridional_streamfunction(self.h, member, *ptr))
+        return out
+
     # ---- measurement -------------------------------------------------------------------
     def profile(self, kernel: str, enable: bool = True):
         self._chk(L.lib().athad_profile(self.h, L.KERNEL_ID[kernel], int(enable)))

@@ -218,6 +218,15 @@ athad_status athad_field_minmax(athad_ctx* ctx, int member, int field, double* v
  * for the "declining" test, as the reference keeps residuum_old. */
 athad_status athad_residuum(athad_ctx* ctx, int member, double* res_max, int* pos_max, double* avg_abs,
                             int64_t* n_finite, int64_t* n_nonfinite, int* pos_first_nonfinite);
+/* write_meridional_streamfunction (atmosphere/MinMax_Atm.cpp:173-227) without downloading v and
+ * r_humid: the zonal means vbar = <v> u_0 and rho_vbar = <r_humid v> u_0 [i][j] (non-finite v
+ * skipped; non-finite or non-positive r_humid replaced by r_air), and the meridional mass
+ * streamfunction psi [i][j] in kg/s, integrated down from the lid (psi = 0 at i = im-1) with
+ * 2 pi a sin(theta) and the float layer-height differences. Summation and integration orders
+ * are the reference's, so the arrays are bit for bit its own. Each output points to im * jm
+ * doubles or is NULL. The CSV and the log line stay the caller's. */
+athad_status athad_meridional_streamfunction(athad_ctx* ctx, int member, double* vbar, double* rho_vbar,
+                                             double* psi);
 
 /* --- measurement --------------------------------------------------------------------- */
 

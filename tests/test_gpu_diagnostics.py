@@ -145,3 +145,96 @@ def test_residuum_needs_32_levels():
         integ.init_state(1)
         with pytest.raises(A.AthadError, match="status 4"):
             integ.residuum()
+
+
+TWO_PI_A = 2.0 * math.pi * (6370.001 * 1000.0)   # 2 pi r_Earth, r_Earth = 6370.001 km
+U_0 = 8.0
+
+
+def layer_heights(im):
+    """init_layer_heights: (exp(zeta (r - r0)) - 1) L_atm, stored as float."""
+    dr, rad = 1.0 / (im - 1), [1.0]
+    for _ in range(im - 1):
+        rad.append(rad[-1] + dr)
+    return np.array([np.float32((math.exp(3.0 * (r - rad[0])) - 1.0) * 6287.5) for r in rad], dtype=np.float32)
+
+
+def colatitudes(jm):
+    dthe, the = (180.0 / (jm - 1)) / (180.0 / math.pi), [0.0]
+    for _ in range(jm - 1):
+        the.append(the[-1] + dthe)
+    return the
+
+
+def streamfunction_numpy(v, rho, r_air):
+    """write_meridional_streamfunction's loops: sequential sums over k, integral down in i."""
+    im, jm, _ = v.shape
+    fin = np.isfinite(v)
+    rho = np.where(np.isfinite(rho) & (rho > 0.0), rho, r_air)
+    vv = np.where(fin, v, 0.0)
+    s = np.cumsum(vv, axis=2)[..., -1]            # cumsum adds in order, as the reference's loop
+    rs = np.cumsum(rho * vv, axis=2)[..., -1]
+    n = fin.sum(axis=2)
+    vbar = np.where(n > 0, (s / np.maximum(n, 1)) * U_0, 0.0)
+    rvbar = np.where(n > 0, (rs / np.maximum(n, 1)) * U_0, 0.0)
+    h = layer_heights(im)
+    coeff = TWO_PI_A * np.sin(np.array(colatitudes(jm)))
+    psi = np.zeros((im, jm))
+    for i in range(im - 2, -1, -1):
+        dz = np.float64(h[i + 1] - h[i])
+        psi[i] = psi[i + 1] + coeff * (0.5 * (rvbar[i] + rvbar[i + 1])) * dz
+    return vbar, rvbar, psi
+
+
+def test_streamfunction_closed_form():
+    im, jm, km = 21, 19, 41
+    print(f"case streamfunction closed form {(im, jm, km)}", flush=True)
+    r_air = A.default_params().r_air
+    c, r = 2.0 ** -4, 0.5   # every partial sum of these is exact
+    with A.Integration((im, jm, km)) as integ:
+        integ.init_state(1)
+        integ.set("v", np.full((im, jm, km), c))
+        integ.set("r_humid", np.full((im, jm, km), r))
+        out = integ.meridional_streamfunction()
+        assert (out["vbar"] == c * U_0).all() and (out["rho_vbar"] == r * c * U_0).all()
+        # constant rho v: psi(i) = 2 pi a sin(theta) rho v u_0 (sum of the layer thicknesses above
+        # i), each thickness the float difference of the float heights (get_layer_height returns
+        # float); the double difference of the same heights is off by ~5e-10 here
+        h = layer_heights(im)
+        dz = (h[1:] - h[:-1]).astype(np.float64)
+        above = np.concatenate([np.cumsum(dz[::-1])[::-1], [0.0]])
+        coeff = TWO_PI_A * np.sin(np.array(colatitudes(jm)))
+        expect = coeff[None, :] * (r * c * U_0) * above[:, None]
+        assert (out["psi"][-1] == 0.0).all()
+        np.testing.assert_allclose(out["psi"], expect, rtol=1e-13, atol=1e-6)
+        # a non-finite v leaves the mean; a non-positive density is replaced by r_air
+        v = np.full((im, jm, km), c)
+        v[3, 4, 5] = np.nan
+        rho = np.full((im, jm, km), r)
+        rho[6, 7, 8] = -1.0
+        integ.set("v", v)
+        integ.set("r_humid", rho)
+        out = integ.meridional_streamfunction()
+        assert out["vbar"][3, 4] == ((c * (km - 1)) / (km - 1)) * U_0
+        assert out["rho_vbar"][3, 4] == ((r * c * (km - 1)) / (km - 1)) * U_0
+        assert out["rho_vbar"][6, 7] == ((r * c * (km - 1) + r_air * c) / km) * U_0
+        # no finite v in a row: 0
+        v[2, 3, :] = np.inf
+        integ.set("v", v)
+        out = integ.meridional_streamfunction()
+        assert out["vbar"][2, 3] == 0.0 and out["rho_vbar"][2, 3] == 0.0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_streamfunction_of_a_developed_state(precision):
+    grid = (21, 37, 73)
+    print(f"case streamfunction developed {grid} {precision}", flush=True)
+    with A.Integration(grid, precision=precision) as integ:
+        integ.init_state(3)
+        integ.run(10)
+        out = integ.meridional_streamfunction()
+        vbar, rvbar, psi = streamfunction_numpy(integ.get("v"), integ.get("r_humid"), A.default_params().r_air)
+        np.testing.assert_array_equal(out["vbar"], vbar)
+        np.testing.assert_array_equal(out["rho_vbar"], rvbar)
+        np.testing.assert_array_equal(out["psi"], psi)
+        assert np.abs(psi).max() > 0.0
