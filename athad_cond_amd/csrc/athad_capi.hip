@@ -178,9 +178,11 @@ void fill_consts(MemberHost& m, const Config& cfg, int order) {
 }
 
 // Reduction scratch per member: level means, field statistics, the checkpoint diagnostics
-// (kernels' diag_offset region) and the streamfunction's three im x jm arrays, with room to spare.
-int64_t scratch_doubles(int im, int jm) {
-    return std::max<int64_t>(8192, (int64_t)sf_offset(im) + 3LL * im * jm + 1024);
+// (kernels' diag_offset region) and the streamfunction's three im x jm arrays or one plane of a
+// field, with room to spare.
+int64_t scratch_doubles(int im, int jm, int km) {
+    const int64_t plane = std::max<int64_t>({3LL * im * jm, (int64_t)jm * km, (int64_t)im * km});
+    return std::max<int64_t>(8192, (int64_t)sf_offset(im) + plane + 1024);
 }
 
 // 1-D tables: rad, sin/cos, layer heights, ABL taper, SatAdj dt_dim, polar passes.
@@ -556,7 +558,7 @@ int64_t athad_footprint_bytes(const athad_grid* g, int precision) {
     if (!g) return -1;
     const int64_t n = (int64_t)g->im * g->jm * g->km;
     const int64_t es = (precision == ATHAD_MIXED) ? 4 : 8;
-    return n * (es * (NFIELD - 1 + NPROG + NTMP) + 8) + (int64_t)g->jm * g->km * 16 + scratch_doubles(g->im, g->jm) * 8;
+    return n * (es * (NFIELD - 1 + NPROG + NTMP) + 8) + (int64_t)g->jm * g->km * 16 + scratch_doubles(g->im, g->jm, g->km) * 8;
 }
 
 athad_status athad_create(int n_members, const athad_grid* grids, const athad_member_params* params,
@@ -611,7 +613,7 @@ athad_status athad_create(int n_members, const athad_grid* grids, const athad_me
         const size_t n2 = (size_t)m.g.jm * m.g.km;
         ok = ok && alloc((void**)&m.vel_star, n2 * 8) && alloc((void**)&m.t_top, n2 * 8) &&
              alloc((void**)&m.t_ref, m.g.im * 8) && alloc((void**)&m.tables, tab.size() * 8) &&
-             alloc((void**)&m.scratch, (size_t)scratch_doubles(m.g.im, m.g.jm) * 8) && alloc((void**)&m.counters, 64) &&
+             alloc((void**)&m.scratch, (size_t)scratch_doubles(m.g.im, m.g.jm, m.g.km) * 8) && alloc((void**)&m.counters, 64) &&
              alloc((void**)&m.passes, passes.size() * 4);
         if (!ok) { g_create_error = "device allocation failed"; free_ctx(c); return ATHAD_ERR_ALLOC; }
         for (int f = 0; f < NFIELD; f++) hipMemsetAsync(m.field[f], 0, field_bytes(c, m, f), c->stream);
@@ -787,6 +789,22 @@ athad_status athad_meridional_streamfunction(athad_ctx* c, int member, double* v
         if (outs[a])
             HIPCHK(c, hipMemcpyAsync(outs[a], m.scratch + sf_offset(m.g.im) + a * nij, nij * 8,
                                      hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ATHAD_OK;
+}
+
+athad_status athad_get_slice(athad_ctx* c, int member, int field, int axis, int index, double* dst) {
+    athad_status s = check_member(c, member, field, NFIELD);
+    if (s) return s;
+    if (!dst) return ATHAD_ERR_ARG;
+    const MemberHost& m = c->mem[member];
+    const int extent[3] = {m.g.im, m.g.jm, m.g.km};
+    if (axis < 0 || axis > 2) { c->err = "slice axis must be 0 (i), 1 (j) or 2 (k)"; return ATHAD_ERR_ARG; }
+    if (index < 0 || index >= extent[axis]) { c->err = "slice index outside the grid"; return ATHAD_ERR_ARG; }
+    const long long n = (long long)m.g.im * m.g.jm * m.g.km / extent[axis];
+    join_s2(c);
+    launch_slice(cur(c), member, field, axis, index, n);
+    HIPCHK(c, hipMemcpyAsync(dst, m.scratch + sf_offset(m.g.im), (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return ATHAD_OK;
 }

@@ -38,6 +38,9 @@ int diag_offset(int im);
 // psi, each im x jm doubles [i][j], at scratch[sf_offset(im)]
 void launch_streamfunction(const Batch& b, int member, int im, int jm, double two_pi_a);
 int sf_offset(int im);
+// one plane (n values) of one field, widened to double, at scratch[sf_offset(im)]: axis 0 fixes
+// i ([j][k]), 1 fixes j ([i][k]), 2 fixes k ([i][j])
+void launch_slice(const Batch& b, int member, int field, int axis, int index, long long n);
 void launch_rk4_jacobi(const Batch& b, bool fused);
 void launch_rk4_lex(const Batch& b, int plane);
 void launch_poisson(const Batch& b, int gate);

@@ -363,6 +363,31 @@ __global__ void k_streamfunction(const MemberDesc* __restrict__ md, int member, 
     }
 }
 
+// One plane of one field, widened to double, at sf_base: the level i = index ([j][k], the
+// ParaView radial slice and the transfer file's surface), the meridian k = index ([i][j], zonal)
+// or the latitude j = index ([i][k], longal); Paraview_Atm.cpp:407-1000, FileIO_Atm.cpp:64-120.
+template <typename S>
+__global__ __launch_bounds__(256) void k_slice(const MemberDesc* __restrict__ md, int member, int field, int axis,
+                                               int index) {
+    const MemberDesc& M = md[member];
+    const Consts& K = M.K;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long n, q;
+    if (axis == 0) {
+        n = (long long)K.jm * K.km;
+        q = (long long)index * n + id;
+    } else if (axis == 1) {
+        n = (long long)K.im * K.km;
+        q = ((id / K.km) * K.jm + index) * K.km + id % K.km;
+    } else {
+        n = (long long)K.im * K.jm;
+        q = id * K.km + index;
+    }
+    if (id >= n) return;
+    const bool dbl = (field == F_P_STAT) || sizeof(S) == 8;
+    M.scratch[sf_base(K) + id] = dbl ? ((const double*)M.f[field])[q] : (double)((const float*)M.f[field])[q];
+}
+
 enum { FT = 0, FU, FV, FW, FC, FCL, FI, FG, FCO, FK, FD };
 
 // Solid-ground clamps of bcSolidGround passes 4-5 (BC_Atm.h:475-548) as a value transform,
@@ -2349,6 +2374,9 @@ void launch_field_stats(const Batch& b, int member, int field) {
 int field_stats_offset(int im) { return 2 * kLevelChunks * im + 2 * kStatChunks; }
 int diag_offset(int im) { return 2 * kLevelChunks * im + 2 * kStatChunks + 2; }
 int sf_offset(int im) { return diag_offset(im) + 6 * kStatChunks + 8; }
+void launch_slice(const Batch& b, int member, int field, int axis, int index, long long n) {
+    ATHAD_DISPATCH(k_slice, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b.d_md, member, field, axis, index);
+}
 void launch_streamfunction(const Batch& b, int member, int im, int jm, double two_pi_a) {
     ATHAD_DISPATCH(k_zonal_mean_v, dim3((im * jm + 63) / 64), dim3(64), 0, b.d_md, member);
     hipLaunchKernelGGL(k_streamfunction, dim3((jm + 63) / 64), dim3(64), 0, b.stream, b.d_md, member, two_pi_a);

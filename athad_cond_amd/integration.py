@@ -207,6 +207,15 @@ class Integration:
         self._chk(L.lib().athad_meridional_streamfunction(self.h, member, *ptr))
         return out
 
+    def get_slice(self, name: str, axis: int, index: int, member: int = 0) -> np.ndarray:
+        """One plane of a field: axis 0 fixes i (jm, km), 1 fixes j (im, km), 2 fixes k (im, jm)."""
+        im, jm, km = self.grids[member]
+        shape = [(jm, km), (im, km), (im, jm)][axis] if axis in (0, 1, 2) else (1,)
+        out = np.empty(shape)
+        self._chk(L.lib().athad_get_slice(self.h, member, L.FIELD_ID[name], axis, index,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
     # ---- measurement -------------------------------------------------------------------
     def profile(self, kernel: str, enable: bool = True):
         self._chk(L.lib().athad_profile(self.h, L.KERNEL_ID[kernel], int(enable)))

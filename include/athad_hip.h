@@ -227,6 +227,14 @@ athad_status athad_residuum(athad_ctx* ctx, int member, double* res_max, int* po
  * doubles or is NULL. The CSV and the log line stay the caller's. */
 athad_status athad_meridional_streamfunction(athad_ctx* ctx, int member, double* vbar, double* rho_vbar,
                                              double* psi);
+/* One plane of one field, widened to double, for the output writers that read a plane only:
+ * axis 0 fixes the level i = index ([j][k], jm * km values: paraview_vtk_radial,
+ * atmosphere/Paraview_Atm.cpp:407-627, and the surface row of AtmosphereDataTransfer,
+ * atmosphere/FileIO_Atm.cpp:64-120), axis 2 the meridian k = index ([i][j], im * jm:
+ * paraview_vtk_zonal, :628-827), axis 1 the latitude j = index ([i][k], im * km:
+ * paraview_vtk_longal, :828-). The writers' scaling, formatting and finiteness checks stay in
+ * the caller. ATHAD_ERR_ARG for an axis or index outside the grid. */
+athad_status athad_get_slice(athad_ctx* ctx, int member, int field, int axis, int index, double* dst);
 
 /* --- measurement --------------------------------------------------------------------- */
 

@@ -238,3 +238,24 @@ def test_streamfunction_of_a_developed_state(precision):
         np.testing.assert_array_equal(out["rho_vbar"], rvbar)
         np.testing.assert_array_equal(out["psi"], psi)
         assert np.abs(psi).max() > 0.0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_slices_are_the_planes_of_the_field(precision):
+    grid = (13, 21, 41)
+    im, jm, km = grid
+    print(f"case slices {grid} {precision}", flush=True)
+    with A.Integration(grid, precision=precision) as integ:
+        integ.init_state(5)
+        integ.run(4)
+        for name in ["t", "v", "w", "p_dyn", "p_stat", "c"]:
+            x = integ.get(name)
+            for i in (0, im // 2, im - 1):
+                np.testing.assert_array_equal(integ.get_slice(name, 0, i), x[i])
+            for j in (0, jm // 2, jm - 1):
+                np.testing.assert_array_equal(integ.get_slice(name, 1, j), x[:, j, :])
+            for k in (0, km // 3, km - 1):
+                np.testing.assert_array_equal(integ.get_slice(name, 2, k), x[:, :, k])
+        for axis, index in [(0, im), (1, -1), (2, km), (3, 0)]:
+            with pytest.raises(A.AthadError, match="status 1"):
+                integ.get_slice("t", axis, index)
