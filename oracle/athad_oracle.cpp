@@ -169,6 +169,10 @@ struct Params {
     double x_H2O = 0.5578, x_CO2 = 0.4109, x_N2 = 0.0313;
     double u_0 = 8.0, cloud_cap = 0.05, t_max_phys = 2000.0, t_00 = 236.15;
     double c_0 = 0.3464, c_h2o_dry_top = 0.00423, co2_0 = 0.6233, co2_scale = 1.0;
+    // radiation (cAtmosphereDefaults.cpp.inc:31-43,72-73; omega_rad: cAtmosphereModel.cpp:801)
+    double sigma = 5.67028e-08, kappa_H2O = 0.01, kappa_CO2 = 0.001, kappa_bg = 1e-06;
+    double geothermal_flux = 150.0, rad_equator_short = 298.0, rad_pole_short = 0.0;
+    double albedo_surface = 0.08, albedo_cloud = 0.5, t_skin_relax = 0.25, omega_rad = 0.05;
     Comp comp;
 };
 
@@ -193,6 +197,9 @@ struct orc_state {
     std::vector<std::vector<double>> f;
     std::vector<std::vector<double>> f2;
     std::vector<double> t_ref;
+    // radiation outputs: upward long-wave flux and emissivity per cell, albedo per column,
+    // incoming short wave per latitude
+    std::vector<double> radiation, epsilon, albedo, sw;
     int iter_n = 0, total_iter = 0;
 
     inline size_t ix(int i, int j, int k) const { return ((size_t)i * jm + j) * km + k; }
@@ -1360,6 +1367,128 @@ void init_state(S& s, uint64_t seed, int stop_after_thermo) {
     s.total_iter = 0;
 }
 
+// ---------------------------------------------------------------------------------------
+// Radiation in run_3D_loop's mode 2 (cAtmosphereModel.cpp:799-801, 863-879, 1655-1661):
+// MultiLayerRadiation::run (MultiLayerRadiation.h:59-384) on the current t, every t blended
+// by omega_rad toward that radiative result, then one step of the t_skin fixed point
+// (updateSkinTemperature + planetaryShortWave, cAtmosphereModel.cpp:241-311).
+// ---------------------------------------------------------------------------------------
+// Short wave at the top per latitude: a parabola from pole to equator, mirrored (MLR.h:98-104).
+void short_wave(S& s) {
+    const Params& P = s.P;
+    const int jmax = s.jm - 1, jhalf = jmax / 2;
+    s.sw.assign(s.jm, P.rad_pole_short);
+    const double span = P.rad_pole_short - P.rad_equator_short;
+    for (int j = jhalf; j >= 0; j--) {
+        const double x = (double)j / (double)jhalf;
+        s.sw[j] = span * (x * x - 2.0 * x) + P.rad_pole_short;   // AtomUtils::parabola
+    }
+    for (int j = jmax; j > jhalf; j--) s.sw[j] = s.sw[jmax - j];
+}
+
+// Grey two-stream sweeps over one column (MLR.h:291-303, 323-333): up from the black surface,
+// dn from a dark sky; level i's emission eps_i sigma T_i^4.
+void sweeps(int im, const double* eps, const double* T, double sigma, double* up, double* dn) {
+    up[0] = sigma * std::pow(T[0], 4.0);
+    for (int i = 1; i < im; i++) up[i] = up[i - 1] * (1.0 - eps[i]) + eps[i] * sigma * std::pow(T[i], 4.0);
+    dn[im] = 0.0;
+    for (int i = im - 1; i >= 1; i--) dn[i] = dn[i + 1] * (1.0 - eps[i]) + eps[i] * sigma * std::pow(T[i], 4.0);
+}
+
+void radiation(S& s) {
+    const Params& P = s.P;
+    const int im = s.im, jm = s.jm, km = s.km;
+    short_wave(s);
+    s.radiation.assign(s.n, 0.0);
+    s.epsilon.assign(s.n, 0.0);
+    s.albedo.assign((size_t)jm * km, 0.0);
+    double *t = s.F(ORC_T), *c = s.F(ORC_C), *co2 = s.F(ORC_CO2), *p = s.F(ORC_P_STAT);
+    double *cloud = s.F(ORC_CLOUD), *ice = s.F(ORC_ICE);
+    const double inv_g = 1.0 / P.g;
+    const int top = im - 1;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < jm; j++) {
+        std::vector<double> T(im), eps(im), up(im), dn(im + 1);
+        for (int k = 0; k < km; k++) {
+            for (int i = 0; i < im; i++) T[i] = t[s.ix(i, j, k)] * P.t_0;
+            // layer optical depth: gas column mass with pressure broadening plus the cloud
+            // liquid and ice paths (MLR.h:156-207)
+            double lwp = 0.0, iwp = 0.0;
+            for (int i = 0; i < im; i++) {
+                const size_t q = s.ix(i, j, k);
+                double dp = (i < top) ? (p[q] - p[s.ix(i + 1, j, k)]) * 100.0 : p[q] * 100.0;
+                if (dp < 0.0) dp = 0.0;
+                const double qv = dmax(0.0, c[q]), qc = dmax(0.0, co2[q]);
+                const double qb = dmax(0.0, 1.0 - qv - qc);
+                const double mass = dp * inv_g;
+                const double broad = p[q] * 100.0 / 1.0e5;
+                const double tau_gas = (P.kappa_H2O * qv + P.kappa_CO2 * qc + P.kappa_bg * qb) * mass * broad;
+                const float h0 = s.hgt[(i < top) ? i : i - 1], h1 = s.hgt[(i < top) ? i + 1 : i];
+                const double dz = (double)(h1 - h0);   // float heights, float difference
+                const double Ti = t[q] * P.t_0;
+                const double rho = (Ti > 0.0) ? (p[q] * 100.0) / (R_of(qv, qc, P.comp.R_bg) * Ti) : 0.0;
+                const double lw = dmax(0.0, cloud[q]) * rho * dz * 1000.0;
+                const double iw = dmax(0.0, ice[q]) * rho * dz * 1000.0;
+                lwp += lw;
+                iwp += iw;
+                const double tau = tau_gas + 0.12 * lw + 0.055 * iw;
+                eps[i] = (tau > 700.0) ? 1.0 : (1.0 - std::exp(-tau));
+                s.epsilon[q] = eps[i];
+            }
+            // clear-sky surface albedo raised toward the cloud albedo by the capped
+            // condensate path (MLR.h:90-96, 231-238)
+            double alb = P.albedo_surface;
+            if (P.albedo_cloud > alb) {
+                const double tau = (lwp + 0.5 * iwp) / 100.0;
+                alb = alb + (P.albedo_cloud - alb) * (tau / (tau + 2.0));
+            }
+            s.albedo[(size_t)j * km + k] = alb;
+            // four Lambda iterations of layer radiative equilibrium sigma T^4 = (up + dn) / 2
+            // (MLR.h:287-320), then the sweeps once more on the final profile
+            for (int it = 0; it < 4; it++) {
+                sweeps(im, eps.data(), T.data(), P.sigma, up.data(), dn.data());
+                for (int i = 1; i <= top; i++) {
+                    const double emit = 0.5 * (up[i - 1] + dn[i + 1]);
+                    if (emit > 0.0 && finite_bits(emit)) T[i] = std::pow(emit / P.sigma, 0.25);
+                }
+            }
+            sweeps(im, eps.data(), T.data(), P.sigma, up.data(), dn.data());
+            // surface energy balance, sigma T^4 linearised about the current surface
+            // temperature, with the bulk flux to the lowest layer (MLR.h:338-371)
+            const double Ts0 = T[0], c_H = 15.0;
+            const double d4 = 4.0 * P.sigma * Ts0 * Ts0 * Ts0;
+            const double Ts = ((1.0 - alb) * s.sw[j] + P.geothermal_flux + dn[1] - P.sigma * std::pow(Ts0, 4.0) +
+                               d4 * Ts0 + c_H * T[1]) / (d4 + c_H);
+            T[0] = Ts;
+            s.radiation[s.ix(0, j, k)] = P.sigma * std::pow(Ts, 4.0);
+            for (int i = 1; i <= top; i++) s.radiation[s.ix(i, j, k)] = up[i];
+            // apply_radiative_heating: t <- (1 - omega_rad) t + omega_rad T_rad / t_0
+            for (int i = 0; i < im; i++) {
+                const size_t q = s.ix(i, j, k);
+                t[q] = t[q] * (1.0 - P.omega_rad) + (T[i] / P.t_0) * P.omega_rad;
+            }
+        }
+    }
+    // t_skin relaxes toward the emission temperature of the absorbed short wave plus the
+    // geothermal flux, with the cos(latitude)-weighted planetary means of the model's albedo
+    if (P.t_skin_relax <= 0.0) return;
+    double wsum = 0.0, sw_abs = 0.0, amax = 0.0;
+    for (int j = 0; j < jm; j++) {
+        const double w = std::cos((j / (double)(jm - 1) - 0.5) * M_PI);
+        double a = 0.0;
+        for (int k = 0; k < km; k++) a += s.albedo[(size_t)j * km + k];
+        a /= (double)km;
+        amax = dmax(amax, a);
+        wsum += w;
+        sw_abs += w * (1.0 - a) * s.sw[j];
+    }
+    if (!(wsum > 0.0) || !(amax > 0.0)) return;
+    const double in = sw_abs / wsum + P.geothermal_flux;
+    if (!(in > 0.0)) return;
+    const double target = std::pow(in / P.sigma, 0.25);
+    s.P.t_skin += dmin(1.0, P.t_skin_relax) * (target - P.t_skin);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
@@ -1392,8 +1521,14 @@ double* orc_field2d_ptr(orc_state* s, int f) { return (f >= 0 && f < ORC2_NFIELD
 int orc_total_iter(const orc_state* s) { return s->total_iter; }
 void orc_set_iter(orc_state* s, int iter_n, int total) { s->iter_n = iter_n; s->total_iter = total; }
 double orc_level_mean(const orc_state* s, int i) { return s->t_ref[i]; }
+void orc_radiation(orc_state* s) { radiation(*s); }
+double* orc_rad_ptr(orc_state* s, int which) {
+    std::vector<double>* v[4] = {&s->radiation, &s->epsilon, &s->albedo, &s->sw};
+    return (which >= 0 && which < 4 && !v[which]->empty()) ? v[which]->data() : nullptr;
+}
 double orc_scalar(const orc_state* s, const char* name) {
     const std::string n(name);
+    if (n == "t_skin") return s->P.t_skin;
     if (n == "dr") return s->dr;
     if (n == "dthe") return s->dthe;
     if (n == "dphi") return s->dphi;

@@ -79,6 +79,14 @@ void orc_run(orc_state* s, int n, int order);
 // ramps are taken from the current total_iter_count.
 void orc_op(orc_state* s, int op, int order);
 
+// Radiation of run_3D_loop's mode 2, called after a step: MultiLayerRadiation on the current t
+// (MultiLayerRadiation.h:59-384), t blended 5 % toward it (cAtmosphereModel.cpp:863-879), one
+// t_skin relaxation (cAtmosphereModel.cpp:276-311; later densities use the new t_skin).
+void orc_radiation(orc_state* s);
+// Its outputs: 0 radiation (upward long wave per cell), 1 epsilon (per cell), 2 albedo
+// (jm x km), 3 top short wave (jm); NULL before the first call. orc_scalar(s, "t_skin").
+double* orc_rad_ptr(orc_state* s, int which);
+
 // Individual thermodynamic functions (known-answer tests).
 double orc_p_sat(double T);
 double orc_p_subl(double T);

@@ -82,6 +82,9 @@ def lib():
         L.orc_M_nonwater.argtypes = [d, d, d]
         L.orc_resolve.argtypes = [d, d, d, dp]
         L.orc_satadj_cells.argtypes = [i, dp, dp, dp, dp, dp, dp, dp, dp, d, i]
+        L.orc_radiation.argtypes = [vp]
+        L.orc_rad_ptr.restype = dp
+        L.orc_rad_ptr.argtypes = [vp, i]
         _lib = L
     return _lib
 
@@ -156,6 +159,20 @@ class OracleState:
     def op(self, name, order="lex"):
         lib().orc_op(self.h, OPS.index(name), ORDER[order])
 
+    def radiation(self):
+        """Mode-2 radiation after a step: MLR, the 5 % blend of t, one t_skin relaxation."""
+        lib().orc_radiation(self.h)
+
+    def rad_outputs(self) -> dict:
+        im, jm, km = self.shape
+        shapes = [(im, jm, km), (im, jm, km), (jm, km), (jm,)]
+        out = {}
+        for w, (name, shp) in enumerate(zip(RAD_OUTPUTS, shapes)):
+            p = lib().orc_rad_ptr(self.h, w)
+            out[name] = np.ctypeslib.as_array(p, shape=shp).copy() if p else None
+        out["t_skin"] = self.scalar("t_skin")
+        return out
+
     def level_mean(self):
         return np.array([lib().orc_level_mean(self.h, i) for i in range(self.shape[0])])
 
@@ -168,6 +185,20 @@ class OracleState:
                 f.write(np.ascontiguousarray(self.field(n), dtype=np.float64).tobytes())
             for n in FIELDS2D:
                 f.write(np.ascontiguousarray(self.field2d(n), dtype=np.float64).tobytes())
+
+
+RAD_OUTPUTS = ["radiation", "epsilon", "albedo", "sw"]
+
+
+def read_rad(path, shape):
+    """Radiation outputs written by oracle/ref_harness.cpp next to a state file."""
+    im, jm, km = shape
+    n = im * jm * km
+    raw = np.fromfile(path, dtype=np.float64)
+    assert raw.size == 2 * n + jm * km + jm + 1, "radiation file size"
+    return {"radiation": raw[:n].reshape(shape), "epsilon": raw[n:2 * n].reshape(shape),
+            "albedo": raw[2 * n:2 * n + jm * km].reshape(jm, km), "sw": raw[2 * n + jm * km:-1],
+            "t_skin": float(raw[-1])}
 
 
 def read_bin(path):

@@ -10,6 +10,11 @@
 //   mode = op:<name>        one operator: poisson satadj damp_moist moist_clamp densities
 //                           evaporation turbulence damp_turb bc rk4 polar radial store
 //   mode = init             the reference's thermodynamic init chain on a blank state
+//   mode = radiation        run_3D_loop's mode-2 radiation once (MultiLayerRadiation, the
+//                           omega_rad blend of t, updateSkinTemperature)
+//   mode = steps_rad        n iterations of the scoped rows, each followed by that radiation
+// The radiation modes also write <out.bin>.rad: radiation and epsilon (im*jm*km doubles each),
+// albedo (jm*km), the top short wave (jm) and t_skin.
 // State file: int32 im,jm,km,iter_n,total_iter; then 35 fields of im*jm*km doubles
 // (athad_oracle.h order); then vel_star and t_top_init (jm*km doubles each).
 #include "cAtmosphereModel.h"
@@ -19,6 +24,7 @@
 #include "UtilsAtm.h"
 #include "BC_Atm.h"
 #include "TurbulenceAtm.h"
+#include "MultiLayerRadiation.h"
 
 #include <cstdio>
 #include <cstring>
@@ -215,6 +221,45 @@ public:
         op(m, "store");
     }
 
+    // apply_radiative_heating + updateSkinTemperature, the radiation_mode == 2 branch of
+    // run_3D_loop (cAtmosphereModel.cpp:863-879, 1655-1661), repeated here because the blend is
+    // a lambda inside run_3D_loop. omega_rad is its constexpr 0.05 (:801).
+    static void radiation(cAtmosphereModel& m) {
+        constexpr double omega_rad = 0.05;
+        std::vector<double> tb((size_t)m.im * m.jm * m.km);
+        size_t q = 0;
+        for (int i = 0; i < m.im; i++)
+            for (int j = 0; j < m.jm; j++)
+                for (int k = 0; k < m.km; k++) tb[q++] = m.t.x[i][j][k];
+        MultiLayerRadiation(m).run();
+        q = 0;
+        for (int i = 0; i < m.im; i++)
+            for (int j = 0; j < m.jm; j++)
+                for (int k = 0; k < m.km; k++) {
+                    const double re = m.t.x[i][j][k];
+                    m.t.x[i][j][k] = tb[q++] * (1.0 - omega_rad) + re * omega_rad;
+                }
+        m.updateSkinTemperature(false);
+    }
+
+    static void save_rad(cAtmosphereModel& m, const std::string& fn) {
+        std::ofstream f(fn, std::ios::binary);
+        std::vector<double> buf((size_t)m.im * m.jm * m.km);
+        for (Array* a : {&m.radiation, &m.epsilon}) {
+            size_t q = 0;
+            for (int i = 0; i < m.im; i++)
+                for (int j = 0; j < m.jm; j++)
+                    for (int k = 0; k < m.km; k++) buf[q++] = a->x[i][j][k];
+            f.write((const char*)buf.data(), buf.size() * 8);
+        }
+        std::vector<double> b2((size_t)m.jm * m.km);
+        for (int j = 0; j < m.jm; j++)
+            for (int k = 0; k < m.km; k++) b2[(size_t)j * m.km + k] = m.albedo.y[j][k];
+        f.write((const char*)b2.data(), b2.size() * 8);
+        f.write((const char*)m.short_wave_radiation.data(), m.jm * 8);
+        f.write((const char*)&m.t_skin, 8);
+    }
+
     // RunTimeSlice's thermodynamic init chain, cAtmosphereModel.cpp:527-568 (without
     // initCloudIce; condensate stays zero as resetArrays left it).
     static void init_chain(cAtmosphereModel& m) {
@@ -244,6 +289,16 @@ int main(int argc, char** argv) {
     if (mode == "steps") {
         const int n = (argc > 4) ? std::atoi(argv[4]) : 1;
         for (int q = 0; q < n; q++) RadiationSelfTest::step(*m);
+    } else if (mode == "radiation") {
+        RadiationSelfTest::radiation(*m);
+        RadiationSelfTest::save_rad(*m, std::string(argv[2]) + ".rad");
+    } else if (mode == "steps_rad") {
+        const int n = (argc > 4) ? std::atoi(argv[4]) : 1;
+        for (int q = 0; q < n; q++) {
+            RadiationSelfTest::step(*m);
+            RadiationSelfTest::radiation(*m);
+        }
+        RadiationSelfTest::save_rad(*m, std::string(argv[2]) + ".rad");
     } else if (mode.rfind("op:", 0) == 0) {
         RadiationSelfTest::op(*m, mode.substr(3));
     } else if (mode == "init") {

@@ -93,6 +93,24 @@ def main():
         print(f"op {op:12s}: worst rel L2 {worst(report['ops'][op]):.3e}  ({time.time()-t0:.1f}s)", flush=True)
         del s2
 
+    # 2b. mode-2 radiation once on the same state (MLR, the 5 % blend of t, t_skin)
+    t0 = time.time()
+    s2 = O.OracleState(im, jm, km, blank=True)
+    s2.load(st.snapshot())
+    s2.set_iter(iter_n, st.total_iter)
+    s2.save_bin(base, iter_n=iter_n)
+    s2.radiation()
+    ref = run_ref(base, "radiation", threads=8, workdir=wd)
+    rr, ro = O.read_rad(os.path.join(wd, "ref_out.bin.rad"), (im, jm, km)), s2.rad_outputs()
+    report["radiation"] = compare(s2.snapshot(), ref, ["t"])
+    for name in ("radiation", "epsilon", "albedo", "sw"):
+        report["radiation"][name] = {"rel_l2": O.rel_l2(ro[name], rr[name]),
+                                     "max_abs": float(np.max(np.abs(ro[name] - rr[name])))}
+    report["radiation"]["t_skin"] = {"rel_l2": abs(ro["t_skin"] / rr["t_skin"] - 1.0), "max_abs": abs(ro["t_skin"] - rr["t_skin"])}
+    print(f"radiation       : worst rel L2 {max(v['rel_l2'] for v in report['radiation'].values()):.3e}  "
+          f"({time.time()-t0:.1f}s)", flush=True)
+    del s2
+
     # 3. whole scoped steps from the initial state (lex = 1 reference thread)
     t0 = time.time()
     st = O.OracleState(im, jm, km, seed=1234)

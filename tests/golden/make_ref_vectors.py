@@ -6,7 +6,8 @@ built-in grid 61 x 181 x 361) and stores, per check, the reference's output at a
 sample of cells plus per-field sums and L2 norms. The inputs are regenerated by the oracle
 in the tests from the same seed, so only outputs are stored.
 
-    python tests/golden/make_ref_vectors.py
+    python tests/golden/make_ref_vectors.py               # ref_vectors.npz
+    python tests/golden/make_ref_vectors.py --radiation   # ref_radiation.npz
 """
 from __future__ import annotations
 
@@ -54,7 +55,53 @@ def pack(prefix, d, idx, store):
         store[f"{prefix}/{n}/l2"] = np.array(np.sqrt(np.sum(a * a)))
 
 
+def pack_rad(prefix, rad, idx, idx2, store):
+    """Radiation outputs: sampled cells (columns for albedo), sums and L2 norms, the short
+    wave per latitude and t_skin whole."""
+    for name, ix in (("radiation", idx), ("epsilon", idx), ("albedo", idx2)):
+        a = rad[name].reshape(-1)
+        store[f"{prefix}/{name}/sample"] = a[ix]
+        store[f"{prefix}/{name}/sum"] = np.array(np.sum(a))
+        store[f"{prefix}/{name}/l2"] = np.array(np.sqrt(np.sum(a * a)))
+    store[f"{prefix}/sw"] = rad["sw"]
+    store[f"{prefix}/t_skin"] = np.array(rad["t_skin"])
+
+
+def radiation_vectors():
+    """Mode-2 radiation (MultiLayerRadiation, the 5 % blend of t, updateSkinTemperature): once
+    on the Jacobi-spun-up state, and after each of two whole lex iterations from the start."""
+    wd = tempfile.mkdtemp(prefix="athad_golden_rad_")
+    n = GRID[0] * GRID[1] * GRID[2]
+    idx, idx2 = sample_index(n), sample_index(GRID[1] * GRID[2], seed=98)
+    out = {"grid": np.array(GRID), "seed": np.array(SEED), "spin": np.array(SPIN), "index": idx,
+           "index2d": idx2}
+    st = O.OracleState(*GRID, seed=SEED)
+    init = os.path.join(wd, "init.bin")
+    st.save_bin(init)
+    o = os.path.join(wd, "out.bin")
+    subprocess.run([HARNESS, init, o, "steps_rad", "2"], check=True, cwd=wd, stdout=subprocess.DEVNULL,
+                   env=dict(os.environ, OMP_NUM_THREADS="1"))
+    pack("steps2_rad", O.read_bin(o), idx, out)
+    pack_rad("steps2_rad", O.read_rad(o + ".rad", GRID), idx, idx2, out)
+    print("steps with radiation done", flush=True)
+    st.run(SPIN, "jacobi")
+    base = os.path.join(wd, "base.bin")
+    st.save_bin(base, iter_n=SPIN)
+    subprocess.run([HARNESS, base, o, "radiation"], check=True, cwd=wd, stdout=subprocess.DEVNULL,
+                   env=dict(os.environ, OMP_NUM_THREADS="8"))
+    pack("op/radiation", O.read_bin(o), idx, out)
+    pack_rad("op/radiation", O.read_rad(o + ".rad", GRID), idx, idx2, out)
+    path = os.path.join(HERE, "ref_radiation.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference harness first: make -C oracle -f Makefile.ref")
+    if "--radiation" in sys.argv[1:]:
+        radiation_vectors()
+        return
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle -f Makefile.ref")
     wd = tempfile.mkdtemp(prefix="athad_golden_")
