@@ -41,7 +41,7 @@ EXPORTS = [
     "athad_query", "athad_device_memory", "athad_damp_tile", "athad_set_damp_tile",
     "athad_field_stats", "athad_set_member_params", "athad_restart_bytes", "athad_save_state_mem",
     "athad_load_state_mem", "athad_save_state", "athad_load_state", "athad_field_minmax", "athad_residuum",
-    "athad_meridional_streamfunction", "athad_get_slice",
+    "athad_meridional_streamfunction", "athad_get_slice", "athad_radiation", "athad_get_radiation",
 ]
 
 # restart_arrays() of the reference checkpoint (atmosphere/FileIO_Atm.cpp:297-304), in file
@@ -123,6 +123,8 @@ def lib():
         L.athad_field_minmax.argtypes = [vp, i, i, dp, ip, dp, ip]
         L.athad_meridional_streamfunction.argtypes = [vp, i, dp, dp, dp]
         L.athad_get_slice.argtypes = [vp, i, i, i, i, dp]
+        L.athad_radiation.argtypes = [vp]
+        L.athad_get_radiation.argtypes = [vp, i, dp, dp, dp, dp]
         L.athad_residuum.argtypes = [vp, i, dp, ip, dp, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64), ip]
         for name in EXPORTS:

@@ -29,6 +29,10 @@ struct Config {
     double dt_visc = 4e-05, L_atm = 6287.5, zeta = 3.0, sc_WaterVapour = 0.61, sc_CO2 = 0.96;
     double pr_turb = 0.9, abl_height = 10000.0, u_0 = 8.0, cloud_cap = 0.05, t_max_phys = 2000.0;
     double t_00 = 236.15, cp_l = 1349.0, co2_scale = 1.0;
+    // radiation: cAtmosphereDefaults.cpp.inc:31-43,72-73 and run_3D_loop's omega_rad
+    double sigma = 5.67028e-08, kappa_H2O = 0.01, kappa_CO2 = 0.001, kappa_bg = 1e-06;
+    double geothermal_flux = 150.0, rad_equator_short = 298.0, rad_pole_short = 0.0;
+    double albedo_surface = 0.08, albedo_cloud = 0.5, t_skin_relax = 0.25, omega_rad = 0.05;
 };
 
 athad_member_params default_member() {
@@ -76,6 +80,7 @@ struct MemberHost {
     double *vel_star = nullptr, *t_top = nullptr, *t_ref = nullptr, *tables = nullptr, *scratch = nullptr;
     int* counters = nullptr;
     int* passes = nullptr;
+    double *radiation = nullptr, *epsilon = nullptr, *albedo = nullptr;   // radiation outputs, on first use
     int iter_n = 0, total = 0;     // host mirror of the device counters
     // table offsets inside 'tables'
     size_t o_rad = 0, o_sint = 0, o_cost = 0, o_sinr = 0, o_hgt = 0, o_nuef = 0, o_step = 0;
@@ -173,6 +178,9 @@ void fill_consts(MemberHost& m, const Config& cfg, int order) {
     K.init_M_nw = M_nonwater_h(m.p.c_0, m.p.co2_0, C.M_bg);
     K.init_f_co2 = (C.q_CO2 + C.q_bg > 0.0) ? C.q_CO2 / (C.q_CO2 + C.q_bg) : m.p.co2_0 * cfg.co2_scale;
     K.co2_scale = cfg.co2_scale;
+    K.sigma = cfg.sigma; K.kappa_h2o = cfg.kappa_H2O; K.kappa_co2 = cfg.kappa_CO2; K.kappa_bg = cfg.kappa_bg;
+    K.geothermal = cfg.geothermal_flux; K.sw_equator = cfg.rad_equator_short; K.sw_pole = cfg.rad_pole_short;
+    K.albedo_surface = cfg.albedo_surface; K.albedo_cloud = cfg.albedo_cloud; K.omega_rad = cfg.omega_rad;
     K.order = (order == ATHAD_ORDER_SYMMETRIC) ? ATHAD_ORDER_JACOBI : order;
     K.sym = (order == ATHAD_ORDER_SYMMETRIC) ? 1 : 0;
 }
@@ -254,6 +262,7 @@ void make_desc(const athad_ctx* c, const MemberHost& m, int parity, MemberDesc& 
     d.rad = m.tables + m.o_rad; d.sint = m.tables + m.o_sint; d.cost = m.tables + m.o_cost;
     d.sinr = m.tables + m.o_sinr; d.hgt = m.tables + m.o_hgt; d.nue_f = m.tables + m.o_nuef;
     d.step_dim = m.tables + m.o_step; d.passes = m.passes; d.counters = m.counters; d.scratch = m.scratch;
+    d.radiation = m.radiation; d.epsilon = m.epsilon; d.albedo = m.albedo;
     (void)c;
 }
 
@@ -276,7 +285,7 @@ void free_ctx(athad_ctx* c) {
         for (auto p : m.alt) if (p) hipFree(p);
         for (auto p : m.tmp) if (p) hipFree(p);
         for (void* p : {(void*)m.vel_star, (void*)m.t_top, (void*)m.t_ref, (void*)m.tables, (void*)m.scratch,
-                        (void*)m.counters, (void*)m.passes})
+                        (void*)m.counters, (void*)m.passes, (void*)m.radiation, (void*)m.epsilon, (void*)m.albedo})
             if (p) hipFree(p);
     }
     for (int p = 0; p < 2; p++) if (c->d_md[p]) hipFree(c->d_md[p]);
@@ -809,6 +818,81 @@ athad_status athad_get_slice(athad_ctx* c, int member, int field, int axis, int 
     return ATHAD_OK;
 }
 
+// Radiation outputs are allocated at the first radiation call, so contexts that never call it
+// hold no extra memory.
+athad_status ensure_radiation_buffers(athad_ctx* c) {
+    bool fresh = false;
+    for (auto& m : c->mem) {
+        if (m.radiation && m.epsilon && m.albedo) continue;
+        const size_t bytes[3] = {m.n * 8, m.n * 8, (size_t)m.g.jm * m.g.km * 8};
+        double** ptr[3] = {&m.radiation, &m.epsilon, &m.albedo};
+        for (int a = 0; a < 3; a++)
+            if (!*ptr[a] && hipMalloc((void**)ptr[a], bytes[a]) != hipSuccess) {
+                *ptr[a] = nullptr;
+                c->err = "out of device memory for the radiation outputs";
+                return ATHAD_ERR_ALLOC;
+            }
+        fresh = true;
+    }
+    return fresh ? upload_descs(c) : ATHAD_OK;
+}
+
+athad_status athad_radiation(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    for (auto& m : c->mem)
+        if (m.g.im > radiation_max_levels()) {
+            c->err = "radiation supports up to " + std::to_string(radiation_max_levels()) + " levels";
+            return ATHAD_ERR_SHAPE;
+        }
+    athad_status s = ensure_radiation_buffers(c);
+    if (s) return s;
+    join_s2(c);
+    launch_radiation(cur(c));
+    // updateSkinTemperature (cAtmosphereModel.cpp:276-311) with planetaryShortWave (:241-270)
+    // on the zonal albedo sums; host arithmetic on jm numbers, as the reference's.
+    const Config cfg;
+    bool changed = false;
+    for (auto& m : c->mem) {
+        const int jm = m.g.jm, km = m.g.km, jmax = jm - 1, jhalf = jmax / 2;
+        std::vector<double> rows(jm);
+        HIPCHK(c, hipMemcpyAsync(rows.data(), m.scratch + sf_offset(m.g.im), jm * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (cfg.t_skin_relax <= 0.0) continue;
+        double wsum = 0.0, absorbed = 0.0, amax = 0.0;
+        for (int j = 0; j < jm; j++) {
+            const double x = (double)((j <= jhalf) ? j : jmax - j) / (double)jhalf;
+            const double sw = (cfg.rad_pole_short - cfg.rad_equator_short) * (x * x - 2.0 * x) + cfg.rad_pole_short;
+            const double w = std::cos((j / (double)(jm - 1) - 0.5) * M_PI);
+            const double a = rows[j] / (double)km;
+            amax = std::max(amax, a);
+            wsum += w;
+            absorbed += w * (1.0 - a) * sw;
+        }
+        if (!(wsum > 0.0) || !(amax > 0.0)) continue;
+        const double in = absorbed / wsum + cfg.geothermal_flux;
+        if (!(in > 0.0)) continue;
+        const double target = std::pow(in / cfg.sigma, 0.25);
+        m.K.t_skin += std::min(1.0, cfg.t_skin_relax) * (target - m.K.t_skin);
+        changed = true;
+    }
+    return changed ? upload_descs(c) : ATHAD_OK;
+}
+
+athad_status athad_get_radiation(athad_ctx* c, int member, double* radiation, double* epsilon, double* albedo,
+                                 double* t_skin) {
+    athad_status s = check_member(c, member, 0, 1);
+    if (s) return s;
+    const MemberHost& m = c->mem[member];
+    if (t_skin) *t_skin = m.K.t_skin;
+    if (!m.radiation && (radiation || epsilon || albedo)) { c->err = "radiation has not run on this context"; return ATHAD_ERR_STATE; }
+    join_s2(c);
+    if (radiation) HIPCHK(c, hipMemcpyAsync(radiation, m.radiation, m.n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (epsilon) HIPCHK(c, hipMemcpyAsync(epsilon, m.epsilon, m.n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (albedo) HIPCHK(c, hipMemcpyAsync(albedo, m.albedo, (size_t)m.g.jm * m.g.km * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ATHAD_OK;
+}
+
 athad_status athad_set_member_params(athad_ctx* c, int member, const athad_member_params* p) {
     if (!c || !p || member < 0 || member >= c->nmem) return ATHAD_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1046,8 +1130,11 @@ athad_status athad_load_state(athad_ctx* ctx, int member, const char* path) {
 
 athad_status athad_init_state(athad_ctx* c, uint64_t seed) {
     if (!c) return ATHAD_ERR_ARG;
-    // member seeds
-    for (int q = 0; q < c->nmem; q++) c->mem[q].K.seed = seed + (uint64_t)q;
+    // member seeds; a new integration starts from the configured t_skin
+    for (int q = 0; q < c->nmem; q++) {
+        c->mem[q].K.seed = seed + (uint64_t)q;
+        c->mem[q].K.t_skin = Config().t_skin;
+    }
     c->parity = 0;
     athad_status s = upload_descs(c);
     if (s) return s;

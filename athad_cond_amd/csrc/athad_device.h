@@ -38,6 +38,9 @@ struct Consts {
     double nue_max, nue_air_nd, tke_max_nd, vel_max_nd, om_wall, sqrt_cnue;
     double hs_k_a, hs_k_s, drag_coeff;
     double evap_r, evap_w_norm;
+    // radiation (MultiLayerRadiation, mode 2)
+    double sigma, kappa_h2o, kappa_co2, kappa_bg, geothermal, sw_equator, sw_pole;
+    double albedo_surface, albedo_cloud, omega_rad;
     // initial-state recipe
     double t_surf_eq, t_surf_pole, init_M_nw, init_f_co2, co2_scale;
     unsigned long long seed;
@@ -64,6 +67,9 @@ struct MemberDesc {
     const int* passes;       // jm       polar filter pass count
     int* counters;           // [0] iter_n, [1] total_iter_count (device resident)
     double* scratch;         // reductions
+    double* radiation;       // n        upward long-wave flux (allocated at the first radiation call)
+    double* epsilon;         // n        layer emissivity
+    double* albedo;          // jm*km    surface short-wave albedo
 };
 
 #define ATHAD_DEV __device__ __forceinline__

@@ -38,6 +38,11 @@ int diag_offset(int im);
 // psi, each im x jm doubles [i][j], at scratch[sf_offset(im)]
 void launch_streamfunction(const Batch& b, int member, int im, int jm, double two_pi_a);
 int sf_offset(int im);
+// run_3D_loop's mode-2 radiation on every member (MultiLayerRadiation and the omega_rad blend of
+// t; outputs in the members' radiation, epsilon, albedo), then each member's zonal albedo sums
+// per latitude at scratch[sf_offset(im)] (jm doubles). Grids up to radiation_max_levels() levels.
+void launch_radiation(const Batch& b);
+int radiation_max_levels();
 // one plane (n values) of one field, widened to double, at scratch[sf_offset(im)]: axis 0 fixes
 // i ([j][k]), 1 fixes j ([i][k]), 2 fixes k ([i][j])
 void launch_slice(const Batch& b, int member, int field, int axis, int index, long long n);

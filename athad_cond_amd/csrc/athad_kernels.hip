@@ -388,6 +388,234 @@ __global__ __launch_bounds__(256) void k_slice(const MemberDesc* __restrict__ md
     M.scratch[sf_base(K) + id] = dbl ? ((const double*)M.f[field])[q] : (double)((const float*)M.f[field])[q];
 }
 
+// ------------------------------------------------------------------------------------
+// Radiation in run_3D_loop's mode 2 (cAtmosphereModel.cpp:799-801, 863-879):
+// MultiLayerRadiation::run (MultiLayerRadiation.h:59-384) on the current t, and t blended by
+// omega_rad toward its result. A block of 256 threads takes kRadCols consecutive columns of
+// one latitude row: the six input fields of those columns are staged through LDS with
+// coalesced loads ([field][i][col]), each wave then takes one column at a time with E levels
+// per lane (lane l holds levels lE .. lE+E-1), and the outputs go back through the same LDS
+// slots with coalesced stores. The two grey flux sweeps
+//   up[i] = up[i-1] (1 - eps_i) + eps_i sigma T_i^4,   dn[i] = dn[i+1] (1 - eps_i) + eps_i sigma T_i^4
+// are affine recurrences: each level is the map x -> a x + b, and the sweep is the running
+// composition of those maps, formed by a Hillis-Steele scan across the 64 lanes (6 steps)
+// instead of 41 dependent steps. Every term is non-negative, so the scan agrees with the
+// sequential loop to a few ulps. T^4 is (T^2)^2 and x^(1/4) is sqrt(sqrt(x)) (correctly
+// rounded square roots) in place of pow. The condensate paths that set the albedo are summed
+// in level order, as the reference, so the albedo is its own to the last bit for equal inputs.
+// Contraction is off in this section: the cell arithmetic keeps the reference's roundings.
+// ------------------------------------------------------------------------------------
+constexpr int kRadCols = 16;
+
+// Inclusive composition scan across the 64 lanes: after it, lane l holds the composition of
+// the maps of lanes 0..l (upward) or l..63 (downward), the nearer lane's map applied last.
+ATHAD_DEV void affine_scan_up(double& A, double& B) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double Ap = __shfl_up(A, d), Bp = __shfl_up(B, d);
+        if (lane >= d) { B = A * Bp + B; A = A * Ap; }
+    }
+}
+ATHAD_DEV void affine_scan_down(double& A, double& B) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double An = __shfl_down(A, d), Bn = __shfl_down(B, d);
+        if (lane + d < 64) { B = A * Bn + B; A = A * An; }
+    }
+}
+
+// Both sweeps of one column (MultiLayerRadiation.h:291-303): up from the black surface (level 0
+// is the constant map sigma T_0^4), dn from a dark sky (level 0 is the identity, so dn at
+// level 0 equals dn[1], the downwelling flux at the ground). Levels at or above im are identities.
+template <int E>
+ATHAD_DEV void rad_sweeps(int lane, int im, const double (&eps)[E], const double (&P4)[E], double sigma,
+                          double (&up)[E], double (&dn)[E]) {
+#pragma clang fp contract(off)
+    double a[E], b[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int i = lane * E + e;
+        if (i >= im) { a[e] = 1.0; b[e] = 0.0; }
+        else if (i == 0) { a[e] = 0.0; b[e] = sigma * P4[e]; }
+        else { a[e] = 1.0 - eps[e]; b[e] = eps[e] * sigma * P4[e]; }
+    }
+    double A = 1.0, B = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; e++) { B = a[e] * B + b[e]; A = a[e] * A; }
+    affine_scan_up(A, B);
+    double x = __shfl_up(B, 1);
+    if (lane == 0) x = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; e++) { x = a[e] * x + b[e]; up[e] = x; }
+    if (lane == 0) { a[0] = 1.0; b[0] = 0.0; }
+    A = 1.0; B = 0.0;
+#pragma unroll
+    for (int e = E - 1; e >= 0; e--) { B = a[e] * B + b[e]; A = a[e] * A; }
+    affine_scan_down(A, B);
+    x = __shfl_down(B, 1);
+    if (lane == 63) x = 0.0;
+#pragma unroll
+    for (int e = E - 1; e >= 0; e--) { x = a[e] * x + b[e]; dn[e] = x; }
+}
+
+template <int E> ATHAD_DEV double pick(const double (&v)[E], int e) {
+    double r = v[0];
+#pragma unroll
+    for (int q = 1; q < E; q++) r = (e == q) ? v[q] : r;
+    return r;
+}
+
+template <typename S, int E>
+__global__ __launch_bounds__(256) void k_radiation(const MemberDesc* __restrict__ md) {
+#pragma clang fp contract(off)
+    extern __shared__ double s_col[];
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const int im = K.im, jm = K.jm, km = K.km, top = im - 1;
+    const int j = blockIdx.y, k0 = blockIdx.x * kRadCols;
+    if (j >= jm || k0 >= km) return;
+    const S *ft = fp<S>(M, F_T), *fc = fp<S>(M, F_C), *fco2 = fp<S>(M, F_CO2);
+    const S *fcl = fp<S>(M, F_CLOUD), *fice = fp<S>(M, F_ICE);
+    const double* fpst = (const double*)M.f[F_P_STAT];
+    const long long si = (long long)jm * km, row = (long long)j * km;
+    const int per = im * kRadCols;
+    auto slot = [&](int f, int i, int kk) -> double& { return s_col[(f * im + i) * kRadCols + kk]; };
+    // stage t, c, co2, p_stat, cloud, ice of the tile's columns
+    for (int q = threadIdx.x; q < 6 * per; q += blockDim.x) {
+        const int f = q / per, r = q - f * per, i = r / kRadCols, kk = r - i * kRadCols;
+        if (k0 + kk >= km) continue;
+        const long long g = i * si + row + k0 + kk;
+        double v;
+        switch (f) {
+            case 0: v = (double)ft[g]; break;
+            case 1: v = (double)fc[g]; break;
+            case 2: v = (double)fco2[g]; break;
+            case 3: v = fpst[g]; break;
+            case 4: v = (double)fcl[g]; break;
+            default: v = (double)fice[g]; break;
+        }
+        slot(f, i, kk) = v;
+    }
+    __syncthreads();
+    // incoming short wave of this latitude (MultiLayerRadiation.h:98-104)
+    const int jmax = jm - 1, jhalf = jmax / 2;
+    const double xs = (double)((j <= jhalf) ? j : jmax - j) / (double)jhalf;
+    const double sw = (K.sw_pole - K.sw_equator) * (xs * xs - 2.0 * xs) + K.sw_pole;
+    const double sigma = K.sigma, inv_g = 1.0 / K.g, t_0 = K.t_0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int kk = wave; kk < kRadCols; kk += 4) {
+        if (k0 + kk >= km) break;
+        double T[E], eps[E], P4[E], up[E], dn[E], t_in[E], lw[E], iw[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int i = lane * E + e;
+            eps[e] = 0.0; lw[e] = 0.0; iw[e] = 0.0; T[e] = 0.0; t_in[e] = 0.0;
+            if (i < im) {
+                // optical depth: gas mass with pressure broadening, cloud liquid and ice paths
+                // (MultiLayerRadiation.h:156-207)
+                t_in[e] = slot(0, i, kk);
+                T[e] = t_in[e] * t_0;
+                const double p = slot(3, i, kk);
+                double dp = (i < top) ? (p - slot(3, i + 1, kk)) * 100.0 : p * 100.0;
+                if (dp < 0.0) dp = 0.0;
+                const double qv = tmax(0.0, slot(1, i, kk)), qc = tmax(0.0, slot(2, i, kk));
+                const double qb = tmax(0.0, 1.0 - qv - qc);
+                const double mass = dp * inv_g;
+                const double broad = p * 100.0 / 1.0e5;
+                const double tau_gas = (K.kappa_h2o * qv + K.kappa_co2 * qc + K.kappa_bg * qb) * mass * broad;
+                const int ia = (i < top) ? i : i - 1;
+                const double dz = (double)((float)M.hgt[ia + 1] - (float)M.hgt[ia]);
+                const double rho = (T[e] > 0.0) ? (p * 100.0) / (th_R_of(qv, qc, K.R_bg) * T[e]) : 0.0;
+                lw[e] = tmax(0.0, slot(4, i, kk)) * rho * dz * 1000.0;
+                iw[e] = tmax(0.0, slot(5, i, kk)) * rho * dz * 1000.0;
+                const double tau = tau_gas + 0.12 * lw[e] + 0.055 * iw[e];
+                eps[e] = (tau > 700.0) ? 1.0 : (1.0 - exp(-tau));
+            }
+            P4[e] = (T[e] * T[e]) * (T[e] * T[e]);
+        }
+        // condensate paths in level order, then the cloud-raised albedo (MultiLayerRadiation.h:231-238)
+        double lwp = 0.0, iwp = 0.0;
+        for (int i = 0; i < im; i++) {
+            const double l = __shfl(pick(lw, i % E), i / E), w = __shfl(pick(iw, i % E), i / E);
+            lwp += l;
+            iwp += w;
+        }
+        double alb = K.albedo_surface;
+        if (K.albedo_cloud > alb) {
+            const double tau = (lwp + 0.5 * iwp) / 100.0;
+            alb = alb + (K.albedo_cloud - alb) * (tau / (tau + 2.0));
+        }
+        // four Lambda iterations of sigma T_i^4 = (up[i-1] + dn[i+1]) / 2, then final sweeps
+        // (MultiLayerRadiation.h:287-333)
+        for (int it = 0; it < 4; it++) {
+            rad_sweeps<E>(lane, im, eps, P4, sigma, up, dn);
+            const double up_prev_lane = __shfl_up(up[E - 1], 1);
+            double dn_next_lane = __shfl_down(dn[0], 1);
+            if (lane == 63) dn_next_lane = 0.0;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int i = lane * E + e;
+                if (i < 1 || i > top) continue;
+                const double below = (e > 0) ? up[e > 0 ? e - 1 : 0] : up_prev_lane;
+                const double above = (e < E - 1) ? dn[e < E - 1 ? e + 1 : 0] : dn_next_lane;
+                const double emit = 0.5 * (below + above);
+                if (emit > 0.0 && finite_bits(emit)) {
+                    T[e] = sqrt(sqrt(emit / sigma));
+                    P4[e] = (T[e] * T[e]) * (T[e] * T[e]);
+                }
+            }
+        }
+        rad_sweeps<E>(lane, im, eps, P4, sigma, up, dn);
+        // surface energy balance with sigma T^4 linearised at the current surface temperature
+        // (MultiLayerRadiation.h:338-371); dn at level 0 is dn[1]
+        const double T1 = (E > 1) ? T[E > 1 ? 1 : 0] : __shfl_down(T[0], 1);
+        double Ts = 0.0;
+        if (lane == 0) {
+            const double Ts0 = T[0], c_H = 15.0;
+            const double d4 = 4.0 * sigma * Ts0 * Ts0 * Ts0;
+            Ts = ((1.0 - alb) * sw + K.geothermal + dn[0] - sigma * P4[0] + d4 * Ts0 + c_H * T1) / (d4 + c_H);
+            M.albedo[row + k0 + kk] = alb;
+        }
+        // outputs into the column's LDS slots: blended t, epsilon, upward flux
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int i = lane * E + e;
+            if (i >= im) continue;
+            const double Tf = (i == 0) ? Ts : T[e];
+            slot(0, i, kk) = t_in[e] * (1.0 - K.omega_rad) + (Tf / t_0) * K.omega_rad;
+            slot(1, i, kk) = eps[e];
+            slot(2, i, kk) = (i == 0) ? sigma * ((Ts * Ts) * (Ts * Ts)) : up[e];
+        }
+    }
+    __syncthreads();
+    S* tw = fp<S>(M, F_T);
+    for (int q = threadIdx.x; q < 3 * per; q += blockDim.x) {
+        const int f = q / per, r = q - f * per, i = r / kRadCols, kk = r - i * kRadCols;
+        if (k0 + kk >= km) continue;
+        const long long g = i * si + row + k0 + kk;
+        const double v = slot(f, i, kk);
+        if (f == 0) tw[g] = (S)v;
+        else if (f == 1) M.epsilon[g] = v;
+        else M.radiation[g] = v;
+    }
+}
+
+// Per latitude, the zonal sum of the albedo in k order (planetaryShortWave's inner loop,
+// cAtmosphereModel.cpp:248-252), at sf_base.
+__global__ void k_albedo_rows(const MemberDesc* __restrict__ md) {
+#pragma clang fp contract(off)
+    const MemberDesc& M = md[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M.K.jm) return;
+    double a = 0.0;
+    for (int k = 0; k < M.K.km; k++) a += M.albedo[(long long)j * M.K.km + k];
+    M.scratch[sf_base(M.K) + j] = a;
+}
+
 enum { FT = 0, FU, FV, FW, FC, FCL, FI, FG, FCO, FK, FD };
 
 // Solid-ground clamps of bcSolidGround passes 4-5 (BC_Atm.h:475-548) as a value transform,
@@ -2377,6 +2605,20 @@ int sf_offset(int im) { return diag_offset(im) + 6 * kStatChunks + 8; }
 void launch_slice(const Batch& b, int member, int field, int axis, int index, long long n) {
     ATHAD_DISPATCH(k_slice, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b.d_md, member, field, axis, index);
 }
+void launch_radiation(const Batch& b) {
+    const dim3 g((b.max_km + kRadCols - 1) / kRadCols, b.max_jm, b.nmem);
+    const size_t lds = (size_t)6 * b.max_im * kRadCols * sizeof(double);
+    if (b.max_im <= 64) {
+        if (b.fp32) hipLaunchKernelGGL((k_radiation<float, 1>), g, dim3(256), lds, b.stream, b.d_md);
+        else hipLaunchKernelGGL((k_radiation<double, 1>), g, dim3(256), lds, b.stream, b.d_md);
+    } else {
+        if (b.fp32) hipLaunchKernelGGL((k_radiation<float, 2>), g, dim3(256), lds, b.stream, b.d_md);
+        else hipLaunchKernelGGL((k_radiation<double, 2>), g, dim3(256), lds, b.stream, b.d_md);
+    }
+    hipLaunchKernelGGL(k_albedo_rows, dim3((b.max_jm + 63) / 64, 1, b.nmem), dim3(64), 0, b.stream, b.d_md);
+}
+// 80 levels keep the tile's LDS (6 fields x levels x 16 columns) within 64 KB.
+int radiation_max_levels() { return 80; }
 void launch_streamfunction(const Batch& b, int member, int im, int jm, double two_pi_a) {
     ATHAD_DISPATCH(k_zonal_mean_v, dim3((im * jm + 63) / 64), dim3(64), 0, b.d_md, member);
     hipLaunchKernelGGL(k_streamfunction, dim3((jm + 63) / 64), dim3(64), 0, b.stream, b.d_md, member, two_pi_a);

@@ -180,6 +180,20 @@ class Integration:
         else:
             self._chk(L.lib().athad_load_state(self.h, member, os.fsencode(src)))
 
+    # ---- radiation (run_3D_loop's radiation_mode 2) ------------------------------------
+    def radiation(self):
+        """MultiLayerRadiation + the 5 % blend of t + one t_skin relaxation, every member."""
+        self._chk(L.lib().athad_radiation(self.h))
+
+    def radiation_outputs(self, member: int = 0) -> dict:
+        im, jm, km = self.grids[member]
+        out = {"radiation": np.empty((im, jm, km)), "epsilon": np.empty((im, jm, km)), "albedo": np.empty((jm, km))}
+        ts = ctypes.c_double()
+        ptr = [a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in out.values()]
+        self._chk(L.lib().athad_get_radiation(self.h, member, *ptr, ctypes.byref(ts)))
+        out["t_skin"] = ts.value
+        return out
+
     # ---- checkpoint diagnostics on the device (MinMax_Atm.cpp, UtilsAtm.h) --------------
     def field_minmax(self, name: str, member: int = 0):
         """searchMinMax_3D: (min, (i, j, k), max, (i, j, k)) of the finite values."""

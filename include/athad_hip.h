@@ -236,6 +236,23 @@ athad_status athad_meridional_streamfunction(athad_ctx* ctx, int member, double*
  * the caller. ATHAD_ERR_ARG for an axis or index outside the grid. */
 athad_status athad_get_slice(athad_ctx* ctx, int member, int field, int axis, int index, double* dst);
 
+/* --- radiation (run_3D_loop's radiation_mode 2) --------------------------------------- */
+
+/* The radiation branch of run_3D_loop (atmosphere/cAtmosphereModel.cpp:1655-1661, called after
+ * the store): apply_radiative_heating (:863-879) runs MultiLayerRadiation
+ * (atmosphere/MultiLayerRadiation.h:59-384) on the current t of every member and blends t by
+ * omega_rad = 0.05 toward its result; then updateSkinTemperature (:276-311) moves each
+ * member's t_skin one relaxation step toward the emission temperature of its absorbed short
+ * wave plus the geothermal flux, which the next densities() uses. The outputs (upward long-wave
+ * flux, emissivity, albedo) are kept on the device; the first call allocates them. Grids up to
+ * 80 levels (ATHAD_ERR_SHAPE above). athad_init_state resets t_skin to its configured value. */
+athad_status athad_radiation(athad_ctx* ctx);
+/* The radiation outputs of one member: radiation and epsilon (im*jm*km doubles, [i][j][k]),
+ * albedo (jm*km) and the current t_skin in K; any may be NULL. ATHAD_ERR_STATE for the arrays
+ * before the first athad_radiation. */
+athad_status athad_get_radiation(athad_ctx* ctx, int member, double* radiation, double* epsilon, double* albedo,
+                                 double* t_skin);
+
 /* --- measurement --------------------------------------------------------------------- */
 
 /* Per-kernel timing with HIP events on the context's stream: kernel ids as in
