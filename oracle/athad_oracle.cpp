@@ -167,7 +167,7 @@ struct Params {
     double dt_visc = 4e-05, L_atm = 6287.5, zeta = 3.0;
     double sc_WaterVapour = 0.61, sc_CO2 = 0.96, pr_turb = 0.9, abl_height = 10000.0;
     double x_H2O = 0.5578, x_CO2 = 0.4109, x_N2 = 0.0313;
-    double u_0 = 8.0, cloud_cap = 0.05, t_max_phys = 2000.0, t_00 = 236.15;
+    double u_0 = 8.0, cloud_cap = 0.05, t_max_phys = 2000.0, t_00 = 236.15, t_000 = 253.15;
     double c_0 = 0.3464, c_h2o_dry_top = 0.00423, co2_0 = 0.6233, co2_scale = 1.0;
     // radiation (cAtmosphereDefaults.cpp.inc:31-43,72-73; omega_rad: cAtmosphereModel.cpp:801)
     double sigma = 5.67028e-08, kappa_H2O = 0.01, kappa_CO2 = 0.001, kappa_bg = 1e-06;
@@ -200,6 +200,9 @@ struct orc_state {
     // radiation outputs: upward long-wave flux and emissivity per cell, albedo per column,
     // incoming short wave per latitude
     std::vector<double> radiation, epsilon, albedo, sw;
+    // precipitation fluxes, their stored copies and the microphysics sources of the two-category
+    // ice scheme (ORC_MP_* order); allocated, zero, at first use
+    std::vector<std::vector<double>> mp;
     int iter_n = 0, total_iter = 0;
 
     inline size_t ix(int i, int j, int k) const { return ((size_t)i * jm + j) * km + k; }
@@ -1489,6 +1492,238 @@ void radiation(S& s) {
     s.P.t_skin += dmin(1.0, P.t_skin_relax) * (target - P.t_skin);
 }
 
+// ---------------------------------------------------------------------------------------
+// Two-category ice scheme: TwoCatIceScheme::run, TwoCatIceScheme.h:89-111 (initArrays
+// :135-155, computeColumns :160-517, applyBoundaryConditions :522-525 with
+// IceSchemeCommon::extrapolateBC IceSchemeCommon.h:28-55, applyTopography a no-op without
+// land), then the two damp_wiggles of run_3D_loop (cAtmosphereModel.cpp:1243-1244).
+// ---------------------------------------------------------------------------------------
+void ensure_mp(S& s) {
+    if ((int)s.mp.size() == ORC_NMP) return;
+    s.mp.assign(ORC_NMP, std::vector<double>(s.n, 0.0));
+}
+
+// IceSchemeCommon::evaporateWhereImpossible (IceSchemeCommon.h:121-190) with the scheme's
+// precipitation fluxes; graupel and its flux are zero in the two-category scheme.
+void ice_evaporate(S& s, size_t id, double t_u) {
+    const Params& P = s.P;
+    double* t = s.F(ORC_T); double* c = s.F(ORC_C); double* cl = s.F(ORC_CLOUD);
+    double* ice = s.F(ORC_ICE); double* gr = s.F(ORC_GR); const double* co2 = s.F(ORC_CO2);
+    const double* rh = s.F(ORC_R_HUMID);
+    const double q_cond = dmax(0.0, cl[id]) + dmax(0.0, ice[id]) + dmax(0.0, gr[id]);
+    if (q_cond > 0.0) {
+        const double c_max = dmax(0.0, 1.0 - co2[id]);
+        const double c_new = dmin(c_max, c[id] + q_cond);
+        const double q_used = dmax(0.0, c_new - c[id]);
+        c[id] = c_new; cl[id] = 0.0; ice[id] = 0.0; gr[id] = 0.0;
+        if (t_u < kTcrit) {
+            const double cp_loc = cp_of(c[id], co2[id], t_u, P.comp.M_bg);
+            if (cp_loc > 0.0) {
+                double T_new = t_u - L_vap(t_u) * q_used / cp_loc;
+                if (T_new < 1.0) T_new = 1.0;
+                t[id] = T_new / P.t_0;
+            }
+        }
+    }
+    double* Pr = s.mp[ORC_MP_P_RAIN].data(); double* Ps = s.mp[ORC_MP_P_SNOW].data();
+    {
+        const double rho = dmax(1.0e-6, rh[id]);
+        const double q_p = dmax(0.0, Pr[id]) / (1.6 * rho) + dmax(0.0, Ps[id]) / (0.96 * rho) + 0.0 / (2.0 * rho);
+        if (q_p > 0.0) {
+            const double c_max = dmax(0.0, 1.0 - co2[id]);
+            c[id] = dmin(c_max, c[id] + q_p);
+        }
+    }
+    Pr[id] = 0.0; Ps[id] = 0.0; s.mp[ORC_MP_PRECIP][id] = 0.0;
+    for (int q : {ORC_MP_S_V, ORC_MP_S_C, ORC_MP_S_I, ORC_MP_S_R, ORC_MP_S_S}) s.mp[q][id] = 0.0;
+    s.F(ORC_S_C_C)[id] = 0.0;
+}
+
+// One (j,k) column of computeColumns (TwoCatIceScheme.h:164-516). The rate variables are
+// column-private and carried from level to level and from pass to pass exactly where the
+// reference leaves them unassigned: N_i and m_i outside the mixed-phase band, S_nuc where
+// ice is zero and no nucleation condition holds, S_i_dep where c equals q_Ice.
+void ice_column(S& s, int j, int k) {
+    const Params& P = s.P;
+    constexpr double N_i_0 = 1.0e2, m_i_0 = 1.0e-12, m_i_max = 1.0e-9, m_s_0 = 3.0e-9;
+    constexpr double c_i_dep = 1.3e-5, c_c_au = 1.0e-3, q_c_crit = 5.0e-4, c_i_au = 1.0e-3;
+    constexpr double c_ac = 0.24, c_rim = 18.6, c_rim_snow = 18.6 / 5.0, c_agg = 10.3;
+    constexpr double c_i_cri = 0.24, c_r_cri = 3.2e-5, a_ev = 1.0e-3, b_ev = 5.9;
+    constexpr double c_s_dep = 1.8e-2, b_s_dep = 12.3, c_s_melt = 8.43e-5, b_s_melt = 12.05;
+    constexpr double c_r_frz = 3.75e-2;
+    constexpr double t_nuc = 267.15, t_d = 248.15, t_hn = 236.15, t_r_frz = 271.15;
+    constexpr int iter_prec_end = 3, i_check = 1;
+    constexpr double P_max = 50.0 / 86400.0;
+    const double t_0 = P.t_0;
+    const double* t = s.F(ORC_T); const double* c = s.F(ORC_C); const double* cl = s.F(ORC_CLOUD);
+    const double* ice = s.F(ORC_ICE); const double* co2 = s.F(ORC_CO2);
+    const double* ps = s.F(ORC_P_STAT); const double* rh = s.F(ORC_R_HUMID);
+    const double* scc = s.F(ORC_S_C_C);
+    double* Pr = s.mp[ORC_MP_P_RAIN].data(); double* Ps = s.mp[ORC_MP_P_SNOW].data();
+    double* Pt = s.mp[ORC_MP_PRECIP].data();
+    double* Sv = s.mp[ORC_MP_S_V].data(); double* Sc = s.mp[ORC_MP_S_C].data();
+    double* Si = s.mp[ORC_MP_S_I].data(); double* Sr = s.mp[ORC_MP_S_R].data();
+    double* Ss = s.mp[ORC_MP_S_S].data();
+    const int im = s.im;
+
+    const double Rain_check = Pr[s.ix(i_check, j, k)];
+    double m_i = m_i_max, N_i = 0.0, S_nuc = 0.0, S_i_dep = 0.0;
+    for (int iter_prec = 1; iter_prec <= iter_prec_end; iter_prec++) {
+        Pr[s.ix(im - 1, j, k)] = 0.0;
+        Ps[s.ix(im - 1, j, k)] = 0.0;
+        for (int i = im - 2; i >= 0; i--) {
+            const size_t id = s.ix(i, j, k);
+            {   // IceSchemeCommon::canCondense (IceSchemeCommon.h:104-107)
+                const double t_guard = t[id] * t_0;
+                bool can = false;
+                if (!(t_guard >= kTcrit))
+                    can = q_sat(p_sat(t_guard), ps[id], M_nonwater(c[id], co2[id], P.comp.M_bg)) < 1.0;
+                if (!can) { ice_evaporate(s, id, t_guard); continue; }
+            }
+            const double Rain = Pr[id], Snow = Ps[id];
+            double R49 = 0.0, R79 = 0.0, R139 = 0.0;
+            if (Rain > 1e-12) {
+                const double rb = std::pow(Rain, 1.0 / 9.0);
+                const double r2 = rb * rb;
+                const double r4 = r2 * r2;
+                R49 = r4; R79 = Rain / r2; R139 = Rain * r4;
+            }
+            const double t_u = t[id] * t_0;
+            const double Mo = M_nonwater(c[id], co2[id], P.comp.M_bg);
+            const double q_sw = q_sat(p_sat(t_u), ps[id], Mo);
+            const double q_Ice = q_sat(p_subl(t_u), ps[id], Mo);
+            const double step = (double)(float)(s.hgt[i + 1] - s.hgt[i]);
+            const double mass_layer = rh[id] * step;
+            const double dt_snow_dim = step / 0.96;
+            const double cv = c[id], cld = cl[id], ic = ice[id];
+
+            if (!(t_u > t_0) && !(t_u <= t_hn)) {
+                N_i = N_i_0 * std::exp(0.2 * (t_0 - t_u));
+                m_i = dmin((rh[id] * ic) / N_i, m_i_max);
+                if (m_i > m_i_max) m_i = m_i_max;
+                if (m_i < m_i_0) m_i = m_i_0;
+            }
+            if (ic == 0.0) {
+                if (((t_u < t_d) && (cv >= q_Ice)) || (((t_d <= t_u) && (t_u <= t_nuc)) && (cv >= q_sw)))
+                    S_nuc = m_i_0 / (rh[id] * dt_snow_dim) * N_i;
+            } else S_nuc = 0.0;
+            double S_c_frz = (t_u < t_hn) ? cld / dt_snow_dim : 0.0;
+            if (cv > q_Ice) S_i_dep = c_i_dep * N_i * std::pow(m_i, 1.0 / 3.0) * (cv - q_Ice);
+            if (cv < q_Ice) S_i_dep = dmax((-ic / dt_snow_dim), ((cv - q_Ice) / dt_snow_dim));
+            double S_c_au = (cld > q_c_crit) ? c_c_au * (cld - q_c_crit) : 0.0;
+            double S_i_au = (ic > 0.0) ? dmax(c_i_au * ic, 0.0) : 0.0;
+            double S_d_au = (S_i_dep > 0.0) ? S_i_dep / (1.5 * (std::pow((m_s_0 / m_i), 2.0 / 3.0) - 1.0)) : 0.0;
+            double S_ac = ((t_u >= t_0) && (cld > q_c_crit)) ? c_ac * (cld - q_c_crit) * R79 : 0.0;
+            double S_rim = (t_u < t_0) ? c_rim_snow * cld * Snow : 0.0;
+            double S_shed = (t_u >= t_0) ? c_rim * cld * Snow : 0.0;
+            double S_s_melt = 0.0, S_i_melt = 0.0;
+            if (t_u > t_0) {
+                if (Snow > 1e-12) {
+                    S_s_melt = c_s_melt * (1.0 + b_s_melt * std::pow(Snow, 5.0 / 26.0)) * (t_u - t_0)
+                             * std::pow(Snow, 1.0 / 3.0);
+                    S_s_melt = dmin(S_s_melt, Snow / mass_layer);
+                }
+                S_i_melt = ic / dt_snow_dim;
+            }
+            double S_agg = 0.0, S_i_cri = 0.0, S_r_cri = 0.0;
+            if (t_u <= t_0) {
+                S_agg = c_agg * ic * Snow;
+                S_i_cri = c_i_cri * ic * R79;
+                S_r_cri = c_r_cri * ic / m_i * R139;
+            }
+            double S_ev = 0.0;
+            if (t_u > t_0 && Rain > 1e-12 && cv < q_sw) {
+                S_ev = a_ev * (1.0 + b_ev * std::pow(Rain, 1.0 / 6.0)) * (q_sw - cv) * R49;
+                S_ev = dmin(S_ev, Rain / mass_layer);
+            }
+            double S_s_dep = 0.0;
+            if (t_u < t_0 && Snow > 1e-12) {
+                S_s_dep = c_s_dep * (1.0 + b_s_dep * std::pow(Snow, 5.0 / 26.0)) * (cv - q_Ice)
+                        * std::pow(Snow, 8.0 / 13.0);
+                if (S_s_dep < 0.0) S_s_dep = dmax(S_s_dep, -Snow / mass_layer);
+            }
+            double S_r_frz = 0.0;
+            if (t_u <= t_0) {
+                const double t_frz = dmax((t_r_frz - t_u), 0.0);
+                S_r_frz = c_r_frz * std::pow(t_frz, 3.0 / 2.0) * std::pow(Rain, 3.0 / 2.0);
+            }
+            const double S_cloud_total = S_c_au + S_ac + S_rim + S_c_frz + S_shed;
+            const double max_cloud_loss = cld / dt_snow_dim;
+            if (S_cloud_total > max_cloud_loss && S_cloud_total > 0) {
+                const double f = max_cloud_loss / S_cloud_total;
+                S_c_au *= f; S_ac *= f; S_rim *= f; S_c_frz *= f; S_shed *= f;
+            }
+            const double S_ice_total = S_i_au + S_d_au + S_agg + S_i_cri + S_i_melt;
+            const double max_ice_loss = ic / dt_snow_dim;
+            if (S_ice_total > max_ice_loss && S_ice_total > 0) {
+                const double f = max_ice_loss / S_ice_total;
+                S_i_au *= f; S_d_au *= f; S_agg *= f; S_i_cri *= f; S_i_melt *= f;
+            }
+            Sv[id] = -scc[id] + S_ev - S_i_dep - S_s_dep - S_nuc;
+            Sc[id] = scc[id] - S_c_au - S_ac - S_c_frz + S_i_melt - S_rim - S_shed;
+            Si[id] = S_nuc + S_c_frz + S_i_dep - S_i_melt - S_i_au - S_d_au - S_agg - S_i_cri;
+            Sr[id] = S_c_au + S_ac - S_ev + S_shed - S_r_cri - S_r_frz + S_s_melt;
+            Ss[id] = S_i_au + S_d_au + S_agg + S_rim + S_s_dep + S_i_cri + S_r_cri + S_r_frz - S_s_melt;
+            const double dP_rain = (S_c_au + S_ac + S_shed + S_s_melt - S_ev - S_r_frz) * mass_layer;
+            Pr[id] = dmin(P_max, dmax(0.0, Pr[s.ix(i + 1, j, k)] + dP_rain));
+            const double dP_snow = (S_i_au + S_d_au + S_rim + S_agg + S_r_frz - S_s_melt) * mass_layer;
+            Ps[id] = (t_u < t_0 && t_u >= P.t_000) ? dmin(P_max, dmax(0.0, Ps[s.ix(i + 1, j, k)] + dP_snow)) : 0.0;
+            Pt[id] = dmin(P_max, Pr[id] + Ps[id]);
+        }
+        if (std::fabs(Pr[s.ix(i_check, j, k)] - Rain_check) * 8.64e4 <= 1.0e-3) break;
+    }
+}
+
+// IceSchemeCommon::extrapolateBC (IceSchemeCommon.h:28-55) with c43 = 1, c13 = 0 (ints,
+// cAtmosphereModel.h:158) and the phi-seam average.
+void extrapolate_bc(S& s, double* P) {
+    const double c43 = 1.0, c13 = 0.0;
+    const int im = s.im, jm = s.jm, km = s.km;
+    for (int j = 0; j < jm; j++)
+        for (int k = 0; k < km; k++)
+            P[s.ix(im - 1, j, k)] = c43 * P[s.ix(im - 2, j, k)] - c13 * P[s.ix(im - 3, j, k)];
+    for (int k = 0; k < km; k++)
+        for (int i = 0; i < im; i++) {
+            P[s.ix(i, 0, k)] = c43 * P[s.ix(i, 1, k)] - c13 * P[s.ix(i, 2, k)];
+            P[s.ix(i, jm - 1, k)] = c43 * P[s.ix(i, jm - 2, k)] - c13 * P[s.ix(i, jm - 3, k)];
+        }
+    for (int i = 0; i < im; i++)
+        for (int j = 0; j < jm; j++) {
+            double& a = P[s.ix(i, j, 0)];
+            double& b = P[s.ix(i, j, km - 1)];
+            a = c43 * P[s.ix(i, j, 1)] - c13 * P[s.ix(i, j, 2)];
+            b = c43 * P[s.ix(i, j, km - 2)] - c13 * P[s.ix(i, j, km - 3)];
+            a = b = (a + b) * 0.5;
+        }
+}
+
+void ice_scheme(S& s) {
+    ensure_mp(s);
+    double* c = s.F(ORC_C); double* cl = s.F(ORC_CLOUD); double* ice = s.F(ORC_ICE);
+    double* Pr = s.mp[ORC_MP_P_RAIN].data(); double* Ps = s.mp[ORC_MP_P_SNOW].data();
+    double* Prn = s.mp[ORC_MP_P_RAINN].data(); double* Psn = s.mp[ORC_MP_P_SNOWN].data();
+    for (size_t q = 0; q < s.n; q++) {   // initArrays, TwoCatIceScheme.h:139-155
+        c[q] = dmax(0.0, c[q]); cl[q] = dmax(0.0, cl[q]); ice[q] = dmax(0.0, ice[q]);
+        Prn[q] = dmax(0.0, Prn[q]); Psn[q] = dmax(0.0, Psn[q]);
+        Pr[q] = Prn[q]; Ps[q] = Psn[q];
+    }
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k < s.km - 1; k++)
+        for (int j = 1; j < s.jm - 1; j++) ice_column(s, j, k);
+    extrapolate_bc(s, Pr);
+    extrapolate_bc(s, Ps);
+    damp_wiggles(s, Pr);
+    damp_wiggles(s, Ps);
+}
+
+// The stored copies of the precipitation fluxes, as storeIntermediateData3D keeps them
+// (UtilsAtm.h:41-42).
+void ice_store(S& s) {
+    ensure_mp(s);
+    s.mp[ORC_MP_P_RAINN] = s.mp[ORC_MP_P_RAIN];
+    s.mp[ORC_MP_P_SNOWN] = s.mp[ORC_MP_P_SNOW];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
@@ -1549,6 +1784,13 @@ double orc_scalar(const orc_state* s, const char* name) {
     if (n.rfind("rad", 0) == 0) return s->rad[std::stoi(n.substr(3))];
     if (n.rfind("the", 0) == 0) return s->the[std::stoi(n.substr(3))];
     return NAN;
+}
+void orc_ice_scheme(orc_state* s) { ice_scheme(*s); }
+void orc_ice_store(orc_state* s) { ice_store(*s); }
+double* orc_mp_ptr(orc_state* s, int which) {
+    if (which < 0 || which >= ORC_NMP) return nullptr;
+    ensure_mp(*s);
+    return s->mp[which].data();
 }
 void orc_step(orc_state* s, int order) { step(*s, order); }
 void orc_run(orc_state* s, int n, int order) { for (int q = 0; q < n; q++) step(*s, order); }

@@ -87,6 +87,22 @@ void orc_radiation(orc_state* s);
 // (jm x km), 3 top short wave (jm); NULL before the first call. orc_scalar(s, "t_skin").
 double* orc_rad_ptr(orc_state* s, int which);
 
+// Two-category ice scheme (CategoryIceScheme = 2): TwoCatIceScheme::run
+// (TwoCatIceScheme.h:89-111) on the current state, then run_3D_loop's damp_wiggles of the
+// two precipitation fluxes (cAtmosphereModel.cpp:1243-1244). Reads t c cloud ice co2 p_stat
+// r_humid S_c_c and the stored fluxes P_rainn, P_snown; writes the fluxes, Precipitation and
+// S_v S_c S_i S_r S_s, and (where no condensed phase can exist) t c cloud ice gr S_c_c.
+enum orc_mp {
+    ORC_MP_P_RAIN = 0, ORC_MP_P_SNOW, ORC_MP_P_RAINN, ORC_MP_P_SNOWN, ORC_MP_PRECIP,
+    ORC_MP_S_V, ORC_MP_S_C, ORC_MP_S_I, ORC_MP_S_R, ORC_MP_S_S,
+    ORC_NMP
+};
+void    orc_ice_scheme(orc_state* s);
+// P_rainn <- P_rain, P_snown <- P_snow (storeIntermediateData3D, UtilsAtm.h:41-42).
+void    orc_ice_store(orc_state* s);
+// The arrays above (im*jm*km doubles, [i][j][k]); allocated, zero, at first use.
+double* orc_mp_ptr(orc_state* s, int which);
+
 // Individual thermodynamic functions (known-answer tests).
 double orc_p_sat(double T);
 double orc_p_subl(double T);

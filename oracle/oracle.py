@@ -85,6 +85,10 @@ def lib():
         L.orc_radiation.argtypes = [vp]
         L.orc_rad_ptr.restype = dp
         L.orc_rad_ptr.argtypes = [vp, i]
+        L.orc_ice_scheme.argtypes = [vp]
+        L.orc_ice_store.argtypes = [vp]
+        L.orc_mp_ptr.restype = dp
+        L.orc_mp_ptr.argtypes = [vp, i]
         _lib = L
     return _lib
 
@@ -173,6 +177,22 @@ class OracleState:
         out["t_skin"] = self.scalar("t_skin")
         return out
 
+    def ice_scheme(self):
+        """The two-category ice scheme and the damping of its two precipitation fluxes."""
+        lib().orc_ice_scheme(self.h)
+
+    def ice_store(self):
+        """P_rainn <- P_rain, P_snown <- P_snow, as the store keeps them."""
+        lib().orc_ice_store(self.h)
+
+    def mp(self, name) -> np.ndarray:
+        """A live (writable) view of one ice-scheme array (MP_FIELDS)."""
+        p = lib().orc_mp_ptr(self.h, MP_FIELDS.index(name))
+        return np.ctypeslib.as_array(p, shape=self.shape)
+
+    def mp_snapshot(self) -> dict:
+        return {n: self.mp(n).copy() for n in MP_FIELDS}
+
     def level_mean(self):
         return np.array([lib().orc_level_mean(self.h, i) for i in range(self.shape[0])])
 
@@ -188,6 +208,61 @@ class OracleState:
 
 
 RAD_OUTPUTS = ["radiation", "epsilon", "albedo", "sw"]
+# Arrays of the two-category ice scheme, in the order of oracle/athad_oracle.h's orc_mp.
+MP_FIELDS = ["P_rain", "P_snow", "P_rainn", "P_snown", "Precipitation", "S_v", "S_c", "S_i", "S_r", "S_s"]
+
+
+def _unit(seed, stream, n):
+    """n uniform doubles in [0, 1): SplitMix64 of (seed, stream, index), 53 high bits.
+    Integer arithmetic only, so the values do not depend on the numpy version."""
+    with np.errstate(over="ignore"):
+        x = (np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) + np.uint64(stream) * np.uint64(0xD1B54A32D192ED03)
+             + np.arange(n, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    return (x >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def plant_microphysics(fields: dict, mp: dict, seed: int):
+    """Deterministic condensate, falling precipitation and temperature spread that drive every
+    branch of the two-category ice scheme: cloud water above and below the autoconversion
+    threshold, ice absent in a third of the cells (nucleation), stored rain and snow fluxes up
+    to the flux cap, vapour scattered around saturation, and the isothermal upper levels spread
+    over 224-259 K (homogeneous freezing, nucleation and deposition bands). Modifies the
+    arrays in place."""
+    shape = fields["t"].shape
+    n = int(np.prod(shape))
+    u = lambda q: _unit(seed, q, n).reshape(shape)   # noqa: E731
+    fields["cloud"][...] = 2.0e-3 * u(1)
+    ice = 6.0e-4 * u(2)
+    ice[u(3) < 0.3] = 0.0
+    fields["ice"][...] = ice
+    fields["c"][...] *= 0.9 + 0.2 * u(4)
+    fields["S_c_c"][...] = 2.0e-6 * (u(5) - 0.5)
+    t = fields["t"]
+    cold = t * 273.15 < 255.0
+    t[...] = np.where(cold, t * (0.88 + 0.14 * u(6)), t * (0.99 + 0.02 * u(7)))
+    rain = 4.0e-4 * u(8)
+    rain[u(9) < 0.1] = 0.0
+    snow = 4.0e-4 * u(10)
+    snow[u(11) < 0.1] = 0.0
+    mp["P_rainn"][...] = rain
+    mp["P_snown"][...] = snow
+
+
+def save_mp(path, d: dict):
+    """Ice-scheme arrays next to a state file, in MP_FIELDS order (oracle/ref_harness.cpp)."""
+    with open(path, "wb") as f:
+        for n in MP_FIELDS:
+            f.write(np.ascontiguousarray(d[n], dtype=np.float64).tobytes())
+
+
+def read_mp(path, shape):
+    raw = np.fromfile(path, dtype=np.float64)
+    n = int(np.prod(shape))
+    assert raw.size == len(MP_FIELDS) * n, "ice-scheme file size"
+    return {name: raw[q * n:(q + 1) * n].reshape(shape) for q, name in enumerate(MP_FIELDS)}
 
 
 def read_rad(path, shape):

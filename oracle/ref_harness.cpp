@@ -13,6 +13,10 @@
 //   mode = radiation        run_3D_loop's mode-2 radiation once (MultiLayerRadiation, the
 //                           omega_rad blend of t, updateSkinTemperature)
 //   mode = steps_rad        n iterations of the scoped rows, each followed by that radiation
+//   mode = ice              TwoCatIceScheme::run once, then run_3D_loop's damp_wiggles of P_rain
+//                           and P_snow (cAtmosphereModel.cpp:1230-1244); reads and writes the
+//                           ice-scheme arrays in <in.bin>.mp / <out.bin>.mp: P_rain, P_snow,
+//                           P_rainn, P_snown, Precipitation, S_v, S_c, S_i, S_r, S_s
 // The radiation modes also write <out.bin>.rad: radiation and epsilon (im*jm*km doubles each),
 // albedo (jm*km), the top short wave (jm) and t_skin.
 // State file: int32 im,jm,km,iter_n,total_iter; then 35 fields of im*jm*km doubles
@@ -25,6 +29,7 @@
 #include "BC_Atm.h"
 #include "TurbulenceAtm.h"
 #include "MultiLayerRadiation.h"
+#include "TwoCatIceScheme.h"
 
 #include <cstdio>
 #include <cstring>
@@ -260,6 +265,42 @@ public:
         f.write((const char*)&m.t_skin, 8);
     }
 
+    static std::vector<Array*> mp_fields(cAtmosphereModel& m) {
+        return {&m.P_rain, &m.P_snow, &m.P_rainn, &m.P_snown, &m.Precipitation,
+                &m.S_v, &m.S_c, &m.S_i, &m.S_r, &m.S_s};
+    }
+
+    static bool load_mp(cAtmosphereModel& m, const std::string& fn) {
+        std::ifstream f(fn, std::ios::binary);
+        std::vector<double> buf((size_t)m.im * m.jm * m.km);
+        for (Array* a : mp_fields(m)) {
+            f.read((char*)buf.data(), buf.size() * 8);
+            size_t q = 0;
+            for (int i = 0; i < m.im; i++)
+                for (int j = 0; j < m.jm; j++)
+                    for (int k = 0; k < m.km; k++) a->x[i][j][k] = buf[q++];
+        }
+        return (bool)f;
+    }
+
+    static void save_mp(cAtmosphereModel& m, const std::string& fn) {
+        std::ofstream f(fn, std::ios::binary);
+        std::vector<double> buf((size_t)m.im * m.jm * m.km);
+        for (Array* a : mp_fields(m)) {
+            size_t q = 0;
+            for (int i = 0; i < m.im; i++)
+                for (int j = 0; j < m.jm; j++)
+                    for (int k = 0; k < m.km; k++) buf[q++] = a->x[i][j][k];
+            f.write((const char*)buf.data(), buf.size() * 8);
+        }
+    }
+
+    static void ice(cAtmosphereModel& m) {
+        TwoCatIceScheme(m).run();
+        damp(m, m.P_rain);
+        damp(m, m.P_snow);
+    }
+
     // RunTimeSlice's thermodynamic init chain, cAtmosphereModel.cpp:527-568 (without
     // initCloudIce; condensate stays zero as resetArrays left it).
     static void init_chain(cAtmosphereModel& m) {
@@ -299,6 +340,10 @@ int main(int argc, char** argv) {
             RadiationSelfTest::radiation(*m);
         }
         RadiationSelfTest::save_rad(*m, std::string(argv[2]) + ".rad");
+    } else if (mode == "ice") {
+        if (!RadiationSelfTest::load_mp(*m, std::string(argv[1]) + ".mp")) return 1;
+        RadiationSelfTest::ice(*m);
+        RadiationSelfTest::save_mp(*m, std::string(argv[2]) + ".mp");
     } else if (mode.rfind("op:", 0) == 0) {
         RadiationSelfTest::op(*m, mode.substr(3));
     } else if (mode == "init") {

@@ -50,17 +50,70 @@ def worst(res, skip=("S_c_c",)):
     return max(v["rel_l2"] for k, v in res.items() if k not in skip)
 
 
+ICE_STATE = ["t", "c", "cloud", "ice", "gr", "S_c_c"]
+
+
+def check_ice(st, iter_n, base, wd, plant=None):
+    """TwoCatIceScheme twice in a row (the second from the first call's fluxes, stored as
+    P_rainn / P_snown), the oracle against the reference on the same inputs; per-field relative
+    L2. plant: seed of oracle.plant_microphysics applied to the input first, else zero fluxes."""
+    shape = st.shape
+    out = {}
+    mp_in = None
+    snap = st.snapshot()
+    if plant is not None:
+        mp_in = {n: np.zeros(shape) for n in O.MP_FIELDS}
+        O.plant_microphysics(snap, mp_in, plant)
+    for call in (1, 2):
+        t0 = time.time()
+        s2 = O.OracleState(*shape, blank=True)
+        s2.load(snap)
+        s2.set_iter(iter_n, st.total_iter)
+        if mp_in is not None:
+            for n, v in mp_in.items():
+                s2.mp(n)[...] = v
+        s2.save_bin(base, iter_n=iter_n)
+        O.save_mp(base + ".mp", s2.mp_snapshot())
+        s2.ice_scheme()
+        ref = run_ref(base, "ice", threads=8, workdir=wd)
+        rmp = O.read_mp(os.path.join(wd, "ref_out.bin.mp"), shape)
+        res = compare(s2.snapshot(), ref, ICE_STATE)
+        omp = s2.mp_snapshot()
+        for n in O.MP_FIELDS:
+            res[n] = {"rel_l2": O.rel_l2(omp[n], rmp[n]), "max_abs": float(np.max(np.abs(omp[n] - rmp[n]))),
+                      "norm": float(np.sqrt(np.sum(rmp[n] ** 2)))}
+        out[f"call{call}"] = res
+        print(f"ice scheme call {call}: worst rel L2 {max(v['rel_l2'] for v in res.values()):.3e}  "
+              f"|P_rain| {res['P_rain']['norm']:.3e} |P_snow| {res['P_snow']['norm']:.3e}  ({time.time()-t0:.1f}s)",
+              flush=True)
+        s2.ice_store()
+        mp_in = s2.mp_snapshot()
+        snap = s2.snapshot()
+        del s2
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--out", default=os.path.join(HERE, "_ref", "validation.json"))
     ap.add_argument("--spin", type=int, default=6, help="oracle steps before the per-operator checks")
+    ap.add_argument("--ice-only", action="store_true", help="only the ice-scheme check")
     args = ap.parse_args()
     if not os.path.exists(HARNESS):
         sys.exit("reference harness not built: make -C oracle -f Makefile.ref")
     report = {"grid": GRID}
     wd = tempfile.mkdtemp(prefix="athad_val_")
     im, jm, km = GRID
+    if args.ice_only:
+        st = O.OracleState(im, jm, km, seed=1234)
+        st.run(args.spin, "lex")
+        report["ice"] = check_ice(st, args.spin, os.path.join(wd, "base.bin"), wd)
+        report["ice_planted"] = check_ice(st, args.spin, os.path.join(wd, "base.bin"), wd, plant=5)
+        for call, res in report["ice_planted"].items():
+            for k, v in res.items():
+                print(f"   {call} {k:14s} {v['rel_l2']:.3e}  max {v['max_abs']:.3e}")
+        return
 
     # 1. thermodynamic init chain
     t0 = time.time()
@@ -110,6 +163,12 @@ def main():
     print(f"radiation       : worst rel L2 {max(v['rel_l2'] for v in report['radiation'].values()):.3e}  "
           f"({time.time()-t0:.1f}s)", flush=True)
     del s2
+
+    # 2c. the two-category ice scheme (+ the damping of its fluxes), twice: from zero stored
+    # fluxes, then from the first call's fluxes stored as P_rainn / P_snown
+    report["ice"] = check_ice(st, iter_n, base, wd)
+    report["ice_planted"] = check_ice(st, iter_n, base, wd, plant=5)
+    del st
 
     # 3. whole scoped steps from the initial state (lex = 1 reference thread)
     t0 = time.time()

@@ -8,6 +8,7 @@ in the tests from the same seed, so only outputs are stored.
 
     python tests/golden/make_ref_vectors.py               # ref_vectors.npz
     python tests/golden/make_ref_vectors.py --radiation   # ref_radiation.npz
+    python tests/golden/make_ref_vectors.py --ice         # ref_ice.npz
 """
 from __future__ import annotations
 
@@ -96,9 +97,58 @@ def radiation_vectors():
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+ICE_STATE = ["t", "c", "cloud", "ice", "gr", "S_c_c"]
+ICE_PLANT = 5     # seed of oracle.plant_microphysics for the ice-scheme input
+
+
+def ice_vectors():
+    """The two-category ice scheme (TwoCatIceScheme::run, then the damping of P_rain and
+    P_snow) twice in a row on the Jacobi-spun-up state with planted condensate, fluxes and
+    temperature spread; the second call starts from the first call's fluxes stored as
+    P_rainn / P_snown."""
+    wd = tempfile.mkdtemp(prefix="athad_golden_ice_")
+    n = GRID[0] * GRID[1] * GRID[2]
+    idx = sample_index(n)
+    out = {"grid": np.array(GRID), "seed": np.array(SEED), "spin": np.array(SPIN), "plant": np.array(ICE_PLANT),
+           "index": idx}
+    st = O.OracleState(*GRID, seed=SEED)
+    st.run(SPIN, "jacobi")
+    snap = st.snapshot()
+    mp = {k: np.zeros(GRID) for k in O.MP_FIELDS}
+    O.plant_microphysics(snap, mp, ICE_PLANT)
+    base, o = os.path.join(wd, "base.bin"), os.path.join(wd, "out.bin")
+    for call in (1, 2):
+        st = O.OracleState(*GRID, blank=True)
+        st.load(snap)
+        st.set_iter(SPIN, SPIN)
+        st.save_bin(base, iter_n=SPIN)
+        O.save_mp(base + ".mp", mp)
+        subprocess.run([HARNESS, base, o, "ice"], check=True, cwd=wd, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS="8"))
+        ref, rmp = O.read_bin(o), O.read_mp(o + ".mp", GRID)
+        d = {k: ref[k] for k in ICE_STATE}
+        d.update(rmp)
+        for name, a in d.items():
+            a = a.reshape(-1)
+            out[f"call{call}/{name}/sample"] = a[idx]
+            out[f"call{call}/{name}/sum"] = np.array(np.sum(a))
+            out[f"call{call}/{name}/l2"] = np.array(np.sqrt(np.sum(a * a)))
+        # next call: the reference's output state, its fluxes stored as the store keeps them
+        snap = {k: ref[k] for k in ref}
+        mp = dict(rmp)
+        mp["P_rainn"], mp["P_snown"] = rmp["P_rain"].copy(), rmp["P_snow"].copy()
+        print(f"ice call {call} done", flush=True)
+    path = os.path.join(HERE, "ref_ice.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle -f Makefile.ref")
+    if "--ice" in sys.argv[1:]:
+        ice_vectors()
+        return
     if "--radiation" in sys.argv[1:]:
         radiation_vectors()
         return

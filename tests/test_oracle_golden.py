@@ -126,3 +126,45 @@ def test_two_steps_with_radiation_lex(oracle):
     check_pack("steps2_rad", st.snapshot(), gold, 1e-11, skip=("S_c_c",))
     # the albedo follows the cloud water, which two iterations carry to ~1e-11 (SatAdj)
     check_rad("steps2_rad", st.rad_outputs(), gold, 1e-10)
+
+
+GOLD_ICE = os.path.join(HERE, "golden", "ref_ice.npz")
+ICE_STATE = ["t", "c", "cloud", "ice", "gr", "S_c_c"]
+
+
+@pytest.mark.skipif(not os.path.exists(GOLD_ICE), reason="ice-scheme vectors not generated")
+def test_ice_scheme_twice(oracle):
+    """TwoCatIceScheme (+ the damping of P_rain, P_snow) twice in a row on the spun-up state
+    with planted condensate, fluxes and temperature spread (oracle.plant_microphysics), against
+    the reference's compiled scheme on the same input. The reference is built with -ffast-math
+    -O2, which reassociates the rate sums: 2e-14 measured, bar 1e-12."""
+    gold = np.load(GOLD_ICE, allow_pickle=False)
+    g = tuple(int(x) for x in gold["grid"])
+    spin = int(gold["spin"])
+    st = oracle.OracleState(*g, seed=int(gold["seed"]))
+    st.run(spin, "jacobi")
+    snap = st.snapshot()
+    mp = {k: np.zeros(g) for k in oracle.MP_FIELDS}
+    oracle.plant_microphysics(snap, mp, int(gold["plant"]))
+    st.load(snap)
+    for k, v in mp.items():
+        st.mp(k)[...] = v
+    idx = gold["index"]
+    for call in (1, 2):
+        st.ice_scheme()
+        got = {k: st.field(k) for k in ICE_STATE}
+        got.update(st.mp_snapshot())
+        bad = {}
+        for name, a in got.items():
+            a = a.reshape(-1)
+            l_ref = float(gold[f"call{call}/{name}/l2"])
+            e = rel(a[idx], gold[f"call{call}/{name}/sample"])
+            e = max(e, abs(float(np.sum(a)) - float(gold[f"call{call}/{name}/sum"])) / max(l_ref, 1e-300),
+                    abs(float(np.sqrt(np.sum(a * a))) - l_ref) / max(l_ref, 1e-300))
+            if e > 1e-12:
+                bad[name] = e
+        assert not bad, f"call {call}: {bad}"
+        # the scheme did something: rain and snow fall, every source is live
+        for name in ("P_rain", "P_snow", "S_v", "S_c", "S_i", "S_r", "S_s"):
+            assert float(gold[f"call{call}/{name}/l2"]) > 0.0, name
+        st.ice_store()
