@@ -42,7 +42,11 @@ EXPORTS = [
     "athad_field_stats", "athad_set_member_params", "athad_restart_bytes", "athad_save_state_mem",
     "athad_load_state_mem", "athad_save_state", "athad_load_state", "athad_field_minmax", "athad_residuum",
     "athad_meridional_streamfunction", "athad_get_slice", "athad_radiation", "athad_get_radiation",
+    "athad_ice_scheme", "athad_set_mp_field", "athad_get_mp_field",
 ]
+
+# Arrays of the two-category ice scheme (athad_mp_field in include/athad_hip.h)
+MP_FIELDS = ["P_rain", "P_snow", "P_rainn", "P_snown", "Precipitation", "S_v", "S_c", "S_i", "S_r", "S_s"]
 
 # restart_arrays() of the reference checkpoint (atmosphere/FileIO_Atm.cpp:297-304), in file
 # order; the four precipitation arrays are not held by a context (written as zeros)
@@ -125,6 +129,9 @@ def lib():
         L.athad_get_slice.argtypes = [vp, i, i, i, i, dp]
         L.athad_radiation.argtypes = [vp]
         L.athad_get_radiation.argtypes = [vp, i, dp, dp, dp, dp]
+        L.athad_ice_scheme.argtypes = [vp]
+        L.athad_set_mp_field.argtypes = [vp, i, i, dp]
+        L.athad_get_mp_field.argtypes = [vp, i, i, dp]
         L.athad_residuum.argtypes = [vp, i, dp, ip, dp, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64), ip]
         for name in EXPORTS:

@@ -28,7 +28,7 @@ struct Config {
     double omega = 0.000317, g = 9.8066, r_Earth = 6370.001, t_0 = 273.15, t_skin = 254.0;
     double dt_visc = 4e-05, L_atm = 6287.5, zeta = 3.0, sc_WaterVapour = 0.61, sc_CO2 = 0.96;
     double pr_turb = 0.9, abl_height = 10000.0, u_0 = 8.0, cloud_cap = 0.05, t_max_phys = 2000.0;
-    double t_00 = 236.15, cp_l = 1349.0, co2_scale = 1.0;
+    double t_00 = 236.15, t_000 = 253.15, cp_l = 1349.0, co2_scale = 1.0;
     // radiation: cAtmosphereDefaults.cpp.inc:31-43,72-73 and run_3D_loop's omega_rad
     double sigma = 5.67028e-08, kappa_H2O = 0.01, kappa_CO2 = 0.001, kappa_bg = 1e-06;
     double geothermal_flux = 150.0, rad_equator_short = 298.0, rad_pole_short = 0.0;
@@ -81,6 +81,7 @@ struct MemberHost {
     int* counters = nullptr;
     int* passes = nullptr;
     double *radiation = nullptr, *epsilon = nullptr, *albedo = nullptr;   // radiation outputs, on first use
+    void* mp[NMP] = {};            // ice-scheme arrays, on first use
     int iter_n = 0, total = 0;     // host mirror of the device counters
     // table offsets inside 'tables'
     size_t o_rad = 0, o_sint = 0, o_cost = 0, o_sinr = 0, o_hgt = 0, o_nuef = 0, o_step = 0;
@@ -178,6 +179,7 @@ void fill_consts(MemberHost& m, const Config& cfg, int order) {
     K.init_M_nw = M_nonwater_h(m.p.c_0, m.p.co2_0, C.M_bg);
     K.init_f_co2 = (C.q_CO2 + C.q_bg > 0.0) ? C.q_CO2 / (C.q_CO2 + C.q_bg) : m.p.co2_0 * cfg.co2_scale;
     K.co2_scale = cfg.co2_scale;
+    K.t_000 = cfg.t_000;
     K.sigma = cfg.sigma; K.kappa_h2o = cfg.kappa_H2O; K.kappa_co2 = cfg.kappa_CO2; K.kappa_bg = cfg.kappa_bg;
     K.geothermal = cfg.geothermal_flux; K.sw_equator = cfg.rad_equator_short; K.sw_pole = cfg.rad_pole_short;
     K.albedo_surface = cfg.albedo_surface; K.albedo_cloud = cfg.albedo_cloud; K.omega_rad = cfg.omega_rad;
@@ -263,6 +265,7 @@ void make_desc(const athad_ctx* c, const MemberHost& m, int parity, MemberDesc& 
     d.sinr = m.tables + m.o_sinr; d.hgt = m.tables + m.o_hgt; d.nue_f = m.tables + m.o_nuef;
     d.step_dim = m.tables + m.o_step; d.passes = m.passes; d.counters = m.counters; d.scratch = m.scratch;
     d.radiation = m.radiation; d.epsilon = m.epsilon; d.albedo = m.albedo;
+    for (int q = 0; q < NMP; q++) d.mp[q] = m.mp[q];
     (void)c;
 }
 
@@ -284,6 +287,7 @@ void free_ctx(athad_ctx* c) {
         for (auto p : m.field) if (p) hipFree(p);
         for (auto p : m.alt) if (p) hipFree(p);
         for (auto p : m.tmp) if (p) hipFree(p);
+        for (auto p : m.mp) if (p) hipFree(p);
         for (void* p : {(void*)m.vel_star, (void*)m.t_top, (void*)m.t_ref, (void*)m.tables, (void*)m.scratch,
                         (void*)m.counters, (void*)m.passes, (void*)m.radiation, (void*)m.epsilon, (void*)m.albedo})
             if (p) hipFree(p);
@@ -551,6 +555,7 @@ void op_step(athad_ctx* c) {
         Prof p(c, KID_STORE);
         launch_store(cur(c), 0x7FF & ~0xE);   // all but u, v, w (stored by the radial filter)
     }
+    if (c->mem[0].mp[0]) launch_ice_store(cur(c));   // P_rainn, P_snown (UtilsAtm.h:41-42)
     for (auto& m : c->mem) { m.iter_n++; m.total++; }
 }
 
@@ -893,6 +898,77 @@ athad_status athad_get_radiation(athad_ctx* c, int member, double* radiation, do
     return ATHAD_OK;
 }
 
+// The ice-scheme arrays are allocated, zero, at the first call that needs them, for every
+// member at once; a 4-iteration graph captured before then did not store the fluxes, so it is
+// dropped and recaptured.
+static athad_status ensure_mp_buffers(athad_ctx* c) {
+    if (c->mem[0].mp[0]) return ATHAD_OK;
+    if (c->capturing) { c->err = "the ice-scheme arrays cannot be allocated during capture"; return ATHAD_ERR_STATE; }
+    for (auto& m : c->mem)
+        for (int q = 0; q < NMP; q++) {
+            if (hipMalloc(&m.mp[q], m.n * elem_size(c)) != hipSuccess) {
+                m.mp[q] = nullptr;
+                for (auto& m2 : c->mem)
+                    for (auto& p : m2.mp) if (p) { hipFree(p); p = nullptr; }
+                c->err = "out of device memory for the ice-scheme arrays";
+                return ATHAD_ERR_ALLOC;
+            }
+            HIPCHK(c, hipMemset(m.mp[q], 0, m.n * elem_size(c)));
+        }
+    if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    return upload_descs(c);
+}
+
+athad_status athad_ice_scheme(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    athad_status s = ensure_mp_buffers(c);
+    if (s) return s;
+    join_s2(c);
+    launch_ice_scheme(cur(c));
+    return after_launch(c);
+}
+
+athad_status athad_set_mp_field(athad_ctx* ctx, int member, int which, const double* src) {
+    athad_status s = check_member(ctx, member, which, NMP);
+    if (s) return s;
+    if (!src) return ATHAD_ERR_ARG;
+    if ((s = ensure_mp_buffers(ctx))) return s;
+    join_s2(ctx);
+    MemberHost& m = ctx->mem[member];
+    if (ctx->precision == ATHAD_FP64) {
+        HIPCHK(ctx, hipMemcpyAsync(m.mp[which], src, m.n * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    } else {
+        std::vector<float> tmp(m.n);
+        for (size_t q = 0; q < m.n; q++) tmp[q] = (float)src[q];
+        HIPCHK(ctx, hipMemcpyAsync(m.mp[which], tmp.data(), m.n * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return ATHAD_OK;
+}
+
+athad_status athad_get_mp_field(athad_ctx* ctx, int member, int which, double* dst) {
+    athad_status s = check_member(ctx, member, which, NMP);
+    if (s) return s;
+    if (!dst) return ATHAD_ERR_ARG;
+    join_s2(ctx);
+    MemberHost& m = ctx->mem[member];
+    if (!m.mp[which]) {   // never allocated: the arrays are zero, as resetArrays leaves them
+        std::fill(dst, dst + m.n, 0.0);
+        return ATHAD_OK;
+    }
+    if (ctx->precision == ATHAD_FP64) {
+        HIPCHK(ctx, hipMemcpyAsync(dst, m.mp[which], m.n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    } else {
+        std::vector<float> tmp(m.n);
+        HIPCHK(ctx, hipMemcpyAsync(tmp.data(), m.mp[which], m.n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        for (size_t q = 0; q < m.n; q++) dst[q] = (double)tmp[q];
+    }
+    return ATHAD_OK;
+}
+
 athad_status athad_set_member_params(athad_ctx* c, int member, const athad_member_params* p) {
     if (!c || !p || member < 0 || member >= c->nmem) return ATHAD_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1212,6 +1288,7 @@ athad_status athad_filters(athad_ctx* c) {
 athad_status athad_store(athad_ctx* c) {
     if (!c) return ATHAD_ERR_ARG;
     { Prof p(c, KID_STORE); launch_store(cur(c), 0x7FF); }
+    if (c->mem[0].mp[0]) launch_ice_store(cur(c));
     return after_launch(c);
 }
 

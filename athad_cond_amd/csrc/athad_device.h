@@ -20,6 +20,9 @@ enum Field {
 };
 constexpr int NPROG = 11;   // t u v w c cloud ice gr co2 tke dis
 constexpr int NTMP = 12;    // scratch fields for the filter passes
+// Arrays of the two-category ice scheme (same numbering as include/athad_hip.h): the rain and
+// snow fluxes, their stored copies, the capped total and the microphysics sources.
+enum MpField { MP_P_RAIN = 0, MP_P_SNOW, MP_P_RAINN, MP_P_SNOWN, MP_PRECIP, MP_S_V, MP_S_C, MP_S_I, MP_S_R, MP_S_S, NMP };
 
 // Per-member constants, all resolved on the host in double.
 struct Consts {
@@ -43,6 +46,7 @@ struct Consts {
     double albedo_surface, albedo_cloud, omega_rad;
     // initial-state recipe
     double t_surf_eq, t_surf_pole, init_M_nw, init_f_co2, co2_scale;
+    double t_000;       // lower bound of the snow band of the ice scheme
     unsigned long long seed;
     // per-member state knobs
     int order;          // 0 lex, 1 jacobi
@@ -70,6 +74,7 @@ struct MemberDesc {
     double* radiation;       // n        upward long-wave flux (allocated at the first radiation call)
     double* epsilon;         // n        layer emissivity
     double* albedo;          // jm*km    surface short-wave albedo
+    void* mp[NMP];           // n        ice-scheme arrays, storage precision S (allocated at first use)
 };
 
 #define ATHAD_DEV __device__ __forceinline__

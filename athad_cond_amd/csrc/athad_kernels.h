@@ -43,6 +43,12 @@ int sf_offset(int im);
 // per latitude at scratch[sf_offset(im)] (jm doubles). Grids up to radiation_max_levels() levels.
 void launch_radiation(const Batch& b);
 int radiation_max_levels();
+// The two-category ice scheme on every member (initArrays, the column passes, extrapolateBC of
+// the fluxes), then the damp_wiggles of P_rain and P_snow through scratch buffers 0-3. Members
+// must hold their ice-scheme arrays (MemberDesc::mp).
+void launch_ice_scheme(const Batch& b);
+// P_rainn <- P_rain, P_snown <- P_snow on members that hold ice-scheme arrays.
+void launch_ice_store(const Batch& b);
 // one plane (n values) of one field, widened to double, at scratch[sf_offset(im)]: axis 0 fixes
 // i ([j][k]), 1 fixes j ([i][k]), 2 fixes k ([i][j])
 void launch_slice(const Batch& b, int member, int field, int axis, int index, long long n);

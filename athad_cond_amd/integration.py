@@ -194,6 +194,24 @@ class Integration:
         out["t_skin"] = ts.value
         return out
 
+    # ---- two-category ice scheme (CategoryIceScheme = 2) -------------------------------
+    def ice_scheme(self):
+        """TwoCatIceScheme::run and the damping of P_rain, P_snow, every member."""
+        self._chk(L.lib().athad_ice_scheme(self.h))
+
+    def get_mp(self, name: str, member: int = 0) -> np.ndarray:
+        """One ice-scheme array (MP_FIELDS) as FP64."""
+        out = np.empty(self.grids[member])
+        self._chk(L.lib().athad_get_mp_field(self.h, member, L.MP_FIELDS.index(name),
+                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def set_mp(self, name: str, arr, member: int = 0):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == tuple(self.grids[member]), "shape mismatch"
+        self._chk(L.lib().athad_set_mp_field(self.h, member, L.MP_FIELDS.index(name),
+                                             a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
     # ---- checkpoint diagnostics on the device (MinMax_Atm.cpp, UtilsAtm.h) --------------
     def field_minmax(self, name: str, member: int = 0):
         """searchMinMax_3D: (min, (i, j, k), max, (i, j, k)) of the finite values."""

@@ -253,6 +253,29 @@ athad_status athad_radiation(athad_ctx* ctx);
 athad_status athad_get_radiation(athad_ctx* ctx, int member, double* radiation, double* epsilon, double* albedo,
                                  double* t_skin);
 
+/* --- two-category ice scheme (CategoryIceScheme = 2) -------------------------------- */
+
+/* Arrays of the ice scheme: Array members P_rain, P_snow, P_rainn, P_snown, Precipitation,
+ * S_v, S_c, S_i, S_r, S_s of cAtmosphereModel (atmosphere/cAtmosphereModel.h). Held in the
+ * context's storage precision from the first call that needs them, zero until then. */
+enum athad_mp_field {
+    ATHAD_P_RAIN = 0, ATHAD_P_SNOW, ATHAD_P_RAINN, ATHAD_P_SNOWN, ATHAD_PRECIPITATION,
+    ATHAD_S_V, ATHAD_S_C, ATHAD_S_I, ATHAD_S_R, ATHAD_S_S,
+    ATHAD_NMP
+};
+/* The ice-scheme call site of run_3D_loop (atmosphere/cAtmosphereModel.cpp:1230-1244 with
+ * CategoryIceScheme = 2): TwoCatIceScheme(*this).run() (atmosphere/TwoCatIceScheme.h:89-111:
+ * initArrays, computeColumns, extrapolateBC of the fluxes; applyTopography is a no-op without
+ * land) and the damp_wiggles of P_rain and P_snow, on every member. Reads t c cloud ice gr co2
+ * p_stat r_humid S_c_c and the stored fluxes P_rainn, P_snown; writes the fluxes,
+ * Precipitation and the five sources, and where no condensed phase can exist
+ * (IceSchemeCommon.h:104-190) t c cloud ice gr S_c_c. athad_store and every iteration's store
+ * then copy P_rain, P_snow to P_rainn, P_snown (UtilsAtm.h:41-42). */
+athad_status athad_ice_scheme(athad_ctx* ctx);
+/* Contiguous FP64 host arrays [im][jm][km] of one ice-scheme array (athad_mp_field). */
+athad_status athad_set_mp_field(athad_ctx* ctx, int member, int which, const double* src);
+athad_status athad_get_mp_field(athad_ctx* ctx, int member, int which, double* dst);
+
 /* --- measurement --------------------------------------------------------------------- */
 
 /* Per-kernel timing with HIP events on the context's stream: kernel ids as in

@@ -1213,6 +1213,11 @@ void store(S& s) {
     const int src[11] = {ORC_U, ORC_V, ORC_W, ORC_T, ORC_C, ORC_CLOUD, ORC_ICE, ORC_GR, ORC_CO2, ORC_TKE, ORC_DIS};
     const int dst[11] = {ORC_UN, ORC_VN, ORC_WN, ORC_TN, ORC_CN, ORC_CLOUDN, ORC_ICEN, ORC_GRN, ORC_CO2N, ORC_TKEN, ORC_DISN};
     for (int f = 0; f < 11; f++) s.f[dst[f]] = s.f[src[f]];
+    // the precipitation fluxes too (UtilsAtm.h:41-42), where the ice scheme has run
+    if (!s.mp.empty()) {
+        s.mp[ORC_MP_P_RAINN] = s.mp[ORC_MP_P_RAIN];
+        s.mp[ORC_MP_P_SNOWN] = s.mp[ORC_MP_P_SNOW];
+    }
 }
 
 void moist_block(S& s) {   // cAtmosphereModel.cpp:1185-1434 (scoped rows)
