@@ -7,6 +7,7 @@
 #include "athad_device.h"
 #include "athad_kernels.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -28,7 +29,7 @@ struct Config {
     double omega = 0.000317, g = 9.8066, r_Earth = 6370.001, t_0 = 273.15, t_skin = 254.0;
     double dt_visc = 4e-05, L_atm = 6287.5, zeta = 3.0, sc_WaterVapour = 0.61, sc_CO2 = 0.96;
     double pr_turb = 0.9, abl_height = 10000.0, u_0 = 8.0, cloud_cap = 0.05, t_max_phys = 2000.0;
-    double t_00 = 236.15, t_000 = 253.15, cp_l = 1349.0, co2_scale = 1.0;
+    double t_00 = 236.15, t_000 = 253.15, cp_l = 1349.0, co2_scale = 1.0, lv = 2520000.0, ls = 2830000.0;
     // radiation: cAtmosphereDefaults.cpp.inc:31-43,72-73 and run_3D_loop's omega_rad
     double sigma = 5.67028e-08, kappa_H2O = 0.01, kappa_CO2 = 0.001, kappa_bg = 1e-06;
     double geothermal_flux = 150.0, rad_equator_short = 298.0, rad_pole_short = 0.0;
@@ -118,6 +119,8 @@ struct athad_ctx {
     std::vector<double> staging;   // host staging for conversions
     // fused damp_wiggles tile: shape index and levels per block (autotuned at creation)
     int tile_shape = 0, tile_ich = 1 << 20;
+    // the two-category ice scheme runs inside the moist block (CategoryIceScheme = 2)
+    bool ice_on = false;
 };
 
 namespace {
@@ -180,6 +183,8 @@ void fill_consts(MemberHost& m, const Config& cfg, int order) {
     K.init_f_co2 = (C.q_CO2 + C.q_bg > 0.0) ? C.q_CO2 / (C.q_CO2 + C.q_bg) : m.p.co2_0 * cfg.co2_scale;
     K.co2_scale = cfg.co2_scale;
     K.t_000 = cfg.t_000;
+    K.coeff_energy = 1.0 / (m.p.c_0 * cfg.cp_l * cfg.t_0);
+    K.lv = cfg.lv; K.ls = cfg.ls;
     K.sigma = cfg.sigma; K.kappa_h2o = cfg.kappa_H2O; K.kappa_co2 = cfg.kappa_CO2; K.kappa_bg = cfg.kappa_bg;
     K.geothermal = cfg.geothermal_flux; K.sw_equator = cfg.rad_equator_short; K.sw_pole = cfg.rad_pole_short;
     K.albedo_surface = cfg.albedo_surface; K.albedo_cloud = cfg.albedo_cloud; K.omega_rad = cfg.omega_rad;
@@ -402,11 +407,31 @@ void op_moist_block(athad_ctx* c, int gate, bool poisson_beside = false) {
         launch_poisson(b2, 0);
     }
     op_turbulence(c, gate, c->stream2);
-    {
-        Prof p(c, KID_SATADJ);
-        launch_satadj(cur(c), gate, 1);
+    if (c->ice_on) {
+        // SatAdj (S_c_c left uncapped for the ice scheme), the damping of t ice c cloud at every
+        // level (the scheme reads t throughout), the ice scheme and its flux damping, then
+        // precipitationSum, cap_S and the microphysics clamp (cAtmosphereModel.cpp:1218-1399)
+        {
+            Prof p(c, KID_SATADJ);
+            launch_satadj(cur(c), gate, 1 | 2);
+        }
+        {
+            Prof p(c, KID_DAMP);
+            const int src[4] = {NFIELD + 6, NFIELD + 7, NFIELD + 8, NFIELD + 9};
+            launch_damp_tile(cur(c), c->tile_shape, c->tile_ich, src, kMoistSrc, 4, 0, gate);
+        }
+        {
+            Prof p(c, KID_ICE);
+            launch_ice_scheme(cur(c), gate);
+            launch_moist_clamp_ice(cur(c), gate);
+        }
+    } else {
+        {
+            Prof p(c, KID_SATADJ);
+            launch_satadj(cur(c), gate, 1);
+        }
+        op_damp_moist_in_block(c, gate);
     }
-    op_damp_moist_in_block(c, gate);
     {
         Prof p(c, KID_DENSITIES);
         launch_densities(cur(c), 0, 1, gate);
@@ -475,6 +500,7 @@ void op_bc(athad_ctx* c, bool solid) {
     {
         Prof p(c, KID_BC);
         launch_bc(cur(c));
+        if (c->mem[0].mp[0]) launch_bc_sources(cur(c));
     }
     if (solid) {
         Prof p(c, KID_SOLID);
@@ -498,11 +524,12 @@ void op_rk4(athad_ctx* c, bool fused) {
     if (c->order == ATHAD_ORDER_LEX) {
         Prof p(c, KID_RK4_PLANE);
         const int dmax = max_plane(c);
-        for (int d = 3; d <= dmax; d++) launch_rk4_lex(cur(c), d);
+        const bool src = c->mem[0].mp[0] != nullptr;
+        for (int d = 3; d <= dmax; d++) launch_rk4_lex(cur(c), d, src);
     } else {
         {
             Prof p(c, KID_RK4);
-            launch_rk4_jacobi(cur(c), fused);
+            launch_rk4_jacobi(cur(c), fused, c->mem[0].mp[0] != nullptr);
         }
         c->parity ^= 1;
     }
@@ -924,8 +951,19 @@ athad_status athad_ice_scheme(athad_ctx* c) {
     athad_status s = ensure_mp_buffers(c);
     if (s) return s;
     join_s2(c);
-    launch_ice_scheme(cur(c));
+    { Prof p(c, KID_ICE); launch_ice_scheme(cur(c), 0); }
     return after_launch(c);
+}
+
+athad_status athad_set_ice_scheme(athad_ctx* c, int on) {
+    if (!c) return ATHAD_ERR_ARG;
+    if (on) {
+        athad_status s = ensure_mp_buffers(c);
+        if (s) return s;
+    }
+    if (c->ice_on != (on != 0) && c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    c->ice_on = on != 0;
+    return ATHAD_OK;
 }
 
 athad_status athad_set_mp_field(athad_ctx* ctx, int member, int which, const double* src) {
@@ -1127,7 +1165,8 @@ constexpr int kRestartArrays = 29;
 constexpr int kRestartF[kRestartArrays] = {
     F_T, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2,
     F_TN, F_UN, F_VN, F_WN, F_CN, F_CLOUDN, F_ICEN, F_GRN, F_CO2N,
-    F_P_DYN, F_P_STAT, F_TKE, F_DIS, F_TKEN, F_DISN, F_NUE, -1, -1, -1, -1};
+    F_P_DYN, F_P_STAT, F_TKE, F_DIS, F_TKEN, F_DISN, F_NUE,
+    -1 - MP_P_RAIN, -1 - MP_P_SNOW, -1 - MP_P_RAINN, -1 - MP_P_SNOWN};   // negative: ice-scheme array
 int64_t restart_bytes(const athad_grid& g) {
     return 5 * 4 + (int64_t)kRestartArrays * g.im * g.jm * g.km * 8;
 }
@@ -1151,8 +1190,10 @@ athad_status athad_save_state_mem(athad_ctx* ctx, int member, void* buf, int64_t
     std::memcpy(out, hdr, sizeof(hdr));
     double* arr = (double*)(out + sizeof(hdr));
     for (int a = 0; a < kRestartArrays; a++, arr += m.n) {
-        if (kRestartF[a] < 0) { std::memset(arr, 0, m.n * 8); continue; }
-        if ((s = athad_get_field(ctx, member, kRestartF[a], arr))) return s;
+        // the precipitation arrays read zero where the ice scheme has never run
+        if (kRestartF[a] < 0) s = athad_get_mp_field(ctx, member, -1 - kRestartF[a], arr);
+        else s = athad_get_field(ctx, member, kRestartF[a], arr);
+        if (s) return s;
     }
     return ATHAD_OK;
 }
@@ -1172,8 +1213,13 @@ athad_status athad_load_state_mem(athad_ctx* ctx, int member, const void* buf, i
     join_s2(ctx);
     const double* arr = (const double*)((const unsigned char*)buf + sizeof(hdr));
     for (int a = 0; a < kRestartArrays; a++, arr += m.n) {
-        if (kRestartF[a] < 0) continue;
-        if ((s = athad_set_field(ctx, member, kRestartF[a], arr))) return s;
+        if (kRestartF[a] < 0) {
+            // precipitation arrays: held once the ice scheme has run; a file holding zeros
+            // (as the scoped step leaves them) allocates nothing
+            if (!m.mp[0] && std::all_of(arr, arr + m.n, [](double v) { return v == 0.0; })) continue;
+            s = athad_set_mp_field(ctx, member, -1 - kRestartF[a], arr);
+        } else s = athad_set_field(ctx, member, kRestartF[a], arr);
+        if (s) return s;
     }
     return athad_set_iter(ctx, member, m.iter_n, hdr[4]);
 }

@@ -47,6 +47,7 @@ struct Consts {
     // initial-state recipe
     double t_surf_eq, t_surf_pole, init_M_nw, init_f_co2, co2_scale;
     double t_000;       // lower bound of the snow band of the ice scheme
+    double coeff_energy, lv, ls;   // latent heating of the microphysics sources (RHS_Atm_Turb.cpp:367, 965-966)
     unsigned long long seed;
     // per-member state knobs
     int order;          // 0 lex, 1 jacobi

@@ -18,7 +18,7 @@ struct Batch {
 // Kernel ids for per-kernel timing.
 enum KernelId {
     KID_RK4 = 0, KID_RK4_PLANE, KID_LEVEL_MEAN, KID_POISSON, KID_SATADJ, KID_DAMP, KID_DENSITIES,
-    KID_TURB, KID_BC, KID_SOLID, KID_POLAR, KID_RADIAL, KID_STORE, KID_INIT, KID_COUNT
+    KID_TURB, KID_BC, KID_SOLID, KID_POLAR, KID_RADIAL, KID_STORE, KID_INIT, KID_ICE, KID_COUNT
 };
 
 void launch_advance(const Batch& b);
@@ -45,15 +45,22 @@ void launch_radiation(const Batch& b);
 int radiation_max_levels();
 // The two-category ice scheme on every member (initArrays, the column passes, extrapolateBC of
 // the fluxes), then the damp_wiggles of P_rain and P_snow through scratch buffers 0-3. Members
-// must hold their ice-scheme arrays (MemberDesc::mp).
-void launch_ice_scheme(const Batch& b);
+// must hold their ice-scheme arrays (MemberDesc::mp). Scratch 6-9 are free for it at the place of
+// the ice scheme inside the moist block (the damping has consumed them).
+void launch_ice_scheme(const Batch& b, int gate);
+// precipitationSum, cap_S and the microphysics clamp that follow the ice scheme in the moist block
+void launch_moist_clamp_ice(const Batch& b, int gate);
+// boundary faces of S_c_c and the five sources on members that hold ice-scheme arrays
+void launch_bc_sources(const Batch& b);
 // P_rainn <- P_rain, P_snown <- P_snow on members that hold ice-scheme arrays.
 void launch_ice_store(const Batch& b);
 // one plane (n values) of one field, widened to double, at scratch[sf_offset(im)]: axis 0 fixes
 // i ([j][k]), 1 fixes j ([i][k]), 2 fixes k ([i][j])
 void launch_slice(const Batch& b, int member, int field, int axis, int index, long long n);
-void launch_rk4_jacobi(const Batch& b, bool fused);
-void launch_rk4_lex(const Batch& b, int plane);
+// src: add the microphysics sources of the ice-scheme arrays to the tendencies of t, c,
+// cloud, ice (RHS_Atm_Turb.cpp:963-966, 1058-1067); every member must hold those arrays
+void launch_rk4_jacobi(const Batch& b, bool fused, bool src = false);
+void launch_rk4_lex(const Batch& b, int plane, bool src = false);
 void launch_poisson(const Batch& b, int gate);
 void launch_satadj(const Batch& b, int gate, int to_tmp);
 // pass 0,1,2 over the fields of one job

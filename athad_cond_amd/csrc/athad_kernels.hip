@@ -639,9 +639,10 @@ __global__ void k_albedo_rows(const MemberDesc* __restrict__ md) {
 template <typename S> ATHAD_DEV S* mpp(const MemberDesc& M, int q) { return (S*)M.mp[q]; }
 
 template <typename S>
-__global__ __launch_bounds__(256) void k_ice_columns(const MemberDesc* __restrict__ md) {
+__global__ __launch_bounds__(256) void k_ice_columns(const MemberDesc* __restrict__ md, int gate) {
 #pragma clang fp contract(off)
     const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
     const Consts& K = M.K;
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= K.jm * K.km) return;
@@ -816,9 +817,10 @@ __global__ __launch_bounds__(256) void k_ice_columns(const MemberDesc* __restric
 }
 
 template <typename S>
-__global__ __launch_bounds__(256) void k_ice_bc(const MemberDesc* __restrict__ md) {
+__global__ __launch_bounds__(256) void k_ice_bc(const MemberDesc* __restrict__ md, int gate) {
 #pragma clang fp contract(off)
     const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
     const Consts& K = M.K;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= K.n) return;
@@ -842,6 +844,71 @@ __global__ __launch_bounds__(256) void k_ice_store(const MemberDesc* __restrict_
     if (id >= M.K.n || !M.mp[0]) return;
     mpp<S>(M, MP_P_RAINN)[id] = mpp<S>(M, MP_P_RAIN)[id];
     mpp<S>(M, MP_P_SNOWN)[id] = mpp<S>(M, MP_P_SNOW)[id];
+}
+
+// With the ice scheme in the moist block, after its flux damping (cAtmosphereModel.cpp:1263-1399):
+// precipitationSum (UtilsAtm.h:52-67, graupel and the moist convection's P_conv zero), cap_S
+// of the sources and of S_c_c, the microphysics clamp and the water-vapour ceiling.
+template <typename S>
+__global__ __launch_bounds__(256) void k_moist_clamp_ice(const MemberDesc* __restrict__ md, int gate) {
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= K.n) return;
+    using C = double;
+    mpp<S>(M, MP_PRECIP)[id] = S(C(mpp<S>(M, MP_P_RAIN)[id]) + C(mpp<S>(M, MP_P_SNOW)[id]) + 0.0 + 0.0);
+    auto cap = [](C v) { return !finite_bits(v) ? C(0) : (v > C(1.0e-3)) ? C(1.0e-3) : (v < C(-1.0e-3)) ? C(-1.0e-3) : v; };
+    S* scc = fp<S>(M, F_S_C_C);
+    scc[id] = S(cap(C(scc[id])));
+    for (int q = MP_S_V; q <= MP_S_S; q++) mpp<S>(M, q)[id] = S(cap(C(mpp<S>(M, q)[id])));
+    S *pc = fp<S>(M, F_C), *pcl = fp<S>(M, F_CLOUD), *pi = fp<S>(M, F_ICE);
+    const C co2 = C(fp<S>(M, F_CO2)[id]);
+    C cv = C(pc[id]), cld = C(pcl[id]), ic = C(pi[id]);
+    if (!finite_bits(cv)) cv = C(0);
+    if (!finite_bits(cld)) cld = C(0);
+    if (!finite_bits(ic)) ic = C(0);
+    if (cv < C(0)) cv = C(0);
+    if (cld < C(0)) cld = C(0);
+    if (ic < C(0)) ic = C(0);
+    const C cv_max = tmax(C(0), C(1) - co2);
+    if (cv > cv_max) cv = cv_max;
+    const C sum = cld + ic;
+    if (sum > C(K.cloud_cap)) { const C sc = C(K.cloud_cap) / sum; cld *= sc; ic *= sc; }
+    if (cv < C(0)) cv = C(0);
+    if (cv > cv_max) cv = cv_max;
+    pc[id] = S(cv); pcl[id] = S(cld); pi[id] = S(ic);
+}
+
+// Boundary faces of the microphysics sources S_c_c, S_v, S_c, S_i, S_r, S_s where the ice
+// scheme has run: bcRadius pattern A (cubic at both ends, BC_Atm.h:567-644), then bcTheta's
+// copy (:761-790), then bcPhi's seam average (:805-842), in the closed form of the three
+// sequential passes (c43 = 1, c13 = 0; the sources are finite, capped on every write). Reads
+// only cells with every index interior, writes only boundary cells.
+template <typename S>
+__global__ __launch_bounds__(256) void k_bc_sources(const MemberDesc* __restrict__ md) {
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= K.n || !M.mp[0]) return;
+    int i, j, k;
+    decode(K, id, i, j, k);
+    const int iml = K.im - 1;
+    const bool seam = (k == 0 || k == K.km - 1);
+    if (!seam && i > 0 && i < iml && j > 0 && j < K.jm - 1) return;
+    const int jj = min(max(j, 1), K.jm - 2);
+    for (int q = 0; q < 6; q++) {
+        S* x = (q == 0) ? fp<S>(M, F_S_C_C) : mpp<S>(M, MP_S_V + q - 1);
+        auto R = [&](int kk) -> double {   // the value after bcRadius at (i, jj, kk), kk interior
+            if (i == 0)
+                return double(x[ix3(K, 3, jj, kk)]) - 3.0 * double(x[ix3(K, 2, jj, kk)]) + 3.0 * double(x[ix3(K, 1, jj, kk)]);
+            if (i == iml)
+                return double(x[ix3(K, iml - 3, jj, kk)]) - 3.0 * double(x[ix3(K, iml - 2, jj, kk)])
+                     + 3.0 * double(x[ix3(K, iml - 1, jj, kk)]);
+            return double(x[ix3(K, i, jj, kk)]);
+        };
+        x[id] = seam ? S((double(S(R(1))) + double(S(R(K.km - 2)))) * 0.5) : S(R(k));
+    }
 }
 
 enum { FT = 0, FU, FV, FW, FC, FCL, FI, FG, FCO, FK, FD };
@@ -1063,7 +1130,7 @@ constexpr int LB = 256;   // LDS slot stride = threads per block of the RK4 kern
 // tendencies involve no other prognostic field, so the group integrates first and records
 // the velocity entering each stage in rec[] for the scalar groups. cen[] = start values in,
 // final values out. A, B: velocity-gradient invariants of the production term.
-template <typename S, typename C, bool CLAMP>
+template <typename S, typename C, bool CLAMP, bool SRC>
 ATHAD_DEV void rk4_dyn(const MemberDesc& M, const S* const* X, long long id, int i, const CellGeo<C>& g,
                        C* ba, C* rec, C* cen, C* aux, C& A, C& B) {
     const Consts& K = M.K;
@@ -1117,14 +1184,24 @@ ATHAD_DEV void rk4_dyn(const MemberDesc& M, const S* const* X, long long id, int
     const C t_ref = C(M.t_ref[i]);
     const C buoy = buoyancy<C>(M);
     const C ce = C(K.coeff_energy_p);
+    // latent heating of the microphysics sources (RHS_Atm_Turb.cpp:965-966), stage-invariant
+    C srcT = C(0);
+    if (SRC) {
+        const C rh = C(gld(fp<S>(M, F_R_HUMID), id));
+        const C Sc = C(gld((const S*)M.mp[MP_S_C], id)), Sr = C(gld((const S*)M.mp[MP_S_R], id));
+        const C Si = C(gld((const S*)M.mp[MP_S_I], id)), Ss = C(gld((const S*)M.mp[MP_S_S], id));
+        const C cen_ = C(K.coeff_energy);
+        srcT = cen_ * (Sc + Sr) * C(K.lv) * rh + cen_ * (Si + Ss + C(0)) * C(K.ls) * rh;
+    }
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
 #pragma unroll 1
     for (int st = 0; st < 4; st++) {
         const C T = cen[FT], u = cen[FU], v = cen[FV], w = cen[FW];
         rec[(3 * st + 0) * LB] = u; rec[(3 * st + 1) * LB] = v; rec[(3 * st + 2) * LB] = w;
         const C ua = u * g.aR, va = v * g.aT, wa = w * g.aP;
-        const C rT = ce * (u * PR + v * PT + w * PP) - minmod3(nb[FT], T, ua, va, wa) + (L0T - g.Lc2 * T) * dif_t
-                   - g.hs_kT * (T - t_eq);
+        C rT = ce * (u * PR + v * PT + w * PP) - minmod3(nb[FT], T, ua, va, wa) + (L0T - g.Lc2 * T) * dif_t;
+        if (SRC) rT = rT + srcT;
+        rT = rT - g.hs_kT * (T - t_eq);
         const C rU = -PR - minmod3(nb[FU], u, ua, va, wa) + (D0u + g.cu * u) * dif_vel + buoy * (T - t_ref);
         const C rV = -PT - minmod3(nb[FV], v, ua, va, wa) + (D0v + g.cvw * v) * dif_vel + g.cor * w - g.drag * v;
         const C rW = -PP - minmod3(nb[FW], w, ua, va, wa) + (D0w + g.cvw * w) * dif_vel - g.cor * v - g.drag * w;
@@ -1138,7 +1215,7 @@ ATHAD_DEV void rk4_dyn(const MemberDesc& M, const S* const* X, long long id, int
 
 // RK4 of the moisture group c, cloud, ice (minmod advection), gr, co2 (central), advected by
 // the recorded stage velocities; cen[0..4] = start values in, final values out.
-template <typename S, typename C, bool CLAMP, typename Vel>
+template <typename S, typename C, bool CLAMP, bool SRC, typename Vel>
 ATHAD_DEV void rk4_moist(const MemberDesc& M, const S* const* X, long long id, const CellGeo<C>& g, C* ba,
                          Vel vel, C* cen) {
     const Consts& K = M.K;
@@ -1161,6 +1238,14 @@ ATHAD_DEV void rk4_moist(const MemberDesc& M, const S* const* X, long long id, c
         }
     }
     const C dpr = C(K.diff_prec_re_inv), dco = C(K.diff_co2_re_inv);
+    // microphysics sources of c, cloud, ice (RHS_Atm_Turb.cpp:1058-1067), stage-invariant
+    C src[3] = {C(0), C(0), C(0)};
+    if (SRC) {
+        const C rh = C(gld(fp<S>(M, F_R_HUMID), id));
+        src[0] = C(1) * C(gld((const S*)M.mp[MP_S_V], id)) * rh;
+        src[1] = C(1) * C(gld((const S*)M.mp[MP_S_C], id)) * rh;
+        src[2] = C(1) * C(gld((const S*)M.mp[MP_S_I], id)) * rh;
+    }
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
 #pragma unroll 1
     for (int st = 0; st < 4; st++) {
@@ -1169,7 +1254,10 @@ ATHAD_DEV void rk4_moist(const MemberDesc& M, const S* const* X, long long id, c
         const C ua = u * g.aR, va = v * g.aT, wa = w * g.aP;
         C r[5];
 #pragma unroll
-        for (int q = 0; q < 3; q++) r[q] = -minmod3(nb[q], cen[q], ua, va, wa) + (L0[q] - g.Lc2 * cen[q]) * dpr;
+        for (int q = 0; q < 3; q++) {
+            r[q] = -minmod3(nb[q], cen[q], ua, va, wa) + (L0[q] - g.Lc2 * cen[q]) * dpr;
+            if (SRC) r[q] = r[q] + src[q];
+        }
         r[3] = -(u * gc[0][0] + v * gc[0][1] + w * gc[0][2]) + (L0[3] - g.Lc2 * cen[3]) * dpr;
         r[4] = -(u * gc[1][0] + v * gc[1][1] + w * gc[1][2]) + (L0[4] - g.Lc2 * cen[4]) * dco;
 #pragma unroll
@@ -1287,7 +1375,7 @@ ATHAD_DEV void group_start(const MemberDesc& M, const S* const* X, long long id,
 
 // RK4 of one interior cell, group by group: dynamics, moisture, turbulence. out(f, value)
 // writes a final value; the aux / prod / source outputs are written here.
-template <typename S, typename C, bool CLAMP, typename Out>
+template <typename S, typename C, bool CLAMP, bool SRC, typename Out>
 ATHAD_DEV void rk4_cell(const MemberDesc& M, const S* const* X, long long id, int i, int j, int k, C* ba, C* rec,
                         Out out) {
     CellGeo<C> g;
@@ -1296,7 +1384,7 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const S* const* X, long long id, in
     {
         C cen[4], aux[3];
         group_start<S, C, CLAMP>(M, X, id, FT, 4, cen, ba);
-        rk4_dyn<S, C, CLAMP>(M, X, id, i, g, ba, rec, cen, aux, A, B);
+        rk4_dyn<S, C, CLAMP, SRC>(M, X, id, i, g, ba, rec, cen, aux, A, B);
 #pragma unroll
         for (int q = 0; q < 4; q++) out(FT + q, cen[q]);
         gst(fp<S>(M, F_AUX_U), id, S(aux[0]));
@@ -1309,7 +1397,7 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const S* const* X, long long id, in
     {
         C cen[5];
         group_start<S, C, CLAMP>(M, X, id, FC, 5, cen, ba);
-        rk4_moist<S, C, CLAMP>(M, X, id, g, ba, vel, cen);
+        rk4_moist<S, C, CLAMP, SRC>(M, X, id, g, ba, vel, cen);
 #pragma unroll
         for (int q = 0; q < 5; q++) out(FC + q, cen[q]);
     }
@@ -1329,7 +1417,7 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const S* const* X, long long id, in
 // start-of-RK4 neighbourhood read from f[] and write alt[]; boundary cells are copied. The
 // solid-ground clamps are applied on read (FUSED=1), and the store of the eight fields that
 // no later filter touches is fused (xn = new value).
-template <typename S, bool FUSED>
+template <typename S, bool FUSED, bool SRC>
 __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>()))) void k_rk4_jacobi(const MemberDesc* __restrict__ md) {
     using C = S;
     __shared__ C s_ba[10 * LB], s_rec[12 * LB];
@@ -1368,7 +1456,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
         }
         return;
     }
-    rk4_cell<S, C, FUSED>(M, X, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x, [&](int f, C v) {
+    rk4_cell<S, C, FUSED, SRC>(M, X, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x, [&](int f, C v) {
         gst((S*)M.alt[f], id, S(v));
         if (FUSED && f != FU && f != FV && f != FW) gst(fp<S>(M, kProgN[f]), id, S(v));
     });
@@ -1377,7 +1465,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
 // Lexicographic-order RK4 restricted to the hyperplane i+j+k = d (in place). Cells of one
 // plane only read neighbours on planes d-1 (final) and d+1 (not yet updated), which is
 // exactly what the reference's single-threaded sweep sees.
-template <typename S>
+template <typename S, bool SRC>
 __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>()))) void k_rk4_lex_plane(const MemberDesc* __restrict__ md, int d) {
     using C = S;
     __shared__ C s_ba[10 * LB], s_rec[12 * LB];
@@ -1393,7 +1481,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
     S* X[NPROG];
 #pragma unroll
     for (int f = 0; f < NPROG; f++) X[f] = fp<S>(M, kProgF[f]);
-    rk4_cell<S, C, false>(M, (const S* const*)X, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
+    rk4_cell<S, C, false, SRC>(M, (const S* const*)X, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
                           [&](int f, C v) { gst(X[f], id, S(v)); });
 }
 
@@ -1742,6 +1830,18 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
                     }
                 }
             }
+            if (M.mp[0]) {
+                // falling rain and snow evaporate too, and the sources clear
+                // (IceSchemeCommon.h:175-189); arrays held once the ice scheme has run
+                S* Pr = (S*)M.mp[MP_P_RAIN];
+                S* Ps = (S*)M.mp[MP_P_SNOW];
+                const double rho = tmax(1.0e-6, (double)fp<S>(M, F_R_HUMID)[id]);
+                const double q_p = tmax(0.0, (double)Pr[id]) / (1.6 * rho) + tmax(0.0, (double)Ps[id]) / (0.96 * rho)
+                                 + 0.0 / (2.0 * rho);
+                if (q_p > 0.0) c = tmin(tmax(0.0, 1.0 - co2), c + q_p);
+                for (int q = MP_P_RAIN; q < NMP; q++)
+                    if (q != MP_P_RAINN && q != MP_P_SNOWN) ((S*)M.mp[q])[id] = S(0);
+            }
             scc = 0.0;
         } else {
             const double qs = q_s;
@@ -1760,10 +1860,14 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
             c *= al; cloud *= al; ice *= al;
         }
     }
-    if (!finite_bits(scc)) scc = 0.0;
-    else if (scc > 1.0e-3) scc = 1.0e-3;
-    else if (scc < -1.0e-3) scc = -1.0e-3;
-    if (to_tmp) {   // scratch 6..9 = t, ice, c, cloud for the fused damping that follows
+    // cap_S of S_c_c (cAtmosphereModel.cpp:1290-1305); with the ice scheme in the moist block
+    // (bit 1 of to_tmp) the cap comes after it, since the scheme reads S_c_c uncapped
+    if (!(to_tmp & 2)) {
+        if (!finite_bits(scc)) scc = 0.0;
+        else if (scc > 1.0e-3) scc = 1.0e-3;
+        else if (scc < -1.0e-3) scc = -1.0e-3;
+    }
+    if (to_tmp & 1) {   // scratch 6..9 = t, ice, c, cloud for the fused damping that follows
         ((S*)M.tmp[6])[id] = S(t); ((S*)M.tmp[7])[id] = S(ice); ((S*)M.tmp[8])[id] = S(c); ((S*)M.tmp[9])[id] = S(cloud);
     } else {
         pt[id] = S(t); pc[id] = S(c); pcl[id] = S(cloud); pi[id] = S(ice);
@@ -2848,23 +2952,35 @@ void launch_radiation(const Batch& b) {
 // 80 levels keep the tile's LDS (6 fields x levels x 16 columns) within 64 KB.
 int radiation_max_levels() { return 80; }
 
-void launch_ice_scheme(const Batch& b) {
+void launch_ice_scheme(const Batch& b, int gate) {
     const dim3 gc((unsigned)((b.max_jm * b.max_km + 255) / 256), 1, b.nmem);
     const dim3 gb((unsigned)((b.max_n + 255) / 256), 1, b.nmem);
     if (b.fp32) {
-        hipLaunchKernelGGL(k_ice_columns<float>, gc, dim3(256), 0, b.stream, b.d_md);
-        hipLaunchKernelGGL(k_ice_bc<float>, gb, dim3(256), 0, b.stream, b.d_md);
+        hipLaunchKernelGGL(k_ice_columns<float>, gc, dim3(256), 0, b.stream, b.d_md, gate);
+        hipLaunchKernelGGL(k_ice_bc<float>, gb, dim3(256), 0, b.stream, b.d_md, gate);
     } else {
-        hipLaunchKernelGGL(k_ice_columns<double>, gc, dim3(256), 0, b.stream, b.d_md);
-        hipLaunchKernelGGL(k_ice_bc<double>, gb, dim3(256), 0, b.stream, b.d_md);
+        hipLaunchKernelGGL(k_ice_columns<double>, gc, dim3(256), 0, b.stream, b.d_md, gate);
+        hipLaunchKernelGGL(k_ice_bc<double>, gb, dim3(256), 0, b.stream, b.d_md, gate);
     }
     // run_3D_loop's damp_wiggles of the two fluxes (cAtmosphereModel.cpp:1243-1244) through
-    // scratch buffers 0-3
+    // scratch buffers 6-9 (0-5 may hold the turbulence update running beside the moist block)
     const int src[2] = {NFIELD + NTMP + MP_P_RAIN, NFIELD + NTMP + MP_P_SNOW};
-    const int t0[2] = {NFIELD + 0, NFIELD + 1}, t1[2] = {NFIELD + 2, NFIELD + 3};
-    launch_damp(b, 0, src, t0, 2, 0, 0);
-    launch_damp(b, 1, t0, t1, 2, 0, 0);
-    launch_damp(b, 2, t1, src, 2, 0, 0);
+    const int t0[2] = {NFIELD + 6, NFIELD + 7}, t1[2] = {NFIELD + 8, NFIELD + 9};
+    launch_damp(b, 0, src, t0, 2, 0, gate);
+    launch_damp(b, 1, t0, t1, 2, 0, gate);
+    launch_damp(b, 2, t1, src, 2, 0, gate);
+}
+
+void launch_moist_clamp_ice(const Batch& b, int gate) {
+    const dim3 g((unsigned)((b.max_n + 255) / 256), 1, b.nmem);
+    if (b.fp32) hipLaunchKernelGGL(k_moist_clamp_ice<float>, g, dim3(256), 0, b.stream, b.d_md, gate);
+    else hipLaunchKernelGGL(k_moist_clamp_ice<double>, g, dim3(256), 0, b.stream, b.d_md, gate);
+}
+
+void launch_bc_sources(const Batch& b) {
+    const dim3 g((unsigned)((b.max_n + 255) / 256), 1, b.nmem);
+    if (b.fp32) hipLaunchKernelGGL(k_bc_sources<float>, g, dim3(256), 0, b.stream, b.d_md);
+    else hipLaunchKernelGGL(k_bc_sources<double>, g, dim3(256), 0, b.stream, b.d_md);
 }
 
 void launch_ice_store(const Batch& b) {
@@ -2888,20 +3004,32 @@ void launch_level_mean(const Batch& b) {
     ATHAD_DISPATCH(k_level_mean, dim3(kLevelChunks, b.max_im, b.nmem), dim3(256), 0, b.d_md);
     hipLaunchKernelGGL(k_level_mean_final, dim3((b.max_im + 63) / 64, 1, b.nmem), dim3(64), 0, b.stream, b.d_md);
 }
-void launch_rk4_jacobi(const Batch& b, bool fused) {
+namespace {
+template <typename S, bool SRC> void rk4_jacobi_t(const Batch& b, const dim3& g, bool fused) {
+    if (fused) hipLaunchKernelGGL((k_rk4_jacobi<S, true, SRC>), g, dim3(256), 0, b.stream, b.d_md);
+    else hipLaunchKernelGGL((k_rk4_jacobi<S, false, SRC>), g, dim3(256), 0, b.stream, b.d_md);
+}
+}  // namespace
+void launch_rk4_jacobi(const Batch& b, bool fused, bool src) {
     const dim3 g(cells_grid(b, 256), 1, b.nmem);
-
     if (b.fp32) {
-        if (fused) hipLaunchKernelGGL((k_rk4_jacobi<float, true>), g, dim3(256), 0, b.stream, b.d_md);
-        else hipLaunchKernelGGL((k_rk4_jacobi<float, false>), g, dim3(256), 0, b.stream, b.d_md);
+        if (src) rk4_jacobi_t<float, true>(b, g, fused);
+        else rk4_jacobi_t<float, false>(b, g, fused);
     } else {
-        if (fused) hipLaunchKernelGGL((k_rk4_jacobi<double, true>), g, dim3(256), 0, b.stream, b.d_md);
-        else hipLaunchKernelGGL((k_rk4_jacobi<double, false>), g, dim3(256), 0, b.stream, b.d_md);
+        if (src) rk4_jacobi_t<double, true>(b, g, fused);
+        else rk4_jacobi_t<double, false>(b, g, fused);
     }
 }
-void launch_rk4_lex(const Batch& b, int plane) {
+void launch_rk4_lex(const Batch& b, int plane, bool src) {
     const long long nq = (long long)(b.max_im - 2) * (b.max_jm - 2);
-    ATHAD_DISPATCH(k_rk4_lex_plane, dim3((unsigned)((nq + 255) / 256), 1, b.nmem), dim3(256), 0, b.d_md, plane);
+    const dim3 g((unsigned)((nq + 255) / 256), 1, b.nmem);
+    if (b.fp32) {
+        if (src) hipLaunchKernelGGL((k_rk4_lex_plane<float, true>), g, dim3(256), 0, b.stream, b.d_md, plane);
+        else hipLaunchKernelGGL((k_rk4_lex_plane<float, false>), g, dim3(256), 0, b.stream, b.d_md, plane);
+    } else {
+        if (src) hipLaunchKernelGGL((k_rk4_lex_plane<double, true>), g, dim3(256), 0, b.stream, b.d_md, plane);
+        else hipLaunchKernelGGL((k_rk4_lex_plane<double, false>), g, dim3(256), 0, b.stream, b.d_md, plane);
+    }
 }
 void launch_poisson(const Batch& b, int gate) {
     const unsigned na = (unsigned)(((long long)(b.max_jm - 2) * (b.max_km - 2) + 255) / 256);

@@ -199,6 +199,10 @@ class Integration:
         """TwoCatIceScheme::run and the damping of P_rain, P_snow, every member."""
         self._chk(L.lib().athad_ice_scheme(self.h))
 
+    def set_ice_scheme(self, on: bool = True):
+        """CategoryIceScheme = 2 inside every later step's moist block."""
+        self._chk(L.lib().athad_set_ice_scheme(self.h, 1 if on else 0))
+
     def get_mp(self, name: str, member: int = 0) -> np.ndarray:
         """One ice-scheme array (MP_FIELDS) as FP64."""
         out = np.empty(self.grids[member])

@@ -272,6 +272,16 @@ enum athad_mp_field {
  * (IceSchemeCommon.h:104-190) t c cloud ice gr S_c_c. athad_store and every iteration's store
  * then copy P_rain, P_snow to P_rainn, P_snown (UtilsAtm.h:41-42). */
 athad_status athad_ice_scheme(athad_ctx* ctx);
+/* CategoryIceScheme = 2 inside the step (on != 0): every later moist iteration of athad_step /
+ * athad_run runs the ice-scheme call site in its place in the moist block, after the damping of
+ * t ice c cloud, then precipitationSum (:1263; graupel and the moist convection's P_conv are
+ * zero, MoistConvection is not part of this library) and cap_S of the sources (:1278-1305).
+ * Independently of this switch, once the ice-scheme arrays exist the RK4 tendencies of t, c,
+ * cloud, ice carry their sources (atmosphere/RHS_Atm_Turb.cpp:963-966, 1058-1067), the
+ * boundaries extrapolate them (atmosphere/BC_Atm.h:567-842), and saturation adjustment
+ * evaporates falling precipitation where no condensed phase can exist. on = 0 restores the
+ * scoped moist block. */
+athad_status athad_set_ice_scheme(athad_ctx* ctx, int on);
 /* Contiguous FP64 host arrays [im][jm][km] of one ice-scheme array (athad_mp_field). */
 athad_status athad_set_mp_field(athad_ctx* ctx, int member, int which, const double* src);
 athad_status athad_get_mp_field(athad_ctx* ctx, int member, int which, double* dst);

@@ -203,6 +203,7 @@ struct orc_state {
     // precipitation fluxes, their stored copies and the microphysics sources of the two-category
     // ice scheme (ORC_MP_* order); allocated, zero, at first use
     std::vector<std::vector<double>> mp;
+    bool ice_on = false;               // the ice scheme runs inside the moist block
     int iter_n = 0, total_iter = 0;
 
     inline size_t ix(int i, int j, int k) const { return ((size_t)i * jm + j) * km + k; }
@@ -366,8 +367,8 @@ void evaporation(S& s) {
 // ---------------------------------------------------------------------------------------
 // SaturationAdjustment: SaturationAdjustment.h:56-287 (adjustSaturation), :289-301
 // (applyTopography, identity without land), :303-451 (clampAndFade) and
-// IceSchemeCommon.h:121-190 (evaporateWhereImpossible; precipitation fluxes are zero in the
-// scoped step).
+// IceSchemeCommon.h:121-190 (evaporateWhereImpossible; the precipitation fluxes and sources
+// are zero in the scoped step and exist once the ice scheme has run).
 // ---------------------------------------------------------------------------------------
 struct SatConsts { double t_0, t_00, M_bg, T_max, cloud_cap; };
 
@@ -422,8 +423,10 @@ void satadj_cell(const SatConsts& K, double& t, double& c, double& cloud, double
     if (T < K.t_00) { cloud = 0.0; ice = 0.0; }
 }
 
+// mp: the ice-scheme arrays where they exist (else null) with the cell's r_humid and index.
 void clamp_fade_cell(const SatConsts& K, double& t, double& c, double& cloud, double& ice,
-                     double& gr, double co2, double p_local, double& s_c_c) {
+                     double& gr, double co2, double p_local, double& s_c_c,
+                     std::vector<std::vector<double>>* mp = nullptr, double r_humid = 0.0, size_t id = 0) {
     const double inv_t_0 = 1.0 / K.t_0;
     if (c < 0.0) c = 0.0;
     if (cloud < 0.0) cloud = 0.0;
@@ -451,6 +454,19 @@ void clamp_fade_cell(const SatConsts& K, double& t, double& c, double& cloud, do
                         t = T_new / K.t_0;
                     }
                 }
+            }
+            if (mp) {   // falling precipitation evaporates, the sources clear (:175-189)
+                std::vector<std::vector<double>>& A = *mp;
+                const double rho = dmax(1.0e-6, r_humid);
+                const double q_p = dmax(0.0, A[ORC_MP_P_RAIN][id]) / (1.6 * rho)
+                                 + dmax(0.0, A[ORC_MP_P_SNOW][id]) / (0.96 * rho) + 0.0 / (2.0 * rho);
+                if (q_p > 0.0) {
+                    const double cm = dmax(0.0, 1.0 - co2);
+                    c = dmin(cm, c + q_p);
+                }
+                for (int q : {ORC_MP_P_RAIN, ORC_MP_P_SNOW, ORC_MP_PRECIP, ORC_MP_S_V, ORC_MP_S_C, ORC_MP_S_I,
+                              ORC_MP_S_R, ORC_MP_S_S})
+                    A[q][id] = 0.0;
             }
             s_c_c = 0.0;
             return;
@@ -501,7 +517,8 @@ void satadj(S& s) {
         for (int j = 0; j < s.jm; j++)
             for (int k = 0; k < s.km; k++) {
                 const size_t id = s.ix(i, j, k);
-                clamp_fade_cell(K, t[id], c[id], cl[id], ice[id], gr[id], co2[id], ps[id], scc[id]);
+                clamp_fade_cell(K, t[id], c[id], cl[id], ice[id], gr[id], co2[id], ps[id], scc[id],
+                                s.mp.empty() ? nullptr : &s.mp, s.F(ORC_R_HUMID)[id], id);
             }
 }
 
@@ -528,6 +545,12 @@ void moist_clamp(S& s) {
         const double sum = cld + ic;
         if (sum > cap) { const double sc = cap / sum; cld *= sc; ic *= sc; }
         c[id] = cv; cl[id] = cld; ice[id] = ic;
+        // cap_S of the ice scheme's sources, where it has run
+        if (!s.mp.empty())
+            for (int q : {ORC_MP_S_V, ORC_MP_S_C, ORC_MP_S_I, ORC_MP_S_R, ORC_MP_S_S}) {
+                const double x = s.mp[q][id];
+                s.mp[q][id] = !finite_bits(x) ? 0.0 : (x > 1.0e-3) ? 1.0e-3 : (x < -1.0e-3) ? -1.0e-3 : x;
+            }
         // :1369-1390 (ceiling, every moist iteration)
         if (c[id] < 0.0) c[id] = 0.0;
         const double cc = dmax(0.0, 1.0 - co2[id]);
@@ -860,6 +883,32 @@ void bc_all(S& s) {
         s.f[ORC_DIS_SRC][id] = sc(s.f[ORC_DIS_SRC][id], -1.0e4, 1.0e4);
         for (int fld : {ORC_U, ORC_UN, ORC_V, ORC_VN, ORC_W, ORC_WN}) s.f[fld][id] = scs(s.f[fld][id], vmax);
     }
+    // Microphysics sources S_c_c, S_v, S_c, S_i, S_r, S_s where the ice scheme has run:
+    // bcRadius pattern A (cubic at both ends, BC_Atm.h:567-644), bcTheta copy (:761-790), bcPhi
+    // seam average (:805-842). Their boundary faces are read by no operator of the step.
+    if (!s.mp.empty()) {
+        double* srcs[6] = {s.F(ORC_S_C_C), s.mp[ORC_MP_S_V].data(), s.mp[ORC_MP_S_C].data(),
+                           s.mp[ORC_MP_S_I].data(), s.mp[ORC_MP_S_R].data(), s.mp[ORC_MP_S_S].data()};
+        const double c43 = 1.0, c13 = 0.0;
+        for (double* x : srcs) {
+            for (int j = 0; j < jm; j++)
+                for (int k = 0; k < km; k++) {
+                    x[s.ix(0, j, k)] = x[s.ix(3, j, k)] - 3.0 * x[s.ix(2, j, k)] + 3.0 * x[s.ix(1, j, k)];
+                    x[s.ix(iml, j, k)] = x[s.ix(iml - 3, j, k)] - 3.0 * x[s.ix(iml - 2, j, k)] + 3.0 * x[s.ix(iml - 1, j, k)];
+                }
+            for (int k = 0; k < km; k++)
+                for (int i = 0; i < im; i++) {
+                    x[s.ix(i, 0, k)] = x[s.ix(i, 1, k)];
+                    x[s.ix(i, jm - 1, k)] = x[s.ix(i, jm - 2, k)];
+                }
+            for (int i = 0; i < im; i++)
+                for (int j = 0; j < jm; j++) {
+                    const double v0 = c43 * x[s.ix(i, j, 1)] - c13 * x[s.ix(i, j, 2)];
+                    const double ve = c43 * x[s.ix(i, j, km - 2)] - c13 * x[s.ix(i, j, km - 3)];
+                    x[s.ix(i, j, 0)] = x[s.ix(i, j, km - 1)] = (v0 + ve) * 0.5;
+                }
+        }
+    }
     (void)t;
 }
 
@@ -1024,6 +1073,19 @@ void rhs_cell(const S& s, const Geo& g, const RhsConsts& K, int i, int j, int k,
 
     const double dpdr_exp = dpdr * g.exp_rm, dpdthe_invrm = dpdthe * g.inv_rm, dpdphi_invrs = dpdphi * g.inv_rmsinthe;
     double rt = pressure_t - tr[FT] + diffusion_t;
+    // Microphysics sources (RHS_Atm_Turb.cpp:963-966, 1058-1070); zero until the ice scheme
+    // has run, so they are added only where its arrays exist. MC_* (moist convection) are zero.
+    double src_c = 0.0, src_cl = 0.0, src_i = 0.0;
+    if (!s.mp.empty()) {
+        const double rh = s.f[ORC_R_HUMID][id];
+        const double Sv = s.mp[ORC_MP_S_V][id], Sc = s.mp[ORC_MP_S_C][id], Si = s.mp[ORC_MP_S_I][id];
+        const double Sr = s.mp[ORC_MP_S_R][id], Ss = s.mp[ORC_MP_S_S][id];
+        const double coeff_energy = 1.0 / (s.P.c_0 * 1349.0 * s.P.t_0);   // cp_l = 1349
+        rt = rt + coeff_energy * (Sc + Sr) * 2520000.0 * rh + coeff_energy * (Si + Ss + 0.0) * 2830000.0 * rh;
+        src_c = 1.0 * Sv * rh;
+        src_cl = 1.0 * Sc * rh;
+        src_i = 1.0 * Si * rh;
+    }
     {   // Held-Suarez :976-992
         const double k_a = 1.0 / (4.0 * 86400.0), k_s = 1.0 / (0.4 * 86400.0), sigma_b = 0.7;
         const double sigma = (p_surf > 0.0) ? p_stat_c / p_surf : 0.0;
@@ -1037,9 +1099,9 @@ void rhs_cell(const S& s, const Geo& g, const RhsConsts& K, int i, int j, int k,
     rhs[FU] = -dpdr_exp - tr[FU] + diffusion_u + buoy_ramp_dt * (cen[FT] - t_ref_i);
     rhs[FV] = -dpdthe_invrm - tr[FV] + diffusion_v + K.force_nd * coriolis_the - surf_drag * v;
     rhs[FW] = -dpdphi_invrs - tr[FW] + diffusion_w + K.force_nd * coriolis_phi - surf_drag * w;
-    rhs[FC] = -tr[FC] + diffusion_c;
-    rhs[FCL] = -tr[FCL] + diffusion_cl;
-    rhs[FI] = -tr[FI] + diffusion_i;
+    rhs[FC] = -tr[FC] + diffusion_c + src_c;
+    rhs[FCL] = -tr[FCL] + diffusion_cl + src_cl;
+    rhs[FI] = -tr[FI] + diffusion_i + src_i;
     rhs[FG] = -tr[FG] + diffusion_g;
     rhs[FCO] = -tr[FCO] + diffusion_co;
     rhs[FK] = -tr[FK] + diffusion_k + tsrc;
@@ -1220,12 +1282,22 @@ void store(S& s) {
     }
 }
 
+void ice_scheme(S& s);
+
 void moist_block(S& s) {   // cAtmosphereModel.cpp:1185-1434 (scoped rows)
     satadj(s);
     damp_wiggles(s, s.F(ORC_T));
     damp_wiggles(s, s.F(ORC_ICE));
     damp_wiggles(s, s.F(ORC_C));
     damp_wiggles(s, s.F(ORC_CLOUD));
+    if (s.ice_on) {
+        // :1230-1244 with CategoryIceScheme = 2, then precipitationSum (:1263, UtilsAtm.h:52-67)
+        // with P_graupel and the moist convection's P_conv zero
+        ice_scheme(s);
+        const double *pr = s.mp[ORC_MP_P_RAIN].data(), *ps = s.mp[ORC_MP_P_SNOW].data();
+        double* pt = s.mp[ORC_MP_PRECIP].data();
+        for (size_t q = 0; q < s.n; q++) pt[q] = pr[q] + ps[q] + 0.0 + 0.0;
+    }
     moist_clamp(s);
     densities(s, false);
     evaporation(s);
@@ -1791,6 +1863,10 @@ double orc_scalar(const orc_state* s, const char* name) {
     return NAN;
 }
 void orc_ice_scheme(orc_state* s) { ice_scheme(*s); }
+void orc_set_ice_scheme(orc_state* s, int on) {
+    s->ice_on = on != 0;
+    if (s->ice_on) ensure_mp(*s);
+}
 void orc_ice_store(orc_state* s) { ice_store(*s); }
 double* orc_mp_ptr(orc_state* s, int which) {
     if (which < 0 || which >= ORC_NMP) return nullptr;

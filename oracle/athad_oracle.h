@@ -98,6 +98,11 @@ enum orc_mp {
     ORC_NMP
 };
 void    orc_ice_scheme(orc_state* s);
+// Puts the ice scheme in its place inside the moist block of every later step (after the
+// damping of t ice c cloud, cAtmosphereModel.cpp:1230-1244, with precipitationSum :1263 and
+// the cap of its sources :1278-1305). Wherever its arrays exist the RHS adds their sources
+// (RHS_Atm_Turb.cpp:963-966, 1058-1070).
+void    orc_set_ice_scheme(orc_state* s, int on);
 // P_rainn <- P_rain, P_snown <- P_snow (storeIntermediateData3D, UtilsAtm.h:41-42).
 void    orc_ice_store(orc_state* s);
 // The arrays above (im*jm*km doubles, [i][j][k]); allocated, zero, at first use.

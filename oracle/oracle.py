@@ -87,6 +87,7 @@ def lib():
         L.orc_rad_ptr.argtypes = [vp, i]
         L.orc_ice_scheme.argtypes = [vp]
         L.orc_ice_store.argtypes = [vp]
+        L.orc_set_ice_scheme.argtypes = [vp, i]
         L.orc_mp_ptr.restype = dp
         L.orc_mp_ptr.argtypes = [vp, i]
         _lib = L
@@ -180,6 +181,10 @@ class OracleState:
     def ice_scheme(self):
         """The two-category ice scheme and the damping of its two precipitation fluxes."""
         lib().orc_ice_scheme(self.h)
+
+    def set_ice_scheme(self, on=True):
+        """The ice scheme inside every later step's moist block (CategoryIceScheme = 2)."""
+        lib().orc_set_ice_scheme(self.h, 1 if on else 0)
 
     def ice_store(self):
         """P_rainn <- P_rain, P_snown <- P_snow, as the store keeps them."""

@@ -17,6 +17,10 @@
 //                           and P_snow (cAtmosphereModel.cpp:1230-1244); reads and writes the
 //                           ice-scheme arrays in <in.bin>.mp / <out.bin>.mp: P_rain, P_snow,
 //                           P_rainn, P_snown, Precipitation, S_v, S_c, S_i, S_r, S_s
+//   mode = steps_ice        n scoped iterations with the ice scheme in its place in the moist
+//                           block (after the damping of t ice c cloud), its flux damping and
+//                           precipitationSum (moist convection not run: P_conv stays zero);
+//                           the .mp files as for mode ice
 // The radiation modes also write <out.bin>.rad: radiation and epsilon (im*jm*km doubles each),
 // albedo (jm*km), the top short wave (jm) and t_skin.
 // State file: int32 im,jm,km,iter_n,total_iter; then 35 fields of im*jm*km doubles
@@ -204,7 +208,7 @@ public:
     }
 
     // One iteration of run_3D_loop restricted to the scoped rows (cAtmosphereModel.cpp:1103-1647).
-    static void step(cAtmosphereModel& m) {
+    static void step(cAtmosphereModel& m, bool ice_scheme = false) {
         m.iter_n++;
         m.total_iter_count++;
         m.diffusion_ramp = 1.0;
@@ -213,6 +217,10 @@ public:
         if (m.iter_n % 2 == 0) {
             op(m, "satadj");
             op(m, "damp_moist");
+            if (ice_scheme) {
+                ice(m);
+                UtilsAtm(m).precipitationSum();
+            }
             op(m, "moist_clamp");
             op(m, "densities");
             op(m, "evaporation");
@@ -340,6 +348,11 @@ int main(int argc, char** argv) {
             RadiationSelfTest::radiation(*m);
         }
         RadiationSelfTest::save_rad(*m, std::string(argv[2]) + ".rad");
+    } else if (mode == "steps_ice") {
+        if (!RadiationSelfTest::load_mp(*m, std::string(argv[1]) + ".mp")) return 1;
+        const int n = (argc > 4) ? std::atoi(argv[4]) : 1;
+        for (int q = 0; q < n; q++) RadiationSelfTest::step(*m, true);
+        RadiationSelfTest::save_mp(*m, std::string(argv[2]) + ".mp");
     } else if (mode == "ice") {
         if (!RadiationSelfTest::load_mp(*m, std::string(argv[1]) + ".mp")) return 1;
         RadiationSelfTest::ice(*m);

@@ -93,12 +93,41 @@ def check_ice(st, iter_n, base, wd, plant=None):
     return out
 
 
+def check_ice_steps(n, wd, plant=5, spin=2):
+    """n lex iterations with the ice scheme inside the moist block (and its sources in the
+    RHS), from the spun-up state with planted microphysics; the reference at one thread."""
+    t0 = time.time()
+    st = O.OracleState(*GRID, seed=1234)
+    st.run(spin, "jacobi")
+    snap = st.snapshot()
+    mp = {k: np.zeros(GRID) for k in O.MP_FIELDS}
+    O.plant_microphysics(snap, mp, plant)
+    s2 = O.OracleState(*GRID, blank=True)
+    s2.load(snap)
+    s2.set_iter(spin, spin)
+    for k, v in mp.items():
+        s2.mp(k)[...] = v
+    s2.set_ice_scheme(True)
+    base = os.path.join(wd, "base.bin")
+    s2.save_bin(base, iter_n=spin)
+    O.save_mp(base + ".mp", mp)
+    s2.run(n, "lex")
+    ref = run_ref(base, "steps_ice", n=n, threads=1, workdir=wd)
+    rmp = O.read_mp(os.path.join(wd, "ref_out.bin.mp"), GRID)
+    res = compare(s2.snapshot(), ref)
+    omp = s2.mp_snapshot()
+    for name in O.MP_FIELDS:
+        res[name] = {"rel_l2": O.rel_l2(omp[name], rmp[name]), "max_abs": float(np.max(np.abs(omp[name] - rmp[name])))}
+    print(f"{n} steps with the ice scheme: worst rel L2 {worst(res):.3e}  ({time.time()-t0:.1f}s)", flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--out", default=os.path.join(HERE, "_ref", "validation.json"))
     ap.add_argument("--spin", type=int, default=6, help="oracle steps before the per-operator checks")
-    ap.add_argument("--ice-only", action="store_true", help="only the ice-scheme check")
+    ap.add_argument("--ice-only", action="store_true", help="only the ice-scheme checks")
     args = ap.parse_args()
     if not os.path.exists(HARNESS):
         sys.exit("reference harness not built: make -C oracle -f Makefile.ref")
@@ -110,9 +139,9 @@ def main():
         st.run(args.spin, "lex")
         report["ice"] = check_ice(st, args.spin, os.path.join(wd, "base.bin"), wd)
         report["ice_planted"] = check_ice(st, args.spin, os.path.join(wd, "base.bin"), wd, plant=5)
-        for call, res in report["ice_planted"].items():
-            for k, v in res.items():
-                print(f"   {call} {k:14s} {v['rel_l2']:.3e}  max {v['max_abs']:.3e}")
+        report["ice_steps"] = check_ice_steps(args.steps, wd)
+        for k, v in report["ice_steps"].items():
+            print(f"   {k:14s} {v['rel_l2']:.3e}  max {v['max_abs']:.3e}")
         return
 
     # 1. thermodynamic init chain
@@ -168,6 +197,7 @@ def main():
     # fluxes, then from the first call's fluxes stored as P_rainn / P_snown
     report["ice"] = check_ice(st, iter_n, base, wd)
     report["ice_planted"] = check_ice(st, iter_n, base, wd, plant=5)
+    report["ice_steps"] = check_ice_steps(args.steps, wd)
     del st
 
     # 3. whole scoped steps from the initial state (lex = 1 reference thread)

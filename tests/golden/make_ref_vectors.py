@@ -105,7 +105,8 @@ def ice_vectors():
     """The two-category ice scheme (TwoCatIceScheme::run, then the damping of P_rain and
     P_snow) twice in a row on the Jacobi-spun-up state with planted condensate, fluxes and
     temperature spread; the second call starts from the first call's fluxes stored as
-    P_rainn / P_snown."""
+    P_rainn / P_snown. Then two whole iterations in the reference's single-thread order with
+    the scheme in its place in the moist block, from the same planted state."""
     wd = tempfile.mkdtemp(prefix="athad_golden_ice_")
     n = GRID[0] * GRID[1] * GRID[2]
     idx = sample_index(n)
@@ -138,6 +139,28 @@ def ice_vectors():
         mp = dict(rmp)
         mp["P_rainn"], mp["P_snown"] = rmp["P_rain"].copy(), rmp["P_snow"].copy()
         print(f"ice call {call} done", flush=True)
+    # two whole lex iterations with the ice scheme in the moist block (the second is a moist
+    # iteration), from the same planted state
+    st = O.OracleState(*GRID, seed=SEED)
+    st.run(SPIN, "jacobi")
+    snap = st.snapshot()
+    mp = {k: np.zeros(GRID) for k in O.MP_FIELDS}
+    O.plant_microphysics(snap, mp, ICE_PLANT)
+    st = O.OracleState(*GRID, blank=True)
+    st.load(snap)
+    st.set_iter(SPIN, SPIN)
+    st.save_bin(base, iter_n=SPIN)
+    O.save_mp(base + ".mp", mp)
+    subprocess.run([HARNESS, base, o, "steps_ice", "2"], check=True, cwd=wd, stdout=subprocess.DEVNULL,
+                   env=dict(os.environ, OMP_NUM_THREADS="1"))
+    d = dict(O.read_bin(o))
+    d.update(O.read_mp(o + ".mp", GRID))
+    for name in O.FIELDS + O.MP_FIELDS:
+        a = d[name].reshape(-1)
+        out[f"steps2_ice/{name}/sample"] = a[idx]
+        out[f"steps2_ice/{name}/sum"] = np.array(np.sum(a))
+        out[f"steps2_ice/{name}/l2"] = np.array(np.sqrt(np.sum(a * a)))
+    print("steps with the ice scheme done", flush=True)
     path = os.path.join(HERE, "ref_ice.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

@@ -146,3 +146,31 @@ def test_arrays_read_zero_before_first_use():
         ptr = buf.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_double))
         assert L.lib().athad_get_mp_field(integ.h, 0, L.MP_FIELDS.index("P_rain") + 99, ptr) == 1   # ATHAD_ERR_ARG
         assert L.lib().athad_get_mp_field(integ.h, 5, 0, ptr) == 1
+
+
+@pytest.mark.parametrize("order,grid,steps", [("jacobi", (21, 37, 73), 8), ("lex", (11, 21, 41), 4)])
+def test_ice_scheme_inside_the_step(oracle, order, grid, steps):
+    """CategoryIceScheme = 2 inside the moist block: SatAdj with S_c_c uncapped, damping of t
+    ice c cloud at every level, the scheme and its flux damping, precipitationSum, cap_S, the
+    clamps; the sources in RK4, their boundaries, falling precipitation evaporated by SatAdj.
+    Steps through the hipGraph too (athad_run), which must have been recaptured for it."""
+    print(f"case ice scheme in the step {order} {grid} {steps}", flush=True)
+    ref, snap, mp = planted(oracle, grid, 19)
+    ref.set_ice_scheme(True)
+    with A.Integration(grid, order=order) as integ:
+        load(integ, ref, snap, mp)
+        integ.run(4)                 # graph of the scoped step captured, then dropped below
+        ref.set_ice_scheme(False)
+        ref.run(4, order)
+        integ.set_ice_scheme(True)
+        ref.set_ice_scheme(True)
+        integ.run(steps)
+        ref.run(steps, order)
+        e = errors(integ, ref)
+        for n in A.PROGNOSTIC + ["p_stat", "r_humid", "p_dyn"]:
+            e[n] = rel(integ.get(n), ref.field(n))
+        print({k: f"{v:.1e}" for k, v in e.items()}, flush=True)
+        bad = {k: v for k, v in e.items() if v > (1e-8 if k == "S_c_c" else 1e-10)}
+        assert not bad, bad
+        for n in ("P_rain", "P_snow", "S_v", "S_r", "S_s"):
+            assert np.abs(ref.mp(n)).max() > 0.0, n

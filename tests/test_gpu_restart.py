@@ -9,7 +9,7 @@ from athad_cond_amd import _lib as L
 pytestmark = pytest.mark.gpu
 
 GRID = (11, 21, 41)
-HELD = L.RESTART_ARRAYS[:25]   # the arrays a context holds; the precipitation four are zeros
+HELD = L.RESTART_ARRAYS[:25]   # the state fields; the precipitation four are the ice scheme's
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -76,3 +76,28 @@ def test_bad_restarts_are_refused(tmp_path):
             a.load_state(str(tmp_path / "missing.bin"))
         # a refused load leaves the state as it was
         assert bytes(data) == a.save_state()
+
+
+def test_precipitation_arrays_travel_once_the_ice_scheme_has_run(oracle):
+    """P_rain, P_snow, P_rainn, P_snown go into the file from the context and come back."""
+    print(f"case restart with ice scheme {GRID}", flush=True)
+    ref = oracle.OracleState(*GRID, seed=4)
+    ref.run(2, "jacobi")
+    snap = ref.snapshot()
+    mp = {k: np.zeros(GRID) for k in oracle.MP_FIELDS}
+    oracle.plant_microphysics(snap, mp, 9)
+    with A.Integration(GRID) as a, A.Integration(GRID) as b:
+        a.load(snap)
+        for k, v in mp.items():
+            a.set_mp(k, v)
+        a.ice_scheme()
+        data = a.save_state()
+        _, arrays = A.read_restart(data)
+        for name in L.RESTART_ARRAYS[25:]:
+            assert arrays[name].any(), name
+            assert np.array_equal(arrays[name], a.get_mp(name)), name
+        b.init_state(1)
+        b.load_state(data)
+        for name in L.RESTART_ARRAYS[25:]:
+            assert np.array_equal(b.get_mp(name), a.get_mp(name)), name
+        assert b.save_state() == data
