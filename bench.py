@@ -58,6 +58,29 @@ def run_form(A, precision, order, steps, warmup, seed=1234, profile_steps=0, gra
             "rk4_launches": rk4_n, "fields": fields}
 
 
+def run_ice(A, precision, steps, warmup, seed=1234, profile_steps=8, graph=True):
+    """The same workload with the two-category ice scheme inside the moist block
+    (CategoryIceScheme = 2): its sources in RK4, its fluxes stored every iteration. Times the
+    step, then the ice-scheme launches (columns, flux boundaries and damping, clamps) alone."""
+    n = GRID[0] * GRID[1] * GRID[2]
+    with A.Integration(GRID, precision=precision) as integ:
+        integ.init_state(seed)
+        integ.set_ice_scheme(True)
+        integ.run(warmup, graph=graph)
+        integ.sync()
+        t0 = time.perf_counter()
+        integ.run(steps, graph=graph)
+        integ.sync()
+        dt = time.perf_counter() - t0
+        integ.profile("ice", True)
+        integ.run(profile_steps)
+        integ.sync()
+        ice_ms, ice_n = integ.profile_result("ice")
+        integ.profile("ice", False)
+    return {"ms_per_step": dt / steps * 1e3, "cells_per_s": n * steps / dt,
+            "ice_ms_per_call": ice_ms / max(ice_n, 1), "calls_profiled": ice_n}
+
+
 def measured_traffic():
     """HBM bytes per k_rk4_jacobi<double> launch at this grid, as recorded by the FETCH_SIZE and
     WRITE_SIZE counter runs (scaled by the copy-kernel calibration) in the committed profile
@@ -124,6 +147,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed and lex forms")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of the hipGraph")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a plain copy of known size")
+    ap.add_argument("--ice-only", action="store_true",
+                    help="only the FP64 step with the two-category ice scheme in the moist block")
     ap.add_argument("--mixed-only", action="store_true",
                     help="only time the mixed form (for profiling it alone); prints its own JSON line")
     args = ap.parse_args()
@@ -139,6 +164,11 @@ def main():
 
     n = GRID[0] * GRID[1] * GRID[2]
     graph = not args.no_graph
+    if args.ice_only:
+        r = run_ice(A, "fp64", args.steps, args.warmup, graph=graph)
+        print(json.dumps({"form": "fp64 with the ice scheme", "grid": list(GRID), "steps": args.steps,
+                          "warmup": args.warmup, **r}))
+        return
     if args.mixed_only:
         mixed = run_form(A, "mixed", "jacobi", args.steps, args.warmup, profile_steps=20, graph=graph)
         rk4_s = mixed["rk4_ms"] / max(mixed["rk4_launches"], 1) * 1e-3
@@ -192,6 +222,9 @@ def main():
         lex_steps = max(4, min(args.steps, 12))
         lex = run_form(A, "fp64", "lex", lex_steps, 4, graph=graph)
         out["lex"] = {"cells_per_s": lex["cells_per_s"], "ms_per_step": lex["ms_per_step"], "steps": lex_steps}
+        out["ice_scheme"] = run_ice(A, "fp64", args.steps, args.warmup, graph=graph)
+        out["ice_scheme"]["workload"] = ("FP64 Jacobi step at 41x181x361 with the two-category ice scheme "
+                                         "in the moist block and its sources in RK4")
         from athad_cond_amd.ensemble import run_ensemble
         out["ensemble"] = {prec: run_ensemble(32, (41, 91, 181), steps=40, precision=prec)
                            for prec in ("fp64", "mixed")}

@@ -159,12 +159,12 @@ def test_ice_scheme_inside_the_step(oracle, order, grid, steps):
     ref.set_ice_scheme(True)
     with A.Integration(grid, order=order) as integ:
         load(integ, ref, snap, mp)
-        integ.run(4)                 # graph of the scoped step captured, then dropped below
+        integ.run(4, graph=True)     # graph of the scoped step captured, then dropped below
         ref.set_ice_scheme(False)
         ref.run(4, order)
         integ.set_ice_scheme(True)
         ref.set_ice_scheme(True)
-        integ.run(steps)
+        integ.run(steps, graph=True)
         ref.run(steps, order)
         e = errors(integ, ref)
         for n in A.PROGNOSTIC + ["p_stat", "r_humid", "p_dyn"]:
