@@ -21,6 +21,8 @@
 namespace athad {
 
 template <typename S> ATHAD_DEV S* fp(const MemberDesc& M, int f) { return (S*)M.f[f]; }
+// ice-scheme array q of a member (allocated once the scheme has run)
+template <typename S> ATHAD_DEV S* mpp(const MemberDesc& M, int q) { return (S*)M.mp[q]; }
 ATHAD_DEV double* pstat(const MemberDesc& M) { return (double*)M.f[F_P_STAT]; }
 
 // 1-D cell index of this thread for a kernel launched over a member's n cells with the
@@ -614,236 +616,6 @@ __global__ void k_albedo_rows(const MemberDesc* __restrict__ md) {
     double a = 0.0;
     for (int k = 0; k < M.K.km; k++) a += M.albedo[(long long)j * M.K.km + k];
     M.scratch[sf_base(M.K) + j] = a;
-}
-
-// ------------------------------------------------------------------------------------
-// Two-category ice scheme (CategoryIceScheme = 2), TwoCatIceScheme::run, TwoCatIceScheme.h:89-111.
-// k_ice_columns: one thread per (j,k) column. initArrays (:139-155) over the whole column, then
-// for interior columns computeColumns (:160-517): up to three top-down passes of the rain and
-// snow fluxes, each level's rates from the flux of the previous pass at that level (Rain, Snow)
-// and the flux just formed one level up; the passes stop once the flux at level 1 stays within
-// 1e-3 mm/d of its value at entry. Rate variables that the reference leaves unassigned on some
-// branches (N_i and m_i outside the mixed-phase band, S_nuc where ice is zero and no nucleation
-// condition holds, S_i_dep where c equals q_Ice) carry over from level to level and pass to pass
-// in registers, as the reference's column-private variables do. Cells where no condensed phase
-// can exist (IceSchemeCommon.h:104-190) evaporate their condensate and falling precipitation.
-// Arithmetic in FP64 in both storage precisions, with contraction off so each operation rounds
-// as the reference's.
-// k_ice_bc: extrapolateBC (IceSchemeCommon.h:28-55) of both fluxes with c43 = 1, c13 = 0
-// (ints, cAtmosphereModel.h:158) and the phi-seam average, written as the closed form of the
-// three sequential face passes: a boundary cell takes the flux of the nearest cell with
-// i <= im-2 and 1 <= j <= jm-2 in its meridian plane, the seam cells the mean of k = 1 and
-// k = km-2. The flux values are finite and non-negative (clamped on every write), so c13 * x
-// contributes an exact zero.
-// ------------------------------------------------------------------------------------
-template <typename S> ATHAD_DEV S* mpp(const MemberDesc& M, int q) { return (S*)M.mp[q]; }
-
-template <typename S>
-__global__ __launch_bounds__(256) void k_ice_columns(const MemberDesc* __restrict__ md, int gate) {
-#pragma clang fp contract(off)
-    const MemberDesc& M = md[blockIdx.z];
-    if (gated_off(M, gate)) return;
-    const Consts& K = M.K;
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= K.jm * K.km) return;
-    const int j = col / K.km, k = col - j * K.km;
-    const int im = K.im;
-    const size_t plane = (size_t)K.jm * K.km;
-    const size_t base = (size_t)j * K.km + k;   // cell (0, j, k); level i adds i * plane
-    S *ft = fp<S>(M, F_T), *fc = fp<S>(M, F_C), *fcl = fp<S>(M, F_CLOUD), *fi = fp<S>(M, F_ICE);
-    S *fg = fp<S>(M, F_GR), *fscc = fp<S>(M, F_S_C_C);
-    const S *fco2 = fp<S>(M, F_CO2), *frh = fp<S>(M, F_R_HUMID);
-    const double* fps = (const double*)M.f[F_P_STAT];
-    S *Pr = mpp<S>(M, MP_P_RAIN), *Ps = mpp<S>(M, MP_P_SNOW), *Prn = mpp<S>(M, MP_P_RAINN), *Psn = mpp<S>(M, MP_P_SNOWN);
-    S *Pt = mpp<S>(M, MP_PRECIP);
-    S *Sv = mpp<S>(M, MP_S_V), *Sc = mpp<S>(M, MP_S_C), *Si = mpp<S>(M, MP_S_I), *Sr = mpp<S>(M, MP_S_R), *Ss = mpp<S>(M, MP_S_S);
-
-    for (int i = 0; i < im; i++) {   // initArrays
-        const size_t id = base + (size_t)i * plane;
-        fc[id] = tmax(S(0), fc[id]); fcl[id] = tmax(S(0), fcl[id]); fi[id] = tmax(S(0), fi[id]);
-        const S rn = tmax(S(0), Prn[id]), sn = tmax(S(0), Psn[id]);
-        Prn[id] = rn; Psn[id] = sn; Pr[id] = rn; Ps[id] = sn;
-    }
-    if (j < 1 || j > K.jm - 2 || k < 1 || k > K.km - 2) return;
-
-    constexpr double N_i_0 = 1.0e2, m_i_0 = 1.0e-12, m_i_max = 1.0e-9, m_s_0 = 3.0e-9;
-    constexpr double c_i_dep = 1.3e-5, c_c_au = 1.0e-3, q_c_crit = 5.0e-4, c_i_au = 1.0e-3;
-    constexpr double c_ac = 0.24, c_rim = 18.6, c_rim_snow = 18.6 / 5.0, c_agg = 10.3;
-    constexpr double c_i_cri = 0.24, c_r_cri = 3.2e-5, a_ev = 1.0e-3, b_ev = 5.9;
-    constexpr double c_s_dep = 1.8e-2, b_s_dep = 12.3, c_s_melt = 8.43e-5, b_s_melt = 12.05;
-    constexpr double c_r_frz = 3.75e-2;
-    constexpr double t_nuc = 267.15, t_d = 248.15, t_hn = 236.15, t_r_frz = 271.15;
-    constexpr double P_max = 50.0 / 86400.0;
-    const double t_0 = K.t_0, M_bg = K.M_bg;
-
-    const double Rain_check = (double)Pr[base + plane];   // level i_check = 1
-    double m_i = m_i_max, N_i = 0.0, S_nuc = 0.0, S_i_dep = 0.0;
-    for (int pass = 1; pass <= 3; pass++) {
-        Pr[base + (size_t)(im - 1) * plane] = S(0);
-        Ps[base + (size_t)(im - 1) * plane] = S(0);
-        double pr_up = 0.0, ps_up = 0.0;   // fluxes of the level above, as just stored
-        for (int i = im - 2; i >= 0; i--) {
-            const size_t id = base + (size_t)i * plane;
-            const double tt = (double)ft[id], cv = (double)fc[id], co2 = (double)fco2[id];
-            const double p_u = fps[id];
-            const double t_u = tt * t_0;
-            const double Mo = th_M_nonwater(cv, co2, M_bg);
-            const double q_sw = (t_u >= kTcrit) ? 1.0 : th_q_sat(th_p_sat(t_u), p_u, Mo);
-            if (!(q_sw < 1.0)) {
-                // evaporateWhereImpossible (IceSchemeCommon.h:121-190)
-                const double cld0 = (double)fcl[id], ic0 = (double)fi[id], gr0 = (double)fg[id];
-                const double q_cond = tmax(0.0, cld0) + tmax(0.0, ic0) + tmax(0.0, gr0);
-                double c_now = cv;
-                if (q_cond > 0.0) {
-                    const double c_max = tmax(0.0, 1.0 - co2);
-                    const double c_new = tmin(c_max, cv + q_cond);
-                    const double q_used = tmax(0.0, c_new - cv);
-                    c_now = c_new;
-                    fcl[id] = S(0); fi[id] = S(0); fg[id] = S(0);
-                    if (t_u < kTcrit) {
-                        const double cp_loc = th_cp_of(c_new, co2, t_u, M_bg);
-                        if (cp_loc > 0.0) {
-                            double T_new = t_u - th_L_vap(t_u) * q_used / cp_loc;
-                            if (T_new < 1.0) T_new = 1.0;
-                            ft[id] = S(T_new / t_0);
-                        }
-                    }
-                }
-                const double rho = tmax(1.0e-6, (double)frh[id]);
-                const double q_p = tmax(0.0, (double)Pr[id]) / (1.6 * rho) + tmax(0.0, (double)Ps[id]) / (0.96 * rho)
-                                 + 0.0 / (2.0 * rho);
-                if (q_p > 0.0) {
-                    const double c_max = tmax(0.0, 1.0 - co2);
-                    c_now = tmin(c_max, c_now + q_p);
-                }
-                if (c_now != cv) fc[id] = S(c_now);
-                Pr[id] = S(0); Ps[id] = S(0); Pt[id] = S(0);
-                Sv[id] = S(0); Sc[id] = S(0); Si[id] = S(0); Sr[id] = S(0); Ss[id] = S(0);
-                fscc[id] = S(0);
-                pr_up = 0.0; ps_up = 0.0;
-                continue;
-            }
-            const double Rain = (double)Pr[id], Snow = (double)Ps[id];
-            double R49 = 0.0, R79 = 0.0, R139 = 0.0;
-            if (Rain > 1e-12) {
-                const double rb = pow(Rain, 1.0 / 9.0);
-                const double r2 = rb * rb;
-                const double r4 = r2 * r2;
-                R49 = r4; R79 = Rain / r2; R139 = Rain * r4;
-            }
-            const double q_Ice = th_q_sat(th_p_subl(t_u), p_u, Mo);
-            const double rh = (double)frh[id];
-            const double step = (double)(float)((float)M.hgt[i + 1] - (float)M.hgt[i]);
-            const double mass_layer = rh * step;
-            const double dt_snow_dim = step / 0.96;
-            const double cld = (double)fcl[id], ic = (double)fi[id];
-
-            if (!(t_u > t_0) && !(t_u <= t_hn)) {
-                N_i = N_i_0 * exp(0.2 * (t_0 - t_u));
-                m_i = tmin((rh * ic) / N_i, m_i_max);
-                if (m_i > m_i_max) m_i = m_i_max;
-                if (m_i < m_i_0) m_i = m_i_0;
-            }
-            if (ic == 0.0) {
-                if (((t_u < t_d) && (cv >= q_Ice)) || (((t_d <= t_u) && (t_u <= t_nuc)) && (cv >= q_sw)))
-                    S_nuc = m_i_0 / (rh * dt_snow_dim) * N_i;
-            } else S_nuc = 0.0;
-            double S_c_frz = (t_u < t_hn) ? cld / dt_snow_dim : 0.0;
-            if (cv > q_Ice) S_i_dep = c_i_dep * N_i * pow(m_i, 1.0 / 3.0) * (cv - q_Ice);
-            if (cv < q_Ice) S_i_dep = tmax((-ic / dt_snow_dim), ((cv - q_Ice) / dt_snow_dim));
-            double S_c_au = (cld > q_c_crit) ? c_c_au * (cld - q_c_crit) : 0.0;
-            double S_i_au = (ic > 0.0) ? tmax(c_i_au * ic, 0.0) : 0.0;
-            double S_d_au = (S_i_dep > 0.0) ? S_i_dep / (1.5 * (pow((m_s_0 / m_i), 2.0 / 3.0) - 1.0)) : 0.0;
-            double S_ac = ((t_u >= t_0) && (cld > q_c_crit)) ? c_ac * (cld - q_c_crit) * R79 : 0.0;
-            double S_rim = (t_u < t_0) ? c_rim_snow * cld * Snow : 0.0;
-            double S_shed = (t_u >= t_0) ? c_rim * cld * Snow : 0.0;
-            double S_s_melt = 0.0, S_i_melt = 0.0;
-            if (t_u > t_0) {
-                if (Snow > 1e-12) {
-                    S_s_melt = c_s_melt * (1.0 + b_s_melt * pow(Snow, 5.0 / 26.0)) * (t_u - t_0) * pow(Snow, 1.0 / 3.0);
-                    S_s_melt = tmin(S_s_melt, Snow / mass_layer);
-                }
-                S_i_melt = ic / dt_snow_dim;
-            }
-            double S_agg = 0.0, S_i_cri = 0.0, S_r_cri = 0.0;
-            if (t_u <= t_0) {
-                S_agg = c_agg * ic * Snow;
-                S_i_cri = c_i_cri * ic * R79;
-                S_r_cri = c_r_cri * ic / m_i * R139;
-            }
-            double S_ev = 0.0;
-            if (t_u > t_0 && Rain > 1e-12 && cv < q_sw) {
-                S_ev = a_ev * (1.0 + b_ev * pow(Rain, 1.0 / 6.0)) * (q_sw - cv) * R49;
-                S_ev = tmin(S_ev, Rain / mass_layer);
-            }
-            double S_s_dep = 0.0;
-            if (t_u < t_0 && Snow > 1e-12) {
-                S_s_dep = c_s_dep * (1.0 + b_s_dep * pow(Snow, 5.0 / 26.0)) * (cv - q_Ice) * pow(Snow, 8.0 / 13.0);
-                if (S_s_dep < 0.0) S_s_dep = tmax(S_s_dep, -Snow / mass_layer);
-            }
-            double S_r_frz = 0.0;
-            if (t_u <= t_0) {
-                const double t_frz = tmax((t_r_frz - t_u), 0.0);
-                S_r_frz = c_r_frz * pow(t_frz, 3.0 / 2.0) * pow(Rain, 3.0 / 2.0);
-            }
-            const double S_cloud_total = S_c_au + S_ac + S_rim + S_c_frz + S_shed;
-            const double max_cloud_loss = cld / dt_snow_dim;
-            if (S_cloud_total > max_cloud_loss && S_cloud_total > 0) {
-                const double f = max_cloud_loss / S_cloud_total;
-                S_c_au *= f; S_ac *= f; S_rim *= f; S_c_frz *= f; S_shed *= f;
-            }
-            const double S_ice_total = S_i_au + S_d_au + S_agg + S_i_cri + S_i_melt;
-            const double max_ice_loss = ic / dt_snow_dim;
-            if (S_ice_total > max_ice_loss && S_ice_total > 0) {
-                const double f = max_ice_loss / S_ice_total;
-                S_i_au *= f; S_d_au *= f; S_agg *= f; S_i_cri *= f; S_i_melt *= f;
-            }
-            const double scc = (double)fscc[id];
-            Sv[id] = S(-scc + S_ev - S_i_dep - S_s_dep - S_nuc);
-            Sc[id] = S(scc - S_c_au - S_ac - S_c_frz + S_i_melt - S_rim - S_shed);
-            Si[id] = S(S_nuc + S_c_frz + S_i_dep - S_i_melt - S_i_au - S_d_au - S_agg - S_i_cri);
-            Sr[id] = S(S_c_au + S_ac - S_ev + S_shed - S_r_cri - S_r_frz + S_s_melt);
-            Ss[id] = S(S_i_au + S_d_au + S_agg + S_rim + S_s_dep + S_i_cri + S_r_cri + S_r_frz - S_s_melt);
-            const double dP_rain = (S_c_au + S_ac + S_shed + S_s_melt - S_ev - S_r_frz) * mass_layer;
-            const double pr = tmin(P_max, tmax(0.0, pr_up + dP_rain));
-            const double dP_snow = (S_i_au + S_d_au + S_rim + S_agg + S_r_frz - S_s_melt) * mass_layer;
-            const double psn = (t_u < t_0 && t_u >= K.t_000) ? tmin(P_max, tmax(0.0, ps_up + dP_snow)) : 0.0;
-            Pr[id] = S(pr); Ps[id] = S(psn);
-            Pt[id] = S(tmin(P_max, (double)S(pr) + (double)S(psn)));
-            pr_up = (double)S(pr); ps_up = (double)S(psn);
-        }
-        if (fabs((double)Pr[base + plane] - Rain_check) * 8.64e4 <= 1.0e-3) break;
-    }
-}
-
-template <typename S>
-__global__ __launch_bounds__(256) void k_ice_bc(const MemberDesc* __restrict__ md, int gate) {
-#pragma clang fp contract(off)
-    const MemberDesc& M = md[blockIdx.z];
-    if (gated_off(M, gate)) return;
-    const Consts& K = M.K;
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= K.n) return;
-    int i, j, k;
-    decode(K, id, i, j, k);
-    const bool seam = (k == 0 || k == K.km - 1);
-    if (!seam && i < K.im - 1 && j > 0 && j < K.jm - 1) return;
-    const int ii = min(i, K.im - 2), jj = min(max(j, 1), K.jm - 2);
-    for (int q = 0; q < 2; q++) {
-        S* P = mpp<S>(M, q == 0 ? MP_P_RAIN : MP_P_SNOW);
-        if (!seam) P[id] = P[ix3(K, ii, jj, k)];
-        else P[id] = S((double(P[ix3(K, ii, jj, 1)]) + double(P[ix3(K, ii, jj, K.km - 2)])) * 0.5);
-    }
-}
-
-// P_rainn <- P_rain, P_snown <- P_snow (storeIntermediateData3D, UtilsAtm.h:41-42).
-template <typename S>
-__global__ __launch_bounds__(256) void k_ice_store(const MemberDesc* __restrict__ md) {
-    const MemberDesc& M = md[blockIdx.z];
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= M.K.n || !M.mp[0]) return;
-    mpp<S>(M, MP_P_RAINN)[id] = mpp<S>(M, MP_P_RAIN)[id];
-    mpp<S>(M, MP_P_SNOWN)[id] = mpp<S>(M, MP_P_SNOW)[id];
 }
 
 // With the ice scheme in the moist block, after its flux damping (cAtmosphereModel.cpp:1263-1399):
@@ -1639,6 +1411,305 @@ template <typename A> ATHAD_DEV void sat_qA(A E, A p, A inv_p, A Mo, A& q, A& Af
     const A rd = rcp_nr(den);
     q = num * rd;
     Af = num * Mo * rd * rd;
+}
+
+// ------------------------------------------------------------------------------------
+// Two-category ice scheme (CategoryIceScheme = 2), TwoCatIceScheme::run, TwoCatIceScheme.h:89-111.
+// k_ice_columns: one thread per (j,k) column. initArrays (:139-155) over the whole column, then
+// for interior columns computeColumns (:160-517): up to three top-down passes of the rain and
+// snow fluxes, each level's rates from the flux of the previous pass at that level (Rain, Snow)
+// and the flux just formed one level up; the passes stop once the flux at level 1 stays within
+// 1e-3 mm/d of its value at entry. Rate variables that the reference leaves unassigned on some
+// branches (N_i and m_i outside the mixed-phase band, S_nuc where ice is zero and no nucleation
+// condition holds, S_i_dep where c equals q_Ice) carry over from level to level and pass to pass
+// in registers, as the reference's column-private variables do. Cells where no condensed phase
+// can exist (IceSchemeCommon.h:104-190) evaporate their condensate and falling precipitation.
+// Arithmetic in FP64 in both storage precisions, with contraction off so each operation rounds
+// as the reference's.
+// k_ice_bc: extrapolateBC (IceSchemeCommon.h:28-55) of both fluxes with c43 = 1, c13 = 0
+// (ints, cAtmosphereModel.h:158) and the phi-seam average, written as the closed form of the
+// three sequential face passes: a boundary cell takes the flux of the nearest cell with
+// i <= im-2 and 1 <= j <= jm-2 in its meridian plane, the seam cells the mean of k = 1 and
+// k = km-2. The flux values are finite and non-negative (clamped on every write), so c13 * x
+// contributes an exact zero.
+// ------------------------------------------------------------------------------------
+
+namespace ice {
+constexpr double N_i_0 = 1.0e2, m_i_0 = 1.0e-12, m_i_max = 1.0e-9, m_s_0 = 3.0e-9;
+constexpr double c_i_dep = 1.3e-5, c_c_au = 1.0e-3, q_c_crit = 5.0e-4, c_i_au = 1.0e-3;
+constexpr double c_ac = 0.24, c_rim = 18.6, c_rim_snow = 18.6 / 5.0, c_agg = 10.3;
+constexpr double c_i_cri = 0.24, c_r_cri = 3.2e-5, a_ev = 1.0e-3, b_ev = 5.9;
+constexpr double c_s_dep = 1.8e-2, b_s_dep = 12.3, c_s_melt = 8.43e-5, b_s_melt = 12.05;
+constexpr double c_r_frz = 3.75e-2;
+constexpr double t_nuc = 267.15, t_d = 248.15, t_hn = 236.15, t_r_frz = 271.15;
+constexpr double P_max = 50.0 / 86400.0;
+constexpr int i_check = 1, passes = 3;
+
+// The state of one condensable level that the rates read (TwoCatIceScheme.h:229-266).
+struct Level {
+    double t_u, cv, cld, ic, rh, scc, q_sw, q_Ice, mass_layer, dt_snow_dim, Rain, Snow;
+    double R49, R79, R139;
+};
+
+// The three rate variables the reference assigns only on some branches. Each returns whether it
+// assigned; otherwise the value carried from the level above (or the previous pass) stands.
+ATHAD_DEV bool assign_Ni_mi(const Level& L, double t_0, double& N_i, double& m_i) {   // :273-281
+#pragma clang fp contract(off)
+    if (!(!(L.t_u > t_0) && !(L.t_u <= t_hn))) return false;
+    N_i = N_i_0 * exp(0.2 * (t_0 - L.t_u));
+    m_i = tmin((L.rh * L.ic) / N_i, m_i_max);
+    if (m_i > m_i_max) m_i = m_i_max;
+    if (m_i < m_i_0) m_i = m_i_0;
+    return true;
+}
+ATHAD_DEV bool assign_S_nuc(const Level& L, double N_i, double& S_nuc) {   // :285-292
+#pragma clang fp contract(off)
+    if (L.ic == 0.0) {
+        if (((L.t_u < t_d) && (L.cv >= L.q_Ice)) || (((t_d <= L.t_u) && (L.t_u <= t_nuc)) && (L.cv >= L.q_sw))) {
+            S_nuc = m_i_0 / (L.rh * L.dt_snow_dim) * N_i;
+            return true;
+        }
+        return false;
+    }
+    S_nuc = 0.0;
+    return true;
+}
+ATHAD_DEV bool assign_S_i_dep(const Level& L, double N_i, double m_i, double& S_i_dep) {   // :302-308
+#pragma clang fp contract(off)
+    bool a = false;
+    if (L.cv > L.q_Ice) { S_i_dep = c_i_dep * N_i * pow(m_i, 1.0 / 3.0) * (L.cv - L.q_Ice); a = true; }
+    if (L.cv < L.q_Ice) { S_i_dep = tmax((-L.ic / L.dt_snow_dim), ((L.cv - L.q_Ice) / L.dt_snow_dim)); a = true; }
+    return a;
+}
+
+struct Rates { double Sv, Sc, Si, Sr, Ss, dPr, dPs; };
+
+// Everything else at one level (:312-455) once the carried variables are resolved.
+ATHAD_DEV Rates rates(const Level& L, double t_0, double N_i, double m_i, double S_nuc, double S_i_dep) {
+#pragma clang fp contract(off)
+    const double t_u = L.t_u, cv = L.cv, cld = L.cld, ic = L.ic, Rain = L.Rain, Snow = L.Snow;
+    const double dt_snow_dim = L.dt_snow_dim, mass_layer = L.mass_layer;
+    (void)N_i;
+    double S_c_frz = (t_u < t_hn) ? cld / dt_snow_dim : 0.0;
+    double S_c_au = (cld > q_c_crit) ? c_c_au * (cld - q_c_crit) : 0.0;
+    double S_i_au = (ic > 0.0) ? tmax(c_i_au * ic, 0.0) : 0.0;
+    double S_d_au = (S_i_dep > 0.0) ? S_i_dep / (1.5 * (pow((m_s_0 / m_i), 2.0 / 3.0) - 1.0)) : 0.0;
+    double S_ac = ((t_u >= t_0) && (cld > q_c_crit)) ? c_ac * (cld - q_c_crit) * L.R79 : 0.0;
+    double S_rim = (t_u < t_0) ? c_rim_snow * cld * Snow : 0.0;
+    double S_shed = (t_u >= t_0) ? c_rim * cld * Snow : 0.0;
+    double S_s_melt = 0.0, S_i_melt = 0.0;
+    if (t_u > t_0) {
+        if (Snow > 1e-12) {
+            S_s_melt = c_s_melt * (1.0 + b_s_melt * pow(Snow, 5.0 / 26.0)) * (t_u - t_0) * pow(Snow, 1.0 / 3.0);
+            S_s_melt = tmin(S_s_melt, Snow / mass_layer);
+        }
+        S_i_melt = ic / dt_snow_dim;
+    }
+    double S_agg = 0.0, S_i_cri = 0.0, S_r_cri = 0.0;
+    if (t_u <= t_0) {
+        S_agg = c_agg * ic * Snow;
+        S_i_cri = c_i_cri * ic * L.R79;
+        S_r_cri = c_r_cri * ic / m_i * L.R139;
+    }
+    double S_ev = 0.0;
+    if (t_u > t_0 && Rain > 1e-12 && cv < L.q_sw) {
+        S_ev = a_ev * (1.0 + b_ev * pow(Rain, 1.0 / 6.0)) * (L.q_sw - cv) * L.R49;
+        S_ev = tmin(S_ev, Rain / mass_layer);
+    }
+    double S_s_dep = 0.0;
+    if (t_u < t_0 && Snow > 1e-12) {
+        S_s_dep = c_s_dep * (1.0 + b_s_dep * pow(Snow, 5.0 / 26.0)) * (cv - L.q_Ice) * pow(Snow, 8.0 / 13.0);
+        if (S_s_dep < 0.0) S_s_dep = tmax(S_s_dep, -Snow / mass_layer);
+    }
+    double S_r_frz = 0.0;
+    if (t_u <= t_0 && Rain != 0.0) {   // with no rain the product is an exact zero
+        const double t_frz = tmax((t_r_frz - t_u), 0.0);
+        S_r_frz = c_r_frz * pow(t_frz, 3.0 / 2.0) * pow(Rain, 3.0 / 2.0);
+    }
+    const double S_cloud_total = S_c_au + S_ac + S_rim + S_c_frz + S_shed;
+    const double max_cloud_loss = cld / dt_snow_dim;
+    if (S_cloud_total > max_cloud_loss && S_cloud_total > 0) {
+        const double f = max_cloud_loss / S_cloud_total;
+        S_c_au *= f; S_ac *= f; S_rim *= f; S_c_frz *= f; S_shed *= f;
+    }
+    const double S_ice_total = S_i_au + S_d_au + S_agg + S_i_cri + S_i_melt;
+    const double max_ice_loss = ic / dt_snow_dim;
+    if (S_ice_total > max_ice_loss && S_ice_total > 0) {
+        const double f = max_ice_loss / S_ice_total;
+        S_i_au *= f; S_d_au *= f; S_agg *= f; S_i_cri *= f; S_i_melt *= f;
+    }
+    Rates r;
+    r.Sv = -L.scc + S_ev - S_i_dep - S_s_dep - S_nuc;
+    r.Sc = L.scc - S_c_au - S_ac - S_c_frz + S_i_melt - S_rim - S_shed;
+    r.Si = S_nuc + S_c_frz + S_i_dep - S_i_melt - S_i_au - S_d_au - S_agg - S_i_cri;
+    r.Sr = S_c_au + S_ac - S_ev + S_shed - S_r_cri - S_r_frz + S_s_melt;
+    r.Ss = S_i_au + S_d_au + S_agg + S_rim + S_s_dep + S_i_cri + S_r_cri + S_r_frz - S_s_melt;
+    r.dPr = (S_c_au + S_ac + S_shed + S_s_melt - S_ev - S_r_frz) * mass_layer;
+    r.dPs = (S_i_au + S_d_au + S_rim + S_agg + S_r_frz - S_s_melt) * mass_layer;
+    return r;
+}
+
+// The level's inputs; false where no condensed phase can exist (IceSchemeCommon.h:104-107).
+ATHAD_DEV bool load_level(const Consts& K, const double* hgt, int i, double tt, double cv, double cld, double ic,
+                          double co2, double p_u, double rh, double scc, double Rain, double Snow, Level& L) {
+#pragma clang fp contract(off)
+    const double t_u = tt * K.t_0;
+    const double Mo = th_M_nonwater(cv, co2, K.M_bg);
+    const double q_sw = (t_u >= kTcrit) ? 1.0 : th_q_sat(sat_p_sat(t_u), p_u, Mo);
+    L.t_u = t_u; L.cv = cv; L.q_sw = q_sw;
+    if (!(q_sw < 1.0)) return false;
+    L.cld = cld; L.ic = ic; L.rh = rh; L.scc = scc; L.Rain = Rain; L.Snow = Snow;
+    L.R49 = 0.0; L.R79 = 0.0; L.R139 = 0.0;
+    if (Rain > 1e-12) {
+        const double rb = pow(Rain, 1.0 / 9.0);
+        const double r2 = rb * rb;
+        const double r4 = r2 * r2;
+        L.R49 = r4; L.R79 = Rain / r2; L.R139 = Rain * r4;
+    }
+    L.q_Ice = th_q_sat(sat_p_subl(t_u), p_u, Mo);
+    const double step = (double)(float)((float)hgt[i + 1] - (float)hgt[i]);
+    L.mass_layer = rh * step;
+    L.dt_snow_dim = step / 0.96;
+    return true;
+}
+
+// evaporateWhereImpossible (IceSchemeCommon.h:121-190) on the values of one cell: new t, c and
+// whether t changed; condensate, fluxes and sources of the cell become zero.
+ATHAD_DEV void evaporate(const Consts& K, double t_u, double cv, double cld, double ic, double gr, double co2,
+                         double rh, double Pr, double Ps, double& t_new, bool& t_set, double& c_new) {
+#pragma clang fp contract(off)
+    const double q_cond = tmax(0.0, cld) + tmax(0.0, ic) + tmax(0.0, gr);
+    c_new = cv;
+    t_set = false;
+    if (q_cond > 0.0) {
+        const double c_max = tmax(0.0, 1.0 - co2);
+        const double cn = tmin(c_max, cv + q_cond);
+        const double q_used = tmax(0.0, cn - cv);
+        c_new = cn;
+        if (t_u < kTcrit) {
+            const double cp_loc = th_cp_of(cn, co2, t_u, K.M_bg);
+            if (cp_loc > 0.0) {
+                double T_new = t_u - th_L_vap(t_u) * q_used / cp_loc;
+                if (T_new < 1.0) T_new = 1.0;
+                t_new = T_new / K.t_0;
+                t_set = true;
+            }
+        }
+    }
+    const double rho = tmax(1.0e-6, rh);
+    const double q_p = tmax(0.0, Pr) / (1.6 * rho) + tmax(0.0, Ps) / (0.96 * rho) + 0.0 / (2.0 * rho);
+    if (q_p > 0.0) c_new = tmin(tmax(0.0, 1.0 - co2), c_new + q_p);
+}
+}  // namespace ice
+
+// One column of the scheme, sequentially as the reference runs it: initArrays, then for an
+// interior column up to three top-down passes.
+template <typename S>
+ATHAD_DEV void ice_column_seq(const MemberDesc& M, int j, int k) {
+#pragma clang fp contract(off)
+    using namespace ice;
+    const Consts& K = M.K;
+    const int im = K.im;
+    const size_t plane = (size_t)K.jm * K.km;
+    const size_t base = (size_t)j * K.km + k;   // cell (0, j, k); level i adds i * plane
+    S *ft = fp<S>(M, F_T), *fc = fp<S>(M, F_C), *fcl = fp<S>(M, F_CLOUD), *fi = fp<S>(M, F_ICE);
+    S *fg = fp<S>(M, F_GR), *fscc = fp<S>(M, F_S_C_C);
+    const S *fco2 = fp<S>(M, F_CO2), *frh = fp<S>(M, F_R_HUMID);
+    const double* fps = (const double*)M.f[F_P_STAT];
+    S *Pr = mpp<S>(M, MP_P_RAIN), *Ps = mpp<S>(M, MP_P_SNOW), *Prn = mpp<S>(M, MP_P_RAINN), *Psn = mpp<S>(M, MP_P_SNOWN);
+    S *Pt = mpp<S>(M, MP_PRECIP);
+    S *Sv = mpp<S>(M, MP_S_V), *Sc = mpp<S>(M, MP_S_C), *Si = mpp<S>(M, MP_S_I), *Sr = mpp<S>(M, MP_S_R), *Ss = mpp<S>(M, MP_S_S);
+
+    for (int i = 0; i < im; i++) {   // initArrays
+        const size_t id = base + (size_t)i * plane;
+        fc[id] = tmax(S(0), fc[id]); fcl[id] = tmax(S(0), fcl[id]); fi[id] = tmax(S(0), fi[id]);
+        const S rn = tmax(S(0), Prn[id]), sn = tmax(S(0), Psn[id]);
+        Prn[id] = rn; Psn[id] = sn; Pr[id] = rn; Ps[id] = sn;
+    }
+    if (j < 1 || j > K.jm - 2 || k < 1 || k > K.km - 2) return;
+
+    const double Rain_check = (double)Pr[base + (size_t)i_check * plane];
+    double m_i = m_i_max, N_i = 0.0, S_nuc = 0.0, S_i_dep = 0.0;
+    for (int pass = 1; pass <= passes; pass++) {
+        Pr[base + (size_t)(im - 1) * plane] = S(0);
+        Ps[base + (size_t)(im - 1) * plane] = S(0);
+        double pr_up = 0.0, ps_up = 0.0;   // fluxes of the level above, as just stored
+        for (int i = im - 2; i >= 0; i--) {
+            const size_t id = base + (size_t)i * plane;
+            Level L;
+            const double cv = (double)fc[id], co2 = (double)fco2[id];
+            if (!load_level(K, M.hgt, i, (double)ft[id], cv, (double)fcl[id], (double)fi[id], co2, fps[id],
+                            (double)frh[id], (double)fscc[id], (double)Pr[id], (double)Ps[id], L)) {
+                double t_new = 0.0, c_new;
+                bool t_set;
+                evaporate(K, L.t_u, cv, (double)fcl[id], (double)fi[id], (double)fg[id], co2, (double)frh[id],
+                          (double)Pr[id], (double)Ps[id], t_new, t_set, c_new);
+                if (tmax(0.0, (double)fcl[id]) + tmax(0.0, (double)fi[id]) + tmax(0.0, (double)fg[id]) > 0.0) {
+                    fcl[id] = S(0); fi[id] = S(0); fg[id] = S(0);
+                }
+                if (t_set) ft[id] = S(t_new);
+                if (c_new != cv) fc[id] = S(c_new);
+                Pr[id] = S(0); Ps[id] = S(0); Pt[id] = S(0);
+                Sv[id] = S(0); Sc[id] = S(0); Si[id] = S(0); Sr[id] = S(0); Ss[id] = S(0);
+                fscc[id] = S(0);
+                pr_up = 0.0; ps_up = 0.0;
+                continue;
+            }
+            assign_Ni_mi(L, K.t_0, N_i, m_i);
+            assign_S_nuc(L, N_i, S_nuc);
+            assign_S_i_dep(L, N_i, m_i, S_i_dep);
+            const Rates r = rates(L, K.t_0, N_i, m_i, S_nuc, S_i_dep);
+            Sv[id] = S(r.Sv); Sc[id] = S(r.Sc); Si[id] = S(r.Si); Sr[id] = S(r.Sr); Ss[id] = S(r.Ss);
+            const double pr = tmin(P_max, tmax(0.0, pr_up + r.dPr));
+            const double psn = (L.t_u < K.t_0 && L.t_u >= K.t_000) ? tmin(P_max, tmax(0.0, ps_up + r.dPs)) : 0.0;
+            Pr[id] = S(pr); Ps[id] = S(psn);
+            Pt[id] = S(tmin(P_max, (double)S(pr) + (double)S(psn)));
+            pr_up = (double)S(pr); ps_up = (double)S(psn);
+        }
+        if (fabs((double)Pr[base + (size_t)i_check * plane] - Rain_check) * 8.64e4 <= 1.0e-3) break;
+    }
+}
+
+// One thread per (j, k) column.
+template <typename S>
+__global__ __launch_bounds__(256) void k_ice_columns(const MemberDesc* __restrict__ md, int gate) {
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= K.jm * K.km) return;
+    const int j = col / K.km;
+    ice_column_seq<S>(M, j, col - j * K.km);
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_ice_bc(const MemberDesc* __restrict__ md, int gate) {
+#pragma clang fp contract(off)
+    const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
+    const Consts& K = M.K;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= K.n) return;
+    int i, j, k;
+    decode(K, id, i, j, k);
+    const bool seam = (k == 0 || k == K.km - 1);
+    if (!seam && i < K.im - 1 && j > 0 && j < K.jm - 1) return;
+    const int ii = min(i, K.im - 2), jj = min(max(j, 1), K.jm - 2);
+    for (int q = 0; q < 2; q++) {
+        S* P = mpp<S>(M, q == 0 ? MP_P_RAIN : MP_P_SNOW);
+        if (!seam) P[id] = P[ix3(K, ii, jj, k)];
+        else P[id] = S((double(P[ix3(K, ii, jj, 1)]) + double(P[ix3(K, ii, jj, K.km - 2)])) * 0.5);
+    }
+}
+
+// P_rainn <- P_rain, P_snown <- P_snow (storeIntermediateData3D, UtilsAtm.h:41-42).
+template <typename S>
+__global__ __launch_bounds__(256) void k_ice_store(const MemberDesc* __restrict__ md) {
+    const MemberDesc& M = md[blockIdx.z];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= M.K.n || !M.mp[0]) return;
+    mpp<S>(M, MP_P_RAINN)[id] = mpp<S>(M, MP_P_RAIN)[id];
+    mpp<S>(M, MP_P_SNOWN)[id] = mpp<S>(M, MP_P_SNOW)[id];
 }
 
 // Column and cell thermodynamics (k_densities, the SatAdj entry and clamp): th_R_of, th_cp_of, th_M_nonwater, th_q_sat and

@@ -51,8 +51,10 @@ def errors(integ, ref, member=0):
     return e
 
 
-@pytest.mark.parametrize("grid", [(21, 37, 73), (41, 91, 181)])
+@pytest.mark.parametrize("grid", [(21, 37, 73), (41, 91, 181), (70, 11, 21)])
 def test_two_calls_match_oracle(oracle, grid):
+    """Grids up to 64 levels run the wave-per-column kernel, taller ones the thread-per-column
+    kernel."""
     print(f"case ice scheme fp64 {grid}", flush=True)
     ref, snap, mp = planted(oracle, grid, 11)
     with A.Integration(grid) as integ:
