@@ -121,6 +121,8 @@ struct athad_ctx {
     int tile_shape = 0, tile_ich = 1 << 20;
     // the two-category ice scheme runs inside the moist block (CategoryIceScheme = 2)
     bool ice_on = false;
+    // the fluxes may differ from their stored copies (changed outside a step's moist block)
+    bool mp_dirty = false;
 };
 
 namespace {
@@ -395,7 +397,7 @@ void op_turbulence(athad_ctx* c, int gate, hipStream_t st = nullptr) {
 // SatAdj, the moist damping and densities read and write the thermodynamic fields and
 // scratch 6..9. The two sets are disjoint, so their order is immaterial and the turbulence
 // runs on the second stream beside the others.
-void op_moist_block(athad_ctx* c, int gate, bool poisson_beside = false) {
+void op_moist_block(athad_ctx* c, int gate, bool poisson_beside = false, bool store_fluxes = false) {
     hipEventRecord(c->ev_fork, c->stream);
     hipStreamWaitEvent(c->stream2, c->ev_fork, 0);
     if (poisson_beside) {
@@ -423,7 +425,7 @@ void op_moist_block(athad_ctx* c, int gate, bool poisson_beside = false) {
         {
             Prof p(c, KID_ICE);
             launch_ice_scheme(cur(c), gate);
-            launch_moist_clamp_ice(cur(c), gate);
+            launch_moist_clamp_ice(cur(c), gate, store_fluxes ? 1 : 0);
         }
     } else {
         {
@@ -555,9 +557,14 @@ void op_step(athad_ctx* c) {
     bool same = true;
     for (auto& m : c->mem) same = same && ((m.iter_n + 1) % 4 == (c->mem[0].iter_n + 1) % 4);
     const int it = c->mem[0].iter_n + 1;
+    // With the ice scheme in every member's moist block in step, the fluxes change only there,
+    // so their store rides on the clamp after the scheme and other iterations need none (the
+    // store would copy equal values); otherwise every iteration stores them.
+    const bool mp = c->mem[0].mp[0] != nullptr;
+    const bool fused_store = mp && c->ice_on && same && !c->mp_dirty;
     if (same) {
         // every 4th iteration is also a moist iteration: the sweep runs beside it
-        if (it % 2 == 0) op_moist_block(c, 0, it % 4 == 0);
+        if (it % 2 == 0) op_moist_block(c, 0, it % 4 == 0, fused_store);
     } else {
         { Prof p(c, KID_POISSON); launch_poisson(cur(c), 4); }
         op_moist_block(c, 2);
@@ -582,7 +589,10 @@ void op_step(athad_ctx* c) {
         Prof p(c, KID_STORE);
         launch_store(cur(c), 0x7FF & ~0xE);   // all but u, v, w (stored by the radial filter)
     }
-    if (c->mem[0].mp[0]) launch_ice_store(cur(c));   // P_rainn, P_snown (UtilsAtm.h:41-42)
+    if (mp && !fused_store) {   // P_rainn, P_snown (UtilsAtm.h:41-42)
+        launch_ice_store(cur(c));
+        c->mp_dirty = false;
+    }
     for (auto& m : c->mem) { m.iter_n++; m.total++; }
 }
 
@@ -928,6 +938,13 @@ athad_status athad_get_radiation(athad_ctx* c, int member, double* radiation, do
 // The ice-scheme arrays are allocated, zero, at the first call that needs them, for every
 // member at once; a 4-iteration graph captured before then did not store the fluxes, so it is
 // dropped and recaptured.
+// The fluxes changed outside a step: the next iteration stores them in full, and a captured
+// graph that stores them only after the scheme is dropped.
+static void mark_mp_dirty(athad_ctx* c) {
+    c->mp_dirty = true;
+    if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+}
+
 static athad_status ensure_mp_buffers(athad_ctx* c) {
     if (c->mem[0].mp[0]) return ATHAD_OK;
     if (c->capturing) { c->err = "the ice-scheme arrays cannot be allocated during capture"; return ATHAD_ERR_STATE; }
@@ -952,6 +969,7 @@ athad_status athad_ice_scheme(athad_ctx* c) {
     if (s) return s;
     join_s2(c);
     { Prof p(c, KID_ICE); launch_ice_scheme(cur(c), 0); }
+    mark_mp_dirty(c);
     return after_launch(c);
 }
 
@@ -972,6 +990,7 @@ athad_status athad_set_mp_field(athad_ctx* ctx, int member, int which, const dou
     if (!src) return ATHAD_ERR_ARG;
     if ((s = ensure_mp_buffers(ctx))) return s;
     join_s2(ctx);
+    mark_mp_dirty(ctx);
     MemberHost& m = ctx->mem[member];
     if (ctx->precision == ATHAD_FP64) {
         HIPCHK(ctx, hipMemcpyAsync(m.mp[which], src, m.n * 8, hipMemcpyHostToDevice, ctx->stream));
@@ -1334,7 +1353,10 @@ athad_status athad_filters(athad_ctx* c) {
 athad_status athad_store(athad_ctx* c) {
     if (!c) return ATHAD_ERR_ARG;
     { Prof p(c, KID_STORE); launch_store(cur(c), 0x7FF); }
-    if (c->mem[0].mp[0]) launch_ice_store(cur(c));
+    if (c->mem[0].mp[0]) {
+        launch_ice_store(cur(c));
+        c->mp_dirty = false;
+    }
     return after_launch(c);
 }
 

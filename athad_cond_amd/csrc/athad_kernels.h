@@ -48,8 +48,9 @@ int radiation_max_levels();
 // must hold their ice-scheme arrays (MemberDesc::mp). Scratch 6-9 are free for it at the place of
 // the ice scheme inside the moist block (the damping has consumed them).
 void launch_ice_scheme(const Batch& b, int gate);
-// precipitationSum, cap_S and the microphysics clamp that follow the ice scheme in the moist block
-void launch_moist_clamp_ice(const Batch& b, int gate);
+// precipitationSum, cap_S and the microphysics clamp that follow the ice scheme in the moist
+// block; store != 0 also copies the fluxes to their stored arrays
+void launch_moist_clamp_ice(const Batch& b, int gate, int store);
 // boundary faces of S_c_c and the five sources on members that hold ice-scheme arrays
 void launch_bc_sources(const Batch& b);
 // P_rainn <- P_rain, P_snown <- P_snow on members that hold ice-scheme arrays.

@@ -621,15 +621,19 @@ __global__ void k_albedo_rows(const MemberDesc* __restrict__ md) {
 // With the ice scheme in the moist block, after its flux damping (cAtmosphereModel.cpp:1263-1399):
 // precipitationSum (UtilsAtm.h:52-67, graupel and the moist convection's P_conv zero), cap_S
 // of the sources and of S_c_c, the microphysics clamp and the water-vapour ceiling.
+// store != 0: also P_rainn <- P_rain, P_snown <- P_snow, the iteration's store of the fluxes
+// (nothing after this point of the iteration changes them).
 template <typename S>
-__global__ __launch_bounds__(256) void k_moist_clamp_ice(const MemberDesc* __restrict__ md, int gate) {
+__global__ __launch_bounds__(256) void k_moist_clamp_ice(const MemberDesc* __restrict__ md, int gate, int store) {
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const Consts& K = M.K;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= K.n) return;
     using C = double;
-    mpp<S>(M, MP_PRECIP)[id] = S(C(mpp<S>(M, MP_P_RAIN)[id]) + C(mpp<S>(M, MP_P_SNOW)[id]) + 0.0 + 0.0);
+    const S pr = mpp<S>(M, MP_P_RAIN)[id], ps = mpp<S>(M, MP_P_SNOW)[id];
+    mpp<S>(M, MP_PRECIP)[id] = S(C(pr) + C(ps) + 0.0 + 0.0);
+    if (store) { mpp<S>(M, MP_P_RAINN)[id] = pr; mpp<S>(M, MP_P_SNOWN)[id] = ps; }
     auto cap = [](C v) { return !finite_bits(v) ? C(0) : (v > C(1.0e-3)) ? C(1.0e-3) : (v < C(-1.0e-3)) ? C(-1.0e-3) : v; };
     S* scc = fp<S>(M, F_S_C_C);
     scc[id] = S(cap(C(scc[id])));
@@ -3042,10 +3046,10 @@ void launch_ice_scheme(const Batch& b, int gate) {
     launch_damp(b, 2, t1, src, 2, 0, gate);
 }
 
-void launch_moist_clamp_ice(const Batch& b, int gate) {
+void launch_moist_clamp_ice(const Batch& b, int gate, int store) {
     const dim3 g((unsigned)((b.max_n + 255) / 256), 1, b.nmem);
-    if (b.fp32) hipLaunchKernelGGL(k_moist_clamp_ice<float>, g, dim3(256), 0, b.stream, b.d_md, gate);
-    else hipLaunchKernelGGL(k_moist_clamp_ice<double>, g, dim3(256), 0, b.stream, b.d_md, gate);
+    if (b.fp32) hipLaunchKernelGGL(k_moist_clamp_ice<float>, g, dim3(256), 0, b.stream, b.d_md, gate, store);
+    else hipLaunchKernelGGL(k_moist_clamp_ice<double>, g, dim3(256), 0, b.stream, b.d_md, gate, store);
 }
 
 void launch_bc_sources(const Batch& b) {

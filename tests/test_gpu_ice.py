@@ -176,3 +176,30 @@ def test_ice_scheme_inside_the_step(oracle, order, grid, steps):
         assert not bad, bad
         for n in ("P_rain", "P_snow", "S_v", "S_r", "S_s"):
             assert np.abs(ref.mp(n)).max() > 0.0, n
+
+
+def test_standalone_call_between_graph_runs(oracle):
+    """With the scheme in the step the fluxes are stored right after it; a standalone call in
+    between changes them outside the step, so the next iterations store them in full (and the
+    captured graph is rebuilt). Checked against the oracle doing the same sequence."""
+    grid = (21, 37, 73)
+    print(f"case ice scheme standalone between graph runs {grid}", flush=True)
+    ref, snap, mp = planted(oracle, grid, 23)
+    ref.set_ice_scheme(True)
+    with A.Integration(grid) as integ:
+        load(integ, ref, snap, mp)
+        integ.set_ice_scheme(True)
+        integ.run(8, graph=True)
+        ref.run(8, "jacobi")
+        integ.ice_scheme()
+        ref.ice_scheme()
+        integ.run(9, graph=True)
+        ref.run(9, "jacobi")
+        e = errors(integ, ref)
+        for n in A.PROGNOSTIC:
+            e[n] = rel(integ.get(n), ref.field(n))
+        print({k: f"{v:.1e}" for k, v in e.items()}, flush=True)
+        bad = {k: v for k, v in e.items() if v > (1e-8 if k == "S_c_c" else 1e-10)}
+        assert not bad, bad
+        np.testing.assert_array_equal(integ.get_mp("P_rainn"), integ.get_mp("P_rain"))
+        np.testing.assert_array_equal(integ.get_mp("P_snown"), integ.get_mp("P_snow"))
