@@ -1489,11 +1489,10 @@ ATHAD_DEV bool assign_S_i_dep(const Level& L, double N_i, double m_i, double& S_
 struct Rates { double Sv, Sc, Si, Sr, Ss, dPr, dPs; };
 
 // Everything else at one level (:312-455) once the carried variables are resolved.
-ATHAD_DEV Rates rates(const Level& L, double t_0, double N_i, double m_i, double S_nuc, double S_i_dep) {
+ATHAD_DEV Rates rates(const Level& L, double t_0, double m_i, double S_nuc, double S_i_dep) {
 #pragma clang fp contract(off)
     const double t_u = L.t_u, cv = L.cv, cld = L.cld, ic = L.ic, Rain = L.Rain, Snow = L.Snow;
     const double dt_snow_dim = L.dt_snow_dim, mass_layer = L.mass_layer;
-    (void)N_i;
     double S_c_frz = (t_u < t_hn) ? cld / dt_snow_dim : 0.0;
     double S_c_au = (cld > q_c_crit) ? c_c_au * (cld - q_c_crit) : 0.0;
     double S_i_au = (ic > 0.0) ? tmax(c_i_au * ic, 0.0) : 0.0;
@@ -1662,7 +1661,7 @@ ATHAD_DEV void ice_column_seq(const MemberDesc& M, int j, int k) {
             assign_Ni_mi(L, K.t_0, N_i, m_i);
             assign_S_nuc(L, N_i, S_nuc);
             assign_S_i_dep(L, N_i, m_i, S_i_dep);
-            const Rates r = rates(L, K.t_0, N_i, m_i, S_nuc, S_i_dep);
+            const Rates r = rates(L, K.t_0, m_i, S_nuc, S_i_dep);
             Sv[id] = S(r.Sv); Sc[id] = S(r.Sc); Si[id] = S(r.Si); Sr[id] = S(r.Sr); Ss[id] = S(r.Ss);
             const double pr = tmin(P_max, tmax(0.0, pr_up + r.dPr));
             const double psn = (L.t_u < K.t_0 && L.t_u >= K.t_000) ? tmin(P_max, tmax(0.0, ps_up + r.dPs)) : 0.0;
@@ -1849,7 +1848,6 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
     const double inv_p = (p_local > 0.0) ? rcp_nr(p_local) : 0.0;
     if (i >= 1 && i <= K.im - 2) {
         // adjustSaturation :91-284
-        const double t_range_inv = 1.0 / (t_0 - t_00);
         const double qv_old = tmax(0.0, c), qc_old = tmax(0.0, cloud), qi_old = tmax(0.0, ice);
         double T = t * t_0;
         if (T < kTcrit) {
