@@ -239,3 +239,32 @@ def test_symmetric_form_stays_mirror_exact_with_the_ice_scheme(oracle):
                 bad[n] = float(np.abs(x - (-m if n in anti else m)).max())
         assert not bad, bad
         assert np.abs(integ.get_mp("P_rain")).max() > 0.0
+
+
+def test_members_at_different_phases_with_the_ice_scheme(oracle):
+    """Members whose iteration counters differ run the moist block in different launches' turns:
+    the cadence gates of the scheme's kernels decide per member, and the fluxes are stored every
+    iteration. Each member still equals its own single run bit for bit."""
+    grids = [(21, 37, 73), (11, 21, 41)]
+    print(f"case ice scheme members at different phases {grids}", flush=True)
+    inputs = [planted(oracle, g, 41 + q) for q, g in enumerate(grids)]
+    phases = [2, 3]
+    single = []
+    for q, (g, (ref, snap, mp)) in enumerate(zip(grids, inputs)):
+        with A.Integration(g) as integ:
+            load(integ, ref, snap, mp)
+            integ.set_iter(phases[q], phases[q])
+            integ.set_ice_scheme(True)
+            integ.run(6, graph=True)
+            single.append({n: integ.get(n) for n in A.PROGNOSTIC} | {n: integ.get_mp(n) for n in L.MP_FIELDS})
+    with A.Integration(grids) as integ:
+        for q, (ref, snap, mp) in enumerate(inputs):
+            load(integ, ref, snap, mp, member=q)
+            integ.set_iter(phases[q], phases[q], member=q)
+        integ.set_ice_scheme(True)
+        integ.run(6, graph=True)
+        for q in range(len(grids)):
+            for n in A.PROGNOSTIC:
+                np.testing.assert_array_equal(integ.get(n, member=q), single[q][n], err_msg=f"{q} {n}")
+            for n in L.MP_FIELDS:
+                np.testing.assert_array_equal(integ.get_mp(n, member=q), single[q][n], err_msg=f"{q} {n}")
