@@ -97,7 +97,8 @@ double athad_dew_point(double e_hPa);
  * ATHAD_ERR_ARG if the sea would boil. */
 athad_status athad_member_from_surface(double p_surf_bar, double t_sea, double x_h2o_dry, double x_co2_dry,
                                        athad_member_params* out);
-/* Device bytes held by a context of this shape (admission control). */
+/* Device bytes held by a context of this shape (admission control); the ice-scheme arrays
+ * (10 fields) and the radiation outputs (2 fields) come on top once first used. */
 int64_t athad_footprint_bytes(const athad_grid* grid, int precision);
 
 /* --- state transfer ---------------------------------------------------------------- */

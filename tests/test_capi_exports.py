@@ -32,6 +32,12 @@ def test_errors_without_context():
     import athad_cond_amd as A
     L = A.lib()
     assert L.athad_step(None) == 1          # ATHAD_ERR_ARG, no device touched
+    assert L.athad_ice_scheme(None) == 1
+    assert L.athad_set_ice_scheme(None, 1) == 1
+    buf = (ctypes.c_double * 8)()
+    assert L.athad_get_mp_field(None, 0, 0, buf) == 1
+    assert L.athad_set_mp_field(None, 0, 0, buf) == 1
+    assert L.athad_radiation(None) == 1
     g = A._lib.Grid(41, 180, 361)           # even jm is rejected before any device call
     h = ctypes.c_void_p()
     assert L.athad_create(1, ctypes.byref(g), None, 0, 1, ctypes.byref(h)) == 4
