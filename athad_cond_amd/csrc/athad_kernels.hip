@@ -410,65 +410,105 @@ __global__ __launch_bounds__(256) void k_slice(const MemberDesc* __restrict__ md
 constexpr int kRadCols = 16;
 
 // Inclusive composition scan across the 64 lanes: after it, lane l holds the composition of
-// the maps of lanes 0..l (upward) or l..63 (downward), the nearer lane's map applied last.
-ATHAD_DEV void affine_scan_up(double& A, double& B) {
+// the maps of lanes 0..l (upward) or l..63 (downward), the nearer lane's map applied last. A
+// step composes B = A B' + B and A = A A' with the partner's (A', B'). The A part depends only
+// on the emissivities, which are fixed for a column, so its per-step values are formed once
+// (rad_prep) and each of the five sweeps shuffles only B.
+struct RadScan { double As_up[6], As_dn[6]; };
+
+ATHAD_DEV void scan_A_up(double A, double (&As)[6]) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const double Ap = __shfl_up(A, d), Bp = __shfl_up(B, d);
-        if (lane >= d) { B = A * Bp + B; A = A * Ap; }
+    for (int s = 0, d = 1; s < 6; s++, d <<= 1) {
+        As[s] = A;
+        const double Ap = __shfl_up(A, d);
+        if (lane >= d) A = A * Ap;
     }
 }
-ATHAD_DEV void affine_scan_down(double& A, double& B) {
+ATHAD_DEV void scan_A_down(double A, double (&As)[6]) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const double An = __shfl_down(A, d), Bn = __shfl_down(B, d);
-        if (lane + d < 64) { B = A * Bn + B; A = A * An; }
+    for (int s = 0, d = 1; s < 6; s++, d <<= 1) {
+        As[s] = A;
+        const double An = __shfl_down(A, d);
+        if (lane + d < 64) A = A * An;
     }
+}
+ATHAD_DEV double scan_B_up(double B, const double (&As)[6]) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 0, d = 1; s < 6; s++, d <<= 1) {
+        const double Bp = __shfl_up(B, d);
+        if (lane >= d) B = As[s] * Bp + B;
+    }
+    return B;
+}
+ATHAD_DEV double scan_B_down(double B, const double (&As)[6]) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 0, d = 1; s < 6; s++, d <<= 1) {
+        const double Bn = __shfl_down(B, d);
+        if (lane + d < 64) B = As[s] * Bn + B;
+    }
+    return B;
 }
 
-// Both sweeps of one column (MultiLayerRadiation.h:291-303): up from the black surface (level 0
-// is the constant map sigma T_0^4), dn from a dark sky (level 0 is the identity, so dn at
-// level 0 equals dn[1], the downwelling flux at the ground). Levels at or above im are identities.
+// The per-level maps of the two sweeps (MultiLayerRadiation.h:291-303): up from the black
+// surface (level 0 is the constant map sigma T_0^4), dn from a dark sky (level 0 is the
+// identity, so dn at level 0 equals dn[1], the downwelling flux at the ground). Levels at or
+// above im are identities. a_up / a_dn: the slopes; the scans' A parts.
 template <int E>
-ATHAD_DEV void rad_sweeps(int lane, int im, const double (&eps)[E], const double (&P4)[E], double sigma,
-                          double (&up)[E], double (&dn)[E]) {
+ATHAD_DEV void rad_prep(int lane, int im, const double (&eps)[E], double (&a_up)[E], double (&a_dn)[E], RadScan& R) {
 #pragma clang fp contract(off)
-    double a[E], b[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int i = lane * E + e;
-        if (i >= im) { a[e] = 1.0; b[e] = 0.0; }
-        else if (i == 0) { a[e] = 0.0; b[e] = sigma * P4[e]; }
-        else { a[e] = 1.0 - eps[e]; b[e] = eps[e] * sigma * P4[e]; }
+        a_up[e] = (i >= im) ? 1.0 : (i == 0) ? 0.0 : 1.0 - eps[e];
+        a_dn[e] = (i == 0) ? 1.0 : a_up[e];
     }
-    double A = 1.0, B = 0.0;
+    double A = 1.0;
 #pragma unroll
-    for (int e = 0; e < E; e++) { B = a[e] * B + b[e]; A = a[e] * A; }
-    affine_scan_up(A, B);
+    for (int e = 0; e < E; e++) A = a_up[e] * A;
+    scan_A_up(A, R.As_up);
+    A = 1.0;
+#pragma unroll
+    for (int e = E - 1; e >= 0; e--) A = a_dn[e] * A;
+    scan_A_down(A, R.As_dn);
+}
+
+// Both sweeps of one column for the current sigma T^4.
+template <int E>
+ATHAD_DEV void rad_sweeps(int lane, int im, const double (&eps)[E], const double (&P4)[E], double sigma,
+                          const double (&a_up)[E], const double (&a_dn)[E], const RadScan& R,
+                          double (&up)[E], double (&dn)[E]) {
+#pragma clang fp contract(off)
+    double b[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int i = lane * E + e;
+        b[e] = (i >= im) ? 0.0 : (i == 0) ? sigma * P4[e] : eps[e] * sigma * P4[e];
+    }
+    double B = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; e++) B = a_up[e] * B + b[e];
+    B = scan_B_up(B, R.As_up);
     double x = __shfl_up(B, 1);
     if (lane == 0) x = 0.0;
 #pragma unroll
-    for (int e = 0; e < E; e++) { x = a[e] * x + b[e]; up[e] = x; }
-    if (lane == 0) { a[0] = 1.0; b[0] = 0.0; }
-    A = 1.0; B = 0.0;
+    for (int e = 0; e < E; e++) { x = a_up[e] * x + b[e]; up[e] = x; }
+    if (lane == 0) b[0] = 0.0;
+    B = 0.0;
 #pragma unroll
-    for (int e = E - 1; e >= 0; e--) { B = a[e] * B + b[e]; A = a[e] * A; }
-    affine_scan_down(A, B);
+    for (int e = E - 1; e >= 0; e--) B = a_dn[e] * B + b[e];
+    B = scan_B_down(B, R.As_dn);
     x = __shfl_down(B, 1);
     if (lane == 63) x = 0.0;
 #pragma unroll
-    for (int e = E - 1; e >= 0; e--) { x = a[e] * x + b[e]; dn[e] = x; }
-}
-
-template <int E> ATHAD_DEV double pick(const double (&v)[E], int e) {
-    double r = v[0];
-#pragma unroll
-    for (int q = 1; q < E; q++) r = (e == q) ? v[q] : r;
-    return r;
+    for (int e = E - 1; e >= 0; e--) { x = a_dn[e] * x + b[e]; dn[e] = x; }
 }
 
 template <typename S, int E>
@@ -539,22 +579,36 @@ __global__ __launch_bounds__(256) void k_radiation(const MemberDesc* __restrict_
             }
             P4[e] = (T[e] * T[e]) * (T[e] * T[e]);
         }
-        // condensate paths in level order, then the cloud-raised albedo (MultiLayerRadiation.h:231-238)
-        double lwp = 0.0, iwp = 0.0;
-        for (int i = 0; i < im; i++) {
-            const double l = __shfl(pick(lw, i % E), i / E), w = __shfl(pick(iw, i % E), i / E);
-            lwp += l;
-            iwp += w;
+        // condensate paths in level order, then the cloud-raised albedo (MultiLayerRadiation.h:231-238):
+        // every lane leaves its levels' paths in the column's cloud and ice slots (read above,
+        // not needed again) and lane 0 sums them in level order; the albedo is lane 0's only
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int i = lane * E + e;
+            if (i < im) { slot(4, i, kk) = lw[e]; slot(5, i, kk) = iw[e]; }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         double alb = K.albedo_surface;
-        if (K.albedo_cloud > alb) {
-            const double tau = (lwp + 0.5 * iwp) / 100.0;
-            alb = alb + (K.albedo_cloud - alb) * (tau / (tau + 2.0));
+        if (lane == 0) {
+            double lwp = 0.0, iwp = 0.0;
+            for (int i = 0; i < im; i++) {
+                lwp += slot(4, i, kk);
+                iwp += slot(5, i, kk);
+            }
+            if (K.albedo_cloud > alb) {
+                const double tau = (lwp + 0.5 * iwp) / 100.0;
+                alb = alb + (K.albedo_cloud - alb) * (tau / (tau + 2.0));
+            }
         }
         // four Lambda iterations of sigma T_i^4 = (up[i-1] + dn[i+1]) / 2, then final sweeps
         // (MultiLayerRadiation.h:287-333)
+        double a_up[E], a_dn[E];
+        RadScan R;
+        rad_prep<E>(lane, im, eps, a_up, a_dn, R);
         for (int it = 0; it < 4; it++) {
-            rad_sweeps<E>(lane, im, eps, P4, sigma, up, dn);
+            rad_sweeps<E>(lane, im, eps, P4, sigma, a_up, a_dn, R, up, dn);
             const double up_prev_lane = __shfl_up(up[E - 1], 1);
             double dn_next_lane = __shfl_down(dn[0], 1);
             if (lane == 63) dn_next_lane = 0.0;
@@ -571,7 +625,7 @@ __global__ __launch_bounds__(256) void k_radiation(const MemberDesc* __restrict_
                 }
             }
         }
-        rad_sweeps<E>(lane, im, eps, P4, sigma, up, dn);
+        rad_sweeps<E>(lane, im, eps, P4, sigma, a_up, a_dn, R, up, dn);
         // surface energy balance with sigma T^4 linearised at the current surface temperature
         // (MultiLayerRadiation.h:338-371); dn at level 0 is dn[1]
         const double T1 = (E > 1) ? T[E > 1 ? 1 : 0] : __shfl_down(T[0], 1);
@@ -607,15 +661,26 @@ __global__ __launch_bounds__(256) void k_radiation(const MemberDesc* __restrict_
 }
 
 // Per latitude, the zonal sum of the albedo in k order (planetaryShortWave's inner loop,
-// cAtmosphereModel.cpp:248-252), at sf_base.
-__global__ void k_albedo_rows(const MemberDesc* __restrict__ md) {
+// cAtmosphereModel.cpp:248-252), at sf_base. One block per latitude stages the row in LDS with
+// coalesced loads, then one thread adds it up in k order.
+constexpr int kAlbedoRowMax = 4096;
+__global__ __launch_bounds__(256) void k_albedo_rows(const MemberDesc* __restrict__ md) {
 #pragma clang fp contract(off)
+    __shared__ double s_row[kAlbedoRowMax];
     const MemberDesc& M = md[blockIdx.z];
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.x, km = M.K.km;
     if (j >= M.K.jm) return;
+    const double* row = M.albedo + (long long)j * km;
     double a = 0.0;
-    for (int k = 0; k < M.K.km; k++) a += M.albedo[(long long)j * M.K.km + k];
-    M.scratch[sf_base(M.K) + j] = a;
+    for (int k0 = 0; k0 < km; k0 += kAlbedoRowMax) {   // rows longer than the stage in pieces
+        const int n = min(kAlbedoRowMax, km - k0);
+        for (int q = threadIdx.x; q < n; q += blockDim.x) s_row[q] = row[k0 + q];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int q = 0; q < n; q++) a += s_row[q];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) M.scratch[sf_base(M.K) + j] = a;
 }
 
 // With the ice scheme in the moist block, after its flux damping (cAtmosphereModel.cpp:1263-1399):
@@ -3020,7 +3085,7 @@ void launch_radiation(const Batch& b) {
         if (b.fp32) hipLaunchKernelGGL((k_radiation<float, 2>), g, dim3(256), lds, b.stream, b.d_md);
         else hipLaunchKernelGGL((k_radiation<double, 2>), g, dim3(256), lds, b.stream, b.d_md);
     }
-    hipLaunchKernelGGL(k_albedo_rows, dim3((b.max_jm + 63) / 64, 1, b.nmem), dim3(64), 0, b.stream, b.d_md);
+    hipLaunchKernelGGL(k_albedo_rows, dim3(b.max_jm, 1, b.nmem), dim3(256), 0, b.stream, b.d_md);
 }
 // 80 levels keep the tile's LDS (6 fields x levels x 16 columns) within 64 KB.
 int radiation_max_levels() { return 80; }

@@ -81,6 +81,42 @@ def run_ice(A, precision, steps, warmup, seed=1234, profile_steps=8, graph=True)
             "ice_ms_per_call": ice_ms / max(ice_n, 1), "calls_profiled": ice_n}
 
 
+def run_next_rows(A, reps=10, seed=1234, warmup=8):
+    """Wall time per call of the operators built beside the step, on the FP64 state at this
+    grid after `warmup` iterations: mode-2 radiation (with the host skin-temperature step),
+    the two-category ice scheme alone, the checkpoint diagnostics and a restart image in
+    memory. Each call returns synchronised; radiation's HBM rate counts its 9 words per cell
+    (t, c, cloud, ice, co2, p_stat read; t, radiation, epsilon written)."""
+    n = GRID[0] * GRID[1] * GRID[2]
+    out = {}
+    with A.Integration(GRID) as integ:
+        integ.init_state(seed)
+        integ.run(warmup, graph=True)
+        integ.sync()
+
+        def timed(name, fn):
+            fn()
+            integ.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            integ.sync()
+            out[name] = {"ms_per_call": (time.perf_counter() - t0) / reps * 1e3}
+
+        timed("radiation", integ.radiation)
+        r = out["radiation"]
+        r["hbm_GBps"] = 9 * 8 * n / (r["ms_per_call"] * 1e-3) / 1e9
+        r["frac_of_hbm_peak"] = r["hbm_GBps"] / HBM_PEAK_GBS
+        timed("ice_scheme", integ.ice_scheme)
+        timed("residuum", integ.residuum)
+        timed("field_minmax", lambda: integ.field_minmax("t"))
+        timed("meridional_streamfunction", integ.meridional_streamfunction)
+        timed("slice", lambda: integ.get_slice("t", 0, GRID[0] // 2))
+        timed("restart_image", integ.save_state)
+    out["workload"] = f"FP64 state at 41x181x361 after {warmup} iterations; {reps} calls each"
+    return out
+
+
 def measured_traffic():
     """HBM bytes per k_rk4_jacobi<double> launch at this grid, as recorded by the FETCH_SIZE and
     WRITE_SIZE counter runs (scaled by the copy-kernel calibration) in the committed profile
@@ -147,6 +183,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed and lex forms")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of the hipGraph")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a plain copy of known size")
+    ap.add_argument("--next-rows-only", action="store_true",
+                    help="only the per-call times of radiation, the ice scheme, diagnostics and restart")
     ap.add_argument("--ice-only", action="store_true",
                     help="only the FP64 step with the two-category ice scheme in the moist block")
     ap.add_argument("--mixed-only", action="store_true",
@@ -164,6 +202,9 @@ def main():
 
     n = GRID[0] * GRID[1] * GRID[2]
     graph = not args.no_graph
+    if args.next_rows_only:
+        print(json.dumps({"grid": list(GRID), **run_next_rows(A)}))
+        return
     if args.ice_only:
         r = run_ice(A, "fp64", args.steps, args.warmup, graph=graph)
         print(json.dumps({"form": "fp64 with the ice scheme", "grid": list(GRID), "steps": args.steps,
@@ -225,6 +266,7 @@ def main():
         out["ice_scheme"] = run_ice(A, "fp64", args.steps, args.warmup, graph=graph)
         out["ice_scheme"]["workload"] = ("FP64 Jacobi step at 41x181x361 with the two-category ice scheme "
                                          "in the moist block and its sources in RK4")
+        out["next_rows"] = run_next_rows(A)
         from athad_cond_amd.ensemble import run_ensemble
         out["ensemble"] = {prec: run_ensemble(32, (41, 91, 181), steps=40, precision=prec)
                            for prec in ("fp64", "mixed")}
