@@ -42,7 +42,7 @@ EXPORTS = [
     "athad_field_stats", "athad_set_member_params", "athad_restart_bytes", "athad_save_state_mem",
     "athad_load_state_mem", "athad_save_state", "athad_load_state", "athad_field_minmax", "athad_residuum",
     "athad_meridional_streamfunction", "athad_get_slice", "athad_radiation", "athad_get_radiation",
-    "athad_ice_scheme", "athad_set_ice_scheme", "athad_set_mp_field", "athad_get_mp_field",
+    "athad_set_radiation", "athad_ice_scheme", "athad_set_ice_scheme", "athad_set_mp_field", "athad_get_mp_field",
 ]
 
 # Arrays of the two-category ice scheme (athad_mp_field in include/athad_hip.h)
@@ -130,6 +130,7 @@ def lib():
         L.athad_radiation.argtypes = [vp]
         L.athad_get_radiation.argtypes = [vp, i, dp, dp, dp, dp]
         L.athad_ice_scheme.argtypes = [vp]
+        L.athad_set_radiation.argtypes = [vp, i]
         L.athad_set_ice_scheme.argtypes = [vp, i]
         L.athad_set_mp_field.argtypes = [vp, i, i, dp]
         L.athad_get_mp_field.argtypes = [vp, i, i, dp]

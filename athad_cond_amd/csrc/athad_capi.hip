@@ -8,6 +8,7 @@
 #include "athad_kernels.h"
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -123,6 +124,9 @@ struct athad_ctx {
     bool ice_on = false;
     // the fluxes may differ from their stored copies (changed outside a step's moist block)
     bool mp_dirty = false;
+    // radiation (mode 2) and the skin-temperature step after every iteration's store; the
+    // device descriptors then hold a newer t_skin than the host mirror (tskin_dev)
+    bool rad_on = false, tskin_dev = false;
 };
 
 namespace {
@@ -276,7 +280,22 @@ void make_desc(const athad_ctx* c, const MemberHost& m, int parity, MemberDesc& 
     (void)c;
 }
 
+// Brings the host mirror of t_skin up to the descriptors' value where the step's radiation has
+// moved it on the device.
+athad_status pull_tskin(athad_ctx* c) {
+    if (!c->tskin_dev) return ATHAD_OK;
+    for (int q = 0; q < c->nmem; q++) {
+        const char* src = (const char*)(c->d_md[0] + q) + offsetof(MemberDesc, K) + offsetof(Consts, t_skin);
+        HIPCHK(c, hipMemcpyAsync(&c->mem[q].K.t_skin, src, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->tskin_dev = false;
+    return ATHAD_OK;
+}
+
 athad_status upload_descs(athad_ctx* c) {
+    athad_status s0 = pull_tskin(c);
+    if (s0) return s0;
     for (int p = 0; p < 2; p++) {
         c->h_md[p].resize(c->nmem);
         for (int q = 0; q < c->nmem; q++) make_desc(c, c->mem[q], p, c->h_md[p][q]);
@@ -593,6 +612,13 @@ void op_step(athad_ctx* c) {
         launch_ice_store(cur(c));
         c->mp_dirty = false;
     }
+    if (c->rad_on) {
+        // radiation_mode 2 after the store (cAtmosphereModel.cpp:1655-1661): touches t and the
+        // radiation outputs, none of what the filters on the second stream read or write
+        launch_radiation(cur(c));
+        launch_skin_step(cur(c), c->d_md[0], c->d_md[1], Config().t_skin_relax);
+        c->tskin_dev = true;
+    }
     for (auto& m : c->mem) { m.iter_n++; m.total++; }
 }
 
@@ -888,6 +914,7 @@ athad_status athad_radiation(athad_ctx* c) {
         }
     athad_status s = ensure_radiation_buffers(c);
     if (s) return s;
+    if ((s = pull_tskin(c))) return s;
     join_s2(c);
     launch_radiation(cur(c));
     // updateSkinTemperature (cAtmosphereModel.cpp:276-311) with planetaryShortWave (:241-270)
@@ -920,10 +947,27 @@ athad_status athad_radiation(athad_ctx* c) {
     return changed ? upload_descs(c) : ATHAD_OK;
 }
 
+athad_status athad_set_radiation(athad_ctx* c, int on) {
+    if (!c) return ATHAD_ERR_ARG;
+    if (on) {
+        for (auto& m : c->mem)
+            if (m.g.im > radiation_max_levels()) {
+                c->err = "radiation supports up to " + std::to_string(radiation_max_levels()) + " levels";
+                return ATHAD_ERR_SHAPE;
+            }
+        athad_status s = ensure_radiation_buffers(c);
+        if (s) return s;
+    }
+    if (c->rad_on != (on != 0) && c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    c->rad_on = on != 0;
+    return ATHAD_OK;
+}
+
 athad_status athad_get_radiation(athad_ctx* c, int member, double* radiation, double* epsilon, double* albedo,
                                  double* t_skin) {
     athad_status s = check_member(c, member, 0, 1);
     if (s) return s;
+    if ((s = pull_tskin(c))) return s;
     const MemberHost& m = c->mem[member];
     if (t_skin) *t_skin = m.K.t_skin;
     if (!m.radiation && (radiation || epsilon || albedo)) { c->err = "radiation has not run on this context"; return ATHAD_ERR_STATE; }
@@ -935,15 +979,16 @@ athad_status athad_get_radiation(athad_ctx* c, int member, double* radiation, do
     return ATHAD_OK;
 }
 
-// The ice-scheme arrays are allocated, zero, at the first call that needs them, for every
-// member at once; a 4-iteration graph captured before then did not store the fluxes, so it is
-// dropped and recaptured.
 // The fluxes changed outside a step: the next iteration stores them in full, and a captured
 // graph that stores them only after the scheme is dropped.
 static void mark_mp_dirty(athad_ctx* c) {
     c->mp_dirty = true;
     if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
 }
+
+// The ice-scheme arrays are allocated, zero, at the first call that needs them, for every
+// member at once; a 4-iteration graph captured before then did not store the fluxes, so it is
+// dropped and recaptured.
 
 static athad_status ensure_mp_buffers(athad_ctx* c) {
     if (c->mem[0].mp[0]) return ATHAD_OK;
@@ -1032,6 +1077,7 @@ athad_status athad_set_member_params(athad_ctx* c, int member, const athad_membe
     MemberHost& m = c->mem[member];
     m.p = *p;
     const Config cfg;
+    c->tskin_dev = false;   // the member's constants, t_skin included, are reset
     fill_consts(m, cfg, c->order);
     std::vector<double> tab;
     std::vector<int> passes;
@@ -1272,6 +1318,7 @@ athad_status athad_load_state(athad_ctx* ctx, int member, const char* path) {
 athad_status athad_init_state(athad_ctx* c, uint64_t seed) {
     if (!c) return ATHAD_ERR_ARG;
     // member seeds; a new integration starts from the configured t_skin
+    c->tskin_dev = false;   // superseded by the reset below
     for (int q = 0; q < c->nmem; q++) {
         c->mem[q].K.seed = seed + (uint64_t)q;
         c->mem[q].K.t_skin = Config().t_skin;
@@ -1381,6 +1428,7 @@ athad_status athad_run(athad_ctx* c, int n_steps, int use_graph) {
         if (c->graph && phase == c->graph_phase && c->parity == c->graph_parity && n_steps - done >= 4) {
             join_s2(c);
             HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+            if (c->rad_on) c->tskin_dev = true;
             for (auto& m : c->mem) { m.iter_n += 4; m.total += 4; }
             done += 4;
             continue;

@@ -43,6 +43,9 @@ int sf_offset(int im);
 // per latitude at scratch[sf_offset(im)] (jm doubles). Grids up to radiation_max_levels() levels.
 void launch_radiation(const Batch& b);
 int radiation_max_levels();
+// updateSkinTemperature on the device after launch_radiation, every member; the new t_skin is
+// written into both descriptor arrays md0, md1
+void launch_skin_step(const Batch& b, MemberDesc* md0, MemberDesc* md1, double relax);
 // The two-category ice scheme on every member (initArrays, the column passes, extrapolateBC of
 // the fluxes), then the damp_wiggles of P_rain and P_snow through scratch buffers 0-3. Members
 // must hold their ice-scheme arrays (MemberDesc::mp). Scratch 6-9 are free for it at the place of

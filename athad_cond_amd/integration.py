@@ -185,6 +185,10 @@ class Integration:
         """MultiLayerRadiation + the 5 % blend of t + one t_skin relaxation, every member."""
         self._chk(L.lib().athad_radiation(self.h))
 
+    def set_radiation(self, on: bool = True):
+        """Radiation (mode 2) and the skin-temperature step after every later step's store."""
+        self._chk(L.lib().athad_set_radiation(self.h, 1 if on else 0))
+
     def radiation_outputs(self, member: int = 0) -> dict:
         im, jm, km = self.grids[member]
         out = {"radiation": np.empty((im, jm, km)), "epsilon": np.empty((im, jm, km)), "albedo": np.empty((jm, km))}

@@ -248,6 +248,12 @@ athad_status athad_get_slice(athad_ctx* ctx, int member, int field, int axis, in
  * flux, emissivity, albedo) are kept on the device; the first call allocates them. Grids up to
  * 80 levels (ATHAD_ERR_SHAPE above). athad_init_state resets t_skin to its configured value. */
 athad_status athad_radiation(athad_ctx* ctx);
+/* radiation_mode 2 inside the step (on != 0): every later athad_step / athad_run iteration ends,
+ * after its store, with the radiation branch of run_3D_loop (:1655-1661) as athad_radiation
+ * runs it, the skin-temperature step included, on the device, so captured graphs carry it and no
+ * host round trip is needed. athad_get_radiation reports the t_skin reached. on = 0 stops it.
+ * ATHAD_ERR_SHAPE above 80 levels. */
+athad_status athad_set_radiation(athad_ctx* ctx, int on);
 /* The radiation outputs of one member: radiation and epsilon (im*jm*km doubles, [i][j][k]),
  * albedo (jm*km) and the current t_skin in K; any may be NULL. ATHAD_ERR_STATE for the arrays
  * before the first athad_radiation. */

@@ -141,3 +141,60 @@ def test_radiation_errors():
         integ.init_state(1)
         with pytest.raises(A.AthadError, match="status 4"):
             integ.radiation()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_radiation_inside_the_step(oracle, graph):
+    """athad_set_radiation: the radiation branch and the skin-temperature step run on the device
+    after every iteration's store (hipGraph replay included); the same as calling
+    athad_radiation after each step, and as the oracle."""
+    grid = (21, 37, 73)
+    print(f"case radiation inside the step graph={graph} {grid}", flush=True)
+    ref = oracle.OracleState(*grid, seed=9)
+    with A.Integration(grid) as integ:
+        integ.init_state(9)
+        integ.set_radiation(True)
+        integ.run(16, graph=graph)
+        for _ in range(16):
+            ref.step("jacobi")
+            ref.radiation()
+        e = check(integ, ref, 1e-10, "16 iterations in the step", fields=A.PROGNOSTIC + ["p_stat", "r_humid"])
+        print({k: f"{v:.1e}" for k, v in e.items()})
+        assert abs(integ.radiation_outputs()["t_skin"] - 254.0) > 1.0
+        # switching it off leaves t_skin where the device moved it and stops the updates
+        integ.set_radiation(False)
+        ts = integ.radiation_outputs()["t_skin"]
+        integ.run(4, graph=graph)
+        assert integ.radiation_outputs()["t_skin"] == ts
+
+
+def test_ice_scheme_and_radiation_inside_the_step(oracle):
+    """Both next operators inside the step: the moist block with the ice scheme, then after the
+    store the radiation; the whole iteration stays on the device."""
+    grid = (21, 37, 73)
+    print(f"case ice scheme and radiation inside the step {grid}", flush=True)
+    ref = oracle.OracleState(*grid, seed=5)
+    ref.run(2, "jacobi")
+    snap = ref.snapshot()
+    mp = {k: np.zeros(grid) for k in oracle.MP_FIELDS}
+    oracle.plant_microphysics(snap, mp, 6)
+    ref.load(snap)
+    for k, v in mp.items():
+        ref.mp(k)[...] = v
+    ref.set_ice_scheme(True)
+    with A.Integration(grid) as integ:
+        integ.load(snap)
+        integ.set_iter(2, ref.total_iter)
+        for k, v in mp.items():
+            integ.set_mp(k, v)
+        integ.set_ice_scheme(True)
+        integ.set_radiation(True)
+        integ.run(8, graph=True)
+        for _ in range(8):
+            ref.step("jacobi")
+            ref.radiation()
+        e = check(integ, ref, 1e-10, "ice + radiation", fields=A.PROGNOSTIC + ["p_stat", "r_humid"])
+        for n in ("P_rain", "P_snow", "S_v"):
+            e[n] = rel(integ.get_mp(n), ref.mp(n))
+        print({k: f"{v:.1e}" for k, v in e.items()})
+        assert max(e[n] for n in ("P_rain", "P_snow", "S_v")) < 1e-10
