@@ -753,27 +753,41 @@ __global__ __launch_bounds__(256) void k_bc_sources(const MemberDesc* __restrict
 }
 
 // updateSkinTemperature (cAtmosphereModel.cpp:276-311) with planetaryShortWave (:241-270) on the
-// zonal albedo sums k_albedo_rows left at sf_base, on the device for radiation inside the step:
-// one thread per member with the host form's arithmetic (athad_radiation); the new t_skin goes
-// into both descriptor arrays, where the next densities reads it.
-__global__ void k_skin_step(MemberDesc* md0, MemberDesc* md1, double relax) {
+// zonal albedo sums k_albedo_rows left at sf_base, on the device for radiation inside the step,
+// one block per member: the threads form each latitude's weight, short wave and mean albedo
+// in LDS, one thread accumulates them in j order with the host form's arithmetic
+// (athad_radiation); the new t_skin goes into both descriptor arrays, where the next densities
+// reads it.
+constexpr int kSkinChunk = 1024;
+__global__ __launch_bounds__(256) void k_skin_step(MemberDesc* md0, MemberDesc* md1, double relax) {
 #pragma clang fp contract(off)
-    if (threadIdx.x != 0 || !(relax > 0.0)) return;
+    __shared__ double s_w[kSkinChunk], s_sw[kSkinChunk], s_a[kSkinChunk];
+    if (!(relax > 0.0)) return;
     const int q = blockIdx.x;
     const Consts& K = md0[q].K;
     const int jm = K.jm, km = K.km, jmax = jm - 1, jhalf = jmax / 2;
     const double* rows = md0[q].scratch + sf_base(K);
     double wsum = 0.0, absorbed = 0.0, amax = 0.0;
-    for (int j = 0; j < jm; j++) {
-        const double x = (double)((j <= jhalf) ? j : jmax - j) / (double)jhalf;
-        const double sw = (K.sw_pole - K.sw_equator) * (x * x - 2.0 * x) + K.sw_pole;
-        const double w = cos((j / (double)(jm - 1) - 0.5) * M_PI);
-        const double a = rows[j] / (double)km;
-        amax = tmax(amax, a);
-        wsum += w;
-        absorbed += w * (1.0 - a) * sw;
+    for (int j0 = 0; j0 < jm; j0 += kSkinChunk) {
+        const int n = min(kSkinChunk, jm - j0);
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            const int j = j0 + t;
+            const double x = (double)((j <= jhalf) ? j : jmax - j) / (double)jhalf;
+            s_sw[t] = (K.sw_pole - K.sw_equator) * (x * x - 2.0 * x) + K.sw_pole;
+            s_w[t] = cos((j / (double)(jm - 1) - 0.5) * M_PI);
+            s_a[t] = rows[j] / (double)km;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int t = 0; t < n; t++) {
+                const double w = s_w[t], a = s_a[t];
+                amax = tmax(amax, a);
+                wsum += w;
+                absorbed += w * (1.0 - a) * s_sw[t];
+            }
+        __syncthreads();
     }
-    if (!(wsum > 0.0) || !(amax > 0.0)) return;
+    if (threadIdx.x != 0 || !(wsum > 0.0) || !(amax > 0.0)) return;
     const double in = absorbed / wsum + K.geothermal;
     if (!(in > 0.0)) return;
     const double target = pow(in / K.sigma, 0.25);
@@ -3118,7 +3132,7 @@ void launch_radiation(const Batch& b) {
     hipLaunchKernelGGL(k_albedo_rows, dim3(b.max_jm, 1, b.nmem), dim3(256), 0, b.stream, b.d_md);
 }
 void launch_skin_step(const Batch& b, MemberDesc* md0, MemberDesc* md1, double relax) {
-    hipLaunchKernelGGL(k_skin_step, dim3(b.nmem), dim3(64), 0, b.stream, md0, md1, relax);
+    hipLaunchKernelGGL(k_skin_step, dim3(b.nmem), dim3(256), 0, b.stream, md0, md1, relax);
 }
 // 80 levels keep the tile's LDS (6 fields x levels x 16 columns) within 64 KB.
 int radiation_max_levels() { return 80; }

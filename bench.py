@@ -58,6 +58,25 @@ def run_form(A, precision, order, steps, warmup, seed=1234, profile_steps=0, gra
             "rk4_launches": rk4_n, "fields": fields}
 
 
+def run_loop(A, precision, steps, warmup, seed=1234, graph=True):
+    """The iteration with both operators built beside the scoped step inside it: the
+    two-category ice scheme in the moist block and mode-2 radiation with the skin-temperature
+    step after the store (run_3D_loop without moist convection and output)."""
+    n = GRID[0] * GRID[1] * GRID[2]
+    with A.Integration(GRID, precision=precision) as integ:
+        integ.init_state(seed)
+        integ.set_ice_scheme(True)
+        integ.set_radiation(True)
+        integ.run(warmup, graph=graph)
+        integ.sync()
+        t0 = time.perf_counter()
+        integ.run(steps, graph=graph)
+        integ.sync()
+        dt = time.perf_counter() - t0
+        t_skin = integ.radiation_outputs()["t_skin"]
+    return {"ms_per_step": dt / steps * 1e3, "cells_per_s": n * steps / dt, "t_skin_end": t_skin}
+
+
 def run_ice(A, precision, steps, warmup, seed=1234, profile_steps=8, graph=True):
     """The same workload with the two-category ice scheme inside the moist block
     (CategoryIceScheme = 2): its sources in RK4, its fluxes stored every iteration. Times the
@@ -185,6 +204,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a plain copy of known size")
     ap.add_argument("--next-rows-only", action="store_true",
                     help="only the per-call times of radiation, the ice scheme, diagnostics and restart")
+    ap.add_argument("--loop-only", action="store_true",
+                    help="only the FP64 iteration with the ice scheme and radiation inside it")
     ap.add_argument("--ice-only", action="store_true",
                     help="only the FP64 step with the two-category ice scheme in the moist block")
     ap.add_argument("--mixed-only", action="store_true",
@@ -204,6 +225,11 @@ def main():
     graph = not args.no_graph
     if args.next_rows_only:
         print(json.dumps({"grid": list(GRID), **run_next_rows(A)}))
+        return
+    if args.loop_only:
+        r = run_loop(A, "fp64", args.steps, args.warmup, graph=graph)
+        print(json.dumps({"form": "fp64 with the ice scheme and radiation", "grid": list(GRID),
+                          "steps": args.steps, "warmup": args.warmup, **r}))
         return
     if args.ice_only:
         r = run_ice(A, "fp64", args.steps, args.warmup, graph=graph)
@@ -266,6 +292,10 @@ def main():
         out["ice_scheme"] = run_ice(A, "fp64", args.steps, args.warmup, graph=graph)
         out["ice_scheme"]["workload"] = ("FP64 Jacobi step at 41x181x361 with the two-category ice scheme "
                                          "in the moist block and its sources in RK4")
+        out["loop_with_ice_and_radiation"] = run_loop(A, "fp64", args.steps, args.warmup, graph=graph)
+        out["loop_with_ice_and_radiation"]["workload"] = (
+            "FP64 Jacobi iteration at 41x181x361 with the ice scheme in the moist block and radiation "
+            "with the skin-temperature step after the store, all on the device (hipGraph)")
         out["next_rows"] = run_next_rows(A)
         from athad_cond_amd.ensemble import run_ensemble
         out["ensemble"] = {prec: run_ensemble(32, (41, 91, 181), steps=40, precision=prec)
