@@ -1662,14 +1662,22 @@ ATHAD_DEV Rates rates(const Level& L, double t_0, double m_i, double S_nuc, doub
 }
 
 // The level's inputs; false where no condensed phase can exist (IceSchemeCommon.h:104-107).
+// cached: the level's q_sw and q_Ice from an earlier pass in which it could condense (its t and
+// c have not changed since), passed in q_cached[2]; else null.
 ATHAD_DEV bool load_level(const Consts& K, const double* hgt, int i, double tt, double cv, double cld, double ic,
-                          double co2, double p_u, double rh, double scc, double Rain, double Snow, Level& L) {
+                          double co2, double p_u, double rh, double scc, double Rain, double Snow, Level& L,
+                          const double* q_cached = nullptr) {
 #pragma clang fp contract(off)
     const double t_u = tt * K.t_0;
-    const double Mo = th_M_nonwater(cv, co2, K.M_bg);
-    const double q_sw = (t_u >= kTcrit) ? 1.0 : th_q_sat(sat_p_sat(t_u), p_u, Mo);
-    L.t_u = t_u; L.cv = cv; L.q_sw = q_sw;
-    if (!(q_sw < 1.0)) return false;
+    L.t_u = t_u; L.cv = cv;
+    double Mo = 0.0;
+    if (q_cached) {
+        L.q_sw = q_cached[0];
+    } else {
+        Mo = th_M_nonwater(cv, co2, K.M_bg);
+        L.q_sw = (t_u >= kTcrit) ? 1.0 : th_q_sat(sat_p_sat(t_u), p_u, Mo);
+        if (!(L.q_sw < 1.0)) return false;
+    }
     L.cld = cld; L.ic = ic; L.rh = rh; L.scc = scc; L.Rain = Rain; L.Snow = Snow;
     L.R49 = 0.0; L.R79 = 0.0; L.R139 = 0.0;
     if (Rain > 1e-12) {
@@ -1678,7 +1686,7 @@ ATHAD_DEV bool load_level(const Consts& K, const double* hgt, int i, double tt, 
         const double r4 = r2 * r2;
         L.R49 = r4; L.R79 = Rain / r2; L.R139 = Rain * r4;
     }
-    L.q_Ice = th_q_sat(sat_p_subl(t_u), p_u, Mo);
+    L.q_Ice = q_cached ? q_cached[1] : th_q_sat(sat_p_subl(t_u), p_u, Mo);
     const double step = (double)(float)((float)hgt[i + 1] - (float)hgt[i]);
     L.mass_layer = rh * step;
     L.dt_snow_dim = step / 0.96;
@@ -1742,16 +1750,26 @@ ATHAD_DEV void ice_column_seq(const MemberDesc& M, int j, int k) {
 
     const double Rain_check = (double)Pr[base + (size_t)i_check * plane];
     double m_i = m_i_max, N_i = 0.0, S_nuc = 0.0, S_i_dep = 0.0;
+    // q_sw and q_Ice of the levels that could condense in the previous pass (their t and c are
+    // unchanged), kept in scratch 10 and 11 for the later passes: FP64 storage, up to 64 levels
+    constexpr bool kCache = sizeof(S) == 8;
+    double* qc0 = (double*)M.tmp[10];
+    double* qc1 = (double*)M.tmp[11];
+    unsigned long long cached = 0ULL;
     for (int pass = 1; pass <= passes; pass++) {
         Pr[base + (size_t)(im - 1) * plane] = S(0);
         Ps[base + (size_t)(im - 1) * plane] = S(0);
         double pr_up = 0.0, ps_up = 0.0;   // fluxes of the level above, as just stored
+        unsigned long long can_mask = 0ULL;
         for (int i = im - 2; i >= 0; i--) {
             const size_t id = base + (size_t)i * plane;
             Level L;
             const double cv = (double)fc[id], co2 = (double)fco2[id];
+            const bool hit = kCache && i < 64 && ((cached >> i) & 1ULL);
+            double qv[2];
+            if (hit) { qv[0] = qc0[id]; qv[1] = qc1[id]; }
             if (!load_level(K, M.hgt, i, (double)ft[id], cv, (double)fcl[id], (double)fi[id], co2, fps[id],
-                            (double)frh[id], (double)fscc[id], (double)Pr[id], (double)Ps[id], L)) {
+                            (double)frh[id], (double)fscc[id], (double)Pr[id], (double)Ps[id], L, hit ? qv : nullptr)) {
                 double t_new = 0.0, c_new;
                 bool t_set;
                 evaporate(K, L.t_u, cv, (double)fcl[id], (double)fi[id], (double)fg[id], co2, (double)frh[id],
@@ -1767,6 +1785,10 @@ ATHAD_DEV void ice_column_seq(const MemberDesc& M, int j, int k) {
                 pr_up = 0.0; ps_up = 0.0;
                 continue;
             }
+            if (kCache && i < 64 && im <= 64) {
+                if (!hit && pass < passes) { qc0[id] = L.q_sw; qc1[id] = L.q_Ice; }
+                can_mask |= 1ULL << i;
+            }
             assign_Ni_mi(L, K.t_0, N_i, m_i);
             assign_S_nuc(L, N_i, S_nuc);
             assign_S_i_dep(L, N_i, m_i, S_i_dep);
@@ -1779,6 +1801,7 @@ ATHAD_DEV void ice_column_seq(const MemberDesc& M, int j, int k) {
             pr_up = (double)S(pr); ps_up = (double)S(psn);
         }
         if (fabs((double)Pr[base + (size_t)i_check * plane] - Rain_check) * 8.64e4 <= 1.0e-3) break;
+        cached = can_mask;
     }
 }
 
