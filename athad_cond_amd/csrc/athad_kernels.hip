@@ -411,49 +411,76 @@ constexpr int kRadCols = 16;
 
 // Inclusive composition scan across the 64 lanes: after it, lane l holds the composition of
 // the maps of lanes 0..l (upward) or l..63 (downward), the nearer lane's map applied last. A
-// step composes B = A B' + B and A = A A' with the partner's (A', B'). The A part depends only
-// on the emissivities, which are fixed for a column, so its per-step values are formed once
-// (rad_prep) and each of the five sweeps shuffles only B.
-struct RadScan { double As_up[6], As_dn[6]; };
+// step composes B = A B' + B and A = A A' with the farther part (A', B'). The scan runs within
+// each 16-lane row with DPP moves (shift by 1, 2, 4, 8 inside the row), then joins the rows
+// through the row totals read from lanes 15, 31, 47 (upward) or 16, 32, 48 (downward); lanes
+// whose DPP source lies outside their row keep the identity (1, 0), an exact no-op. The A part
+// depends only on the emissivities, fixed for a column, so its per-step values and the rows'
+// totals are formed once (rad_prep) and each of the five sweeps moves only B. All terms are
+// non-negative, so the grouping agrees with the sequential recurrence to a few ulps.
+constexpr int kDppRowShr = 0x110, kDppRowShl = 0x100, kDppWaveShr1 = 0x138, kDppWaveShl1 = 0x130;
 
-ATHAD_DEV void scan_A_up(double A, double (&As)[6]) {
-#pragma clang fp contract(off)
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int s = 0, d = 1; s < 6; s++, d <<= 1) {
-        As[s] = A;
-        const double Ap = __shfl_up(A, d);
-        if (lane >= d) A = A * Ap;
-    }
+// 64-bit DPP move; lanes whose source is invalid (outside the row or the wave) get `old`.
+template <int CTRL> ATHAD_DEV double dpp_move(double v, double old) {
+    const long long x = __double_as_longlong(v), o = __double_as_longlong(old);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)x, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
-ATHAD_DEV void scan_A_down(double A, double (&As)[6]) {
-#pragma clang fp contract(off)
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int s = 0, d = 1; s < 6; s++, d <<= 1) {
-        As[s] = A;
-        const double An = __shfl_down(A, d);
-        if (lane + d < 64) A = A * An;
-    }
+ATHAD_DEV double read_lane(double v, int lane) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)x, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(x >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
-ATHAD_DEV double scan_B_up(double B, const double (&As)[6]) {
+
+struct RadScan {
+    double As_up[4], Af_up, a15, a31, a47;   // upward: own A before each row step, after them, row totals
+    double As_dn[4], Af_dn, a16, a32, a48;   // downward
+};
+
+ATHAD_DEV void scan_A_up(double A, RadScan& R) {
 #pragma clang fp contract(off)
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int s = 0, d = 1; s < 6; s++, d <<= 1) {
-        const double Bp = __shfl_up(B, d);
-        if (lane >= d) B = As[s] * Bp + B;
-    }
+    R.As_up[0] = A; A = A * dpp_move<kDppRowShr + 1>(A, 1.0);
+    R.As_up[1] = A; A = A * dpp_move<kDppRowShr + 2>(A, 1.0);
+    R.As_up[2] = A; A = A * dpp_move<kDppRowShr + 4>(A, 1.0);
+    R.As_up[3] = A; A = A * dpp_move<kDppRowShr + 8>(A, 1.0);
+    R.Af_up = A;
+    R.a15 = read_lane(A, 15); R.a31 = read_lane(A, 31); R.a47 = read_lane(A, 47);
+}
+ATHAD_DEV void scan_A_down(double A, RadScan& R) {
+#pragma clang fp contract(off)
+    R.As_dn[0] = A; A = A * dpp_move<kDppRowShl + 1>(A, 1.0);
+    R.As_dn[1] = A; A = A * dpp_move<kDppRowShl + 2>(A, 1.0);
+    R.As_dn[2] = A; A = A * dpp_move<kDppRowShl + 4>(A, 1.0);
+    R.As_dn[3] = A; A = A * dpp_move<kDppRowShl + 8>(A, 1.0);
+    R.Af_dn = A;
+    R.a16 = read_lane(A, 16); R.a32 = read_lane(A, 32); R.a48 = read_lane(A, 48);
+}
+ATHAD_DEV double scan_B_up(double B, const RadScan& R) {
+#pragma clang fp contract(off)
+    const int row = (threadIdx.x & 63) >> 4;
+    B = R.As_up[0] * dpp_move<kDppRowShr + 1>(B, 0.0) + B;
+    B = R.As_up[1] * dpp_move<kDppRowShr + 2>(B, 0.0) + B;
+    B = R.As_up[2] * dpp_move<kDppRowShr + 4>(B, 0.0) + B;
+    B = R.As_up[3] * dpp_move<kDppRowShr + 8>(B, 0.0) + B;
+    // prefixes of the rows below: row 0, rows 0-1, rows 0-2
+    const double b15 = read_lane(B, 15), b31 = read_lane(B, 31), b47 = read_lane(B, 47);
+    const double q1 = b15, q2 = R.a31 * q1 + b31, q3 = R.a47 * q2 + b47;
+    if (row > 0) B = R.Af_up * ((row == 1) ? q1 : (row == 2) ? q2 : q3) + B;
     return B;
 }
-ATHAD_DEV double scan_B_down(double B, const double (&As)[6]) {
+ATHAD_DEV double scan_B_down(double B, const RadScan& R) {
 #pragma clang fp contract(off)
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int s = 0, d = 1; s < 6; s++, d <<= 1) {
-        const double Bn = __shfl_down(B, d);
-        if (lane + d < 64) B = As[s] * Bn + B;
-    }
+    const int row = (threadIdx.x & 63) >> 4;
+    B = R.As_dn[0] * dpp_move<kDppRowShl + 1>(B, 0.0) + B;
+    B = R.As_dn[1] * dpp_move<kDppRowShl + 2>(B, 0.0) + B;
+    B = R.As_dn[2] * dpp_move<kDppRowShl + 4>(B, 0.0) + B;
+    B = R.As_dn[3] * dpp_move<kDppRowShl + 8>(B, 0.0) + B;
+    // suffixes of the rows above: row 3, rows 2-3, rows 1-3
+    const double b16 = read_lane(B, 16), b32 = read_lane(B, 32), b48 = read_lane(B, 48);
+    const double q2 = b48, q1 = R.a32 * q2 + b32, q0 = R.a16 * q1 + b16;
+    if (row < 3) B = R.Af_dn * ((row == 2) ? q2 : (row == 1) ? q1 : q0) + B;
     return B;
 }
 
@@ -473,11 +500,11 @@ ATHAD_DEV void rad_prep(int lane, int im, const double (&eps)[E], double (&a_up)
     double A = 1.0;
 #pragma unroll
     for (int e = 0; e < E; e++) A = a_up[e] * A;
-    scan_A_up(A, R.As_up);
+    scan_A_up(A, R);
     A = 1.0;
 #pragma unroll
     for (int e = E - 1; e >= 0; e--) A = a_dn[e] * A;
-    scan_A_down(A, R.As_dn);
+    scan_A_down(A, R);
 }
 
 // Both sweeps of one column for the current sigma T^4.
@@ -495,18 +522,16 @@ ATHAD_DEV void rad_sweeps(int lane, int im, const double (&eps)[E], const double
     double B = 0.0;
 #pragma unroll
     for (int e = 0; e < E; e++) B = a_up[e] * B + b[e];
-    B = scan_B_up(B, R.As_up);
-    double x = __shfl_up(B, 1);
-    if (lane == 0) x = 0.0;
+    B = scan_B_up(B, R);
+    double x = dpp_move<kDppWaveShr1>(B, 0.0);   // the lane below's prefix; 0 for lane 0
 #pragma unroll
     for (int e = 0; e < E; e++) { x = a_up[e] * x + b[e]; up[e] = x; }
     if (lane == 0) b[0] = 0.0;
     B = 0.0;
 #pragma unroll
     for (int e = E - 1; e >= 0; e--) B = a_dn[e] * B + b[e];
-    B = scan_B_down(B, R.As_dn);
-    x = __shfl_down(B, 1);
-    if (lane == 63) x = 0.0;
+    B = scan_B_down(B, R);
+    x = dpp_move<kDppWaveShl1>(B, 0.0);          // the lane above's suffix; 0 for lane 63
 #pragma unroll
     for (int e = E - 1; e >= 0; e--) { x = a_dn[e] * x + b[e]; dn[e] = x; }
 }
@@ -524,23 +549,21 @@ __global__ __launch_bounds__(256) void k_radiation(const MemberDesc* __restrict_
     const S *fcl = fp<S>(M, F_CLOUD), *fice = fp<S>(M, F_ICE);
     const double* fpst = (const double*)M.f[F_P_STAT];
     const long long si = (long long)jm * km, row = (long long)j * km;
-    const int per = im * kRadCols;
     auto slot = [&](int f, int i, int kk) -> double& { return s_col[(f * im + i) * kRadCols + kk]; };
-    // stage t, c, co2, p_stat, cloud, ice of the tile's columns
-    for (int q = threadIdx.x; q < 6 * per; q += blockDim.x) {
-        const int f = q / per, r = q - f * per, i = r / kRadCols, kk = r - i * kRadCols;
-        if (k0 + kk >= km) continue;
-        const long long g = i * si + row + k0 + kk;
-        double v;
-        switch (f) {
-            case 0: v = (double)ft[g]; break;
-            case 1: v = (double)fc[g]; break;
-            case 2: v = (double)fco2[g]; break;
-            case 3: v = fpst[g]; break;
-            case 4: v = (double)fcl[g]; break;
-            default: v = (double)fice[g]; break;
-        }
-        slot(f, i, kk) = v;
+    // stage t, c, co2, p_stat, cloud, ice of the tile's columns: thread (i mod 16, kk) of the
+    // block walks the levels 16 apart, so each load covers 16 neighbouring longitudes
+    static_assert(kRadCols == 16, "the staging maps 256 threads onto 16 levels x 16 columns");
+    const int kk_s = threadIdx.x & 15, i_s = threadIdx.x >> 4;
+    const bool col_in = k0 + kk_s < km;
+    for (int i = i_s; i < im; i += 16) {
+        if (!col_in) break;
+        const long long g = i * si + row + k0 + kk_s;
+        slot(0, i, kk_s) = (double)ft[g];
+        slot(1, i, kk_s) = (double)fc[g];
+        slot(2, i, kk_s) = (double)fco2[g];
+        slot(3, i, kk_s) = fpst[g];
+        slot(4, i, kk_s) = (double)fcl[g];
+        slot(5, i, kk_s) = (double)fice[g];
     }
     __syncthreads();
     // incoming short wave of this latitude (MultiLayerRadiation.h:98-104)
@@ -609,9 +632,8 @@ __global__ __launch_bounds__(256) void k_radiation(const MemberDesc* __restrict_
         rad_prep<E>(lane, im, eps, a_up, a_dn, R);
         for (int it = 0; it < 4; it++) {
             rad_sweeps<E>(lane, im, eps, P4, sigma, a_up, a_dn, R, up, dn);
-            const double up_prev_lane = __shfl_up(up[E - 1], 1);
-            double dn_next_lane = __shfl_down(dn[0], 1);
-            if (lane == 63) dn_next_lane = 0.0;
+            const double up_prev_lane = dpp_move<kDppWaveShr1>(up[E - 1], 0.0);
+            const double dn_next_lane = dpp_move<kDppWaveShl1>(dn[0], 0.0);   // 0 for lane 63
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const int i = lane * E + e;
@@ -628,7 +650,7 @@ __global__ __launch_bounds__(256) void k_radiation(const MemberDesc* __restrict_
         rad_sweeps<E>(lane, im, eps, P4, sigma, a_up, a_dn, R, up, dn);
         // surface energy balance with sigma T^4 linearised at the current surface temperature
         // (MultiLayerRadiation.h:338-371); dn at level 0 is dn[1]
-        const double T1 = (E > 1) ? T[E > 1 ? 1 : 0] : __shfl_down(T[0], 1);
+        const double T1 = (E > 1) ? T[E > 1 ? 1 : 0] : dpp_move<kDppWaveShl1>(T[0], 0.0);
         double Ts = 0.0;
         if (lane == 0) {
             const double Ts0 = T[0], c_H = 15.0;
@@ -649,14 +671,12 @@ __global__ __launch_bounds__(256) void k_radiation(const MemberDesc* __restrict_
     }
     __syncthreads();
     S* tw = fp<S>(M, F_T);
-    for (int q = threadIdx.x; q < 3 * per; q += blockDim.x) {
-        const int f = q / per, r = q - f * per, i = r / kRadCols, kk = r - i * kRadCols;
-        if (k0 + kk >= km) continue;
-        const long long g = i * si + row + k0 + kk;
-        const double v = slot(f, i, kk);
-        if (f == 0) tw[g] = (S)v;
-        else if (f == 1) M.epsilon[g] = v;
-        else M.radiation[g] = v;
+    for (int i = i_s; i < im; i += 16) {
+        if (!col_in) break;
+        const long long g = i * si + row + k0 + kk_s;
+        tw[g] = (S)slot(0, i, kk_s);
+        M.epsilon[g] = slot(1, i, kk_s);
+        M.radiation[g] = slot(2, i, kk_s);
     }
 }
 
