@@ -434,6 +434,11 @@ ATHAD_DEV double read_lane(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+template <int CTRL> ATHAD_DEV float dpp_move(float v, float old) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+ATHAD_DEV float read_lane(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
+
 struct RadScan {
     double As_up[4], Af_up, a15, a31, a47;   // upward: own A before each row step, after them, row totals
     double As_dn[4], Af_dn, a16, a32, a48;   // downward
@@ -2813,14 +2818,14 @@ __global__ __launch_bounds__(256) void k_polar_wave(const MemberDesc* __restrict
         v[s] = (e < km) ? C(gld(x, (long long)e)) : C(0);
     }
     for (int p = 0; p < np; p++) {
-        const C up = __shfl_up(v[E - 1], 1);   // lane l-1's last longitude
-        const C dn = __shfl_down(v[0], 1);     // lane l+1's first longitude
+        const C up = dpp_move<kDppWaveShr1>(v[E - 1], C(0));   // lane l-1's last longitude
+        const C dn = dpp_move<kDppWaveShl1>(v[0], C(0));       // lane l+1's first longitude
         C lv = v[0];
 #pragma unroll
         for (int s = 1; s < E; s++)
             if (s == s_last) lv = v[s];
-        const C v_last = __shfl(lv, l_last);   // longitude km-1
-        const C v_first = __shfl(v[0], 0);     // longitude 0
+        const C v_last = read_lane(lv, l_last);   // longitude km-1 (l_last is wave-uniform)
+        const C v_first = read_lane(v[0], 0);     // longitude 0
         C nv[E];
 #pragma unroll
         for (int s = 0; s < E; s++) {
