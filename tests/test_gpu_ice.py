@@ -268,3 +268,28 @@ def test_members_at_different_phases_with_the_ice_scheme(oracle):
                 np.testing.assert_array_equal(integ.get(n, member=q), single[q][n], err_msg=f"{q} {n}")
             for n in L.MP_FIELDS:
                 np.testing.assert_array_equal(integ.get_mp(n, member=q), single[q][n], err_msg=f"{q} {n}")
+
+
+def test_full_grid_loop_with_ice_scheme_and_radiation(oracle):
+    """41x181x361 from the nominal initial state, 20 iterations with the ice scheme in the moist
+    block and radiation after each store, through the hipGraph, against the oracle doing the
+    same: every prognostic field, the fluxes and sources, and t_skin within 1e-10."""
+    grid = (41, 181, 361)
+    print(f"case full-grid loop with ice scheme and radiation {grid}", flush=True)
+    ref = oracle.OracleState(*grid, seed=1234)
+    ref.set_ice_scheme(True)
+    with A.Integration(grid) as integ:
+        integ.init_state(1234)
+        integ.set_ice_scheme(True)
+        integ.set_radiation(True)
+        integ.run(20, graph=True)
+        for _ in range(20):
+            ref.step("jacobi")
+            ref.radiation()
+        e = {n: rel(integ.get(n), ref.field(n)) for n in A.PROGNOSTIC + ["p_stat", "r_humid", "p_dyn"]}
+        for n in L.MP_FIELDS:
+            e[n] = rel(integ.get_mp(n), ref.mp(n))
+        e["t_skin"] = abs(integ.radiation_outputs()["t_skin"] / ref.scalar("t_skin") - 1.0)
+        print({k: f"{v:.1e}" for k, v in e.items()}, flush=True)
+        bad = {k: v for k, v in e.items() if v > 1e-10}
+        assert not bad, bad
