@@ -204,6 +204,10 @@ struct orc_state {
     // ice scheme (ORC_MP_* order); allocated, zero, at first use
     std::vector<std::vector<double>> mp;
     bool ice_on = false;               // the ice scheme runs inside the moist block
+    // moist-convection arrays (ORC_MC_* order) and the evaporation diagnostic [mm/d] per
+    // column that its surface-flux perturbation reads; allocated, zero, at first use
+    std::vector<std::vector<double>> mc;
+    std::vector<double> evap;
     int iter_n = 0, total_iter = 0;
 
     inline size_t ix(int i, int j, int k) const { return ((size_t)i * jm + j) * km + k; }
@@ -1801,6 +1805,429 @@ void ice_store(S& s) {
     s.mp[ORC_MP_P_SNOWN] = s.mp[ORC_MP_P_SNOW];
 }
 
+// ---------------------------------------------------------------------------------------
+// Moist convection: MoistConvection::run (MoistConvection.h:106-140) with the model's
+// defaults iter_prec = 3, convection_perturbation = 1, convection_mode = 1 and no land.
+// Every phase reads only its own column's convection arrays and the fields c, cloud, t, u, v,
+// w, p_stat, r_humid, co2 (c also at the four horizontal neighbours, which no phase writes),
+// so the phases of one column run back to back here; the result is the reference's
+// phase-by-phase sweep over all columns. computeCAPE (:1259-1299) is left out: it fills a
+// 1-D diagnostic that ends holding the last column's values and feeds nothing.
+// Q_sensible_2D is zero throughout the reference (nothing fills it).
+// ---------------------------------------------------------------------------------------
+namespace conv {
+constexpr double a_ev = 1.0e-3, b_ev = 5.9, t_00 = 236.15, t_00_m2 = 236.15 - 2.0;
+constexpr double b_u = 0.33, alf_1 = 0.05, alf_2 = 0.05, bet_p = 2.0e-3;
+const double eps_u = 0.2 / 100.0, del_u = 0.2 / 100.0;           // :30-32 (R_cloud 100 m)
+constexpr double eps_d = 1.0e-4, del_d = 1.0e-4, gam_d = -0.3;
+constexpr double a_u = 3.0e-2, a_d = 3.0e-2, vel = 1.1;
+constexpr double f_beg = 1000.0 / 1013.25, f_end = 970.0 / 1013.25, f_cb = 900.0 / 1013.25;
+constexpr double f_diff = 200.0 / 1013.25, f_mid = 700.0 / 1013.25;
+constexpr double t_add_u = 0.2, q_v_u_add = 1.0e-4, coeff_recurr = 0.1, scale = 0.98;
+constexpr double q_c_u_max = 1.0e-2, w_conv_max = 50.0, M_max = 3.0, c_mb = 0.003;
+constexpr double alpha_sfp = 0.2;                                  // :404
+constexpr double MCt_max = 0.01, MCq_max = 2.0e-5, MCv_max = 0.01;   // :1323-1339
+constexpr double cp_l = 1349.0, s_0 = 274515.75, p_0 = 60000.0, lv = 2.52e6, ls = 2.83e6;
+
+inline double cap(double v, double mag) {                          // :216-222, :1341-1346
+    if (!finite_bits(v)) return 0.0;
+    return (v < -mag) ? -mag : (v > mag) ? mag : v;
+}
+}  // namespace conv
+
+void ensure_mc(S& s) {
+    if (s.mc.size() == ORC_NMC) return;
+    s.mc.assign(ORC_NMC, std::vector<double>(s.n, 0.0));
+    s.evap.assign((size_t)s.jm * s.km, 0.0);
+}
+
+void moist_convection(S& s) {
+    using namespace conv;
+    ensure_mc(s);
+    const Params& P = s.P;
+    const int im = s.im, jm = s.jm, km = s.km;
+    const size_t pl = (size_t)jm * km;
+    const double t0 = P.t_0;
+    const double *T = s.F(ORC_T), *U = s.F(ORC_U), *V = s.F(ORC_V), *W = s.F(ORC_W);
+    const double *C = s.F(ORC_C), *CL = s.F(ORC_CLOUD), *CO2 = s.F(ORC_CO2);
+    const double *PS = s.F(ORC_P_STAT), *RH = s.F(ORC_R_HUMID);
+    auto A = [&](int q) { return s.mc[q].data(); };
+    double *Sx = A(ORC_MC_S), *Mu = A(ORC_MC_M_U), *Md = A(ORC_MC_M_D), *Su = A(ORC_MC_S_U);
+    double *Qvu = A(ORC_MC_Q_V_U), *Qcu = A(ORC_MC_Q_C_U), *Cu = A(ORC_MC_C_U), *El = A(ORC_MC_E_L);
+    double *Edv = A(ORC_MC_E_D_EVAP), *Ep = A(ORC_MC_E_P), *Gp = A(ORC_MC_G_P), *Sd = A(ORC_MC_S_D);
+    double *Qvd = A(ORC_MC_Q_V_D), *Pc = A(ORC_MC_P_CONV), *Eu = A(ORC_MC_E_U), *Du = A(ORC_MC_D_U);
+    double *Uu = A(ORC_MC_U_U), *Vu = A(ORC_MC_V_U), *Wu = A(ORC_MC_W_U), *Ed = A(ORC_MC_E_D);
+    double *Dd = A(ORC_MC_D_D), *Ud = A(ORC_MC_U_D), *Vd = A(ORC_MC_V_D), *Wd = A(ORC_MC_W_D);
+    double *MCt = A(ORC_MC_MC_T), *MCq = A(ORC_MC_MC_Q), *MCv = A(ORC_MC_MC_V), *MCw = A(ORC_MC_MC_W);
+    double *Dbeg = A(ORC_MC_DEEP_BEG), *Dend = A(ORC_MC_DEEP_END), *CB = A(ORC_MC_CLOUD_BASE);
+    double *LFS = A(ORC_MC_LFS);
+    // layer heights (float in the reference, cAtmosphereModel.h:241-246) and level steps (:300-309)
+    std::vector<double> h(im), step(im, 0.0);
+    for (int i = 0; i < im; i++) h[i] = (double)s.hgt[i];
+    for (int i = 0; i + 1 < im; i++) step[i] = h[i + 1] - h[i];
+    if (im >= 2) step[im - 1] = step[im - 2];
+    const double Mb = P.comp.M_bg, Rb = P.comp.R_bg;
+    auto qs_env = [&](double E, double p, double qv) { return q_sat(E, p, M_nonwater(qv, P.co2_0, Mb)); };  // :181-184
+    auto cc_factor = [&](double TK, double p) {                                                          // :191-203
+        auto Es = [&](double Tx) { return (Tx >= t0) ? p_sat(Tx) : p_subl(Tx); };
+        const double a = qs_env(Es(TK), p, P.c_0), b = qs_env(Es(288.15), p, P.c_0);
+        return (b > 0.0) ? a / b : 1.0;
+    };
+    const double inv_a_u = 1.0 / (a_u * P.u_0), inv_a_d = 1.0 / (a_d * P.u_0);
+    const double uu_cap = w_conv_max / P.u_0;
+    std::vector<double> qs(im);
+
+    for (int j = 0; j < jm; j++)
+        for (int k = 0; k < km; k++) {
+            auto id = [&](int i) { return (size_t)i * pl + (size_t)j * km + k; };
+            const bool inner = j >= 1 && j <= jm - 2 && k >= 1 && k <= km - 2;
+            const double p_sfc = PS[id(0)];
+            // entrainment gradients of c (:238-297, no land): centred differences
+            auto grad = [&](int i, double& dr, double& dth, double& dph, double& cc) {
+                const double costhe = std::cos(s.the[90] - s.the[j]);
+                const double rth = (P.r_Earth * 1.0e3 + h[i]) * costhe;
+                const double Ue = rth * 2.0 * M_PI;
+                const int ip = std::min(i + 1, im - 1), imn = std::max(i - 1, 0);
+                dr = (C[id(ip)] - C[id(imn)]) * (0.5 / step[i]);
+                dth = (C[id(i) + km] - C[id(i) - km]) * (0.5 / (Ue / (jm - 1)));
+                dph = (C[id(i) + 1] - C[id(i) - 1]) * (1.0 / (Ue / (km - 1)));
+                cc = dmax(C[id(i)], 1e-4);
+            };
+            // precompute (:336-376): s from t, the per-call arrays cleared
+            for (int i = 0; i < im; i++) {
+                const size_t q = id(i);
+                Sx[q] = cp_l * T[q] * t0 / s_0;
+                Mu[q] = Md[q] = Su[q] = Qvu[q] = Qcu[q] = Cu[q] = El[q] = Edv[q] = Ep[q] = Gp[q] = 0.0;
+                Sd[q] = Qvd[q] = Pc[q] = 0.0;
+            }
+            int base = 0, lfs = 0, beg = -1, end = -1;
+            double MdLFS = 0.0, cape_c = 0.0, dT = 0.0, dq = 0.0;
+            // findCloudBaseLFS (:627-777)
+            for (int i = 0; i < im; i++)
+                if (PS[id(i)] <= f_beg * p_sfc) { beg = i; Dbeg[id(i)] = h[i]; break; }
+            for (int i = im - 1; i >= 0; i--)
+                if (PS[id(i)] >= f_end * p_sfc) { end = i; Dend[id(i)] = h[i]; break; }
+            if (beg >= 0 && end >= 0) {
+                for (int i = beg; i < im; i++)
+                    qs[i] = qs_env(p_auto(T[id(i)] * t0 + t_add_u), PS[id(i)], C[id(i)]);
+                for (int i = beg; i < im; i++) {
+                    const size_t q = id(i);
+                    const double t_u = T[q] * t0;
+                    const double w = 1.0 / (1.0 + std::exp(-(t_u - t_00) / 2.0));
+                    Qvu[q] = C[q] + q_v_u_add;
+                    if (Qvu[q] >= scale * qs[i] && t_u >= t_00) Qcu[q] = Qvu[q] - scale * qs[i];
+                    if (Qvu[q] <= 0.0) Qvu[q] = 0.0;
+                    if (Qcu[q] <= 0.0) Qcu[q] = 0.0;
+                    if (Qvu[q] >= scale * qs[i] && w > 0.01 && PS[q] <= f_cb * p_sfc) {
+                        base = i;
+                        CB[q] = h[i] * w;
+                        break;
+                    }
+                }
+                if (base > 0) {   // level of neutral buoyancy of the theta-e parcel from cloud base
+                    const size_t qb = id(base);
+                    const double Tb = T[qb] * t0 + t_add_u;
+                    const double kap = R_of(C[qb], CO2[qb], Rb) / cp_of(C[qb], CO2[qb], Tb, Mb);
+                    const double Lb = (Tb >= t0) ? lv : ls;
+                    const double thp = Tb * std::pow(p_0 / PS[qb], kap) * std::exp(Lb * qs[base] / (cp_l * Tb));
+                    bool buoy = false;
+                    int top = base;
+                    double cape = 0.0;
+                    for (int i = base + 1; i < im; i++) {
+                        const double Te = T[id(i)] * t0;
+                        const double Le = (Te >= t0) ? lv : ls;
+                        const double the = Te * std::pow(p_0 / PS[id(i)], kap) * std::exp(Le * qs[i] / (cp_l * Te));
+                        if (thp - the > 0.0) {
+                            buoy = true;
+                            top = i;
+                            cape += P.g * (thp - the) / the * (h[i] - h[i - 1]);
+                        } else if (buoy) {
+                            break;
+                        }
+                    }
+                    const int lnb = std::min(std::max(top, base + 1), im - 2);
+                    lfs = lnb;
+                    cape_c = dmax(0.0, cape);
+                    LFS[id(lnb)] = h[lnb];
+                }
+            }
+            const bool deep = PS[id(base)] - PS[id(lfs)] >= f_diff * p_sfc;
+            // surfaceFluxPerturbations, switch 1 (:402-485)
+            if (inner && base >= 0 && lfs > base) {
+                const double Ts = T[id(0)] * t0;
+                const double rho = dmax(RH[id(0)], 1e-6);
+                const double qcap = q_v_u_add * cc_factor(Ts, p_sfc);
+                if (PS[id(base)] < f_mid * p_sfc) {
+                    dT = t_add_u;
+                    dq = qcap;
+                } else if (!deep) {
+                    const double Hs = 0.0;
+                    const double E = dmax(0.0, s.evap[(size_t)j * km + k]) / 86400.0;
+                    const double z = h[base];
+                    if (!(E <= 0.0 && Hs <= 0.0) && z > 0.0) {
+                        const double B = (P.g / Ts) * (Hs / (rho * cp_l) + 0.61 * Ts * E / rho);
+                        const double ws = std::cbrt(dmax(0.0, B * z));
+                        if (ws > 0.0) {
+                            const double ir = 1.0 / (rho * ws);
+                            dT = dmin(alpha_sfp * Hs * ir / cp_l, t_add_u);
+                            dq = dmin(alpha_sfp * E * ir, qcap);
+                        }
+                    }
+                }
+            }
+            auto entrain = [&](int i, bool with_m) {   // E_u at level i (:534-543, :841-851)
+                const size_t q = id(i);
+                double e = 0.0;
+                if (PS[id(base)] - PS[id(lfs)] >= f_diff * p_sfc) {
+                    double dr, dth, dph, cc;
+                    grad(i, dr, dth, dph, cc);
+                    e = -RH[q] / cc * (U[q] * dr + V[q] * dth + W[q] * dph);
+                    if (with_m) e += eps_u * Mu[q];
+                }
+                if (T[q] * t0 <= t_00_m2) e = 0.0;
+                return e;
+            };
+            auto seed_M = [&](int i) { return cap(RH[id(i)] * c_mb * std::sqrt(2.0 * cape_c), M_max); };
+            // initUpdraft (:490-622)
+            if (inner && beg != -1 && end != -1) {
+                Mu[id(beg)] = seed_M(beg);
+                for (int i = beg + 1; i <= end; i++) {
+                    const size_t q = id(i);
+                    const double t_u = T[q] * t0, t_a = t_u + dT;
+                    const double qsa = qs_env(p_auto(t_a), PS[q], C[q]);
+                    Qvu[q] = C[q] + dq;
+                    if (Qvu[q] >= scale * qsa && t_u >= t_00) Qcu[q] = Qvu[q] - scale * qsa;
+                    if (Qvu[q] <= 0.0) Qvu[q] = 0.0;
+                    if (Qcu[q] <= 0.0) Qcu[q] = 0.0;
+                    Eu[q] = entrain(i, false);
+                    Du[q] = 0.0;
+                    Mu[q] = cap(Mu[id(i - 1)] + (Eu[q] - Du[q]) * step[i], M_max);
+                    Sx[q] = cp_l * t_u / s_0;
+                    Su[q] = cp_l * t_a / s_0;
+                    Uu[q] = vel * U[q]; Vu[q] = vel * V[q]; Wu[q] = vel * W[q];
+                }
+                MdLFS = cap(gam_d * Mu[id(beg)], M_max);
+                Md[id(lfs)] = MdLFS;
+                for (int i = end + 1; i <= lfs - 1; i++) {
+                    const size_t q = id(i);
+                    const double t_u = T[q] * t0, t_a = t_u + dT;
+                    const double qsa = qs_env(p_auto(t_a), PS[q], C[q]);
+                    if (i == end + 1) Mu[id(i - 1)] = Mu[id(beg)];
+                    Sx[q] = cp_l * t_u / s_0;
+                    Su[q] = cp_l * t_u / s_0;
+                    Qvu[q] = C[q] + dq;
+                    if (Qvu[q] >= scale * qsa && t_a >= t_00) Qcu[q] = Qvu[q] - scale * qsa;
+                    Eu[q] = entrain(i, false);
+                    Du[q] = 0.0;
+                    Mu[q] = cap(Mu[id(i - 1)] + (Eu[q] - Du[q]) * step[i], M_max);
+                    Uu[q] = dmax(-uu_cap, dmin(U[q] + Mu[q] * inv_a_u / dmax(RH[q], 1e-6), uu_cap));
+                    Vu[q] = V[q]; Wu[q] = W[q];
+                }
+            }
+            // CloudBaseUpdate (:1462-1482) with simpsonCxEu (:1426-1455)
+            auto cloud_base_update = [&]() {
+                if (!inner || beg < 0 || lfs <= beg) return;
+                int hit = -1;
+                double cum = 0.0;
+                int i = beg;
+                auto f = [&](int l) { return -(C[id(l)] * Eu[id(l)]); };
+                for (; i + 2 <= lfs && hit < 0; i += 2) {
+                    const double h1 = step[i], h2 = step[i + 1];
+                    const double f0 = f(i), f1 = f(i + 1), f2 = f(i + 2);
+                    const double pc = (h1 + h2) / 6.0 * ((2.0 - h2 / h1) * f0 + (h1 + h2) * (h1 + h2) / (h1 * h2) * f1
+                                                         + (2.0 - h1 / h2) * f2);
+                    if (cum + pc > 0.0) { hit = (cum + 0.5 * h1 * (f0 + f1) > 0.0) ? i + 1 : i + 2; break; }
+                    cum += pc;
+                }
+                if (hit < 0 && i + 1 <= lfs && cum + 0.5 * step[i] * (f(i) + f(i + 1)) > 0.0) hit = i + 1;
+                if (hit >= 0 && hit < im && PS[id(hit)] <= f_cb * p_sfc && (base <= 0 || hit <= base)) {
+                    base = hit;
+                    CB[id(hit)] = h[hit];
+                }
+            };
+            cloud_base_update();
+            for (int ip = 1; ip <= 3; ip++) {
+                // updraftEntrainment (:780-908)
+                if (inner && base >= 0 && lfs > base) {
+                    const bool shallow = PS[id(base)] - PS[id(lfs)] < f_diff * p_sfc;   // convection_mode 1
+                    const double dic = shallow ? 1.0e9 : 500.0;
+                    const double hb = h[base];
+                    for (int i = 0; i < base; i++) Mu[id(i)] = 0.0;
+                    Mu[id(base)] = seed_M(base);
+                    Su[id(base)] = cp_l * (T[id(base)] * t0 + dT) / s_0;
+                    Qvu[id(base)] = dmax(C[id(base)] + dq, 0.0);
+                    MdLFS = cap(gam_d * Mu[id(base)], M_max);
+                    for (int i = base + 1; i <= lfs; i++) {
+                        const size_t q = id(i);
+                        const double t_u = T[q] * t0;
+                        Eu[q] = entrain(i, true);
+                        double d = eps_u * Mu[q];
+                        if (ip > 1 && i == lfs) d = del_u * Mu[q] + b_u * Mu[q] / step[i];
+                        if (ip > 1 && i == lfs - 1) d = del_u * Mu[q] + (1.0 - b_u) * Mu[q] / step[i];
+                        if (t_u <= t_00_m2) d = 0.0;
+                        Du[q] = d;
+                        Mu[q] = cap(Mu[id(i - 1)] + (Eu[q] - Du[q]) * step[i], M_max);
+                        Cu[q] = 0.0;
+                        if (ip == 1) {
+                            El[q] = 0.0;
+                        } else {
+                            El[q] = Du[q] * Qcu[q] / dmax(RH[q], 0.01);
+                            if (El[q] <= 0.0) El[q] = 0.0;
+                        }
+                        const double Kp = (h[i] > hb + dic) ? bet_p : 0.0;
+                        Gp[q] = (t_u >= t0 && ip != 1) ? Kp * Qcu[q] : 0.0;
+                    }
+                    Mu[id(lfs)] = 0.0;
+                }
+                cloud_base_update();
+                // updraftRecurrence (:1032-1163)
+                if (inner && base >= 0 && lfs > base) {
+                    for (int i = base + 1; i <= lfs - 1; i++) {
+                        const size_t q = id(i), qm = id(i - 1);
+                        const double t_u = T[qm] * t0;
+                        const double sp = step[i - 1];
+                        Mu[q] = cap(Mu[qm] + (Eu[qm] - Du[qm]) * sp, M_max);
+                        if (t_u <= t_00_m2) Mu[q] = 0.0;
+                        const double Mp = Mu[qm];
+                        const double n_qv = Mp * Qvu[qm] + sp * (Eu[qm] * C[qm] - Du[qm] * Qvu[qm]);
+                        const double n_qc = Mp * Qcu[qm] + sp * (-Du[qm] * CL[qm] - RH[qm] * Gp[qm]);
+                        const double n_v = Mp * Vu[qm] + sp * Eu[qm] * V[qm];
+                        const double n_w = Mp * Wu[qm] + sp * Eu[qm] * W[qm];
+                        const double n_s = (Mp * Su[qm] + sp * (Eu[qm] * Sx[qm] - Du[qm] * Su[qm])) / s_0;
+                        if (std::fabs(Mu[q]) > coeff_recurr) {
+                            const double r = 1.0 / Mu[q];
+                            Qvu[q] = n_qv * r; Qcu[q] = n_qc * r; Vu[q] = n_v * r; Wu[q] = n_w * r; Su[q] = n_s * r;
+                        } else {
+                            Su[q] = Sx[q]; Qvu[q] = C[q]; Qcu[q] = 0.0; Vu[q] = V[q]; Wu[q] = W[q];
+                        }
+                        Cu[q] = 0.0;
+                        if (std::fabs(Mu[q]) > coeff_recurr) {   // parcel saturation adjustment, two passes
+                            double Tu = Su[q] * s_0 / cp_l;
+                            const double pu = PS[q];
+                            double tot = 0.0;
+                            for (int it = 0; it < 2; ++it) {
+                                const double ex = Qvu[q] - qs_env(p_auto(Tu), pu, C[q]);
+                                if (ex <= 0.0) break;
+                                const double L = (Tu >= t0) ? lv : ls;
+                                const double G = (L / cp_l) * dqdT_from(p_auto(Tu), L_auto(Tu), Tu, pu,
+                                                                        M_nonwater(C[q], CO2[q], Mb));
+                                const double dc = ex / (1.0 + G);
+                                Qvu[q] -= dc;
+                                Qcu[q] += dc;
+                                Tu += (L / cp_l) * dc;
+                                tot += dc;
+                            }
+                            Su[q] = cp_l * Tu / s_0;
+                            Cu[q] = tot * Mu[q] / (dmax(RH[q], 0.01) * step[i]);
+                        }
+                        Uu[q] = dmax(-uu_cap, dmin(U[q] + Mu[q] * inv_a_u / dmax(RH[q], 1e-6), uu_cap));
+                        if (Qvu[q] <= 0.0) Qvu[q] = 0.0;
+                        if (Qcu[q] <= 0.0) Qcu[q] = 0.0;
+                        if (Qcu[q] > q_c_u_max) Qcu[q] = q_c_u_max;
+                        if (Su[q] <= 0.0) Su[q] = 0.0;
+                        if (t_u <= t_00) {
+                            Eu[q] = Du[q] = Mu[q] = Qvu[q] = Qcu[q] = Uu[q] = Vu[q] = Wu[q] = 0.0;
+                            Su[q] = 1.0;
+                            Gp[q] = Cu[q] = El[q] = 0.0;
+                        }
+                    }
+                }
+                // downdraftEntrainment (:913-1027)
+                if (lfs > 0 && lfs > base) {
+                    const size_t ql = id(lfs);
+                    const double t_l = T[ql] * t0, p_l = PS[ql];
+                    Md[ql] = cap(MdLFS, M_max);
+                    Ed[ql] = eps_d * std::fabs(Md[ql]);
+                    Dd[ql] = del_d * std::fabs(Md[ql]);
+                    Qvd[ql] = 0.5 * (CL[ql] + scale * qs_env(p_auto(t_l), p_l, C[ql]));
+                    const double r_l = 1e2 * p_l / (R_of(C[ql], CO2[ql], Rb) * t_l);
+                    Ud[ql] = U[ql] + Md[ql] * inv_a_d / dmax(r_l, 1e-6);
+                    Vd[ql] = V[ql]; Wd[ql] = W[ql]; Sd[ql] = Sx[ql];
+                    Pc[ql] = 0.0;
+                    double sig = 0.0;
+                    const double wcb = std::fabs(Wu[id(base)]) * P.u_0;
+                    if (wcb > 0.0) sig = std::fabs(Mu[id(base)]) / (dmax(RH[id(base)], 1e-6) * wcb);
+                    sig = dmin(sig, 1.0);
+                    for (int i = lfs - 1; i >= 0; i--) {
+                        const size_t q = id(i);
+                        const double t_u = T[q] * t0;
+                        Sd[q] = cp_l * t_u / s_0;
+                        const double qsd = qs_env(p_auto(t_u), PS[q], C[q]);
+                        Qvd[q] = 0.5 * (CL[q] + scale * qsd);
+                        Ed[q] = eps_d * std::fabs(Md[q]);
+                        Dd[q] = del_d * std::fabs(Md[q]);
+                        Md[q] = cap(Md[id(i + 1)] - (Ed[q] - Dd[q]) * step[i], M_max);
+                        Ud[q] = U[q] + Md[q] * inv_a_d / dmax(RH[q], 1e-6);
+                        Vd[q] = V[q]; Wd[q] = W[q];
+                        if (i <= base) {
+                            if (ip == 1) {
+                                Ep[q] = 0.0;
+                            } else {
+                                const double pr = Pc[q];
+                                if (pr > 0.0 && sig > 0.0)
+                                    Ep[q] = sig * alf_1 * (qsd - Qvd[q]) * std::sqrt(1e1 / alf_2 * pr / sig);
+                                if (Ep[q] <= 0.0) Ep[q] = 0.0;
+                            }
+                        }
+                        if (ip == 1) {
+                            Edv[q] = 0.0;
+                        } else {
+                            const double pr = Pc[q];
+                            Edv[q] = (pr > 0.0) ? a_ev * (1.0 + b_ev * std::pow(pr, 1.0 / 6.0)) * (qsd - Qvd[q])
+                                                      * std::pow(pr, 4.0 / 9.0) : 0.0;
+                            if (Edv[q] <= 0.0) Edv[q] = 0.0;
+                        }
+                    }
+                }
+                // downdraftRecurrence (:1168-1254), every column
+                for (int i = lfs - 1; i >= 0; i--) {
+                    const size_t q = id(i), qp = id(i + 1);
+                    const double t_u = T[q] * t0;
+                    const double sp = step[i + 1], rp = RH[qp];
+                    Md[q] = cap(Md[qp] - (Ed[q] - Dd[q]) * sp, M_max);
+                    if (t_u <= t_00_m2) Md[q] = 0.0;
+                    const double L = (t_u >= t0) ? lv : ls;
+                    const double Mp = Md[qp];
+                    const double n_qv = Mp * Qvd[qp] - sp * (Ed[qp] * C[qp] - Dd[qp] * Qvd[qp] + rp * Edv[qp]);
+                    const double n_v = Mp * Vd[qp] - sp * (Ed[qp] * V[qp] - Dd[qp] * Vd[qp]);
+                    const double n_w = Mp * Wd[qp] - sp * (Ed[qp] * W[qp] - Dd[qp] * Wd[qp]);
+                    const double n_s = (Mp * Sd[qp] - sp * (Ed[qp] * Sx[qp] - Dd[qp] * Sd[qp] - L * rp * Edv[qp])) / s_0;
+                    if (std::fabs(Md[q]) > coeff_recurr) {
+                        const double r = 1.0 / Md[q];
+                        Qvd[q] = n_qv * r; Vd[q] = n_v * r; Wd[q] = n_w * r; Sd[q] = n_s * r;
+                    } else {
+                        Qvd[q] = C[q]; Vd[q] = V[q]; Wd[q] = W[q]; Sd[q] = Sx[q];
+                    }
+                    Ud[q] = U[q] + Md[q] * inv_a_d / dmax(RH[q], 1e-6);
+                    if (Qvd[q] <= 0.0) Qvd[q] = 0.0;
+                    if (Sd[q] <= 0.0) Sd[q] = 0.0;
+                    if (t_u <= t_00) {
+                        Ed[q] = Dd[q] = Md[q] = Qvd[q] = Ud[q] = Vd[q] = Wd[q] = 0.0;
+                        Sd[q] = 1.0;
+                        Edv[q] = Ep[q] = 0.0;
+                    }
+                    if (i > 0) Pc[q] = dmax(0.0, Pc[qp] + step[i] * RH[q] * (Gp[q] - Edv[q] - Ep[q]));
+                }
+                Pc[id(0)] = Pc[id(1)];
+            }
+            // rhsForcing (:1304-1417)
+            for (int i = 0; i < im; i++) { Mu[id(i)] = cap(Mu[id(i)], M_max); Md[id(i)] = cap(Md[id(i)], M_max); }
+            for (int i = 0; i < im - 1; i++) {
+                const size_t q = id(i), qp = id(i + 1);
+                const double L = (T[q] * t0 >= t0) ? lv : ls;
+                const double isr = 1.0 / (step[i] * dmax(RH[q], 0.01));
+                auto flux = [&](size_t x, const double* up, const double* dn, const double* env) {
+                    return Mu[x] * (up[x] - env[x]) + Md[x] * (dn[x] - env[x]);
+                };
+                const double src = Cu[q] - Edv[q] - El[q] - Ep[q];
+                MCt[q] = cap(-(flux(qp, Su, Sd, Sx) - flux(q, Su, Sd, Sx)) * isr * t0 + (L / cp_l) * src * t0, MCt_max);
+                MCq[q] = cap(-(flux(qp, Qvu, Qvd, C) - flux(q, Qvu, Qvd, C)) * isr - src, MCq_max);
+                MCv[q] = cap(-(flux(qp, Vu, Vd, V) - flux(q, Vu, Vd, V)) * isr * P.u_0, MCv_max);
+                MCw[q] = cap(-(flux(qp, Wu, Wd, W) - flux(q, Wu, Wd, W)) * isr * P.u_0, MCv_max);
+            }
+        }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
@@ -1868,6 +2295,16 @@ void orc_set_ice_scheme(orc_state* s, int on) {
     if (s->ice_on) ensure_mp(*s);
 }
 void orc_ice_store(orc_state* s) { ice_store(*s); }
+void orc_moist_convection(orc_state* s) { moist_convection(*s); }
+double* orc_mc_ptr(orc_state* s, int which) {
+    if (which < 0 || which >= ORC_NMC) return nullptr;
+    ensure_mc(*s);
+    return s->mc[which].data();
+}
+double* orc_evap_ptr(orc_state* s) {
+    ensure_mc(*s);
+    return s->evap.data();
+}
 double* orc_mp_ptr(orc_state* s, int which) {
     if (which < 0 || which >= ORC_NMP) return nullptr;
     ensure_mp(*s);

@@ -108,6 +108,20 @@ void    orc_ice_store(orc_state* s);
 // The arrays above (im*jm*km doubles, [i][j][k]); allocated, zero, at first use.
 double* orc_mp_ptr(orc_state* s, int which);
 
+// Moist convection (MoistConvection.h:106-140): the convection arrays, in the order the
+// reference harness writes them, and one call of the scheme on the state's fields.
+enum orc_mc {
+    ORC_MC_S = 0, ORC_MC_M_U, ORC_MC_M_D, ORC_MC_S_U, ORC_MC_Q_V_U, ORC_MC_Q_C_U, ORC_MC_C_U,
+    ORC_MC_E_L, ORC_MC_E_D_EVAP, ORC_MC_E_P, ORC_MC_G_P, ORC_MC_S_D, ORC_MC_Q_V_D, ORC_MC_P_CONV,
+    ORC_MC_E_U, ORC_MC_D_U, ORC_MC_U_U, ORC_MC_V_U, ORC_MC_W_U, ORC_MC_E_D, ORC_MC_D_D, ORC_MC_U_D,
+    ORC_MC_V_D, ORC_MC_W_D, ORC_MC_MC_T, ORC_MC_MC_Q, ORC_MC_MC_V, ORC_MC_MC_W, ORC_MC_DEEP_BEG,
+    ORC_MC_DEEP_END, ORC_MC_CLOUD_BASE, ORC_MC_LFS,
+    ORC_NMC
+};
+void    orc_moist_convection(orc_state* s);
+double* orc_mc_ptr(orc_state* s, int which);      // im*jm*km doubles
+double* orc_evap_ptr(orc_state* s);               // jm*km doubles, the Evaporation diagnostic [mm/d]
+
 // Individual thermodynamic functions (known-answer tests).
 double orc_p_sat(double T);
 double orc_p_subl(double T);

@@ -90,6 +90,11 @@ def lib():
         L.orc_set_ice_scheme.argtypes = [vp, i]
         L.orc_mp_ptr.restype = dp
         L.orc_mp_ptr.argtypes = [vp, i]
+        L.orc_moist_convection.argtypes = [vp]
+        L.orc_mc_ptr.restype = dp
+        L.orc_mc_ptr.argtypes = [vp, i]
+        L.orc_evap_ptr.restype = dp
+        L.orc_evap_ptr.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -198,6 +203,22 @@ class OracleState:
     def mp_snapshot(self) -> dict:
         return {n: self.mp(n).copy() for n in MP_FIELDS}
 
+    def moist_convection(self):
+        """One call of MoistConvection::run on the state's fields (MoistConvection.h:106-140)."""
+        lib().orc_moist_convection(self.h)
+
+    def mc(self, name) -> np.ndarray:
+        """A live (writable) view of one moist-convection array (MC_FIELDS)."""
+        p = lib().orc_mc_ptr(self.h, MC_FIELDS.index(name))
+        return np.ctypeslib.as_array(p, shape=self.shape)
+
+    def evap(self) -> np.ndarray:
+        """A live view of the per-column evaporation diagnostic [mm/d] convection reads."""
+        return np.ctypeslib.as_array(lib().orc_evap_ptr(self.h), shape=self.shape[1:])
+
+    def mc_snapshot(self) -> dict:
+        return {n: self.mc(n).copy() for n in MC_FIELDS}
+
     def level_mean(self):
         return np.array([lib().orc_level_mean(self.h, i) for i in range(self.shape[0])])
 
@@ -214,6 +235,9 @@ class OracleState:
 
 RAD_OUTPUTS = ["radiation", "epsilon", "albedo", "sw"]
 # Arrays of the two-category ice scheme, in the order of oracle/athad_oracle.h's orc_mp.
+MC_FIELDS = ["s", "M_u", "M_d", "s_u", "q_v_u", "q_c_u", "c_u", "e_l", "e_d", "e_p", "g_p", "s_d", "q_v_d",
+             "P_conv", "E_u", "D_u", "u_u", "v_u", "w_u", "E_d", "D_d", "u_d", "v_d", "w_d", "MC_t", "MC_q",
+             "MC_v", "MC_w", "Deep_beg", "Deep_end", "CloudBase", "LevelFreeSinking"]
 MP_FIELDS = ["P_rain", "P_snow", "P_rainn", "P_snown", "Precipitation", "S_v", "S_c", "S_i", "S_r", "S_s"]
 
 

@@ -21,6 +21,11 @@
 //                           block (after the damping of t ice c cloud), its flux damping and
 //                           precipitationSum (moist convection not run: P_conv stays zero);
 //                           the .mp files as for mode ice
+//   mode = conv             n calls of MoistConvection::run (its parameters as the model's
+//                           defaults set them: iter_prec 3, convection_perturbation 1,
+//                           convection_mode 1); reads <in.bin>.evap (the Evaporation diagnostic,
+//                           jm*km doubles) and, if present, <in.bin>.mc, and writes <out.bin>.mc:
+//                           the 32 convection arrays in mc_fields order
 // The radiation modes also write <out.bin>.rad: radiation and epsilon (im*jm*km doubles each),
 // albedo (jm*km), the top short wave (jm) and t_skin.
 // State file: int32 im,jm,km,iter_n,total_iter; then 35 fields of im*jm*km doubles
@@ -34,6 +39,7 @@
 #include "TurbulenceAtm.h"
 #include "MultiLayerRadiation.h"
 #include "TwoCatIceScheme.h"
+#include "MoistConvection.h"
 
 #include <cstdio>
 #include <cstring>
@@ -303,6 +309,44 @@ public:
         }
     }
 
+    static std::vector<Array*> mc_fields(cAtmosphereModel& m) {
+        return {&m.s, &m.M_u, &m.M_d, &m.s_u, &m.q_v_u, &m.q_c_u, &m.c_u, &m.e_l, &m.e_d, &m.e_p,
+                &m.g_p, &m.s_d, &m.q_v_d, &m.P_conv, &m.E_u, &m.D_u, &m.u_u, &m.v_u, &m.w_u,
+                &m.E_d, &m.D_d, &m.u_d, &m.v_d, &m.w_d, &m.MC_t, &m.MC_q, &m.MC_v, &m.MC_w,
+                &m.Deep_beg, &m.Deep_end, &m.CloudBase, &m.LevelFreeSinking};
+    }
+
+    static void io_fields(cAtmosphereModel& m, const std::vector<Array*>& arrs, const std::string& fn, bool rd) {
+        std::vector<double> buf((size_t)m.im * m.jm * m.km);
+        std::fstream f(fn, rd ? std::ios::in | std::ios::binary : std::ios::out | std::ios::binary);
+        if (!f) return;
+        for (Array* a : arrs) {
+            if (rd) f.read((char*)buf.data(), buf.size() * 8);
+            size_t q = 0;
+            for (int i = 0; i < m.im; i++)
+                for (int j = 0; j < m.jm; j++)
+                    for (int k = 0; k < m.km; k++) {
+                        if (rd) a->x[i][j][k] = buf[q++]; else buf[q++] = a->x[i][j][k];
+                    }
+            if (!rd) f.write((const char*)buf.data(), buf.size() * 8);
+        }
+    }
+
+    static bool conv(cAtmosphereModel& m, const std::string& in, const std::string& out, int n) {
+        std::ifstream fe(in + ".evap", std::ios::binary);
+        std::vector<double> b2((size_t)m.jm * m.km);
+        fe.read((char*)b2.data(), b2.size() * 8);
+        if (!fe) return false;
+        for (int j = 0; j < m.jm; j++)
+            for (int k = 0; k < m.km; k++) m.Evaporation.y[j][k] = b2[(size_t)j * m.km + k];
+        io_fields(m, mc_fields(m), in + ".mc", true);
+        std::fprintf(stderr, "conv: iter_prec %d perturbation %d mode %d p_0 %.17g s_0 %.17g\n",
+                     m.iter_prec, m.convection_perturbation, m.convection_mode, m.p_0, m.s_0);
+        for (int q = 0; q < n; q++) MoistConvection(m).run(m.iter_n);
+        io_fields(m, mc_fields(m), out + ".mc", false);
+        return true;
+    }
+
     static void ice(cAtmosphereModel& m) {
         TwoCatIceScheme(m).run();
         damp(m, m.P_rain);
@@ -357,6 +401,9 @@ int main(int argc, char** argv) {
         if (!RadiationSelfTest::load_mp(*m, std::string(argv[1]) + ".mp")) return 1;
         RadiationSelfTest::ice(*m);
         RadiationSelfTest::save_mp(*m, std::string(argv[2]) + ".mp");
+    } else if (mode == "conv") {
+        const int n = (argc > 4) ? std::atoi(argv[4]) : 1;
+        if (!RadiationSelfTest::conv(*m, argv[1], argv[2], n)) return 1;
     } else if (mode.rfind("op:", 0) == 0) {
         RadiationSelfTest::op(*m, mode.substr(3));
     } else if (mode == "init") {

@@ -9,6 +9,7 @@ in the tests from the same seed, so only outputs are stored.
     python tests/golden/make_ref_vectors.py               # ref_vectors.npz
     python tests/golden/make_ref_vectors.py --radiation   # ref_radiation.npz
     python tests/golden/make_ref_vectors.py --ice         # ref_ice.npz
+    python tests/golden/make_ref_vectors.py --conv        # ref_conv.npz
 """
 from __future__ import annotations
 
@@ -166,9 +167,57 @@ def ice_vectors():
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+CONV_EVAP_SEED = 11   # numpy seed of the evaporation diagnostic convection reads
+
+
+def conv_evaporation(grid):
+    """The per-column evaporation input [mm/d] of the moist-convection vectors: uniform in
+    [-1, 6), so some columns have none (the surface-flux perturbation then stays zero)."""
+    return np.random.default_rng(CONV_EVAP_SEED).uniform(-1.0, 6.0, grid[1:])
+
+
+def conv_vectors():
+    """Moist convection (MoistConvection::run) twice in a row on the Jacobi-spun-up state with
+    the ice-scheme vectors' planted condensate (so convective precipitation survives the
+    downdraft evaporation), from zeroed convection arrays; the second call reads what the first
+    left in the arrays the scheme does not clear (E_u, D_u, the updraft and downdraft winds)."""
+    wd = tempfile.mkdtemp(prefix="athad_golden_conv_")
+    n = GRID[0] * GRID[1] * GRID[2]
+    idx = sample_index(n)
+    out = {"grid": np.array(GRID), "seed": np.array(SEED), "spin": np.array(SPIN), "plant": np.array(ICE_PLANT),
+           "evap_seed": np.array(CONV_EVAP_SEED), "index": idx}
+    st = O.OracleState(*GRID, seed=SEED)
+    st.run(SPIN, "jacobi")
+    snap = st.snapshot()
+    O.plant_microphysics(snap, {k: np.zeros(GRID) for k in O.MP_FIELDS}, ICE_PLANT)
+    st = O.OracleState(*GRID, blank=True)
+    st.load(snap)
+    st.set_iter(SPIN, SPIN)
+    base = os.path.join(wd, "base.bin")
+    st.save_bin(base, iter_n=SPIN)
+    conv_evaporation(GRID).tofile(base + ".evap")
+    for call in (1, 2):
+        o = os.path.join(wd, f"out{call}.bin")
+        subprocess.run([HARNESS, base, o, "conv", str(call)], check=True, cwd=wd, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS="8"))
+        ref = np.fromfile(o + ".mc").reshape(len(O.MC_FIELDS), n)
+        for q, name in enumerate(O.MC_FIELDS):
+            a = ref[q]
+            out[f"call{call}/{name}/sample"] = a[idx]
+            out[f"call{call}/{name}/sum"] = np.array(np.sum(a))
+            out[f"call{call}/{name}/l2"] = np.array(np.sqrt(np.sum(a * a)))
+        print(f"convection call {call} done", flush=True)
+    path = os.path.join(HERE, "ref_conv.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path)
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle -f Makefile.ref")
+    if "--conv" in sys.argv[1:]:
+        conv_vectors()
+        return
     if "--ice" in sys.argv[1:]:
         ice_vectors()
         return
