@@ -28,7 +28,7 @@ LEX, JACOBI, SYMMETRIC = 0, 1, 2
 
 KERNEL_ID = {"rk4": 0, "rk4_plane": 1, "level_mean": 2, "poisson": 3, "satadj": 4, "damp": 5,
              "densities": 6, "turb": 7, "bc": 8, "solid": 9, "polar": 10, "radial": 11,
-             "store": 12, "init": 13, "ice": 14}
+             "store": 12, "init": 13, "ice": 14, "conv": 15}
 
 # Exported entry points (include/athad_hip.h)
 EXPORTS = [
@@ -42,10 +42,14 @@ EXPORTS = [
     "athad_field_stats", "athad_set_member_params", "athad_restart_bytes", "athad_save_state_mem",
     "athad_load_state_mem", "athad_save_state", "athad_load_state", "athad_field_minmax", "athad_residuum",
     "athad_meridional_streamfunction", "athad_get_slice", "athad_radiation", "athad_get_radiation",
-    "athad_set_radiation", "athad_ice_scheme", "athad_set_ice_scheme", "athad_set_mp_field", "athad_get_mp_field",
+    "athad_set_radiation", "athad_moist_convection", "athad_set_conv_field", "athad_get_conv_field",
+    "athad_ice_scheme", "athad_set_ice_scheme", "athad_set_mp_field", "athad_get_mp_field",
 ]
 
 # Arrays of the two-category ice scheme (athad_mp_field in include/athad_hip.h)
+CONV_FIELDS = ["s", "M_u", "M_d", "s_u", "q_v_u", "q_c_u", "c_u", "e_l", "e_d", "e_p", "g_p", "s_d", "q_v_d",
+               "P_conv", "E_u", "D_u", "u_u", "v_u", "w_u", "E_d", "D_d", "u_d", "v_d", "w_d", "MC_t", "MC_q",
+               "MC_v", "MC_w", "Deep_beg", "Deep_end", "CloudBase", "LevelFreeSinking", "Evaporation"]
 MP_FIELDS = ["P_rain", "P_snow", "P_rainn", "P_snown", "Precipitation", "S_v", "S_c", "S_i", "S_r", "S_s"]
 
 # restart_arrays() of the reference checkpoint (atmosphere/FileIO_Atm.cpp:297-304), in file
@@ -133,6 +137,9 @@ def lib():
         L.athad_set_radiation.argtypes = [vp, i]
         L.athad_set_ice_scheme.argtypes = [vp, i]
         L.athad_set_mp_field.argtypes = [vp, i, i, dp]
+        L.athad_moist_convection.argtypes = [vp]
+        L.athad_set_conv_field.argtypes = [vp, i, i, dp]
+        L.athad_get_conv_field.argtypes = [vp, i, i, dp]
         L.athad_get_mp_field.argtypes = [vp, i, i, dp]
         L.athad_residuum.argtypes = [vp, i, dp, ip, dp, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64), ip]

@@ -84,6 +84,8 @@ struct MemberHost {
     int* passes = nullptr;
     double *radiation = nullptr, *epsilon = nullptr, *albedo = nullptr;   // radiation outputs, on first use
     void* mp[NMP] = {};            // ice-scheme arrays, on first use
+    double* mc[NMC] = {};          // moist-convection arrays, on first use
+    double *evap = nullptr, *conv_cos = nullptr;
     int iter_n = 0, total = 0;     // host mirror of the device counters
     // table offsets inside 'tables'
     size_t o_rad = 0, o_sint = 0, o_cost = 0, o_sinr = 0, o_hgt = 0, o_nuef = 0, o_step = 0;
@@ -191,6 +193,7 @@ void fill_consts(MemberHost& m, const Config& cfg, int order) {
     K.t_000 = cfg.t_000;
     K.coeff_energy = 1.0 / (m.p.c_0 * cfg.cp_l * cfg.t_0);
     K.lv = cfg.lv; K.ls = cfg.ls;
+    K.c_0 = m.p.c_0; K.co2_0 = m.p.co2_0; K.r_earth = cfg.r_Earth;
     K.sigma = cfg.sigma; K.kappa_h2o = cfg.kappa_H2O; K.kappa_co2 = cfg.kappa_CO2; K.kappa_bg = cfg.kappa_bg;
     K.geothermal = cfg.geothermal_flux; K.sw_equator = cfg.rad_equator_short; K.sw_pole = cfg.rad_pole_short;
     K.albedo_surface = cfg.albedo_surface; K.albedo_cloud = cfg.albedo_cloud; K.omega_rad = cfg.omega_rad;
@@ -277,6 +280,8 @@ void make_desc(const athad_ctx* c, const MemberHost& m, int parity, MemberDesc& 
     d.step_dim = m.tables + m.o_step; d.passes = m.passes; d.counters = m.counters; d.scratch = m.scratch;
     d.radiation = m.radiation; d.epsilon = m.epsilon; d.albedo = m.albedo;
     for (int q = 0; q < NMP; q++) d.mp[q] = m.mp[q];
+    for (int q = 0; q < NMC; q++) d.mc[q] = m.mc[q];
+    d.evap = m.evap; d.conv_cos = m.conv_cos;
     (void)c;
 }
 
@@ -314,6 +319,9 @@ void free_ctx(athad_ctx* c) {
         for (auto p : m.alt) if (p) hipFree(p);
         for (auto p : m.tmp) if (p) hipFree(p);
         for (auto p : m.mp) if (p) hipFree(p);
+        for (auto p : m.mc) if (p) hipFree(p);
+        if (m.evap) hipFree(m.evap);
+        if (m.conv_cos) hipFree(m.conv_cos);
         for (void* p : {(void*)m.vel_star, (void*)m.t_top, (void*)m.t_ref, (void*)m.tables, (void*)m.scratch,
                         (void*)m.counters, (void*)m.passes, (void*)m.radiation, (void*)m.epsilon, (void*)m.albedo})
             if (p) hipFree(p);
@@ -1077,6 +1085,91 @@ athad_status athad_get_mp_field(athad_ctx* ctx, int member, int which, double* d
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         for (size_t q = 0; q < m.n; q++) dst[q] = (double)tmp[q];
     }
+    return ATHAD_OK;
+}
+
+// ---- moist convection (MoistConvection.h:106-140) ---------------------------------------
+// The 32 convection arrays, the evaporation input and the latitude factor table are allocated,
+// zero, at the first call that needs them, for every member at once.
+static athad_status ensure_mc_buffers(athad_ctx* c) {
+    if (c->mem[0].mc[0]) return ATHAD_OK;
+    if (c->capturing) { c->err = "the convection arrays cannot be allocated during capture"; return ATHAD_ERR_STATE; }
+    auto fail = [&]() {
+        for (auto& m2 : c->mem) {
+            for (auto& p : m2.mc) if (p) { hipFree(p); p = nullptr; }
+            if (m2.evap) { hipFree(m2.evap); m2.evap = nullptr; }
+            if (m2.conv_cos) { hipFree(m2.conv_cos); m2.conv_cos = nullptr; }
+        }
+        c->err = "out of device memory for the convection arrays";
+        return ATHAD_ERR_ALLOC;
+    };
+    for (auto& m : c->mem) {
+        const int jm = m.g.jm, km = m.g.km;
+        for (int q = 0; q < NMC; q++) {
+            if (hipMalloc(&m.mc[q], m.n * 8) != hipSuccess) { m.mc[q] = nullptr; return fail(); }
+            HIPCHK(c, hipMemset(m.mc[q], 0, m.n * 8));
+        }
+        if (hipMalloc(&m.evap, (size_t)jm * km * 8) != hipSuccess) { m.evap = nullptr; return fail(); }
+        HIPCHK(c, hipMemset(m.evap, 0, (size_t)jm * km * 8));
+        if (hipMalloc(&m.conv_cos, (size_t)jm * 8) != hipSuccess) { m.conv_cos = nullptr; return fail(); }
+        // theta by accumulation as the grid builds it; the circle factor is taken about row 90
+        // (MoistConvection.h:239), so the scheme needs jm > 90
+        std::vector<double> the(jm), cs(jm);
+        the[0] = 0.0;
+        for (int l = 1; l < jm; l++) the[l] = the[l - 1] + m.K.dthe;
+        for (int j = 0; j < jm; j++) cs[j] = std::cos(the[90] - the[j]);
+        HIPCHK(c, hipMemcpy(m.conv_cos, cs.data(), (size_t)jm * 8, hipMemcpyHostToDevice));
+    }
+    if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    return upload_descs(c);
+}
+
+static athad_status check_conv_grid(athad_ctx* c) {
+    for (auto& m : c->mem)
+        if (m.g.jm <= 90 || m.g.km < 3 || m.g.im < 3) {
+            c->err = "moist convection needs jm > 90 (its circle factor is taken about row 90), km >= 3, im >= 3";
+            return ATHAD_ERR_ARG;
+        }
+    return ATHAD_OK;
+}
+
+athad_status athad_moist_convection(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    athad_status s = check_conv_grid(c);
+    if (s || (s = ensure_mc_buffers(c))) return s;
+    join_s2(c);
+    { Prof p(c, KID_CONV); launch_moist_convection(cur(c)); }
+    return after_launch(c);
+}
+
+athad_status athad_set_conv_field(athad_ctx* ctx, int member, int which, const double* src) {
+    athad_status s = check_member(ctx, member, which, NMC + 1);
+    if (s) return s;
+    if (!src) return ATHAD_ERR_ARG;
+    if ((s = check_conv_grid(ctx)) || (s = ensure_mc_buffers(ctx))) return s;
+    join_s2(ctx);
+    MemberHost& m = ctx->mem[member];
+    double* dst = (which == NMC) ? m.evap : m.mc[which];
+    const size_t cnt = (which == NMC) ? (size_t)m.g.jm * m.g.km : m.n;
+    HIPCHK(ctx, hipMemcpyAsync(dst, src, cnt * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return ATHAD_OK;
+}
+
+athad_status athad_get_conv_field(athad_ctx* ctx, int member, int which, double* dst) {
+    athad_status s = check_member(ctx, member, which, NMC + 1);
+    if (s) return s;
+    if (!dst) return ATHAD_ERR_ARG;
+    join_s2(ctx);
+    MemberHost& m = ctx->mem[member];
+    const size_t cnt = (which == NMC) ? (size_t)m.g.jm * m.g.km : m.n;
+    const double* src = (which == NMC) ? m.evap : m.mc[which];
+    if (!src) {   // never allocated: zero, as resetArrays leaves them
+        std::fill(dst, dst + cnt, 0.0);
+        return ATHAD_OK;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(dst, src, cnt * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return ATHAD_OK;
 }
 

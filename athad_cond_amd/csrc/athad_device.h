@@ -22,6 +22,13 @@ constexpr int NPROG = 11;   // t u v w c cloud ice gr co2 tke dis
 constexpr int NTMP = 12;    // scratch fields for the filter passes
 // Arrays of the two-category ice scheme (same numbering as include/athad_hip.h): the rain and
 // snow fluxes, their stored copies, the capped total and the microphysics sources.
+// Moist-convection arrays (MoistConvection.h), in the reference harness's order.
+enum McField {
+    MC_S = 0, MC_M_U, MC_M_D, MC_S_U, MC_Q_V_U, MC_Q_C_U, MC_C_U, MC_E_L, MC_E_D_EVAP, MC_E_P, MC_G_P,
+    MC_S_D, MC_Q_V_D, MC_P_CONV, MC_E_U, MC_D_U, MC_U_U, MC_V_U, MC_W_U, MC_E_D, MC_D_D, MC_U_D, MC_V_D,
+    MC_W_D, MC_MC_T, MC_MC_Q, MC_MC_V, MC_MC_W, MC_DEEP_BEG, MC_DEEP_END, MC_CLOUD_BASE, MC_LFS, NMC
+};
+
 enum MpField { MP_P_RAIN = 0, MP_P_SNOW, MP_P_RAINN, MP_P_SNOWN, MP_PRECIP, MP_S_V, MP_S_C, MP_S_I, MP_S_R, MP_S_S, NMP };
 
 // Per-member constants, all resolved on the host in double.
@@ -47,6 +54,7 @@ struct Consts {
     // initial-state recipe
     double t_surf_eq, t_surf_pole, init_M_nw, init_f_co2, co2_scale;
     double t_000;       // lower bound of the snow band of the ice scheme
+    double c_0, co2_0, r_earth;   // reference water and CO2 mass fractions, planet radius [km]
     double coeff_energy, lv, ls;   // latent heating of the microphysics sources (RHS_Atm_Turb.cpp:367, 965-966)
     unsigned long long seed;
     // per-member state knobs
@@ -76,6 +84,9 @@ struct MemberDesc {
     double* epsilon;         // n        layer emissivity
     double* albedo;          // jm*km    surface short-wave albedo
     void* mp[NMP];           // n        ice-scheme arrays, storage precision S (allocated at first use)
+    double* mc[NMC];         // n        moist-convection arrays, FP64 (allocated at first use)
+    double* evap;            // jm*km    evaporation diagnostic [mm/d] the convection trigger reads
+    const double* conv_cos;  // jm       cos(theta[90] - theta[j]), the convection's zonal circle factor
 };
 
 #define ATHAD_DEV __device__ __forceinline__

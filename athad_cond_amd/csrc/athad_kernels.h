@@ -18,7 +18,7 @@ struct Batch {
 // Kernel ids for per-kernel timing.
 enum KernelId {
     KID_RK4 = 0, KID_RK4_PLANE, KID_LEVEL_MEAN, KID_POISSON, KID_SATADJ, KID_DAMP, KID_DENSITIES,
-    KID_TURB, KID_BC, KID_SOLID, KID_POLAR, KID_RADIAL, KID_STORE, KID_INIT, KID_ICE, KID_COUNT
+    KID_TURB, KID_BC, KID_SOLID, KID_POLAR, KID_RADIAL, KID_STORE, KID_INIT, KID_ICE, KID_CONV, KID_COUNT
 };
 
 void launch_advance(const Batch& b);
@@ -51,6 +51,7 @@ void launch_skin_step(const Batch& b, MemberDesc* md0, MemberDesc* md1, double r
 // must hold their ice-scheme arrays (MemberDesc::mp). Scratch 6-9 are free for it at the place of
 // the ice scheme inside the moist block (the damping has consumed them).
 void launch_ice_scheme(const Batch& b, int gate);
+void launch_moist_convection(const Batch& b);
 // precipitationSum, cap_S and the microphysics clamp that follow the ice scheme in the moist
 // block; store != 0 also copies the fluxes to their stored arrays
 void launch_moist_clamp_ice(const Batch& b, int gate, int store);

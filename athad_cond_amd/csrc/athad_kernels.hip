@@ -3435,3 +3435,448 @@ void launch_init_rest(const Batch& b) {
 }
 
 }  // namespace athad
+
+namespace athad {
+
+// ------------------------------------------------------------------------------------
+// Moist convection, MoistConvection::run (MoistConvection.h:106-140), with the model's
+// defaults iter_prec = 3, convection_perturbation = 1, convection_mode = 1 and no land.
+// One thread per (j, k) column. Every phase of the reference reads only its own column's
+// convection arrays and the fields c, cloud, t, u, v, w, p_stat, r_humid, co2 (c also at
+// the four horizontal neighbours, which nothing here writes), so a thread runs the phases of
+// its column back to back: precompute's clearing, findCloudBaseLFS, the surface-flux
+// perturbation, initUpdraft, CloudBaseUpdate, three rounds of updraft entrainment / base
+// update / updraft recurrence / downdraft entrainment / downdraft recurrence, rhsForcing.
+// computeCAPE is not run (a 1-D diagnostic that ends holding the last column's values and
+// feeds nothing). The convection arrays are FP64 in every precision; fields are read in
+// their storage precision S. Consecutive threads take consecutive k, so every level's loads
+// and stores are coalesced across the wave.
+// ------------------------------------------------------------------------------------
+namespace conv {
+constexpr double t_00 = 236.15, t_00_m2 = 236.15 - 2.0;
+constexpr double eps_u = 0.2 / 100.0, del_u = 0.2 / 100.0, b_u = 0.33;   // MoistConvection.h:21-88
+constexpr double eps_d = 1.0e-4, del_d = 1.0e-4, gam_d = -0.3, a_u = 3.0e-2, a_d = 3.0e-2, vel = 1.1;
+constexpr double a_ev = 1.0e-3, b_ev = 5.9, alf_1 = 0.05, alf_2 = 0.05, bet_p = 2.0e-3;
+constexpr double f_beg = 1000.0 / 1013.25, f_end = 970.0 / 1013.25, f_cb = 900.0 / 1013.25;
+constexpr double f_diff = 200.0 / 1013.25, f_mid = 700.0 / 1013.25;
+constexpr double t_add_u = 0.2, q_v_u_add = 1.0e-4, coeff_recurr = 0.1, scale = 0.98;
+constexpr double q_c_u_max = 1.0e-2, w_conv_max = 50.0, M_max = 3.0, c_mb = 0.003, alpha_sfp = 0.2;
+constexpr double MCt_max = 0.01, MCq_max = 2.0e-5, MCv_max = 0.01;     // :1323-1339
+constexpr double cp_l = 1349.0, s_0 = 274515.75, p_0 = 60000.0;       // cAtmosphereDefaults.cpp.inc:25,99,109
+
+ATHAD_DEV double cap(double v, double mag) {                          // :216-222, :1341-1346
+    if (!finite_bits(v)) return 0.0;
+    return (v < -mag) ? -mag : (v > mag) ? mag : v;
+}
+}  // namespace conv
+
+template <typename S>
+__global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __restrict__ mds) {
+    using namespace conv;
+    const MemberDesc& d = mds[blockIdx.z];
+    const Consts& K = d.K;
+    const int im = K.im, jm = K.jm, km = K.km;
+    const int col = blockIdx.x * 64 + threadIdx.x;
+    if (col >= jm * km) return;
+    const int j = col / km, k = col - j * km;
+    const size_t pl = (size_t)jm * km;
+    const S* __restrict__ T = (const S*)d.f[F_T];
+    const S* __restrict__ U = (const S*)d.f[F_U];
+    const S* __restrict__ V = (const S*)d.f[F_V];
+    const S* __restrict__ W = (const S*)d.f[F_W];
+    const S* __restrict__ C = (const S*)d.f[F_C];
+    const S* __restrict__ CL = (const S*)d.f[F_CLOUD];
+    const S* __restrict__ CO2 = (const S*)d.f[F_CO2];
+    const S* __restrict__ RH = (const S*)d.f[F_R_HUMID];
+    const double* __restrict__ PS = (const double*)d.f[F_P_STAT];
+    double* __restrict__ Sx = d.mc[MC_S];   double* __restrict__ Mu = d.mc[MC_M_U];
+    double* __restrict__ Md = d.mc[MC_M_D]; double* __restrict__ Su = d.mc[MC_S_U];
+    double* __restrict__ Qvu = d.mc[MC_Q_V_U]; double* __restrict__ Qcu = d.mc[MC_Q_C_U];
+    double* __restrict__ Cu = d.mc[MC_C_U]; double* __restrict__ El = d.mc[MC_E_L];
+    double* __restrict__ Edv = d.mc[MC_E_D_EVAP]; double* __restrict__ Ep = d.mc[MC_E_P];
+    double* __restrict__ Gp = d.mc[MC_G_P]; double* __restrict__ Sd = d.mc[MC_S_D];
+    double* __restrict__ Qvd = d.mc[MC_Q_V_D]; double* __restrict__ Pc = d.mc[MC_P_CONV];
+    double* __restrict__ Eu = d.mc[MC_E_U]; double* __restrict__ Du = d.mc[MC_D_U];
+    double* __restrict__ Uu = d.mc[MC_U_U]; double* __restrict__ Vu = d.mc[MC_V_U];
+    double* __restrict__ Wu = d.mc[MC_W_U]; double* __restrict__ Ed = d.mc[MC_E_D];
+    double* __restrict__ Dd = d.mc[MC_D_D]; double* __restrict__ Ud = d.mc[MC_U_D];
+    double* __restrict__ Vd = d.mc[MC_V_D]; double* __restrict__ Wd = d.mc[MC_W_D];
+    const double* __restrict__ h = d.hgt;
+    const double t0 = K.t_0, Mb = K.M_bg, Rb = K.R_bg, lv = K.lv, ls = K.ls;
+    auto id = [&](int i) { return (size_t)i * pl + col; };
+    auto step = [&](int i) { return (i + 1 < im) ? h[i + 1] - h[i] : h[im - 1] - h[im - 2]; };   // :306-309
+    auto qs_env = [&](double E, double p, double qv) {                // :181-184
+        return th_q_sat(E, p, th_M_nonwater(qv, K.co2_0, Mb));
+    };
+    auto t_of = [&](int i) { return (double)T[id(i)] * t0; };
+    const bool inner = j >= 1 && j <= jm - 2 && k >= 1 && k <= km - 2;
+    const double p_sfc = PS[id(0)];
+    const double inv_a_u = 1.0 / (a_u * K.u_0), inv_a_d = 1.0 / (a_d * K.u_0), uu_cap = w_conv_max / K.u_0;
+
+    // precompute (:336-376)
+    for (int i = 0; i < im; i++) {
+        const size_t q = id(i);
+        Sx[q] = cp_l * (double)T[q] * t0 / s_0;
+        Mu[q] = 0.0; Md[q] = 0.0; Su[q] = 0.0; Qvu[q] = 0.0; Qcu[q] = 0.0; Cu[q] = 0.0; El[q] = 0.0;
+        Edv[q] = 0.0; Ep[q] = 0.0; Gp[q] = 0.0; Sd[q] = 0.0; Qvd[q] = 0.0; Pc[q] = 0.0;
+    }
+    int base = 0, lfs = 0, beg = -1, end = -1;
+    double MdLFS = 0.0, cape_c = 0.0, dT = 0.0, dq = 0.0;
+    // findCloudBaseLFS (:627-777)
+    for (int i = 0; i < im; i++)
+        if (PS[id(i)] <= f_beg * p_sfc) { beg = i; d.mc[MC_DEEP_BEG][id(i)] = h[i]; break; }
+    for (int i = im - 1; i >= 0; i--)
+        if (PS[id(i)] >= f_end * p_sfc) { end = i; d.mc[MC_DEEP_END][id(i)] = h[i]; break; }
+    auto qs_lift = [&](int i) { return qs_env(th_p_auto(t_of(i) + t_add_u), PS[id(i)], (double)C[id(i)]); };
+    if (beg >= 0 && end >= 0) {
+        for (int i = beg; i < im; i++) {
+            const size_t q = id(i);
+            const double t_u = t_of(i), qs = qs_lift(i);
+            const double w = 1.0 / (1.0 + exp(-(t_u - t_00) / 2.0));
+            double qv = (double)C[q] + q_v_u_add, qc = 0.0;
+            if (qv >= scale * qs && t_u >= t_00) qc = qv - scale * qs;
+            if (qv <= 0.0) qv = 0.0;
+            if (qc <= 0.0) qc = 0.0;
+            Qvu[q] = qv; Qcu[q] = qc;
+            if (qv >= scale * qs && w > 0.01 && PS[q] <= f_cb * p_sfc) {
+                base = i;
+                d.mc[MC_CLOUD_BASE][q] = h[i] * w;
+                break;
+            }
+        }
+        if (base > 0) {   // level of neutral buoyancy of the theta-e parcel lifted from cloud base
+            const size_t qb = id(base);
+            const double Tb = t_of(base) + t_add_u;
+            const double kap = th_R_of((double)C[qb], (double)CO2[qb], Rb) / th_cp_of((double)C[qb], (double)CO2[qb], Tb, Mb);
+            const double Lb = (Tb >= t0) ? lv : ls;
+            const double thp = Tb * pow(p_0 / PS[qb], kap) * exp(Lb * qs_lift(base) / (cp_l * Tb));
+            bool buoy = false;
+            int top = base;
+            double cape = 0.0;
+            for (int i = base + 1; i < im; i++) {
+                const double Te = t_of(i);
+                const double Le = (Te >= t0) ? lv : ls;
+                const double the = Te * pow(p_0 / PS[id(i)], kap) * exp(Le * qs_lift(i) / (cp_l * Te));
+                if (thp - the > 0.0) {
+                    buoy = true;
+                    top = i;
+                    cape += K.g * (thp - the) / the * (h[i] - h[i - 1]);
+                } else if (buoy) {
+                    break;
+                }
+            }
+            lfs = tmin(tmax(top, base + 1), im - 2);
+            cape_c = tmax(0.0, cape);
+            d.mc[MC_LFS][id(lfs)] = h[lfs];
+        }
+    }
+    // surfaceFluxPerturbations, switch 1 (:402-485); Q_sensible_2D is zero in the reference
+    if (inner && lfs > base) {
+        const double Ts = t_of(0);
+        const double rho = tmax((double)RH[id(0)], 1e-6);
+        auto Es = [&](double Tx) { return (Tx >= t0) ? th_p_sat(Tx) : th_p_subl(Tx); };
+        const double qa = qs_env(Es(Ts), p_sfc, K.c_0), qb = qs_env(Es(288.15), p_sfc, K.c_0);
+        const double qcap = q_v_u_add * ((qb > 0.0) ? qa / qb : 1.0);
+        if (PS[id(base)] < f_mid * p_sfc) {
+            dT = t_add_u;
+            dq = qcap;
+        } else if (!(PS[id(base)] - PS[id(lfs)] >= f_diff * p_sfc)) {
+            const double Hs = 0.0;
+            const double E = tmax(0.0, d.evap[col]) / 86400.0;
+            const double z = h[base];
+            if (!(E <= 0.0 && Hs <= 0.0) && z > 0.0) {
+                const double B = (K.g / Ts) * (Hs / (rho * cp_l) + 0.61 * Ts * E / rho);
+                const double ws = cbrt(tmax(0.0, B * z));
+                if (ws > 0.0) {
+                    const double ir = 1.0 / (rho * ws);
+                    dT = tmin(alpha_sfp * Hs * ir / cp_l, t_add_u);
+                    dq = tmin(alpha_sfp * E * ir, qcap);
+                }
+            }
+        }
+    }
+    // E_u at level i (:534-543, :841-851) with the entrainment gradients of c (:238-297)
+    auto entrain = [&](int i, bool with_m) {
+        const size_t q = id(i);
+        double e = 0.0;
+        if (PS[id(base)] - PS[id(lfs)] >= f_diff * p_sfc) {
+            const double rth = (K.r_earth * 1.0e3 + h[i]) * d.conv_cos[j];
+            const double Ue = rth * 2.0 * M_PI;
+            const int ip = tmin(i + 1, im - 1), imn = tmax(i - 1, 0);
+            const double dr = ((double)C[id(ip)] - (double)C[id(imn)]) * (0.5 / step(i));
+            const double dth = ((double)C[q + km] - (double)C[q - km]) * (0.5 / (Ue / (jm - 1)));
+            const double dph = ((double)C[q + 1] - (double)C[q - 1]) * (1.0 / (Ue / (km - 1)));
+            const double cc = tmax((double)C[q], 1e-4);
+            e = -(double)RH[q] / cc * ((double)U[q] * dr + (double)V[q] * dth + (double)W[q] * dph);
+            if (with_m) e += eps_u * Mu[q];
+        }
+        if (t_of(i) <= t_00_m2) e = 0.0;
+        return e;
+    };
+    const double seed_w = c_mb * sqrt(2.0 * cape_c);
+    // initUpdraft (:490-622)
+    if (inner && beg != -1 && end != -1) {
+        Mu[id(beg)] = cap((double)RH[id(beg)] * seed_w, M_max);
+        for (int i = beg + 1; i <= end; i++) {
+            const size_t q = id(i);
+            const double t_u = t_of(i), t_a = t_u + dT;
+            const double qsa = qs_env(th_p_auto(t_a), PS[q], (double)C[q]);
+            double qv = (double)C[q] + dq, qc = Qcu[q];
+            if (qv >= scale * qsa && t_u >= t_00) qc = qv - scale * qsa;
+            if (qv <= 0.0) qv = 0.0;
+            if (qc <= 0.0) qc = 0.0;
+            Qvu[q] = qv; Qcu[q] = qc;
+            const double e = entrain(i, false);
+            Eu[q] = e;
+            Du[q] = 0.0;
+            Mu[q] = cap(Mu[id(i - 1)] + e * step(i), M_max);
+            Sx[q] = cp_l * t_u / s_0;
+            Su[q] = cp_l * t_a / s_0;
+            Uu[q] = vel * (double)U[q]; Vu[q] = vel * (double)V[q]; Wu[q] = vel * (double)W[q];
+        }
+        MdLFS = cap(gam_d * Mu[id(beg)], M_max);
+        Md[id(lfs)] = MdLFS;
+        for (int i = end + 1; i <= lfs - 1; i++) {
+            const size_t q = id(i);
+            const double t_u = t_of(i), t_a = t_u + dT;
+            const double qsa = qs_env(th_p_auto(t_a), PS[q], (double)C[q]);
+            if (i == end + 1) Mu[id(i - 1)] = Mu[id(beg)];
+            Sx[q] = cp_l * t_u / s_0;
+            Su[q] = cp_l * t_u / s_0;
+            const double qv = (double)C[q] + dq;
+            Qvu[q] = qv;
+            if (qv >= scale * qsa && t_a >= t_00) Qcu[q] = qv - scale * qsa;
+            const double e = entrain(i, false);
+            Eu[q] = e;
+            Du[q] = 0.0;
+            const double m = cap(Mu[id(i - 1)] + e * step(i), M_max);
+            Mu[q] = m;
+            Uu[q] = tmax(-uu_cap, tmin((double)U[q] + m * inv_a_u / tmax((double)RH[q], 1e-6), uu_cap));
+            Vu[q] = (double)V[q]; Wu[q] = (double)W[q];
+        }
+    }
+    // CloudBaseUpdate (:1462-1482) with simpsonCxEu (:1426-1455)
+    auto cloud_base_update = [&]() {
+        if (!inner || beg < 0 || lfs <= beg) return;
+        int hit = -1;
+        double cum = 0.0;
+        int i = beg;
+        auto f = [&](int l) { return -((double)C[id(l)] * Eu[id(l)]); };
+        for (; i + 2 <= lfs; i += 2) {
+            const double h1 = step(i), h2 = step(i + 1);
+            const double f0 = f(i), f1 = f(i + 1), f2 = f(i + 2);
+            const double pc = (h1 + h2) / 6.0 * ((2.0 - h2 / h1) * f0 + (h1 + h2) * (h1 + h2) / (h1 * h2) * f1
+                                                 + (2.0 - h1 / h2) * f2);
+            if (cum + pc > 0.0) { hit = (cum + 0.5 * h1 * (f0 + f1) > 0.0) ? i + 1 : i + 2; break; }
+            cum += pc;
+        }
+        if (hit < 0 && i + 1 <= lfs && cum + 0.5 * step(i) * (f(i) + f(i + 1)) > 0.0) hit = i + 1;
+        if (hit >= 0 && hit < im && PS[id(hit)] <= f_cb * p_sfc && (base <= 0 || hit <= base)) {
+            base = hit;
+            d.mc[MC_CLOUD_BASE][id(hit)] = h[hit];
+        }
+    };
+    cloud_base_update();
+    for (int ip = 1; ip <= 3; ip++) {
+        // updraftEntrainment (:780-908)
+        if (inner && lfs > base) {
+            const bool shallow = PS[id(base)] - PS[id(lfs)] < f_diff * p_sfc;   // convection_mode 1
+            const double dic = shallow ? 1.0e9 : 500.0, hb = h[base];
+            for (int i = 0; i < base; i++) Mu[id(i)] = 0.0;
+            Mu[id(base)] = cap((double)RH[id(base)] * seed_w, M_max);
+            Su[id(base)] = cp_l * (t_of(base) + dT) / s_0;
+            Qvu[id(base)] = tmax((double)C[id(base)] + dq, 0.0);
+            MdLFS = cap(gam_d * Mu[id(base)], M_max);
+            for (int i = base + 1; i <= lfs; i++) {
+                const size_t q = id(i);
+                const double t_u = t_of(i), st = step(i), m0 = Mu[q];
+                const double e = entrain(i, true);
+                double dd = eps_u * m0;
+                if (ip > 1 && i == lfs) dd = del_u * m0 + b_u * m0 / st;
+                if (ip > 1 && i == lfs - 1) dd = del_u * m0 + (1.0 - b_u) * m0 / st;
+                if (t_u <= t_00_m2) dd = 0.0;
+                Eu[q] = e;
+                Du[q] = dd;
+                Mu[q] = cap(Mu[id(i - 1)] + (e - dd) * st, M_max);
+                Cu[q] = 0.0;
+                double el = 0.0;
+                if (ip != 1) {
+                    el = dd * Qcu[q] / tmax((double)RH[q], 0.01);
+                    if (el <= 0.0) el = 0.0;
+                }
+                El[q] = el;
+                const double Kp = (h[i] > hb + dic) ? bet_p : 0.0;
+                Gp[q] = (t_u >= t0 && ip != 1) ? Kp * Qcu[q] : 0.0;
+            }
+            Mu[id(lfs)] = 0.0;
+        }
+        cloud_base_update();
+        // updraftRecurrence (:1032-1163)
+        if (inner && lfs > base) {
+            for (int i = base + 1; i <= lfs - 1; i++) {
+                const size_t q = id(i), qm = id(i - 1);
+                const double t_u = t_of(i - 1), sp = step(i - 1);
+                const double Mp = Mu[qm], Em = Eu[qm], Dm = Du[qm];
+                double m = cap(Mp + (Em - Dm) * sp, M_max);
+                if (t_u <= t_00_m2) m = 0.0;
+                const double n_qv = Mp * Qvu[qm] + sp * (Em * (double)C[qm] - Dm * Qvu[qm]);
+                const double n_qc = Mp * Qcu[qm] + sp * (-Dm * (double)CL[qm] - (double)RH[qm] * Gp[qm]);
+                const double n_v = Mp * Vu[qm] + sp * Em * (double)V[qm];
+                const double n_w = Mp * Wu[qm] + sp * Em * (double)W[qm];
+                const double n_s = (Mp * Su[qm] + sp * (Em * Sx[qm] - Dm * Su[qm])) / s_0;
+                double qv, qc, vu, wu, su;
+                if (fabs(m) > coeff_recurr) {
+                    const double r = 1.0 / m;
+                    qv = n_qv * r; qc = n_qc * r; vu = n_v * r; wu = n_w * r; su = n_s * r;
+                } else {
+                    su = Sx[q]; qv = (double)C[q]; qc = 0.0; vu = (double)V[q]; wu = (double)W[q];
+                }
+                double cu = 0.0;
+                if (fabs(m) > coeff_recurr) {   // the rising parcel's saturation adjustment, two passes
+                    double Tu = su * s_0 / cp_l;
+                    const double pu = PS[q], cq = (double)C[q];
+                    double tot = 0.0;
+                    for (int it = 0; it < 2; ++it) {
+                        const double ex = qv - qs_env(th_p_auto(Tu), pu, cq);
+                        if (ex <= 0.0) break;
+                        const double L = (Tu >= t0) ? lv : ls;
+                        const double G = (L / cp_l) * th_dqdT(th_p_auto(Tu), th_L_auto(Tu), Tu, pu,
+                                                              th_M_nonwater(cq, (double)CO2[q], Mb));
+                        const double dc = ex / (1.0 + G);
+                        qv -= dc;
+                        qc += dc;
+                        Tu += (L / cp_l) * dc;
+                        tot += dc;
+                    }
+                    su = cp_l * Tu / s_0;
+                    cu = tot * m / (tmax((double)RH[q], 0.01) * step(i));
+                }
+                double uu = tmax(-uu_cap, tmin((double)U[q] + m * inv_a_u / tmax((double)RH[q], 1e-6), uu_cap));
+                if (qv <= 0.0) qv = 0.0;
+                if (qc <= 0.0) qc = 0.0;
+                if (qc > q_c_u_max) qc = q_c_u_max;
+                if (su <= 0.0) su = 0.0;
+                if (t_u <= t_00) {
+                    Eu[q] = 0.0; Du[q] = 0.0; m = 0.0; qv = 0.0; qc = 0.0; uu = 0.0; vu = 0.0; wu = 0.0;
+                    su = 1.0; Gp[q] = 0.0; cu = 0.0; El[q] = 0.0;
+                }
+                Mu[q] = m; Qvu[q] = qv; Qcu[q] = qc; Vu[q] = vu; Wu[q] = wu; Su[q] = su; Cu[q] = cu; Uu[q] = uu;
+            }
+        }
+        // downdraftEntrainment (:913-1027)
+        if (lfs > 0 && lfs > base) {
+            const size_t ql = id(lfs);
+            const double t_l = t_of(lfs), p_l = PS[ql];
+            const double mdl = cap(MdLFS, M_max);
+            Md[ql] = mdl;
+            Ed[ql] = eps_d * fabs(mdl);
+            Dd[ql] = del_d * fabs(mdl);
+            Qvd[ql] = 0.5 * ((double)CL[ql] + scale * qs_env(th_p_auto(t_l), p_l, (double)C[ql]));
+            const double r_l = 1e2 * p_l / (th_R_of((double)C[ql], (double)CO2[ql], Rb) * t_l);
+            Ud[ql] = (double)U[ql] + mdl * inv_a_d / tmax(r_l, 1e-6);
+            Vd[ql] = (double)V[ql]; Wd[ql] = (double)W[ql]; Sd[ql] = Sx[ql];
+            Pc[ql] = 0.0;
+            double sig = 0.0;
+            const double wcb = fabs(Wu[id(base)]) * K.u_0;
+            if (wcb > 0.0) sig = fabs(Mu[id(base)]) / (tmax((double)RH[id(base)], 1e-6) * wcb);
+            sig = tmin(sig, 1.0);
+            for (int i = lfs - 1; i >= 0; i--) {
+                const size_t q = id(i);
+                const double t_u = t_of(i);
+                Sd[q] = cp_l * t_u / s_0;
+                const double qsd = qs_env(th_p_auto(t_u), PS[q], (double)C[q]);
+                const double qvd = 0.5 * ((double)CL[q] + scale * qsd);
+                Qvd[q] = qvd;
+                const double m0 = Md[q];
+                const double e = eps_d * fabs(m0), dd = del_d * fabs(m0);
+                Ed[q] = e; Dd[q] = dd;
+                const double m = cap(Md[id(i + 1)] - (e - dd) * step(i), M_max);
+                Md[q] = m;
+                Ud[q] = (double)U[q] + m * inv_a_d / tmax((double)RH[q], 1e-6);
+                Vd[q] = (double)V[q]; Wd[q] = (double)W[q];
+                if (i <= base) {
+                    double ep = 0.0;
+                    if (ip != 1) {
+                        ep = Ep[q];
+                        const double pr = Pc[q];
+                        if (pr > 0.0 && sig > 0.0) ep = sig * alf_1 * (qsd - qvd) * sqrt(1e1 / alf_2 * pr / sig);
+                        if (ep <= 0.0) ep = 0.0;
+                    }
+                    Ep[q] = ep;
+                }
+                double ed = 0.0;
+                if (ip != 1) {
+                    const double pr = Pc[q];
+                    if (pr > 0.0) ed = a_ev * (1.0 + b_ev * pow(pr, 1.0 / 6.0)) * (qsd - qvd) * pow(pr, 4.0 / 9.0);
+                    if (ed <= 0.0) ed = 0.0;
+                }
+                Edv[q] = ed;
+            }
+        }
+        // downdraftRecurrence (:1168-1254), every column
+        for (int i = lfs - 1; i >= 0; i--) {
+            const size_t q = id(i), qp = id(i + 1);
+            const double t_u = t_of(i), sp = step(i + 1), rp = (double)RH[qp];
+            const double Mp = Md[qp], Ep1 = Ed[qp], Dp1 = Dd[qp], ev = Edv[qp];
+            double m = cap(Mp - (Ed[q] - Dd[q]) * sp, M_max);
+            if (t_u <= t_00_m2) m = 0.0;
+            const double L = (t_u >= t0) ? lv : ls;
+            const double n_qv = Mp * Qvd[qp] - sp * (Ep1 * (double)C[qp] - Dp1 * Qvd[qp] + rp * ev);
+            const double n_v = Mp * Vd[qp] - sp * (Ep1 * (double)V[qp] - Dp1 * Vd[qp]);
+            const double n_w = Mp * Wd[qp] - sp * (Ep1 * (double)W[qp] - Dp1 * Wd[qp]);
+            const double n_s = (Mp * Sd[qp] - sp * (Ep1 * Sx[qp] - Dp1 * Sd[qp] - L * rp * ev)) / s_0;
+            double qv, vd, wd, sd;
+            if (fabs(m) > coeff_recurr) {
+                const double r = 1.0 / m;
+                qv = n_qv * r; vd = n_v * r; wd = n_w * r; sd = n_s * r;
+            } else {
+                qv = (double)C[q]; vd = (double)V[q]; wd = (double)W[q]; sd = Sx[q];
+            }
+            double ud = (double)U[q] + m * inv_a_d / tmax((double)RH[q], 1e-6);
+            if (qv <= 0.0) qv = 0.0;
+            if (sd <= 0.0) sd = 0.0;
+            if (t_u <= t_00) {
+                Ed[q] = 0.0; Dd[q] = 0.0; m = 0.0; qv = 0.0; ud = 0.0; vd = 0.0; wd = 0.0; sd = 1.0;
+                Edv[q] = 0.0; Ep[q] = 0.0;
+            }
+            Md[q] = m; Qvd[q] = qv; Vd[q] = vd; Wd[q] = wd; Sd[q] = sd; Ud[q] = ud;
+            if (i > 0) Pc[q] = tmax(0.0, Pc[qp] + step(i) * (double)RH[q] * (Gp[q] - Edv[q] - Ep[q]));
+        }
+        Pc[id(0)] = Pc[id(1)];
+    }
+    // rhsForcing (:1304-1417)
+    double* __restrict__ MCt = d.mc[MC_MC_T]; double* __restrict__ MCq = d.mc[MC_MC_Q];
+    double* __restrict__ MCv = d.mc[MC_MC_V]; double* __restrict__ MCw = d.mc[MC_MC_W];
+    for (int i = 0; i < im; i++) { Mu[id(i)] = cap(Mu[id(i)], M_max); Md[id(i)] = cap(Md[id(i)], M_max); }
+    double fs0 = 0, fq0 = 0, fv0 = 0, fw0 = 0;
+    auto fluxes = [&](int i, double& fs, double& fq, double& fv, double& fw) {
+        const size_t q = id(i);
+        const double mu = Mu[q], md = Md[q], s = Sx[q], c = (double)C[q], v = (double)V[q], w = (double)W[q];
+        fs = mu * (Su[q] - s) + md * (Sd[q] - s);
+        fq = mu * (Qvu[q] - c) + md * (Qvd[q] - c);
+        fv = mu * (Vu[q] - v) + md * (Vd[q] - v);
+        fw = mu * (Wu[q] - w) + md * (Wd[q] - w);
+    };
+    fluxes(0, fs0, fq0, fv0, fw0);
+    for (int i = 0; i < im - 1; i++) {
+        const size_t q = id(i);
+        double fs1, fq1, fv1, fw1;
+        fluxes(i + 1, fs1, fq1, fv1, fw1);
+        const double L = ((double)T[q] * t0 >= t0) ? lv : ls;
+        const double isr = 1.0 / (step(i) * tmax((double)RH[q], 0.01));
+        const double src = Cu[q] - Edv[q] - El[q] - Ep[q];
+        MCt[q] = cap(-(fs1 - fs0) * isr * t0 + (L / cp_l) * src * t0, MCt_max);
+        MCq[q] = cap(-(fq1 - fq0) * isr - src, MCq_max);
+        MCv[q] = cap(-(fv1 - fv0) * isr * K.u_0, MCv_max);
+        MCw[q] = cap(-(fw1 - fw0) * isr * K.u_0, MCv_max);
+        fs0 = fs1; fq0 = fq1; fv0 = fv1; fw0 = fw1;
+    }
+}
+
+void launch_moist_convection(const Batch& b) {
+    const dim3 g((unsigned)((b.max_jm * b.max_km + 63) / 64), 1, b.nmem);
+    if (b.fp32) hipLaunchKernelGGL(k_moist_convection<float>, g, dim3(64), 0, b.stream, b.d_md);
+    else hipLaunchKernelGGL(k_moist_convection<double>, g, dim3(64), 0, b.stream, b.d_md);
+}
+
+}  // namespace athad

@@ -207,6 +207,26 @@ class Integration:
         """CategoryIceScheme = 2 inside every later step's moist block."""
         self._chk(L.lib().athad_set_ice_scheme(self.h, 1 if on else 0))
 
+    def moist_convection(self):
+        """MoistConvection::run on every member's current fields (MoistConvection.h:106-140)."""
+        self._chk(L.lib().athad_moist_convection(self.h))
+
+    def _conv_shape(self, name, member):
+        return tuple(self.grids[member])[1:] if name == "Evaporation" else tuple(self.grids[member])
+
+    def get_conv(self, name: str, member: int = 0) -> np.ndarray:
+        """One convection array (CONV_FIELDS; "Evaporation" is the jm x km input) as FP64."""
+        out = np.empty(self._conv_shape(name, member))
+        self._chk(L.lib().athad_get_conv_field(self.h, member, L.CONV_FIELDS.index(name),
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def set_conv(self, name: str, arr, member: int = 0):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self._conv_shape(name, member), "shape mismatch"
+        self._chk(L.lib().athad_set_conv_field(self.h, member, L.CONV_FIELDS.index(name),
+                                               a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
     def get_mp(self, name: str, member: int = 0) -> np.ndarray:
         """One ice-scheme array (MP_FIELDS) as FP64."""
         out = np.empty(self.grids[member])

@@ -293,6 +293,34 @@ athad_status athad_set_ice_scheme(athad_ctx* ctx, int on);
 athad_status athad_set_mp_field(athad_ctx* ctx, int member, int which, const double* src);
 athad_status athad_get_mp_field(athad_ctx* ctx, int member, int which, double* dst);
 
+/* --- moist convection (atmosphere/MoistConvection.h) ------------------------------------
+ * The 32 convection arrays, in the order of athad_conv_field (FP64 in every precision), and
+ * the per-column evaporation diagnostic [mm/d] (jm*km) the surface-flux trigger reads. */
+typedef enum athad_conv_field {
+    ATHAD_CONV_S = 0, ATHAD_CONV_M_U, ATHAD_CONV_M_D, ATHAD_CONV_S_U, ATHAD_CONV_Q_V_U, ATHAD_CONV_Q_C_U,
+    ATHAD_CONV_C_U, ATHAD_CONV_E_L, ATHAD_CONV_E_D_EVAP, ATHAD_CONV_E_P, ATHAD_CONV_G_P, ATHAD_CONV_S_D,
+    ATHAD_CONV_Q_V_D, ATHAD_CONV_P_CONV, ATHAD_CONV_E_U, ATHAD_CONV_D_U, ATHAD_CONV_U_U, ATHAD_CONV_V_U,
+    ATHAD_CONV_W_U, ATHAD_CONV_E_D, ATHAD_CONV_D_D, ATHAD_CONV_U_D, ATHAD_CONV_V_D, ATHAD_CONV_W_D,
+    ATHAD_CONV_MC_T, ATHAD_CONV_MC_Q, ATHAD_CONV_MC_V, ATHAD_CONV_MC_W, ATHAD_CONV_DEEP_BEG,
+    ATHAD_CONV_DEEP_END, ATHAD_CONV_CLOUD_BASE, ATHAD_CONV_LEVEL_FREE_SINKING,
+    ATHAD_CONV_EVAPORATION = 32   /* input, jm*km */
+} athad_conv_field;
+/* Replaces MoistConvection(*this).run(iter_n) (atmosphere/cAtmosphereModel.cpp:1247;
+ * MoistConvection.h:106-140) with the model's defaults iter_prec 3, convection_perturbation 1,
+ * convection_mode 1, every member, on the current fields: cloud base and level of free sinking
+ * per column, the updraft and downdraft mass fluxes and their recurrences, convective
+ * precipitation P_conv, and the tendencies MC_t, MC_q, MC_v, MC_w of rhsForcing. The arrays the
+ * reference does not clear between calls (E_u, D_u, E_d, D_d, the updraft / downdraft winds,
+ * the MC_* tendencies, the level markers) carry over as there. computeCAPE's 1-D diagnostic is
+ * not produced. Arrays are allocated, zero, at the first call (32 * im*jm*km * 8 bytes per
+ * member). Needs jm > 90 (the circle factor is taken about row 90, MoistConvection.h:239):
+ * ATHAD_ERR_ARG otherwise. Not part of athad_step: a standalone operator. */
+athad_status athad_moist_convection(athad_ctx* ctx);
+/* Contiguous FP64 host arrays [im][jm][km] of one convection array, or [jm][km] for
+ * ATHAD_CONV_EVAPORATION. */
+athad_status athad_set_conv_field(athad_ctx* ctx, int member, int which, const double* src);
+athad_status athad_get_conv_field(athad_ctx* ctx, int member, int which, double* dst);
+
 /* --- measurement --------------------------------------------------------------------- */
 
 /* Per-kernel timing with HIP events on the context's stream: kernel ids as in

@@ -1,0 +1,103 @@
+"""GPU moist convection (MoistConvection::run, MoistConvection.h:106-140) against the CPU oracle.
+
+The oracle is pinned to the reference's compiled scheme by
+tests/test_oracle_golden.py::test_moist_convection_twice on the same kind of input: a spun-up
+state with the condensate that oracle.plant_microphysics plants, and a seeded per-column
+evaporation diagnostic, so that updrafts, downdrafts, parcel condensation and convective
+precipitation are all live. Two calls in a row from zeroed arrays: the second reads what the
+first left in the arrays the scheme does not clear. Tolerance 1e-11 relative L2 per array in
+FP64 (the device's exp / pow / cbrt differ from glibc's by an ulp and a flipped threshold moves
+one column; measured values are printed).
+"""
+import numpy as np
+import pytest
+
+import athad_cond_amd as A
+from athad_cond_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+ARRAYS = L.CONV_FIELDS[:-1]
+
+
+def rel(a, b):
+    den = np.sqrt(np.sum(b * b))
+    num = np.sqrt(np.sum((a - b) ** 2))
+    return num / den if den > 0 else num
+
+
+def planted(oracle, grid, seed, spin=2):
+    ref = oracle.OracleState(*grid, seed=seed)
+    ref.run(spin, "jacobi")
+    snap = ref.snapshot()
+    oracle.plant_microphysics(snap, {k: np.zeros(grid) for k in oracle.MP_FIELDS}, seed + 1)
+    ref.load(snap)
+    evap = np.random.default_rng(seed).uniform(-1.0, 6.0, grid[1:])
+    ref.evap()[...] = evap
+    return ref, snap, evap
+
+
+def errors(integ, ref, member=0):
+    return {n: rel(integ.get_conv(n, member=member), ref.mc(n)) for n in ARRAYS}
+
+
+@pytest.mark.parametrize("grid", [(41, 181, 361), (61, 91, 73)])
+def test_two_calls_match_oracle(oracle, grid):
+    print(f"case moist convection fp64 {grid}", flush=True)
+    ref, snap, evap = planted(oracle, grid, 21)
+    with A.Integration(grid) as integ:
+        integ.load(snap)
+        integ.set_conv("Evaporation", evap)
+        for call in (1, 2):
+            integ.moist_convection()
+            ref.moist_convection()
+            e = errors(integ, ref)
+            print(call, {k: f"{v:.1e}" for k, v in e.items() if v > 0})
+            bad = {k: v for k, v in e.items() if v > 1e-11}
+            assert not bad, (call, bad)
+        # the input makes the scheme do something
+        for n in ("M_u", "M_d", "q_c_u", "c_u", "e_d", "MC_t", "MC_q", "MC_v", "MC_w"):
+            assert np.any(ref.mc(n) != 0.0), n
+        assert np.all(np.isfinite(integ.get_conv("MC_t")))
+
+
+def test_members_are_independent(oracle):
+    """Two members with different inputs in one call: each matches its own oracle run."""
+    grid = (41, 91, 73)
+    print(f"case moist convection members {grid}", flush=True)
+    refs = [planted(oracle, grid, s) for s in (5, 9)]
+    with A.Integration([grid, grid]) as integ:
+        for m, (ref, snap, evap) in enumerate(refs):
+            integ.load(snap, member=m)
+            integ.set_conv("Evaporation", evap, member=m)
+        integ.moist_convection()
+        for m, (ref, _, _) in enumerate(refs):
+            ref.moist_convection()
+            e = errors(integ, ref, member=m)
+            bad = {k: v for k, v in e.items() if v > 1e-11}
+            assert not bad, (m, bad)
+
+
+def test_mixed_precision_fields(oracle):
+    """FP32 field storage: the scheme reads the rounded fields and computes in FP64; against
+    the oracle run on the same rounded fields."""
+    grid = (41, 91, 73)
+    print(f"case moist convection mixed {grid}", flush=True)
+    ref, snap, evap = planted(oracle, grid, 13)
+    with A.Integration(grid, precision="mixed") as integ:
+        integ.load(snap)
+        integ.set_conv("Evaporation", evap)
+        rounded = {n: integ.get(n) for n in snap}
+        ref.load(rounded)
+        integ.moist_convection()
+        ref.moist_convection()
+        e = errors(integ, ref)
+        print({k: f"{v:.1e}" for k, v in e.items() if v > 0})
+        bad = {k: v for k, v in e.items() if v > 1e-11}
+        assert not bad, bad
+
+
+def test_small_latitude_count_is_refused():
+    with A.Integration((21, 37, 73)) as integ:
+        with pytest.raises(Exception):
+            integ.moist_convection()
