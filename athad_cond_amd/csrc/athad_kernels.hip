@@ -3616,7 +3616,9 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
     const double seed_w = c_mb * sqrt(2.0 * cape_c);
     // initUpdraft (:490-622)
     if (inner && beg != -1 && end != -1) {
-        Mu[id(beg)] = cap((double)RH[id(beg)] * seed_w, M_max);
+        const double mbeg = cap((double)RH[id(beg)] * seed_w, M_max);
+        Mu[id(beg)] = mbeg;
+        double mprev = mbeg;   // M_u one level down, carried in a register
         for (int i = beg + 1; i <= end; i++) {
             const size_t q = id(i);
             const double t_u = t_of(i), t_a = t_u + dT;
@@ -3629,18 +3631,20 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             const double e = entrain(i, false);
             Eu[q] = e;
             Du[q] = 0.0;
-            Mu[q] = cap(Mu[id(i - 1)] + e * step(i), M_max);
+            mprev = cap(mprev + e * step(i), M_max);
+            Mu[q] = mprev;
             Sx[q] = cp_l * t_u / s_0;
             Su[q] = cp_l * t_a / s_0;
             Uu[q] = vel * (double)U[q]; Vu[q] = vel * (double)V[q]; Wu[q] = vel * (double)W[q];
         }
-        MdLFS = cap(gam_d * Mu[id(beg)], M_max);
+        MdLFS = cap(gam_d * mbeg, M_max);
         Md[id(lfs)] = MdLFS;
+        mprev = mbeg;
         for (int i = end + 1; i <= lfs - 1; i++) {
             const size_t q = id(i);
             const double t_u = t_of(i), t_a = t_u + dT;
             const double qsa = qs_env(th_p_auto(t_a), PS[q], (double)C[q]);
-            if (i == end + 1) Mu[id(i - 1)] = Mu[id(beg)];
+            if (i == end + 1) Mu[id(i - 1)] = mbeg;
             Sx[q] = cp_l * t_u / s_0;
             Su[q] = cp_l * t_u / s_0;
             const double qv = (double)C[q] + dq;
@@ -3649,7 +3653,8 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             const double e = entrain(i, false);
             Eu[q] = e;
             Du[q] = 0.0;
-            const double m = cap(Mu[id(i - 1)] + e * step(i), M_max);
+            const double m = cap(mprev + e * step(i), M_max);
+            mprev = m;
             Mu[q] = m;
             Uu[q] = tmax(-uu_cap, tmin((double)U[q] + m * inv_a_u / tmax((double)RH[q], 1e-6), uu_cap));
             Vu[q] = (double)V[q]; Wu[q] = (double)W[q];
@@ -3683,10 +3688,11 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             const bool shallow = PS[id(base)] - PS[id(lfs)] < f_diff * p_sfc;   // convection_mode 1
             const double dic = shallow ? 1.0e9 : 500.0, hb = h[base];
             for (int i = 0; i < base; i++) Mu[id(i)] = 0.0;
-            Mu[id(base)] = cap((double)RH[id(base)] * seed_w, M_max);
+            double mprev = cap((double)RH[id(base)] * seed_w, M_max);
+            Mu[id(base)] = mprev;
             Su[id(base)] = cp_l * (t_of(base) + dT) / s_0;
             Qvu[id(base)] = tmax((double)C[id(base)] + dq, 0.0);
-            MdLFS = cap(gam_d * Mu[id(base)], M_max);
+            MdLFS = cap(gam_d * mprev, M_max);
             for (int i = base + 1; i <= lfs; i++) {
                 const size_t q = id(i);
                 const double t_u = t_of(i), st = step(i), m0 = Mu[q];
@@ -3697,7 +3703,8 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
                 if (t_u <= t_00_m2) dd = 0.0;
                 Eu[q] = e;
                 Du[q] = dd;
-                Mu[q] = cap(Mu[id(i - 1)] + (e - dd) * st, M_max);
+                mprev = cap(mprev + (e - dd) * st, M_max);
+                Mu[q] = mprev;
                 Cu[q] = 0.0;
                 double el = 0.0;
                 if (ip != 1) {
@@ -3713,17 +3720,21 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
         cloud_base_update();
         // updraftRecurrence (:1032-1163)
         if (inner && lfs > base) {
+            // the level below, carried in registers from one level to the next
+            const size_t qb = id(base);
+            double Mp = Mu[qb], Em = Eu[qb], Dm = Du[qb], Gm = Gp[qb];
+            double Qvm = Qvu[qb], Qcm = Qcu[qb], Vm = Vu[qb], Wm = Wu[qb], Sm = Su[qb];
             for (int i = base + 1; i <= lfs - 1; i++) {
                 const size_t q = id(i), qm = id(i - 1);
                 const double t_u = t_of(i - 1), sp = step(i - 1);
-                const double Mp = Mu[qm], Em = Eu[qm], Dm = Du[qm];
                 double m = cap(Mp + (Em - Dm) * sp, M_max);
                 if (t_u <= t_00_m2) m = 0.0;
-                const double n_qv = Mp * Qvu[qm] + sp * (Em * (double)C[qm] - Dm * Qvu[qm]);
-                const double n_qc = Mp * Qcu[qm] + sp * (-Dm * (double)CL[qm] - (double)RH[qm] * Gp[qm]);
-                const double n_v = Mp * Vu[qm] + sp * Em * (double)V[qm];
-                const double n_w = Mp * Wu[qm] + sp * Em * (double)W[qm];
-                const double n_s = (Mp * Su[qm] + sp * (Em * Sx[qm] - Dm * Su[qm])) / s_0;
+                const double n_qv = Mp * Qvm + sp * (Em * (double)C[qm] - Dm * Qvm);
+                const double n_qc = Mp * Qcm + sp * (-Dm * (double)CL[qm] - (double)RH[qm] * Gm);
+                const double n_v = Mp * Vm + sp * Em * (double)V[qm];
+                const double n_w = Mp * Wm + sp * Em * (double)W[qm];
+                const double n_s = (Mp * Sm + sp * (Em * Sx[qm] - Dm * Sm)) / s_0;
+                double Ei = Eu[q], Di = Du[q], Gi = Gp[q];
                 double qv, qc, vu, wu, su;
                 if (fabs(m) > coeff_recurr) {
                     const double r = 1.0 / m;
@@ -3759,8 +3770,10 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
                 if (t_u <= t_00) {
                     Eu[q] = 0.0; Du[q] = 0.0; m = 0.0; qv = 0.0; qc = 0.0; uu = 0.0; vu = 0.0; wu = 0.0;
                     su = 1.0; Gp[q] = 0.0; cu = 0.0; El[q] = 0.0;
+                    Ei = 0.0; Di = 0.0; Gi = 0.0;
                 }
                 Mu[q] = m; Qvu[q] = qv; Qcu[q] = qc; Vu[q] = vu; Wu[q] = wu; Su[q] = su; Cu[q] = cu; Uu[q] = uu;
+                Mp = m; Em = Ei; Dm = Di; Gm = Gi; Qvm = qv; Qcm = qc; Vm = vu; Wm = wu; Sm = su;
             }
         }
         // downdraftEntrainment (:913-1027)
@@ -3780,6 +3793,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             const double wcb = fabs(Wu[id(base)]) * K.u_0;
             if (wcb > 0.0) sig = fabs(Mu[id(base)]) / (tmax((double)RH[id(base)], 1e-6) * wcb);
             sig = tmin(sig, 1.0);
+            double mnext = mdl;   // M_d one level up, carried in a register
             for (int i = lfs - 1; i >= 0; i--) {
                 const size_t q = id(i);
                 const double t_u = t_of(i);
@@ -3790,7 +3804,8 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
                 const double m0 = Md[q];
                 const double e = eps_d * fabs(m0), dd = del_d * fabs(m0);
                 Ed[q] = e; Dd[q] = dd;
-                const double m = cap(Md[id(i + 1)] - (e - dd) * step(i), M_max);
+                const double m = cap(mnext - (e - dd) * step(i), M_max);
+                mnext = m;
                 Md[q] = m;
                 Ud[q] = (double)U[q] + m * inv_a_d / tmax((double)RH[q], 1e-6);
                 Vd[q] = (double)V[q]; Wd[q] = (double)W[q];
@@ -3814,17 +3829,25 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             }
         }
         // downdraftRecurrence (:1168-1254), every column
+        // the level above, carried in registers from one level to the next
+        double Mp = 0, Ep1 = 0, Dp1 = 0, ev = 0, Qvp = 0, Vp = 0, Wp = 0, Sp = 0, Pp = 0;
+        if (lfs > 0) {
+            const size_t ql = id(lfs);
+            Mp = Md[ql]; Ep1 = Ed[ql]; Dp1 = Dd[ql]; ev = Edv[ql];
+            Qvp = Qvd[ql]; Vp = Vd[ql]; Wp = Wd[ql]; Sp = Sd[ql]; Pp = Pc[ql];
+        }
         for (int i = lfs - 1; i >= 0; i--) {
             const size_t q = id(i), qp = id(i + 1);
             const double t_u = t_of(i), sp = step(i + 1), rp = (double)RH[qp];
-            const double Mp = Md[qp], Ep1 = Ed[qp], Dp1 = Dd[qp], ev = Edv[qp];
-            double m = cap(Mp - (Ed[q] - Dd[q]) * sp, M_max);
+            double Ei = Ed[q], Di = Dd[q];
+            double m = cap(Mp - (Ei - Di) * sp, M_max);
             if (t_u <= t_00_m2) m = 0.0;
             const double L = (t_u >= t0) ? lv : ls;
-            const double n_qv = Mp * Qvd[qp] - sp * (Ep1 * (double)C[qp] - Dp1 * Qvd[qp] + rp * ev);
-            const double n_v = Mp * Vd[qp] - sp * (Ep1 * (double)V[qp] - Dp1 * Vd[qp]);
-            const double n_w = Mp * Wd[qp] - sp * (Ep1 * (double)W[qp] - Dp1 * Wd[qp]);
-            const double n_s = (Mp * Sd[qp] - sp * (Ep1 * Sx[qp] - Dp1 * Sd[qp] - L * rp * ev)) / s_0;
+            const double n_qv = Mp * Qvp - sp * (Ep1 * (double)C[qp] - Dp1 * Qvp + rp * ev);
+            const double n_v = Mp * Vp - sp * (Ep1 * (double)V[qp] - Dp1 * Vp);
+            const double n_w = Mp * Wp - sp * (Ep1 * (double)W[qp] - Dp1 * Wp);
+            const double n_s = (Mp * Sp - sp * (Ep1 * Sx[qp] - Dp1 * Sp - L * rp * ev)) / s_0;
+            double evi = Edv[q], epi = Ep[q];
             double qv, vd, wd, sd;
             if (fabs(m) > coeff_recurr) {
                 const double r = 1.0 / m;
@@ -3838,9 +3861,14 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             if (t_u <= t_00) {
                 Ed[q] = 0.0; Dd[q] = 0.0; m = 0.0; qv = 0.0; ud = 0.0; vd = 0.0; wd = 0.0; sd = 1.0;
                 Edv[q] = 0.0; Ep[q] = 0.0;
+                Ei = 0.0; Di = 0.0; evi = 0.0; epi = 0.0;
             }
             Md[q] = m; Qvd[q] = qv; Vd[q] = vd; Wd[q] = wd; Sd[q] = sd; Ud[q] = ud;
-            if (i > 0) Pc[q] = tmax(0.0, Pc[qp] + step(i) * (double)RH[q] * (Gp[q] - Edv[q] - Ep[q]));
+            if (i > 0) {
+                Pp = tmax(0.0, Pp + step(i) * (double)RH[q] * (Gp[q] - evi - epi));
+                Pc[q] = Pp;
+            }
+            Mp = m; Ep1 = Ei; Dp1 = Di; ev = evi; Qvp = qv; Vp = vd; Wp = wd; Sp = sd;
         }
         Pc[id(0)] = Pc[id(1)];
     }

@@ -127,6 +127,11 @@ def run_next_rows(A, reps=10, seed=1234, warmup=8):
         r["hbm_GBps"] = 9 * 8 * n / (r["ms_per_call"] * 1e-3) / 1e9
         r["frac_of_hbm_peak"] = r["hbm_GBps"] / HBM_PEAK_GBS
         timed("ice_scheme", integ.ice_scheme)
+        # moist convection: one thread per column; 41 words per cell is its floor (9 fields
+        # read, 32 arrays written), the serial column recurrences bound it, not HBM
+        timed("moist_convection", integ.moist_convection)
+        r = out["moist_convection"]
+        r["hbm_GBps_41_words"] = 41 * 8 * n / (r["ms_per_call"] * 1e-3) / 1e9
         timed("residuum", integ.residuum)
         timed("field_minmax", lambda: integ.field_minmax("t"))
         timed("meridional_streamfunction", integ.meridional_streamfunction)
