@@ -86,6 +86,7 @@ struct MemberHost {
     void* mp[NMP] = {};            // ice-scheme arrays, on first use
     double* mc[NMC] = {};          // moist-convection arrays, on first use
     double *evap = nullptr, *conv_cos = nullptr;
+    double* conv_qs[2] = {};
     int iter_n = 0, total = 0;     // host mirror of the device counters
     // table offsets inside 'tables'
     size_t o_rad = 0, o_sint = 0, o_cost = 0, o_sinr = 0, o_hgt = 0, o_nuef = 0, o_step = 0;
@@ -282,6 +283,7 @@ void make_desc(const athad_ctx* c, const MemberHost& m, int parity, MemberDesc& 
     for (int q = 0; q < NMP; q++) d.mp[q] = m.mp[q];
     for (int q = 0; q < NMC; q++) d.mc[q] = m.mc[q];
     d.evap = m.evap; d.conv_cos = m.conv_cos;
+    d.conv_qs[0] = m.conv_qs[0]; d.conv_qs[1] = m.conv_qs[1];
     (void)c;
 }
 
@@ -322,6 +324,7 @@ void free_ctx(athad_ctx* c) {
         for (auto p : m.mc) if (p) hipFree(p);
         if (m.evap) hipFree(m.evap);
         if (m.conv_cos) hipFree(m.conv_cos);
+        for (auto p : m.conv_qs) if (p) hipFree(p);
         for (void* p : {(void*)m.vel_star, (void*)m.t_top, (void*)m.t_ref, (void*)m.tables, (void*)m.scratch,
                         (void*)m.counters, (void*)m.passes, (void*)m.radiation, (void*)m.epsilon, (void*)m.albedo})
             if (p) hipFree(p);
@@ -1099,6 +1102,7 @@ static athad_status ensure_mc_buffers(athad_ctx* c) {
             for (auto& p : m2.mc) if (p) { hipFree(p); p = nullptr; }
             if (m2.evap) { hipFree(m2.evap); m2.evap = nullptr; }
             if (m2.conv_cos) { hipFree(m2.conv_cos); m2.conv_cos = nullptr; }
+            for (auto& p : m2.conv_qs) if (p) { hipFree(p); p = nullptr; }
         }
         c->err = "out of device memory for the convection arrays";
         return ATHAD_ERR_ALLOC;
@@ -1112,6 +1116,8 @@ static athad_status ensure_mc_buffers(athad_ctx* c) {
         if (hipMalloc(&m.evap, (size_t)jm * km * 8) != hipSuccess) { m.evap = nullptr; return fail(); }
         HIPCHK(c, hipMemset(m.evap, 0, (size_t)jm * km * 8));
         if (hipMalloc(&m.conv_cos, (size_t)jm * 8) != hipSuccess) { m.conv_cos = nullptr; return fail(); }
+        for (auto& p : m.conv_qs)
+            if (hipMalloc(&p, m.n * 8) != hipSuccess) { p = nullptr; return fail(); }
         // theta by accumulation as the grid builds it; the circle factor is taken about row 90
         // (MoistConvection.h:239), so the scheme needs jm > 90
         std::vector<double> the(jm), cs(jm);

@@ -87,6 +87,7 @@ struct MemberDesc {
     double* mc[NMC];         // n        moist-convection arrays, FP64 (allocated at first use)
     double* evap;            // jm*km    evaporation diagnostic [mm/d] the convection trigger reads
     const double* conv_cos;  // jm       cos(theta[90] - theta[j]), the convection's zonal circle factor
+    double* conv_qs[2];      // n        per-cell saturation of the lifted (+0.2 K) and the resting air
 };
 
 #define ATHAD_DEV __device__ __forceinline__

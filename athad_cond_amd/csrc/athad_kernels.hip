@@ -3470,6 +3470,24 @@ ATHAD_DEV double cap(double v, double mag) {                          // :216-22
 }
 }  // namespace conv
 
+// The saturation mass fractions every column pass needs at each cell, computed beforehand
+// with one thread per cell (the column kernel runs one wave per SIMD, where these IAPWS
+// evaluations would sit on its serial path): [0] the air lifted by t_add_u (findCloudBaseLFS's
+// q_sat_col, :675-677, and initUpdraft's where its perturbation is t_add_u), [1] the air at rest
+// (the downdraft's, :935-936, :976-977, and initUpdraft's where its perturbation is zero).
+template <typename S>
+__global__ __launch_bounds__(256) void k_conv_sat(const MemberDesc* __restrict__ mds) {
+    const MemberDesc& d = mds[blockIdx.z];
+    const Consts& K = d.K;
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= K.n) return;
+    const double t_u = (double)((const S*)d.f[F_T])[q] * K.t_0;
+    const double p = ((const double*)d.f[F_P_STAT])[q];
+    const double Mo = th_M_nonwater((double)((const S*)d.f[F_C])[q], K.co2_0, K.M_bg);
+    d.conv_qs[0][q] = th_q_sat(th_p_auto(t_u + conv::t_add_u), p, Mo);
+    d.conv_qs[1][q] = th_q_sat(th_p_auto(t_u), p, Mo);
+}
+
 template <typename S>
 __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __restrict__ mds) {
     using namespace conv;
@@ -3527,7 +3545,14 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
         if (PS[id(i)] <= f_beg * p_sfc) { beg = i; d.mc[MC_DEEP_BEG][id(i)] = h[i]; break; }
     for (int i = im - 1; i >= 0; i--)
         if (PS[id(i)] >= f_end * p_sfc) { end = i; d.mc[MC_DEEP_END][id(i)] = h[i]; break; }
-    auto qs_lift = [&](int i) { return qs_env(th_p_auto(t_of(i) + t_add_u), PS[id(i)], (double)C[id(i)]); };
+    const double* __restrict__ QSL = d.conv_qs[0];
+    const double* __restrict__ QSR = d.conv_qs[1];
+    auto qs_lift = [&](int i) { return QSL[id(i)]; };
+    // initUpdraft's saturation at t + dT: the precomputed one where dT is 0 or t_add_u
+    auto qs_pert = [&](int i, double t_a) {
+        return (dT == 0.0) ? QSR[id(i)] : (dT == t_add_u) ? QSL[id(i)]
+                                                           : qs_env(th_p_auto(t_a), PS[id(i)], (double)C[id(i)]);
+    };
     if (beg >= 0 && end >= 0) {
         for (int i = beg; i < im; i++) {
             const size_t q = id(i);
@@ -3622,7 +3647,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
         for (int i = beg + 1; i <= end; i++) {
             const size_t q = id(i);
             const double t_u = t_of(i), t_a = t_u + dT;
-            const double qsa = qs_env(th_p_auto(t_a), PS[q], (double)C[q]);
+            const double qsa = qs_pert(i, t_a);
             double qv = (double)C[q] + dq, qc = Qcu[q];
             if (qv >= scale * qsa && t_u >= t_00) qc = qv - scale * qsa;
             if (qv <= 0.0) qv = 0.0;
@@ -3643,7 +3668,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
         for (int i = end + 1; i <= lfs - 1; i++) {
             const size_t q = id(i);
             const double t_u = t_of(i), t_a = t_u + dT;
-            const double qsa = qs_env(th_p_auto(t_a), PS[q], (double)C[q]);
+            const double qsa = qs_pert(i, t_a);
             if (i == end + 1) Mu[id(i - 1)] = mbeg;
             Sx[q] = cp_l * t_u / s_0;
             Su[q] = cp_l * t_u / s_0;
@@ -3784,7 +3809,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             Md[ql] = mdl;
             Ed[ql] = eps_d * fabs(mdl);
             Dd[ql] = del_d * fabs(mdl);
-            Qvd[ql] = 0.5 * ((double)CL[ql] + scale * qs_env(th_p_auto(t_l), p_l, (double)C[ql]));
+            Qvd[ql] = 0.5 * ((double)CL[ql] + scale * QSR[ql]);
             const double r_l = 1e2 * p_l / (th_R_of((double)C[ql], (double)CO2[ql], Rb) * t_l);
             Ud[ql] = (double)U[ql] + mdl * inv_a_d / tmax(r_l, 1e-6);
             Vd[ql] = (double)V[ql]; Wd[ql] = (double)W[ql]; Sd[ql] = Sx[ql];
@@ -3798,7 +3823,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
                 const size_t q = id(i);
                 const double t_u = t_of(i);
                 Sd[q] = cp_l * t_u / s_0;
-                const double qsd = qs_env(th_p_auto(t_u), PS[q], (double)C[q]);
+                const double qsd = QSR[q];
                 const double qvd = 0.5 * ((double)CL[q] + scale * qsd);
                 Qvd[q] = qvd;
                 const double m0 = Md[q];
@@ -3903,8 +3928,14 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
 
 void launch_moist_convection(const Batch& b) {
     const dim3 g((unsigned)((b.max_jm * b.max_km + 63) / 64), 1, b.nmem);
-    if (b.fp32) hipLaunchKernelGGL(k_moist_convection<float>, g, dim3(64), 0, b.stream, b.d_md);
-    else hipLaunchKernelGGL(k_moist_convection<double>, g, dim3(64), 0, b.stream, b.d_md);
+    const dim3 gs((unsigned)((b.max_n + 255) / 256), 1, b.nmem);
+    if (b.fp32) {
+        hipLaunchKernelGGL(k_conv_sat<float>, gs, dim3(256), 0, b.stream, b.d_md);
+        hipLaunchKernelGGL(k_moist_convection<float>, g, dim3(64), 0, b.stream, b.d_md);
+    } else {
+        hipLaunchKernelGGL(k_conv_sat<double>, gs, dim3(256), 0, b.stream, b.d_md);
+        hipLaunchKernelGGL(k_moist_convection<double>, g, dim3(64), 0, b.stream, b.d_md);
+    }
 }
 
 }  // namespace athad
