@@ -282,7 +282,7 @@ athad_status athad_ice_scheme(athad_ctx* ctx);
 /* CategoryIceScheme = 2 inside the step (on != 0): every later moist iteration of athad_step /
  * athad_run runs the ice-scheme call site in its place in the moist block, after the damping of
  * t ice c cloud, then precipitationSum (:1263; graupel and the moist convection's P_conv are
- * zero, MoistConvection is not part of this library) and cap_S of the sources (:1278-1305).
+ * zero: athad_step does not call athad_moist_convection) and cap_S of the sources (:1278-1305).
  * Independently of this switch, once the ice-scheme arrays exist the RK4 tendencies of t, c,
  * cloud, ice carry their sources (atmosphere/RHS_Atm_Turb.cpp:963-966, 1058-1067), the
  * boundaries extrapolate them (atmosphere/BC_Atm.h:567-842), and saturation adjustment
