@@ -1146,7 +1146,7 @@ void rk4(S& s, int order) {
     // Neighbour source: the live arrays (lex) or a snapshot taken now (jacobi).
     std::vector<std::vector<double>> snap;
     const double* nb[NF];
-    if (order == ORC_ORDER_JACOBI) {
+    if (order == ORC_ORDER_JACOBI || order == ORC_ORDER_JACOBI_REVERSE) {
         snap.resize(NF);
         for (int f = 0; f < NF; f++) { snap[f] = s.f[fid[f]]; nb[f] = snap[f].data(); }
     } else {
@@ -1158,66 +1158,103 @@ void rk4(S& s, int order) {
     double *prod = s.F(ORC_PROD), *tks = s.F(ORC_TKE_SRC), *dss = s.F(ORC_DIS_SRC),
            *au = s.F(ORC_AUX_U), *av = s.F(ORC_AUX_V), *aw = s.F(ORC_AUX_W);
 
-    auto do_column = [&](int i, int j) {
-        Geo g;
-        const double rm = s.rad[i];
-        g.exp_rm = 1.0 / (rm + 1.0);
-        g.exp_2_rm = g.exp_rm * g.exp_rm;
-        g.sinthe = sint[j];
-        g.sinthe2 = g.sinthe * g.sinthe;
-        g.costhe = cost[j];
-        const double rmet = s.metric_radius(rm);
-        const double rmet2 = rmet * rmet;
-        g.inv_rm = 1.0 / rmet;
-        g.inv_rm2 = 1.0 / rmet2;
-        g.inv_rmsinthe = 1.0 / (rmet * g.sinthe);
-        g.inv_rm2sinthe = g.inv_rm2 / g.sinthe;
-        g.inv_rm2sinthe2 = g.inv_rm2 / g.sinthe2;
-        g.costhe_inv_rm2sinthe = g.costhe * g.inv_rm2sinthe;
-        g.inv_2dr = 1.0 / (2.0 * s.dr); g.inv_2dthe = 1.0 / (2.0 * s.dthe); g.inv_2dphi = 1.0 / (2.0 * s.dphi);
-        g.inv_dr2 = 1.0 / (s.dr * s.dr); g.inv_dthe2 = 1.0 / (s.dthe * s.dthe); g.inv_dphi2 = 1.0 / (s.dphi * s.dphi);
+    // Per-(i, j) metric and per-i drag, shared by every order (the same values the reference
+    // forms at the top of each column).
+    std::vector<Geo> geo((size_t)im * jm);
+    std::vector<double> drag(im);
+    for (int i = 0; i < im; i++) {
         // Rayleigh drag :1014-1019 (i_topography = 0)
         double drag_profile = 1.0 - (double)(i - 0) / 5.0;
         if (drag_profile < 0.0) drag_profile = 0.0;
         if (drag_profile > 1.0) drag_profile = 1.0;
-        const double surf_drag = ((1.0 / 86400.0) * P.L_atm / P.u_0 * dt) * drag_profile;
-        for (int k = 1; k < km - 1; k++) {
-            const size_t id = s.ix(i, j, k);
-            double base[NF], cen[NF], kk[4][NF], rhs[NF], aux[3];
-            for (int f = 0; f < NF; f++) { base[f] = XN[f][id]; cen[f] = X[f][id]; }
-            double pr = 0, ts = 0, ds = 0;
-            const double p_surf = ps[s.ix(0, j, k)];
-            for (int st = 0; st < 4; st++) {
-                rhs_cell(s, g, K, i, j, k, cen, nb, pd, nue[id], ps[id], p_surf, teq[id], s.t_ref[i],
-                         buoy_ramp_dt, surf_drag, rhs, aux, pr, ts, ds);
-                for (int f = 0; f < NF; f++) kk[st][f] = rhs[f];
-                const double h = (st < 2) ? half_dt : dt;
-                for (int f = 0; f < NF; f++) {
-                    double val;
-                    if (st < 3) val = base[f] + kk[st][f] * h;
-                    else val = base[f] + (kk[0][f] + 2.0 * kk[1][f] + 2.0 * kk[2][f] + kk[3][f]) * dt_sixth;
-                    if (f == FC || f == FCL || f == FI || f == FG) val = dmax(0.0, val);
-                    else if (f == FK) {
-                        if (!finite_bits(val)) val = 0.0;
-                        else val = (val < 0.0) ? 0.0 : (val > tke_max_nd) ? tke_max_nd : val;
-                    } else if (f == FD) val = dmax(1.0e-10, val);
-                    cen[f] = val;
-                }
-                if (order == ORC_ORDER_LEX)
-                    for (int f = 0; f < NF; f++) X[f][id] = cen[f];
-            }
-            for (int f = 0; f < NF; f++) X[f][id] = cen[f];
-            prod[id] = pr; tks[id] = ts; dss[id] = ds;
-            au[id] = aux[0]; av[id] = aux[1]; aw[id] = aux[2];
+        drag[i] = ((1.0 / 86400.0) * P.L_atm / P.u_0 * dt) * drag_profile;
+        for (int j = 0; j < jm; j++) {
+            Geo& g = geo[(size_t)i * jm + j];
+            const double rm = s.rad[i];
+            g.exp_rm = 1.0 / (rm + 1.0);
+            g.exp_2_rm = g.exp_rm * g.exp_rm;
+            g.sinthe = sint[j];
+            g.sinthe2 = g.sinthe * g.sinthe;
+            g.costhe = cost[j];
+            const double rmet = s.metric_radius(rm);
+            const double rmet2 = rmet * rmet;
+            g.inv_rm = 1.0 / rmet;
+            g.inv_rm2 = 1.0 / rmet2;
+            g.inv_rmsinthe = 1.0 / (rmet * g.sinthe);
+            g.inv_rm2sinthe = g.inv_rm2 / g.sinthe;
+            g.inv_rm2sinthe2 = g.inv_rm2 / g.sinthe2;
+            g.costhe_inv_rm2sinthe = g.costhe * g.inv_rm2sinthe;
+            g.inv_2dr = 1.0 / (2.0 * s.dr); g.inv_2dthe = 1.0 / (2.0 * s.dthe); g.inv_2dphi = 1.0 / (2.0 * s.dphi);
+            g.inv_dr2 = 1.0 / (s.dr * s.dr); g.inv_dthe2 = 1.0 / (s.dthe * s.dthe); g.inv_dphi2 = 1.0 / (s.dphi * s.dphi);
         }
+    }
+    // In place (lex orders): every stage's value is written back before the next RHS, as the
+    // reference's in-place sweep leaves it (RungeKutta_Atm_Turb.cpp:167-263).
+    const bool in_place = (order == ORC_ORDER_LEX || order == ORC_ORDER_LEX_PLANES);
+
+    auto do_cell = [&](int i, int j, int k) {
+        const Geo& g = geo[(size_t)i * jm + j];
+        const double surf_drag = drag[i];
+        const size_t id = s.ix(i, j, k);
+        double base[NF], cen[NF], kk[4][NF], rhs[NF], aux[3];
+        for (int f = 0; f < NF; f++) { base[f] = XN[f][id]; cen[f] = X[f][id]; }
+        double pr = 0, ts = 0, ds = 0;
+        const double p_surf = ps[s.ix(0, j, k)];
+        for (int st = 0; st < 4; st++) {
+            rhs_cell(s, g, K, i, j, k, cen, nb, pd, nue[id], ps[id], p_surf, teq[id], s.t_ref[i],
+                     buoy_ramp_dt, surf_drag, rhs, aux, pr, ts, ds);
+            for (int f = 0; f < NF; f++) kk[st][f] = rhs[f];
+            const double h = (st < 2) ? half_dt : dt;
+            for (int f = 0; f < NF; f++) {
+                double val;
+                if (st < 3) val = base[f] + kk[st][f] * h;
+                else val = base[f] + (kk[0][f] + 2.0 * kk[1][f] + 2.0 * kk[2][f] + kk[3][f]) * dt_sixth;
+                if (f == FC || f == FCL || f == FI || f == FG) val = dmax(0.0, val);
+                else if (f == FK) {
+                    if (!finite_bits(val)) val = 0.0;
+                    else val = (val < 0.0) ? 0.0 : (val > tke_max_nd) ? tke_max_nd : val;
+                } else if (f == FD) val = dmax(1.0e-10, val);
+                cen[f] = val;
+            }
+            if (in_place)
+                for (int f = 0; f < NF; f++) X[f][id] = cen[f];
+        }
+        for (int f = 0; f < NF; f++) X[f][id] = cen[f];
+        prod[id] = pr; tks[id] = ts; dss[id] = ds;
+        au[id] = aux[0]; av[id] = aux[1]; aw[id] = aux[2];
     };
+    const int ni = im - 2, nj = jm - 2;
     if (order == ORC_ORDER_LEX) {
+        // RungeKutta_Atm_Turb.cpp:99-264 at OMP_NUM_THREADS=1: i, then j, then k ascending.
         for (int i = 1; i < im - 1; i++)
-            for (int j = 1; j < jm - 1; j++) do_column(i, j);
+            for (int j = 1; j < jm - 1; j++)
+                for (int k = 1; k < km - 1; k++) do_cell(i, j, k);
+    } else if (order == ORC_ORDER_LEX_PLANES) {
+        // The same sweep by hyperplanes i + j + k = d. A cell reads its six neighbours only,
+        // which lie on planes d - 1 (already final in the sequential sweep) and d + 1 (not yet
+        // reached), and writes only itself, so the cells of one plane are independent and the
+        // planes in increasing d reproduce the sequential sweep bit for bit.
+        const int dmax_ = (im - 2) + (jm - 2) + (km - 2);
+        #pragma omp parallel
+        for (int d = 3; d <= dmax_; d++) {
+            #pragma omp for schedule(static)
+            for (int q = 0; q < ni * nj; q++) {
+                const int i = 1 + q / nj, j = 1 + q % nj, k = d - i - j;
+                if (k >= 1 && k <= km - 2) do_cell(i, j, k);
+            }
+        }
+    } else if (order == ORC_ORDER_JACOBI_REVERSE) {
+        // Jacobi order swept backwards in i, j and k: identical results unless some read sees
+        // a value written in this sweep.
+        #pragma omp parallel for collapse(2) schedule(static)
+        for (int i = im - 2; i >= 1; i--)
+            for (int j = jm - 2; j >= 1; j--)
+                for (int k = km - 2; k >= 1; k--) do_cell(i, j, k);
     } else {
         #pragma omp parallel for collapse(2) schedule(static)
         for (int i = 1; i < im - 1; i++)
-            for (int j = 1; j < jm - 1; j++) do_column(i, j);
+            for (int j = 1; j < jm - 1; j++)
+                for (int k = 1; k < km - 1; k++) do_cell(i, j, k);
     }
 }
 
@@ -2309,6 +2346,15 @@ double* orc_mp_ptr(orc_state* s, int which) {
     if (which < 0 || which >= ORC_NMP) return nullptr;
     ensure_mp(*s);
     return s->mp[which].data();
+}
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }
 void orc_step(orc_state* s, int order) { step(*s, order); }
 void orc_run(orc_state* s, int n, int order) { for (int q = 0; q < n; q++) step(*s, order); }

@@ -13,6 +13,12 @@
 //                 (RungeKutta_Atm_Turb.cpp:99-264 with one thread)
 //   ORDER_JACOBI  every cell integrates its four stages against the start-of-RK4
 //                 neighbourhood (order-free; the GPU throughput form)
+//   ORDER_LEX_PLANES  ORDER_LEX computed hyperplane by hyperplane (i+j+k = d) with the cells
+//                 of a plane in parallel: bit-identical to ORDER_LEX, fast enough for the
+//                 full grid
+//   ORDER_JACOBI_REVERSE  ORDER_JACOBI swept with i, j, k descending: identical results
+//                 unless a read sees a value written in the same sweep (a check of the
+//                 Jacobi restatement)
 #pragma once
 #include <cstdint>
 
@@ -31,7 +37,11 @@ enum orc_field {
 // 2-D (jm x km) fields
 enum orc_field2d { ORC2_VEL_STAR = 0, ORC2_T_TOP_INIT = 1, ORC2_NFIELD = 2 };
 
-enum orc_order { ORC_ORDER_LEX = 0, ORC_ORDER_JACOBI = 1 };
+enum orc_order { ORC_ORDER_LEX = 0, ORC_ORDER_JACOBI = 1, ORC_ORDER_LEX_PLANES = 2, ORC_ORDER_JACOBI_REVERSE = 3 };
+
+// OpenMP threads of the oracle's parallel loops (n <= 0: the OpenMP default); returns the
+// number now in use.
+int orc_set_threads(int n);
 
 // Operators that can be called one at a time (unit parity tests).
 enum orc_op {

@@ -24,7 +24,7 @@ FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 FIELDS2D = ["vel_star", "t_top_init"]
 OPS = ["poisson", "satadj", "damp_moist", "moist_clamp", "densities", "evaporation",
        "turbulence", "damp_turb", "bc", "rk4", "polar", "radial", "store", "level_mean"]
-ORDER = {"lex": 0, "jacobi": 1}
+ORDER = {"lex": 0, "jacobi": 1, "lex_planes": 2, "jacobi_reverse": 3}
 
 
 class Member(ctypes.Structure):
@@ -64,6 +64,8 @@ def lib():
         L.orc_level_mean.argtypes = [vp, i]
         L.orc_step.argtypes = [vp, i]
         L.orc_run.argtypes = [vp, i, i]
+        L.orc_set_threads.restype = i
+        L.orc_set_threads.argtypes = [i]
         L.orc_op.argtypes = [vp, i, i]
         for name in ("orc_p_sat", "orc_p_subl", "orc_p_auto", "orc_latent_heat"):
             getattr(L, name).restype = d
@@ -97,6 +99,12 @@ def lib():
         L.orc_evap_ptr.argtypes = [vp]
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's parallel loops (n <= 0: the OpenMP default); returns the
+    number in use."""
+    return lib().orc_set_threads(int(n))
 
 
 def resolve(x_h2o, x_co2, x_n2):

@@ -10,6 +10,12 @@ in the tests from the same seed, so only outputs are stored.
     python tests/golden/make_ref_vectors.py --radiation   # ref_radiation.npz
     python tests/golden/make_ref_vectors.py --ice         # ref_ice.npz
     python tests/golden/make_ref_vectors.py --conv        # ref_conv.npz
+    python tests/golden/make_ref_vectors.py --grids       # ref_grids.npz
+
+--grids needs the harness built at the benchmark grids by oracle/Makefile.grid (a copy of
+the reference's sources with only the grid constants and the spacing changed):
+    make -C oracle -f Makefile.grid IM=21 JM=91 KM=181 -j8
+    make -C oracle -f Makefile.grid IM=41 JM=181 KM=361 -j8
 """
 from __future__ import annotations
 
@@ -212,7 +218,83 @@ def conv_vectors():
     print("wrote", path)
 
 
+# Whole lex-order integrations written by the reference at the benchmark grids and at its
+# built-in grid: (grid, seed, steps). 21 x 91 x 181 and 41 x 181 x 361 run on a copy of the
+# reference compiled at that grid (oracle/Makefile.grid).
+GRID_RUNS = [((21, 91, 181), 77, 10), ((21, 91, 181), 77, 100), ((41, 181, 361), 1234, 100),
+             ((61, 181, 361), 1234, 20)]
+GRID_FIELDS = O.FIELDS[:11] + ["p_dyn", "p_stat", "nue", "aux_u", "aux_v", "aux_w", "prod", "tke_source",
+                               "dis_source", "r_humid"]
+GRID_NSAMPLE = 2048
+
+
+def grid_harness(grid):
+    if tuple(grid) == GRID:
+        return HARNESS
+    return os.path.join(ROOT, "oracle", "_ref", "g%dx%dx%d" % tuple(grid), "ref_harness")
+
+
+def grid_vectors():
+    """Every run of GRID_RUNS in the reference's single-thread order from the nominal initial
+    state of its seed (the runs go in parallel, one reference thread each), plus the
+    reference's thermodynamic init chain at each grid. Stored per run: 2048 sampled cells of
+    every field in GRID_FIELDS, and every field's sum and L2 norm."""
+    wd = tempfile.mkdtemp(prefix="athad_golden_grids_")
+    out = {}
+    procs = []
+    for q, (grid, seed, steps) in enumerate(GRID_RUNS):
+        h = grid_harness(grid)
+        if not os.path.exists(h):
+            sys.exit(f"missing {h}: make -C oracle -f Makefile.grid IM={grid[0]} JM={grid[1]} KM={grid[2]}")
+        st = O.OracleState(*grid, seed=seed)
+        init = os.path.join(wd, f"init{q}.bin")
+        st.save_bin(init)
+        del st
+        o = os.path.join(wd, f"out{q}.bin")
+        procs.append((q, o, subprocess.Popen([h, init, o, "steps", str(steps)], cwd=wd, stdout=subprocess.DEVNULL,
+                                             env=dict(os.environ, OMP_NUM_THREADS="1"))))
+    inits = {}
+    for grid in sorted({g for g, _, _ in GRID_RUNS}):
+        st = O.OracleState(*grid, thermo_only=True)
+        blank = os.path.join(wd, "blank%dx%dx%d.bin" % grid)
+        st.save_bin(blank)
+        del st
+        o = os.path.join(wd, "initref%dx%dx%d.bin" % grid)
+        subprocess.run([grid_harness(grid), blank, o, "init"], check=True, cwd=wd, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS="2"))
+        ref = O.read_bin(o)
+        tag = "%dx%dx%d" % grid
+        js = [0, grid[1] // 4, grid[1] // 2, (3 * grid[1]) // 4, grid[1] - 1]
+        for name in ["t", "p_stat", "r_humid", "r_dry", "c", "co2"]:
+            out[f"{tag}/init/{name}"] = ref[name][:, js, 0]
+        out[f"{tag}/init/t_top_init"] = ref["t_top_init"][:, 0]
+        out[f"{tag}/init/js"] = np.array(js)
+        print("init chain", tag, flush=True)
+    for q, o, p in procs:
+        if p.wait() != 0:
+            sys.exit(f"reference run {GRID_RUNS[q]} failed")
+        grid, seed, steps = GRID_RUNS[q]
+        ref = O.read_bin(o)
+        tag = "%dx%dx%d/seed%d/steps%d" % (grid + (seed, steps))
+        n = grid[0] * grid[1] * grid[2]
+        idx = np.sort(np.random.default_rng(99).choice(n, size=GRID_NSAMPLE, replace=False))
+        out[f"{tag}/index"] = idx
+        for name in GRID_FIELDS:
+            a = ref[name].reshape(-1)
+            out[f"{tag}/{name}/sample"] = a[idx]
+            out[f"{tag}/{name}/sum"] = np.array(np.sum(a))
+            out[f"{tag}/{name}/l2"] = np.array(np.sqrt(np.sum(a * a)))
+        print("run", tag, flush=True)
+    out["runs"] = np.array([list(g) + [s, n] for g, s, n in GRID_RUNS])
+    path = os.path.join(HERE, "ref_grids.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def main():
+    if "--grids" in sys.argv[1:]:
+        grid_vectors()
+        return
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle -f Makefile.ref")
     if "--conv" in sys.argv[1:]:
