@@ -901,7 +901,12 @@ template <typename C> ATHAD_DEV C log_fast(C x) {
 // cancellation, within 3 ulp of the library tanh over [1e-9, 25] (mean 0.5 ulp), at about a
 // third of its cost (the library's FP64 tanh is ~165 instructions, expm1 ~50). Below 1e-9 (double) or
 // 1e-4 (float) the correctly rounded tanh(x) is x itself (x^3/3 is below half an ulp of x).
-template <typename C> ATHAD_DEV C tanh_pos(C x) {
+template <typename C, bool BRANCHLESS = false> ATHAD_DEV C tanh_pos(C x) {
+    if constexpr (BRANCHLESS && sizeof(C) == 8) {   // both results, then a select
+        const C m = expm1(C(-2) * x);
+        const C r = -m * rcp_nr(C(2) + m);
+        return (x >= C(1.0e-9)) ? r : x;
+    }
     C r = x;
     if (x >= C(sizeof(C) == 8 ? 1.0e-9 : 1.0e-4)) {   // a branch: skipped when no lane needs it
         if constexpr (sizeof(C) == 8) {
@@ -947,7 +952,7 @@ template <typename C> struct CellGeo {
 };
 
 template <typename C>
-ATHAD_DEV void cell_geo(const MemberDesc& M, long long id, int i, int j, int k, CellGeo<C>& g) {
+ATHAD_DEV void cell_geo(const MemberDesc& M, int i, int j, double ps_c, double p_surf, CellGeo<C>& g) {
     const Consts& K = M.K;
     const double rm = M.rad[i];
     const double exp_rm = 1.0 / (rm + 1.0);
@@ -973,9 +978,7 @@ ATHAD_DEV void cell_geo(const MemberDesc& M, long long id, int i, int j, int k, 
     g.cu = C(2.0 * inv_rm2 - Lc2);
     g.cvw = C(-v_metric - Lc2);
     g.cirm2 = C(inv_rm2); g.cirm2s = C(inv_rm2sinthe); g.c2cos = C(2.0 * costhe * inv_rm2s2);
-    const double* ps = pstat(M);
-    const double p_surf = gld(ps, (long long)ix3(K, 0, j, k));
-    const double sigma = (p_surf > 0.0) ? gld(ps, id) / p_surf : 0.0;
+    const double sigma = (p_surf > 0.0) ? ps_c / p_surf : 0.0;
     double bl = (sigma - 0.7) / (1.0 - 0.7);
     if (bl < 0.0) bl = 0.0;
     const double kT = K.hs_k_a + (K.hs_k_s - K.hs_k_a) * bl * (sinthe2 * sinthe2);
@@ -998,6 +1001,89 @@ ATHAD_DEV void load6(const S* x, long long id, long long si, long long sj, int f
         for (int d = 0; d < 6; d++) v[d] = solid_clamp(f, v[d], tkm, vm);
     }
 }
+
+constexpr int kProgF[NPROG] = {F_T, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2, F_TKE, F_DIS};
+constexpr int kProgN[NPROG] = {F_TN, F_UN, F_VN, F_WN, F_CN, F_CLOUDN, F_ICEN, F_GRN, F_CO2N, F_TKEN, F_DISN};
+
+// Where one cell's RK4 reads its inputs (the 7-point neighbourhoods of the 11 prognostic
+// fields and p_dyn, the start and base values, nue, t_eq, p_stat and the surface p_stat, and
+// with the ice scheme r_humid and the five sources).
+// NbLoad reads each value from global memory where the computation first needs it: the
+// Jacobi kernel runs several waves per SIMD, which hide the latency. NbRegs issues every load
+// of the cell in one batch into registers before any arithmetic: the lex plane kernel runs
+// about one wave per SIMD (a plane holds a few thousand cells), so every dependent memory
+// round trip of its cell chain is exposed and one round trip is what it can afford.
+template <typename S, typename C, bool CLAMP> struct NbLoad {
+    const MemberDesc& M;
+    const S* const* X;
+    long long id, si, sj, isurf;
+    ATHAD_DEV void six(int f, C* v) const {
+        load6<S, C, CLAMP>(X[f], id, si, sj, f, C(M.K.tke_max_nd), C(M.K.vel_max_nd), v);
+    }
+    ATHAD_DEV void pd6(C* v) const {
+        const S* p = fp<S>(M, F_P_DYN);
+        v[0] = C(gld(p, id + si)); v[1] = C(gld(p, id - si)); v[2] = C(gld(p, id + sj));
+        v[3] = C(gld(p, id - sj)); v[4] = C(gld(p, id + 1)); v[5] = C(gld(p, id - 1));
+    }
+    ATHAD_DEV C field(int fid) const { return C(gld(fp<S>(M, fid), id)); }   // nue, t_eq, r_humid
+    ATHAD_DEV C mp(int q) const { return C(gld((const S*)M.mp[q], id)); }
+    ATHAD_DEV C start(int f) const {
+        C c0 = C(gld(X[f], id));
+        if (CLAMP) c0 = solid_clamp(f, c0, C(M.K.tke_max_nd), C(M.K.vel_max_nd));
+        return c0;
+    }
+    ATHAD_DEV C base(int f) const {
+        C b0 = C(gld(fp<S>(M, kProgN[f]), id));
+        if (CLAMP) b0 = solid_clamp_n(f, b0, C(M.K.tke_max_nd), C(M.K.vel_max_nd));
+        return b0;
+    }
+    ATHAD_DEV double ps() const { return gld(pstat(M), id); }
+    ATHAD_DEV double psurf() const { return gld(pstat(M), isurf); }
+};
+
+template <typename S, typename C, bool SRC> struct NbRegs {
+    C nb[NPROG][6], pd[6], c0[NPROG], b0[NPROG], nu, te, rh, src[5];
+    double psv, pss;
+    ATHAD_DEV NbRegs(const MemberDesc& M, const S* const* X, long long id, long long si, long long sj,
+                     long long isurf) {
+        const C tkm = C(M.K.tke_max_nd), vm = C(M.K.vel_max_nd);
+#pragma unroll
+        for (int f = 0; f < NPROG; f++) {
+            load6<S, C, false>(X[f], id, si, sj, f, tkm, vm, nb[f]);
+            c0[f] = C(gld(X[f], id));
+            b0[f] = C(gld(fp<S>(M, kProgN[f]), id));
+        }
+        const S* p = fp<S>(M, F_P_DYN);
+        pd[0] = C(gld(p, id + si)); pd[1] = C(gld(p, id - si)); pd[2] = C(gld(p, id + sj));
+        pd[3] = C(gld(p, id - sj)); pd[4] = C(gld(p, id + 1)); pd[5] = C(gld(p, id - 1));
+        nu = C(gld(fp<S>(M, F_NUE), id));
+        te = C(gld(fp<S>(M, F_T_EQ), id));
+        psv = gld(pstat(M), id);
+        pss = gld(pstat(M), isurf);
+        rh = C(0);
+#pragma unroll
+        for (int q = 0; q < 5; q++) src[q] = C(0);
+        if (SRC) {
+            rh = C(gld(fp<S>(M, F_R_HUMID), id));
+#pragma unroll
+            for (int q = 0; q < 5; q++) src[q] = C(gld((const S*)M.mp[MP_S_V + q], id));
+        }
+    }
+    ATHAD_DEV void six(int f, C* v) const {
+#pragma unroll
+        for (int d = 0; d < 6; d++) v[d] = nb[f][d];
+    }
+    ATHAD_DEV void pd6(C* v) const {
+#pragma unroll
+        for (int d = 0; d < 6; d++) v[d] = pd[d];
+    }
+    ATHAD_DEV C field(int fid) const { return fid == F_NUE ? nu : fid == F_T_EQ ? te : rh; }
+    ATHAD_DEV C mp(int q) const { return src[q - MP_S_V]; }
+    ATHAD_DEV C start(int f) const { return c0[f]; }
+    ATHAD_DEV C base(int f) const { return b0[f]; }
+    ATHAD_DEV double ps() const { return psv; }
+    ATHAD_DEV double psurf() const { return pss; }
+};
 
 template <typename C> ATHAD_DEV C minmod3(const C* nb, C c0, C ua, C va, C wa) {
     return ua * minmod(c0 - nb[1], nb[0] - c0) + va * minmod(c0 - nb[3], nb[2] - c0)
@@ -1036,21 +1122,32 @@ template <typename C> ATHAD_DEV C stage_update(int st, C r, C* b, C* a, C dt, C 
 
 constexpr int LB = 256;   // LDS slot stride = threads per block of the RK4 kernels
 
+// The four RK4 stages of a field group: a loop (U = false: the code of one stage, the fewest
+// registers, for the Jacobi kernel's occupancy) or unrolled (U = true: every stage's branches
+// fold, and the independent chains of a group's fields schedule into one another, for the
+// lex plane kernel, whose one wave per SIMD exposes every dependent-issue latency).
+template <bool U, typename F> ATHAD_DEV void for_stages(F&& f) {
+    if constexpr (U) {
+        f(0); f(1); f(2); f(3);
+    } else {
+#pragma unroll 1
+        for (int st = 0; st < 4; st++) f(st);
+    }
+}
+
 // RK4 of the dynamics group T, u, v, w (RHS_Atm_Turb.cpp rhs_t/rhs_u/rhs_v/rhs_w). Their
 // tendencies involve no other prognostic field, so the group integrates first and records
 // the velocity entering each stage in rec[] for the scalar groups. cen[] = start values in,
 // final values out. A, B: velocity-gradient invariants of the production term.
-template <typename S, typename C, bool CLAMP, bool SRC>
-ATHAD_DEV void rk4_dyn(const MemberDesc& M, const S* const* X, long long id, int i, const CellGeo<C>& g,
+template <typename S, typename C, bool CLAMP, bool SRC, bool U, typename Nb>
+ATHAD_DEV void rk4_dyn(const MemberDesc& M, const Nb& src_nb, int i, const CellGeo<C>& g,
                        C* ba, C* rec, C* cen, C* aux, C& A, C& B) {
     const Consts& K = M.K;
-    const long long si = (long long)K.jm * K.km, sj = K.km;
-    const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
     const C i2r = C(K.inv_2dr), i2t = C(K.inv_2dthe), i2p = C(K.inv_2dphi);
     C nb[4][6], g1r[4], g1t[4], g1p[4], Sr[4], St[4], Sp[4];
 #pragma unroll
     for (int f = 0; f < 4; f++) {
-        load6<S, C, CLAMP>(X[f], id, si, sj, f, tkm, vm, nb[f]);
+        src_nb.six(f, nb[f]);
         g1r[f] = (nb[f][0] - nb[f][1]) * i2r;
         g1t[f] = (nb[f][2] - nb[f][3]) * i2t;
         g1p[f] = (nb[f][4] - nb[f][5]) * i2p;
@@ -1077,35 +1174,35 @@ ATHAD_DEV void rk4_dyn(const MemberDesc& M, const S* const* X, long long id, int
     }
     C PR, PT, PP;
     {
-        const S* p = fp<S>(M, F_P_DYN);
-        PR = (C(gld(p, id + si)) - C(gld(p, id - si))) * i2r * g.cexp;
-        PT = (C(gld(p, id + sj)) - C(gld(p, id - sj))) * i2t * g.cirm;
-        PP = (C(gld(p, id + 1)) - C(gld(p, id - 1))) * i2p * g.cirs;
+        C p[6];
+        src_nb.pd6(p);
+        PR = (p[0] - p[1]) * i2r * g.cexp;
+        PT = (p[2] - p[3]) * i2t * g.cirm;
+        PP = (p[4] - p[5]) * i2p * g.cirs;
     }
     C dif_t, dif_vel;
     {
-        C nv = C(gld(fp<S>(M, F_NUE), id));
+        C nv = src_nb.field(F_NUE);
         if (CLAMP) nv = finite_bits(nv) ? tclamp(nv, C(0), C(K.nue_max)) : C(0);
         const C inv_re = C(1.0 / K.re_turb);
         dif_vel = inv_re + nv;
         dif_t = (inv_re + nv) * C(1.0 / K.pr_turb);
     }
-    const C t_eq = C(gld(fp<S>(M, F_T_EQ), id));
+    const C t_eq = src_nb.field(F_T_EQ);
     const C t_ref = C(M.t_ref[i]);
     const C buoy = buoyancy<C>(M);
     const C ce = C(K.coeff_energy_p);
     // latent heating of the microphysics sources (RHS_Atm_Turb.cpp:965-966), stage-invariant
     C srcT = C(0);
     if (SRC) {
-        const C rh = C(gld(fp<S>(M, F_R_HUMID), id));
-        const C Sc = C(gld((const S*)M.mp[MP_S_C], id)), Sr = C(gld((const S*)M.mp[MP_S_R], id));
-        const C Si = C(gld((const S*)M.mp[MP_S_I], id)), Ss = C(gld((const S*)M.mp[MP_S_S], id));
+        const C rh = src_nb.field(F_R_HUMID);
+        const C Sc = src_nb.mp(MP_S_C), Sr = src_nb.mp(MP_S_R);
+        const C Si = src_nb.mp(MP_S_I), Ss = src_nb.mp(MP_S_S);
         const C cen_ = C(K.coeff_energy);
         srcT = cen_ * (Sc + Sr) * C(K.lv) * rh + cen_ * (Si + Ss + C(0)) * C(K.ls) * rh;
     }
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
-#pragma unroll 1
-    for (int st = 0; st < 4; st++) {
+for_stages<U>([&](const int st) {
         const C T = cen[FT], u = cen[FU], v = cen[FV], w = cen[FW];
         rec[(3 * st + 0) * LB] = u; rec[(3 * st + 1) * LB] = v; rec[(3 * st + 2) * LB] = w;
         const C ua = u * g.aR, va = v * g.aT, wa = w * g.aP;
@@ -1120,24 +1217,22 @@ ATHAD_DEV void rk4_dyn(const MemberDesc& M, const S* const* X, long long id, int
         cen[FU] = stage_update(st, rU, ba + 1 * LB, ba + 5 * LB, dt, half_dt, dt_sixth);
         cen[FV] = stage_update(st, rV, ba + 2 * LB, ba + 6 * LB, dt, half_dt, dt_sixth);
         cen[FW] = stage_update(st, rW, ba + 3 * LB, ba + 7 * LB, dt, half_dt, dt_sixth);
-    }
+    });
 }
 
 // RK4 of the moisture group c, cloud, ice (minmod advection), gr, co2 (central), advected by
 // the recorded stage velocities; cen[0..4] = start values in, final values out.
-template <typename S, typename C, bool CLAMP, bool SRC, typename Vel>
-ATHAD_DEV void rk4_moist(const MemberDesc& M, const S* const* X, long long id, const CellGeo<C>& g, C* ba,
+template <typename S, typename C, bool CLAMP, bool SRC, bool U, typename Nb, typename Vel>
+ATHAD_DEV void rk4_moist(const MemberDesc& M, const Nb& src_nb, const CellGeo<C>& g, C* ba,
                          Vel vel, C* cen) {
     const Consts& K = M.K;
-    const long long si = (long long)K.jm * K.km, sj = K.km;
-    const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
     const C i2r = C(K.inv_2dr), i2t = C(K.inv_2dthe), i2p = C(K.inv_2dphi);
     C nb[3][6], L0[5], gc[2][3];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         const int f = FC + q;
         C v[6];
-        load6<S, C, CLAMP>(X[f], id, si, sj, f, tkm, vm, v);
+        src_nb.six(f, v);
         const C d1r = (v[0] - v[1]) * i2r, d1t = (v[2] - v[3]) * i2t, d1p = (v[4] - v[5]) * i2p;
         L0[q] = (v[0] + v[1]) * g.wr + d1r * g.two_over + (v[2] + v[3]) * g.wt + d1t * g.crs + (v[4] + v[5]) * g.wp;
         if (q < 3) {
@@ -1151,14 +1246,13 @@ ATHAD_DEV void rk4_moist(const MemberDesc& M, const S* const* X, long long id, c
     // microphysics sources of c, cloud, ice (RHS_Atm_Turb.cpp:1058-1067), stage-invariant
     C src[3] = {C(0), C(0), C(0)};
     if (SRC) {
-        const C rh = C(gld(fp<S>(M, F_R_HUMID), id));
-        src[0] = C(1) * C(gld((const S*)M.mp[MP_S_V], id)) * rh;
-        src[1] = C(1) * C(gld((const S*)M.mp[MP_S_C], id)) * rh;
-        src[2] = C(1) * C(gld((const S*)M.mp[MP_S_I], id)) * rh;
+        const C rh = src_nb.field(F_R_HUMID);
+        src[0] = C(1) * src_nb.mp(MP_S_V) * rh;
+        src[1] = C(1) * src_nb.mp(MP_S_C) * rh;
+        src[2] = C(1) * src_nb.mp(MP_S_I) * rh;
     }
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
-#pragma unroll 1
-    for (int st = 0; st < 4; st++) {
+for_stages<U>([&](const int st) {
         C u, v, w;
         vel(st, u, v, w);
         const C ua = u * g.aR, va = v * g.aT, wa = w * g.aP;
@@ -1175,25 +1269,24 @@ ATHAD_DEV void rk4_moist(const MemberDesc& M, const S* const* X, long long id, c
             const C val = stage_update(st, r[q], ba + q * LB, ba + (5 + q) * LB, dt, half_dt, dt_sixth);
             cen[q] = (q < 4) ? tmax(C(0), val) : val;
         }
-    }
+    });
 }
 
 // RK4 of the turbulence group tke, dis with the k-omega SST closure (RHS_Atm_Turb.cpp:633-721),
 // advected by the recorded stage velocities. cen[0..1] = start values in, final values out;
 // prod and the sources are those of the fourth stage.
-template <typename S, typename C, bool CLAMP, typename Vel>
-ATHAD_DEV void rk4_turb(const MemberDesc& M, const S* const* X, long long id, int i, int j, int k,
+template <typename S, typename C, bool CLAMP, bool U, typename Nb, typename Vel>
+ATHAD_DEV void rk4_turb(const MemberDesc& M, const Nb& src_nb, int i, int j, int k,
                         const CellGeo<C>& g, C* ba, Vel vel, C A, C B, C* cen, C& prod, C& tsrc, C& dsrc) {
     const Consts& K = M.K;
-    const long long si = (long long)K.jm * K.km, sj = K.km;
-    const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
+    const C tkm = C(K.tke_max_nd);
     const C i2r = C(K.inv_2dr), i2t = C(K.inv_2dthe), i2p = C(K.inv_2dphi);
     C L0[2], gc[2][3], d1[2][3];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
         const int f = FK + q;
         C v[6];
-        load6<S, C, CLAMP>(X[f], id, si, sj, f, tkm, vm, v);
+        src_nb.six(f, v);
         d1[q][0] = (v[0] - v[1]) * i2r; d1[q][1] = (v[2] - v[3]) * i2t; d1[q][2] = (v[4] - v[5]) * i2p;
         L0[q] = (v[0] + v[1]) * g.wr + d1[q][0] * g.two_over + (v[2] + v[3]) * g.wt + d1[q][1] * g.crs
               + (v[4] + v[5]) * g.wp;
@@ -1201,7 +1294,7 @@ ATHAD_DEV void rk4_turb(const MemberDesc& M, const S* const* X, long long id, in
     }
     const C cross = d1[0][0] * d1[1][0] + (d1[0][1] * g.cirm) * (d1[1][1] * g.cirm)
                   + (d1[0][2] * g.cirs) * (d1[1][2] * g.cirs);
-    C nue = C(gld(fp<S>(M, F_NUE), id));
+    C nue = src_nb.field(F_NUE);
     if (CLAMP) nue = finite_bits(nue) ? tclamp(nue, C(0), C(K.nue_max)) : C(0);
     // SST wall distance min(i, j, k) (RHS_Atm_Turb.cpp:640-671 without land); the symmetric
     // variant measures j from the nearer pole.
@@ -1219,8 +1312,7 @@ ATHAD_DEV void rk4_turb(const MemberDesc& M, const S* const* X, long long id, in
     const C bet1 = C(0.0333), bet2 = C(0.0368), gam1 = C(0.413), gam2 = C(0.2);
     const C inv_re = C(1.0 / K.re_turb);
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
-#pragma unroll 1
-    for (int st = 0; st < 4; st++) {
+for_stages<U>([&](const int st) {
         C u, v, w;
         vel(st, u, v, w);
         const C tke = cen[0], dis = cen[1];
@@ -1232,9 +1324,9 @@ ATHAD_DEV void rk4_turb(const MemberDesc& M, const S* const* X, long long id, in
         const C cnue = tmin(tmax(tke, C(0)) * inv_dis, C(K.nue_max));
         prod = tmax(C(0), cnue * A - C(0.66667) * tke * B);
         const C CD_kw = tmax(C(2) * sig_w2 * inv_dis * cross, C(1.0e-20));
-        const C arg1 = tmin(tmax(sqrt(tke) * inv_dis * K2, K1 * inv_dis), tke * K3 * rcp_nr(CD_kw));
+        const C arg1 = tmin(tmax(C(sqrt(tke)) * inv_dis * K2, K1 * inv_dis), tke * K3 * rcp_nr(CD_kw));
         const C a2 = arg1 * arg1;
-        const C F1 = tanh_pos(a2 * a2);
+        const C F1 = tanh_pos<C, U>(a2 * a2);
         const C dif_tke = inv_re + nue * rcp_nr(F1 * sig_k1 + (C(1) - F1) * sig_k2);
         const C dif_dis = inv_re + nue * rcp_nr(F1 * sig_w1 + (C(1) - F1) * sig_w2);
         const C tke_s = tmax(tke, C(1.0e-10));
@@ -1253,11 +1345,8 @@ ATHAD_DEV void rk4_turb(const MemberDesc& M, const S* const* X, long long id, in
         const C vd = stage_update(st, rd, ba + 1 * LB, ba + 3 * LB, dt, half_dt, dt_sixth);
         cen[0] = vk;
         cen[1] = tmax(C(1.0e-10), vd);
-    }
+    });
 }
-
-constexpr int kProgF[NPROG] = {F_T, F_U, F_V, F_W, F_C, F_CLOUD, F_ICE, F_GR, F_CO2, F_TKE, F_DIS};
-constexpr int kProgN[NPROG] = {F_TN, F_UN, F_VN, F_WN, F_CN, F_CLOUDN, F_ICEN, F_GRN, F_CO2N, F_TKEN, F_DISN};
 
 // Waves per SIMD the RK4 kernels are compiled for (register budget 512 / waves).
 #ifndef ATHAD_RK4_WAVES_F64
@@ -1269,32 +1358,27 @@ constexpr int kProgN[NPROG] = {F_TN, F_UN, F_VN, F_WN, F_CN, F_CLOUDN, F_ICEN, F
 template <typename S> constexpr int rk4_waves() { return sizeof(S) == 8 ? ATHAD_RK4_WAVES_F64 : ATHAD_RK4_WAVES_F32; }
 
 // Start values (optionally clamped) into cen[] and the bases into the LDS slots of a group.
-template <typename S, typename C, bool CLAMP>
-ATHAD_DEV void group_start(const MemberDesc& M, const S* const* X, long long id, int f0, int nf, C* cen, C* ba) {
-    const C tkm = C(M.K.tke_max_nd), vm = C(M.K.vel_max_nd);
+template <typename C, typename Nb>
+ATHAD_DEV void group_start(const Nb& src_nb, int f0, int nf, C* cen, C* ba) {
 #pragma unroll
     for (int q = 0; q < nf; q++) {
-        const int f = f0 + q;
-        C c0 = C(gld(X[f], id));
-        C b0 = C(gld(fp<S>(M, kProgN[f]), id));
-        if (CLAMP) { c0 = solid_clamp(f, c0, tkm, vm); b0 = solid_clamp_n(f, b0, tkm, vm); }
-        cen[q] = c0;
-        ba[q * LB] = b0;
+        cen[q] = src_nb.start(f0 + q);
+        ba[q * LB] = src_nb.base(f0 + q);
     }
 }
 
 // RK4 of one interior cell, group by group: dynamics, moisture, turbulence. out(f, value)
 // writes a final value; the aux / prod / source outputs are written here.
-template <typename S, typename C, bool CLAMP, bool SRC, typename Out>
-ATHAD_DEV void rk4_cell(const MemberDesc& M, const S* const* X, long long id, int i, int j, int k, C* ba, C* rec,
+template <typename S, typename C, bool CLAMP, bool SRC, bool U, typename Nb, typename Out>
+ATHAD_DEV void rk4_cell(const MemberDesc& M, const Nb& nb, long long id, int i, int j, int k, C* ba, C* rec,
                         Out out) {
     CellGeo<C> g;
-    cell_geo<C>(M, id, i, j, k, g);
+    cell_geo<C>(M, i, j, nb.ps(), nb.psurf(), g);
     C A, B;
     {
         C cen[4], aux[3];
-        group_start<S, C, CLAMP>(M, X, id, FT, 4, cen, ba);
-        rk4_dyn<S, C, CLAMP, SRC>(M, X, id, i, g, ba, rec, cen, aux, A, B);
+        group_start(nb, FT, 4, cen, ba);
+        rk4_dyn<S, C, CLAMP, SRC, U>(M, nb, i, g, ba, rec, cen, aux, A, B);
 #pragma unroll
         for (int q = 0; q < 4; q++) out(FT + q, cen[q]);
         gst(fp<S>(M, F_AUX_U), id, S(aux[0]));
@@ -1306,15 +1390,15 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const S* const* X, long long id, in
     };
     {
         C cen[5];
-        group_start<S, C, CLAMP>(M, X, id, FC, 5, cen, ba);
-        rk4_moist<S, C, CLAMP, SRC>(M, X, id, g, ba, vel, cen);
+        group_start(nb, FC, 5, cen, ba);
+        rk4_moist<S, C, CLAMP, SRC, U>(M, nb, g, ba, vel, cen);
 #pragma unroll
         for (int q = 0; q < 5; q++) out(FC + q, cen[q]);
     }
     {
         C cen[2], prod, ts, ds;
-        group_start<S, C, CLAMP>(M, X, id, FK, 2, cen, ba);
-        rk4_turb<S, C, CLAMP>(M, X, id, i, j, k, g, ba, vel, A, B, cen, prod, ts, ds);
+        group_start(nb, FK, 2, cen, ba);
+        rk4_turb<S, C, CLAMP, U>(M, nb, i, j, k, g, ba, vel, A, B, cen, prod, ts, ds);
         out(FK, cen[0]);
         out(FD, cen[1]);
         gst(fp<S>(M, F_PROD), id, S(prod));
@@ -1366,7 +1450,8 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
         }
         return;
     }
-    rk4_cell<S, C, FUSED, SRC>(M, X, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x, [&](int f, C v) {
+    const NbLoad<S, C, FUSED> nb{M, X, id, (long long)K.jm * K.km, (long long)K.km, (long long)ix3(K, 0, j, k)};
+    rk4_cell<S, C, FUSED, SRC, false>(M, nb, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x, [&](int f, C v) {
         gst((S*)M.alt[f], id, S(v));
         if (FUSED && f != FU && f != FV && f != FW) gst(fp<S>(M, kProgN[f]), id, S(v));
     });
@@ -1376,7 +1461,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
 // plane only read neighbours on planes d-1 (final) and d+1 (not yet updated), which is
 // exactly what the reference's single-threaded sweep sees.
 template <typename S, bool SRC>
-__global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>()))) void k_rk4_lex_plane(const MemberDesc* __restrict__ md, int d) {
+__global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_rk4_lex_plane(const MemberDesc* __restrict__ md, int d) {
     using C = S;
     __shared__ C s_ba[10 * LB], s_rec[12 * LB];
     const MemberDesc& M = md[blockIdx.z];
@@ -1391,7 +1476,9 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
     S* X[NPROG];
 #pragma unroll
     for (int f = 0; f < NPROG; f++) X[f] = fp<S>(M, kProgF[f]);
-    rk4_cell<S, C, false, SRC>(M, (const S* const*)X, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
+    const NbRegs<S, C, SRC> nb(M, (const S* const*)X, id, (long long)K.jm * K.km, (long long)K.km,
+                               (long long)ix3(K, 0, j, k));
+    rk4_cell<S, C, false, SRC, true>(M, nb, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
                           [&](int f, C v) { gst(X[f], id, S(v)); });
 }
 

@@ -58,6 +58,38 @@ def run_form(A, precision, order, steps, warmup, seed=1234, profile_steps=0, gra
             "rk4_launches": rk4_n, "fields": fields}
 
 
+def order_spread(A, steps=100, seed=1234):
+    """The three FP64 update orders over the same `steps` iterations from the nominal state:
+    lex (the reference's single-thread sweep, timed here), jacobi (the throughput form) and
+    symmetric (mirror-exact). Reports, per field, the relative L2 distance of jacobi and
+    symmetric from lex: how far the throughput form ends from the reference's order."""
+    n = GRID[0] * GRID[1] * GRID[2]
+    names = A.PROGNOSTIC + ["p_dyn", "p_stat", "nue"]
+    out, fields = {}, {}
+    for order in ("lex", "jacobi", "symmetric"):
+        with A.Integration(GRID, order=order) as integ:
+            integ.init_state(seed)
+            integ.sync()
+            t0 = time.perf_counter()
+            integ.run(steps, graph=True)
+            integ.sync()
+            dt = time.perf_counter() - t0
+            fields[order] = {k: integ.get(k) for k in names}
+        out[order] = {"ms_per_step": dt / steps * 1e3, "cells_per_s": n * steps / dt}
+    import numpy as np
+
+    def rel(a, b):
+        return float(np.sqrt(np.sum((a - b) ** 2)) / max(np.sqrt(np.sum(b * b)), 1e-300))
+    for order in ("jacobi", "symmetric"):
+        e = {k: rel(fields[order][k], fields["lex"][k]) for k in names}
+        out[f"{order}_vs_lex_rel_l2"] = e
+        out[f"{order}_vs_lex_max"] = max(e.values())
+    out["workload"] = (f"{steps} FP64 iterations at 41x181x361 from seed {seed} in each order; "
+                       "the reference's own thread-count spread is ~1e-8 to 6e-6 after 2 iterations "
+                       "(README.md:205-208, survey probe)")
+    return out
+
+
 def run_loop(A, precision, steps, warmup, seed=1234, graph=True):
     """The iteration with both operators built beside the scoped step inside it: the
     two-category ice scheme in the moist block and mode-2 radiation with the skin-temperature
@@ -213,6 +245,8 @@ def main():
                     help="only the FP64 iteration with the ice scheme and radiation inside it")
     ap.add_argument("--ice-only", action="store_true",
                     help="only the FP64 step with the two-category ice scheme in the moist block")
+    ap.add_argument("--orders-only", action="store_true",
+                    help="only the lex / jacobi / symmetric spread over 100 iterations")
     ap.add_argument("--mixed-only", action="store_true",
                     help="only time the mixed form (for profiling it alone); prints its own JSON line")
     args = ap.parse_args()
@@ -230,6 +264,9 @@ def main():
     graph = not args.no_graph
     if args.next_rows_only:
         print(json.dumps({"grid": list(GRID), **run_next_rows(A)}))
+        return
+    if args.orders_only:
+        print(json.dumps({"grid": list(GRID), **order_spread(A, steps=args.steps)}))
         return
     if args.loop_only:
         r = run_loop(A, "fp64", args.steps, args.warmup, graph=graph)
@@ -291,9 +328,11 @@ def main():
         out["mixed"] = {"cells_per_s": mixed["cells_per_s"], "ms_per_step": mixed["ms_per_step"],
                         "speedup_vs_fp64": mixed["cells_per_s"] / fp64["cells_per_s"],
                         "rel_l2_vs_fp64": err, "max_rel_l2": max(err.values())}
-        lex_steps = max(4, min(args.steps, 12))
-        lex = run_form(A, "fp64", "lex", lex_steps, 4, graph=graph)
-        out["lex"] = {"cells_per_s": lex["cells_per_s"], "ms_per_step": lex["ms_per_step"], "steps": lex_steps}
+        out["orders"] = order_spread(A, steps=100)
+        out["lex"] = {"cells_per_s": out["orders"]["lex"]["cells_per_s"],
+                      "ms_per_step": out["orders"]["lex"]["ms_per_step"], "steps": 100}
+        out["jacobi_vs_lex_rel_l2"] = out["orders"]["jacobi_vs_lex_rel_l2"]
+        out["symmetric_vs_lex_rel_l2"] = out["orders"]["symmetric_vs_lex_rel_l2"]
         out["ice_scheme"] = run_ice(A, "fp64", args.steps, args.warmup, graph=graph)
         out["ice_scheme"]["workload"] = ("FP64 Jacobi step at 41x181x361 with the two-category ice scheme "
                                          "in the moist block and its sources in RK4")

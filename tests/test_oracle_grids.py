@@ -1,0 +1,78 @@
+"""The oracle's update orders, and the oracle on the benchmark grids against the reference (CPU).
+
+1. The plane-parallel lex sweep (the only way the full grid's lex run fits a test) is bit for
+   bit the sequential lex sweep of RungeKutta_Atm_Turb.cpp:99-264 at one thread.
+2. The Jacobi restatement reads only the start-of-RK4 neighbourhood: its result does not depend
+   on the thread count or on the sweep direction. A read of a value written in the same sweep
+   (the lex behaviour) would make both differ; lex and Jacobi differ here, so the checks bite.
+3. tests/golden/ref_grids.npz holds the reference's own lex runs at 21 x 91 x 181 (10 and 100
+   iterations) and at its built-in 61 x 181 x 361 (20 iterations), written by copies of the
+   reference compiled at each grid (oracle/Makefile.grid), plus its thermodynamic init chain
+   at each grid. The oracle must reproduce them to 1e-10 (S_c_c excluded: a diagnostic whose
+   Newton break-off flips at rounding level and never feeds back).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from golden_check import GRIDS, grid_run_errors, rel
+
+C1 = (21, 91, 181)
+TOL = 1e-10
+NAMES = ["t", "u", "v", "w", "c", "cloud", "ice", "gr", "co2", "tke", "dis", "p_dyn", "p_stat", "nue", "aux_u",
+         "aux_v", "aux_w", "prod", "tke_source", "dis_source", "r_humid"]
+
+
+def run(oracle, order, steps=3, threads=0, grid=C1, seed=5):
+    oracle.set_threads(threads)
+    st = oracle.OracleState(*grid, seed=seed)
+    st.run(steps, order)
+    snap = st.snapshot()
+    oracle.set_threads(0)
+    return snap
+
+
+def test_lex_planes_equal_sequential_lex(oracle):
+    a, b = run(oracle, "lex"), run(oracle, "lex_planes")
+    for n in a:
+        assert np.array_equal(a[n], b[n]), n
+
+
+def test_jacobi_independent_of_threads_and_direction(oracle):
+    one = run(oracle, "jacobi", threads=1)
+    many = run(oracle, "jacobi", threads=8)
+    rev = run(oracle, "jacobi_reverse", threads=3)
+    lex = run(oracle, "lex")
+    for n in one:
+        assert np.array_equal(one[n], many[n]), ("threads", n)
+        assert np.array_equal(one[n], rev[n]), ("direction", n)
+    assert not np.array_equal(one["u"], lex["u"])
+
+
+gold = pytest.mark.skipif(not os.path.exists(GRIDS), reason="reference vectors at the benchmark grids not generated")
+
+
+@gold
+@pytest.mark.parametrize("grid", [(21, 91, 181), (41, 181, 361), (61, 181, 361)])
+def test_init_chain_on_grid(oracle, grid):
+    g = np.load(GRIDS, allow_pickle=False)
+    tag = "%dx%dx%d" % grid
+    st = oracle.OracleState(*grid, thermo_only=True)
+    js = g[f"{tag}/init/js"]
+    for name in ["t", "p_stat", "r_humid", "r_dry", "c", "co2"]:
+        assert rel(st.field(name)[:, js, 0], g[f"{tag}/init/{name}"]) < 1e-13, name
+    assert rel(st.field2d("t_top_init")[:, 0], g[f"{tag}/init/t_top_init"]) < 1e-13
+
+
+@gold
+@pytest.mark.parametrize("grid,seed,steps", [((21, 91, 181), 77, 10), ((21, 91, 181), 77, 100),
+                                             ((61, 181, 361), 1234, 20)])
+def test_lex_run_on_grid(oracle, grid, seed, steps):
+    g = np.load(GRIDS, allow_pickle=False)
+    st = oracle.OracleState(*grid, seed=seed)
+    st.run(steps, "lex_planes")
+    e = grid_run_errors(st.field, g, grid, seed, steps, NAMES)
+    print({k: f"{v:.2e}" for k, v in e.items()})
+    bad = {k: v for k, v in e.items() if not v <= TOL}
+    assert not bad, f"{grid} {steps} steps: {bad}"
