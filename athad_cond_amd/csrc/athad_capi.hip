@@ -89,7 +89,7 @@ struct MemberHost {
     double* conv_qs[2] = {};
     int iter_n = 0, total = 0;     // host mirror of the device counters
     // table offsets inside 'tables'
-    size_t o_rad = 0, o_sint = 0, o_cost = 0, o_sinr = 0, o_hgt = 0, o_nuef = 0, o_step = 0;
+    size_t o_rad = 0, o_sint = 0, o_cost = 0, o_sinr = 0, o_hgt = 0, o_nuef = 0, o_step = 0, o_geo = 0;
 };
 
 }  // namespace
@@ -177,6 +177,9 @@ void fill_consts(MemberHost& m, const Config& cfg, int order) {
     K.coeff_energy_p = cfg.u_0 * cfg.u_0 / (cfg.cp_l * cfg.t_0);
     K.diff_prec_re_inv = 1.0 / (cfg.sc_WaterVapour * K.re_turb);
     K.diff_co2_re_inv = 1.0 / (cfg.sc_CO2 * K.re_turb);
+    K.inv_re_turb = 1.0 / K.re_turb;
+    K.inv_pr_turb = 1.0 / K.pr_turb;
+    K.dt_sixth = K.dt / 6.0;
     K.nue_max = 1000.0 / (cfg.u_0 * cfg.L_atm);
     K.nue_air_nd = 1.5e-5 / (cfg.u_0 * cfg.L_atm);
     K.tke_max_nd = 1000.0 / (cfg.u_0 * cfg.u_0);
@@ -253,6 +256,47 @@ void build_tables(MemberHost& m, const Config& cfg, std::vector<double>& tab, st
     // SaturationAdjustment.h:56-60,99
     for (int i = 0; i + 1 < im; i++) tab[m.o_step + i] = (double)(float)((float)hgt[i + 1] - (float)hgt[i]) / 1.6;
     tab[m.o_step + im - 1] = 1.0;
+    // RK4 metric coefficients per (i, j) (GeoCoeff order), with the arithmetic the cell kernel
+    // used to repeat per cell: RHS_Atm_Turb.cpp:86-130 (metric radius, sin theta floored at
+    // 0.55), the Laplacian and advection weights, Coriolis :392-395 with force_nd, the Rayleigh
+    // drag profile :1014-1019 and sin^4 theta of the Held-Suarez rate :976-992.
+    m.o_geo = (tab.size() + 1) & ~(size_t)1;   // even: 16-byte aligned entries
+    tab.resize(m.o_geo + (size_t)im * jm * NGEO, 0.0);
+    for (int i = 0; i < im; i++) {
+        const double rm = rad[i];
+        const double exp_rm = 1.0 / (rm + 1.0);
+        const double rmet = (K.metric_r0 > 0.0) ? (K.metric_r0 + (rm - K.rad0)) : rm;
+        const double inv_rm = 1.0 / rmet;
+        const double inv_rm2 = 1.0 / (rmet * rmet);
+        double dp = 1.0 - (double)i / 5.0;
+        if (dp < 0.0) dp = 0.0;
+        if (dp > 1.0) dp = 1.0;
+        for (int j = 0; j < jm; j++) {
+            const double sinthe = tab[m.o_sint + j], costhe = tab[m.o_cost + j];
+            const double sinthe2 = sinthe * sinthe;
+            const double inv_rms = 1.0 / (rmet * sinthe);
+            const double inv_rm2sinthe = inv_rm2 / sinthe;
+            const double inv_rm2s2 = inv_rm2 / sinthe2;
+            const double v_metric = (1.0 + costhe * costhe / sinthe2) * inv_rm2;
+            const double wr = K.inv_dr2 * (exp_rm * exp_rm), wt = K.inv_dthe2 * inv_rm2, wp = K.inv_dphi2 * inv_rm2s2;
+            const double Lc2 = 2.0 * (wr + wt + wp);
+            double* q = &tab[m.o_geo + ((size_t)i * jm + j) * NGEO];
+            q[G_AR] = exp_rm * 2.0 * K.inv_2dr;
+            q[G_AT] = inv_rm * 2.0 * K.inv_2dthe;
+            q[G_AP] = inv_rms * 2.0 * K.inv_2dphi;
+            q[G_CEXP] = exp_rm; q[G_CIRM] = inv_rm; q[G_CIRS] = inv_rms;
+            q[G_WR] = wr; q[G_WT] = wt; q[G_WP] = wp;
+            q[G_TWO_OVER] = 2.0 * inv_rm * exp_rm;
+            q[G_CRS] = costhe * inv_rm2sinthe;
+            q[G_LC2] = Lc2;
+            q[G_CU] = 2.0 * inv_rm2 - Lc2;
+            q[G_CVW] = -v_metric - Lc2;
+            q[G_CIRM2] = inv_rm2; q[G_CIRM2S] = inv_rm2sinthe; q[G_C2COS] = 2.0 * costhe * inv_rm2s2;
+            q[G_DRAG] = K.drag_coeff * dp;
+            q[G_COR] = K.force_nd * 2.0 * costhe;
+            q[G_S4] = sinthe2 * sinthe2;
+        }
+    }
     // polar_zonal_filter pass counts (lib/Utils.cpp:557-571)
     passes.assign(jm, 0);
     const double sin_ref = std::cos(30.0 * (M_PI / 180.0));
@@ -278,7 +322,7 @@ void make_desc(const athad_ctx* c, const MemberHost& m, int parity, MemberDesc& 
     d.vel_star = m.vel_star; d.t_top = m.t_top; d.t_ref = m.t_ref;
     d.rad = m.tables + m.o_rad; d.sint = m.tables + m.o_sint; d.cost = m.tables + m.o_cost;
     d.sinr = m.tables + m.o_sinr; d.hgt = m.tables + m.o_hgt; d.nue_f = m.tables + m.o_nuef;
-    d.step_dim = m.tables + m.o_step; d.passes = m.passes; d.counters = m.counters; d.scratch = m.scratch;
+    d.step_dim = m.tables + m.o_step; d.geo = m.tables + m.o_geo; d.passes = m.passes; d.counters = m.counters; d.scratch = m.scratch;
     d.radiation = m.radiation; d.epsilon = m.epsilon; d.albedo = m.albedo;
     for (int q = 0; q < NMP; q++) d.mp[q] = m.mp[q];
     for (int q = 0; q < NMC; q++) d.mc[q] = m.mc[q];
@@ -655,7 +699,11 @@ int64_t athad_footprint_bytes(const athad_grid* g, int precision) {
     if (!g) return -1;
     const int64_t n = (int64_t)g->im * g->jm * g->km;
     const int64_t es = (precision == ATHAD_MIXED) ? 4 : 8;
-    return n * (es * (NFIELD - 1 + NPROG + NTMP) + 8) + (int64_t)g->jm * g->km * 16 + scratch_doubles(g->im, g->jm, g->km) * 8;
+    // fields, Jacobi buffers and scratch; p_stat in FP64; vel_star and t_top; reductions; the 1-D
+    // tables and the per-(i, j) metric table
+    const int64_t tables = 4LL * g->im + 4LL * g->jm + 2 + (int64_t)g->im * g->jm * NGEO;
+    return n * (es * (NFIELD - 1 + NPROG + NTMP) + 8) + (int64_t)g->jm * g->km * 16 +
+           scratch_doubles(g->im, g->jm, g->km) * 8 + tables * 8;
 }
 
 athad_status athad_create(int n_members, const athad_grid* grids, const athad_member_params* params,

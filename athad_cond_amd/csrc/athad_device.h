@@ -20,6 +20,13 @@ enum Field {
 };
 constexpr int NPROG = 11;   // t u v w c cloud ice gr co2 tke dis
 constexpr int NTMP = 12;    // scratch fields for the filter passes
+// Per-(i, j) metric coefficients of the RK4 stencils (cAtmosphereModel.h:170-178, 282-284;
+// RHS_Atm_Turb.cpp:86-130, 392-395, 976-992, 1014-1019), formed once on the host: they depend
+// on the radius and the latitude only. Entry (i, j) holds NGEO doubles in this order.
+enum GeoCoeff {
+    G_AR = 0, G_AT, G_AP, G_CEXP, G_CIRM, G_CIRS, G_WR, G_WT, G_WP, G_TWO_OVER, G_CRS, G_LC2,
+    G_CU, G_CVW, G_CIRM2, G_CIRM2S, G_C2COS, G_DRAG, G_COR, G_S4, NGEO
+};
 // Arrays of the two-category ice scheme (same numbering as include/athad_hip.h): the rain and
 // snow fluxes, their stored copies, the capped total and the microphysics sources.
 // Moist-convection arrays (MoistConvection.h), in the reference harness's order.
@@ -47,6 +54,7 @@ struct Consts {
     double force_nd, re_turb, coeff_energy_p, diff_prec_re_inv, diff_co2_re_inv;
     double nue_max, nue_air_nd, tke_max_nd, vel_max_nd, om_wall, sqrt_cnue;
     double hs_k_a, hs_k_s, drag_coeff;
+    double inv_re_turb, inv_pr_turb, dt_sixth;   // 1/re_turb, 1/pr_turb, dt/6 (kernel-wide)
     double evap_r, evap_w_norm;
     // radiation (MultiLayerRadiation, mode 2)
     double sigma, kappa_h2o, kappa_co2, kappa_bg, geothermal, sw_equator, sw_pole;
@@ -78,6 +86,7 @@ struct MemberDesc {
     const double* nue_f;     // im       ABL taper factor of clamp_nue
     const double* step_dim;  // im       SaturationAdjustment dt_dim = float(h[i+1]-h[i])/1.6
     const int* passes;       // jm       polar filter pass count
+    const double* geo;       // im*jm*NGEO  per-(i, j) metric coefficients of the RK4 stencils
     int* counters;           // [0] iter_n, [1] total_iter_count (device resident)
     double* scratch;         // reductions
     double* radiation;       // n        upward long-wave flux (allocated at the first radiation call)

@@ -954,6 +954,40 @@ template <typename C> struct CellGeo {
 template <typename C>
 ATHAD_DEV void cell_geo(const MemberDesc& M, int i, int j, double ps_c, double p_surf, CellGeo<C>& g) {
     const Consts& K = M.K;
+    // the (i, j) entry of the metric table: 16-byte aligned, read as NGEO / 2 pairs
+    typedef double pair __attribute__((ext_vector_type(2)));
+    const pair* q2 = (const pair*)(M.geo + ((long long)i * K.jm + j) * NGEO);
+    double q[NGEO];
+#pragma unroll
+    for (int e = 0; e < NGEO / 2; e++) {
+        const pair v = gld(q2, e);
+        q[2 * e] = v.x;
+        q[2 * e + 1] = v.y;
+    }
+    g.aR = C(q[G_AR]); g.aT = C(q[G_AT]); g.aP = C(q[G_AP]);
+    g.cexp = C(q[G_CEXP]); g.cirm = C(q[G_CIRM]); g.cirs = C(q[G_CIRS]);
+    g.wr = C(q[G_WR]); g.wt = C(q[G_WT]); g.wp = C(q[G_WP]);
+    g.two_over = C(q[G_TWO_OVER]);
+    g.crs = C(q[G_CRS]);
+    g.Lc2 = C(q[G_LC2]);
+    g.cu = C(q[G_CU]);
+    g.cvw = C(q[G_CVW]);
+    g.cirm2 = C(q[G_CIRM2]); g.cirm2s = C(q[G_CIRM2S]); g.c2cos = C(q[G_C2COS]);
+    const double sigma = (p_surf > 0.0) ? ps_c / p_surf : 0.0;
+    double bl = (sigma - 0.7) / (1.0 - 0.7);
+    if (bl < 0.0) bl = 0.0;
+    const double kT = K.hs_k_a + (K.hs_k_s - K.hs_k_a) * bl * q[G_S4];
+    g.hs_kT = C(kT * K.L_atm / K.u_0);
+    g.drag = C(q[G_DRAG]);
+    g.cor = C(q[G_COR]);
+}
+
+// The same coefficients formed in place from the 1-D tables (ten divisions): fewer live
+// registers than the table's twenty values, which the FP64 Jacobi kernel, at three waves per
+// SIMD on the edge of spilling, needs more than the saved arithmetic.
+template <typename C>
+ATHAD_DEV void cell_geo_alu(const MemberDesc& M, int i, int j, double ps_c, double p_surf, CellGeo<C>& g) {
+    const Consts& K = M.K;
     const double rm = M.rad[i];
     const double exp_rm = 1.0 / (rm + 1.0);
     const double sinthe = M.sint[j], costhe = M.cost[j];
@@ -1184,9 +1218,9 @@ ATHAD_DEV void rk4_dyn(const MemberDesc& M, const Nb& src_nb, int i, const CellG
     {
         C nv = src_nb.field(F_NUE);
         if (CLAMP) nv = finite_bits(nv) ? tclamp(nv, C(0), C(K.nue_max)) : C(0);
-        const C inv_re = C(1.0 / K.re_turb);
+        const C inv_re = C(K.inv_re_turb);
         dif_vel = inv_re + nv;
-        dif_t = (inv_re + nv) * C(1.0 / K.pr_turb);
+        dif_t = (inv_re + nv) * C(K.inv_pr_turb);
     }
     const C t_eq = src_nb.field(F_T_EQ);
     const C t_ref = C(M.t_ref[i]);
@@ -1201,7 +1235,7 @@ ATHAD_DEV void rk4_dyn(const MemberDesc& M, const Nb& src_nb, int i, const CellG
         const C cen_ = C(K.coeff_energy);
         srcT = cen_ * (Sc + Sr) * C(K.lv) * rh + cen_ * (Si + Ss + C(0)) * C(K.ls) * rh;
     }
-    const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
+    const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt_sixth);
 for_stages<U>([&](const int st) {
         const C T = cen[FT], u = cen[FU], v = cen[FV], w = cen[FW];
         rec[(3 * st + 0) * LB] = u; rec[(3 * st + 1) * LB] = v; rec[(3 * st + 2) * LB] = w;
@@ -1251,7 +1285,7 @@ ATHAD_DEV void rk4_moist(const MemberDesc& M, const Nb& src_nb, const CellGeo<C>
         src[1] = C(1) * src_nb.mp(MP_S_C) * rh;
         src[2] = C(1) * src_nb.mp(MP_S_I) * rh;
     }
-    const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
+    const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt_sixth);
 for_stages<U>([&](const int st) {
         C u, v, w;
         vel(st, u, v, w);
@@ -1310,8 +1344,8 @@ ATHAD_DEV void rk4_turb(const MemberDesc& M, const Nb& src_nb, int i, int j, int
     }
     const C sig_k1 = C(1.176), sig_k2 = C(1.0), sig_w1 = C(2.0), sig_w2 = C(1.168), bet_star = C(0.09);
     const C bet1 = C(0.0333), bet2 = C(0.0368), gam1 = C(0.413), gam2 = C(0.2);
-    const C inv_re = C(1.0 / K.re_turb);
-    const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt / 6.0);
+    const C inv_re = C(K.inv_re_turb);
+    const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt_sixth);
 for_stages<U>([&](const int st) {
         C u, v, w;
         vel(st, u, v, w);
@@ -1369,11 +1403,12 @@ ATHAD_DEV void group_start(const Nb& src_nb, int f0, int nf, C* cen, C* ba) {
 
 // RK4 of one interior cell, group by group: dynamics, moisture, turbulence. out(f, value)
 // writes a final value; the aux / prod / source outputs are written here.
-template <typename S, typename C, bool CLAMP, bool SRC, bool U, typename Nb, typename Out>
+template <typename S, typename C, bool CLAMP, bool SRC, bool U, bool GT, typename Nb, typename Out>
 ATHAD_DEV void rk4_cell(const MemberDesc& M, const Nb& nb, long long id, int i, int j, int k, C* ba, C* rec,
                         Out out) {
     CellGeo<C> g;
-    cell_geo<C>(M, i, j, nb.ps(), nb.psurf(), g);
+    if constexpr (GT) cell_geo<C>(M, i, j, nb.ps(), nb.psurf(), g);
+    else cell_geo_alu<C>(M, i, j, nb.ps(), nb.psurf(), g);
     C A, B;
     {
         C cen[4], aux[3];
@@ -1451,7 +1486,9 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
         return;
     }
     const NbLoad<S, C, FUSED> nb{M, X, id, (long long)K.jm * K.km, (long long)K.km, (long long)ix3(K, 0, j, k)};
-    rk4_cell<S, C, FUSED, SRC, false>(M, nb, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x, [&](int f, C v) {
+    // the metric table in the FP32 kernel (4 waves per SIMD, registers to spare), formed in
+    // place in the FP64 kernel
+    rk4_cell<S, C, FUSED, SRC, false, sizeof(S) == 4>(M, nb, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x, [&](int f, C v) {
         gst((S*)M.alt[f], id, S(v));
         if (FUSED && f != FU && f != FV && f != FW) gst(fp<S>(M, kProgN[f]), id, S(v));
     });
@@ -1478,7 +1515,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     for (int f = 0; f < NPROG; f++) X[f] = fp<S>(M, kProgF[f]);
     const NbRegs<S, C, SRC> nb(M, (const S* const*)X, id, (long long)K.jm * K.km, (long long)K.km,
                                (long long)ix3(K, 0, j, k));
-    rk4_cell<S, C, false, SRC, true>(M, nb, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
+    rk4_cell<S, C, false, SRC, true, true>(M, nb, id, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
                           [&](int f, C v) { gst(X[f], id, S(v)); });
 }
 
@@ -2104,7 +2141,7 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
             if ((qv_old > qs0 && alpha > 0.01) || (qv_old < qs0 && (qc_old > 1e-12 || qi_old > 1e-12))) {
                 double qvb = qv_old, qcb = qc_old, qib = qi_old;
                 const double T_orig = t * t_0;
-                sat_newton<S>(T, qvb, qcb, qib, co2, p_local, M_other, Mb, t_00, t_0);
+                sat_newton<double>(T, qvb, qcb, qib, co2, p_local, M_other, Mb, t_00, t_0);
                 qcb = tmax(0.0, qcb);
                 qib = tmax(0.0, qib);
                 if (T > kTcrit) T = kTcrit;
@@ -2402,7 +2439,7 @@ __global__ void k_copy_job(const MemberDesc* __restrict__ md, DampJob job) {
 // ------------------------------------------------------------------------------------
 template <typename S>
 __global__ __launch_bounds__(128) void k_densities(const MemberDesc* __restrict__ md, int set_water, int evaporate, int gate) {
-    using A = S;
+    using A = double;   // the hydrostatic column in FP64 in both storage precisions
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const Consts& K = M.K;

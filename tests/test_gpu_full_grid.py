@@ -29,6 +29,13 @@ STEPS = 100
 TOL = 1e-10
 PARITY_FIELDS = A.PROGNOSTIC + ["p_dyn", "p_stat", "nue", "aux_u", "aux_v", "aux_w"]
 GOLD_FIELDS = PARITY_FIELDS + ["prod", "tke_source", "dis_source", "r_humid"]
+# aux_u = rhs_u + dp/dr at the fourth stage (RHS_Atm_Turb.cpp:1077): the sum adds back the
+# pressure gradient the tendency subtracted, so it carries the rounding of that gradient
+# amplified by |dp/dr| / |aux_u|. The device rounds rhs_u differently from the host (fused
+# multiply-adds, device math), so its aux_u differs from the reference by more than the state
+# does (measured 2.2e-10 after 100 iterations, with u at 8e-13 and p_dyn at 3e-11; the oracle,
+# which keeps the reference's evaluation order, is at 2e-14). Bar for aux_u: 1e-9.
+BAR = {"aux_u": 1e-9}
 
 
 def gpu_run(order, precision="fp64"):
@@ -84,8 +91,8 @@ def test_lex_full_grid_matches_reference(gpu_lex, oracle_lex):
     for tag, get in (("gpu", gpu_lex.__getitem__), ("oracle", oracle_lex.field)):
         e = grid_run_errors(get, gold, C2, SEED, STEPS, GOLD_FIELDS)
         print(tag, "vs reference", {k: f"{v:.2e}" for k, v in e.items()})
-        bad = {k: v for k, v in e.items() if not v <= TOL}
-        assert not bad, f"{tag}: fields above {TOL} against the reference: {bad}"
+        bad = {k: v for k, v in e.items() if not v <= BAR.get(k, TOL)}
+        assert not bad, f"{tag}: fields above their bar against the reference: {bad}"
 
 
 def test_jacobi_full_grid_matches_oracle(gpu_jacobi, oracle_jacobi):
@@ -108,15 +115,24 @@ def test_order_spread_against_reference_order(gpu_lex, gpu_jacobi):
 
 
 def test_mixed_precision_error_full_grid(oracle_jacobi):
-    """The mixed form after 100 iterations at the full grid against the FP64 oracle.
-    Velocities, temperature, pressures, vapour, CO2 and the turbulence fields stay within
-    1e-5 relative L2; cloud water and ice are reported (their threshold-driven creation in
-    the saturation adjustment sees FP32 inputs)."""
+    """The mixed form after 100 iterations at the full grid against the FP64 oracle: every
+    prognostic field, p_dyn, p_stat and nue within 1e-5 relative L2, except cloud water and
+    ice, whose errors are bounded by 1e-5 of the total water instead. Cloud water forms as the
+    excess of vapour over q_sat(T) in barely supersaturated cells: a difference of two values
+    near 0.33 stored in FP32 (relative 6e-8), so a storage rounding of T or c moves cloud by
+    about 6e-8 x 0.33 / 8e-5 ~ 2.5e-4 of itself (measured 5.4e-4 relative, 2.2e-7 of the
+    total water). Ice is a trace below 3e-13 kg/kg made the same way (measured 1.7e-3 of
+    itself, 3e-16 of the total water)."""
     print("case full grid 41x181x361 mixed jacobi n=100", flush=True)
     f = gpu_run("jacobi", precision="mixed")
-    core = ["t", "u", "v", "w", "c", "co2", "tke", "dis", "p_dyn", "p_stat"]
-    w = worst(f, oracle_jacobi, core)
-    extra = worst(f, oracle_jacobi, ["cloud", "ice", "gr", "nue"])
+    fields = ["t", "u", "v", "w", "c", "gr", "co2", "tke", "dis", "p_dyn", "p_stat", "nue"]
+    w = worst(f, oracle_jacobi, fields)
+    tw = oracle_jacobi.field("c") + oracle_jacobi.field("cloud") + oracle_jacobi.field("ice")
+    tw_l2 = float(np.sqrt(np.sum(tw * tw)))
+    for n in ("cloud", "ice"):
+        d = f[n] - oracle_jacobi.field(n)
+        w[n + "_of_total_water"] = float(np.sqrt(np.sum(d * d))) / tw_l2
+    extra = worst(f, oracle_jacobi, ["cloud", "ice"])
     print({k: f"{v:.2e}" for k, v in {**w, **extra}.items()})
     bad = {k: v for k, v in w.items() if not v <= 1e-5}
     assert not bad, f"mixed fields above 1e-5: {bad}"

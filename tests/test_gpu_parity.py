@@ -116,15 +116,26 @@ PARITY_FIELDS = A.PROGNOSTIC + ["p_dyn", "p_stat", "nue", "aux_u", "aux_v", "aux
 
 def test_lex_100_steps(oracle):
     """The parity form (the reference's single-thread sweep) over 100 iterations at
-    21 x 91 x 181."""
+    21 x 91 x 181: the device against the oracle, and both against the reference's own
+    100-iteration run at this grid (tests/golden/ref_grids.npz)."""
+    import os
+    from golden_check import GRIDS, grid_run_errors
     print("case lex 21x91x181 n=100", flush=True)
     ref = oracle.OracleState(*C1, seed=77)
-    ref.run(100, "lex")
+    ref.run(100, "lex_planes")
     with A.Integration(C1, order="lex") as integ:
         integ.init_state(77)
         integ.run(100)
         w = compare(integ, ref, PARITY_FIELDS, TOL, "C1 lex 100")
         print({k: f"{v:.2e}" for k, v in w.items()})
+        if os.path.exists(GRIDS):
+            gold = np.load(GRIDS, allow_pickle=False)
+            names = PARITY_FIELDS + ["prod", "tke_source", "dis_source", "r_humid"]
+            for tag, get in (("gpu", integ.get), ("oracle", ref.field)):
+                e = grid_run_errors(get, gold, C1, 77, 100, names)
+                print(tag, "vs reference", {k: f"{v:.2e}" for k, v in e.items()})
+                bad = {k: v for k, v in e.items() if not v <= (1e-9 if k == "aux_u" else TOL)}
+                assert not bad, f"{tag} vs reference: {bad}"
 
 
 def test_damp_tiles_give_identical_results(oracle):

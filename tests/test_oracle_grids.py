@@ -6,10 +6,12 @@
    on the thread count or on the sweep direction. A read of a value written in the same sweep
    (the lex behaviour) would make both differ; lex and Jacobi differ here, so the checks bite.
 3. tests/golden/ref_grids.npz holds the reference's own lex runs at 21 x 91 x 181 (10 and 100
-   iterations) and at its built-in 61 x 181 x 361 (20 iterations), written by copies of the
+   iterations), 41 x 181 x 361 (100) and its built-in 61 x 181 x 361 (20 iterations), written by copies of the
    reference compiled at each grid (oracle/Makefile.grid), plus its thermodynamic init chain
    at each grid. The oracle must reproduce them to 1e-10 (S_c_c excluded: a diagnostic whose
-   Newton break-off flips at rounding level and never feeds back).
+   Newton break-off flips at rounding level and never feeds back). The 100-iteration runs are
+   checked where their oracle runs already happen, beside the device (tests/test_gpu_parity.py
+   at 21 x 91 x 181, tests/test_gpu_full_grid.py at 41 x 181 x 361).
 """
 import os
 
@@ -66,8 +68,8 @@ def test_init_chain_on_grid(oracle, grid):
 
 
 @gold
-@pytest.mark.parametrize("grid,seed,steps", [((21, 91, 181), 77, 10), ((21, 91, 181), 77, 100),
-                                             ((61, 181, 361), 1234, 20)])
+@pytest.mark.parametrize("grid,seed,steps", [((21, 91, 181), 77, 10), ((61, 181, 361), 1234, 20)],
+                         ids=["21x91x181-10", "61x181x361-20"])
 def test_lex_run_on_grid(oracle, grid, seed, steps):
     g = np.load(GRIDS, allow_pickle=False)
     st = oracle.OracleState(*grid, seed=seed)
