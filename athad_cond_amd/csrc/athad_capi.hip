@@ -130,6 +130,9 @@ struct athad_ctx {
     // radiation (mode 2) and the skin-temperature step after every iteration's store; the
     // device descriptors then hold a newer t_skin than the host mirror (tskin_dev)
     bool rad_on = false, tskin_dev = false;
+    // the on-demand arrays of the ice scheme (mp) and of moist convection (mc) are complete on
+    // every member and published in the device descriptors
+    bool mp_ready = false, mc_ready = false;
 };
 
 namespace {
@@ -1058,22 +1061,38 @@ static void mark_mp_dirty(athad_ctx* c) {
 // member at once; a 4-iteration graph captured before then did not store the fluxes, so it is
 // dropped and recaptured.
 
+// Publishes the members' pointers in the device descriptors after on-demand arrays were freed
+// again: kernels test those pointers, so none may keep a freed one.
+static void republish(athad_ctx* c) {
+    const std::string e = c->err;
+    upload_descs(c);
+    c->err = e;
+}
+
 static athad_status ensure_mp_buffers(athad_ctx* c) {
-    if (c->mem[0].mp[0]) return ATHAD_OK;
+    if (c->mp_ready) return ATHAD_OK;
     if (c->capturing) { c->err = "the ice-scheme arrays cannot be allocated during capture"; return ATHAD_ERR_STATE; }
+    // every failure frees every member's arrays, so a non-null mp[0] always means complete
+    auto fail = [&](athad_status st, const char* msg) {
+        for (auto& m2 : c->mem)
+            for (auto& p : m2.mp) if (p) { hipFree(p); p = nullptr; }
+        c->err = msg;
+        republish(c);
+        return st;
+    };
     for (auto& m : c->mem)
         for (int q = 0; q < NMP; q++) {
             if (hipMalloc(&m.mp[q], m.n * elem_size(c)) != hipSuccess) {
                 m.mp[q] = nullptr;
-                for (auto& m2 : c->mem)
-                    for (auto& p : m2.mp) if (p) { hipFree(p); p = nullptr; }
-                c->err = "out of device memory for the ice-scheme arrays";
-                return ATHAD_ERR_ALLOC;
+                return fail(ATHAD_ERR_ALLOC, "out of device memory for the ice-scheme arrays");
             }
-            HIPCHK(c, hipMemset(m.mp[q], 0, m.n * elem_size(c)));
+            if (hipMemset(m.mp[q], 0, m.n * elem_size(c)) != hipSuccess)
+                return fail(ATHAD_ERR_DEVICE, "clearing the ice-scheme arrays failed");
         }
     if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
-    return upload_descs(c);
+    if (upload_descs(c) != ATHAD_OK) return fail(ATHAD_ERR_DEVICE, "publishing the ice-scheme arrays failed");
+    c->mp_ready = true;
+    return ATHAD_OK;
 }
 
 athad_status athad_ice_scheme(athad_ctx* c) {
@@ -1143,45 +1162,57 @@ athad_status athad_get_mp_field(athad_ctx* ctx, int member, int which, double* d
 // The 32 convection arrays, the evaporation input and the latitude factor table are allocated,
 // zero, at the first call that needs them, for every member at once.
 static athad_status ensure_mc_buffers(athad_ctx* c) {
-    if (c->mem[0].mc[0]) return ATHAD_OK;
+    if (c->mc_ready) return ATHAD_OK;
     if (c->capturing) { c->err = "the convection arrays cannot be allocated during capture"; return ATHAD_ERR_STATE; }
-    auto fail = [&]() {
+    // every failure frees every member's arrays, so a non-null mc[0] always means complete
+    auto fail = [&](athad_status st, const char* msg) {
         for (auto& m2 : c->mem) {
             for (auto& p : m2.mc) if (p) { hipFree(p); p = nullptr; }
             if (m2.evap) { hipFree(m2.evap); m2.evap = nullptr; }
             if (m2.conv_cos) { hipFree(m2.conv_cos); m2.conv_cos = nullptr; }
             for (auto& p : m2.conv_qs) if (p) { hipFree(p); p = nullptr; }
         }
-        c->err = "out of device memory for the convection arrays";
-        return ATHAD_ERR_ALLOC;
+        c->err = msg;
+        republish(c);
+        return st;
     };
+    const char* oom = "out of device memory for the convection arrays";
     for (auto& m : c->mem) {
         const int jm = m.g.jm, km = m.g.km;
         for (int q = 0; q < NMC; q++) {
-            if (hipMalloc(&m.mc[q], m.n * 8) != hipSuccess) { m.mc[q] = nullptr; return fail(); }
-            HIPCHK(c, hipMemset(m.mc[q], 0, m.n * 8));
+            if (hipMalloc(&m.mc[q], m.n * 8) != hipSuccess) { m.mc[q] = nullptr; return fail(ATHAD_ERR_ALLOC, oom); }
+            if (hipMemset(m.mc[q], 0, m.n * 8) != hipSuccess)
+                return fail(ATHAD_ERR_DEVICE, "clearing the convection arrays failed");
         }
-        if (hipMalloc(&m.evap, (size_t)jm * km * 8) != hipSuccess) { m.evap = nullptr; return fail(); }
-        HIPCHK(c, hipMemset(m.evap, 0, (size_t)jm * km * 8));
-        if (hipMalloc(&m.conv_cos, (size_t)jm * 8) != hipSuccess) { m.conv_cos = nullptr; return fail(); }
+        if (hipMalloc(&m.evap, (size_t)jm * km * 8) != hipSuccess) { m.evap = nullptr; return fail(ATHAD_ERR_ALLOC, oom); }
+        if (hipMemset(m.evap, 0, (size_t)jm * km * 8) != hipSuccess)
+            return fail(ATHAD_ERR_DEVICE, "clearing the evaporation input failed");
+        if (hipMalloc(&m.conv_cos, (size_t)jm * 8) != hipSuccess) { m.conv_cos = nullptr; return fail(ATHAD_ERR_ALLOC, oom); }
         for (auto& p : m.conv_qs)
-            if (hipMalloc(&p, m.n * 8) != hipSuccess) { p = nullptr; return fail(); }
-        // theta by accumulation as the grid builds it; the circle factor is taken about row 90
-        // (MoistConvection.h:239), so the scheme needs jm > 90
+            if (hipMalloc(&p, m.n * 8) != hipSuccess) { p = nullptr; return fail(ATHAD_ERR_ALLOC, oom); }
+        // theta by accumulation as the grid builds it; the reference takes the circle factor
+        // about row 90 (MoistConvection.h:239), which is the equator on its 181-row grid. Here
+        // it is taken about the equator row (jm - 1) / 2 of every grid: the same table at
+        // jm = 181, and on coarser grids a factor that stays positive instead of crossing zero
+        // at row 45 and turning negative beyond it.
         std::vector<double> the(jm), cs(jm);
         the[0] = 0.0;
         for (int l = 1; l < jm; l++) the[l] = the[l - 1] + m.K.dthe;
-        for (int j = 0; j < jm; j++) cs[j] = std::cos(the[90] - the[j]);
-        HIPCHK(c, hipMemcpy(m.conv_cos, cs.data(), (size_t)jm * 8, hipMemcpyHostToDevice));
+        const int jeq = (jm - 1) / 2;
+        for (int j = 0; j < jm; j++) cs[j] = std::cos(the[jeq] - the[j]);
+        if (hipMemcpy(m.conv_cos, cs.data(), (size_t)jm * 8, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(ATHAD_ERR_DEVICE, "uploading the convection circle factors failed");
     }
     if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
-    return upload_descs(c);
+    if (upload_descs(c) != ATHAD_OK) return fail(ATHAD_ERR_DEVICE, "publishing the convection arrays failed");
+    c->mc_ready = true;
+    return ATHAD_OK;
 }
 
 static athad_status check_conv_grid(athad_ctx* c) {
     for (auto& m : c->mem)
-        if (m.g.jm <= 90 || m.g.km < 3 || m.g.im < 3) {
-            c->err = "moist convection needs jm > 90 (its circle factor is taken about row 90), km >= 3, im >= 3";
+        if (m.g.jm < 3 || m.g.km < 3 || m.g.im < 3) {
+            c->err = "moist convection needs jm >= 3, km >= 3, im >= 3";
             return ATHAD_ERR_ARG;
         }
     return ATHAD_OK;
@@ -1409,12 +1440,15 @@ athad_status athad_save_state_mem(athad_ctx* ctx, int member, void* buf, int64_t
     const int32_t hdr[5] = {kRestartMagic, m.g.im, m.g.jm, m.g.km, total};
     unsigned char* out = (unsigned char*)buf;
     std::memcpy(out, hdr, sizeof(hdr));
-    double* arr = (double*)(out + sizeof(hdr));
-    for (int a = 0; a < kRestartArrays; a++, arr += m.n) {
+    // the arrays start 20 bytes in, so they are not 8-byte aligned: each goes through an
+    // aligned staging array and a byte copy
+    std::vector<double> arr(m.n);
+    for (int a = 0; a < kRestartArrays; a++) {
         // the precipitation arrays read zero where the ice scheme has never run
-        if (kRestartF[a] < 0) s = athad_get_mp_field(ctx, member, -1 - kRestartF[a], arr);
-        else s = athad_get_field(ctx, member, kRestartF[a], arr);
+        if (kRestartF[a] < 0) s = athad_get_mp_field(ctx, member, -1 - kRestartF[a], arr.data());
+        else s = athad_get_field(ctx, member, kRestartF[a], arr.data());
         if (s) return s;
+        std::memcpy(out + sizeof(hdr) + (size_t)a * m.n * 8, arr.data(), m.n * 8);
     }
     return ATHAD_OK;
 }
@@ -1432,14 +1466,17 @@ athad_status athad_load_state_mem(athad_ctx* ctx, int member, const void* buf, i
     }
     if (size < restart_bytes(m.g)) { ctx->err = "restart: truncated"; return ATHAD_ERR_ARG; }
     join_s2(ctx);
-    const double* arr = (const double*)((const unsigned char*)buf + sizeof(hdr));
-    for (int a = 0; a < kRestartArrays; a++, arr += m.n) {
+    // the arrays start 20 bytes in (not 8-byte aligned): byte-copied into an aligned array
+    const unsigned char* in = (const unsigned char*)buf + sizeof(hdr);
+    std::vector<double> arr(m.n);
+    for (int a = 0; a < kRestartArrays; a++) {
+        std::memcpy(arr.data(), in + (size_t)a * m.n * 8, m.n * 8);
         if (kRestartF[a] < 0) {
             // precipitation arrays: held once the ice scheme has run; a file holding zeros
             // (as the scoped step leaves them) allocates nothing
-            if (!m.mp[0] && std::all_of(arr, arr + m.n, [](double v) { return v == 0.0; })) continue;
-            s = athad_set_mp_field(ctx, member, -1 - kRestartF[a], arr);
-        } else s = athad_set_field(ctx, member, kRestartF[a], arr);
+            if (!m.mp[0] && std::all_of(arr.begin(), arr.end(), [](double v) { return v == 0.0; })) continue;
+            s = athad_set_mp_field(ctx, member, -1 - kRestartF[a], arr.data());
+        } else s = athad_set_field(ctx, member, kRestartF[a], arr.data());
         if (s) return s;
     }
     return athad_set_iter(ctx, member, m.iter_n, hdr[4]);

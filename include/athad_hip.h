@@ -97,8 +97,11 @@ double athad_dew_point(double e_hPa);
  * ATHAD_ERR_ARG if the sea would boil. */
 athad_status athad_member_from_surface(double p_surf_bar, double t_sea, double x_h2o_dry, double x_co2_dry,
                                        athad_member_params* out);
-/* Device bytes held by a context of this shape (admission control); the ice-scheme arrays
- * (10 fields) and the radiation outputs (2 fields) come on top once first used. */
+/* Device bytes held by a context of this shape (admission control): the fields, the Jacobi
+ * buffers, scratch, the tables and reductions. Not counted, because they are allocated only
+ * once their operator is first used: the ice-scheme arrays (10 fields in the storage
+ * precision), the radiation outputs (2 FP64 fields and jm*km) and the moist-convection arrays
+ * (34 FP64 fields, jm*km and jm doubles). */
 int64_t athad_footprint_bytes(const athad_grid* grid, int precision);
 
 /* --- state transfer ---------------------------------------------------------------- */
@@ -312,9 +315,10 @@ typedef enum athad_conv_field {
  * precipitation P_conv, and the tendencies MC_t, MC_q, MC_v, MC_w of rhsForcing. The arrays the
  * reference does not clear between calls (E_u, D_u, E_d, D_d, the updraft / downdraft winds,
  * the MC_* tendencies, the level markers) carry over as there. computeCAPE's 1-D diagnostic is
- * not produced. Arrays are allocated, zero, at the first call (32 * im*jm*km * 8 bytes per
- * member). Needs jm > 90 (the circle factor is taken about row 90, MoistConvection.h:239):
- * ATHAD_ERR_ARG otherwise. Not part of athad_step: a standalone operator. */
+ * not produced. Arrays are allocated, zero, at the first call (the 32 arrays and two per-cell
+ * saturation fractions: 34 * im*jm*km * 8 bytes per member, plus jm*km and jm doubles). The zonal circle factor cos(theta_eq - theta_j) of the entrainment gradients is taken
+ * about the equator row (jm - 1) / 2 (MoistConvection.h:239 takes row 90, the equator of its
+ * 181-row grid). Grids need jm, km, im >= 3 (ATHAD_ERR_ARG otherwise). */
 athad_status athad_moist_convection(athad_ctx* ctx);
 /* Contiguous FP64 host arrays [im][jm][km] of one convection array, or [jm][km] for
  * ATHAD_CONV_EVAPORATION. */

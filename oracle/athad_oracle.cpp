@@ -1921,7 +1921,9 @@ void moist_convection(S& s) {
             const double p_sfc = PS[id(0)];
             // entrainment gradients of c (:238-297, no land): centred differences
             auto grad = [&](int i, double& dr, double& dth, double& dph, double& cc) {
-                const double costhe = std::cos(s.the[90] - s.the[j]);
+                // about the equator row (jm - 1) / 2: row 90 of the reference's 181-row grid
+                // (MoistConvection.h:239), kept at the equator on every grid
+                const double costhe = std::cos(s.the[(jm - 1) / 2] - s.the[j]);
                 const double rth = (P.r_Earth * 1.0e3 + h[i]) * costhe;
                 const double Ue = rth * 2.0 * M_PI;
                 const int ip = std::min(i + 1, im - 1), imn = std::max(i - 1, 0);

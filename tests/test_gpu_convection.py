@@ -97,7 +97,21 @@ def test_mixed_precision_fields(oracle):
         assert not bad, bad
 
 
-def test_small_latitude_count_is_refused():
-    with A.Integration((21, 37, 73)) as integ:
-        with pytest.raises(Exception):
+def test_coarse_latitude_grid_matches_oracle(oracle):
+    """At 37 latitudes the zonal circle factor is taken about the equator row 18 (the
+    reference takes row 90, the equator of its 181 rows): two calls against the oracle, with the
+    entrainment rate bounded."""
+    grid = (21, 37, 73)
+    print(f"case moist convection fp64 {grid}", flush=True)
+    ref, snap, evap = planted(oracle, grid, 23)
+    with A.Integration(grid) as integ:
+        integ.load(snap)
+        integ.set_conv("Evaporation", evap)
+        for call in (1, 2):
             integ.moist_convection()
+            ref.moist_convection()
+            e = errors(integ, ref)
+            bad = {k: v for k, v in e.items() if v > 1e-11}
+            assert not bad, (call, bad)
+        E = integ.get_conv("E_u")
+        assert np.all(np.isfinite(E)) and np.abs(E).max() < 0.05

@@ -43,3 +43,29 @@ def test_admission_respects_budget():
     assert stats["peak_hbm_bytes"] <= 2 * one + 1
     assert all(r.finished > 0 for r in reqs)
     assert stats["p99_latency_s"] >= stats["p50_latency_s"] > 0
+
+
+def test_c5_grids_alone_and_in_mix():
+    """The benchmark's serving grids, the largest two included (41 x 181 x 361 and
+    61 x 361 x 721, a few iterations each): requests admitted together overlap on the device
+    and each result equals the same request served alone, bit for bit."""
+    print("case serving C5 grids alone vs mix", flush=True)
+    grids = [(41, 181, 361), (61, 361, 721), (21, 91, 181)]
+    reqs = S.poisson_requests(6, rate_per_s=200.0, seed=17, steps=4, grids=grids)
+    got = {tuple(r.grid) for r in reqs}
+    assert (61, 361, 721) in got and (41, 181, 361) in got, got
+    server = S.Server(max_slots=3, chunk=2)
+    stats = server.run(reqs)
+    server.close()
+    print(stats, flush=True)
+    assert stats["peak_active"] > 1
+    for r in reqs:
+        if tuple(r.grid) == (21, 91, 181):
+            continue
+        alone = S.Request(r.rid, 0.0, r.grid, r.state, r.steps, "mixed", r.seed)
+        one = S.Server(max_slots=1)
+        one.run([alone])
+        one.close()
+        for n in alone.result:
+            if n != "fields":
+                assert alone.result[n] == r.result[n], (r.rid, r.grid, n)

@@ -78,3 +78,25 @@ def test_lex_run_on_grid(oracle, grid, seed, steps):
     print({k: f"{v:.2e}" for k, v in e.items()})
     bad = {k: v for k, v in e.items() if not v <= TOL}
     assert not bad, f"{grid} {steps} steps: {bad}"
+
+
+@pytest.mark.parametrize("jm", [91, 181])
+def test_convection_circle_factor_on_coarse_grids(oracle, jm):
+    """Moist convection's zonal circle factor is taken about the equator row of every grid
+    (row 90 of the reference's 181 rows, MoistConvection.h:239). Taken about row 90 on a
+    91-row grid it would be the pole: the factor would turn negative beyond row 45 and the
+    entrainment gradients would blow up where it crosses zero. On the same kind of input the
+    entrainment rate stays bounded and of the same size on both grids."""
+    grid = (21, jm, 73)
+    ref = oracle.OracleState(*grid, seed=21)
+    ref.run(2, "jacobi")
+    snap = ref.snapshot()
+    oracle.plant_microphysics(snap, {k: np.zeros(grid) for k in oracle.MP_FIELDS}, 22)
+    ref.load(snap)
+    ref.evap()[...] = np.random.default_rng(21).uniform(-1.0, 6.0, grid[1:])
+    ref.moist_convection()
+    ref.moist_convection()
+    E = ref.mc("E_u")
+    assert np.all(np.isfinite(E))
+    assert 1e-4 < np.abs(E).max() < 0.05, np.abs(E).max()
+    assert np.all(np.abs(ref.mc("MC_t")) <= 0.01 + 1e-15)
