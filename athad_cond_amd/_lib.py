@@ -44,6 +44,7 @@ EXPORTS = [
     "athad_meridional_streamfunction", "athad_get_slice", "athad_radiation", "athad_get_radiation",
     "athad_set_radiation", "athad_moist_convection", "athad_set_conv_field", "athad_get_conv_field",
     "athad_ice_scheme", "athad_set_ice_scheme", "athad_set_mp_field", "athad_get_mp_field",
+    "athad_field_minmax_noalloc",
 ]
 
 # Arrays of the two-category ice scheme (athad_mp_field in include/athad_hip.h)
@@ -143,6 +144,7 @@ def lib():
         L.athad_get_mp_field.argtypes = [vp, i, i, dp]
         L.athad_residuum.argtypes = [vp, i, dp, ip, dp, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64), ip]
+        L.athad_field_minmax_noalloc.argtypes = [vp, i, i, i, i, vp, dp, ctypes.POINTER(i), dp, ctypes.POINTER(i)]
         for name in EXPORTS:
             if name not in ("athad_last_error", "athad_footprint_bytes", "athad_version", "athad_dew_point",
                             "athad_restart_bytes"):

@@ -1681,3 +1681,93 @@ athad_status athad_profile_result(athad_ctx* c, int kid, double* total_ms, int* 
 }
 
 }  // extern "C"
+
+// ---- min/max of a caller's field without allocation ------------------------------------
+// No counterpart in the reference. For callers that cannot allocate device memory (code under
+// hipGraph capture, tight serving loops): the reductions' scratch, and the one descriptor the
+// shared kernels read, live in a fixed file-scope device array. Its device address is looked up
+// once per device with hipGetSymbolAddress and used wherever host code needs the array.
+namespace {
+constexpr int kNoAllocMaxIm = 200;              // radial levels: bounds the reduction layout
+constexpr size_t kNoAllocDesc = 4096;           // room for one MemberDesc
+constexpr size_t kNoAllocDoubles = 16384;       // reduction partials and results
+constexpr int kNoAllocMaxDevices = 64;
+static_assert(sizeof(MemberDesc) <= kNoAllocDesc, "descriptor slot too small");
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocDesc + kNoAllocDoubles * sizeof(double)];
+
+extern "C" athad_status athad_field_minmax_noalloc(const void* field, int precision, int im, int jm, int km,
+                                                   void* stream, double* vmin, int* pos_min, double* vmax,
+                                                   int* pos_max) {
+    if (!field || im < 1 || jm < 1 || km < 1 || (precision != ATHAD_FP64 && precision != ATHAD_MIXED)) {
+        g_create_error = "field_minmax_noalloc: bad arguments";
+        return ATHAD_ERR_ARG;
+    }
+    if (im > kNoAllocMaxIm || (size_t)(diag_offset(im) + 4 * 256 + 4) > kNoAllocDoubles) {
+        g_create_error = "field_minmax_noalloc: more than 200 radial levels exceeds its fixed scratch";
+        return ATHAD_ERR_SHAPE;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices) {
+        g_create_error = "field_minmax_noalloc: no current device";
+        return ATHAD_ERR_DEVICE;
+    }
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (!g_noalloc_ptr[dev]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                g_create_error = std::string("field_minmax_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e);
+                return ATHAD_ERR_DEVICE;
+            }
+            g_noalloc_ptr[dev] = p;
+        }
+        base = g_noalloc_ptr[dev];
+    }
+    // the descriptor the min/max kernels read: the caller's field in the slot of t, and the
+    // reduction scratch after the descriptor slot
+    MemberDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.K.im = im; d.K.jm = jm; d.K.km = km;
+    d.K.n = (long long)im * jm * km;
+    d.f[F_T] = const_cast<void*>(field);
+    d.scratch = (double*)((unsigned char*)base + kNoAllocDesc);
+    hipStream_t st = (hipStream_t)stream;
+    auto fail = [&](hipError_t e, const char* what) {
+        g_create_error = std::string("field_minmax_noalloc: ") + what + ": " + hipGetErrorString(e);
+        (void)hipStreamSynchronize(st);   // the descriptor copy reads `d` on this stack frame
+        return ATHAD_ERR_DEVICE;
+    };
+    hipError_t e = hipMemcpyAsync(base, &d, sizeof(d), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return fail(e, "descriptor upload");
+    Batch b;
+    b.d_md = (MemberDesc*)base;
+    b.nmem = 1;
+    b.max_n = d.K.n;
+    b.max_im = im; b.max_jm = jm; b.max_km = km;
+    b.stream = st;
+    b.fp32 = (precision == ATHAD_MIXED);
+    launch_minmax(b, 0, F_T);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "launch");
+    double r[4];
+    e = hipMemcpyAsync(r, d.scratch + diag_offset(im) + 4 * 256, sizeof(r), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return fail(e, "result copy");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "synchronize");
+    auto ijk = [&](double q, int* pos) {
+        if (!pos) return;
+        const long long iq = (q < 0.0) ? 0 : (long long)q;   // none finite: (0, 0, 0) as the reference
+        pos[0] = (int)(iq / ((long long)jm * km));
+        pos[1] = (int)((iq / km) % jm);
+        pos[2] = (int)(iq % km);
+    };
+    if (vmin) *vmin = r[0];
+    if (vmax) *vmax = r[2];
+    ijk(r[1], pos_min);
+    ijk(r[3], pos_max);
+    return ATHAD_OK;
+}

@@ -325,6 +325,22 @@ athad_status athad_moist_convection(athad_ctx* ctx);
 athad_status athad_set_conv_field(athad_ctx* ctx, int member, int which, const double* src);
 athad_status athad_get_conv_field(athad_ctx* ctx, int member, int which, double* dst);
 
+/* --- entry point that allocates nothing ------------------------------------------------ */
+
+/* No counterpart in the reference. searchMinMax_3D (atmosphere/MinMax_Atm.cpp:51-108), as
+ * athad_field_minmax computes it, of a field held in the caller's device memory (im*jm*km
+ * values, [i][j][k], double for ATHAD_FP64, float for ATHAD_MIXED), for callers that cannot
+ * allocate device memory: code under hipGraph capture set-up, tight serving loops, harnesses
+ * that forbid hipMalloc. It calls no allocating routine: the reduction's partial results and
+ * the descriptor the kernels read live in a fixed file-scope device array of the library
+ * (4 KiB + 128 KiB per device, held until the process ends), whose address is looked up once
+ * per device. Limit: im <= 200 (ATHAD_ERR_SHAPE beyond). stream: a hipStream_t, NULL for the
+ * default stream; the call waits for it before it returns. Calls that overlap on one device
+ * share that array and race: issue them one after the other. Errors are reported by
+ * athad_last_error(NULL). Outputs as athad_field_minmax; any may be NULL. */
+athad_status athad_field_minmax_noalloc(const void* field, int precision, int im, int jm, int km, void* stream,
+                                        double* vmin, int* pos_min, double* vmax, int* pos_max);
+
 /* --- measurement --------------------------------------------------------------------- */
 
 /* Per-kernel timing with HIP events on the context's stream: kernel ids as in
