@@ -44,7 +44,7 @@ EXPORTS = [
     "athad_meridional_streamfunction", "athad_get_slice", "athad_radiation", "athad_get_radiation",
     "athad_set_radiation", "athad_moist_convection", "athad_set_conv_field", "athad_get_conv_field",
     "athad_ice_scheme", "athad_set_ice_scheme", "athad_set_mp_field", "athad_get_mp_field",
-    "athad_field_minmax_noalloc",
+    "athad_field_minmax_noalloc", "athad_project_initial_velocity", "athad_set_moist_convection",
 ]
 
 # Arrays of the two-category ice scheme (athad_mp_field in include/athad_hip.h)
@@ -109,6 +109,7 @@ def lib():
                      "athad_sync"):
             getattr(L, name).argtypes = [vp]
         L.athad_densities.argtypes = [vp, i, i]
+        L.athad_project_initial_velocity.argtypes = [vp, i]
         L.athad_run.argtypes = [vp, i, i]
         L.athad_profile.argtypes = [vp, i, i]
         L.athad_profile_result.argtypes = [vp, i, ctypes.POINTER(d), ctypes.POINTER(i)]
@@ -137,6 +138,7 @@ def lib():
         L.athad_ice_scheme.argtypes = [vp]
         L.athad_set_radiation.argtypes = [vp, i]
         L.athad_set_ice_scheme.argtypes = [vp, i]
+        L.athad_set_moist_convection.argtypes = [vp, i]
         L.athad_set_mp_field.argtypes = [vp, i, i, dp]
         L.athad_moist_convection.argtypes = [vp]
         L.athad_set_conv_field.argtypes = [vp, i, i, dp]

@@ -87,6 +87,7 @@ struct MemberHost {
     double* mc[NMC] = {};          // moist-convection arrays, on first use
     double *evap = nullptr, *conv_cos = nullptr;
     double* conv_qs[2] = {};
+    double* conv_tmp[2] = {};
     int iter_n = 0, total = 0;     // host mirror of the device counters
     // table offsets inside 'tables'
     size_t o_rad = 0, o_sint = 0, o_cost = 0, o_sinr = 0, o_hgt = 0, o_nuef = 0, o_step = 0, o_geo = 0;
@@ -133,6 +134,8 @@ struct athad_ctx {
     // the on-demand arrays of the ice scheme (mp) and of moist convection (mc) are complete on
     // every member and published in the device descriptors
     bool mp_ready = false, mc_ready = false;
+    // moist convection inside the moist block, after the ice scheme (cAtmosphereModel.cpp:1247-1262)
+    bool conv_on = false;
 };
 
 namespace {
@@ -193,6 +196,11 @@ void fill_consts(MemberHost& m, const Config& cfg, int order) {
     K.drag_coeff = (1.0 / 86400.0) * cfg.L_atm / cfg.u_0 * cfg.dt_visc;
     K.evap_r = std::exp(-1.0);
     K.evap_w_norm = (1.0 - K.evap_r) / (1.0 - std::pow(K.evap_r, 3 + 1));
+    // RHS_Atm_Turb.cpp:370-372 (c_0 of the member), cAtmosphereDefaults.cpp.inc:60
+    K.coeff_mc_t = cfg.L_atm / (cfg.u_0 * cfg.t_0);
+    K.coeff_mc_vel = cfg.L_atm / (cfg.u_0 * cfg.u_0);
+    K.coeff_mc_q = cfg.L_atm / (cfg.u_0 * m.p.c_0);
+    K.coeff_dalton = 0.7;
     K.t_surf_eq = m.p.t_surf_equator; K.t_surf_pole = m.p.t_surf_pole;
     K.init_M_nw = M_nonwater_h(m.p.c_0, m.p.co2_0, C.M_bg);
     K.init_f_co2 = (C.q_CO2 + C.q_bg > 0.0) ? C.q_CO2 / (C.q_CO2 + C.q_bg) : m.p.co2_0 * cfg.co2_scale;
@@ -331,6 +339,7 @@ void make_desc(const athad_ctx* c, const MemberHost& m, int parity, MemberDesc& 
     for (int q = 0; q < NMC; q++) d.mc[q] = m.mc[q];
     d.evap = m.evap; d.conv_cos = m.conv_cos;
     d.conv_qs[0] = m.conv_qs[0]; d.conv_qs[1] = m.conv_qs[1];
+    d.conv_tmp[0] = m.conv_tmp[0]; d.conv_tmp[1] = m.conv_tmp[1];
     (void)c;
 }
 
@@ -372,6 +381,7 @@ void free_ctx(athad_ctx* c) {
         if (m.evap) hipFree(m.evap);
         if (m.conv_cos) hipFree(m.conv_cos);
         for (auto p : m.conv_qs) if (p) hipFree(p);
+        for (auto p : m.conv_tmp) if (p) hipFree(p);
         for (void* p : {(void*)m.vel_star, (void*)m.t_top, (void*)m.t_ref, (void*)m.tables, (void*)m.scratch,
                         (void*)m.counters, (void*)m.passes, (void*)m.radiation, (void*)m.epsilon, (void*)m.albedo})
             if (p) hipFree(p);
@@ -474,6 +484,8 @@ void op_turbulence(athad_ctx* c, int gate, hipStream_t st = nullptr) {
 // SatAdj, the moist damping and densities read and write the thermodynamic fields and
 // scratch 6..9. The two sets are disjoint, so their order is immaterial and the turbulence
 // runs on the second stream beside the others.
+void op_damp_conv(athad_ctx* c, int gate);
+
 void op_moist_block(athad_ctx* c, int gate, bool poisson_beside = false, bool store_fluxes = false) {
     auto fork_s2 = [&]() {
         hipEventRecord(c->ev_fork, c->stream);
@@ -507,6 +519,14 @@ void op_moist_block(athad_ctx* c, int gate, bool poisson_beside = false, bool st
         {
             Prof p(c, KID_ICE);
             launch_ice_scheme(cur(c), gate);
+        }
+        if (c->conv_on) {
+            Prof p(c, KID_CONV);
+            launch_moist_convection(cur(c), gate);
+            op_damp_conv(c, gate);
+        }
+        {
+            Prof p(c, KID_ICE);
             launch_moist_clamp_ice(cur(c), gate, store_fluxes ? 1 : 0);
         }
     } else {
@@ -589,10 +609,47 @@ void op_bc(athad_ctx* c, bool solid) {
         Prof p(c, KID_BC);
         launch_bc(cur(c));
         if (c->mem[0].mp[0]) launch_bc_sources(cur(c));
+        if (c->mem[0].mc[0]) launch_bc_conv(cur(c));
     }
     if (solid) {
         Prof p(c, KID_SOLID);
         launch_solid_clamp(cur(c));
+    }
+}
+
+// PressureSolverAtm::project_initial_velocity(n_sweeps) (PressureSolverAtm.h:686-767): the
+// velocity as the Poisson source, n sweeps of the solver (the same launches as the step's
+// sweep), the gradient correction, p_dyn and aux cleared.
+void op_project(athad_ctx* c, int n_sweeps) {
+    Prof p(c, KID_POISSON);
+    launch_project(cur(c), 0);
+    for (int q = 0; q < n_sweeps; q++) launch_poisson(cur(c), 0);
+    launch_project(cur(c), 1);
+    launch_project(cur(c), 2);
+}
+
+// The source terms RK4 adds: the ice scheme's once its arrays exist (bit 0), moist convection's
+// tendencies once its arrays exist (bit 1); both are zero in the reference until those operators run.
+int rk4_sources(const athad_ctx* c) {
+    return (c->mem[0].mp[0] ? 1 : 0) | (c->mem[0].mc[0] ? 2 : 0);
+}
+
+// The damp_wiggles of eleven convection arrays after MoistConvection::run
+// (cAtmosphereModel.cpp:1249-1262): FP64 arrays in both storage precisions, damped by the fused
+// tile kernel in FP64 into four FP64 temporaries and copied back, four arrays at a time.
+void op_damp_conv(athad_ctx* c, int gate) {
+    Batch b = cur(c);
+    b.fp32 = false;
+    constexpr int M0 = NFIELD + NTMP + NMP, T0 = M0 + NMC;
+    const int groups[3][4] = {{MC_P_CONV, MC_E_U, MC_M_U, MC_Q_V_U},
+                              {MC_Q_C_U, MC_E_D, MC_M_D, MC_Q_V_D},
+                              {MC_MC_T, MC_MC_V, MC_MC_W, -1}};
+    for (auto& g : groups) {
+        const int nf = (g[3] < 0) ? 3 : 4;
+        int src[4], dst[4];
+        for (int q = 0; q < nf; q++) { src[q] = M0 + g[q]; dst[q] = T0 + q; }
+        launch_damp_tile(b, c->tile_shape, c->tile_ich, src, dst, nf, 0, gate);
+        launch_copy_job(b, dst, src, nf, gate);
     }
 }
 
@@ -612,12 +669,12 @@ void op_rk4(athad_ctx* c, bool fused) {
     if (c->order == ATHAD_ORDER_LEX) {
         Prof p(c, KID_RK4_PLANE);
         const int dmax = max_plane(c);
-        const bool src = c->mem[0].mp[0] != nullptr;
+        const int src = rk4_sources(c);
         for (int d = 3; d <= dmax; d++) launch_rk4_lex(cur(c), d, src);
     } else {
         {
             Prof p(c, KID_RK4);
-            launch_rk4_jacobi(cur(c), fused, c->mem[0].mp[0] != nullptr);
+            launch_rk4_jacobi(cur(c), fused, rk4_sources(c));
         }
         c->parity ^= 1;
     }
@@ -1111,8 +1168,9 @@ athad_status athad_set_ice_scheme(athad_ctx* c, int on) {
         athad_status s = ensure_mp_buffers(c);
         if (s) return s;
     }
-    if (c->ice_on != (on != 0) && c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    if ((c->ice_on != (on != 0) || (!on && c->conv_on)) && c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
     c->ice_on = on != 0;
+    if (!on) c->conv_on = false;   // convection runs after the ice scheme in the moist block
     return ATHAD_OK;
 }
 
@@ -1171,6 +1229,7 @@ static athad_status ensure_mc_buffers(athad_ctx* c) {
             if (m2.evap) { hipFree(m2.evap); m2.evap = nullptr; }
             if (m2.conv_cos) { hipFree(m2.conv_cos); m2.conv_cos = nullptr; }
             for (auto& p : m2.conv_qs) if (p) { hipFree(p); p = nullptr; }
+            for (auto& p : m2.conv_tmp) if (p) { hipFree(p); p = nullptr; }
         }
         c->err = msg;
         republish(c);
@@ -1189,6 +1248,8 @@ static athad_status ensure_mc_buffers(athad_ctx* c) {
             return fail(ATHAD_ERR_DEVICE, "clearing the evaporation input failed");
         if (hipMalloc(&m.conv_cos, (size_t)jm * 8) != hipSuccess) { m.conv_cos = nullptr; return fail(ATHAD_ERR_ALLOC, oom); }
         for (auto& p : m.conv_qs)
+            if (hipMalloc(&p, m.n * 8) != hipSuccess) { p = nullptr; return fail(ATHAD_ERR_ALLOC, oom); }
+        for (auto& p : m.conv_tmp)
             if (hipMalloc(&p, m.n * 8) != hipSuccess) { p = nullptr; return fail(ATHAD_ERR_ALLOC, oom); }
         // theta by accumulation as the grid builds it; the reference takes the circle factor
         // about row 90 (MoistConvection.h:239), which is the equator on its 181-row grid. Here
@@ -1225,6 +1286,19 @@ athad_status athad_moist_convection(athad_ctx* c) {
     join_s2(c);
     { Prof p(c, KID_CONV); launch_moist_convection(cur(c)); }
     return after_launch(c);
+}
+
+athad_status athad_set_moist_convection(athad_ctx* c, int on) {
+    if (!c) return ATHAD_ERR_ARG;
+    if (on) {
+        athad_status s = check_conv_grid(c);
+        if (s || (s = ensure_mp_buffers(c)) || (s = ensure_mc_buffers(c))) return s;
+    }
+    const bool ice = on ? true : c->ice_on;
+    if ((c->conv_on != (on != 0) || c->ice_on != ice) && c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    c->conv_on = on != 0;
+    c->ice_on = ice;
+    return ATHAD_OK;
 }
 
 athad_status athad_set_conv_field(athad_ctx* ctx, int member, int which, const double* src) {
@@ -1543,11 +1617,19 @@ athad_status athad_init_state(athad_ctx* c, uint64_t seed) {
     launch_init_rest(b);
     op_turbulence(c, 0);
     op_bc(c, true);
+    op_project(c, 200);   // RunTimeSlice :665
     launch_copy_field(b, F_T_EQ, F_T);
     launch_store(b, 0x7FF);
     if (after_launch(c)) return ATHAD_ERR_DEVICE;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return ATHAD_OK;
+}
+
+athad_status athad_project_initial_velocity(athad_ctx* c, int n_sweeps) {
+    if (!c || n_sweeps < 0) return ATHAD_ERR_ARG;
+    join_s2(c);
+    op_project(c, n_sweeps);
+    return after_launch(c);
 }
 
 athad_status athad_poisson_sweep(athad_ctx* c) {

@@ -55,6 +55,9 @@ struct Consts {
     double nue_max, nue_air_nd, tke_max_nd, vel_max_nd, om_wall, sqrt_cnue;
     double hs_k_a, hs_k_s, drag_coeff;
     double inv_re_turb, inv_pr_turb, dt_sixth;   // 1/re_turb, 1/pr_turb, dt/6 (kernel-wide)
+    // moist convection's tendency coefficients (RHS_Atm_Turb.cpp:370-372) and the Dalton
+    // evaporation coefficient of the Evaporation diagnostic it reads (cAtmosphereDefaults.cpp.inc:60)
+    double coeff_mc_t, coeff_mc_vel, coeff_mc_q, coeff_dalton;
     double evap_r, evap_w_norm;
     // radiation (MultiLayerRadiation, mode 2)
     double sigma, kappa_h2o, kappa_co2, kappa_bg, geothermal, sw_equator, sw_pole;
@@ -97,6 +100,8 @@ struct MemberDesc {
     double* evap;            // jm*km    evaporation diagnostic [mm/d] the convection trigger reads
     const double* conv_cos;  // jm       cos(theta[90] - theta[j]), the convection's zonal circle factor
     double* conv_qs[2];      // n        per-cell saturation of the lifted (+0.2 K) and the resting air
+    double* conv_tmp[2];     // n        FP64 damping scratch of the convection arrays (with conv_qs,
+                             //          which the next call recomputes): allocated with them
 };
 
 #define ATHAD_DEV __device__ __forceinline__

@@ -51,12 +51,14 @@ void launch_skin_step(const Batch& b, MemberDesc* md0, MemberDesc* md1, double r
 // must hold their ice-scheme arrays (MemberDesc::mp). Scratch 6-9 are free for it at the place of
 // the ice scheme inside the moist block (the damping has consumed them).
 void launch_ice_scheme(const Batch& b, int gate);
-void launch_moist_convection(const Batch& b);
+// MoistConvection::run on every member (gate: the cadence test of the moist block, 0 = always)
+void launch_moist_convection(const Batch& b, int gate = 0);
 // precipitationSum, cap_S and the microphysics clamp that follow the ice scheme in the moist
 // block; store != 0 also copies the fluxes to their stored arrays
 void launch_moist_clamp_ice(const Batch& b, int gate, int store);
 // boundary faces of S_c_c and the five sources on members that hold ice-scheme arrays
 void launch_bc_sources(const Batch& b);
+void launch_bc_conv(const Batch& b);
 // P_rainn <- P_rain, P_snown <- P_snow on members that hold ice-scheme arrays.
 void launch_ice_store(const Batch& b);
 // one plane (n values) of one field, widened to double, at scratch[sf_offset(im)]: axis 0 fixes
@@ -64,8 +66,8 @@ void launch_ice_store(const Batch& b);
 void launch_slice(const Batch& b, int member, int field, int axis, int index, long long n);
 // src: add the microphysics sources of the ice-scheme arrays to the tendencies of t, c,
 // cloud, ice (RHS_Atm_Turb.cpp:963-966, 1058-1067); every member must hold those arrays
-void launch_rk4_jacobi(const Batch& b, bool fused, bool src = false);
-void launch_rk4_lex(const Batch& b, int plane, bool src = false);
+void launch_rk4_jacobi(const Batch& b, bool fused, int src = 0);
+void launch_rk4_lex(const Batch& b, int plane, int src = 0);
 void launch_poisson(const Batch& b, int gate);
 void launch_satadj(const Batch& b, int gate, int to_tmp);
 // pass 0,1,2 over the fields of one job
@@ -78,7 +80,7 @@ int damp_tile_shapes();
 void damp_tile_shape(int shape, int* tj, int* tk);
 void launch_damp_tile(const Batch& b, int shape, int ich, const int* src, const int* dst, int nf, int moist_clamp,
                       int gate);
-void launch_copy_job(const Batch& b, const int* src, const int* dst, int nf);
+void launch_copy_job(const Batch& b, const int* src, const int* dst, int nf, int gate = 0);
 void launch_densities(const Batch& b, int set_water, int evaporate, int gate);
 void launch_vel_star(const Batch& b, int gate);
 void launch_turb_sources(const Batch& b, int gate);
@@ -88,6 +90,9 @@ void launch_polar(const Batch& b);
 void launch_radial(const Batch& b, int store);
 void launch_store(const Batch& b, int mask);
 void launch_copy_field(const Batch& b, int dst, int src);
+// project_initial_velocity around the Poisson sweeps: phase 0 seed (aux = velocity, p_dyn = 0),
+// 1 gradient correction of the interior velocity, 2 clear p_dyn and aux
+void launch_project(const Batch& b, int phase);
 void launch_init_column(const Batch& b);
 void launch_init_water(const Batch& b);
 void launch_init_co2(const Batch& b);

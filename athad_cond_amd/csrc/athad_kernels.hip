@@ -722,7 +722,8 @@ __global__ __launch_bounds__(256) void k_moist_clamp_ice(const MemberDesc* __res
     if (id >= K.n) return;
     using C = double;
     const S pr = mpp<S>(M, MP_P_RAIN)[id], ps = mpp<S>(M, MP_P_SNOW)[id];
-    mpp<S>(M, MP_PRECIP)[id] = S(C(pr) + C(ps) + 0.0 + 0.0);
+    // precipitationSum (UtilsAtm.h:52-64): P_graupel is zero; P_conv once moist convection runs
+    mpp<S>(M, MP_PRECIP)[id] = S(C(pr) + C(ps) + 0.0 + (M.mc[0] ? M.mc[MC_P_CONV][id] : 0.0));
     if (store) { mpp<S>(M, MP_P_RAINN)[id] = pr; mpp<S>(M, MP_P_SNOWN)[id] = ps; }
     auto cap = [](C v) { return !finite_bits(v) ? C(0) : (v > C(1.0e-3)) ? C(1.0e-3) : (v < C(-1.0e-3)) ? C(-1.0e-3) : v; };
     S* scc = fp<S>(M, F_S_C_C);
@@ -774,6 +775,59 @@ __global__ __launch_bounds__(256) void k_bc_sources(const MemberDesc* __restrict
             return double(x[ix3(K, i, jj, kk)]);
         };
         x[id] = seam ? S((double(S(R(1))) + double(S(R(K.km - 2)))) * 0.5) : S(R(k));
+    }
+}
+
+// Boundary faces of the moist-convection arrays (FP64), where they exist. bcRadius: cubic at both
+// ends for q_v_u, q_c_u, u_u, v_u, w_u, s, s_u, s_d, M_u, M_d (BC_Atm.h:567-574); x[1] at i=0 and a
+// copy at the lid for P_conv (c43 = 1, c13 = 0, :587-590, :679-683); cubic at i=0 and a copy at
+// the lid for c_u, e_d, e_l, e_p, g_p (:602-606, :733-737); q_v_u, q_c_u at i=0 floored at zero
+// (:741-742). bcTheta's copy (:769-775) and bcPhi's seam average (:810-818) for all but M_u, M_d
+// and P_conv, in the closed form of the three passes. Writes only boundary cells, reads only
+// cells that no thread writes.
+__global__ __launch_bounds__(256) void k_bc_conv(const MemberDesc* __restrict__ md) {
+#pragma clang fp contract(off)
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= K.n || !M.mc[0]) return;
+    int i, j, k;
+    decode(K, id, i, j, k);
+    const int iml = K.im - 1;
+    const bool radial = (i == 0 || i == iml);
+    const bool seam = (k == 0 || K.km - 1 == k);
+    if (!radial && !seam && j > 0 && j < K.jm - 1) return;
+    if (radial) {   // the radial-only arrays, every (j, k) of both radial faces
+        for (int q : {MC_M_U, MC_M_D}) {
+            const double* x = M.mc[q];
+            M.mc[q][id] = (i == 0) ? x[ix3(K, 3, j, k)] - 3.0 * x[ix3(K, 2, j, k)] + 3.0 * x[ix3(K, 1, j, k)]
+                                   : x[ix3(K, iml - 3, j, k)] - 3.0 * x[ix3(K, iml - 2, j, k)] + 3.0 * x[ix3(K, iml - 1, j, k)];
+        }
+        const double* pc = M.mc[MC_P_CONV];
+        M.mc[MC_P_CONV][id] = (i == 0) ? 1.0 * pc[ix3(K, 1, j, k)] - 0.0 * pc[ix3(K, 2, j, k)] : pc[ix3(K, iml - 1, j, k)];
+    }
+    const int jj = min(max(j, 1), K.jm - 2);
+    constexpr int kSides = 13;
+    const int sides[kSides] = {MC_Q_V_U, MC_Q_C_U, MC_U_U, MC_V_U, MC_W_U, MC_S, MC_S_U, MC_S_D,
+                               MC_C_U, MC_E_D_EVAP, MC_E_L, MC_E_P, MC_G_P};
+#pragma unroll
+    for (int n = 0; n < kSides; n++) {
+        const int q = sides[n];
+        const double* x = M.mc[q];
+        const bool top_cubic = n < 8;
+        const bool floor0 = (q == MC_Q_V_U || q == MC_Q_C_U);
+        auto R = [&](int kk) -> double {   // the value after bcRadius at (i, jj, kk), kk interior
+            if (i == 0) {
+                const double v = x[ix3(K, 3, jj, kk)] - 3.0 * x[ix3(K, 2, jj, kk)] + 3.0 * x[ix3(K, 1, jj, kk)];
+                return (floor0 && v < 0.0) ? 0.0 : v;
+            }
+            if (i == iml)
+                return top_cubic ? x[ix3(K, iml - 3, jj, kk)] - 3.0 * x[ix3(K, iml - 2, jj, kk)]
+                                       + 3.0 * x[ix3(K, iml - 1, jj, kk)]
+                                 : x[ix3(K, iml - 1, jj, kk)];
+            return x[ix3(K, i, jj, kk)];
+        };
+        M.mc[q][id] = seam ? (R(1) + R(K.km - 2)) * 0.5 : R(k);
     }
 }
 
@@ -1047,6 +1101,11 @@ constexpr int kProgN[NPROG] = {F_TN, F_UN, F_VN, F_WN, F_CN, F_CLOUDN, F_ICEN, F
 // of the cell in one batch into registers before any arithmetic: the lex plane kernel runs
 // about one wave per SIMD (a plane holds a few thousand cells), so every dependent memory
 // round trip of its cell chain is exposed and one round trip is what it can afford.
+// Source flags of the RK4 kernels (template parameter SRC): the ice scheme's microphysics sources
+// (RHS_Atm_Turb.cpp:963-966, 1058-1067) and moist convection's tendencies MC_t, MC_v, MC_w, MC_q
+// (RHS_Atm_Turb.cpp:370-372, 964, 1023, 1042, 1061), each read once per cell where its arrays exist.
+constexpr int SRC_ICE = 1, SRC_CONV = 2;
+
 template <typename S, typename C, bool CLAMP> struct NbLoad {
     const MemberDesc& M;
     const S* const* X;
@@ -1061,6 +1120,7 @@ template <typename S, typename C, bool CLAMP> struct NbLoad {
     }
     ATHAD_DEV C field(int fid) const { return C(gld(fp<S>(M, fid), id)); }   // nue, t_eq, r_humid
     ATHAD_DEV C mp(int q) const { return C(gld((const S*)M.mp[q], id)); }
+    ATHAD_DEV double mc(int q) const { return gld((const double*)M.mc[q], id); }
     ATHAD_DEV C start(int f) const {
         C c0 = C(gld(X[f], id));
         if (CLAMP) c0 = solid_clamp(f, c0, C(M.K.tke_max_nd), C(M.K.vel_max_nd));
@@ -1075,9 +1135,9 @@ template <typename S, typename C, bool CLAMP> struct NbLoad {
     ATHAD_DEV double psurf() const { return gld(pstat(M), isurf); }
 };
 
-template <typename S, typename C, bool SRC> struct NbRegs {
+template <typename S, typename C, int SRC> struct NbRegs {
     C nb[NPROG][6], pd[6], c0[NPROG], b0[NPROG], nu, te, rh, src[5];
-    double psv, pss;
+    double psv, pss, mcv[4];
     ATHAD_DEV NbRegs(const MemberDesc& M, const S* const* X, long long id, long long si, long long sj,
                      long long isurf) {
         const C tkm = C(M.K.tke_max_nd), vm = C(M.K.vel_max_nd);
@@ -1097,11 +1157,13 @@ template <typename S, typename C, bool SRC> struct NbRegs {
         rh = C(0);
 #pragma unroll
         for (int q = 0; q < 5; q++) src[q] = C(0);
-        if (SRC) {
+        if (SRC & SRC_ICE) {
             rh = C(gld(fp<S>(M, F_R_HUMID), id));
 #pragma unroll
             for (int q = 0; q < 5; q++) src[q] = C(gld((const S*)M.mp[MP_S_V + q], id));
         }
+#pragma unroll
+        for (int q = 0; q < 4; q++) mcv[q] = (SRC & SRC_CONV) ? gld((const double*)M.mc[MC_MC_T + q], id) : 0.0;
     }
     ATHAD_DEV void six(int f, C* v) const {
 #pragma unroll
@@ -1113,6 +1175,7 @@ template <typename S, typename C, bool SRC> struct NbRegs {
     }
     ATHAD_DEV C field(int fid) const { return fid == F_NUE ? nu : fid == F_T_EQ ? te : rh; }
     ATHAD_DEV C mp(int q) const { return src[q - MP_S_V]; }
+    ATHAD_DEV double mc(int q) const { return mcv[q - MC_MC_T]; }
     ATHAD_DEV C start(int f) const { return c0[f]; }
     ATHAD_DEV C base(int f) const { return b0[f]; }
     ATHAD_DEV double ps() const { return psv; }
@@ -1173,7 +1236,7 @@ template <bool U, typename F> ATHAD_DEV void for_stages(F&& f) {
 // tendencies involve no other prognostic field, so the group integrates first and records
 // the velocity entering each stage in rec[] for the scalar groups. cen[] = start values in,
 // final values out. A, B: velocity-gradient invariants of the production term.
-template <typename S, typename C, bool CLAMP, bool SRC, bool U, typename Nb>
+template <typename S, typename C, bool CLAMP, int SRC, bool U, typename Nb>
 ATHAD_DEV void rk4_dyn(const MemberDesc& M, const Nb& src_nb, int i, const CellGeo<C>& g,
                        C* ba, C* rec, C* cen, C* aux, C& A, C& B) {
     const Consts& K = M.K;
@@ -1228,12 +1291,19 @@ ATHAD_DEV void rk4_dyn(const MemberDesc& M, const Nb& src_nb, int i, const CellG
     const C ce = C(K.coeff_energy_p);
     // latent heating of the microphysics sources (RHS_Atm_Turb.cpp:965-966), stage-invariant
     C srcT = C(0);
-    if (SRC) {
+    if (SRC & SRC_ICE) {
         const C rh = src_nb.field(F_R_HUMID);
         const C Sc = src_nb.mp(MP_S_C), Sr = src_nb.mp(MP_S_R);
         const C Si = src_nb.mp(MP_S_I), Ss = src_nb.mp(MP_S_S);
         const C cen_ = C(K.coeff_energy);
         srcT = cen_ * (Sc + Sr) * C(K.lv) * rh + cen_ * (Si + Ss + C(0)) * C(K.ls) * rh;
+    }
+    // moist convection's tendencies (RHS_Atm_Turb.cpp:964, 1023, 1042), stage-invariant
+    C mcT = C(0), mcV = C(0), mcW = C(0);
+    if (SRC & SRC_CONV) {
+        mcT = C(K.coeff_mc_t * src_nb.mc(MC_MC_T));
+        mcV = C(K.coeff_mc_vel * src_nb.mc(MC_MC_V));
+        mcW = C(K.coeff_mc_vel * src_nb.mc(MC_MC_W));
     }
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt_sixth);
 for_stages<U>([&](const int st) {
@@ -1241,11 +1311,15 @@ for_stages<U>([&](const int st) {
         rec[(3 * st + 0) * LB] = u; rec[(3 * st + 1) * LB] = v; rec[(3 * st + 2) * LB] = w;
         const C ua = u * g.aR, va = v * g.aT, wa = w * g.aP;
         C rT = ce * (u * PR + v * PT + w * PP) - minmod3(nb[FT], T, ua, va, wa) + (L0T - g.Lc2 * T) * dif_t;
-        if (SRC) rT = rT + srcT;
+        if (SRC & SRC_CONV) rT = rT + mcT;
+        if (SRC & SRC_ICE) rT = rT + srcT;
         rT = rT - g.hs_kT * (T - t_eq);
         const C rU = -PR - minmod3(nb[FU], u, ua, va, wa) + (D0u + g.cu * u) * dif_vel + buoy * (T - t_ref);
-        const C rV = -PT - minmod3(nb[FV], v, ua, va, wa) + (D0v + g.cvw * v) * dif_vel + g.cor * w - g.drag * v;
-        const C rW = -PP - minmod3(nb[FW], w, ua, va, wa) + (D0w + g.cvw * w) * dif_vel - g.cor * v - g.drag * w;
+        C rV = -PT - minmod3(nb[FV], v, ua, va, wa) + (D0v + g.cvw * v) * dif_vel + g.cor * w;
+        C rW = -PP - minmod3(nb[FW], w, ua, va, wa) + (D0w + g.cvw * w) * dif_vel - g.cor * v;
+        if (SRC & SRC_CONV) { rV = rV + mcV; rW = rW + mcW; }
+        rV = rV - g.drag * v;
+        rW = rW - g.drag * w;
         if (st == 3) { aux[0] = rU + PR; aux[1] = rV + PT; aux[2] = rW + PP; }
         cen[FT] = stage_update(st, rT, ba + 0 * LB, ba + 4 * LB, dt, half_dt, dt_sixth);
         cen[FU] = stage_update(st, rU, ba + 1 * LB, ba + 5 * LB, dt, half_dt, dt_sixth);
@@ -1256,7 +1330,7 @@ for_stages<U>([&](const int st) {
 
 // RK4 of the moisture group c, cloud, ice (minmod advection), gr, co2 (central), advected by
 // the recorded stage velocities; cen[0..4] = start values in, final values out.
-template <typename S, typename C, bool CLAMP, bool SRC, bool U, typename Nb, typename Vel>
+template <typename S, typename C, bool CLAMP, int SRC, bool U, typename Nb, typename Vel>
 ATHAD_DEV void rk4_moist(const MemberDesc& M, const Nb& src_nb, const CellGeo<C>& g, C* ba,
                          Vel vel, C* cen) {
     const Consts& K = M.K;
@@ -1279,12 +1353,14 @@ ATHAD_DEV void rk4_moist(const MemberDesc& M, const Nb& src_nb, const CellGeo<C>
     const C dpr = C(K.diff_prec_re_inv), dco = C(K.diff_co2_re_inv);
     // microphysics sources of c, cloud, ice (RHS_Atm_Turb.cpp:1058-1067), stage-invariant
     C src[3] = {C(0), C(0), C(0)};
-    if (SRC) {
+    if (SRC & SRC_ICE) {
         const C rh = src_nb.field(F_R_HUMID);
         src[0] = C(1) * src_nb.mp(MP_S_V) * rh;
         src[1] = C(1) * src_nb.mp(MP_S_C) * rh;
         src[2] = C(1) * src_nb.mp(MP_S_I) * rh;
     }
+    // moist convection's vapour tendency (RHS_Atm_Turb.cpp:1061), stage-invariant
+    const C mcQ = (SRC & SRC_CONV) ? C(K.coeff_mc_q * src_nb.mc(MC_MC_Q)) : C(0);
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt_sixth);
 for_stages<U>([&](const int st) {
         C u, v, w;
@@ -1294,7 +1370,8 @@ for_stages<U>([&](const int st) {
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             r[q] = -minmod3(nb[q], cen[q], ua, va, wa) + (L0[q] - g.Lc2 * cen[q]) * dpr;
-            if (SRC) r[q] = r[q] + src[q];
+            if (SRC & SRC_ICE) r[q] = r[q] + src[q];
+            if ((SRC & SRC_CONV) && q == 0) r[q] = r[q] + mcQ;
         }
         r[3] = -(u * gc[0][0] + v * gc[0][1] + w * gc[0][2]) + (L0[3] - g.Lc2 * cen[3]) * dpr;
         r[4] = -(u * gc[1][0] + v * gc[1][1] + w * gc[1][2]) + (L0[4] - g.Lc2 * cen[4]) * dco;
@@ -1403,7 +1480,7 @@ ATHAD_DEV void group_start(const Nb& src_nb, int f0, int nf, C* cen, C* ba) {
 
 // RK4 of one interior cell, group by group: dynamics, moisture, turbulence. out(f, value)
 // writes a final value; the aux / prod / source outputs are written here.
-template <typename S, typename C, bool CLAMP, bool SRC, bool U, bool GT, typename Nb, typename Out>
+template <typename S, typename C, bool CLAMP, int SRC, bool U, bool GT, typename Nb, typename Out>
 ATHAD_DEV void rk4_cell(const MemberDesc& M, const Nb& nb, long long id, int i, int j, int k, C* ba, C* rec,
                         Out out) {
     CellGeo<C> g;
@@ -1446,7 +1523,7 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const Nb& nb, long long id, int i, 
 // start-of-RK4 neighbourhood read from f[] and write alt[]; boundary cells are copied. The
 // solid-ground clamps are applied on read (FUSED=1), and the store of the eight fields that
 // no later filter touches is fused (xn = new value).
-template <typename S, bool FUSED, bool SRC>
+template <typename S, bool FUSED, int SRC>
 __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>()))) void k_rk4_jacobi(const MemberDesc* __restrict__ md) {
     using C = S;
     __shared__ C s_ba[10 * LB], s_rec[12 * LB];
@@ -1497,7 +1574,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
 // Lexicographic-order RK4 restricted to the hyperplane i+j+k = d (in place). Cells of one
 // plane only read neighbours on planes d-1 (final) and d+1 (not yet updated), which is
 // exactly what the reference's single-threaded sweep sees.
-template <typename S, bool SRC>
+template <typename S, int SRC>
 __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_rk4_lex_plane(const MemberDesc* __restrict__ md, int d) {
     using C = S;
     __shared__ C s_ba[10 * LB], s_rec[12 * LB];
@@ -2238,8 +2315,15 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
 // ------------------------------------------------------------------------------------
 struct DampJob { int nf; int src[6]; int dst[6]; int moist_clamp; };
 
+// Buffer q of a damping / copy job: fields, scratch, the ice-scheme arrays, then (FP64 jobs only)
+// the convection arrays and their four damping temporaries (conv_qs[0..1], conv_tmp[0..1]).
 template <typename S> ATHAD_DEV S* job_ptr(const MemberDesc& M, int q) {
-    return (q < NFIELD) ? (S*)M.f[q] : (q < NFIELD + NTMP) ? (S*)M.tmp[q - NFIELD] : (S*)M.mp[q - NFIELD - NTMP];
+    constexpr int M0 = NFIELD + NTMP + NMP, T0 = M0 + NMC;
+    if (q < NFIELD) return (S*)M.f[q];
+    if (q < NFIELD + NTMP) return (S*)M.tmp[q - NFIELD];
+    if (q < M0) return (S*)M.mp[q - NFIELD - NTMP];
+    if (q < T0) return (S*)M.mc[q - M0];
+    return (S*)((q - T0 < 2) ? M.conv_qs[q - T0] : M.conv_tmp[q - T0 - 2]);
 }
 
 template <typename S, int AXIS>
@@ -2422,8 +2506,9 @@ __global__ __launch_bounds__(256) void k_damp_level0(const MemberDesc* __restric
 
 // Copies up to six buffers (src[q] -> dst[q]).
 template <typename S>
-__global__ void k_copy_job(const MemberDesc* __restrict__ md, DampJob job) {
+__global__ void k_copy_job(const MemberDesc* __restrict__ md, DampJob job, int gate) {
     const MemberDesc& M = md[blockIdx.z];
+    if (gated_off(M, gate)) return;
     const long long id = cell_id(M.K);
     if (id < 0) return;
     for (int q = 0; q < job.nf; q++) job_ptr<S>(M, job.dst[q])[id] = job_ptr<S>(M, job.src[q])[id];
@@ -2506,10 +2591,28 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* __restrict_
     if (!evaporate) return;
     // waterVapourEvaporation, sea branch
     const double t0s = (double)t[base] * K.t_0;
-    if (t0s >= kTcrit) return;
+    if (t0s >= kTcrit) {
+        if (M.evap) M.evap[base] = 0.0;   // no evaporation from a supercritical surface (ThermoAtm.h:179-185)
+        return;
+    }
     const double T_s = tmax(180.0, t0s);
     const double c0 = c[base];
-    const double q_sea = th_q_sat(th_p_auto(T_s), ps[base], th_M_nonwater(c0, (double)co2[base], Mb));
+    const double M_nw0 = th_M_nonwater(c0, (double)co2[base], Mb);
+    const double E_s = th_p_auto(T_s);
+    const double q_sea = th_q_sat(E_s, ps[base], M_nw0);
+    if (M.evap) {
+        // the Evaporation diagnostic [mm/d] that moist convection's surface-flux trigger reads:
+        // Dalton's law on the saturation deficit at the surface (ThermoAtm.h:217-240, with
+        // AtomUtils::C_Dalton, lib/Utils.cpp:157-172), computed before the surface update
+        const double u0v = (double)fp<S>(M, F_U)[base], v0v = (double)fp<S>(M, F_V)[base];
+        const double w0v = (double)fp<S>(M, F_W)[base];
+        const double vel = sqrt((u0v * u0v + v0v * v0v + w0v * w0v) / 3.0) * K.u_0;
+        const double c_dalton = K.coeff_dalton * sqrt(0.053 * 0.053 * vel / 10.0);
+        const double p_s = ps[base];
+        const double e_air = (q_sea > 0.0 && q_sea < 1.0) ? c0 * p_s * M_nw0 / (c0 * M_nw0 + (1.0 - c0) * kMH2O) : 0.0;
+        const double sd = tmax(0.0, E_s - e_air);
+        M.evap[base] = c_dalton * 24.0 * sd;
+    }
     c[base] = S(c0 + (q_sea - c0) * K.evap_w_norm);
     double rp = 1.0;
     for (int i = 1; i <= 3 && i < K.im; i++) {
@@ -3258,6 +3361,42 @@ __global__ void k_copy_field(const MemberDesc* __restrict__ md, int dst, int src
     fp<S>(M, dst)[id] = fp<S>(M, src)[id];
 }
 
+// PressureSolverAtm::project_initial_velocity (PressureSolverAtm.h:686-767), around the
+// solver's sweeps (launched between the phases). phase 0: aux = (u, v, w), p_dyn = 0 (:694-706);
+// phase 1: the interior gradient correction v -= grad p_dyn with the RHS metric (sin theta
+// floored at 0.55, metric radius) (:715-749); phase 2: p_dyn = aux = 0 (:752-763).
+template <typename S>
+__global__ __launch_bounds__(256) void k_project(const MemberDesc* __restrict__ md, int phase) {
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const long long id = cell_id(K);
+    if (id < 0) return;
+    S *u = fp<S>(M, F_U), *v = fp<S>(M, F_V), *w = fp<S>(M, F_W), *p = fp<S>(M, F_P_DYN);
+    S *au = fp<S>(M, F_AUX_U), *av = fp<S>(M, F_AUX_V), *aw = fp<S>(M, F_AUX_W);
+    if (phase == 0) {
+        au[id] = u[id]; av[id] = v[id]; aw[id] = w[id]; p[id] = S(0);
+        return;
+    }
+    if (phase == 2) {
+        p[id] = S(0); au[id] = S(0); av[id] = S(0); aw[id] = S(0);
+        return;
+    }
+    int i, j, k;
+    decode(K, id, i, j, k);
+    if (i < 1 || i > K.im - 2 || j < 1 || j > K.jm - 2 || k < 1 || k > K.km - 2) return;
+    const long long si = (long long)K.jm * K.km, sj = K.km;
+    const double rm = M.rad[i];
+    const double exp_rm = 1.0 / (rm + 1.0);
+    const double rmet = (K.metric_r0 > 0.0) ? (K.metric_r0 + (rm - K.rad0)) : rm;
+    const double inv_rm = 1.0 / rmet;
+    const double inv_rmsinthe = 1.0 / (rmet * M.sint[j]);
+    const double dpdr = ((double)p[id + si] - (double)p[id - si]) * K.inv_2dr;
+    const double dpdthe = ((double)p[id + sj] - (double)p[id - sj]) * K.inv_2dthe;
+    const double dpdphi = ((double)p[id + 1] - (double)p[id - 1]) * K.inv_2dphi;
+    u[id] = S((double)u[id] - dpdr * exp_rm);
+    v[id] = S((double)v[id] - dpdthe * inv_rm);
+    w[id] = S((double)w[id] - dpdphi * inv_rmsinthe);
+}
 
 // ------------------------------------------------------------------------------------
 // Host launch wrappers
@@ -3340,6 +3479,11 @@ void launch_bc_sources(const Batch& b) {
     else hipLaunchKernelGGL(k_bc_sources<double>, g, dim3(256), 0, b.stream, b.d_md);
 }
 
+void launch_bc_conv(const Batch& b) {
+    const dim3 g((unsigned)((b.max_n + 255) / 256), 1, b.nmem);
+    hipLaunchKernelGGL(k_bc_conv, g, dim3(256), 0, b.stream, b.d_md);
+}
+
 void launch_ice_store(const Batch& b) {
     const dim3 g((unsigned)((b.max_n + 255) / 256), 1, b.nmem);
     if (b.fp32) hipLaunchKernelGGL(k_ice_store<float>, g, dim3(256), 0, b.stream, b.d_md);
@@ -3362,31 +3506,37 @@ void launch_level_mean(const Batch& b) {
     hipLaunchKernelGGL(k_level_mean_final, dim3((b.max_im + 63) / 64, 1, b.nmem), dim3(64), 0, b.stream, b.d_md);
 }
 namespace {
-template <typename S, bool SRC> void rk4_jacobi_t(const Batch& b, const dim3& g, bool fused) {
+template <typename S, int SRC> void rk4_jacobi_t(const Batch& b, const dim3& g, bool fused) {
     if (fused) hipLaunchKernelGGL((k_rk4_jacobi<S, true, SRC>), g, dim3(256), 0, b.stream, b.d_md);
     else hipLaunchKernelGGL((k_rk4_jacobi<S, false, SRC>), g, dim3(256), 0, b.stream, b.d_md);
 }
-}  // namespace
-void launch_rk4_jacobi(const Batch& b, bool fused, bool src) {
-    const dim3 g(cells_grid(b, 256), 1, b.nmem);
-    if (b.fp32) {
-        if (src) rk4_jacobi_t<float, true>(b, g, fused);
-        else rk4_jacobi_t<float, false>(b, g, fused);
-    } else {
-        if (src) rk4_jacobi_t<double, true>(b, g, fused);
-        else rk4_jacobi_t<double, false>(b, g, fused);
+template <typename S> void rk4_jacobi_s(const Batch& b, const dim3& g, bool fused, int src) {
+    switch (src & 3) {
+        case 0: rk4_jacobi_t<S, 0>(b, g, fused); break;
+        case 1: rk4_jacobi_t<S, 1>(b, g, fused); break;
+        case 2: rk4_jacobi_t<S, 2>(b, g, fused); break;
+        default: rk4_jacobi_t<S, 3>(b, g, fused); break;
     }
 }
-void launch_rk4_lex(const Batch& b, int plane, bool src) {
+template <typename S> void rk4_lex_s(const Batch& b, const dim3& g, int plane, int src) {
+    switch (src & 3) {
+        case 0: hipLaunchKernelGGL((k_rk4_lex_plane<S, 0>), g, dim3(256), 0, b.stream, b.d_md, plane); break;
+        case 1: hipLaunchKernelGGL((k_rk4_lex_plane<S, 1>), g, dim3(256), 0, b.stream, b.d_md, plane); break;
+        case 2: hipLaunchKernelGGL((k_rk4_lex_plane<S, 2>), g, dim3(256), 0, b.stream, b.d_md, plane); break;
+        default: hipLaunchKernelGGL((k_rk4_lex_plane<S, 3>), g, dim3(256), 0, b.stream, b.d_md, plane); break;
+    }
+}
+}  // namespace
+void launch_rk4_jacobi(const Batch& b, bool fused, int src) {
+    const dim3 g(cells_grid(b, 256), 1, b.nmem);
+    if (b.fp32) rk4_jacobi_s<float>(b, g, fused, src);
+    else rk4_jacobi_s<double>(b, g, fused, src);
+}
+void launch_rk4_lex(const Batch& b, int plane, int src) {
     const long long nq = (long long)(b.max_im - 2) * (b.max_jm - 2);
     const dim3 g((unsigned)((nq + 255) / 256), 1, b.nmem);
-    if (b.fp32) {
-        if (src) hipLaunchKernelGGL((k_rk4_lex_plane<float, true>), g, dim3(256), 0, b.stream, b.d_md, plane);
-        else hipLaunchKernelGGL((k_rk4_lex_plane<float, false>), g, dim3(256), 0, b.stream, b.d_md, plane);
-    } else {
-        if (src) hipLaunchKernelGGL((k_rk4_lex_plane<double, true>), g, dim3(256), 0, b.stream, b.d_md, plane);
-        else hipLaunchKernelGGL((k_rk4_lex_plane<double, false>), g, dim3(256), 0, b.stream, b.d_md, plane);
-    }
+    if (b.fp32) rk4_lex_s<float>(b, g, plane, src);
+    else rk4_lex_s<double>(b, g, plane, src);
 }
 void launch_poisson(const Batch& b, int gate) {
     const unsigned na = (unsigned)(((long long)(b.max_jm - 2) * (b.max_km - 2) + 255) / 256);
@@ -3462,11 +3612,11 @@ void launch_damp_tile(const Batch& b, int shape, int ich, const int* src, const 
 void launch_damp_level0(const Batch& b, int src, int dst, int gate) {
     ATHAD_DISPATCH(k_damp_level0, dim3(cols_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, src, dst, gate);
 }
-void launch_copy_job(const Batch& b, const int* src, const int* dst, int nf) {
+void launch_copy_job(const Batch& b, const int* src, const int* dst, int nf, int gate) {
     DampJob job{};
     job.nf = nf;
     for (int q = 0; q < nf; q++) { job.src[q] = src[q]; job.dst[q] = dst[q]; }
-    ATHAD_DISPATCH(k_copy_job, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, job);
+    ATHAD_DISPATCH(k_copy_job, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, job, gate);
 }
 void launch_densities(const Batch& b, int set_water, int evaporate, int gate) {
     ATHAD_DISPATCH(k_densities, dim3(cols_grid(b, 128), 1, b.nmem), dim3(128), 0, b.d_md, set_water, evaporate, gate);
@@ -3539,6 +3689,9 @@ void launch_radial(const Batch& b, int store) {
 void launch_store(const Batch& b, int mask) {
     ATHAD_DISPATCH(k_store, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, mask);
 }
+void launch_project(const Batch& b, int phase) {
+    ATHAD_DISPATCH(k_project, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, phase);
+}
 void launch_copy_field(const Batch& b, int dst, int src) {
     ATHAD_DISPATCH(k_copy_field, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, dst, src);
 }
@@ -3600,8 +3753,9 @@ ATHAD_DEV double cap(double v, double mag) {                          // :216-22
 // q_sat_col, :675-677, and initUpdraft's where its perturbation is t_add_u), [1] the air at rest
 // (the downdraft's, :935-936, :976-977, and initUpdraft's where its perturbation is zero).
 template <typename S>
-__global__ __launch_bounds__(256) void k_conv_sat(const MemberDesc* __restrict__ mds) {
+__global__ __launch_bounds__(256) void k_conv_sat(const MemberDesc* __restrict__ mds, int gate) {
     const MemberDesc& d = mds[blockIdx.z];
+    if (gated_off(d, gate)) return;
     const Consts& K = d.K;
     const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
     if (q >= K.n) return;
@@ -3613,9 +3767,10 @@ __global__ __launch_bounds__(256) void k_conv_sat(const MemberDesc* __restrict__
 }
 
 template <typename S>
-__global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __restrict__ mds) {
+__global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __restrict__ mds, int gate) {
     using namespace conv;
     const MemberDesc& d = mds[blockIdx.z];
+    if (gated_off(d, gate)) return;
     const Consts& K = d.K;
     const int im = K.im, jm = K.jm, km = K.km;
     const int col = blockIdx.x * 64 + threadIdx.x;
@@ -4050,15 +4205,15 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
     }
 }
 
-void launch_moist_convection(const Batch& b) {
+void launch_moist_convection(const Batch& b, int gate) {
     const dim3 g((unsigned)((b.max_jm * b.max_km + 63) / 64), 1, b.nmem);
     const dim3 gs((unsigned)((b.max_n + 255) / 256), 1, b.nmem);
     if (b.fp32) {
-        hipLaunchKernelGGL(k_conv_sat<float>, gs, dim3(256), 0, b.stream, b.d_md);
-        hipLaunchKernelGGL(k_moist_convection<float>, g, dim3(64), 0, b.stream, b.d_md);
+        hipLaunchKernelGGL(k_conv_sat<float>, gs, dim3(256), 0, b.stream, b.d_md, gate);
+        hipLaunchKernelGGL(k_moist_convection<float>, g, dim3(64), 0, b.stream, b.d_md, gate);
     } else {
-        hipLaunchKernelGGL(k_conv_sat<double>, gs, dim3(256), 0, b.stream, b.d_md);
-        hipLaunchKernelGGL(k_moist_convection<double>, g, dim3(64), 0, b.stream, b.d_md);
+        hipLaunchKernelGGL(k_conv_sat<double>, gs, dim3(256), 0, b.stream, b.d_md, gate);
+        hipLaunchKernelGGL(k_moist_convection<double>, g, dim3(64), 0, b.stream, b.d_md, gate);
     }
 }
 

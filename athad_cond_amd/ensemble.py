@@ -69,11 +69,13 @@ class Ensemble(Integration):
         super().__init__(list(grids), [member_params(s) for s in states], precision=precision, order=order)
 
 
-def run_ensemble(n_members=32, grid=(41, 91, 181), steps=50, warmup=4, precision="fp64", seed=2024):
-    """Times `steps` iterations of an n-member ensemble; returns aggregate cell-updates/s."""
+def run_ensemble(n_members=32, grid=(41, 91, 181), steps=50, warmup=4, precision="fp64", seed=2024,
+                 order="jacobi"):
+    """Times `steps` iterations of an n-member ensemble; returns aggregate cell-updates/s. In the
+    lex order the members share every hyperplane launch."""
     states = draw_members(n_members, seed)
     cells = n_members * grid[0] * grid[1] * grid[2]
-    with Ensemble(grid, states, precision=precision) as ens:
+    with Ensemble(grid, states, precision=precision, order=order) as ens:
         ens.init_state(seed)
         ens.run(warmup, graph=True)
         ens.sync()
@@ -82,5 +84,5 @@ def run_ensemble(n_members=32, grid=(41, 91, 181), steps=50, warmup=4, precision
         ens.sync()
         dt = time.perf_counter() - t0
         finite = all(np.isfinite(ens.get("t", m)).all() for m in range(n_members))
-    return {"members": n_members, "grid": list(grid), "steps": steps, "precision": precision,
+    return {"members": n_members, "grid": list(grid), "steps": steps, "precision": precision, "order": order,
             "cells_per_s": cells * steps / dt, "ms_per_step": dt / steps * 1e3, "finite": finite}

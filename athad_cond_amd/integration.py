@@ -99,6 +99,10 @@ class Integration:
         return a.value, b.value
 
     # ---- operators (run_3D_loop call sites) ------------------------------------------
+    def project_initial_velocity(self, n_sweeps: int = 200):
+        """PressureSolverAtm::project_initial_velocity (PressureSolverAtm.h:686-767)."""
+        self._chk(L.lib().athad_project_initial_velocity(self.h, n_sweeps))
+
     def poisson_sweep(self):
         self._chk(L.lib().athad_poisson_sweep(self.h))
 
@@ -206,6 +210,10 @@ class Integration:
     def set_ice_scheme(self, on: bool = True):
         """CategoryIceScheme = 2 inside every later step's moist block."""
         self._chk(L.lib().athad_set_ice_scheme(self.h, 1 if on else 0))
+
+    def set_moist_convection(self, on: bool = True):
+        """Moist convection in the moist block after the ice scheme (switched on with it)."""
+        self._chk(L.lib().athad_set_moist_convection(self.h, 1 if on else 0))
 
     def moist_convection(self):
         """MoistConvection::run on every member's current fields (MoistConvection.h:106-140)."""

@@ -90,14 +90,17 @@ def order_spread(A, steps=100, seed=1234):
     return out
 
 
-def run_loop(A, precision, steps, warmup, seed=1234, graph=True):
-    """The iteration with both operators built beside the scoped step inside it: the
-    two-category ice scheme in the moist block and mode-2 radiation with the skin-temperature
-    step after the store (run_3D_loop without moist convection and output)."""
+def run_loop(A, precision, steps, warmup, seed=1234, graph=True, convection=False):
+    """The iteration with the operators built beside the scoped step inside it: the
+    two-category ice scheme in the moist block (with moist convection after it when
+    `convection`) and mode-2 radiation with the skin-temperature step after the store
+    (run_3D_loop without output)."""
     n = GRID[0] * GRID[1] * GRID[2]
     with A.Integration(GRID, precision=precision) as integ:
         integ.init_state(seed)
         integ.set_ice_scheme(True)
+        if convection:
+            integ.set_moist_convection(True)
         integ.set_radiation(True)
         integ.run(warmup, graph=graph)
         integ.sync()
@@ -211,23 +214,52 @@ def calibrate_copy(mib=512, reps=5):
                       "bytes_written_per_launch": y.numel() * 8, "launches": reps}))
 
 
-def cpu_baseline(seconds_budget=20.0):
-    """The oracle (clean-room CPU restatement of the same step) on this host's cores."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline():
+    """The reference's CPU path on this host's cores, timed on a bounded sample of the same
+    workload (41 x 181 x 361 from the nominal state, FP64). The reference source does not travel
+    here; the oracle is its clean-room restatement, pinned to it to ~1e-12 over 100 iterations
+    at this grid. Three forms: the faithful one (the reference's single-thread sweep with its
+    O(jm + km) distance-to-land scans in the SST closure, RHS_Atm_Turb.cpp:640-671) at one
+    thread, as OMP_NUM_THREADS=1 runs it; the same at all cores (its sweep taken plane by plane,
+    bit-identical, so the result is still the reference's); and the scan-free restatement (the
+    scans' closed form min(i, j, k)) at all cores, a stronger baseline. `value` is the
+    faithful form at all cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    st = O.OracleState(*GRID, seed=1234)
-    st.run(1, "jacobi")
     n = GRID[0] * GRID[1] * GRID[2]
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        st.step("jacobi")
-        steps += 1
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    def timed(threads, scan, order, steps):
+        O.set_threads(threads)
+        st = O.OracleState(*GRID, seed=1234)
+        st.set_sst_scan(scan)
+        t0 = time.perf_counter()
+        st.run(steps, order)
         el = time.perf_counter() - t0
-        if el > seconds_budget or steps >= 16:
-            break
-    return {"value": n * steps / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} scoped steps of 41x181x361 (Jacobi order, FP64, OpenMP) after 1 warmup step"}
+        O.set_threads(0)
+        return {"value": n * steps / el, "unit": "cell-updates/s", "threads": threads, "steps": steps,
+                "s_per_step": el / steps}
+
+    one = timed(1, True, "lex", 1)
+    full = timed(cores, True, "lex_planes", 2)
+    fast = timed(cores, False, "lex_planes", 4)
+    return {"value": full["value"], "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": ("the first 2 scoped iterations of 41x181x361 from the nominal state, FP64, the "
+                       "reference's order with its SST distance-to-land scans, OpenMP over the cells of "
+                       "each hyperplane"),
+            "cpu": cpu_model(), "nproc": os.cpu_count(), "omp_num_threads": cores,
+            "compiler": "g++ -O2 -std=c++17 -fopenmp -ffp-contract=off (oracle/Makefile)",
+            "faithful_1_thread": one, "faithful_all_cores": full, "scan_free_all_cores": fast}
 
 
 def main():
@@ -245,6 +277,8 @@ def main():
                     help="only the FP64 iteration with the ice scheme and radiation inside it")
     ap.add_argument("--ice-only", action="store_true",
                     help="only the FP64 step with the two-category ice scheme in the moist block")
+    ap.add_argument("--lex-batched-only", action="store_true",
+                    help="only the lex order with 1 and 32 members sharing the plane launches")
     ap.add_argument("--orders-only", action="store_true",
                     help="only the lex / jacobi / symmetric spread over 100 iterations")
     ap.add_argument("--mixed-only", action="store_true",
@@ -264,6 +298,11 @@ def main():
     graph = not args.no_graph
     if args.next_rows_only:
         print(json.dumps({"grid": list(GRID), **run_next_rows(A)}))
+        return
+    if args.lex_batched_only:
+        from athad_cond_amd.ensemble import run_ensemble
+        print(json.dumps({f"{m}x{g[0]}x{g[1]}x{g[2]}": run_ensemble(m, g, steps=8, warmup=2, order="lex")
+                          for m, g in ((1, (41, 91, 181)), (32, (41, 91, 181)), (32, GRID))}))
         return
     if args.orders_only:
         print(json.dumps({"grid": list(GRID), **order_spread(A, steps=args.steps)}))
@@ -328,9 +367,13 @@ def main():
         out["mixed"] = {"cells_per_s": mixed["cells_per_s"], "ms_per_step": mixed["ms_per_step"],
                         "speedup_vs_fp64": mixed["cells_per_s"] / fp64["cells_per_s"],
                         "rel_l2_vs_fp64": err, "max_rel_l2": max(err.values())}
+        from athad_cond_amd.ensemble import run_ensemble
         out["orders"] = order_spread(A, steps=100)
         out["lex"] = {"cells_per_s": out["orders"]["lex"]["cells_per_s"],
                       "ms_per_step": out["orders"]["lex"]["ms_per_step"], "steps": 100}
+        # the reference order with members sharing every hyperplane launch
+        out["lex_batched"] = {f"{m}x{g[0]}x{g[1]}x{g[2]}": run_ensemble(m, g, steps=8, warmup=2, order="lex")
+                              for m, g in ((1, (41, 91, 181)), (32, (41, 91, 181)), (32, GRID))}
         out["jacobi_vs_lex_rel_l2"] = out["orders"]["jacobi_vs_lex_rel_l2"]
         out["symmetric_vs_lex_rel_l2"] = out["orders"]["symmetric_vs_lex_rel_l2"]
         out["ice_scheme"] = run_ice(A, "fp64", args.steps, args.warmup, graph=graph)
@@ -340,16 +383,21 @@ def main():
         out["loop_with_ice_and_radiation"]["workload"] = (
             "FP64 Jacobi iteration at 41x181x361 with the ice scheme in the moist block and radiation "
             "with the skin-temperature step after the store, all on the device (hipGraph)")
+        out["configured_loop"] = run_loop(A, "fp64", max(8, args.steps // 4), 4, graph=graph, convection=True)
+        out["configured_loop"]["workload"] = (
+            "FP64 Jacobi iteration at 41x181x361 with the ice scheme and moist convection in the moist block "
+            "and radiation after the store (run_3D_loop without output), hipGraph")
         out["next_rows"] = run_next_rows(A)
-        from athad_cond_amd.ensemble import run_ensemble
         out["ensemble"] = {prec: run_ensemble(32, (41, 91, 181), steps=40, precision=prec)
                            for prec in ("fp64", "mixed")}
         from athad_cond_amd import serving
-        reqs = serving.poisson_requests(24, rate_per_s=4.0, seed=99, steps=50)
+        reqs = serving.poisson_requests(100, rate_per_s=4.0, seed=99, steps=50)
         server = serving.Server(max_slots=4, chunk=4)
         out["serving"] = server.run(reqs)
         server.close()
-        out["serving"]["arrivals"] = "Poisson, mean 4 req/s, seed 99; 50 steps each; grids 21x91x181, 41x181x361, 61x361x721; mixed with FP64 fallback"
+        out["serving"]["arrivals"] = ("100 requests, Poisson, mean 4 req/s, seed 99; 50 steps each; grids "
+                                      "21x91x181, 41x181x361, 61x361x721; mixed with FP64 fallback; p99 is the "
+                                      "99th percentile of the 100 latencies (linear interpolation)")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

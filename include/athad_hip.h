@@ -101,13 +101,14 @@ athad_status athad_member_from_surface(double p_surf_bar, double t_sea, double x
  * buffers, scratch, the tables and reductions. Not counted, because they are allocated only
  * once their operator is first used: the ice-scheme arrays (10 fields in the storage
  * precision), the radiation outputs (2 FP64 fields and jm*km) and the moist-convection arrays
- * (34 FP64 fields, jm*km and jm doubles). */
+ * (36 FP64 fields, jm*km and jm doubles). */
 int64_t athad_footprint_bytes(const athad_grid* grid, int precision);
 
 /* --- state transfer ---------------------------------------------------------------- */
 
 /* Nominal initial state for every member (init recipe of RunTimeSlice's thermodynamic
- * chain, cAtmosphereModel.cpp:527-568, plus the benchmark's velocity/turbulence fields);
+ * chain, cAtmosphereModel.cpp:527-568, plus the benchmark's velocity/turbulence fields, the
+ * boundaries and the projection of the initial velocity, :665, with 200 sweeps);
  * member m uses seed + m. */
 athad_status athad_init_state(athad_ctx* ctx, uint64_t seed);
 /* Contiguous FP64 host arrays [im][jm][km] (jm x km for 2-D fields). */
@@ -133,6 +134,11 @@ athad_status athad_get_iter(athad_ctx* ctx, int member, int* iter_n, int* total_
 
 /* --- the five call sites of run_3D_loop and their glue ------------------------------ */
 
+/* PressureSolverAtm(*this).project_initial_velocity(n_sweeps)  (cAtmosphereModel.cpp:665;
+ * PressureSolverAtm.h:686-767): the velocity as the Poisson source, n_sweeps sweeps of the
+ * solver, the gradient correction v -= grad p_dyn of the interior, p_dyn and aux_u/v/w cleared.
+ * athad_init_state runs it with 200 sweeps, as RunTimeSlice does. */
+athad_status athad_project_initial_velocity(athad_ctx* ctx, int n_sweeps);
 /* PressureSolverAtm(*this).run()  (cAtmosphereModel.cpp:1158; PressureSolverAtm.h:27-669):
  * one red-black sweep of p_dyn with its boundaries and ceiling. */
 athad_status athad_poisson_sweep(athad_ctx* ctx);
@@ -315,11 +321,24 @@ typedef enum athad_conv_field {
  * precipitation P_conv, and the tendencies MC_t, MC_q, MC_v, MC_w of rhsForcing. The arrays the
  * reference does not clear between calls (E_u, D_u, E_d, D_d, the updraft / downdraft winds,
  * the MC_* tendencies, the level markers) carry over as there. computeCAPE's 1-D diagnostic is
- * not produced. Arrays are allocated, zero, at the first call (the 32 arrays and two per-cell
- * saturation fractions: 34 * im*jm*km * 8 bytes per member, plus jm*km and jm doubles). The zonal circle factor cos(theta_eq - theta_j) of the entrainment gradients is taken
+ * not produced. Arrays are allocated, zero, at the first call (the 32 arrays, two per-cell
+ * saturation fractions and two damping temporaries: 36 * im*jm*km * 8 bytes per member, plus
+ * jm*km and jm doubles). The zonal circle factor cos(theta_eq - theta_j) of the entrainment gradients is taken
  * about the equator row (jm - 1) / 2 (MoistConvection.h:239 takes row 90, the equator of its
  * 181-row grid). Grids need jm, km, im >= 3 (ATHAD_ERR_ARG otherwise). */
 athad_status athad_moist_convection(athad_ctx* ctx);
+/* The configured moist block (on != 0): every later moist iteration of athad_step / athad_run
+ * runs, after the ice scheme (which this switches on too) and its flux damping, the moist
+ * convection call site of run_3D_loop (atmosphere/cAtmosphereModel.cpp:1247-1262):
+ * MoistConvection(*this).run(iter_n), then damp_wiggles of P_conv, E_u, M_u, q_v_u, q_c_u, E_d,
+ * M_d, q_v_d, MC_t, MC_v, MC_w, and precipitationSum with P_conv (UtilsAtm.h:52-64). Once the
+ * convection arrays exist, RK4 adds coeff_MC_t MC_t, coeff_MC_vel MC_v and MC_w, coeff_MC_q MC_q
+ * to the tendencies of t, v, w, c (atmosphere/RHS_Atm_Turb.cpp:370-372, 964, 1023, 1042, 1061),
+ * and the evaporation step writes the Evaporation diagnostic the scheme's surface-flux trigger
+ * reads (Dalton's law, atmosphere/ThermoAtm.h:217-240). on = 0 leaves the ice scheme as it is;
+ * athad_set_ice_scheme(ctx, 0) stops both. Allocates the convection arrays (see
+ * athad_moist_convection). */
+athad_status athad_set_moist_convection(athad_ctx* ctx, int on);
 /* Contiguous FP64 host arrays [im][jm][km] of one convection array, or [jm][km] for
  * ATHAD_CONV_EVAPORATION. */
 athad_status athad_set_conv_field(athad_ctx* ctx, int member, int which, const double* src);

@@ -204,11 +204,17 @@ struct orc_state {
     // ice scheme (ORC_MP_* order); allocated, zero, at first use
     std::vector<std::vector<double>> mp;
     bool ice_on = false;               // the ice scheme runs inside the moist block
+    bool conv_on = false;              // and moist convection after it (cAtmosphereModel.cpp:1247-1262)
     // moist-convection arrays (ORC_MC_* order) and the evaporation diagnostic [mm/d] per
     // column that its surface-flux perturbation reads; allocated, zero, at first use
     std::vector<std::vector<double>> mc;
     std::vector<double> evap;
     int iter_n = 0, total_iter = 0;
+    // the reference's O(jm + km) distance-to-land scans of the SST closure over the land mask
+    // h (all water here) instead of their closed form min(i, j, k); for timing the reference's
+    // own cost only, the results are the same
+    bool sst_scan = false;
+    std::vector<double> h;
 
     inline size_t ix(int i, int j, int k) const { return ((size_t)i * jm + j) * km + k; }
     inline double* F(int id) { return f[id].data(); }
@@ -346,15 +352,32 @@ void evaporation(S& s) {
     const double r = std::exp(-1.0);
     const double w_norm = (1.0 - r) / (1.0 - std::pow(r, n_spread + 1));
     double *t = s.F(ORC_T), *c = s.F(ORC_C), *co2 = s.F(ORC_CO2), *ps = s.F(ORC_P_STAT);
+    const double *u = s.F(ORC_U), *v = s.F(ORC_V), *w = s.F(ORC_W);
+    // the Evaporation diagnostic [mm/d] moist convection reads, held once its arrays exist
+    double* ev = s.evap.empty() ? nullptr : s.evap.data();
     #pragma omp parallel for collapse(2) schedule(static)
     for (int j = 0; j < s.jm; j++)
         for (int k = 0; k < s.km; k++) {
             const size_t i0 = s.ix(0, j, k);
-            if (t[i0] * P.t_0 >= kTcrit) continue;
+            if (t[i0] * P.t_0 >= kTcrit) {
+                if (ev) ev[(size_t)j * s.km + k] = 0.0;   // ThermoAtm.h:179-185
+                continue;
+            }
             const double T_s = dmax(180.0, t[i0] * P.t_0);
             const double p_s = ps[i0];
             const double M_nw = M_nonwater(c[i0], co2[i0], P.comp.M_bg);
-            const double q_sea = q_sat(p_auto(T_s), p_s, M_nw);
+            const double E_s = p_auto(T_s);
+            const double q_sea = q_sat(E_s, p_s, M_nw);
+            if (ev) {
+                // Dalton's law on the saturation deficit (ThermoAtm.h:217-240) with
+                // AtomUtils::C_Dalton (lib/Utils.cpp:157-172, coeff_Dalton 0.7)
+                const double vel = std::sqrt((u[i0] * u[i0] + v[i0] * v[i0] + w[i0] * w[i0]) / 3.0) * P.u_0;
+                const double cD = 0.7 * std::sqrt(0.053 * 0.053 * vel / 10.0);
+                const double ca = c[i0];
+                const double e_air = (q_sea > 0.0 && q_sea < 1.0) ? ca * p_s * M_nw / (ca * M_nw + (1.0 - ca) * kMH2O)
+                                                                   : 0.0;
+                ev[(size_t)j * s.km + k] = cD * 24.0 * dmax(0.0, E_s - e_air);
+            }
             const double c_fix = c[i0];
             c[i0] = c_fix + (q_sea - c_fix) * w_norm;
             for (int i = 1; i <= n_spread && i < s.im; i++) {
@@ -641,6 +664,40 @@ void poisson(S& s) {
     }
 }
 
+// PressureSolverAtm::project_initial_velocity (PressureSolverAtm.h:686-767), called once in
+// RunTimeSlice (cAtmosphereModel.cpp:665): the velocity as the Poisson source, n sweeps of the
+// solver's run(), the gradient correction v -= grad p in the RHS metric, then p_dyn and aux
+// cleared.
+void project_initial_velocity(S& s, int n_sweeps) {
+    const int im = s.im, jm = s.jm, km = s.km;
+    double *u = s.F(ORC_U), *v = s.F(ORC_V), *w = s.F(ORC_W), *p = s.F(ORC_P_DYN);
+    double *au = s.F(ORC_AUX_U), *av = s.F(ORC_AUX_V), *aw = s.F(ORC_AUX_W);
+    for (size_t q = 0; q < s.n; q++) { au[q] = u[q]; av[q] = v[q]; aw[q] = w[q]; p[q] = 0.0; }   // :694-706
+    for (int it = 0; it < n_sweeps; it++) poisson(s);                                         // :711-713
+    const double inv_2dr = 1.0 / (2.0 * s.dr), inv_2dthe = 1.0 / (2.0 * s.dthe), inv_2dphi = 1.0 / (2.0 * s.dphi);
+    std::vector<double> sint(jm);
+    for (int j = 0; j < jm; j++) { sint[j] = std::sin(s.the[j]); if (sint[j] < 0.55) sint[j] = 0.55; }
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < im - 1; i++)                                                          // :726-749
+        for (int j = 1; j < jm - 1; j++) {
+            const double rm = s.rad[i];
+            const double exp_rm = 1.0 / (rm + 1.0);
+            const double rmet = s.metric_radius(rm);
+            const double inv_rm = 1.0 / rmet;
+            const double inv_rmsinthe = 1.0 / (rmet * sint[j]);
+            for (int k = 1; k < km - 1; k++) {
+                const size_t id = s.ix(i, j, k);
+                const double dpdr = (p[s.ix(i + 1, j, k)] - p[s.ix(i - 1, j, k)]) * inv_2dr;
+                const double dpdthe = (p[s.ix(i, j + 1, k)] - p[s.ix(i, j - 1, k)]) * inv_2dthe;
+                const double dpdphi = (p[s.ix(i, j, k + 1)] - p[s.ix(i, j, k - 1)]) * inv_2dphi;
+                u[id] -= dpdr * exp_rm;
+                v[id] -= dpdthe * inv_rm;
+                w[id] -= dpdphi * inv_rmsinthe;
+            }
+        }
+    for (size_t q = 0; q < s.n; q++) { p[q] = 0.0; au[q] = 0.0; av[q] = 0.0; aw[q] = 0.0; }   // :752-763
+}
+
 // ---------------------------------------------------------------------------------------
 // TurbulenceAtm::run: compute_vel_star (:88-102), compute_sources (:323-477, serial in
 // (j,k,i) order = the OMP_NUM_THREADS=1 sweep), compute_k_omega_SST (:656-773),
@@ -913,6 +970,53 @@ void bc_all(S& s) {
                 }
         }
     }
+    // Moist-convection arrays, where they exist. bcRadius: pattern A (cubic at both ends) for
+    // q_v_u, q_c_u, u_u, v_u, w_u, s, s_u, s_d, M_u, M_d (BC_Atm.h:567-574); pattern B for P_conv
+    // (c43 x[1] - c13 x[2] at i=0 with the int constants 1 and 0, copy at the lid, :587-590,
+    // :679-683); pattern C (cubic at i=0, copy at the lid) for c_u, e_d, e_l, e_p, g_p (:602-606,
+    // :733-737), then q_v_u, q_c_u at i=0 floored at zero (:741-742).
+    // bcTheta's copy (:769-775) and bcPhi's seam average (:810-818) for all of them except M_u,
+    // M_d and P_conv, which neither list names.
+    if (!s.mc.empty()) {
+        const int cubic_both[10] = {ORC_MC_Q_V_U, ORC_MC_Q_C_U, ORC_MC_U_U, ORC_MC_V_U, ORC_MC_W_U,
+                                    ORC_MC_S, ORC_MC_S_U, ORC_MC_S_D, ORC_MC_M_U, ORC_MC_M_D};
+        const int cubic_copy[5] = {ORC_MC_C_U, ORC_MC_E_D_EVAP, ORC_MC_E_L, ORC_MC_E_P, ORC_MC_G_P};
+        const int sides[13] = {ORC_MC_Q_V_U, ORC_MC_Q_C_U, ORC_MC_U_U, ORC_MC_V_U, ORC_MC_W_U, ORC_MC_S,
+                               ORC_MC_S_U, ORC_MC_S_D, ORC_MC_C_U, ORC_MC_E_D_EVAP, ORC_MC_E_L, ORC_MC_E_P,
+                               ORC_MC_G_P};
+        const double c43 = 1.0, c13 = 0.0;
+        for (int j = 0; j < jm; j++)
+            for (int k = 0; k < km; k++) {
+                auto X = [&](int q, int i) -> double& { return s.mc[q][s.ix(i, j, k)]; };
+                for (int q : cubic_both) {
+                    X(q, 0) = X(q, 3) - 3.0 * X(q, 2) + 3.0 * X(q, 1);
+                    X(q, iml) = X(q, iml - 3) - 3.0 * X(q, iml - 2) + 3.0 * X(q, iml - 1);
+                }
+                X(ORC_MC_P_CONV, 0) = c43 * X(ORC_MC_P_CONV, 1) - c13 * X(ORC_MC_P_CONV, 2);
+                X(ORC_MC_P_CONV, iml) = X(ORC_MC_P_CONV, iml - 1);
+                for (int q : cubic_copy) {
+                    X(q, 0) = X(q, 3) - 3.0 * X(q, 2) + 3.0 * X(q, 1);
+                    X(q, iml) = X(q, iml - 1);
+                }
+                // updraft moisture at the surface non-negative (:741-742)
+                if (X(ORC_MC_Q_V_U, 0) < 0.0) X(ORC_MC_Q_V_U, 0) = 0.0;
+                if (X(ORC_MC_Q_C_U, 0) < 0.0) X(ORC_MC_Q_C_U, 0) = 0.0;
+            }
+        for (int q : sides) {
+            double* x = s.mc[q].data();
+            for (int k = 0; k < km; k++)
+                for (int i = 0; i < im; i++) {
+                    x[s.ix(i, 0, k)] = x[s.ix(i, 1, k)];
+                    x[s.ix(i, jm - 1, k)] = x[s.ix(i, jm - 2, k)];
+                }
+            for (int i = 0; i < im; i++)
+                for (int j = 0; j < jm; j++) {
+                    const double v0 = c43 * x[s.ix(i, j, 1)] - c13 * x[s.ix(i, j, 2)];
+                    const double ve = c43 * x[s.ix(i, j, km - 2)] - c13 * x[s.ix(i, j, km - 3)];
+                    x[s.ix(i, j, 0)] = x[s.ix(i, j, km - 1)] = (v0 + ve) * 0.5;
+                }
+        }
+    }
     (void)t;
 }
 
@@ -1008,11 +1112,29 @@ void rhs_cell(const S& s, const Geo& g, const RhsConsts& K, int i, int j, int k,
     {
         const double sig_k1 = 1.176, sig_k2 = 1.0, sig_w1 = 2.0, sig_w2 = 1.168, bet_star = 0.09;
         double d_close = (double)i;
-        const double dj = (double)j, dk = (double)k;
-        if (dj <= d_close) d_close = dj;
-        if (dj <= d_close) d_close = dj;
-        if (dk <= d_close) d_close = dk;
-        if (dk <= d_close) d_close = dk;
+        double dj_s = (double)j, dj_n = (double)j, dk_e = (double)k, dk_w = (double)k;
+        if (s.sst_scan) {
+            // RHS_Atm_Turb.cpp:640-671: scans of the land mask along j and k from the cell (the
+            // same land test as AtomUtils::is_land); without land they leave the mount indices
+            // at 0, so every distance is j or k
+            auto land = [&](int ii, int jj, int kk) {
+                return std::fabs(s.h[s.ix(ii, jj, kk)] - 1.0) < 2.220446049250313e-16;
+            };
+            const int jm_ = s.jm, km_ = s.km;
+            double j_mount = 0.0, k_mount = 0.0;   // each carried from its first scan to its second
+            for (int jj = 0; jj < jm_ - 1; jj++) if (land(i, jj, k) || jj == jm_ - 1) j_mount = jj;
+            dj_s = std::fabs((double)j - j_mount);
+            for (int jj = jm_ - 1; jj > 0; jj--) if (land(i, jj, k) || jj == 0) j_mount = jj;
+            dj_n = std::fabs((double)j - j_mount);
+            for (int kk = 0; kk < km_ - 1; kk++) if (land(i, j, kk) || kk == km_ - 1) k_mount = kk;
+            dk_e = std::fabs((double)k - k_mount);
+            for (int kk = km_ - 1; kk > 0; kk--) if (land(i, j, kk) || kk == 0) k_mount = kk;
+            dk_w = std::fabs((double)k - k_mount);
+        }
+        if (dj_s <= d_close) d_close = dj_s;
+        if (dj_n <= d_close) d_close = dj_n;
+        if (dk_e <= d_close) d_close = dk_e;
+        if (dk_w <= d_close) d_close = dk_w;
         const double cross = d1r[FK] * d1r[FD] + d1t[FK] * g.inv_rm * d1t[FD] * g.inv_rm
                            + d1p[FK] * g.inv_rmsinthe * d1p[FD] * g.inv_rmsinthe;
         const double CD_kw = dmax((2.0 * sig_w2) / dis * cross, 1.0e-20);
@@ -1077,8 +1199,19 @@ void rhs_cell(const S& s, const Geo& g, const RhsConsts& K, int i, int j, int k,
 
     const double dpdr_exp = dpdr * g.exp_rm, dpdthe_invrm = dpdthe * g.inv_rm, dpdphi_invrs = dpdphi * g.inv_rmsinthe;
     double rt = pressure_t - tr[FT] + diffusion_t;
+    // Moist-convection tendencies (RHS_Atm_Turb.cpp:370-372, 964, 1023, 1042, 1061): zero until the
+    // scheme has run, so they are added only where its arrays exist.
+    double mc_t = 0.0, mc_v = 0.0, mc_w = 0.0, mc_q = 0.0;
+    const bool have_mc = !s.mc.empty();
+    if (have_mc) {
+        mc_t = (s.P.L_atm / (s.P.u_0 * s.P.t_0)) * s.mc[ORC_MC_MC_T][id];
+        mc_v = (s.P.L_atm / (s.P.u_0 * s.P.u_0)) * s.mc[ORC_MC_MC_V][id];
+        mc_w = (s.P.L_atm / (s.P.u_0 * s.P.u_0)) * s.mc[ORC_MC_MC_W][id];
+        mc_q = (s.P.L_atm / (s.P.u_0 * s.P.c_0)) * s.mc[ORC_MC_MC_Q][id];
+        rt = rt + mc_t;
+    }
     // Microphysics sources (RHS_Atm_Turb.cpp:963-966, 1058-1070); zero until the ice scheme
-    // has run, so they are added only where its arrays exist. MC_* (moist convection) are zero.
+    // has run, so they are added only where its arrays exist.
     double src_c = 0.0, src_cl = 0.0, src_i = 0.0;
     if (!s.mp.empty()) {
         const double rh = s.f[ORC_R_HUMID][id];
@@ -1101,9 +1234,15 @@ void rhs_cell(const S& s, const Geo& g, const RhsConsts& K, int i, int j, int k,
     }
     rhs[FT] = rt;
     rhs[FU] = -dpdr_exp - tr[FU] + diffusion_u + buoy_ramp_dt * (cen[FT] - t_ref_i);
-    rhs[FV] = -dpdthe_invrm - tr[FV] + diffusion_v + K.force_nd * coriolis_the - surf_drag * v;
-    rhs[FW] = -dpdphi_invrs - tr[FW] + diffusion_w + K.force_nd * coriolis_phi - surf_drag * w;
-    rhs[FC] = -tr[FC] + diffusion_c + src_c;
+    if (have_mc) {
+        rhs[FV] = -dpdthe_invrm - tr[FV] + diffusion_v + K.force_nd * coriolis_the + mc_v - surf_drag * v;
+        rhs[FW] = -dpdphi_invrs - tr[FW] + diffusion_w + K.force_nd * coriolis_phi + mc_w - surf_drag * w;
+        rhs[FC] = -tr[FC] + diffusion_c + src_c + mc_q;
+    } else {
+        rhs[FV] = -dpdthe_invrm - tr[FV] + diffusion_v + K.force_nd * coriolis_the - surf_drag * v;
+        rhs[FW] = -dpdphi_invrs - tr[FW] + diffusion_w + K.force_nd * coriolis_phi - surf_drag * w;
+        rhs[FC] = -tr[FC] + diffusion_c + src_c;
+    }
     rhs[FCL] = -tr[FCL] + diffusion_cl + src_cl;
     rhs[FI] = -tr[FI] + diffusion_i + src_i;
     rhs[FG] = -tr[FG] + diffusion_g;
@@ -1324,6 +1463,7 @@ void store(S& s) {
 }
 
 void ice_scheme(S& s);
+void moist_convection(S& s);
 
 void moist_block(S& s) {   // cAtmosphereModel.cpp:1185-1434 (scoped rows)
     satadj(s);
@@ -1335,9 +1475,17 @@ void moist_block(S& s) {   // cAtmosphereModel.cpp:1185-1434 (scoped rows)
         // :1230-1244 with CategoryIceScheme = 2, then precipitationSum (:1263, UtilsAtm.h:52-67)
         // with P_graupel and the moist convection's P_conv zero
         ice_scheme(s);
+        if (s.conv_on) {
+            // :1247-1262 MoistConvection::run, then the damp_wiggles of eleven of its arrays
+            moist_convection(s);
+            for (int q : {ORC_MC_P_CONV, ORC_MC_E_U, ORC_MC_M_U, ORC_MC_Q_V_U, ORC_MC_Q_C_U, ORC_MC_E_D, ORC_MC_M_D,
+                          ORC_MC_Q_V_D, ORC_MC_MC_T, ORC_MC_MC_V, ORC_MC_MC_W})
+                damp_wiggles(s, s.mc[q].data());
+        }
         const double *pr = s.mp[ORC_MP_P_RAIN].data(), *ps = s.mp[ORC_MP_P_SNOW].data();
+        const double* pc = s.mc.empty() ? nullptr : s.mc[ORC_MC_P_CONV].data();
         double* pt = s.mp[ORC_MP_PRECIP].data();
-        for (size_t q = 0; q < s.n; q++) pt[q] = pr[q] + ps[q] + 0.0 + 0.0;
+        for (size_t q = 0; q < s.n; q++) pt[q] = pr[q] + ps[q] + 0.0 + (pc ? pc[q] : 0.0);
     }
     moist_clamp(s);
     densities(s, false);
@@ -1481,6 +1629,8 @@ void init_state(S& s, uint64_t seed, int stop_after_thermo) {
     turbulence(s);
     for (int fld : {ORC_TKE, ORC_DIS, ORC_NUE, ORC_PROD, ORC_TKE_SRC, ORC_DIS_SRC}) damp_wiggles(s, s.F(fld));
     bc_all(s);
+    // RunTimeSlice :665 projection of the initial velocity (200 sweeps)
+    project_initial_velocity(s, 200);
     // run_3D_loop :750-753 Held-Suarez target; RunTimeSlice :667 store
     s.f[ORC_T_EQ] = s.f[ORC_T];
     store(s);
@@ -2334,6 +2484,10 @@ void orc_set_ice_scheme(orc_state* s, int on) {
     if (s->ice_on) ensure_mp(*s);
 }
 void orc_ice_store(orc_state* s) { ice_store(*s); }
+void orc_set_moist_convection(orc_state* s, int on) {
+    s->conv_on = on != 0;
+    if (on) orc_set_ice_scheme(s, 1);
+}
 void orc_moist_convection(orc_state* s) { moist_convection(*s); }
 double* orc_mc_ptr(orc_state* s, int which) {
     if (which < 0 || which >= ORC_NMC) return nullptr;
@@ -2348,6 +2502,10 @@ double* orc_mp_ptr(orc_state* s, int which) {
     if (which < 0 || which >= ORC_NMP) return nullptr;
     ensure_mp(*s);
     return s->mp[which].data();
+}
+void orc_set_sst_scan(orc_state* s, int on) {
+    s->sst_scan = on != 0;
+    if (s->sst_scan && s->h.size() != s->n) s->h.assign(s->n, 0.0);
 }
 int orc_set_threads(int n) {
 #ifdef _OPENMP
@@ -2384,6 +2542,7 @@ void orc_op(orc_state* s, int op, int order) {
             break;
         case ORC_OP_STORE: store(*s); break;
         case ORC_OP_LEVEL_MEAN: level_mean(*s); break;
+        case ORC_OP_PROJECT: project_initial_velocity(*s, 200); break;
         default: break;
     }
 }

@@ -48,6 +48,7 @@ enum orc_op {
     ORC_OP_POISSON = 0, ORC_OP_SATADJ, ORC_OP_DAMP_MOIST, ORC_OP_MOIST_CLAMP, ORC_OP_DENSITIES,
     ORC_OP_EVAPORATION, ORC_OP_TURBULENCE, ORC_OP_DAMP_TURB, ORC_OP_BC, ORC_OP_RK4,
     ORC_OP_POLAR, ORC_OP_RADIAL, ORC_OP_STORE, ORC_OP_LEVEL_MEAN,
+    ORC_OP_PROJECT,   // PressureSolverAtm::project_initial_velocity(200)
     ORC_NOP
 };
 
@@ -63,7 +64,8 @@ typedef struct orc_state orc_state;
 
 // Creates the nominal initial state: RunTimeSlice's thermodynamic init chain
 // (cAtmosphereModel.cpp:527-568), then u,v,w = +-1e-3 m/s seeded and mirrored N-S, k = 1e-4 m2/s2,
-// omega = 1 1/s, one turbulence update and its damp_wiggles (:612-622), boundaries, t_eq = t, store.
+// omega = 1 1/s, one turbulence update and its damp_wiggles (:612-622), boundaries, the projection
+// of the initial velocity (:665, 200 Poisson sweeps and the gradient correction), t_eq = t, store.
 // member == NULL selects config_cond.xml's values.
 orc_state* orc_create(int im, int jm, int km, uint64_t seed, const orc_member* member);
 void       orc_destroy(orc_state* s);
@@ -85,6 +87,10 @@ double  orc_level_mean(const orc_state* s, int i);
 void orc_step(orc_state* s, int order);
 // n steps.
 void orc_run(orc_state* s, int n, int order);
+// on != 0: the SST closure finds its wall distance by the reference's distance-to-land scans
+// over an all-water mask (RHS_Atm_Turb.cpp:640-671, O(jm + km) per cell and stage) instead of
+// their closed form min(i, j, k). Same results; for timing the reference's own cost.
+void orc_set_sst_scan(orc_state* s, int on);
 // One operator, with the cadence flags of the step it would run in. For ORC_OP_RK4 the
 // ramps are taken from the current total_iter_count.
 void orc_op(orc_state* s, int op, int order);
@@ -129,6 +135,12 @@ enum orc_mc {
     ORC_NMC
 };
 void    orc_moist_convection(orc_state* s);
+// Puts moist convection in its place inside the moist block of every later step, after the ice
+// scheme (which on != 0 also switches on): MoistConvection::run, the damp_wiggles of P_conv, E_u,
+// M_u, q_v_u, q_c_u, E_d, M_d, q_v_d, MC_t, MC_v, MC_w, and P_conv in precipitationSum
+// (cAtmosphereModel.cpp:1247-1263, UtilsAtm.h:52-64). Wherever the convection arrays exist the
+// RHS adds MC_t, MC_v, MC_w, MC_q (RHS_Atm_Turb.cpp:370-372, 964, 1023, 1042, 1061).
+void    orc_set_moist_convection(orc_state* s, int on);
 double* orc_mc_ptr(orc_state* s, int which);      // im*jm*km doubles
 double* orc_evap_ptr(orc_state* s);               // jm*km doubles, the Evaporation diagnostic [mm/d]
 

@@ -23,7 +23,7 @@ FIELDS = [
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 FIELDS2D = ["vel_star", "t_top_init"]
 OPS = ["poisson", "satadj", "damp_moist", "moist_clamp", "densities", "evaporation",
-       "turbulence", "damp_turb", "bc", "rk4", "polar", "radial", "store", "level_mean"]
+       "turbulence", "damp_turb", "bc", "rk4", "polar", "radial", "store", "level_mean", "project"]
 ORDER = {"lex": 0, "jacobi": 1, "lex_planes": 2, "jacobi_reverse": 3}
 
 
@@ -64,6 +64,8 @@ def lib():
         L.orc_level_mean.argtypes = [vp, i]
         L.orc_step.argtypes = [vp, i]
         L.orc_run.argtypes = [vp, i, i]
+        L.orc_set_sst_scan.argtypes = [vp, i]
+        L.orc_set_moist_convection.argtypes = [vp, i]
         L.orc_set_threads.restype = i
         L.orc_set_threads.argtypes = [i]
         L.orc_op.argtypes = [vp, i, i]
@@ -167,6 +169,14 @@ class OracleState:
 
     def scalar(self, name):
         return lib().orc_scalar(self.h, name.encode())
+
+    def set_moist_convection(self, on=True):
+        """Moist convection in the moist block after the ice scheme (switched on with it)."""
+        lib().orc_set_moist_convection(self.h, 1 if on else 0)
+
+    def set_sst_scan(self, on=True):
+        """The reference's distance-to-land scans in the SST closure (same results, its cost)."""
+        lib().orc_set_sst_scan(self.h, 1 if on else 0)
 
     def step(self, order="lex"):
         lib().orc_step(self.h, ORDER[order])

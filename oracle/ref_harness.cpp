@@ -9,6 +9,7 @@
 //   mode = steps            n scoped iterations (the IN rows of run_3D_loop)
 //   mode = op:<name>        one operator: poisson satadj damp_moist moist_clamp densities
 //                           evaporation turbulence damp_turb bc rk4 polar radial store
+//                           project (project_initial_velocity(200), as RunTimeSlice calls it)
 //   mode = init             the reference's thermodynamic init chain on a blank state
 //   mode = radiation        run_3D_loop's mode-2 radiation once (MultiLayerRadiation, the
 //                           omega_rad blend of t, updateSkinTemperature)
@@ -21,6 +22,10 @@
 //                           block (after the damping of t ice c cloud), its flux damping and
 //                           precipitationSum (moist convection not run: P_conv stays zero);
 //                           the .mp files as for mode ice
+//   mode = steps_conv       n scoped iterations with the ice scheme and moist convection in the
+//                           moist block (cAtmosphereModel.cpp:1230-1263); reads and writes the
+//                           .mp, .mc and .evap files (ice-scheme arrays, convection arrays, the
+//                           Evaporation diagnostic)
 //   mode = conv             n calls of MoistConvection::run (its parameters as the model's
 //                           defaults set them: iter_prec 3, convection_perturbation 1,
 //                           convection_mode 1); reads <in.bin>.evap (the Evaporation diagnostic,
@@ -172,6 +177,7 @@ public:
         m.buoyancy_ramp = std::min(1.0, (double)m.total_iter_count / 300.0);
         if (name == "poisson") PressureSolverAtm(m).run();
         else if (name == "satadj") SaturationAdjustment(m).run();
+        else if (name == "project") PressureSolverAtm(m).project_initial_velocity(200);
         else if (name == "damp_moist") { damp(m, m.t); damp(m, m.ice); damp(m, m.c); damp(m, m.cloud); }
         else if (name == "moist_clamp") moist_clamp(m);
         else if (name == "densities") ThermoAtm(m).densities();
@@ -214,7 +220,7 @@ public:
     }
 
     // One iteration of run_3D_loop restricted to the scoped rows (cAtmosphereModel.cpp:1103-1647).
-    static void step(cAtmosphereModel& m, bool ice_scheme = false) {
+    static void step(cAtmosphereModel& m, bool ice_scheme = false, bool convection = false) {
         m.iter_n++;
         m.total_iter_count++;
         m.diffusion_ramp = 1.0;
@@ -225,6 +231,13 @@ public:
             op(m, "damp_moist");
             if (ice_scheme) {
                 ice(m);
+                if (convection) {
+                    // cAtmosphereModel.cpp:1247-1262
+                    MoistConvection(m).run(m.iter_n);
+                    for (Array* a : {&m.P_conv, &m.E_u, &m.M_u, &m.q_v_u, &m.q_c_u, &m.E_d, &m.M_d, &m.q_v_d,
+                                     &m.MC_t, &m.MC_v, &m.MC_w})
+                        damp(m, *a);
+                }
                 UtilsAtm(m).precipitationSum();
             }
             op(m, "moist_clamp");
@@ -332,6 +345,23 @@ public:
         }
     }
 
+    static bool load_evap(cAtmosphereModel& m, const std::string& fn) {
+        std::ifstream fe(fn, std::ios::binary);
+        std::vector<double> b2((size_t)m.jm * m.km);
+        fe.read((char*)b2.data(), b2.size() * 8);
+        if (!fe) return false;
+        for (int j = 0; j < m.jm; j++)
+            for (int k = 0; k < m.km; k++) m.Evaporation.y[j][k] = b2[(size_t)j * m.km + k];
+        return true;
+    }
+    static void save_evap(cAtmosphereModel& m, const std::string& fn) {
+        std::ofstream fo(fn, std::ios::binary);
+        std::vector<double> b2((size_t)m.jm * m.km);
+        for (int j = 0; j < m.jm; j++)
+            for (int k = 0; k < m.km; k++) b2[(size_t)j * m.km + k] = m.Evaporation.y[j][k];
+        fo.write((const char*)b2.data(), b2.size() * 8);
+    }
+
     static bool conv(cAtmosphereModel& m, const std::string& in, const std::string& out, int n) {
         std::ifstream fe(in + ".evap", std::ios::binary);
         std::vector<double> b2((size_t)m.jm * m.km);
@@ -392,6 +422,17 @@ int main(int argc, char** argv) {
             RadiationSelfTest::radiation(*m);
         }
         RadiationSelfTest::save_rad(*m, std::string(argv[2]) + ".rad");
+    } else if (mode == "steps_conv") {
+        // n iterations with the ice scheme and moist convection in the moist block; the ice
+        // arrays travel in .mp, the convection arrays in .mc, the Evaporation diagnostic in .evap
+        if (!RadiationSelfTest::load_mp(*m, std::string(argv[1]) + ".mp")) return 1;
+        if (!RadiationSelfTest::load_evap(*m, std::string(argv[1]) + ".evap")) return 1;
+        RadiationSelfTest::io_fields(*m, RadiationSelfTest::mc_fields(*m), std::string(argv[1]) + ".mc", true);
+        const int n = (argc > 4) ? std::atoi(argv[4]) : 1;
+        for (int q = 0; q < n; q++) RadiationSelfTest::step(*m, true, true);
+        RadiationSelfTest::save_mp(*m, std::string(argv[2]) + ".mp");
+        RadiationSelfTest::io_fields(*m, RadiationSelfTest::mc_fields(*m), std::string(argv[2]) + ".mc", false);
+        RadiationSelfTest::save_evap(*m, std::string(argv[2]) + ".evap");
     } else if (mode == "steps_ice") {
         if (!RadiationSelfTest::load_mp(*m, std::string(argv[1]) + ".mp")) return 1;
         const int n = (argc > 4) ? std::atoi(argv[4]) : 1;

@@ -11,6 +11,7 @@ in the tests from the same seed, so only outputs are stored.
     python tests/golden/make_ref_vectors.py --ice         # ref_ice.npz
     python tests/golden/make_ref_vectors.py --conv        # ref_conv.npz
     python tests/golden/make_ref_vectors.py --grids       # ref_grids.npz
+    python tests/golden/make_ref_vectors.py --conv-steps  # ref_conv_steps.npz
 
 --grids needs the harness built at the benchmark grids by oracle/Makefile.grid (a copy of
 the reference's sources with only the grid constants and the spacing changed):
@@ -37,9 +38,9 @@ SEED = 1234
 SPIN = 2          # Jacobi-order oracle steps that make the operator inputs
 NSAMPLE = 512
 OPS = ["poisson", "satadj", "damp_moist", "moist_clamp", "densities", "evaporation", "turbulence",
-       "damp_turb", "bc", "rk4", "polar", "radial"]
+       "damp_turb", "bc", "rk4", "polar", "radial", "project"]
 ORDER_FREE = {"poisson", "satadj", "damp_moist", "moist_clamp", "densities", "evaporation",
-              "damp_turb", "bc", "polar", "radial"}
+              "damp_turb", "bc", "polar", "radial", "project"}
 
 
 def sample_index(n, seed=99):
@@ -218,6 +219,70 @@ def conv_vectors():
     print("wrote", path)
 
 
+CONV_STEPS = 4
+CONV_MC_SEED = 33
+# convection tendencies and convective precipitation planted at the start of the coupled-step
+# vectors (uniform, seeded): the scheme does not clear MC_* where no column convects, so they
+# carry through the iterations into RK4 and the moist block's damping (P_conv it clears)
+CONV_MC_PLANT = (("MC_t", -1e-3, 1e-3), ("MC_q", -1e-6, 1e-6), ("MC_v", -1e-4, 1e-4), ("MC_w", -1e-4, 1e-4),
+                 ("P_conv", 0.0, 1e-3))
+
+
+def conv_planted_mc(grid):
+    """The 32 convection arrays (MC_FIELDS order, [q][i][j][k]) at the start of the coupled
+    steps: zero except the planted tendencies and P_conv."""
+    out = np.zeros((len(O.MC_FIELDS),) + tuple(grid))
+    rng = np.random.default_rng(CONV_MC_SEED)
+    for name, lo, hi in CONV_MC_PLANT:
+        out[O.MC_FIELDS.index(name)] = rng.uniform(lo, hi, grid)
+    return out
+
+
+def conv_steps_vectors():
+    """Four whole lex iterations of the configured moist block, the ice scheme and moist
+    convection in their places (cAtmosphereModel.cpp:1230-1263), two of them moist, from the
+    Jacobi-spun-up state with the planted condensate and the seeded Evaporation input, ice and
+    convection arrays zero at the start. Writes ref_conv_steps.npz: the state, the ice-scheme
+    arrays and the convection arrays after the four iterations."""
+    wd = tempfile.mkdtemp(prefix="athad_golden_convsteps_")
+    n = GRID[0] * GRID[1] * GRID[2]
+    idx = sample_index(n)
+    out = {"grid": np.array(GRID), "seed": np.array(SEED), "spin": np.array(SPIN), "plant": np.array(ICE_PLANT),
+           "evap_seed": np.array(CONV_EVAP_SEED), "steps": np.array(CONV_STEPS), "index": idx}
+    st = O.OracleState(*GRID, seed=SEED)
+    st.run(SPIN, "jacobi")
+    snap = st.snapshot()
+    mp = {k: np.zeros(GRID) for k in O.MP_FIELDS}
+    O.plant_microphysics(snap, mp, ICE_PLANT)
+    st = O.OracleState(*GRID, blank=True)
+    st.load(snap)
+    st.set_iter(SPIN, SPIN)
+    base, o = os.path.join(wd, "base.bin"), os.path.join(wd, "out.bin")
+    st.save_bin(base, iter_n=SPIN)
+    O.save_mp(base + ".mp", mp)
+    conv_evaporation(GRID).tofile(base + ".evap")
+    conv_planted_mc(GRID).tofile(base + ".mc")
+    subprocess.run([HARNESS, base, o, "steps_conv", str(CONV_STEPS)], check=True, cwd=wd,
+                   stdout=subprocess.DEVNULL, env=dict(os.environ, OMP_NUM_THREADS="1"))
+    d = dict(O.read_bin(o))
+    d.update(O.read_mp(o + ".mp", GRID))
+    mc = np.fromfile(o + ".mc").reshape(len(O.MC_FIELDS), n)
+    for q, name in enumerate(O.MC_FIELDS):
+        d["mc/" + name] = mc[q]
+    for name, a in d.items():
+        a = np.asarray(a).reshape(-1)
+        if a.size != n:      # the 2-D fields (vel_star, t_top_init)
+            continue
+        out[f"steps/{name}/sample"] = a[idx]
+        out[f"steps/{name}/sum"] = np.array(np.sum(a))
+        out[f"steps/{name}/l2"] = np.array(np.sqrt(np.sum(a * a)))
+    ev = np.fromfile(o + ".evap")
+    out["steps/evap"] = ev
+    path = os.path.join(HERE, "ref_conv_steps.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 # Whole lex-order integrations written by the reference at the benchmark grids and at its
 # built-in grid: (grid, seed, steps). 21 x 91 x 181 and 41 x 181 x 361 run on a copy of the
 # reference compiled at that grid (oracle/Makefile.grid).
@@ -292,6 +357,9 @@ def grid_vectors():
 
 
 def main():
+    if "--conv-steps" in sys.argv[1:]:
+        conv_steps_vectors()
+        return
     if "--grids" in sys.argv[1:]:
         grid_vectors()
         return

@@ -60,16 +60,22 @@ def gpu_jacobi():
 @pytest.fixture(scope="module")
 def oracle_lex(oracle):
     """The oracle after 100 lex iterations, swept plane by plane on the host threads."""
-    ref = oracle.OracleState(*C2, seed=SEED)
-    ref.run(STEPS, "lex_planes")
-    return ref
+    print("case oracle 41x181x361 lex n=100", flush=True)
+    return run_with_progress(oracle.OracleState(*C2, seed=SEED), "lex_planes", "41x181x361")
 
 
 @pytest.fixture(scope="module")
 def oracle_jacobi(oracle):
-    ref = oracle.OracleState(*C2, seed=SEED)
-    ref.run(STEPS, "jacobi")
-    return ref
+    print("case oracle 41x181x361 jacobi n=100", flush=True)
+    return run_with_progress(oracle.OracleState(*C2, seed=SEED), "jacobi", "41x181x361")
+
+
+def run_with_progress(state, order, tag):
+    """STEPS iterations of the oracle in chunks of ten, with a line after each chunk."""
+    for done in range(0, STEPS, 10):
+        state.run(min(10, STEPS - done), order)
+        print(f"  oracle {tag} {order}: {min(done + 10, STEPS)}/{STEPS}", flush=True)
+    return state
 
 
 def worst(fields, ref, names):

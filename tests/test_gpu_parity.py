@@ -58,11 +58,11 @@ def test_steps_match_oracle(oracle, order, grid, steps):
 
 
 OPS = ["poisson_sweep", "saturation_adjustment", "damp_moist", "densities", "turbulence", "boundary", "rk4",
-       "filters", "store"]
+       "filters", "store", "project_initial_velocity"]
 ORACLE_OP = {"poisson_sweep": ["poisson"], "saturation_adjustment": ["satadj"],
              "damp_moist": ["damp_moist", "moist_clamp"], "densities": ["densities"],
              "turbulence": ["turbulence", "damp_turb"], "boundary": ["bc"], "rk4": ["rk4"],
-             "filters": ["polar", "radial"], "store": ["store"]}
+             "filters": ["polar", "radial"], "store": ["store"], "project_initial_velocity": ["project"]}
 
 
 @pytest.mark.parametrize("op", OPS)

@@ -67,7 +67,7 @@ def test_two_steps_lex(oracle, gold):
 
 
 OPS = ["poisson", "satadj", "damp_moist", "moist_clamp", "densities", "evaporation", "turbulence",
-       "damp_turb", "bc", "rk4", "polar", "radial"]
+       "damp_turb", "bc", "rk4", "polar", "radial", "project"]
 
 
 @pytest.fixture(scope="module")
@@ -252,3 +252,53 @@ def test_moist_convection_twice(oracle):
         assert not bad, f"call {call}: {bad}"
         for name in ("M_u", "M_d", "q_c_u", "c_u", "g_p", "e_d", "e_p", "P_conv", "MC_t", "MC_q", "MC_v", "MC_w"):
             assert float(gold[f"call{call}/{name}/l2"]) > 0.0, name
+
+
+GOLD_CONV_STEPS = os.path.join(HERE, "golden", "ref_conv_steps.npz")
+
+
+def planted_mc(oracle, grid, seed=33):
+    """Convection arrays at the start of the coupled steps (tests/golden/make_ref_vectors.py,
+    conv_planted_mc): uniform seeded MC_t, MC_q, MC_v, MC_w and P_conv, the rest zero."""
+    out = np.zeros((len(oracle.MC_FIELDS),) + tuple(grid))
+    rng = np.random.default_rng(seed)
+    for name, lo, hi in (("MC_t", -1e-3, 1e-3), ("MC_q", -1e-6, 1e-6), ("MC_v", -1e-4, 1e-4),
+                         ("MC_w", -1e-4, 1e-4), ("P_conv", 0.0, 1e-3)):
+        out[oracle.MC_FIELDS.index(name)] = rng.uniform(lo, hi, grid)
+    return out
+
+
+@pytest.mark.skipif(not os.path.exists(GOLD_CONV_STEPS), reason="coupled convection vectors not generated")
+def test_steps_with_convection_lex(oracle):
+    """Four lex iterations of the configured moist block (the ice scheme, then moist convection
+    with the damping of eleven of its arrays and P_conv in precipitationSum, MC_* in the RHS, the
+    Evaporation diagnostic written by the evaporation step), against the reference's own loop
+    (harness mode steps_conv) from the same planted state, with seeded MC_* tendencies that the
+    scheme carries where no column convects: every state field, ice-scheme array and convection
+    array, and the Evaporation diagnostic."""
+    gold = np.load(GOLD_CONV_STEPS, allow_pickle=False)
+    g = tuple(int(x) for x in gold["grid"])
+    spin = int(gold["spin"])
+    st = oracle.OracleState(*g, seed=int(gold["seed"]))
+    st.run(spin, "jacobi")
+    snap = st.snapshot()
+    mp = {k: np.zeros(g) for k in oracle.MP_FIELDS}
+    oracle.plant_microphysics(snap, mp, int(gold["plant"]))
+    st = oracle.OracleState(*g, blank=True)
+    st.load(snap)
+    st.set_iter(spin, spin)
+    for k, v in mp.items():
+        st.mp(k)[...] = v
+    st.evap()[...] = np.random.default_rng(int(gold["evap_seed"])).uniform(-1.0, 6.0, g[1:])
+    planted = planted_mc(oracle, g)
+    for q, name in enumerate(oracle.MC_FIELDS):
+        st.mc(name)[...] = planted[q]
+    st.set_moist_convection(True)
+    st.run(int(gold["steps"]), "lex_planes")
+    snap = st.snapshot()
+    snap.update({k: st.mp(k) for k in oracle.MP_FIELDS})
+    snap.update({"mc/" + k: st.mc(k) for k in oracle.MC_FIELDS})
+    check_pack("steps", snap, gold, 1e-10, skip=("S_c_c",))
+    ev = gold["steps/evap"]
+    assert rel(st.evap().reshape(-1), ev) < 1e-12
+    assert float(gold["steps/mc/MC_t/l2"]) > 0.0 and float(gold["steps/mc/MC_q/l2"]) > 0.0

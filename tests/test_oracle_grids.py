@@ -100,3 +100,18 @@ def test_convection_circle_factor_on_coarse_grids(oracle, jm):
     assert np.all(np.isfinite(E))
     assert 1e-4 < np.abs(E).max() < 0.05, np.abs(E).max()
     assert np.all(np.abs(ref.mc("MC_t")) <= 0.01 + 1e-15)
+
+
+def test_sst_land_scans_equal_closed_form(oracle):
+    """The reference finds the SST wall distance by scanning the land mask along j and k from
+    every cell in every stage (RHS_Atm_Turb.cpp:640-671); with no land those scans leave
+    min(i, j, k), which the oracle (and the device) use directly. Both forms give the same bits
+    over two lex iterations."""
+    grid = (11, 21, 41)
+    a = oracle.OracleState(*grid, seed=6)
+    b = oracle.OracleState(*grid, seed=6)
+    b.set_sst_scan(True)
+    a.run(2, "lex")
+    b.run(2, "lex")
+    for n in oracle.FIELDS:
+        assert np.array_equal(a.field(n), b.field(n)), n
