@@ -779,8 +779,9 @@ athad_status athad_create(int n_members, const athad_grid* grids, const athad_me
     for (int q = 0; q < n_members; q++) {
         const athad_grid& g = grids[q];
         if (g.im < 6 || g.jm < 6 || g.km < 6 || g.im > 1024 || g.jm > 8192 || g.km > 16384 || (g.jm % 2) == 0 ||
-            (long long)g.im * g.jm * g.km >= (1LL << 31)) {
-            g_create_error = "unsupported grid (need im,jm,km >= 6, jm odd, fewer than 2^31 cells)";
+            (long long)g.im * g.jm * g.km >= (1LL << 28)) {
+            // 2^28 cells: the RK4 kernel addresses a field, and its stencil margin, by 32-bit byte offsets
+            g_create_error = "unsupported grid (need im,jm,km >= 6, jm odd, fewer than 2^28 cells)";
             return ATHAD_ERR_SHAPE;
         }
     }

@@ -25,12 +25,17 @@ GRID = (41, 181, 361)
 METRIC = ("RK4 cell-updates/s at 41×181×361, mixed vs FP64 at stated error; p99 latency")
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector, published
-# Algorithmic bytes per cell of the dominant kernel (k_rk4_jacobi, FP64): reads 11 current
+# Algorithmic bytes per cell of the dominant kernel (k_rk4_rows, FP64): reads 11 current
 # fields + 11 stored (*n) fields + p_dyn, nue, p_stat, t_eq = 26 words; writes 11 new
 # fields + aux_u/v/w + prod, tke_source, dis_source + the 8 fused stores = 25 words.
 RK4_WORDS = 26 + 25
 # Whole scoped step, compulsory traffic per cell-update (FP64): 84 words.
 STEP_WORDS = 84
+
+
+def note(msg):
+    """A progress line on stderr as each leg of the benchmark ends (stdout holds the JSON)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def run_form(A, precision, order, steps, warmup, seed=1234, profile_steps=0, graph=True):
@@ -177,10 +182,10 @@ def run_next_rows(A, reps=10, seed=1234, warmup=8):
 
 
 def measured_traffic():
-    """HBM bytes per k_rk4_jacobi<double> launch at this grid, as recorded by the FETCH_SIZE and
+    """HBM bytes per k_rk4_rows<double> launch at this grid, as recorded by the FETCH_SIZE and
     WRITE_SIZE counter runs (scaled by the copy-kernel calibration) in the committed profile
     summary; None when that summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k_rk4_jacobi_fp64_41x181x361_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", "k_rk4_rows_fp64_41x181x361_hbm_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -190,9 +195,9 @@ def measured_traffic():
 
 
 def measured_flops():
-    """FP64 operations per k_rk4_jacobi<double> launch at this grid, from the SQ_INSTS_VALU_*_F64
+    """FP64 operations per k_rk4_rows<double> launch at this grid, from the SQ_INSTS_VALU_*_F64
     counter pass recorded in the committed profile summary; None when that summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k_rk4_jacobi_fp64_41x181x361_fp64_ops.md")
+    path = os.path.join(ROOT, "profiles", "k_rk4_rows_fp64_41x181x361_fp64_ops.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -247,6 +252,7 @@ def cpu_baseline():
         st.run(steps, order)
         el = time.perf_counter() - t0
         O.set_threads(0)
+        note(f"cpu {order} threads={threads} scan={scan}: {el / steps:.2f} s/step")
         return {"value": n * steps / el, "unit": "cell-updates/s", "threads": threads, "steps": steps,
                 "s_per_step": el / steps}
 
@@ -327,6 +333,7 @@ def main():
                                   "algorithmic_GBps": ach, "frac_of_hbm_peak": ach / HBM_PEAK_GBS}}))
         return
     fp64 = run_form(A, "fp64", "jacobi", args.steps, args.warmup, profile_steps=20, graph=graph)
+    note(f"fp64 jacobi: {fp64['ms_per_step']:.3f} ms/step")
     avg_rk4_s = fp64["rk4_ms"] / max(fp64["rk4_launches"], 1) * 1e-3
     achieved = RK4_WORDS * 8 * n / avg_rk4_s / 1e9
     traffic = measured_traffic()
@@ -348,7 +355,7 @@ def main():
                    "launch": "hipGraph of 4 iterations" if graph else "eager"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_rk4_jacobi<double>", "avg_launch_ms": avg_rk4_s * 1e3,
+                     "kernel": "k_rk4_rows<double>", "avg_launch_ms": avg_rk4_s * 1e3,
                      "bytes_per_cell": RK4_WORDS * 8, "algorithmic_bytes_per_launch": RK4_WORDS * 8 * n},
         "step_hbm_frac": STEP_WORDS * 8 * fp64["cells_per_s"] / (HBM_PEAK_GBS * 1e9),
     }
@@ -359,9 +366,10 @@ def main():
         out["alu_roofline"] = {"bound": "fp64 vector", "achieved": flops / avg_rk4_s / 1e12,
                                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                                "frac": flops / avg_rk4_s / 1e12 / FP64_PEAK_TFLOPS,
-                               "kernel": "k_rk4_jacobi<double>", "flop_per_launch": flops}
+                               "kernel": "k_rk4_rows<double>", "flop_per_launch": flops}
     if not args.no_extra:
         mixed = run_form(A, "mixed", "jacobi", args.steps, args.warmup, graph=graph)
+        note(f"mixed jacobi: {mixed['ms_per_step']:.3f} ms/step")
         err = {k: float(np.sqrt(np.sum((mixed["fields"][k] - fp64["fields"][k]) ** 2))
                         / max(np.sqrt(np.sum(fp64["fields"][k] ** 2)), 1e-300)) for k in A.PROGNOSTIC}
         out["mixed"] = {"cells_per_s": mixed["cells_per_s"], "ms_per_step": mixed["ms_per_step"],
@@ -369,17 +377,21 @@ def main():
                         "rel_l2_vs_fp64": err, "max_rel_l2": max(err.values())}
         from athad_cond_amd.ensemble import run_ensemble
         out["orders"] = order_spread(A, steps=100)
+        note("order spread done")
         out["lex"] = {"cells_per_s": out["orders"]["lex"]["cells_per_s"],
                       "ms_per_step": out["orders"]["lex"]["ms_per_step"], "steps": 100}
         # the reference order with members sharing every hyperplane launch
         out["lex_batched"] = {f"{m}x{g[0]}x{g[1]}x{g[2]}": run_ensemble(m, g, steps=8, warmup=2, order="lex")
                               for m, g in ((1, (41, 91, 181)), (32, (41, 91, 181)), (32, GRID))}
+        note("batched lex done")
         out["jacobi_vs_lex_rel_l2"] = out["orders"]["jacobi_vs_lex_rel_l2"]
         out["symmetric_vs_lex_rel_l2"] = out["orders"]["symmetric_vs_lex_rel_l2"]
         out["ice_scheme"] = run_ice(A, "fp64", args.steps, args.warmup, graph=graph)
         out["ice_scheme"]["workload"] = ("FP64 Jacobi step at 41x181x361 with the two-category ice scheme "
                                          "in the moist block and its sources in RK4")
+        note("ice scheme done")
         out["loop_with_ice_and_radiation"] = run_loop(A, "fp64", args.steps, args.warmup, graph=graph)
+        note("loop with ice and radiation done")
         out["loop_with_ice_and_radiation"]["workload"] = (
             "FP64 Jacobi iteration at 41x181x361 with the ice scheme in the moist block and radiation "
             "with the skin-temperature step after the store, all on the device (hipGraph)")
@@ -387,19 +399,24 @@ def main():
         out["configured_loop"]["workload"] = (
             "FP64 Jacobi iteration at 41x181x361 with the ice scheme and moist convection in the moist block "
             "and radiation after the store (run_3D_loop without output), hipGraph")
+        note("configured loop done")
         out["next_rows"] = run_next_rows(A)
+        note("next rows done")
         out["ensemble"] = {prec: run_ensemble(32, (41, 91, 181), steps=40, precision=prec)
                            for prec in ("fp64", "mixed")}
+        note("ensemble done")
         from athad_cond_amd import serving
         reqs = serving.poisson_requests(100, rate_per_s=4.0, seed=99, steps=50)
         server = serving.Server(max_slots=4, chunk=4)
         out["serving"] = server.run(reqs)
         server.close()
+        note("serving done")
         out["serving"]["arrivals"] = ("100 requests, Poisson, mean 4 req/s, seed 99; 50 steps each; grids "
                                       "21x91x181, 41x181x361, 61x361x721; mixed with FP64 fallback; p99 is the "
                                       "99th percentile of the 100 latencies (linear interpolation)")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        note("cpu baseline done")
     print(json.dumps(out))
 
 
