@@ -2353,7 +2353,6 @@ ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi
                           double M_other, double Mb_d, double t_00d, double t_0d) {
     const N t_00 = N(t_00d), t_0 = N(t_0d), t_range_inv = N(1.0 / (t_0d - t_00d)), co2 = N(co2d), Mb = N(Mb_d);
     const double inv_p_d = (p_local > 0.0) ? 1.0 / p_local : 0.0;
-    const N p_n = N(p_local), inv_p = N(inv_p_d), Mo = N(M_other);
     N T = N(T_io), qvb = N(qv_io), qcb = N(qc_io), qib = N(qi_io), qv_hyp = qvb;
     // species heat capacities and L(T) of iteration k+1's update are those of iteration k's
     // new temperature
@@ -2388,9 +2387,20 @@ ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi
         const bool hot = (T >= t_0), some = (qsum > N(1e-12));
         const bool need_l = (CND > N(0)) || (some ? (qcb > N(0)) : hot);
         const bool need_i = (DEP > N(0)) || (some ? (qib > N(0)) : !hot);
+        // the saturation vapour pressures and the exact q_sat with its derivative factor in FP64
+        // in both instantiations; N = float carries the iteration's state and its latent heat
+        // and heat capacity in FP32 (the mixed form)
         N qs = N(0), Ac = N(0), qi = N(0), Ad = N(0);
-        if (need_l) sat_qA(sat_p_sat(T), p_n, inv_p, Mo, qs, Ac);
-        if (need_i) sat_qA(sat_p_subl(T), p_n, inv_p, Mo, qi, Ad);
+        if (need_l) {
+            double q_, A_;
+            sat_qA(sat_p_sat((double)T), p_local, inv_p_d, M_other, q_, A_);
+            qs = N(q_); Ac = N(A_);
+        }
+        if (need_i) {
+            double q_, A_;
+            sat_qA(sat_p_subl((double)T), p_local, inv_p_d, M_other, q_, A_);
+            qi = N(q_); Ad = N(A_);
+        }
         const N target = some ? (qcb * qs + qib * qi) * rcp_nr(qsum) : (hot ? qs : qi);
         const N inv_cp = rcp_nr(sat_mix_cp(qvb, co2, cpH, cpC, cpB));
         const N inv_RT2 = (T > N(0)) ? rcp_nr(N(kRH2O) * T * T) : N(0);
@@ -2443,7 +2453,8 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
             if ((qv_old > qs0 && alpha > 0.01) || (qv_old < qs0 && (qc_old > 1e-12 || qi_old > 1e-12))) {
                 double qvb = qv_old, qcb = qc_old, qib = qi_old;
                 const double T_orig = t * t_0;
-                sat_newton<double>(T, qvb, qcb, qib, co2, p_local, M_other, Mb, t_00, t_0);
+                if constexpr (sizeof(S) == 4) sat_newton<float>(T, qvb, qcb, qib, co2, p_local, M_other, Mb, t_00, t_0);
+                else sat_newton<double>(T, qvb, qcb, qib, co2, p_local, M_other, Mb, t_00, t_0);
                 qcb = tmax(0.0, qcb);
                 qib = tmax(0.0, qib);
                 if (T > kTcrit) T = kTcrit;
