@@ -1561,7 +1561,7 @@ for_stages<U>([&](const int st) {
 #define ATHAD_RK4_WAVES_F64 3
 #endif
 #ifndef ATHAD_RK4_WAVES_F32
-#define ATHAD_RK4_WAVES_F32 4
+#define ATHAD_RK4_WAVES_F32 5
 #endif
 template <typename S> constexpr int rk4_waves() { return sizeof(S) == 8 ? ATHAD_RK4_WAVES_F64 : ATHAD_RK4_WAVES_F32; }
 #ifndef ATHAD_RK4_ROW_WAVES_F64
@@ -1640,14 +1640,15 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const Nb& nb, int i, int j, int k, 
     }
 }
 
-// Jacobi-order RK4 over the whole grid. Interior cells integrate four stages against the
-// start-of-RK4 neighbourhood read from f[] and write alt[]; boundary cells are copied. The
-// solid-ground clamps are applied on read (FUSED=1), and the store of the eight fields that
-// no later filter touches is fused (xn = new value).
+// Jacobi-order RK4 over the whole grid, one thread per cell in storage order (the FP32 form;
+// FP64 runs k_rk4_rows). Interior cells integrate four stages against the start-of-RK4
+// neighbourhood read from f[] and write alt[]; boundary cells are copied. The solid-ground
+// clamps are applied on read (FUSED=1), and the store of the eight fields that no later filter
+// touches is fused (xn = new value).
 template <typename S, bool FUSED, int SRC>
 __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>()))) void k_rk4_jacobi(const MemberDesc* __restrict__ md) {
     using C = S;
-    __shared__ C s_ba[10 * LB], s_rec[12 * LB];
+    __shared__ C s_ba[8 * LB], s_rec[12 * LB];
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const long long id = cell_id(K);
@@ -1683,15 +1684,18 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
         }
         return;
     }
-    const NbLoad<S, C, FUSED> nb{M, X, id, (long long)K.jm * K.km, (long long)K.km, (long long)ix3(K, 0, j, k)};
-    // the metric table in the FP32 kernel (4 waves per SIMD, registers to spare), formed in
-    // place in the FP64 kernel
-    rk4_cell<S, C, FUSED, SRC, false, sizeof(S) == 4, false>(M, nb, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
+    // buffer-resource addressing as in the row kernel; the metric table read per lane in FP32
+    // (5 waves per SIMD), formed in place in FP64
+    constexpr unsigned E = sizeof(S);
+    const unsigned si = (unsigned)(K.jm * K.km), nfld = (unsigned)K.n * E;
+    const NbRow<S, C, FUSED> nb{M, (unsigned)id * E, (unsigned)(j * K.km + k) * 8u, si * E, (unsigned)K.km * E,
+                                nfld + (2u * si + 2u) * E, nfld};
+    rk4_cell<S, C, FUSED, SRC, false, sizeof(S) == 4, true>(M, nb, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
         [&](int f, C v) {
-            gst((S*)M.alt[f], id, S(v));
-            if (FUSED && f != FU && f != FV && f != FW) gst(fp<S>(M, kProgN[f]), id, S(v));
+            nb.put(M.alt[f], v);
+            if (FUSED && f != FU && f != FV && f != FW) nb.put(M.f[kProgN[f]], v);
         },
-        [&](int fid, C v) { gst(fp<S>(M, fid), id, S(v)); });
+        [&](int fid, C v) { nb.put(M.f[fid], v); });
 }
 
 // The metric coefficients of row (i, j) when i and j are the same for the whole wave: the
@@ -2419,6 +2423,23 @@ ATHAD_DEV void sat_newton(double& T_io, double& qv_io, double& qc_io, double& qi
     T_io = (double)T; qv_io = (double)qvb; qc_io = (double)qcb; qi_io = (double)qib;
 }
 
+// The saturation adjustment's fade factor 1 / (1 + exp(-(T - t_00) / 5)) and, where condensate
+// is released or formed, the mixture's heat capacity and the latent heat: FP64 in the FP64
+// form, FP32 arithmetic in the mixed form (storage S = float), where only the saturation
+// vapour pressures and q_sat stay FP64.
+template <typename S> ATHAD_DEV double fade(double T, double t_00) {
+    if constexpr (sizeof(S) == 4) return (double)rcp_nr(1.0f + expf((float)(-(T - t_00) * 0.2)));
+    else return rcp_nr(1.0 + exp(-(T - t_00) * 0.2));
+}
+template <typename S> ATHAD_DEV double heat_cap(double c, double co2, double T, double inv_Mb) {
+    if constexpr (sizeof(S) == 4) return (double)dens_cp(dens_split((float)c, (float)co2), (float)T, (float)inv_Mb);
+    else return dens_cp(dens_split(c, co2), T, inv_Mb);
+}
+template <typename S> ATHAD_DEV double latent(double T) {
+    if constexpr (sizeof(S) == 4) return (double)sat_L_vap((float)T);
+    else return sat_L_vap(T);
+}
+
 template <typename S>
 __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ md, int gate, int to_tmp) {
     const MemberDesc& M = md[blockIdx.z];
@@ -2448,7 +2469,7 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
             double qs0, A0;
             E_entry = dens_p_auto(T);
             sat_qA(E_entry, p_local, inv_p, M_other, qs0, A0);
-            const double alpha = rcp_nr(1.0 + exp(-(T - t_00) * 0.2));
+            const double alpha = fade<S>(T, t_00);
             T_entry = T; alpha_entry = alpha;
             if ((qv_old > qs0 && alpha > 0.01) || (qv_old < qs0 && (qc_old > 1e-12 || qi_old > 1e-12))) {
                 double qvb = qv_old, qcb = qc_old, qib = qi_old;
@@ -2489,9 +2510,9 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
                 const double q_used = tmax(0.0, c_new - c);
                 c = c_new; cloud = 0.0; ice = 0.0; gr = 0.0;
                 if (Td < kTcrit) {
-                    const double cp_loc = dens_cp(dens_split(c, co2), Td, inv_Mb);
+                    const double cp_loc = heat_cap<S>(c, co2, Td, inv_Mb);
                     if (cp_loc > 0.0) {
-                        double T_new = Td - sat_L_vap(Td) * q_used * rcp_nr(cp_loc);
+                        double T_new = Td - latent<S>(Td) * q_used * rcp_nr(cp_loc);
                         if (T_new < 1.0) T_new = 1.0;
                         t = T_new / t_0;
                     }
@@ -2515,15 +2536,15 @@ __global__ __launch_bounds__(256) void k_satadj(const MemberDesc* __restrict__ m
             if (c > qs) {
                 const double excess = c - qs;
                 c = qs;
-                const double cpx = dens_cp(dens_split(c, co2), Td, inv_Mb);
-                if (Td >= t_00) { cloud += excess; Td += sat_L_vap(Td) * excess * rcp_nr(cpx); }
-                else { ice += excess; Td += (sat_L_vap(Td) + 3.337e5) * excess * rcp_nr(cpx); }
+                const double cpx = heat_cap<S>(c, co2, Td, inv_Mb);
+                if (Td >= t_00) { cloud += excess; Td += latent<S>(Td) * excess * rcp_nr(cpx); }
+                else { ice += excess; Td += (latent<S>(Td) + 3.337e5) * excess * rcp_nr(cpx); }
                 if (Td > K.t_max_phys) Td = K.t_max_phys;
                 t = Td * inv_t_0;
             }
             if (cloud > K.cloud_cap) cloud = K.cloud_cap;
             if (ice > K.cloud_cap) ice = K.cloud_cap;
-            const double al = (Td == T_entry) ? alpha_entry : rcp_nr(1.0 + exp(-(Td - t_00) * 0.2));
+            const double al = (Td == T_entry) ? alpha_entry : fade<S>(Td, t_00);
             c *= al; cloud *= al; ice *= al;
         }
     }
@@ -2778,6 +2799,10 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* __restrict_
     A p_prev = A(K.p_sl_factor) * T_prev;
     A q_sat_min = A(1);
     A qv_prev = A(0), qc_prev = A(0);
+    // the saturation pressure at the previous level's temperature, kept with that temperature:
+    // above the cold trap the column is isothermal at t_skin, and a repeated temperature gives
+    // the same pressure bit for bit without evaluating it again
+    A T_sat = A(-1), E_sat = A(0);
     // this level's inputs; the next level's are loaded before this level's stores so their
     // latency overlaps the column arithmetic
     A qv = A(gld(c, (long long)base)), qc = A(gld(co2, (long long)base));
@@ -2798,9 +2823,11 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* __restrict_
             const A cp_loc = dens_cp(mp, T_prev, inv_Mb);
             const A M_nw = dens_Mnw(mp, Mb_a, inv_Mb);
             A q_s, Af;
-            sat_qA(dens_p_auto(T_prev), p_prev, (p_prev > A(0)) ? rcp_nr(p_prev) : A(0), M_nw, q_s, Af);
-            const A gamma = (qv_prev >= A(0.99) * q_s) ? dens_lapse(T_prev, Af, cp_loc, R_loc, g)
-                                                       : g * rcp_nr(cp_loc);
+            if (T_prev != T_sat) { E_sat = dens_p_auto(T_prev); T_sat = T_prev; }
+            sat_qA(E_sat, p_prev, (p_prev > A(0)) ? rcp_nr(p_prev) : A(0), M_nw, q_s, Af);
+            A gamma;
+            if (qv_prev >= A(0.99) * q_s) gamma = dens_lapse(T_prev, Af, cp_loc, R_loc, g);
+            else gamma = g * rcp_nr(cp_loc);
             const A T_ad = T_prev - gamma * dz;
             Ti = tmax(A(K.t_skin), T_ad);
             const A T_mean = A(0.5) * (T_prev + Ti);
@@ -3777,8 +3804,8 @@ template <typename S> void rk4_lex_s(const Batch& b, const dim3& g, int plane, i
 }  // namespace
 // FP64: the row-segment kernel, waves taken i fastest, then the longitude segment, then j
 // (measured at 41 x 181 x 361: 0.445 ms against 0.476 with the storage order and 0.498 with
-// the segment slowest; the cell-linear FP64 kernel 0.530). FP32: the cell-linear kernel
-// (0.224 ms against 0.245 for the row kernel at five waves per SIMD).
+// the segment slowest; the cell-linear FP64 kernel 0.530). FP32: the cell-linear kernel at five
+// waves per SIMD (0.214 ms; the row kernel 0.245).
 void launch_rk4_jacobi(const Batch& b, bool fused, int src) {
     if (!b.fp32) {
         const int nseg = (b.max_km + 63) / 64;
