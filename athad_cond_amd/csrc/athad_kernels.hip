@@ -975,6 +975,7 @@ template <typename C, bool BRANCHLESS = false> ATHAD_DEV C tanh_pos(C x) {
 
 // Global-memory access through address space 1 (pointers read from a MemberDesc are
 // otherwise generic and compile to flat instructions).
+template <typename T> using G1 = __attribute__((address_space(1))) T;   // global-memory object
 template <typename T> ATHAD_DEV T gld(const T* p, long long i) {
     return ((const __attribute__((address_space(1))) T*)p)[i];
 }
@@ -4062,27 +4063,31 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
     if (col >= jm * km) return;
     const int j = col / km, k = col - j * km;
     const size_t pl = (size_t)jm * km;
-    const S* __restrict__ T = (const S*)d.f[F_T];
-    const S* __restrict__ U = (const S*)d.f[F_U];
-    const S* __restrict__ V = (const S*)d.f[F_V];
-    const S* __restrict__ W = (const S*)d.f[F_W];
-    const S* __restrict__ C = (const S*)d.f[F_C];
-    const S* __restrict__ CL = (const S*)d.f[F_CLOUD];
-    const S* __restrict__ CO2 = (const S*)d.f[F_CO2];
-    const S* __restrict__ RH = (const S*)d.f[F_R_HUMID];
-    const double* __restrict__ PS = (const double*)d.f[F_P_STAT];
-    double* __restrict__ Sx = d.mc[MC_S];   double* __restrict__ Mu = d.mc[MC_M_U];
-    double* __restrict__ Md = d.mc[MC_M_D]; double* __restrict__ Su = d.mc[MC_S_U];
-    double* __restrict__ Qvu = d.mc[MC_Q_V_U]; double* __restrict__ Qcu = d.mc[MC_Q_C_U];
-    double* __restrict__ Cu = d.mc[MC_C_U]; double* __restrict__ El = d.mc[MC_E_L];
-    double* __restrict__ Edv = d.mc[MC_E_D_EVAP]; double* __restrict__ Ep = d.mc[MC_E_P];
-    double* __restrict__ Gp = d.mc[MC_G_P]; double* __restrict__ Sd = d.mc[MC_S_D];
-    double* __restrict__ Qvd = d.mc[MC_Q_V_D]; double* __restrict__ Pc = d.mc[MC_P_CONV];
-    double* __restrict__ Eu = d.mc[MC_E_U]; double* __restrict__ Du = d.mc[MC_D_U];
-    double* __restrict__ Uu = d.mc[MC_U_U]; double* __restrict__ Vu = d.mc[MC_V_U];
-    double* __restrict__ Wu = d.mc[MC_W_U]; double* __restrict__ Ed = d.mc[MC_E_D];
-    double* __restrict__ Dd = d.mc[MC_D_D]; double* __restrict__ Ud = d.mc[MC_U_D];
-    double* __restrict__ Vd = d.mc[MC_V_D]; double* __restrict__ Wd = d.mc[MC_W_D];
+    // global-memory pointers (plain pointers read from a descriptor would compile to flat
+    // accesses, whose waits also cover the scalar loads)
+    typedef const G1<S>* __restrict__ FieldP;
+    typedef G1<double>* __restrict__ ArrP;
+    FieldP T = (FieldP)d.f[F_T];
+    FieldP U = (FieldP)d.f[F_U];
+    FieldP V = (FieldP)d.f[F_V];
+    FieldP W = (FieldP)d.f[F_W];
+    FieldP C = (FieldP)d.f[F_C];
+    FieldP CL = (FieldP)d.f[F_CLOUD];
+    FieldP CO2 = (FieldP)d.f[F_CO2];
+    FieldP RH = (FieldP)d.f[F_R_HUMID];
+    const G1<double>* __restrict__ PS = (const G1<double>*)d.f[F_P_STAT];
+    ArrP Sx = (ArrP)d.mc[MC_S];   ArrP Mu = (ArrP)d.mc[MC_M_U];
+    ArrP Md = (ArrP)d.mc[MC_M_D]; ArrP Su = (ArrP)d.mc[MC_S_U];
+    ArrP Qvu = (ArrP)d.mc[MC_Q_V_U]; ArrP Qcu = (ArrP)d.mc[MC_Q_C_U];
+    ArrP Cu = (ArrP)d.mc[MC_C_U]; ArrP El = (ArrP)d.mc[MC_E_L];
+    ArrP Edv = (ArrP)d.mc[MC_E_D_EVAP]; ArrP Ep = (ArrP)d.mc[MC_E_P];
+    ArrP Gp = (ArrP)d.mc[MC_G_P]; ArrP Sd = (ArrP)d.mc[MC_S_D];
+    ArrP Qvd = (ArrP)d.mc[MC_Q_V_D]; ArrP Pc = (ArrP)d.mc[MC_P_CONV];
+    ArrP Eu = (ArrP)d.mc[MC_E_U]; ArrP Du = (ArrP)d.mc[MC_D_U];
+    ArrP Uu = (ArrP)d.mc[MC_U_U]; ArrP Vu = (ArrP)d.mc[MC_V_U];
+    ArrP Wu = (ArrP)d.mc[MC_W_U]; ArrP Ed = (ArrP)d.mc[MC_E_D];
+    ArrP Dd = (ArrP)d.mc[MC_D_D]; ArrP Ud = (ArrP)d.mc[MC_U_D];
+    ArrP Vd = (ArrP)d.mc[MC_V_D]; ArrP Wd = (ArrP)d.mc[MC_W_D];
     const double* __restrict__ h = d.hgt;
     const double t0 = K.t_0, Mb = K.M_bg, Rb = K.R_bg, lv = K.lv, ls = K.ls;
     auto id = [&](int i) { return (size_t)i * pl + col; };
@@ -4096,38 +4101,52 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
     const double inv_a_u = 1.0 / (a_u * K.u_0), inv_a_d = 1.0 / (a_d * K.u_0), uu_cap = w_conv_max / K.u_0;
 
     // precompute (:336-376)
+    // Every level loop below loads the next level's inputs before it stores this level's
+    // results: a wave's vector-memory counter retires in issue order, so a load issued after a
+    // store could only be waited for together with that store.
+    double t_next = (double)T[id(0)];
     for (int i = 0; i < im; i++) {
         const size_t q = id(i);
-        Sx[q] = cp_l * (double)T[q] * t0 / s_0;
+        const double t_cur = t_next;
+        if (i + 1 < im) t_next = (double)T[id(i + 1)];
+        Sx[q] = cp_l * t_cur * t0 / s_0;
         Mu[q] = 0.0; Md[q] = 0.0; Su[q] = 0.0; Qvu[q] = 0.0; Qcu[q] = 0.0; Cu[q] = 0.0; El[q] = 0.0;
         Edv[q] = 0.0; Ep[q] = 0.0; Gp[q] = 0.0; Sd[q] = 0.0; Qvd[q] = 0.0; Pc[q] = 0.0;
     }
     int base = 0, lfs = 0, beg = -1, end = -1;
     double MdLFS = 0.0, cape_c = 0.0, dT = 0.0, dq = 0.0;
     // findCloudBaseLFS (:627-777)
-    for (int i = 0; i < im; i++)
-        if (PS[id(i)] <= f_beg * p_sfc) { beg = i; d.mc[MC_DEEP_BEG][id(i)] = h[i]; break; }
-    for (int i = im - 1; i >= 0; i--)
-        if (PS[id(i)] >= f_end * p_sfc) { end = i; d.mc[MC_DEEP_END][id(i)] = h[i]; break; }
-    const double* __restrict__ QSL = d.conv_qs[0];
-    const double* __restrict__ QSR = d.conv_qs[1];
+    {   // the column's pressures, read once for both searches
+        double pcol[4];
+        for (int i0 = 0; i0 < im && beg < 0; i0 += 4) {
+            for (int l = 0; l < 4; l++) pcol[l] = (i0 + l < im) ? PS[id(i0 + l)] : 0.0;
+            for (int l = 0; l < 4 && i0 + l < im; l++)
+                if (pcol[l] <= f_beg * p_sfc) { beg = i0 + l; break; }
+        }
+        if (beg >= 0) d.mc[MC_DEEP_BEG][id(beg)] = h[beg];
+        for (int i0 = im - 1; i0 >= 0 && end < 0; i0 -= 4) {
+            for (int l = 0; l < 4; l++) pcol[l] = (i0 - l >= 0) ? PS[id(i0 - l)] : 0.0;
+            for (int l = 0; l < 4 && i0 - l >= 0; l++)
+                if (pcol[l] >= f_end * p_sfc) { end = i0 - l; break; }
+        }
+        if (end >= 0) d.mc[MC_DEEP_END][id(end)] = h[end];
+    }
+    const G1<double>* __restrict__ QSL = (const G1<double>*)d.conv_qs[0];
+    const G1<double>* __restrict__ QSR = (const G1<double>*)d.conv_qs[1];
     auto qs_lift = [&](int i) { return QSL[id(i)]; };
-    // initUpdraft's saturation at t + dT: the precomputed one where dT is 0 or t_add_u
-    auto qs_pert = [&](int i, double t_a) {
-        return (dT == 0.0) ? QSR[id(i)] : (dT == t_add_u) ? QSL[id(i)]
-                                                           : qs_env(th_p_auto(t_a), PS[id(i)], (double)C[id(i)]);
-    };
     if (beg >= 0 && end >= 0) {
+        double tn = t_of(beg), qsn = qs_lift(beg), cn = (double)C[id(beg)], psn = PS[id(beg)];
         for (int i = beg; i < im; i++) {
             const size_t q = id(i);
-            const double t_u = t_of(i), qs = qs_lift(i);
+            const double t_u = tn, qs = qsn, cv = cn, psv = psn;
+            if (i + 1 < im) { tn = t_of(i + 1); qsn = qs_lift(i + 1); cn = (double)C[id(i + 1)]; psn = PS[id(i + 1)]; }
             const double w = 1.0 / (1.0 + exp(-(t_u - t_00) / 2.0));
-            double qv = (double)C[q] + q_v_u_add, qc = 0.0;
+            double qv = cv + q_v_u_add, qc = 0.0;
             if (qv >= scale * qs && t_u >= t_00) qc = qv - scale * qs;
             if (qv <= 0.0) qv = 0.0;
             if (qc <= 0.0) qc = 0.0;
             Qvu[q] = qv; Qcu[q] = qc;
-            if (qv >= scale * qs && w > 0.01 && PS[q] <= f_cb * p_sfc) {
+            if (qv >= scale * qs && w > 0.01 && psv <= f_cb * p_sfc) {
                 base = i;
                 d.mc[MC_CLOUD_BASE][q] = h[i] * w;
                 break;
@@ -4142,10 +4161,13 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             bool buoy = false;
             int top = base;
             double cape = 0.0;
+            double tn = 0.0, pn = 1.0, qn = 0.0;
+            if (base + 1 < im) { tn = t_of(base + 1); pn = PS[id(base + 1)]; qn = qs_lift(base + 1); }
             for (int i = base + 1; i < im; i++) {
-                const double Te = t_of(i);
+                const double Te = tn, pe = pn, qe = qn;
+                if (i + 1 < im) { tn = t_of(i + 1); pn = PS[id(i + 1)]; qn = qs_lift(i + 1); }
                 const double Le = (Te >= t0) ? lv : ls;
-                const double the = Te * pow(p_0 / PS[id(i)], kap) * exp(Le * qs_lift(i) / (cp_l * Te));
+                const double the = Te * pow(p_0 / pe, kap) * exp(Le * qe / (cp_l * Te));
                 if (thp - the > 0.0) {
                     buoy = true;
                     top = i;
@@ -4184,69 +4206,100 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             }
         }
     }
-    // E_u at level i (:534-543, :841-851) with the entrainment gradients of c (:238-297)
-    auto entrain = [&](int i, bool with_m) {
+    // The inputs of E_u at a level of an inner column (the horizontal neighbours exist there).
+    struct EnvL { double t, c, cip, cim, cjp, cjm, ckp, ckm, rh, u, v, w; };
+    auto load_env = [&](int i) {
         const size_t q = id(i);
+        EnvL e;
+        e.t = t_of(i); e.c = (double)C[q];
+        e.cip = (double)C[id(tmin(i + 1, im - 1))]; e.cim = (double)C[id(tmax(i - 1, 0))];
+        e.cjp = (double)C[q + km]; e.cjm = (double)C[q - km]; e.ckp = (double)C[q + 1]; e.ckm = (double)C[q - 1];
+        e.rh = (double)RH[q]; e.u = (double)U[q]; e.v = (double)V[q]; e.w = (double)W[q];
+        return e;
+    };
+    // E_u at level i (:534-543, :841-851) with the entrainment gradients of c (:238-297); deep is
+    // the cloud-depth test p(base) - p(lfs) >= f_diff p_sfc, mu the level's M_u
+    auto entrain = [&](int i, const EnvL& v, double mu, bool with_m, bool deep) {
         double e = 0.0;
-        if (PS[id(base)] - PS[id(lfs)] >= f_diff * p_sfc) {
+        if (deep) {
             const double rth = (K.r_earth * 1.0e3 + h[i]) * d.conv_cos[j];
             const double Ue = rth * 2.0 * M_PI;
-            const int ip = tmin(i + 1, im - 1), imn = tmax(i - 1, 0);
-            const double dr = ((double)C[id(ip)] - (double)C[id(imn)]) * (0.5 / step(i));
-            const double dth = ((double)C[q + km] - (double)C[q - km]) * (0.5 / (Ue / (jm - 1)));
-            const double dph = ((double)C[q + 1] - (double)C[q - 1]) * (1.0 / (Ue / (km - 1)));
-            const double cc = tmax((double)C[q], 1e-4);
-            e = -(double)RH[q] / cc * ((double)U[q] * dr + (double)V[q] * dth + (double)W[q] * dph);
-            if (with_m) e += eps_u * Mu[q];
+            const double dr = (v.cip - v.cim) * (0.5 / step(i));
+            const double dth = (v.cjp - v.cjm) * (0.5 / (Ue / (jm - 1)));
+            const double dph = (v.ckp - v.ckm) * (1.0 / (Ue / (km - 1)));
+            const double cc = tmax(v.c, 1e-4);
+            e = -v.rh / cc * (v.u * dr + v.v * dth + v.w * dph);
+            if (with_m) e += eps_u * mu;
         }
-        if (t_of(i) <= t_00_m2) e = 0.0;
+        if (v.t <= t_00_m2) e = 0.0;
         return e;
+    };
+    // a level of the updraft passes: E_u's inputs, the lifted and the resting saturation, p, and
+    // the arrays the pass reads
+    struct UpL { EnvL e; double qsr, qsl, ps, qc, mu; };
+    auto load_up = [&](int i) {
+        const size_t q = id(i);
+        UpL u;
+        u.e = load_env(i); u.qsr = QSR[q]; u.qsl = QSL[q]; u.ps = PS[q]; u.qc = Qcu[q]; u.mu = Mu[q];
+        return u;
+    };
+    // initUpdraft's saturation at t + dT: the precomputed one where dT is 0 or t_add_u
+    auto qs_pert_v = [&](const UpL& u, double t_a) {
+        return (dT == 0.0) ? u.qsr : (dT == t_add_u) ? u.qsl : qs_env(th_p_auto(t_a), u.ps, u.e.c);
     };
     const double seed_w = c_mb * sqrt(2.0 * cape_c);
     // initUpdraft (:490-622)
     if (inner && beg != -1 && end != -1) {
+        const bool deep = PS[id(base)] - PS[id(lfs)] >= f_diff * p_sfc;
+        UpL nx;
+        if (beg + 1 <= end) nx = load_up(beg + 1);
         const double mbeg = cap((double)RH[id(beg)] * seed_w, M_max);
         Mu[id(beg)] = mbeg;
         double mprev = mbeg;   // M_u one level down, carried in a register
         for (int i = beg + 1; i <= end; i++) {
             const size_t q = id(i);
-            const double t_u = t_of(i), t_a = t_u + dT;
-            const double qsa = qs_pert(i, t_a);
-            double qv = (double)C[q] + dq, qc = Qcu[q];
+            const UpL L_ = nx;
+            if (i + 1 <= end) nx = load_up(i + 1);
+            const double t_u = L_.e.t, t_a = t_u + dT;
+            const double qsa = qs_pert_v(L_, t_a);
+            double qv = L_.e.c + dq, qc = L_.qc;
             if (qv >= scale * qsa && t_u >= t_00) qc = qv - scale * qsa;
             if (qv <= 0.0) qv = 0.0;
             if (qc <= 0.0) qc = 0.0;
             Qvu[q] = qv; Qcu[q] = qc;
-            const double e = entrain(i, false);
+            const double e = entrain(i, L_.e, 0.0, false, deep);
             Eu[q] = e;
             Du[q] = 0.0;
             mprev = cap(mprev + e * step(i), M_max);
             Mu[q] = mprev;
             Sx[q] = cp_l * t_u / s_0;
             Su[q] = cp_l * t_a / s_0;
-            Uu[q] = vel * (double)U[q]; Vu[q] = vel * (double)V[q]; Wu[q] = vel * (double)W[q];
+            Uu[q] = vel * L_.e.u; Vu[q] = vel * L_.e.v; Wu[q] = vel * L_.e.w;
         }
+        if (end + 1 <= lfs - 1) nx = load_up(end + 1);
         MdLFS = cap(gam_d * mbeg, M_max);
         Md[id(lfs)] = MdLFS;
         mprev = mbeg;
         for (int i = end + 1; i <= lfs - 1; i++) {
             const size_t q = id(i);
-            const double t_u = t_of(i), t_a = t_u + dT;
-            const double qsa = qs_pert(i, t_a);
+            const UpL L_ = nx;
+            if (i + 1 <= lfs - 1) nx = load_up(i + 1);
+            const double t_u = L_.e.t, t_a = t_u + dT;
+            const double qsa = qs_pert_v(L_, t_a);
             if (i == end + 1) Mu[id(i - 1)] = mbeg;
             Sx[q] = cp_l * t_u / s_0;
             Su[q] = cp_l * t_u / s_0;
-            const double qv = (double)C[q] + dq;
+            const double qv = L_.e.c + dq;
             Qvu[q] = qv;
             if (qv >= scale * qsa && t_a >= t_00) Qcu[q] = qv - scale * qsa;
-            const double e = entrain(i, false);
+            const double e = entrain(i, L_.e, 0.0, false, deep);
             Eu[q] = e;
             Du[q] = 0.0;
             const double m = cap(mprev + e * step(i), M_max);
             mprev = m;
             Mu[q] = m;
-            Uu[q] = tmax(-uu_cap, tmin((double)U[q] + m * inv_a_u / tmax((double)RH[q], 1e-6), uu_cap));
-            Vu[q] = (double)V[q]; Wu[q] = (double)W[q];
+            Uu[q] = tmax(-uu_cap, tmin(L_.e.u + m * inv_a_u / tmax(L_.e.rh, 1e-6), uu_cap));
+            Vu[q] = L_.e.v; Wu[q] = L_.e.w;
         }
     }
     // CloudBaseUpdate (:1462-1482) with simpsonCxEu (:1426-1455)
@@ -4256,15 +4309,19 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
         double cum = 0.0;
         int i = beg;
         auto f = [&](int l) { return -((double)C[id(l)] * Eu[id(l)]); };
+        // f at i, i + 1, i + 2, the next pair read one step ahead
+        double fa = f(i), fb = (i + 1 <= lfs) ? f(i + 1) : 0.0, fc = (i + 2 <= lfs) ? f(i + 2) : 0.0;
         for (; i + 2 <= lfs; i += 2) {
             const double h1 = step(i), h2 = step(i + 1);
-            const double f0 = f(i), f1 = f(i + 1), f2 = f(i + 2);
+            const double f0 = fa, f1 = fb, f2 = fc;
+            const double nb = (i + 3 <= lfs) ? f(i + 3) : 0.0, nc = (i + 4 <= lfs) ? f(i + 4) : 0.0;
             const double pc = (h1 + h2) / 6.0 * ((2.0 - h2 / h1) * f0 + (h1 + h2) * (h1 + h2) / (h1 * h2) * f1
                                                  + (2.0 - h1 / h2) * f2);
             if (cum + pc > 0.0) { hit = (cum + 0.5 * h1 * (f0 + f1) > 0.0) ? i + 1 : i + 2; break; }
             cum += pc;
+            fa = f2; fb = nb; fc = nc;
         }
-        if (hit < 0 && i + 1 <= lfs && cum + 0.5 * step(i) * (f(i) + f(i + 1)) > 0.0) hit = i + 1;
+        if (hit < 0 && i + 1 <= lfs && cum + 0.5 * step(i) * (fa + fb) > 0.0) hit = i + 1;
         if (hit >= 0 && hit < im && PS[id(hit)] <= f_cb * p_sfc && (base <= 0 || hit <= base)) {
             base = hit;
             d.mc[MC_CLOUD_BASE][id(hit)] = h[hit];
@@ -4276,6 +4333,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
         if (inner && lfs > base) {
             const bool shallow = PS[id(base)] - PS[id(lfs)] < f_diff * p_sfc;   // convection_mode 1
             const double dic = shallow ? 1.0e9 : 500.0, hb = h[base];
+            UpL nx = load_up(base + 1);
             for (int i = 0; i < base; i++) Mu[id(i)] = 0.0;
             double mprev = cap((double)RH[id(base)] * seed_w, M_max);
             Mu[id(base)] = mprev;
@@ -4284,8 +4342,10 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             MdLFS = cap(gam_d * mprev, M_max);
             for (int i = base + 1; i <= lfs; i++) {
                 const size_t q = id(i);
-                const double t_u = t_of(i), st = step(i), m0 = Mu[q];
-                const double e = entrain(i, true);
+                const UpL L_ = nx;
+                if (i + 1 <= lfs) nx = load_up(i + 1);
+                const double t_u = L_.e.t, st = step(i), m0 = L_.mu;
+                const double e = entrain(i, L_.e, m0, true, !shallow);
                 double dd = eps_u * m0;
                 if (ip > 1 && i == lfs) dd = del_u * m0 + b_u * m0 / st;
                 if (ip > 1 && i == lfs - 1) dd = del_u * m0 + (1.0 - b_u) * m0 / st;
@@ -4297,51 +4357,65 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
                 Cu[q] = 0.0;
                 double el = 0.0;
                 if (ip != 1) {
-                    el = dd * Qcu[q] / tmax((double)RH[q], 0.01);
+                    el = dd * L_.qc / tmax(L_.e.rh, 0.01);
                     if (el <= 0.0) el = 0.0;
                 }
                 El[q] = el;
                 const double Kp = (h[i] > hb + dic) ? bet_p : 0.0;
-                Gp[q] = (t_u >= t0 && ip != 1) ? Kp * Qcu[q] : 0.0;
+                Gp[q] = (t_u >= t0 && ip != 1) ? Kp * L_.qc : 0.0;
             }
             Mu[id(lfs)] = 0.0;
         }
         cloud_base_update();
         // updraftRecurrence (:1032-1163)
         if (inner && lfs > base) {
-            // the level below, carried in registers from one level to the next
+            // the level's fields and the arrays this pass reads
+            struct RecL { double t, c, cl, rh, u, v, w, sx, ps, co2, eu, du, gp; };
+            auto load_rec = [&](int i) {
+                const size_t q = id(i);
+                RecL r;
+                r.t = t_of(i); r.c = (double)C[q]; r.cl = (double)CL[q]; r.rh = (double)RH[q];
+                r.u = (double)U[q]; r.v = (double)V[q]; r.w = (double)W[q]; r.sx = Sx[q]; r.ps = PS[q];
+                r.co2 = (double)CO2[q]; r.eu = Eu[q]; r.du = Du[q]; r.gp = Gp[q];
+                return r;
+            };
+            RecL lm = load_rec(base), nx;   // lm: the level below
+            if (base + 1 <= lfs - 1) nx = load_rec(base + 1);
+            // the level below's updraft, carried in registers from one level to the next
             const size_t qb = id(base);
             double Mp = Mu[qb], Em = Eu[qb], Dm = Du[qb], Gm = Gp[qb];
             double Qvm = Qvu[qb], Qcm = Qcu[qb], Vm = Vu[qb], Wm = Wu[qb], Sm = Su[qb];
             for (int i = base + 1; i <= lfs - 1; i++) {
-                const size_t q = id(i), qm = id(i - 1);
-                const double t_u = t_of(i - 1), sp = step(i - 1);
+                const size_t q = id(i);
+                const RecL L_ = nx;
+                if (i + 1 <= lfs - 1) nx = load_rec(i + 1);
+                const double t_u = lm.t, sp = step(i - 1);
                 double m = cap(Mp + (Em - Dm) * sp, M_max);
                 if (t_u <= t_00_m2) m = 0.0;
-                const double n_qv = Mp * Qvm + sp * (Em * (double)C[qm] - Dm * Qvm);
-                const double n_qc = Mp * Qcm + sp * (-Dm * (double)CL[qm] - (double)RH[qm] * Gm);
-                const double n_v = Mp * Vm + sp * Em * (double)V[qm];
-                const double n_w = Mp * Wm + sp * Em * (double)W[qm];
-                const double n_s = (Mp * Sm + sp * (Em * Sx[qm] - Dm * Sm)) / s_0;
-                double Ei = Eu[q], Di = Du[q], Gi = Gp[q];
+                const double n_qv = Mp * Qvm + sp * (Em * lm.c - Dm * Qvm);
+                const double n_qc = Mp * Qcm + sp * (-Dm * lm.cl - lm.rh * Gm);
+                const double n_v = Mp * Vm + sp * Em * lm.v;
+                const double n_w = Mp * Wm + sp * Em * lm.w;
+                const double n_s = (Mp * Sm + sp * (Em * lm.sx - Dm * Sm)) / s_0;
+                double Ei = L_.eu, Di = L_.du, Gi = L_.gp;
                 double qv, qc, vu, wu, su;
                 if (fabs(m) > coeff_recurr) {
                     const double r = 1.0 / m;
                     qv = n_qv * r; qc = n_qc * r; vu = n_v * r; wu = n_w * r; su = n_s * r;
                 } else {
-                    su = Sx[q]; qv = (double)C[q]; qc = 0.0; vu = (double)V[q]; wu = (double)W[q];
+                    su = L_.sx; qv = L_.c; qc = 0.0; vu = L_.v; wu = L_.w;
                 }
                 double cu = 0.0;
                 if (fabs(m) > coeff_recurr) {   // the rising parcel's saturation adjustment, two passes
                     double Tu = su * s_0 / cp_l;
-                    const double pu = PS[q], cq = (double)C[q];
+                    const double pu = L_.ps, cq = L_.c;
                     double tot = 0.0;
                     for (int it = 0; it < 2; ++it) {
                         const double ex = qv - qs_env(th_p_auto(Tu), pu, cq);
                         if (ex <= 0.0) break;
                         const double L = (Tu >= t0) ? lv : ls;
                         const double G = (L / cp_l) * th_dqdT(th_p_auto(Tu), th_L_auto(Tu), Tu, pu,
-                                                              th_M_nonwater(cq, (double)CO2[q], Mb));
+                                                              th_M_nonwater(cq, L_.co2, Mb));
                         const double dc = ex / (1.0 + G);
                         qv -= dc;
                         qc += dc;
@@ -4349,9 +4423,9 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
                         tot += dc;
                     }
                     su = cp_l * Tu / s_0;
-                    cu = tot * m / (tmax((double)RH[q], 0.01) * step(i));
+                    cu = tot * m / (tmax(L_.rh, 0.01) * step(i));
                 }
-                double uu = tmax(-uu_cap, tmin((double)U[q] + m * inv_a_u / tmax((double)RH[q], 1e-6), uu_cap));
+                double uu = tmax(-uu_cap, tmin(L_.u + m * inv_a_u / tmax(L_.rh, 1e-6), uu_cap));
                 if (qv <= 0.0) qv = 0.0;
                 if (qc <= 0.0) qc = 0.0;
                 if (qc > q_c_u_max) qc = q_c_u_max;
@@ -4363,10 +4437,20 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
                 }
                 Mu[q] = m; Qvu[q] = qv; Qcu[q] = qc; Vu[q] = vu; Wu[q] = wu; Su[q] = su; Cu[q] = cu; Uu[q] = uu;
                 Mp = m; Em = Ei; Dm = Di; Gm = Gi; Qvm = qv; Qcm = qc; Vm = vu; Wm = wu; Sm = su;
+                lm = L_;
             }
         }
         // downdraftEntrainment (:913-1027)
         if (lfs > 0 && lfs > base) {
+            struct DnL { double t, qsr, cl, md, rh, u, v, w, ep, pc; };
+            auto load_dn = [&](int i) {
+                const size_t q = id(i);
+                DnL r;
+                r.t = t_of(i); r.qsr = QSR[q]; r.cl = (double)CL[q]; r.md = Md[q]; r.rh = (double)RH[q];
+                r.u = (double)U[q]; r.v = (double)V[q]; r.w = (double)W[q]; r.ep = Ep[q]; r.pc = Pc[q];
+                return r;
+            };
+            DnL nx = load_dn(lfs - 1);
             const size_t ql = id(lfs);
             const double t_l = t_of(lfs), p_l = PS[ql];
             const double mdl = cap(MdLFS, M_max);
@@ -4385,24 +4469,26 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             double mnext = mdl;   // M_d one level up, carried in a register
             for (int i = lfs - 1; i >= 0; i--) {
                 const size_t q = id(i);
-                const double t_u = t_of(i);
+                const DnL L_ = nx;
+                if (i - 1 >= 0) nx = load_dn(i - 1);
+                const double t_u = L_.t;
                 Sd[q] = cp_l * t_u / s_0;
-                const double qsd = QSR[q];
-                const double qvd = 0.5 * ((double)CL[q] + scale * qsd);
+                const double qsd = L_.qsr;
+                const double qvd = 0.5 * (L_.cl + scale * qsd);
                 Qvd[q] = qvd;
-                const double m0 = Md[q];
+                const double m0 = L_.md;
                 const double e = eps_d * fabs(m0), dd = del_d * fabs(m0);
                 Ed[q] = e; Dd[q] = dd;
                 const double m = cap(mnext - (e - dd) * step(i), M_max);
                 mnext = m;
                 Md[q] = m;
-                Ud[q] = (double)U[q] + m * inv_a_d / tmax((double)RH[q], 1e-6);
-                Vd[q] = (double)V[q]; Wd[q] = (double)W[q];
+                Ud[q] = L_.u + m * inv_a_d / tmax(L_.rh, 1e-6);
+                Vd[q] = L_.v; Wd[q] = L_.w;
                 if (i <= base) {
                     double ep = 0.0;
                     if (ip != 1) {
-                        ep = Ep[q];
-                        const double pr = Pc[q];
+                        ep = L_.ep;
+                        const double pr = L_.pc;
                         if (pr > 0.0 && sig > 0.0) ep = sig * alf_1 * (qsd - qvd) * sqrt(1e1 / alf_2 * pr / sig);
                         if (ep <= 0.0) ep = 0.0;
                     }
@@ -4410,7 +4496,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
                 }
                 double ed = 0.0;
                 if (ip != 1) {
-                    const double pr = Pc[q];
+                    const double pr = L_.pc;
                     if (pr > 0.0) ed = a_ev * (1.0 + b_ev * pow(pr, 1.0 / 6.0)) * (qsd - qvd) * pow(pr, 4.0 / 9.0);
                     if (ed <= 0.0) ed = 0.0;
                 }
@@ -4420,31 +4506,44 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
         // downdraftRecurrence (:1168-1254), every column
         // the level above, carried in registers from one level to the next
         double Mp = 0, Ep1 = 0, Dp1 = 0, ev = 0, Qvp = 0, Vp = 0, Wp = 0, Sp = 0, Pp = 0;
+        struct DrL { double t, rh, c, u, v, w, sx, ed, dd, edv, ep, gp; };
+        auto load_dr = [&](int i) {
+            const size_t q = id(i);
+            DrL r;
+            r.t = t_of(i); r.rh = (double)RH[q]; r.c = (double)C[q]; r.u = (double)U[q]; r.v = (double)V[q];
+            r.w = (double)W[q]; r.sx = Sx[q]; r.ed = Ed[q]; r.dd = Dd[q]; r.edv = Edv[q]; r.ep = Ep[q]; r.gp = Gp[q];
+            return r;
+        };
+        DrL up, nx;   // up: the level above
         if (lfs > 0) {
+            nx = load_dr(lfs - 1);
+            up = load_dr(lfs);
             const size_t ql = id(lfs);
             Mp = Md[ql]; Ep1 = Ed[ql]; Dp1 = Dd[ql]; ev = Edv[ql];
             Qvp = Qvd[ql]; Vp = Vd[ql]; Wp = Wd[ql]; Sp = Sd[ql]; Pp = Pc[ql];
         }
         for (int i = lfs - 1; i >= 0; i--) {
-            const size_t q = id(i), qp = id(i + 1);
-            const double t_u = t_of(i), sp = step(i + 1), rp = (double)RH[qp];
-            double Ei = Ed[q], Di = Dd[q];
+            const size_t q = id(i);
+            const DrL L_ = nx;
+            if (i - 1 >= 0) nx = load_dr(i - 1);
+            const double t_u = L_.t, sp = step(i + 1), rp = up.rh;
+            double Ei = L_.ed, Di = L_.dd;
             double m = cap(Mp - (Ei - Di) * sp, M_max);
             if (t_u <= t_00_m2) m = 0.0;
             const double L = (t_u >= t0) ? lv : ls;
-            const double n_qv = Mp * Qvp - sp * (Ep1 * (double)C[qp] - Dp1 * Qvp + rp * ev);
-            const double n_v = Mp * Vp - sp * (Ep1 * (double)V[qp] - Dp1 * Vp);
-            const double n_w = Mp * Wp - sp * (Ep1 * (double)W[qp] - Dp1 * Wp);
-            const double n_s = (Mp * Sp - sp * (Ep1 * Sx[qp] - Dp1 * Sp - L * rp * ev)) / s_0;
-            double evi = Edv[q], epi = Ep[q];
+            const double n_qv = Mp * Qvp - sp * (Ep1 * up.c - Dp1 * Qvp + rp * ev);
+            const double n_v = Mp * Vp - sp * (Ep1 * up.v - Dp1 * Vp);
+            const double n_w = Mp * Wp - sp * (Ep1 * up.w - Dp1 * Wp);
+            const double n_s = (Mp * Sp - sp * (Ep1 * up.sx - Dp1 * Sp - L * rp * ev)) / s_0;
+            double evi = L_.edv, epi = L_.ep;
             double qv, vd, wd, sd;
             if (fabs(m) > coeff_recurr) {
                 const double r = 1.0 / m;
                 qv = n_qv * r; vd = n_v * r; wd = n_w * r; sd = n_s * r;
             } else {
-                qv = (double)C[q]; vd = (double)V[q]; wd = (double)W[q]; sd = Sx[q];
+                qv = L_.c; vd = L_.v; wd = L_.w; sd = L_.sx;
             }
-            double ud = (double)U[q] + m * inv_a_d / tmax((double)RH[q], 1e-6);
+            double ud = L_.u + m * inv_a_d / tmax(L_.rh, 1e-6);
             if (qv <= 0.0) qv = 0.0;
             if (sd <= 0.0) sd = 0.0;
             if (t_u <= t_00) {
@@ -4454,39 +4553,55 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             }
             Md[q] = m; Qvd[q] = qv; Vd[q] = vd; Wd[q] = wd; Sd[q] = sd; Ud[q] = ud;
             if (i > 0) {
-                Pp = tmax(0.0, Pp + step(i) * (double)RH[q] * (Gp[q] - evi - epi));
+                Pp = tmax(0.0, Pp + step(i) * L_.rh * (L_.gp - evi - epi));
                 Pc[q] = Pp;
             }
             Mp = m; Ep1 = Ei; Dp1 = Di; ev = evi; Qvp = qv; Vp = vd; Wp = wd; Sp = sd;
+            up = L_;
         }
         Pc[id(0)] = Pc[id(1)];
     }
-    // rhsForcing (:1304-1417)
-    double* __restrict__ MCt = d.mc[MC_MC_T]; double* __restrict__ MCq = d.mc[MC_MC_Q];
-    double* __restrict__ MCv = d.mc[MC_MC_V]; double* __restrict__ MCw = d.mc[MC_MC_W];
-    for (int i = 0; i < im; i++) { Mu[id(i)] = cap(Mu[id(i)], M_max); Md[id(i)] = cap(Md[id(i)], M_max); }
-    double fs0 = 0, fq0 = 0, fv0 = 0, fw0 = 0;
-    auto fluxes = [&](int i, double& fs, double& fq, double& fv, double& fw) {
+    // rhsForcing (:1304-1417). The caps of M_u and M_d over the column come first in the
+    // reference; here each level is capped (and stored back) where its fluxes are formed, which
+    // is before any flux reads it.
+    ArrP MCt = (ArrP)d.mc[MC_MC_T]; ArrP MCq = (ArrP)d.mc[MC_MC_Q];
+    ArrP MCv = (ArrP)d.mc[MC_MC_V]; ArrP MCw = (ArrP)d.mc[MC_MC_W];
+    struct FxL { double mu, md, s, su, sd, c, qvu, qvd, v, vu, vd, w, wu, wd, t, rh, cu, edv, el, ep; };
+    auto load_fx = [&](int i) {
         const size_t q = id(i);
-        const double mu = Mu[q], md = Md[q], s = Sx[q], c = (double)C[q], v = (double)V[q], w = (double)W[q];
-        fs = mu * (Su[q] - s) + md * (Sd[q] - s);
-        fq = mu * (Qvu[q] - c) + md * (Qvd[q] - c);
-        fv = mu * (Vu[q] - v) + md * (Vd[q] - v);
-        fw = mu * (Wu[q] - w) + md * (Wd[q] - w);
+        FxL x;
+        x.mu = Mu[q]; x.md = Md[q]; x.s = Sx[q]; x.su = Su[q]; x.sd = Sd[q]; x.c = (double)C[q]; x.qvu = Qvu[q];
+        x.qvd = Qvd[q]; x.v = (double)V[q]; x.vu = Vu[q]; x.vd = Vd[q]; x.w = (double)W[q]; x.wu = Wu[q];
+        x.wd = Wd[q]; x.t = (double)T[q]; x.rh = (double)RH[q]; x.cu = Cu[q]; x.edv = Edv[q]; x.el = El[q];
+        x.ep = Ep[q];
+        return x;
     };
-    fluxes(0, fs0, fq0, fv0, fw0);
+    auto fluxes = [&](int i, const FxL& x, double& fs, double& fq, double& fv, double& fw) {
+        const size_t q = id(i);
+        const double mu = cap(x.mu, M_max), md = cap(x.md, M_max);
+        Mu[q] = mu; Md[q] = md;
+        fs = mu * (x.su - x.s) + md * (x.sd - x.s);
+        fq = mu * (x.qvu - x.c) + md * (x.qvd - x.c);
+        fv = mu * (x.vu - x.v) + md * (x.vd - x.v);
+        fw = mu * (x.wu - x.w) + md * (x.wd - x.w);
+    };
+    FxL a = load_fx(0), b = load_fx(1), c2;
+    double fs0 = 0, fq0 = 0, fv0 = 0, fw0 = 0;
+    fluxes(0, a, fs0, fq0, fv0, fw0);
     for (int i = 0; i < im - 1; i++) {
         const size_t q = id(i);
+        if (i + 2 < im) c2 = load_fx(i + 2);
         double fs1, fq1, fv1, fw1;
-        fluxes(i + 1, fs1, fq1, fv1, fw1);
-        const double L = ((double)T[q] * t0 >= t0) ? lv : ls;
-        const double isr = 1.0 / (step(i) * tmax((double)RH[q], 0.01));
-        const double src = Cu[q] - Edv[q] - El[q] - Ep[q];
+        fluxes(i + 1, b, fs1, fq1, fv1, fw1);
+        const double L = (a.t * t0 >= t0) ? lv : ls;
+        const double isr = 1.0 / (step(i) * tmax(a.rh, 0.01));
+        const double src = a.cu - a.edv - a.el - a.ep;
         MCt[q] = cap(-(fs1 - fs0) * isr * t0 + (L / cp_l) * src * t0, MCt_max);
         MCq[q] = cap(-(fq1 - fq0) * isr - src, MCq_max);
         MCv[q] = cap(-(fv1 - fv0) * isr * K.u_0, MCv_max);
         MCw[q] = cap(-(fw1 - fw0) * isr * K.u_0, MCv_max);
         fs0 = fs1; fq0 = fq1; fv0 = fv1; fw0 = fw1;
+        a = b; b = c2;
     }
 }
 
