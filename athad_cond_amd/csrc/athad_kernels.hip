@@ -2188,49 +2188,77 @@ ATHAD_DEV void ice_column_seq(const MemberDesc& M, int j, int k) {
     const int im = K.im;
     const size_t plane = (size_t)K.jm * K.km;
     const size_t base = (size_t)j * K.km + k;   // cell (0, j, k); level i adds i * plane
-    S *ft = fp<S>(M, F_T), *fc = fp<S>(M, F_C), *fcl = fp<S>(M, F_CLOUD), *fi = fp<S>(M, F_ICE);
-    S *fg = fp<S>(M, F_GR), *fscc = fp<S>(M, F_S_C_C);
-    const S *fco2 = fp<S>(M, F_CO2), *frh = fp<S>(M, F_R_HUMID);
-    const double* fps = (const double*)M.f[F_P_STAT];
-    S *Pr = mpp<S>(M, MP_P_RAIN), *Ps = mpp<S>(M, MP_P_SNOW), *Prn = mpp<S>(M, MP_P_RAINN), *Psn = mpp<S>(M, MP_P_SNOWN);
-    S *Pt = mpp<S>(M, MP_PRECIP);
-    S *Sv = mpp<S>(M, MP_S_V), *Sc = mpp<S>(M, MP_S_C), *Si = mpp<S>(M, MP_S_I), *Sr = mpp<S>(M, MP_S_R), *Ss = mpp<S>(M, MP_S_S);
+    // global-memory pointers (descriptor pointers would compile to flat accesses)
+    typedef G1<S>* __restrict__ P;
+    P ft = (P)M.f[F_T]; P fc = (P)M.f[F_C]; P fcl = (P)M.f[F_CLOUD]; P fi = (P)M.f[F_ICE];
+    P fg = (P)M.f[F_GR]; P fscc = (P)M.f[F_S_C_C];
+    const G1<S>* __restrict__ fco2 = (const G1<S>*)M.f[F_CO2];
+    const G1<S>* __restrict__ frh = (const G1<S>*)M.f[F_R_HUMID];
+    const G1<double>* __restrict__ fps = (const G1<double>*)M.f[F_P_STAT];
+    P Pr = (P)M.mp[MP_P_RAIN]; P Ps = (P)M.mp[MP_P_SNOW]; P Prn = (P)M.mp[MP_P_RAINN]; P Psn = (P)M.mp[MP_P_SNOWN];
+    P Pt = (P)M.mp[MP_PRECIP];
+    P Sv = (P)M.mp[MP_S_V]; P Sc = (P)M.mp[MP_S_C]; P Si = (P)M.mp[MP_S_I]; P Sr = (P)M.mp[MP_S_R];
+    P Ss = (P)M.mp[MP_S_S];
+    auto at = [&](int i) { return base + (size_t)i * plane; };
 
-    for (int i = 0; i < im; i++) {   // initArrays
-        const size_t id = base + (size_t)i * plane;
-        fc[id] = tmax(S(0), fc[id]); fcl[id] = tmax(S(0), fcl[id]); fi[id] = tmax(S(0), fi[id]);
-        const S rn = tmax(S(0), Prn[id]), sn = tmax(S(0), Psn[id]);
-        Prn[id] = rn; Psn[id] = sn; Pr[id] = rn; Ps[id] = sn;
+    // Each level loop reads the next level before it stores this one: a wave's vector-memory
+    // counter retires in issue order, so a load issued after a store would wait for the store.
+    {   // initArrays
+        S c_n = fc[at(0)], cl_n = fcl[at(0)], i_n = fi[at(0)], rn_n = Prn[at(0)], sn_n = Psn[at(0)];
+        for (int i = 0; i < im; i++) {
+            const size_t id = at(i);
+            const S c0 = c_n, cl0 = cl_n, i0 = i_n, rn0 = rn_n, sn0 = sn_n;
+            if (i + 1 < im) {
+                const size_t nx = at(i + 1);
+                c_n = fc[nx]; cl_n = fcl[nx]; i_n = fi[nx]; rn_n = Prn[nx]; sn_n = Psn[nx];
+            }
+            fc[id] = tmax(S(0), c0); fcl[id] = tmax(S(0), cl0); fi[id] = tmax(S(0), i0);
+            const S rn = tmax(S(0), rn0), sn = tmax(S(0), sn0);
+            Prn[id] = rn; Psn[id] = sn; Pr[id] = rn; Ps[id] = sn;
+        }
     }
     if (j < 1 || j > K.jm - 2 || k < 1 || k > K.km - 2) return;
 
-    const double Rain_check = (double)Pr[base + (size_t)i_check * plane];
+    const double Rain_check = (double)Pr[at(i_check)];
     double m_i = m_i_max, N_i = 0.0, S_nuc = 0.0, S_i_dep = 0.0;
     // q_sw and q_Ice of the levels that could condense in the previous pass (their t and c are
     // unchanged), kept in scratch 10 and 11 for the later passes: FP64 storage, up to 64 levels
     constexpr bool kCache = sizeof(S) == 8;
-    double* qc0 = (double*)M.tmp[10];
-    double* qc1 = (double*)M.tmp[11];
+    G1<double>* __restrict__ qc0 = (G1<double>*)M.tmp[10];
+    G1<double>* __restrict__ qc1 = (G1<double>*)M.tmp[11];
     unsigned long long cached = 0ULL;
+    // a level's inputs to one pass
+    struct IceL { double t, c, cl, ic, gr, co2, ps, rh, scc, pr, psn, q0, q1; bool hit; };
+    auto load_ice = [&](int i) {
+        const size_t id = at(i);
+        IceL v;
+        v.t = (double)ft[id]; v.c = (double)fc[id]; v.cl = (double)fcl[id]; v.ic = (double)fi[id];
+        v.gr = (double)fg[id]; v.co2 = (double)fco2[id]; v.ps = fps[id]; v.rh = (double)frh[id];
+        v.scc = (double)fscc[id]; v.pr = (double)Pr[id]; v.psn = (double)Ps[id];
+        v.hit = kCache && i < 64 && ((cached >> i) & 1ULL);
+        v.q0 = 0.0; v.q1 = 0.0;
+        if (v.hit) { v.q0 = qc0[id]; v.q1 = qc1[id]; }
+        return v;
+    };
     for (int pass = 1; pass <= passes; pass++) {
-        Pr[base + (size_t)(im - 1) * plane] = S(0);
-        Ps[base + (size_t)(im - 1) * plane] = S(0);
+        IceL nx = load_ice(im - 2);
+        Pr[at(im - 1)] = S(0);
+        Ps[at(im - 1)] = S(0);
         double pr_up = 0.0, ps_up = 0.0;   // fluxes of the level above, as just stored
         unsigned long long can_mask = 0ULL;
         for (int i = im - 2; i >= 0; i--) {
-            const size_t id = base + (size_t)i * plane;
+            const size_t id = at(i);
+            const IceL V = nx;
+            if (i - 1 >= 0) nx = load_ice(i - 1);
             Level L;
-            const double cv = (double)fc[id], co2 = (double)fco2[id];
-            const bool hit = kCache && i < 64 && ((cached >> i) & 1ULL);
-            double qv[2];
-            if (hit) { qv[0] = qc0[id]; qv[1] = qc1[id]; }
-            if (!load_level(K, M.hgt, i, (double)ft[id], cv, (double)fcl[id], (double)fi[id], co2, fps[id],
-                            (double)frh[id], (double)fscc[id], (double)Pr[id], (double)Ps[id], L, hit ? qv : nullptr)) {
+            const double cv = V.c, co2 = V.co2;
+            double qv[2] = {V.q0, V.q1};
+            if (!load_level(K, M.hgt, i, V.t, cv, V.cl, V.ic, co2, V.ps, V.rh, V.scc, V.pr, V.psn, L,
+                            V.hit ? qv : nullptr)) {
                 double t_new = 0.0, c_new;
                 bool t_set;
-                evaporate(K, L.t_u, cv, (double)fcl[id], (double)fi[id], (double)fg[id], co2, (double)frh[id],
-                          (double)Pr[id], (double)Ps[id], t_new, t_set, c_new);
-                if (tmax(0.0, (double)fcl[id]) + tmax(0.0, (double)fi[id]) + tmax(0.0, (double)fg[id]) > 0.0) {
+                evaporate(K, L.t_u, cv, V.cl, V.ic, V.gr, co2, V.rh, V.pr, V.psn, t_new, t_set, c_new);
+                if (tmax(0.0, V.cl) + tmax(0.0, V.ic) + tmax(0.0, V.gr) > 0.0) {
                     fcl[id] = S(0); fi[id] = S(0); fg[id] = S(0);
                 }
                 if (t_set) ft[id] = S(t_new);
@@ -2242,7 +2270,7 @@ ATHAD_DEV void ice_column_seq(const MemberDesc& M, int j, int k) {
                 continue;
             }
             if (kCache && i < 64 && im <= 64) {
-                if (!hit && pass < passes) { qc0[id] = L.q_sw; qc1[id] = L.q_Ice; }
+                if (!V.hit && pass < passes) { qc0[id] = L.q_sw; qc1[id] = L.q_Ice; }
                 can_mask |= 1ULL << i;
             }
             assign_Ni_mi(L, K.t_0, N_i, m_i);
@@ -2256,7 +2284,7 @@ ATHAD_DEV void ice_column_seq(const MemberDesc& M, int j, int k) {
             Pt[id] = S(tmin(P_max, (double)S(pr) + (double)S(psn)));
             pr_up = (double)S(pr); ps_up = (double)S(psn);
         }
-        if (fabs((double)Pr[base + (size_t)i_check * plane] - Rain_check) * 8.64e4 <= 1.0e-3) break;
+        if (fabs((double)Pr[at(i_check)] - Rain_check) * 8.64e4 <= 1.0e-3) break;
         cached = can_mask;
     }
 }
