@@ -1234,7 +1234,6 @@ template <typename S, typename C, int SRC> struct NbRegs {
             c0[f] = C(gld(X[f], id));
             b0[f] = C(gld(fp<S>(M, kProgN[f]), id));
         }
-
         const S* p = fp<S>(M, F_P_DYN);
         pd[0] = C(gld(p, id + si)); pd[1] = C(gld(p, id - si)); pd[2] = C(gld(p, id + sj));
         pd[3] = C(gld(p, id - sj)); pd[4] = C(gld(p, id + 1)); pd[5] = C(gld(p, id - 1));
@@ -1306,6 +1305,8 @@ template <typename C> ATHAD_DEV C stage_update(int st, C r, C* b, C* a, C dt, C 
 }
 
 constexpr int LB = 256;   // LDS slot stride = threads per block of the RK4 kernels
+// SB: stride of a thread's stage slots (base, accumulator, recorded velocities) in LDS. (Private
+// slots in registers for the lex plane kernel, stride 1, measured no faster.)
 
 // The four RK4 stages of a field group: a loop (U = false: the code of one stage, the fewest
 // registers, for the Jacobi kernel's occupancy) or unrolled (U = true: every stage's branches
@@ -1324,7 +1325,7 @@ template <bool U, typename F> ATHAD_DEV void for_stages(F&& f) {
 // tendencies involve no other prognostic field, so the group integrates first and records
 // the velocity entering each stage in rec[] for the scalar groups. cen[] = start values in,
 // final values out. A, B: velocity-gradient invariants of the production term.
-template <typename S, typename C, bool CLAMP, int SRC, bool U, typename Nb>
+template <typename S, typename C, bool CLAMP, int SRC, bool U, int SB, typename Nb>
 ATHAD_DEV void rk4_dyn(const MemberDesc& M, const Nb& src_nb, int i, const CellGeo<C>& g,
                        C* ba, C* rec, C* cen, C* aux, C& A, C& B) {
     const Consts& K = M.K;
@@ -1396,7 +1397,7 @@ ATHAD_DEV void rk4_dyn(const MemberDesc& M, const Nb& src_nb, int i, const CellG
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt_sixth);
 for_stages<U>([&](const int st) {
         const C T = cen[FT], u = cen[FU], v = cen[FV], w = cen[FW];
-        rec[(3 * st + 0) * LB] = u; rec[(3 * st + 1) * LB] = v; rec[(3 * st + 2) * LB] = w;
+        rec[(3 * st + 0) * SB] = u; rec[(3 * st + 1) * SB] = v; rec[(3 * st + 2) * SB] = w;
         const C ua = u * g.aR, va = v * g.aT, wa = w * g.aP;
         C rT = ce * (u * PR + v * PT + w * PP) - minmod3(nb[FT], T, ua, va, wa) + (L0T - g.Lc2 * T) * dif_t;
         if (SRC & SRC_CONV) rT = rT + mcT;
@@ -1409,10 +1410,10 @@ for_stages<U>([&](const int st) {
         rV = rV - g.drag * v;
         rW = rW - g.drag * w;
         if (st == 3) { aux[0] = rU + PR; aux[1] = rV + PT; aux[2] = rW + PP; }
-        cen[FT] = stage_update(st, rT, ba + 0 * LB, ba + 4 * LB, dt, half_dt, dt_sixth);
-        cen[FU] = stage_update(st, rU, ba + 1 * LB, ba + 5 * LB, dt, half_dt, dt_sixth);
-        cen[FV] = stage_update(st, rV, ba + 2 * LB, ba + 6 * LB, dt, half_dt, dt_sixth);
-        cen[FW] = stage_update(st, rW, ba + 3 * LB, ba + 7 * LB, dt, half_dt, dt_sixth);
+        cen[FT] = stage_update(st, rT, ba + 0 * SB, ba + 4 * SB, dt, half_dt, dt_sixth);
+        cen[FU] = stage_update(st, rU, ba + 1 * SB, ba + 5 * SB, dt, half_dt, dt_sixth);
+        cen[FV] = stage_update(st, rV, ba + 2 * SB, ba + 6 * SB, dt, half_dt, dt_sixth);
+        cen[FW] = stage_update(st, rW, ba + 3 * SB, ba + 7 * SB, dt, half_dt, dt_sixth);
     });
 }
 
@@ -1421,7 +1422,7 @@ for_stages<U>([&](const int st) {
 // field's tendency involves only itself and the velocities, so the group can run whole or in
 // parts. cen[0..NQ-1] = start values in, final values out; ba holds NQ bases, then NQ
 // accumulators.
-template <typename S, typename C, bool CLAMP, int SRC, bool U, int Q0, int NQ, typename Nb, typename Vel>
+template <typename S, typename C, bool CLAMP, int SRC, bool U, int SB, int Q0, int NQ, typename Nb, typename Vel>
 ATHAD_DEV void rk4_moist(const MemberDesc& M, const Nb& src_nb, const CellGeo<C>& g, C* ba,
                          Vel vel, C* cen) {
     const Consts& K = M.K;
@@ -1475,7 +1476,7 @@ for_stages<U>([&](const int st) {
         }
 #pragma unroll
         for (int p = 0; p < NQ; p++) {
-            const C val = stage_update(st, r[p], ba + p * LB, ba + (NQ + p) * LB, dt, half_dt, dt_sixth);
+            const C val = stage_update(st, r[p], ba + p * SB, ba + (NQ + p) * SB, dt, half_dt, dt_sixth);
             cen[p] = (Q0 + p < 4) ? tmax(C(0), val) : val;
         }
     });
@@ -1484,7 +1485,7 @@ for_stages<U>([&](const int st) {
 // RK4 of the turbulence group tke, dis with the k-omega SST closure (RHS_Atm_Turb.cpp:633-721),
 // advected by the recorded stage velocities. cen[0..1] = start values in, final values out;
 // prod and the sources are those of the fourth stage.
-template <typename S, typename C, bool CLAMP, bool U, typename Nb, typename Vel>
+template <typename S, typename C, bool CLAMP, bool U, int SB, typename Nb, typename Vel>
 ATHAD_DEV void rk4_turb(const MemberDesc& M, const Nb& src_nb, int i, int j, int k,
                         const CellGeo<C>& g, C* ba, Vel vel, C A, C B, C* cen, C& prod, C& tsrc, C& dsrc) {
     const Consts& K = M.K;
@@ -1549,9 +1550,9 @@ for_stages<U>([&](const int st) {
         dsrc = tclamp(P_w - Y_w + D_w, -dsm, dsm);
         const C rk = -(u * gc[0][0] + v * gc[0][1] + w * gc[0][2]) + (L0[0] - g.Lc2 * tke) * dif_tke + tsrc;
         const C rd = -(u * gc[1][0] + v * gc[1][1] + w * gc[1][2]) + (L0[1] - g.Lc2 * dis) * dif_dis + dsrc;
-        C vk = stage_update(st, rk, ba + 0 * LB, ba + 2 * LB, dt, half_dt, dt_sixth);
+        C vk = stage_update(st, rk, ba + 0 * SB, ba + 2 * SB, dt, half_dt, dt_sixth);
         vk = finite_bits(vk) ? ((vk < C(0)) ? C(0) : (vk > tkm) ? tkm : vk) : C(0);
-        const C vd = stage_update(st, rd, ba + 1 * LB, ba + 3 * LB, dt, half_dt, dt_sixth);
+        const C vd = stage_update(st, rd, ba + 1 * SB, ba + 3 * SB, dt, half_dt, dt_sixth);
         cen[0] = vk;
         cen[1] = tmax(C(1.0e-10), vd);
     });
@@ -1574,12 +1575,12 @@ template <typename S> constexpr int rk4_waves() { return sizeof(S) == 8 ? ATHAD_
 template <typename S> constexpr int rk4_row_waves() { return sizeof(S) == 8 ? ATHAD_RK4_ROW_WAVES_F64 : ATHAD_RK4_ROW_WAVES_F32; }
 
 // Start values (optionally clamped) into cen[] and the bases into the LDS slots of a group.
-template <typename C, typename Nb>
+template <int SB, typename C, typename Nb>
 ATHAD_DEV void group_start(const Nb& src_nb, int f0, int nf, C* cen, C* ba) {
 #pragma unroll
     for (int q = 0; q < nf; q++) {
         cen[q] = src_nb.start(f0 + q);
-        ba[q * LB] = src_nb.base(f0 + q);
+        ba[q * SB] = src_nb.base(f0 + q);
     }
 }
 
@@ -1587,7 +1588,7 @@ ATHAD_DEV void group_start(const Nb& src_nb, int f0, int nf, C* cen, C* ba) {
 // writes a final value; the aux / prod / source outputs are written here.
 // GT: where the metric coefficients come from (0: formed from the 1-D tables, 1: the (i, j)
 // table read per lane, 2: the (i, j) table read by scalar loads, i and j wave-uniform).
-template <typename S, typename C, bool CLAMP, int SRC, bool U, int GT, bool SPLIT, typename Nb, typename Out, typename OutX>
+template <typename S, typename C, bool CLAMP, int SRC, bool U, int GT, bool SPLIT, int SB, typename Nb, typename Out, typename OutX>
 ATHAD_DEV void rk4_cell(const MemberDesc& M, const Nb& nb, int i, int j, int k, C* ba, C* rec, Out out, OutX outx) {
     CellGeo<C> g;
     if constexpr (GT == 2) cell_geo_row<C>(M, i, j, nb.ps(), nb.psurf(), g);
@@ -1596,8 +1597,8 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const Nb& nb, int i, int j, int k, 
     C A, B;
     {
         C cen[4], aux[3];
-        group_start(nb, FT, 4, cen, ba);
-        rk4_dyn<S, C, CLAMP, SRC, U>(M, nb, i, g, ba, rec, cen, aux, A, B);
+        group_start<SB>(nb, FT, 4, cen, ba);
+        rk4_dyn<S, C, CLAMP, SRC, U, SB>(M, nb, i, g, ba, rec, cen, aux, A, B);
 #pragma unroll
         for (int q = 0; q < 4; q++) out(FT + q, cen[q]);
         outx(F_AUX_U, aux[0]);
@@ -1605,34 +1606,34 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const Nb& nb, int i, int j, int k, 
         outx(F_AUX_W, aux[2]);
     }
     auto vel = [&](int st, C& u, C& v, C& w) {
-        u = rec[(3 * st + 0) * LB]; v = rec[(3 * st + 1) * LB]; w = rec[(3 * st + 2) * LB];
+        u = rec[(3 * st + 0) * SB]; v = rec[(3 * st + 1) * SB]; w = rec[(3 * st + 2) * SB];
     };
     if constexpr (SPLIT) {   // c, cloud, ice, then gr, co2: six LDS slots instead of ten
         {
             C cen[3];
-            group_start(nb, FC, 3, cen, ba);
-            rk4_moist<S, C, CLAMP, SRC, U, 0, 3>(M, nb, g, ba, vel, cen);
+            group_start<SB>(nb, FC, 3, cen, ba);
+            rk4_moist<S, C, CLAMP, SRC, U, SB, 0, 3>(M, nb, g, ba, vel, cen);
 #pragma unroll
             for (int q = 0; q < 3; q++) out(FC + q, cen[q]);
         }
         {
             C cen[2];
-            group_start(nb, FC + 3, 2, cen, ba);
-            rk4_moist<S, C, CLAMP, SRC, U, 3, 2>(M, nb, g, ba, vel, cen);
+            group_start<SB>(nb, FC + 3, 2, cen, ba);
+            rk4_moist<S, C, CLAMP, SRC, U, SB, 3, 2>(M, nb, g, ba, vel, cen);
 #pragma unroll
             for (int q = 0; q < 2; q++) out(FC + 3 + q, cen[q]);
         }
     } else {
         C cen[5];
-        group_start(nb, FC, 5, cen, ba);
-        rk4_moist<S, C, CLAMP, SRC, U, 0, 5>(M, nb, g, ba, vel, cen);
+        group_start<SB>(nb, FC, 5, cen, ba);
+        rk4_moist<S, C, CLAMP, SRC, U, SB, 0, 5>(M, nb, g, ba, vel, cen);
 #pragma unroll
         for (int q = 0; q < 5; q++) out(FC + q, cen[q]);
     }
     {
         C cen[2], prod, ts, ds;
-        group_start(nb, FK, 2, cen, ba);
-        rk4_turb<S, C, CLAMP, U>(M, nb, i, j, k, g, ba, vel, A, B, cen, prod, ts, ds);
+        group_start<SB>(nb, FK, 2, cen, ba);
+        rk4_turb<S, C, CLAMP, U, SB>(M, nb, i, j, k, g, ba, vel, A, B, cen, prod, ts, ds);
         out(FK, cen[0]);
         out(FD, cen[1]);
         outx(F_PROD, prod);
@@ -1691,7 +1692,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
     const unsigned si = (unsigned)(K.jm * K.km), nfld = (unsigned)K.n * E;
     const NbRow<S, C, FUSED> nb{M, (unsigned)id * E, (unsigned)(j * K.km + k) * 8u, si * E, (unsigned)K.km * E,
                                 nfld + (2u * si + 2u) * E, nfld};
-    rk4_cell<S, C, FUSED, SRC, false, sizeof(S) == 4, true>(M, nb, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
+    rk4_cell<S, C, FUSED, SRC, false, sizeof(S) == 4, true, LB>(M, nb, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
         [&](int f, C v) {
             nb.put(M.alt[f], v);
             if (FUSED && f != FU && f != FV && f != FW) nb.put(M.f[kProgN[f]], v);
@@ -1794,7 +1795,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_row_wave
     const unsigned si = (unsigned)(K.jm * K.km), nfld = (unsigned)K.n * E;
     const NbRow<S, C, FUSED> nb{M, (unsigned)id * E, (unsigned)(j * K.km + k) * 8u, si * E, (unsigned)K.km * E,
                                 nfld + (2u * si + 2u) * E, nfld};
-    rk4_cell<S, C, FUSED, SRC, false, 2, true>(M, nb, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x, [&](int f, C v) {
+    rk4_cell<S, C, FUSED, SRC, false, 2, true, LB>(M, nb, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x, [&](int f, C v) {
         nb.put(M.alt[f], v);
         if (FUSED && f != FU && f != FV && f != FW) nb.put(M.f[kProgN[f]], v);
     }, [&](int fid, C v) { nb.put(M.f[fid], v); });
@@ -1821,7 +1822,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     for (int f = 0; f < NPROG; f++) X[f] = fp<S>(M, kProgF[f]);
     const NbRegs<S, C, SRC> nb(M, (const S* const*)X, id, (long long)K.jm * K.km, (long long)K.km,
                                (long long)ix3(K, 0, j, k));
-    rk4_cell<S, C, false, SRC, true, true, false>(M, nb, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
+    rk4_cell<S, C, false, SRC, true, true, false, LB>(M, nb, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
                           [&](int f, C v) { gst(X[f], id, S(v)); },
                           [&](int fid, C v) { gst(fp<S>(M, fid), id, S(v)); });
 }
@@ -3833,7 +3834,8 @@ template <typename S> void rk4_lex_s(const Batch& b, const dim3& g, int plane, i
 }  // namespace
 // FP64: the row-segment kernel, waves taken i fastest, then the longitude segment, then j
 // (measured at 41 x 181 x 361: 0.445 ms against 0.476 with the storage order and 0.498 with
-// the segment slowest; the cell-linear FP64 kernel 0.530). FP32: the cell-linear kernel at five
+// the segment slowest, 0.453 with the longitudes in two halves each swept i fastest, then j;
+// the cell-linear FP64 kernel 0.530). FP32: the cell-linear kernel at five
 // waves per SIMD (0.214 ms; the row kernel 0.245).
 void launch_rk4_jacobi(const Batch& b, bool fused, int src) {
     if (!b.fp32) {
