@@ -45,6 +45,7 @@ EXPORTS = [
     "athad_set_radiation", "athad_moist_convection", "athad_set_conv_field", "athad_get_conv_field",
     "athad_ice_scheme", "athad_set_ice_scheme", "athad_set_mp_field", "athad_get_mp_field",
     "athad_field_minmax_noalloc", "athad_project_initial_velocity", "athad_set_moist_convection",
+    "athad_get_conv_cape",
 ]
 
 # Arrays of the two-category ice scheme (athad_mp_field in include/athad_hip.h)
@@ -143,6 +144,7 @@ def lib():
         L.athad_moist_convection.argtypes = [vp]
         L.athad_set_conv_field.argtypes = [vp, i, i, dp]
         L.athad_get_conv_field.argtypes = [vp, i, i, dp]
+        L.athad_get_conv_cape.argtypes = [vp, i, dp]
         L.athad_get_mp_field.argtypes = [vp, i, i, dp]
         L.athad_residuum.argtypes = [vp, i, dp, ip, dp, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64), ip]

@@ -88,6 +88,7 @@ struct MemberHost {
     double *evap = nullptr, *conv_cos = nullptr;
     double* conv_qs[2] = {};
     double* conv_tmp[2] = {};
+    double* conv_cape = nullptr;   // im doubles, then im ints (the winning column per level)
     int iter_n = 0, total = 0;     // host mirror of the device counters
     // table offsets inside 'tables'
     size_t o_rad = 0, o_sint = 0, o_cost = 0, o_sinr = 0, o_hgt = 0, o_nuef = 0, o_step = 0, o_geo = 0;
@@ -340,6 +341,8 @@ void make_desc(const athad_ctx* c, const MemberHost& m, int parity, MemberDesc& 
     d.evap = m.evap; d.conv_cos = m.conv_cos;
     d.conv_qs[0] = m.conv_qs[0]; d.conv_qs[1] = m.conv_qs[1];
     d.conv_tmp[0] = m.conv_tmp[0]; d.conv_tmp[1] = m.conv_tmp[1];
+    d.conv_cape = m.conv_cape;
+    d.conv_cape_col = m.conv_cape ? (int*)(m.conv_cape + m.g.im) : nullptr;
     (void)c;
 }
 
@@ -382,6 +385,7 @@ void free_ctx(athad_ctx* c) {
         if (m.conv_cos) hipFree(m.conv_cos);
         for (auto p : m.conv_qs) if (p) hipFree(p);
         for (auto p : m.conv_tmp) if (p) hipFree(p);
+        if (m.conv_cape) hipFree(m.conv_cape);
         for (void* p : {(void*)m.vel_star, (void*)m.t_top, (void*)m.t_ref, (void*)m.tables, (void*)m.scratch,
                         (void*)m.counters, (void*)m.passes, (void*)m.radiation, (void*)m.epsilon, (void*)m.albedo})
             if (p) hipFree(p);
@@ -1231,6 +1235,7 @@ static athad_status ensure_mc_buffers(athad_ctx* c) {
             if (m2.conv_cos) { hipFree(m2.conv_cos); m2.conv_cos = nullptr; }
             for (auto& p : m2.conv_qs) if (p) { hipFree(p); p = nullptr; }
             for (auto& p : m2.conv_tmp) if (p) { hipFree(p); p = nullptr; }
+            if (m2.conv_cape) { hipFree(m2.conv_cape); m2.conv_cape = nullptr; }
         }
         c->err = msg;
         republish(c);
@@ -1252,6 +1257,12 @@ static athad_status ensure_mc_buffers(athad_ctx* c) {
             if (hipMalloc(&p, m.n * 8) != hipSuccess) { p = nullptr; return fail(ATHAD_ERR_ALLOC, oom); }
         for (auto& p : m.conv_tmp)
             if (hipMalloc(&p, m.n * 8) != hipSuccess) { p = nullptr; return fail(ATHAD_ERR_ALLOC, oom); }
+        if (hipMalloc(&m.conv_cape, (size_t)m.g.im * 12) != hipSuccess) {
+            m.conv_cape = nullptr;
+            return fail(ATHAD_ERR_ALLOC, oom);
+        }
+        if (hipMemset(m.conv_cape, 0, (size_t)m.g.im * 12) != hipSuccess)
+            return fail(ATHAD_ERR_DEVICE, "clearing the CAPE diagnostic failed");
         // theta by accumulation as the grid builds it; the reference takes the circle factor
         // about row 90 (MoistConvection.h:239), which is the equator on its 181-row grid. Here
         // it is taken about the equator row (jm - 1) / 2 of every grid: the same table at
@@ -1329,6 +1340,19 @@ athad_status athad_get_conv_field(athad_ctx* ctx, int member, int which, double*
         return ATHAD_OK;
     }
     HIPCHK(ctx, hipMemcpyAsync(dst, src, cnt * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return ATHAD_OK;
+}
+
+athad_status athad_get_conv_cape(athad_ctx* ctx, int member, double* dst) {
+    if (!ctx || !dst || member < 0 || member >= ctx->nmem) return ATHAD_ERR_ARG;
+    join_s2(ctx);
+    MemberHost& m = ctx->mem[member];
+    if (!m.conv_cape) {   // convection never ran: the reference's CAPE is empty, reported as zeros
+        std::fill(dst, dst + m.g.im, 0.0);
+        return ATHAD_OK;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(dst, m.conv_cape, (size_t)m.g.im * 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return ATHAD_OK;
 }

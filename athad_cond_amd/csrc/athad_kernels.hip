@@ -4082,6 +4082,27 @@ __global__ __launch_bounds__(256) void k_conv_sat(const MemberDesc* __restrict__
     d.conv_qs[1][q] = th_q_sat(th_p_auto(t_u), p, Mo);
 }
 
+// computeCAPE (MoistConvection.h:1259-1299) fills one array of im values, cleared, in the
+// column order k outer, j inner, so each level ends holding the last column that reached it.
+// Here every column writes its profile to scratch (conv_tmp[0]) and records its order index
+// k jm + j at each level it reaches with an atomic maximum; k_cape_select then takes each
+// level's value from the winning column.
+__global__ void k_cape_init(const MemberDesc* __restrict__ mds, int gate) {
+    const MemberDesc& d = mds[blockIdx.z];
+    if (gated_off(d, gate)) return;
+    for (int i = threadIdx.x; i < d.K.im; i += blockDim.x) d.conv_cape_col[i] = -1;
+}
+__global__ void k_cape_select(const MemberDesc* __restrict__ mds, int gate) {
+    const MemberDesc& d = mds[blockIdx.z];
+    if (gated_off(d, gate)) return;
+    const Consts& K = d.K;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K.im) return;
+    const int o = d.conv_cape_col[i];
+    const int j = o % K.jm, k = o / K.jm;
+    d.conv_cape[i] = (o < 0) ? 0.0 : d.conv_tmp[0][((size_t)i * K.jm + j) * K.km + k];
+}
+
 template <typename S>
 __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __restrict__ mds, int gate) {
     using namespace conv;
@@ -4633,6 +4654,31 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
         fs0 = fs1; fq0 = fq1; fv0 = fv1; fw0 = fw1;
         a = b; b = c2;
     }
+    // computeCAPE's profile of this column (the virtual-temperature excess of the lifted parcel
+    // over its environment, integrated from the cloud base to the level of free sinking);
+    // alf = (1 - eps) / eps with eps = 1 / (R_WaterVapour / R_Air) (:99-101, defaults 461.5 and
+    // 296.8, cAtmosphereDefaults.cpp.inc:93-94)
+    {
+        const double rwra = 461.5 / 296.8, eps_thermo = 1.0 / rwra, alf = (1.0 - eps_thermo) / eps_thermo;
+        G1<double>* __restrict__ prof = (G1<double>*)d.conv_tmp[0];
+        auto term = [&](int i) {
+            const size_t q = id(i);
+            const double t_u = (double)T[q] * t0, t_u_add = t_u + t_add_u;
+            const double t_vir = t_u_add * (1.0 + alf * Qvu[q] - Qcu[q]);
+            const double t_vir_env = tmax(t_u * (1.0 + alf * (double)C[q] - (double)CL[q]), 1.0);
+            const double exp_rm = 1.0 / (d.rad[i] + 1.0);
+            return K.g * step(i) / exp_rm * (t_vir - t_vir_env) / t_vir_env;
+        };
+        const int o = k * jm + j;
+        double v = term(base);
+        prof[id(base)] = v;
+        atomicMax(&d.conv_cape_col[base], o);
+        for (int i = base; i < lfs && i < im - 1; i++) {
+            v += term(i);
+            prof[id(i + 1)] = v;
+            atomicMax(&d.conv_cape_col[i + 1], o);
+        }
+    }
 }
 
 void launch_moist_convection(const Batch& b, int gate) {
@@ -4640,11 +4686,15 @@ void launch_moist_convection(const Batch& b, int gate) {
     const dim3 gs((unsigned)((b.max_n + 255) / 256), 1, b.nmem);
     if (b.fp32) {
         hipLaunchKernelGGL(k_conv_sat<float>, gs, dim3(256), 0, b.stream, b.d_md, gate);
+        hipLaunchKernelGGL(k_cape_init, dim3(1, 1, b.nmem), dim3(64), 0, b.stream, b.d_md, gate);
         hipLaunchKernelGGL(k_moist_convection<float>, g, dim3(64), 0, b.stream, b.d_md, gate);
     } else {
         hipLaunchKernelGGL(k_conv_sat<double>, gs, dim3(256), 0, b.stream, b.d_md, gate);
+        hipLaunchKernelGGL(k_cape_init, dim3(1, 1, b.nmem), dim3(64), 0, b.stream, b.d_md, gate);
         hipLaunchKernelGGL(k_moist_convection<double>, g, dim3(64), 0, b.stream, b.d_md, gate);
     }
+    hipLaunchKernelGGL(k_cape_select, dim3((unsigned)((b.max_im + 63) / 64), 1, b.nmem), dim3(64), 0, b.stream,
+                       b.d_md, gate);
 }
 
 }  // namespace athad

@@ -229,6 +229,12 @@ class Integration:
                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    def conv_cape(self, member: int = 0) -> np.ndarray:
+        """computeCAPE's im values after the last convection call (MoistConvection.h:1259-1299)."""
+        out = np.empty(self.grids[member][0])
+        self._chk(L.lib().athad_get_conv_cape(self.h, member, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
     def set_conv(self, name: str, arr, member: int = 0):
         a = np.ascontiguousarray(arr, dtype=np.float64)
         assert a.shape == self._conv_shape(name, member), "shape mismatch"

@@ -343,6 +343,10 @@ athad_status athad_set_moist_convection(athad_ctx* ctx, int on);
  * ATHAD_CONV_EVAPORATION. */
 athad_status athad_set_conv_field(athad_ctx* ctx, int member, int which, const double* src);
 athad_status athad_get_conv_field(athad_ctx* ctx, int member, int which, double* dst);
+/* computeCAPE's diagnostic after the last convection call (MoistConvection.h:1259-1299,
+ * cAtmosphereModel::CAPE): im doubles, each level holding the last column (k outer, j inner)
+ * whose cloud-base-to-free-sinking integral reached it, zero where none did. */
+athad_status athad_get_conv_cape(athad_ctx* ctx, int member, double* dst);
 
 /* --- entry point that allocates nothing ------------------------------------------------ */
 

@@ -209,6 +209,7 @@ struct orc_state {
     // column that its surface-flux perturbation reads; allocated, zero, at first use
     std::vector<std::vector<double>> mc;
     std::vector<double> evap;
+    std::vector<double> cape;          // computeCAPE's im values (MoistConvection.h:1259-1299)
     int iter_n = 0, total_iter = 0;
     // the reference's O(jm + km) distance-to-land scans of the SST closure over the land mask
     // h (all water here) instead of their closed form min(i, j, k); for timing the reference's
@@ -1998,8 +1999,8 @@ void ice_store(S& s) {
 // Every phase reads only its own column's convection arrays and the fields c, cloud, t, u, v,
 // w, p_stat, r_humid, co2 (c also at the four horizontal neighbours, which no phase writes),
 // so the phases of one column run back to back here; the result is the reference's
-// phase-by-phase sweep over all columns. computeCAPE (:1259-1299) is left out: it fills a
-// 1-D diagnostic that ends holding the last column's values and feeds nothing.
+// phase-by-phase sweep over all columns. computeCAPE (:1259-1299), a 1-D diagnostic that
+// feeds nothing, runs after all columns, in its own column order (k outer, j inner).
 // Q_sensible_2D is zero throughout the reference (nothing fills it).
 // ---------------------------------------------------------------------------------------
 namespace conv {
@@ -2063,6 +2064,8 @@ void moist_convection(S& s) {
     const double inv_a_u = 1.0 / (a_u * P.u_0), inv_a_d = 1.0 / (a_d * P.u_0);
     const double uu_cap = w_conv_max / P.u_0;
     std::vector<double> qs(im);
+    // each column's cloud base and level of free sinking after the passes, for computeCAPE
+    std::vector<int> base_of(pl, 0), lfs_of(pl, 0);
 
     for (int j = 0; j < jm; j++)
         for (int k = 0; k < km; k++) {
@@ -2399,6 +2402,8 @@ void moist_convection(S& s) {
                 }
                 Pc[id(0)] = Pc[id(1)];
             }
+            base_of[(size_t)j * km + k] = base;
+            lfs_of[(size_t)j * km + k] = lfs;
             // rhsForcing (:1304-1417)
             for (int i = 0; i < im; i++) { Mu[id(i)] = cap(Mu[id(i)], M_max); Md[id(i)] = cap(Md[id(i)], M_max); }
             for (int i = 0; i < im - 1; i++) {
@@ -2414,6 +2419,27 @@ void moist_convection(S& s) {
                 MCv[q] = cap(-(flux(qp, Vu, Vd, V) - flux(q, Vu, Vd, V)) * isr * P.u_0, MCv_max);
                 MCw[q] = cap(-(flux(qp, Wu, Wd, W) - flux(q, Wu, Wd, W)) * isr * P.u_0, MCv_max);
             }
+        }
+    // computeCAPE (:1259-1299): one array of im values, cleared, then every column in the order
+    // k outer, j inner writes its cloud-base value and the integral up to its level of free
+    // sinking, so each level ends holding the last column that reached it
+    // alf = (1 - eps) / eps with eps = 1 / (R_WaterVapour / R_Air) (:99-101; the defaults'
+    // R_Air 296.8 and R_WaterVapour 461.5, cAtmosphereDefaults.cpp.inc:93-94)
+    const double rwra = 461.5 / 296.8, eps_thermo = 1.0 / rwra, alf = (1.0 - eps_thermo) / eps_thermo;
+    s.cape.assign(im, 0.0);
+    for (int k = 0; k < km; k++)
+        for (int j = 0; j < jm; j++) {
+            auto id = [&](int i) { return (size_t)i * pl + (size_t)j * km + k; };
+            const int ib = base_of[(size_t)j * km + k], il = lfs_of[(size_t)j * km + k];
+            auto term = [&](int i) {
+                const double t_u = T[id(i)] * t0, t_u_add = t_u + t_add_u;
+                const double t_vir = t_u_add * (1.0 + alf * Qvu[id(i)] - Qcu[id(i)]);
+                const double t_vir_env = dmax(t_u * (1.0 + alf * C[id(i)] - CL[id(i)]), 1.0);
+                const double exp_rm = 1.0 / (s.rad[i] + 1.0);
+                return P.g * step[i] / exp_rm * (t_vir - t_vir_env) / t_vir_env;
+            };
+            s.cape[ib] = term(ib);
+            for (int i = ib; i < il && i < im - 1; i++) s.cape[i + 1] = s.cape[i] + term(i);
         }
 }
 
@@ -2489,6 +2515,11 @@ void orc_set_moist_convection(orc_state* s, int on) {
     if (on) orc_set_ice_scheme(s, 1);
 }
 void orc_moist_convection(orc_state* s) { moist_convection(*s); }
+int orc_cape(const orc_state* s, double* out) {
+    if (s->cape.size() != (size_t)s->im) return -1;
+    for (int i = 0; i < s->im; i++) out[i] = s->cape[i];
+    return 0;
+}
 double* orc_mc_ptr(orc_state* s, int which) {
     if (which < 0 || which >= ORC_NMC) return nullptr;
     ensure_mc(*s);

@@ -135,6 +135,7 @@ enum orc_mc {
     ORC_NMC
 };
 void    orc_moist_convection(orc_state* s);
+int     orc_cape(const orc_state* s, double* out);   // computeCAPE's im values after the last call; -1 before any
 // Puts moist convection in its place inside the moist block of every later step, after the ice
 // scheme (which on != 0 also switches on): MoistConvection::run, the damp_wiggles of P_conv, E_u,
 // M_u, q_v_u, q_c_u, E_d, M_d, q_v_d, MC_t, MC_v, MC_w, and P_conv in precipitationSum

@@ -95,6 +95,8 @@ def lib():
         L.orc_mp_ptr.restype = dp
         L.orc_mp_ptr.argtypes = [vp, i]
         L.orc_moist_convection.argtypes = [vp]
+        L.orc_cape.restype = ctypes.c_int
+        L.orc_cape.argtypes = [vp, dp]
         L.orc_mc_ptr.restype = dp
         L.orc_mc_ptr.argtypes = [vp, i]
         L.orc_evap_ptr.restype = dp
@@ -224,6 +226,13 @@ class OracleState:
     def moist_convection(self):
         """One call of MoistConvection::run on the state's fields (MoistConvection.h:106-140)."""
         lib().orc_moist_convection(self.h)
+
+    def cape(self) -> np.ndarray:
+        """computeCAPE's im values (MoistConvection.h:1259-1299) after the last convection call."""
+        out = np.zeros(self.shape[0])
+        if lib().orc_cape(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) != 0:
+            raise RuntimeError("no convection call yet")
+        return out
 
     def mc(self, name) -> np.ndarray:
         """A live (writable) view of one moist-convection array (MC_FIELDS)."""

@@ -30,7 +30,8 @@
 //                           defaults set them: iter_prec 3, convection_perturbation 1,
 //                           convection_mode 1); reads <in.bin>.evap (the Evaporation diagnostic,
 //                           jm*km doubles) and, if present, <in.bin>.mc, and writes <out.bin>.mc:
-//                           the 32 convection arrays in mc_fields order
+//                           the 32 convection arrays in mc_fields order (and <out.bin>.cape,
+//                           computeCAPE's im values)
 // The radiation modes also write <out.bin>.rad: radiation and epsilon (im*jm*km doubles each),
 // albedo (jm*km), the top short wave (jm) and t_skin.
 // State file: int32 im,jm,km,iter_n,total_iter; then 35 fields of im*jm*km doubles
@@ -374,6 +375,8 @@ public:
                      m.iter_prec, m.convection_perturbation, m.convection_mode, m.p_0, m.s_0);
         for (int q = 0; q < n; q++) MoistConvection(m).run(m.iter_n);
         io_fields(m, mc_fields(m), out + ".mc", false);
+        std::ofstream fc(out + ".cape", std::ios::binary);   // computeCAPE's im values
+        fc.write((const char*)m.CAPE.data(), m.CAPE.size() * 8);
         return true;
     }
 

@@ -213,6 +213,7 @@ def conv_vectors():
             out[f"call{call}/{name}/sample"] = a[idx]
             out[f"call{call}/{name}/sum"] = np.array(np.sum(a))
             out[f"call{call}/{name}/l2"] = np.array(np.sqrt(np.sum(a * a)))
+        out[f"call{call}/cape"] = np.fromfile(o + ".cape")   # computeCAPE's im values
         print(f"convection call {call} done", flush=True)
     path = os.path.join(HERE, "ref_conv.npz")
     np.savez_compressed(path, **out)

@@ -55,6 +55,13 @@ def test_two_calls_match_oracle(oracle, grid):
             print(call, {k: f"{v:.1e}" for k, v in e.items() if v > 0})
             bad = {k: v for k, v in e.items() if v > 1e-11}
             assert not bad, (call, bad)
+            # computeCAPE: each level from the same (last) column as the oracle's, zero where no
+            # column reached it
+            cape, cref = integ.conv_cape(), ref.cape()
+            print(call, "cape", f"{rel(cape, cref):.1e}", int(np.count_nonzero(cref)), "levels set")
+            assert np.count_nonzero(cref) > 0
+            assert np.array_equal(cape == 0.0, cref == 0.0)
+            assert rel(cape, cref) <= 1e-11
         # the input makes the scheme do something
         for n in ("M_u", "M_d", "q_c_u", "c_u", "e_d", "MC_t", "MC_q", "MC_v", "MC_w"):
             assert np.any(ref.mc(n) != 0.0), n

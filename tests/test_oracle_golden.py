@@ -252,6 +252,12 @@ def test_moist_convection_twice(oracle):
         assert not bad, f"call {call}: {bad}"
         for name in ("M_u", "M_d", "q_c_u", "c_u", "g_p", "e_d", "e_p", "P_conv", "MC_t", "MC_q", "MC_v", "MC_w"):
             assert float(gold[f"call{call}/{name}/l2"]) > 0.0, name
+        # computeCAPE's diagnostic: each level holds the last column (k outer, j inner) that
+        # reached it; zero where none did
+        cape, ref = st.cape(), gold[f"call{call}/cape"]
+        assert np.count_nonzero(ref) > 10
+        assert np.array_equal(cape == 0.0, ref == 0.0)
+        assert rel(cape, ref) <= 1e-11, rel(cape, ref)
 
 
 GOLD_CONV_STEPS = os.path.join(HERE, "golden", "ref_conv_steps.npz")
