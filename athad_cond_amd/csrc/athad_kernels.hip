@@ -1881,8 +1881,19 @@ __global__ __launch_bounds__(256) void k_poisson_colour(const MemberDesc* __rest
     if (!finite_bits(div_src)) div_src = C(0);
     else if (div_src > src_max) div_src = src_max;
     else if (div_src < -src_max) div_src = -src_max;
-    p[id] = ((C(p[id + si]) + C(p[id - si])) * num1 + (C(p[id + sj]) + C(p[id - sj])) * num2
-           + (C(p[id + 1]) + C(p[id - 1])) * num3 - div_src) * inv_denom;
+    if constexpr (sizeof(S) == 8) {
+        p[id] = ((C(p[id + si]) + C(p[id - si])) * num1 + (C(p[id + sj]) + C(p[id - sj])) * num2
+               + (C(p[id + 1]) + C(p[id - 1])) * num3 - div_src) * inv_denom;
+    } else {
+        // The mixed form writes the sweep as a defect correction: the defect of the cell's
+        // equation denom p - sum num p_nb = -src is evaluated in FP64 from the FP32 values, and
+        // the FP32 correction defect / denom is added to p. Algebraically the same update.
+        const double pc = (double)p[id];
+        const double defect = -(double)div_src - (denom * pc - ((double)p[id + si] + (double)p[id - si]) * (exp_2_rm * K.inv_dr2)
+                                                  - ((double)p[id + sj] + (double)p[id - sj]) * (inv_rm2 * K.inv_dthe2)
+                                                  - ((double)p[id + 1] + (double)p[id - 1]) * (m_phi * K.inv_dphi2));
+        p[id] = S(pc + (double)(C(defect) * inv_denom));
+    }
 }
 
 // Boundary values of p_dyn, composed: radial copy, then pole copy, then the seam average
@@ -1929,8 +1940,9 @@ __global__ void k_p_ceiling(const MemberDesc* __restrict__ md, int gate) {
 // Newton-loop helpers of the saturation adjustment, templated on the Newton arithmetic type N
 // (double in the drop-in form, float in the mixed form). They restate th_* with shared
 // sub-expressions evaluated once and divisions by positive bounded operands done through
-// refined reciprocals. In the mixed form the whole loop, saturation vapour pressures and q_sat
-// included, runs in FP32; the entry test and the clamp around it stay FP64.
+// refined reciprocals. In the mixed form the loop's state, latent heat and heat capacity are
+// FP32 while the saturation vapour pressures and q_sat stay FP64 (sat_newton); the entry test
+// and the clamp around it are FP64 too.
 template <typename N> ATHAD_DEV void sat_species_cp(N T, N Mb, N& cpH, N& cpC, N& cpB) {
     const N t = tmin(tmax(T, N(298)), N(2000)) * N(1.0e-3);
     const N t2 = t * t, t3 = t2 * t, it2 = rcp_nr(t2);
