@@ -88,7 +88,7 @@ struct MemberHost {
     double *evap = nullptr, *conv_cos = nullptr;
     double* conv_qs[2] = {};
     double* conv_tmp[2] = {};
-    double* conv_cape = nullptr;   // im doubles, then im ints (the winning column per level)
+    double* conv_cape = nullptr;   // im doubles, then 2 jm km ints (each column's CAPE level range)
     int iter_n = 0, total = 0;     // host mirror of the device counters
     // table offsets inside 'tables'
     size_t o_rad = 0, o_sint = 0, o_cost = 0, o_sinr = 0, o_hgt = 0, o_nuef = 0, o_step = 0, o_geo = 0;
@@ -342,7 +342,7 @@ void make_desc(const athad_ctx* c, const MemberHost& m, int parity, MemberDesc& 
     d.conv_qs[0] = m.conv_qs[0]; d.conv_qs[1] = m.conv_qs[1];
     d.conv_tmp[0] = m.conv_tmp[0]; d.conv_tmp[1] = m.conv_tmp[1];
     d.conv_cape = m.conv_cape;
-    d.conv_cape_col = m.conv_cape ? (int*)(m.conv_cape + m.g.im) : nullptr;
+    d.conv_cape_rng = m.conv_cape ? (int*)(m.conv_cape + m.g.im) : nullptr;
     (void)c;
 }
 
@@ -1257,11 +1257,12 @@ static athad_status ensure_mc_buffers(athad_ctx* c) {
             if (hipMalloc(&p, m.n * 8) != hipSuccess) { p = nullptr; return fail(ATHAD_ERR_ALLOC, oom); }
         for (auto& p : m.conv_tmp)
             if (hipMalloc(&p, m.n * 8) != hipSuccess) { p = nullptr; return fail(ATHAD_ERR_ALLOC, oom); }
-        if (hipMalloc(&m.conv_cape, (size_t)m.g.im * 12) != hipSuccess) {
+        const size_t cape_bytes = (size_t)m.g.im * 8 + (size_t)jm * km * 8;
+        if (hipMalloc(&m.conv_cape, cape_bytes) != hipSuccess) {
             m.conv_cape = nullptr;
             return fail(ATHAD_ERR_ALLOC, oom);
         }
-        if (hipMemset(m.conv_cape, 0, (size_t)m.g.im * 12) != hipSuccess)
+        if (hipMemset(m.conv_cape, 0, cape_bytes) != hipSuccess)
             return fail(ATHAD_ERR_DEVICE, "clearing the CAPE diagnostic failed");
         // theta by accumulation as the grid builds it; the reference takes the circle factor
         // about row 90 (MoistConvection.h:239), which is the equator on its 181-row grid. Here

@@ -101,7 +101,7 @@ struct MemberDesc {
     const double* conv_cos;  // jm       cos(theta[90] - theta[j]), the convection's zonal circle factor
     double* conv_qs[2];      // n        per-cell saturation of the lifted (+0.2 K) and the resting air
     double* conv_cape;       // im       computeCAPE's diagnostic (MoistConvection.h:1259-1299)
-    int* conv_cape_col;      // im       per level, the last column (k jm + j) whose profile reached it
+    int* conv_cape_rng;      // 2 jm km  per column, the first and last level its CAPE profile reached
     double* conv_tmp[2];     // n        FP64 damping scratch of the convection arrays (with conv_qs,
                              //          which the next call recomputes): allocated with them
 };

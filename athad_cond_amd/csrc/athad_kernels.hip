@@ -4096,23 +4096,31 @@ __global__ __launch_bounds__(256) void k_conv_sat(const MemberDesc* __restrict__
 
 // computeCAPE (MoistConvection.h:1259-1299) fills one array of im values, cleared, in the
 // column order k outer, j inner, so each level ends holding the last column that reached it.
-// Here every column writes its profile to scratch (conv_tmp[0]) and records its order index
-// k jm + j at each level it reaches with an atomic maximum; k_cape_select then takes each
-// level's value from the winning column.
-__global__ void k_cape_init(const MemberDesc* __restrict__ mds, int gate) {
-    const MemberDesc& d = mds[blockIdx.z];
-    if (gated_off(d, gate)) return;
-    for (int i = threadIdx.x; i < d.K.im; i += blockDim.x) d.conv_cape_col[i] = -1;
-}
-__global__ void k_cape_select(const MemberDesc* __restrict__ mds, int gate) {
+// Here every column writes its profile to scratch (conv_tmp[0]) and the range of levels it
+// reaches (conv_cape_rng); k_cape_select then finds, per level (one block each), the column of
+// largest order index k jm + j that reaches it, and takes the level's value from its profile.
+__global__ __launch_bounds__(256) void k_cape_select(const MemberDesc* __restrict__ mds, int gate) {
     const MemberDesc& d = mds[blockIdx.z];
     if (gated_off(d, gate)) return;
     const Consts& K = d.K;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x;
     if (i >= K.im) return;
-    const int o = d.conv_cape_col[i];
-    const int j = o % K.jm, k = o / K.jm;
-    d.conv_cape[i] = (o < 0) ? 0.0 : d.conv_tmp[0][((size_t)i * K.jm + j) * K.km + k];
+    __shared__ int best[256];
+    int b = -1;
+    for (int c = threadIdx.x; c < K.jm * K.km; c += blockDim.x) {
+        const int lo = d.conv_cape_rng[2 * c], hi = d.conv_cape_rng[2 * c + 1];
+        if (i >= lo && i <= hi) b = max(b, (c % K.km) * K.jm + c / K.km);
+    }
+    best[threadIdx.x] = b;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) best[threadIdx.x] = max(best[threadIdx.x], best[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int o = best[0];
+        d.conv_cape[i] = (o < 0) ? 0.0 : d.conv_tmp[0][((size_t)i * K.jm + o % K.jm) * K.km + o / K.jm];
+    }
 }
 
 template <typename S>
@@ -4681,15 +4689,15 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             const double exp_rm = 1.0 / (d.rad[i] + 1.0);
             return K.g * step(i) / exp_rm * (t_vir - t_vir_env) / t_vir_env;
         };
-        const int o = k * jm + j;
         double v = term(base);
         prof[id(base)] = v;
-        atomicMax(&d.conv_cape_col[base], o);
         for (int i = base; i < lfs && i < im - 1; i++) {
             v += term(i);
             prof[id(i + 1)] = v;
-            atomicMax(&d.conv_cape_col[i + 1], o);
         }
+        // the levels reached: base, and base + 1 .. min(lfs, im - 1) when lfs > base
+        d.conv_cape_rng[2 * col] = base;
+        d.conv_cape_rng[2 * col + 1] = (lfs > base) ? tmin(lfs, im - 1) : base;
     }
 }
 
@@ -4698,15 +4706,12 @@ void launch_moist_convection(const Batch& b, int gate) {
     const dim3 gs((unsigned)((b.max_n + 255) / 256), 1, b.nmem);
     if (b.fp32) {
         hipLaunchKernelGGL(k_conv_sat<float>, gs, dim3(256), 0, b.stream, b.d_md, gate);
-        hipLaunchKernelGGL(k_cape_init, dim3(1, 1, b.nmem), dim3(64), 0, b.stream, b.d_md, gate);
         hipLaunchKernelGGL(k_moist_convection<float>, g, dim3(64), 0, b.stream, b.d_md, gate);
     } else {
         hipLaunchKernelGGL(k_conv_sat<double>, gs, dim3(256), 0, b.stream, b.d_md, gate);
-        hipLaunchKernelGGL(k_cape_init, dim3(1, 1, b.nmem), dim3(64), 0, b.stream, b.d_md, gate);
         hipLaunchKernelGGL(k_moist_convection<double>, g, dim3(64), 0, b.stream, b.d_md, gate);
     }
-    hipLaunchKernelGGL(k_cape_select, dim3((unsigned)((b.max_im + 63) / 64), 1, b.nmem), dim3(64), 0, b.stream,
-                       b.d_md, gate);
+    hipLaunchKernelGGL(k_cape_select, dim3((unsigned)b.max_im, 1, b.nmem), dim3(256), 0, b.stream, b.d_md, gate);
 }
 
 }  // namespace athad
