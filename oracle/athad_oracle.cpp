@@ -216,6 +216,9 @@ struct orc_state {
     // own cost only, the results are the same
     bool sst_scan = false;
     std::vector<double> h;
+    // the turbulence closure, TurbulenceAtm.h:24 order: 0 k-epsilon (Chien 1982), 1 k-omega
+    // (Wilcox 1988), 2 k-omega SST (Menter 1994, the configuration's choice)
+    int turb_model = 2;
 
     inline size_t ix(int i, int j, int k) const { return ((size_t)i * jm + j) * km + k; }
     inline double* F(int id) { return f[id].data(); }
@@ -704,19 +707,26 @@ void project_initial_velocity(S& s, int n_sweeps) {
 // (j,k,i) order = the OMP_NUM_THREADS=1 sweep), compute_k_omega_SST (:656-773),
 // zero_land_cells (no-op), clamp_nue (:271-297)
 // ---------------------------------------------------------------------------------------
-void turbulence(S& s) {
-    const Params& P = s.P;
+void vel_star(S& s) {
     const int im = s.im, jm = s.jm, km = s.km;
-    double *u = s.F(ORC_U), *v = s.F(ORC_V), *w = s.F(ORC_W), *tke = s.F(ORC_TKE), *dis = s.F(ORC_DIS),
-           *nue = s.F(ORC_NUE), *prod = s.F(ORC_PROD), *tks = s.F(ORC_TKE_SRC), *dss = s.F(ORC_DIS_SRC);
+    const double *v = s.F(ORC_V), *w = s.F(ORC_W);
     double* vst = s.f2[ORC2_VEL_STAR].data();
     for (int j = 0; j < jm; j++)
         for (int k = 0; k < km; k++) {
             const int i_surf = std::min(0 + 1, im - 1);
             const double vs = v[s.ix(i_surf, j, k)], ws = w[s.ix(i_surf, j, k)];
-            const double U = P.u_0 * std::sqrt(vs * vs + ws * ws);
+            const double U = s.P.u_0 * std::sqrt(vs * vs + ws * ws);
             vst[(size_t)j * km + k] = dmax(0.05, std::sqrt(0.002) * U);
         }
+}
+
+void turbulence(S& s) {
+    const Params& P = s.P;
+    const int im = s.im, jm = s.jm, km = s.km;
+    double *u = s.F(ORC_U), *v = s.F(ORC_V), *w = s.F(ORC_W), *tke = s.F(ORC_TKE), *dis = s.F(ORC_DIS),
+           *nue = s.F(ORC_NUE), *prod = s.F(ORC_PROD), *tks = s.F(ORC_TKE_SRC), *dss = s.F(ORC_DIS_SRC);
+    const double* vst = s.f2[ORC2_VEL_STAR].data();
+    vel_star(s);
     const double nue_max = 1000.0 / (P.u_0 * P.L_atm);
     const double y_mount = (double)s.hgt[0];
     const double inv_2dr = 1.0 / (2.0 * s.dr), inv_2dthe = 1.0 / (2.0 * s.dthe), inv_2dphi = 1.0 / (2.0 * s.dphi);
@@ -757,7 +767,7 @@ void turbulence(S& s) {
                 {
                     const double dis_here = dmax(ds, kDisMin);
                     const double tke_here = dmax(tk, 0.0);
-                    cnue = tke_here / dis_here;
+                    cnue = (s.turb_model == 0) ? kCnue * tke_here * tke_here / dis_here : tke_here / dis_here;
                     cnue = dmin(cnue, nue_max);
                 }
                 const double der = 0.66667 * (dudr + dvdthe + dwdphi);
@@ -773,8 +783,66 @@ void turbulence(S& s) {
                     + (cnue * (dwdthe + dvdphi)) * dwdthe);
                 const double W12 = dudthe - dvdr, W13 = dudphi - dwdr, W23 = dvdphi - dwdthe;
                 const double Omega = std::sqrt(W12 * W12 + W13 * W13 + W23 * W23);
-                // compute_k_omega_SST, TurbulenceAtm.h:656-773
-                {
+                if (s.turb_model == 0) {
+                    // compute_k_epsilon, TurbulenceAtm.h:481-530 (no land: y_mount = hgt[0])
+                    const double C_eps_1 = 1.35, C_eps_2 = 1.80;
+                    const double y_phys = dmax((double)s.hgt[i] - y_mount, P.L_atm);
+                    const double y_star = y_phys / P.L_atm;
+                    nue[id] = kCnue * tk * tk / dmax(ds, kDisMin);
+                    const double d_plus = y_phys * vs / kNueAir;
+                    const double f_nue = 1.0 - std::exp(-0.0115 * d_plus);
+                    const double Re_T = tk * tk * P.u_0 * P.L_atm / dmax(ds * kNueAir, kDisMin * kNueAir);
+                    const double f_2 = 1.0 - 0.4 / 1.8 * std::exp(-Re_T * Re_T / 36.0);
+                    nue[id] *= f_nue;
+                    const double y_star2 = y_star * y_star;
+                    const double L_k = -2.0 * tk / y_star2;
+                    const double L_w = -2.0 * ds / y_star2 * std::exp(-d_plus / 2.0);
+                    const double P_k = prod[id];
+                    const double Y_k = ds;
+                    const double tke_safe = dmax(tk, kDisMin);
+                    const double P_w = C_eps_1 * ds / tke_safe * P_k;
+                    const double Y_w = C_eps_2 * f_2 * ds * ds / tke_safe;
+                    tks[id] = P_k - Y_k + L_k / s.re_turb;
+                    dss[id] = P_w - Y_w + L_w / s.re_turb;
+                } else if (s.turb_model == 1) {
+                    // compute_k_omega, TurbulenceAtm.h:533-651 (no land: the Neumann variants
+                    // are the plain central differences; grid rm in the theta/phi gradients)
+                    const double bet_star = 0.09, gam = 0.52, C_lim = 0.875, bet_0 = 0.0708;
+                    const double dtkedthe_n = (tke[id + sj] - tke[id - sj]) / (rm * 2.0 * s.dthe);
+                    const double ddisdthe_n = (dis[id + sj] - dis[id - sj]) / (rm * 2.0 * s.dthe);
+                    const double dtkedphi_n = (tke[id + 1] - tke[id - 1]) / (rmsinthe * 2.0 * s.dphi);
+                    const double ddisdphi_n = (dis[id + 1] - dis[id - 1]) / (rmsinthe * 2.0 * s.dphi);
+                    const double dudr_k = (u[id + si] - u[id - si]) / (2.0 * s.dr) / (rm + 1.0);
+                    const double dvdthe_k = (v[id + sj] - v[id - sj]) / (rm * 2.0 * s.dthe);
+                    const double dwdphi_k = (w[id + 1] - w[id - 1]) / (rmsinthe * 2.0 * s.dphi);
+                    const double dudthe_u = (u[id + sj] - u[id - sj]) / (rm * 2.0 * s.dthe);
+                    const double dudphi_u = (u[id + 1] - u[id - 1]) / (rmsinthe * 2.0 * s.dphi);
+                    const double dvdr_u = (v[id + si] - v[id - si]) / (2.0 * s.dr) / (rm + 1.0);
+                    const double dwdr_u = (w[id + si] - w[id - si]) / (2.0 * s.dr) / (rm + 1.0);
+                    const double dvdphi_u = (v[id + 1] - v[id - 1]) / (rmsinthe * 2.0 * s.dphi);
+                    const double dwdthe_u = (w[id + sj] - w[id - sj]) / (rm * 2.0 * s.dthe);
+                    const double S11 = dudr_k, S22 = dvdthe_k, S33 = dwdphi_k;
+                    const double S12 = 0.5 * (dudthe_u + dvdr_u);
+                    const double S13 = 0.5 * (dudphi_u + dwdr_u);
+                    const double S23 = 0.5 * (dvdphi_u + dwdthe_u);
+                    const double S_mag = std::sqrt(2.0 * (S11 * S11 + S22 * S22 + S33 * S33
+                                                          + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23)));
+                    const double w_hat = dmax(ds, C_lim * S_mag / std::sqrt(bet_star));
+                    const double cross = dtkedr * ddisdr + dtkedthe_n * ddisdthe_n + dtkedphi_n * ddisdphi_n;
+                    const double sig_d = (cross <= 0.0) ? 0.0 : 0.125;
+                    const double chi_w = std::fabs(Omega * Omega * S_mag / std::pow(bet_star * dmax(ds, 1.0e-20), 3));
+                    const double f_bet = (1.0 + 85.0 * chi_w) / (1.0 + 100.0 * chi_w);
+                    const double bet_wc = bet_0 * f_bet;
+                    nue[id] = tk / dmax(w_hat, 1.0e-20);
+                    const double P_k = dmin(prod[id], 20.0 * bet_star * tk * ds);
+                    const double Y_k = bet_star * tk * ds;
+                    const double P_w = gam * ds / dmax(tk, 1.0e-20) * P_k;
+                    const double Y_w = bet_wc * ds * ds;
+                    const double D_w = sig_d / dmax(ds, 1.0e-20) * cross;
+                    tks[id] = P_k - Y_k;
+                    dss[id] = P_w - Y_w + D_w;
+                } else {
+                    // compute_k_omega_SST, TurbulenceAtm.h:656-773
                     const double y_phys = dmax((double)s.hgt[i] - y_mount, 1.0e-6);
                     const double y_star = y_phys / P.L_atm;
                     const double dtkedthe_n = (tke[id + sj] - tke[id - sj]) / (rm * 2.0 * s.dthe);
@@ -833,12 +901,79 @@ void turbulence(S& s) {
 // bcScalarSurfSur (:1026-1130, identity without land), bcSolidGround (:23-551: passes 1-3
 // are identities without land; passes 4-5 are the clamps)
 // ---------------------------------------------------------------------------------------
+// TurbulenceAtm::apply_wall_bc (:109-147): the log-layer tke, zero eddy viscosity and the
+// closure's wall value of dis at i = 0 (k-epsilon: 0; k-omega: 6 nu / (0.075 y1^2); SST:
+// 60 nu / (0.0333 y1^2), each made dimensionless by L_atm / u_0)
+void wall_bc(S& s) {
+    const Params& P = s.P;
+    const double* vst = s.f2[ORC2_VEL_STAR].data();
+    const double y1 = (double)s.hgt[1];
+    const double y1s = dmax(y1, 1.0e-6);
+    const double nd_omega = P.L_atm / P.u_0;
+    for (int j = 0; j < s.jm; j++)
+        for (int k = 0; k < s.km; k++) {
+            const double vs = vst[(size_t)j * s.km + k];
+            s.f[ORC_TKE][s.ix(0, j, k)] = vs * vs / (std::sqrt(kCnue) * P.u_0 * P.u_0);
+            s.f[ORC_NUE][s.ix(0, j, k)] = 0.0;
+            double d0 = 0.0;
+            if (s.turb_model == 1) d0 = 6.0 * kNueAir / (0.075 * y1s * y1s) * nd_omega;
+            else if (s.turb_model == 2) d0 = 60.0 * kNueAir / (0.0333 * y1s * y1s) * nd_omega;
+            s.f[ORC_DIS][s.ix(0, j, k)] = d0;
+        }
+}
+
+// TurbulenceAtm::init (:39-59): compute_vel_star, init_fields (:193-255, the boundary-layer
+// profile k = u_tau^2 / sqrt(C_mu) (1 - z/h_abl)^2 above a 1 % background floor, with
+// omega = C_mu^-1/4 sqrt(k) / (kappa z) or epsilon = C_mu^3/4 k^3/2 / (kappa z) above theirs),
+// apply_wall_bc, then compute_sources and clamp_nue as in run (zero_land_cells is an identity
+// without land). m.re_turb is set as the state's own.
+void turbulence_init(S& s) {
+    const Params& P = s.P;
+    const int im = s.im, jm = s.jm, km = s.km;
+    vel_star(s);
+    const double* vst = s.f2[ORC2_VEL_STAR].data();
+    double *tke = s.F(ORC_TKE), *dis = s.F(ORC_DIS), *tkn = s.F(ORC_TKEN), *dsn = s.F(ORC_DISN);
+    const double kKarman = 0.42;
+    const double inv_u02 = 1.0 / (P.u_0 * P.u_0);
+    const double nd_omega = P.L_atm / P.u_0;
+    const double nd_eps = P.L_atm / (P.u_0 * P.u_0 * P.u_0);
+    for (int j = 1; j < jm - 1; j++)
+        for (int k = 1; k < km - 1; k++) {
+            const double vs = vst[(size_t)j * km + k];
+            const double k_bg = 0.01 * vs * vs / std::sqrt(kCnue);
+            const double om_bg = std::pow(kCnue, -0.25) * std::sqrt(k_bg) / (kKarman * P.L_atm);
+            const double eps_bg = std::pow(kCnue, 0.75) * std::pow(k_bg, 1.5) / (kKarman * P.L_atm);
+            const double y_mount = (double)s.hgt[0];
+            for (int i = 1; i < im - 1; i++) {
+                const double z_i = (double)s.hgt[i] - y_mount;
+                const double z_safe = dmax(z_i, 1.0e-6);
+                const double z_frac = dmin(z_i, P.abl_height) / P.abl_height;
+                const double k_abl = vs * vs / std::sqrt(kCnue) * std::pow(1.0 - z_frac, 2);
+                const double k_phys = dmax(k_abl, k_bg);
+                double dis_nd;
+                if (s.turb_model == 0) {
+                    const double eps_phys = std::pow(kCnue, 0.75) * std::pow(k_phys, 1.5) / (kKarman * z_safe);
+                    dis_nd = dmax(eps_phys, eps_bg) * nd_eps;
+                } else {
+                    const double om_phys = std::pow(kCnue, -0.25) * std::sqrt(k_phys) / (kKarman * z_safe);
+                    dis_nd = dmax(om_phys, om_bg) * nd_omega;
+                }
+                const size_t id = s.ix(i, j, k);
+                tke[id] = dmax(0.0, k_phys * inv_u02);
+                dis[id] = dmax(kDisMin, dis_nd);
+                tkn[id] = tke[id];
+                dsn[id] = dis[id];
+            }
+        }
+    wall_bc(s);
+    turbulence(s);
+}
+
 void bc_all(S& s) {
     const Params& P = s.P;
     const int im = s.im, jm = s.jm, km = s.km, iml = im - 1;
     double* t = s.F(ORC_T);
     const double* tti = s.f2[ORC2_T_TOP_INIT].data();
-    const double* vst = s.f2[ORC2_VEL_STAR].data();
     const int hydro[3] = {ORC_P_STAT, ORC_R_HUMID, ORC_R_DRY};
     const int cubic_vn[10] = {ORC_CLOUD, ORC_ICE, ORC_GR, ORC_CO2, ORC_TKE, ORC_DIS, ORC_NUE,
                               ORC_PROD, ORC_TKE_SRC, ORC_DIS_SRC};
@@ -862,20 +997,7 @@ void bc_all(S& s) {
                 X(fld, iml) = X(fld, iml - 1);
             }
         }
-    // apply_wall_bc (k-omega SST branch)
-    {
-        const double y1 = (double)s.hgt[1];
-        const double y1s = dmax(y1, 1.0e-6);
-        const double nd_omega = P.L_atm / P.u_0;
-        const double om_phys = 60.0 * kNueAir / (0.0333 * y1s * y1s);
-        for (int j = 0; j < jm; j++)
-            for (int k = 0; k < km; k++) {
-                const double vs = vst[(size_t)j * km + k];
-                s.f[ORC_TKE][s.ix(0, j, k)] = vs * vs / (std::sqrt(kCnue) * P.u_0 * P.u_0);
-                s.f[ORC_NUE][s.ix(0, j, k)] = 0.0;
-                s.f[ORC_DIS][s.ix(0, j, k)] = om_phys * nd_omega;
-            }
-    }
+    wall_bc(s);
     // bcTheta
     const int theta_f[18] = {ORC_T, ORC_P_STAT, ORC_R_HUMID, ORC_R_DRY, ORC_U, ORC_V, ORC_W, ORC_C,
                              ORC_CLOUD, ORC_ICE, ORC_GR, ORC_CO2, ORC_TKE, ORC_DIS, ORC_NUE, ORC_PROD,
@@ -1047,6 +1169,7 @@ struct Geo {
 struct RhsConsts {
     double coeff_energy_p, force_nd, re_turb, pr_turb, diff_prec_re_inv, diff_co2_re_inv, nue_max, nue_air_nd,
            buoy_coeff, hs_scale, u_0, L_atm;
+    int turb;   // the closure, orc_state::turb_model
 };
 
 enum { FT = 0, FU, FV, FW, FC, FCL, FI, FG, FCO, FK, FD, NF };
@@ -1094,7 +1217,7 @@ void rhs_cell(const S& s, const Geo& g, const RhsConsts& K, int i, int j, int k,
                      dwdphi_s = d1p[FW] * g.inv_rmsinthe;
         const double dis_here = dmax(dis, 1.0e-10);
         const double tke_here = dmax(tke, 0.0);
-        double cnue = tke_here / dis_here;
+        double cnue = (K.turb == 0) ? kCnue * tke_here * tke_here / dis_here : tke_here / dis_here;
         cnue = dmin(cnue, K.nue_max);
         const double der = 0.66667 * (dudr_s + dvdthe_s + dwdphi_s);
         prod = dmax(0.0,
@@ -1108,9 +1231,44 @@ void rhs_cell(const S& s, const Geo& g, const RhsConsts& K, int i, int j, int k,
             + (cnue * (dwdr_s + dudphi_s)) * dwdr_s
             + (cnue * (dwdthe_s + dvdphi_s)) * dwdthe_s);
     }
-    // SST :633-721 with d_close = min(i, j, k) (the land scans evaluate to this without land)
     double diffusion_t_re, diffusion_vel_re, diffusion_tke_re, diffusion_dis_re;
-    {
+    if (K.turb == 0) {
+        // k-epsilon :563-599: the production-destruction balance, no wall terms
+        const double inv_re = 1.0 / K.re_turb;
+        diffusion_t_re = (inv_re + nue_c) / K.pr_turb;
+        diffusion_vel_re = inv_re + nue_c;
+        diffusion_tke_re = inv_re + nue_c / 1.0;
+        diffusion_dis_re = inv_re + nue_c / 1.3;
+        const double tke_s = dmax(tke, 1.0e-10), dis_s = dmax(dis, 1.0e-10);
+        const double P_k = prod;
+        const double Y_k = dis_s;
+        const double P_w = 1.35 * (dis_s / tke_s) * P_k;
+        const double Y_w = 1.80 * dis_s * dis_s / tke_s;
+        const double dsm = 20.0 * dis_s * dis_s, tsm = 20.0 * tke_s * dis_s;
+        tsrc = dclamp(P_k - Y_k, -tsm, tsm);
+        dsrc = dclamp(P_w - Y_w, -dsm, dsm);
+    } else if (K.turb == 1) {
+        // k-omega :601-631, with the cross-diffusion limiter sigma_d
+        const double inv_re = 1.0 / K.re_turb;
+        diffusion_t_re = (inv_re + nue_c) / K.pr_turb;
+        diffusion_vel_re = inv_re + nue_c;
+        diffusion_tke_re = inv_re + 0.6 * nue_c;
+        diffusion_dis_re = inv_re + 0.5 * nue_c;
+        const double bet_star = 0.09, gam = 0.52, bet_0 = 0.0708;
+        const double tke_s = dmax(tke, 1.0e-10), dis_s = dmax(dis, 1.0e-10);
+        const double P_k = dmin(prod, 20.0 * bet_star * tke_s * dis_s);
+        const double Y_k = bet_star * tke_s * dis_s;
+        const double P_w = gam * (dis_s / tke_s) * P_k;
+        const double Y_w = bet_0 * dis_s * dis_s;
+        const double cross = d1r[FK] * d1r[FD] + d1t[FK] * g.inv_rm * d1t[FD] * g.inv_rm
+                           + d1p[FK] * g.inv_rmsinthe * d1p[FD] * g.inv_rmsinthe;
+        const double sig_d = (cross <= 0.0) ? 0.0 : 0.125;
+        const double D_w = sig_d * cross / dis_s;
+        const double dsm = 20.0 * dis_s * dis_s, tsm = 20.0 * tke_s * dis_s;
+        tsrc = dclamp(P_k - Y_k, -tsm, tsm);
+        dsrc = dclamp(P_w - Y_w + D_w, -dsm, dsm);
+    } else {
+        // SST :633-721 with d_close = min(i, j, k) (the land scans evaluate to this without land)
         const double sig_k1 = 1.176, sig_k2 = 1.0, sig_w1 = 2.0, sig_w2 = 1.168, bet_star = 0.09;
         double d_close = (double)i;
         double dj_s = (double)j, dj_n = (double)j, dk_e = (double)k, dk_w = (double)k;
@@ -1276,6 +1434,7 @@ void rk4(S& s, int order) {
     K.nue_max = 1000.0 / (P.u_0 * P.L_atm);
     K.nue_air_nd = 1.5e-5 / (P.u_0 * P.L_atm);
     K.u_0 = P.u_0; K.L_atm = P.L_atm;
+    K.turb = s.turb_model;
     const double buoy_ramp_dt = buoy_ramp * 1.0 * P.g * dt / P.u_0;
     std::vector<double> sint(jm), cost(jm);
     for (int j = 0; j < jm; j++) {
@@ -2534,6 +2693,12 @@ double* orc_mp_ptr(orc_state* s, int which) {
     ensure_mp(*s);
     return s->mp[which].data();
 }
+int orc_set_turb_model(orc_state* s, int model) {
+    if (model < 0 || model > 2) return -1;
+    s->turb_model = model;
+    return 0;
+}
+void orc_turbulence_init(orc_state* s) { turbulence_init(*s); }
 void orc_set_sst_scan(orc_state* s, int on) {
     s->sst_scan = on != 0;
     if (s->sst_scan && s->h.size() != s->n) s->h.assign(s->n, 0.0);

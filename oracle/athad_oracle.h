@@ -91,6 +91,11 @@ void orc_run(orc_state* s, int n, int order);
 // over an all-water mask (RHS_Atm_Turb.cpp:640-671, O(jm + km) per cell and stage) instead of
 // their closed form min(i, j, k). Same results; for timing the reference's own cost.
 void orc_set_sst_scan(orc_state* s, int on);
+// The turbulence closure (TurbulenceAtm.h:24 order): 0 k-epsilon, 1 k-omega, 2 k-omega SST
+// (the default). Returns -1 for any other value.
+int orc_set_turb_model(orc_state* s, int model);
+// TurbulenceAtm::init (TurbulenceAtm.h:39-59) with the state's closure.
+void orc_turbulence_init(orc_state* s);
 // One operator, with the cadence flags of the step it would run in. For ORC_OP_RK4 the
 // ramps are taken from the current total_iter_count.
 void orc_op(orc_state* s, int op, int order);

@@ -25,6 +25,8 @@ FIELDS2D = ["vel_star", "t_top_init"]
 OPS = ["poisson", "satadj", "damp_moist", "moist_clamp", "densities", "evaporation",
        "turbulence", "damp_turb", "bc", "rk4", "polar", "radial", "store", "level_mean", "project"]
 ORDER = {"lex": 0, "jacobi": 1, "lex_planes": 2, "jacobi_reverse": 3}
+# the turbulence closures in TurbulenceAtm.h:24 order
+TURB_MODELS = ["k_epsilon", "k_omega", "k_omega_SST"]
 
 
 class Member(ctypes.Structure):
@@ -65,6 +67,9 @@ def lib():
         L.orc_step.argtypes = [vp, i]
         L.orc_run.argtypes = [vp, i, i]
         L.orc_set_sst_scan.argtypes = [vp, i]
+        L.orc_set_turb_model.restype = i
+        L.orc_set_turb_model.argtypes = [vp, i]
+        L.orc_turbulence_init.argtypes = [vp]
         L.orc_set_moist_convection.argtypes = [vp, i]
         L.orc_set_threads.restype = i
         L.orc_set_threads.argtypes = [i]
@@ -179,6 +184,16 @@ class OracleState:
     def set_sst_scan(self, on=True):
         """The reference's distance-to-land scans in the SST closure (same results, its cost)."""
         lib().orc_set_sst_scan(self.h, 1 if on else 0)
+
+    def set_turb_model(self, model):
+        """The turbulence closure, by name or TurbulenceAtm.h:24 index."""
+        m = TURB_MODELS.index(model) if isinstance(model, str) else int(model)
+        if lib().orc_set_turb_model(self.h, m) != 0:
+            raise ValueError(f"unknown turbulence model {model!r}")
+
+    def turbulence_init(self):
+        """TurbulenceAtm::init with the state's closure."""
+        lib().orc_turbulence_init(self.h)
 
     def step(self, order="lex"):
         lib().orc_step(self.h, ORDER[order])

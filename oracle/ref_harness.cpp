@@ -32,6 +32,9 @@
 //                           jm*km doubles) and, if present, <in.bin>.mc, and writes <out.bin>.mc:
 //                           the 32 convection arrays in mc_fields order (and <out.bin>.cape,
 //                           computeCAPE's im values)
+//   mode = turb:<model>     the turbulence closure <model> (k_epsilon, k_omega or k_omega_SST)
+//                           selected as cAtmosphereModel.cpp:484-489 does, TurbulenceAtm::init,
+//                           the state written to <out.bin>.init, then n scoped iterations
 // The radiation modes also write <out.bin>.rad: radiation and epsilon (im*jm*km doubles each),
 // albedo (jm*km), the top short wave (jm) and t_skin.
 // State file: int32 im,jm,km,iter_n,total_iter; then 35 fields of im*jm*km doubles
@@ -60,6 +63,16 @@ public:
                 &m.tn, &m.un, &m.vn, &m.wn, &m.cn, &m.cloudn, &m.icen, &m.grn, &m.co2n, &m.tken, &m.disn,
                 &m.p_dyn, &m.p_stat, &m.r_humid, &m.r_dry, &m.t_eq, &m.nue, &m.prod, &m.tke_source,
                 &m.dis_source, &m.aux_u, &m.aux_v, &m.aux_w, &m.S_c_c};
+    }
+
+    // the closure selected as cAtmosphereModel.cpp:484-489 does, then TurbulenceAtm::init
+    static void turb_init(cAtmosphereModel& m, const std::string& model) {
+        m.turb_model = model;
+        m.use_turbulence_model = true;
+        m.use_k_epsilon_turbulence_model = (model == "k_epsilon");
+        m.use_k_omega_turbulence_model = (model == "k_omega");
+        m.use_k_omega_SST_turbulence_model = (model == "k_omega_SST");
+        TurbulenceAtm(m).init();
     }
 
     static void setup(cAtmosphereModel& m) {
@@ -452,6 +465,16 @@ int main(int argc, char** argv) {
         RadiationSelfTest::op(*m, mode.substr(3));
     } else if (mode == "init") {
         RadiationSelfTest::init_chain(*m);
+    } else if (mode.rfind("turb:", 0) == 0) {
+        const std::string model = mode.substr(5);
+        if (model != "k_epsilon" && model != "k_omega" && model != "k_omega_SST") {
+            std::fprintf(stderr, "unknown turbulence model %s\n", model.c_str());
+            return 2;
+        }
+        RadiationSelfTest::turb_init(*m, model);
+        RadiationSelfTest::save(*m, (std::string(argv[2]) + ".init").c_str());
+        const int n = (argc > 4) ? std::atoi(argv[4]) : 1;
+        for (int q = 0; q < n; q++) RadiationSelfTest::step(*m);
     }
     RadiationSelfTest::save(*m, argv[2]);
     std::cout.rdbuf(old);

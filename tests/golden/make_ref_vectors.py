@@ -12,6 +12,7 @@ in the tests from the same seed, so only outputs are stored.
     python tests/golden/make_ref_vectors.py --conv        # ref_conv.npz
     python tests/golden/make_ref_vectors.py --grids       # ref_grids.npz
     python tests/golden/make_ref_vectors.py --conv-steps  # ref_conv_steps.npz
+    python tests/golden/make_ref_vectors.py --turb        # ref_turb.npz
 
 --grids needs the harness built at the benchmark grids by oracle/Makefile.grid (a copy of
 the reference's sources with only the grid constants and the spacing changed):
@@ -357,6 +358,34 @@ def grid_vectors():
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+TURB_MODELS = ["k_epsilon", "k_omega", "k_omega_SST"]
+TURB_STEPS = 2
+
+
+def turb_vectors():
+    """The three turbulence closures: for each, the closure selected as
+    cAtmosphereModel.cpp:484-489 selects it, TurbulenceAtm::init on the nominal state (its
+    boundary-layer profile of k and omega or epsilon, the wall values, compute_sources), then
+    two whole iterations in the reference's single-thread order with that closure."""
+    wd = tempfile.mkdtemp(prefix="athad_golden_turb_")
+    n = GRID[0] * GRID[1] * GRID[2]
+    idx = sample_index(n)
+    out = {"grid": np.array(GRID), "seed": np.array(SEED), "steps": np.array(TURB_STEPS), "index": idx}
+    st = O.OracleState(*GRID, seed=SEED)
+    init = os.path.join(wd, "init.bin")
+    st.save_bin(init)
+    o = os.path.join(wd, "out.bin")
+    for model in TURB_MODELS:
+        subprocess.run([HARNESS, init, o, "turb:" + model, str(TURB_STEPS)], check=True, cwd=wd,
+                       stdout=subprocess.DEVNULL, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        pack(f"{model}/init", O.read_bin(o + ".init"), idx, out)
+        pack(f"{model}/steps", O.read_bin(o), idx, out)
+        print(model, "done", flush=True)
+    path = os.path.join(HERE, "ref_turb.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def main():
     if "--conv-steps" in sys.argv[1:]:
         conv_steps_vectors()
@@ -368,6 +397,9 @@ def main():
         sys.exit("build the reference harness first: make -C oracle -f Makefile.ref")
     if "--conv" in sys.argv[1:]:
         conv_vectors()
+        return
+    if "--turb" in sys.argv[1:]:
+        turb_vectors()
         return
     if "--ice" in sys.argv[1:]:
         ice_vectors()

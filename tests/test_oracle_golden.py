@@ -66,6 +66,42 @@ def test_two_steps_lex(oracle, gold):
     check_pack("steps2", st.snapshot(), gold, 1e-11, skip=("S_c_c",))
 
 
+TURB_GOLD = os.path.join(HERE, "golden", "ref_turb.npz")
+
+
+@pytest.mark.parametrize("model", ["k_epsilon", "k_omega", "k_omega_SST"])
+def test_turbulence_closures(oracle, model):
+    """Each closure of TurbulenceAtm (ref_turb.npz): TurbulenceAtm::init on the nominal state
+    (the boundary-layer profile, the closure's wall value, compute_sources), then two whole
+    iterations in the reference's single-thread order with the closure's RHS terms."""
+    gold = np.load(TURB_GOLD, allow_pickle=False)
+    g = tuple(int(x) for x in gold["grid"])
+    st = oracle.OracleState(*g, seed=int(gold["seed"]))
+    st.set_turb_model(model)
+    st.turbulence_init()
+    check_pack(f"{model}/init", st.snapshot(), gold, 1e-13)
+    st.run(int(gold["steps"]), "lex")
+    check_pack(f"{model}/steps", st.snapshot(), gold, 1e-11, skip=("S_c_c",))
+
+
+def test_turbulence_closures_differ():
+    """The golden outputs of the three closures are not one another's: dis is epsilon in the
+    k-epsilon closure (orders of magnitude below omega) and the k-omega and SST sources differ."""
+    gold = np.load(TURB_GOLD, allow_pickle=False)
+    l2 = lambda m, f: float(gold[f"{m}/steps/{f}/l2"])
+    assert l2("k_epsilon", "dis") < 1e-3 * l2("k_omega", "dis")
+    assert abs(l2("k_omega", "dis_source") - l2("k_omega_SST", "dis_source")) > 0.1 * l2("k_omega_SST", "dis_source")
+    assert abs(l2("k_omega", "nue") - l2("k_omega_SST", "nue")) > 1e-4 * l2("k_omega_SST", "nue")
+
+
+def test_unknown_turbulence_model_is_refused(oracle):
+    st = oracle.OracleState(11, 21, 41, seed=1)
+    with pytest.raises(ValueError):
+        st.set_turb_model(3)
+    with pytest.raises(ValueError):
+        st.set_turb_model("laminar")
+
+
 OPS = ["poisson", "satadj", "damp_moist", "moist_clamp", "densities", "evaporation", "turbulence",
        "damp_turb", "bc", "rk4", "polar", "radial", "project"]
 
