@@ -45,8 +45,11 @@ EXPORTS = [
     "athad_set_radiation", "athad_moist_convection", "athad_set_conv_field", "athad_get_conv_field",
     "athad_ice_scheme", "athad_set_ice_scheme", "athad_set_mp_field", "athad_get_mp_field",
     "athad_field_minmax_noalloc", "athad_project_initial_velocity", "athad_set_moist_convection",
-    "athad_get_conv_cape",
+    "athad_get_conv_cape", "athad_set_turbulence_model", "athad_turbulence_init",
 ]
+
+# Turbulence closures, TurbulenceAtm.h:24 order (ATHAD_TURB_* in include/athad_hip.h)
+TURB_MODELS = ["k_epsilon", "k_omega", "k_omega_SST"]
 
 # Arrays of the two-category ice scheme (athad_mp_field in include/athad_hip.h)
 CONV_FIELDS = ["s", "M_u", "M_d", "s_u", "q_v_u", "q_c_u", "c_u", "e_l", "e_d", "e_p", "g_p", "s_d", "q_v_d",
@@ -107,9 +110,10 @@ def lib():
         L.athad_get_iter.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i)]
         for name in ("athad_poisson_sweep", "athad_satadj", "athad_damp_moist", "athad_turbulence",
                      "athad_boundary", "athad_rk4", "athad_filters", "athad_store", "athad_step",
-                     "athad_sync"):
+                     "athad_sync", "athad_turbulence_init"):
             getattr(L, name).argtypes = [vp]
         L.athad_densities.argtypes = [vp, i, i]
+        L.athad_set_turbulence_model.argtypes = [vp, i]
         L.athad_project_initial_velocity.argtypes = [vp, i]
         L.athad_run.argtypes = [vp, i, i]
         L.athad_profile.argtypes = [vp, i, i]

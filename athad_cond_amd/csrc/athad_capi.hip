@@ -90,6 +90,7 @@ struct MemberHost {
     double* conv_tmp[2] = {};
     double* conv_cape = nullptr;   // im doubles, then 2 jm km ints (each column's CAPE level range)
     int iter_n = 0, total = 0;     // host mirror of the device counters
+    double wall_y1 = 1.0e-6;       // first layer height [m], floored as apply_wall_bc floors it
     // table offsets inside 'tables'
     size_t o_rad = 0, o_sint = 0, o_cost = 0, o_sinr = 0, o_hgt = 0, o_nuef = 0, o_step = 0, o_geo = 0;
 };
@@ -98,6 +99,7 @@ struct MemberHost {
 
 struct athad_ctx {
     int nmem = 0, precision = ATHAD_FP64, order = ATHAD_ORDER_JACOBI;
+    int turb_model = kTurbKOmegaSST;   // the turbulence closure of every member
     std::vector<MemberHost> mem;
     MemberDesc* d_md[2] = {nullptr, nullptr};
     std::vector<MemberDesc> h_md[2];
@@ -155,10 +157,11 @@ size_t field_bytes(const athad_ctx* c, const MemberHost& m, int f) {
     return m.n * ((f == F_P_STAT) ? 8 : elem_size(c));
 }
 
-void fill_consts(MemberHost& m, const Config& cfg, int order) {
+void fill_consts(MemberHost& m, const Config& cfg, int order, int turb) {
     Consts& K = m.K;
     const int im = m.g.im, jm = m.g.jm, km = m.g.km;
     K.im = im; K.jm = jm; K.km = km;
+    K.turb = turb;
     K.n = (long long)im * jm * km;
     const double pi180 = 180.0 / M_PI;
     K.dr = 1.0 / (double)(im - 1);
@@ -215,6 +218,16 @@ void fill_consts(MemberHost& m, const Config& cfg, int order) {
     K.albedo_surface = cfg.albedo_surface; K.albedo_cloud = cfg.albedo_cloud; K.omega_rad = cfg.omega_rad;
     K.order = (order == ATHAD_ORDER_SYMMETRIC) ? ATHAD_ORDER_JACOBI : order;
     K.sym = (order == ATHAD_ORDER_SYMMETRIC) ? 1 : 0;
+}
+
+// The closure's value of dis at the wall (TurbulenceAtm.h:128-143), made dimensionless by
+// L_atm / u_0: k-epsilon 0; k-omega 6 nu / (0.075 y1^2); SST 60 nu / (0.0333 y1^2), y1 the first
+// layer height floored at 1e-6 m.
+double wall_dis(int turb, double y1s, const Config& cfg) {
+    const double nd_omega = cfg.L_atm / cfg.u_0;
+    if (turb == kTurbKOmega) return 6.0 * 1.5e-5 / (0.075 * y1s * y1s) * nd_omega;
+    if (turb == kTurbKOmegaSST) return 60.0 * 1.5e-5 / (0.0333 * y1s * y1s) * nd_omega;
+    return 0.0;
 }
 
 // Reduction scratch per member: level means, field statistics, the checkpoint diagnostics
@@ -319,9 +332,8 @@ void build_tables(MemberHost& m, const Config& cfg, std::vector<double>& tab, st
         int p = (int)std::lround(3.0 * (ratio * ratio - 1.0));
         passes[j] = std::min(std::max(p, 0), 12);
     }
-    // wall omega (TurbulenceAtm.h:110-143, SST branch)
-    const double y1s = std::max(hgt[1], 1.0e-6);
-    m.K.om_wall = (60.0 * 1.5e-5 / (0.0333 * y1s * y1s)) * (cfg.L_atm / cfg.u_0);
+    m.wall_y1 = std::max(hgt[1], 1.0e-6);
+    m.K.om_wall = wall_dis(m.K.turb, m.wall_y1, cfg);
 }
 
 void make_desc(const athad_ctx* c, const MemberHost& m, int parity, MemberDesc& d) {
@@ -808,7 +820,7 @@ athad_status athad_create(int n_members, const athad_grid* grids, const athad_me
         m.g = grids[q];
         m.p = params ? params[q] : default_member();
         m.n = (size_t)m.g.im * m.g.jm * m.g.km;
-        fill_consts(m, cfg, order);
+        fill_consts(m, cfg, order, kTurbKOmegaSST);
         std::vector<double> tab;
         std::vector<int> passes;
         build_tables(m, cfg, tab, passes);
@@ -1365,7 +1377,7 @@ athad_status athad_set_member_params(athad_ctx* c, int member, const athad_membe
     m.p = *p;
     const Config cfg;
     c->tskin_dev = false;   // the member's constants, t_skin included, are reset
-    fill_consts(m, cfg, c->order);
+    fill_consts(m, cfg, c->order, c->turb_model);
     std::vector<double> tab;
     std::vector<int> passes;
     build_tables(m, cfg, tab, passes);
@@ -1681,6 +1693,40 @@ athad_status athad_densities(athad_ctx* c, int set_water, int with_evap) {
 athad_status athad_turbulence(athad_ctx* c) {
     if (!c) return ATHAD_ERR_ARG;
     op_turbulence(c, 0);
+    return after_launch(c);
+}
+athad_status athad_set_turbulence_model(athad_ctx* c, int model) {
+    if (!c) return ATHAD_ERR_ARG;
+    if (model != ATHAD_TURB_K_EPSILON && model != ATHAD_TURB_K_OMEGA && model != ATHAD_TURB_K_OMEGA_SST) {
+        c->err = "turbulence model must be ATHAD_TURB_K_EPSILON, ATHAD_TURB_K_OMEGA or ATHAD_TURB_K_OMEGA_SST";
+        return ATHAD_ERR_ARG;
+    }
+    join_s2(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const Config cfg;
+    if (c->turb_model != model && c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    c->turb_model = model;
+    c->batch.turb = model;
+    for (auto& m : c->mem) {
+        m.K.turb = model;
+        m.K.om_wall = wall_dis(model, m.wall_y1, cfg);
+    }
+    return upload_descs(c);
+}
+athad_status athad_turbulence_init(athad_ctx* c) {
+    if (!c) return ATHAD_ERR_ARG;
+    join_s2(c);
+    Batch& b = cur(c);
+    {
+        Prof p(c, KID_TURB);
+        launch_vel_star(b, 0);
+        launch_turb_init_fields(b);
+        launch_wall_bc(b);
+        launch_turb_sources(b, 0);
+    }
+    const int src[6] = {NFIELD + 0, NFIELD + 1, NFIELD + 2, NFIELD + 3, NFIELD + 4, NFIELD + 5};
+    const int dst[6] = {F_TKE, F_DIS, F_NUE, F_PROD, F_TKE_SRC, F_DIS_SRC};
+    launch_copy_job(b, src, dst, 6, 0);
     return after_launch(c);
 }
 athad_status athad_boundary(athad_ctx* c) {

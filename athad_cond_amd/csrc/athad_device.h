@@ -40,7 +40,8 @@ enum MpField { MP_P_RAIN = 0, MP_P_SNOW, MP_P_RAINN, MP_P_SNOWN, MP_PRECIP, MP_S
 
 // Per-member constants, all resolved on the host in double.
 struct Consts {
-    int im, jm, km, pad_;
+    int im, jm, km;
+    int turb;           // turbulence closure, TurbulenceAtm.h:24 order (kTurbKEpsilon ...)
     long long n;                                   // im*jm*km
     double dr, dthe, dphi;
     double inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2;
@@ -135,6 +136,8 @@ ATHAD_DEV long long xcd_block(unsigned b, unsigned nb_launch, long long nb_real)
 // ------------------------------------------------------------------------------------
 constexpr double kTcrit = 647.096, kPcrit = 220640.0, kTtriple = 273.16, kPtriple = 6.11657;
 constexpr double kNoSat = -1.0;
+// the turbulence closures in TurbulenceAtm.h:24 order
+constexpr int kTurbKEpsilon = 0, kTurbKOmega = 1, kTurbKOmegaSST = 2;
 constexpr double kRstar = 8.314462618, kMH2O = 0.018015, kMCO2 = 0.044010;
 #define kRH2O (kRstar / kMH2O)
 #define kRCO2 (kRstar / kMCO2)

@@ -13,6 +13,7 @@ struct Batch {
     int max_im = 0, max_jm = 0, max_km = 0;
     hipStream_t stream = nullptr;
     bool fp32 = false;
+    int turb = kTurbKOmegaSST;    // turbulence closure of every member (selects the compute_sources kernel)
 };
 
 // Kernel ids for per-kernel timing.
@@ -84,6 +85,8 @@ void launch_copy_job(const Batch& b, const int* src, const int* dst, int nf, int
 void launch_densities(const Batch& b, int set_water, int evaporate, int gate);
 void launch_vel_star(const Batch& b, int gate);
 void launch_turb_sources(const Batch& b, int gate);
+void launch_turb_init_fields(const Batch& b);
+void launch_wall_bc(const Batch& b);
 void launch_bc(const Batch& b);
 void launch_solid_clamp(const Batch& b);
 void launch_polar(const Batch& b);

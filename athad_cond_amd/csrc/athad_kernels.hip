@@ -1482,9 +1482,11 @@ for_stages<U>([&](const int st) {
     });
 }
 
-// RK4 of the turbulence group tke, dis with the k-omega SST closure (RHS_Atm_Turb.cpp:633-721),
-// advected by the recorded stage velocities. cen[0..1] = start values in, final values out;
-// prod and the sources are those of the fourth stage.
+// RK4 of the turbulence group tke, dis, advected by the recorded stage velocities, with the
+// member's closure: k-omega SST (RHS_Atm_Turb.cpp:633-721), k-epsilon (:563-599) or k-omega
+// (:601-631); the production refresh (:528-561) takes nu_t = C_mu k^2 / eps in the k-epsilon
+// closure, k / omega otherwise. cen[0..1] = start values in, final values out; prod and the
+// sources are those of the fourth stage.
 template <typename S, typename C, bool CLAMP, bool U, int SB, typename Nb, typename Vel>
 ATHAD_DEV void rk4_turb(const MemberDesc& M, const Nb& src_nb, int i, int j, int k,
                         const CellGeo<C>& g, C* ba, Vel vel, C A, C B, C* cen, C& prod, C& tsrc, C& dsrc) {
@@ -1522,6 +1524,13 @@ ATHAD_DEV void rk4_turb(const MemberDesc& M, const Nb& src_nb, int i, int j, int
     const C bet1 = C(0.0333), bet2 = C(0.0368), gam1 = C(0.413), gam2 = C(0.2);
     const C inv_re = C(K.inv_re_turb);
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt_sixth);
+    const int model = K.turb;
+    // k-epsilon and k-omega: the diffusion coefficients and the cross-diffusion product
+    // sigma_d * cross do not change between stages
+    const bool eps = (model == kTurbKEpsilon);
+    const C dif_tke_2 = eps ? inv_re + nue / C(1.0) : inv_re + C(0.6) * nue;
+    const C dif_dis_2 = eps ? inv_re + nue / C(1.3) : inv_re + C(0.5) * nue;
+    const C sig_cross = (eps || cross <= C(0)) ? C(0) : C(0.125) * cross;
 for_stages<U>([&](const int st) {
         C u, v, w;
         vel(st, u, v, w);
@@ -1531,23 +1540,48 @@ for_stages<U>([&](const int st) {
         // serves every 1/dis.
         const C dis_s = tmax(dis, C(1.0e-10));
         const C inv_dis = rcp_nr(dis_s);
-        const C cnue = tmin(tmax(tke, C(0)) * inv_dis, C(K.nue_max));
-        prod = tmax(C(0), cnue * A - C(0.66667) * tke * B);
-        const C CD_kw = tmax(C(2) * sig_w2 * inv_dis * cross, C(1.0e-20));
-        const C arg1 = tmin(tmax(C(sqrt(tke)) * inv_dis * K2, K1 * inv_dis), tke * K3 * rcp_nr(CD_kw));
-        const C a2 = arg1 * arg1;
-        const C F1 = tanh_pos<C, U>(a2 * a2);
-        const C dif_tke = inv_re + nue * rcp_nr(F1 * sig_k1 + (C(1) - F1) * sig_k2);
-        const C dif_dis = inv_re + nue * rcp_nr(F1 * sig_w1 + (C(1) - F1) * sig_w2);
         const C tke_s = tmax(tke, C(1.0e-10));
-        const C P_k = tmin(prod, C(20) * bet_star * tke_s * dis_s);
-        const C Y_k = bet_star * tke_s * dis_s;
-        const C P_w = (F1 * gam1 + (C(1) - F1) * gam2) * P_k * dis_s * rcp_nr(tke_s);
-        const C Y_w = (F1 * bet1 + (C(1) - F1) * bet2) * dis_s * dis_s;
-        const C D_w = C(2) * (C(1) - F1) * sig_w2 * inv_dis * cross;
-        const C dsm = C(20) * dis_s * dis_s, tsm = C(20) * tke_s * dis_s;
-        tsrc = tclamp(P_k - Y_k, -tsm, tsm);
-        dsrc = tclamp(P_w - Y_w + D_w, -dsm, dsm);
+        C dif_tke, dif_dis;
+        if (model == kTurbKOmegaSST) {
+            const C cnue = tmin(tmax(tke, C(0)) * inv_dis, C(K.nue_max));
+            prod = tmax(C(0), cnue * A - C(0.66667) * tke * B);
+            const C CD_kw = tmax(C(2) * sig_w2 * inv_dis * cross, C(1.0e-20));
+            const C arg1 = tmin(tmax(C(sqrt(tke)) * inv_dis * K2, K1 * inv_dis), tke * K3 * rcp_nr(CD_kw));
+            const C a2 = arg1 * arg1;
+            const C F1 = tanh_pos<C, U>(a2 * a2);
+            dif_tke = inv_re + nue * rcp_nr(F1 * sig_k1 + (C(1) - F1) * sig_k2);
+            dif_dis = inv_re + nue * rcp_nr(F1 * sig_w1 + (C(1) - F1) * sig_w2);
+            const C P_k = tmin(prod, C(20) * bet_star * tke_s * dis_s);
+            const C Y_k = bet_star * tke_s * dis_s;
+            const C P_w = (F1 * gam1 + (C(1) - F1) * gam2) * P_k * dis_s * rcp_nr(tke_s);
+            const C Y_w = (F1 * bet1 + (C(1) - F1) * bet2) * dis_s * dis_s;
+            const C D_w = C(2) * (C(1) - F1) * sig_w2 * inv_dis * cross;
+            const C dsm = C(20) * dis_s * dis_s, tsm = C(20) * tke_s * dis_s;
+            tsrc = tclamp(P_k - Y_k, -tsm, tsm);
+            dsrc = tclamp(P_w - Y_w + D_w, -dsm, dsm);
+        } else {
+            const C tke_h = tmax(tke, C(0));
+            const C cnue = tmin(eps ? C(0.028) * tke_h * tke_h * inv_dis : tke_h * inv_dis, C(K.nue_max));
+            prod = tmax(C(0), cnue * A - C(0.66667) * tke * B);
+            dif_tke = dif_tke_2;
+            dif_dis = dif_dis_2;
+            const C ratio = dis_s * rcp_nr(tke_s);   // dis / tke
+            C P_k, Y_k, P_w, Y_w;
+            if (eps) {   // Chien's k-epsilon without its wall terms
+                P_k = prod;
+                Y_k = dis_s;
+                P_w = C(1.35) * ratio * P_k;
+                Y_w = C(1.80) * dis_s * ratio;
+            } else {     // Wilcox's k-omega
+                P_k = tmin(prod, C(20) * C(0.09) * tke_s * dis_s);
+                Y_k = C(0.09) * tke_s * dis_s;
+                P_w = C(0.52) * ratio * P_k;
+                Y_w = C(0.0708) * dis_s * dis_s;
+            }
+            const C dsm = C(20) * dis_s * dis_s, tsm = C(20) * tke_s * dis_s;
+            tsrc = tclamp(P_k - Y_k, -tsm, tsm);
+            dsrc = tclamp(P_w - Y_w + sig_cross * inv_dis, -dsm, dsm);
+        }
         const C rk = -(u * gc[0][0] + v * gc[0][1] + w * gc[0][2]) + (L0[0] - g.Lc2 * tke) * dif_tke + tsrc;
         const C rd = -(u * gc[1][0] + v * gc[1][1] + w * gc[1][2]) + (L0[1] - g.Lc2 * dis) * dif_dis + dsrc;
         C vk = stage_update(st, rk, ba + 0 * SB, ba + 2 * SB, dt, half_dt, dt_sixth);
@@ -2953,7 +2987,7 @@ __global__ void k_vel_star(const MemberDesc* __restrict__ md, int gate) {
     M.vel_star[q] = tmax(0.05, sqrt(0.002) * U);
 }
 
-template <typename S>
+template <typename S, int MODEL>
 __global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* __restrict__ md, int gate) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
@@ -3024,7 +3058,9 @@ __global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* __restri
     const C dwdphi = (C(w[id + 1]) - C(w[id - 1])) * inv_rms2dphi;
     const C dtkedr = (tk_ip1 - tk_im1) * inv_2dr * exp_rm;
     const C ddisdr = (ds_ip1 - ds_im1) * inv_2dr * exp_rm;
-    C cnue = tmax(tk, C(0)) / tmax(ds, dis_min);
+    constexpr int model = MODEL;   // one instantiation per closure: the others' registers stay out
+    C cnue = (model == kTurbKEpsilon) ? C(0.028) * tmax(tk, C(0)) * tmax(tk, C(0)) / tmax(ds, dis_min)
+                                      : tmax(tk, C(0)) / tmax(ds, dis_min);
     cnue = tmin(cnue, nue_max);
     const C der = C(0.66667) * (dudr + dvdthe + dwdphi);
     const C prod = tmax(C(0),
@@ -3039,14 +3075,57 @@ __global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* __restri
         + (cnue * (dwdthe + dvdphi)) * dwdthe);
     const C W12 = dudthe - dvdr, W13 = dudphi - dwdr, W23 = dvdphi - dwdthe;
     const C Omega = sqrt(W12 * W12 + W13 * W13 + W23 * W23);
-    // compute_k_omega_SST (grid rm in the theta/phi gradients, TurbulenceAtm.h:722-725)
-    const C a1 = C(0.31), bet_star = C(0.09), bet1 = C(0.0333), bet2 = C(0.0368), gam1 = C(0.413), gam2 = C(0.2);
-    const C sig_w2 = C(1.168);
-    const C y_star = C(tmax(M.hgt[i] - M.hgt[0], 1.0e-6) / K.L_atm);
     const C rm2dthe = C(rm * 2.0 * K.dthe), rms2dphi = C(rmsinthe * 2.0 * K.dphi);
     const C dtkedthe = (tk_jp1 - tk_jm1) / rm2dthe, ddisdthe = (ds_jp1 - ds_jm1) / rm2dthe;
     const C dtkedphi = (tk_kp1 - tk_km1) / rms2dphi, ddisdphi = (ds_kp1 - ds_km1) / rms2dphi;
     const C cross = dtkedr * ddisdr + dtkedthe * ddisdthe + dtkedphi * ddisdphi;
+    C nue, tsrc, dsrc;
+    if constexpr (model == kTurbKEpsilon) {
+        // compute_k_epsilon (TurbulenceAtm.h:481-530): Chien's damping functions and wall terms,
+        // the wall distance from the surface floored at one L_atm
+        const double y_phys = tmax(M.hgt[i] - M.hgt[0], K.L_atm);
+        const C y_star = C(y_phys / K.L_atm);
+        const C d_plus = C(y_phys * M.vel_star[(size_t)j * K.km + k] / 1.5e-5);
+        const C f_nue = C(1) - exp(C(-0.0115) * d_plus);
+        const C nu_air = C(1.5e-5);
+        const C Re_T = tk * tk * C(K.u_0) * C(K.L_atm) / tmax(ds * nu_air, dis_min * nu_air);
+        const C f_2 = C(1) - C(0.4 / 1.8) * exp(-Re_T * Re_T / C(36));
+        nue = C(0.028) * tk * tk / tmax(ds, dis_min) * f_nue;
+        const C y_star2 = y_star * y_star;
+        const C L_k = C(-2) * tk / y_star2;
+        const C L_w = C(-2) * ds / y_star2 * exp(-d_plus / C(2));
+        const C tke_safe = tmax(tk, dis_min);
+        const C P_w = C(1.35) * ds / tke_safe * prod;
+        const C Y_w = C(1.80) * f_2 * ds * ds / tke_safe;
+        tsrc = prod - ds + L_k / C(K.re_turb);
+        dsrc = P_w - Y_w + L_w / C(K.re_turb);
+    } else if constexpr (model == kTurbKOmega) {
+        // compute_k_omega (TurbulenceAtm.h:533-651, grid rm): strain-rate limiter on omega,
+        // vorticity-strain correction of beta, cross-diffusion above zero
+        const C bet_star = C(0.09);
+        const C r2dr = C(2.0 * K.dr), rp1 = C(rm + 1.0);
+        const C S11 = (C(u[id + si]) - C(u[id - si])) / r2dr / rp1;
+        const C S22 = (C(v[id + sj]) - C(v[id - sj])) / rm2dthe;
+        const C S33 = (C(w[id + 1]) - C(w[id - 1])) / rms2dphi;
+        const C S12 = C(0.5) * ((C(u[id + sj]) - C(u[id - sj])) / rm2dthe + (C(v[id + si]) - C(v[id - si])) / r2dr / rp1);
+        const C S13 = C(0.5) * ((C(u[id + 1]) - C(u[id - 1])) / rms2dphi + (C(w[id + si]) - C(w[id - si])) / r2dr / rp1);
+        const C S23 = C(0.5) * ((C(v[id + 1]) - C(v[id - 1])) / rms2dphi + (C(w[id + sj]) - C(w[id - sj])) / rm2dthe);
+        const C S_mag = sqrt(C(2) * (S11 * S11 + S22 * S22 + S33 * S33 + C(2) * (S12 * S12 + S13 * S13 + S23 * S23)));
+        const C w_hat = tmax(ds, C(0.875) * S_mag / C(0.3));   // sqrt(0.09), which rounds to 0.3
+        const C sig_d = (cross <= C(0)) ? C(0) : C(0.125);
+        const C bw = bet_star * tmax(ds, C(1.0e-20));
+        const C chi_w = fabs(Omega * Omega * S_mag / (bw * bw * bw));
+        const C f_bet = (C(1) + C(85) * chi_w) / (C(1) + C(100) * chi_w);
+        nue = tk / tmax(w_hat, C(1.0e-20));
+        const C P_k = tmin(prod, C(20) * bet_star * tk * ds);
+        const C P_w = C(0.52) * ds / tmax(tk, C(1.0e-20)) * P_k;
+        tsrc = P_k - bet_star * tk * ds;
+        dsrc = P_w - C(0.0708) * f_bet * ds * ds + sig_d / tmax(ds, C(1.0e-20)) * cross;
+    } else {
+    // compute_k_omega_SST (grid rm in the theta/phi gradients, TurbulenceAtm.h:722-725)
+    const C a1 = C(0.31), bet_star = C(0.09), bet1 = C(0.0333), bet2 = C(0.0368), gam1 = C(0.413), gam2 = C(0.2);
+    const C sig_w2 = C(1.168);
+    const C y_star = C(tmax(M.hgt[i] - M.hgt[0], 1.0e-6) / K.L_atm);
     const C CD_kw = tmax(C(2) * sig_w2 / tmax(ds, C(1.0e-20)) * cross, C(1.0e-20));
     const C dis_safe = tmax(ds, C(1.0e-20));
     const C nu_nd = C(K.nue_air_nd);
@@ -3056,14 +3135,16 @@ __global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* __restri
     const C a1sq = arg1 * arg1;
     const C F1 = tanh_pos(a1sq * a1sq);
     const C F2 = tanh_pos(arg2 * arg2);
-    C nue = a1 * tk / tmax(a1 * dis_safe, Omega * F2);
+    nue = a1 * tk / tmax(a1 * dis_safe, Omega * F2);
     const C tke_safe = tmax(tk, C(1.0e-20));
     const C P_k = tmin(prod, C(20) * bet_star * tke_safe * dis_safe);
     const C Y_k = bet_star * tke_safe * dis_safe;
     const C P_w = (F1 * gam1 + (C(1) - F1) * gam2) * P_k * dis_safe / tke_safe;
     const C Y_w = (F1 * bet1 + (C(1) - F1) * bet2) * dis_safe * dis_safe;
     const C D_w = C(2) * (C(1) - F1) * sig_w2 / dis_safe * cross;
-    C tsrc = P_k - Y_k, dsrc = P_w - Y_w + D_w;
+    tsrc = P_k - Y_k;
+    dsrc = P_w - Y_w + D_w;
+    }
     const C tk_out = tmax(kbg(j, k), tk);
     const C ds_out = tmax(dis_min, ds);
     nue = tclamp(nue, C(0), nue_max);
@@ -3658,6 +3739,60 @@ __global__ void k_init_rest(const MemberDesc* __restrict__ md) {
     fp<S>(M, F_DIS)[id] = S(1.0 * K.L_atm / K.u_0);
 }
 
+// TurbulenceAtm::init_fields (TurbulenceAtm.h:193-255): per interior cell the boundary-layer
+// profile k = u_tau^2 / sqrt(C_mu) (1 - z/h_abl)^2 above 1 % of its surface value, and with it
+// omega = C_mu^-1/4 sqrt(k) / (kappa z) (k-omega closures) or eps = C_mu^3/4 k^3/2 / (kappa z)
+// (k-epsilon) above the same expressions at the background k and z = L_atm; z is the height
+// above the surface, written to tke, dis and their stored copies.
+template <typename S>
+__global__ __launch_bounds__(256) void k_turb_init_fields(const MemberDesc* __restrict__ md) {
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const long long id = cell_id(K);
+    if (id < 0) return;
+    int i, j, k;
+    decode(K, id, i, j, k);
+    if (i < 1 || i > K.im - 2 || j < 1 || j > K.jm - 2 || k < 1 || k > K.km - 2) return;
+    constexpr double C_nue = 0.028, Karman = 0.42;
+    const double vs = M.vel_star[(size_t)j * K.km + k];
+    const double k_bg = 0.01 * vs * vs / sqrt(C_nue);
+    const double z_i = M.hgt[i] - M.hgt[0];
+    const double z_safe = tmax(z_i, 1.0e-6);
+    const double z_frac = tmin(z_i, K.abl_height) / K.abl_height;
+    const double k_abl = vs * vs / sqrt(C_nue) * ((1.0 - z_frac) * (1.0 - z_frac));
+    const double k_phys = tmax(k_abl, k_bg);
+    double dis_nd;
+    if (K.turb == kTurbKEpsilon) {
+        const double eps_bg = pow(C_nue, 0.75) * pow(k_bg, 1.5) / (Karman * K.L_atm);
+        const double eps_phys = pow(C_nue, 0.75) * pow(k_phys, 1.5) / (Karman * z_safe);
+        dis_nd = tmax(eps_phys, eps_bg) * (K.L_atm / (K.u_0 * K.u_0 * K.u_0));
+    } else {
+        const double om_bg = pow(C_nue, -0.25) * sqrt(k_bg) / (Karman * K.L_atm);
+        const double om_phys = pow(C_nue, -0.25) * sqrt(k_phys) / (Karman * z_safe);
+        dis_nd = tmax(om_phys, om_bg) * (K.L_atm / K.u_0);
+    }
+    const S tk = S(tmax(0.0, k_phys * (1.0 / (K.u_0 * K.u_0))));
+    const S ds = S(tmax(1.0e-10, dis_nd));
+    fp<S>(M, F_TKE)[id] = tk;
+    fp<S>(M, F_TKEN)[id] = tk;
+    fp<S>(M, F_DIS)[id] = ds;
+    fp<S>(M, F_DISN)[id] = ds;
+}
+
+// TurbulenceAtm::apply_wall_bc (TurbulenceAtm.h:109-147) alone, per (j,k) column: the log-layer
+// tke, zero eddy viscosity and the closure's wall value of dis at i = 0.
+template <typename S>
+__global__ void k_wall_bc(const MemberDesc* __restrict__ md) {
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)K.jm * K.km) return;
+    const double vs = M.vel_star[q];
+    fp<S>(M, F_TKE)[q] = S(vs * vs / (K.sqrt_cnue * K.u_0 * K.u_0));
+    fp<S>(M, F_NUE)[q] = S(0);
+    fp<S>(M, F_DIS)[q] = S(K.om_wall);
+}
+
 template <typename S>
 __global__ void k_copy_field(const MemberDesc* __restrict__ md, int dst, int src) {
     const MemberDesc& M = md[blockIdx.z];
@@ -3952,8 +4087,15 @@ void launch_densities(const Batch& b, int set_water, int evaporate, int gate) {
 void launch_vel_star(const Batch& b, int gate) {
     ATHAD_DISPATCH(k_vel_star, dim3(cols_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate);
 }
+template <int MODEL> void launch_turb_sources_m(const Batch& b, int gate) {
+    const dim3 grid(cells_grid(b, 256), 1, b.nmem);
+    if (b.fp32) hipLaunchKernelGGL((k_turb_sources<float, MODEL>), grid, dim3(256), 0, b.stream, b.d_md, gate);
+    else hipLaunchKernelGGL((k_turb_sources<double, MODEL>), grid, dim3(256), 0, b.stream, b.d_md, gate);
+}
 void launch_turb_sources(const Batch& b, int gate) {
-    ATHAD_DISPATCH(k_turb_sources, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate);
+    if (b.turb == kTurbKEpsilon) launch_turb_sources_m<kTurbKEpsilon>(b, gate);
+    else if (b.turb == kTurbKOmega) launch_turb_sources_m<kTurbKOmega>(b, gate);
+    else launch_turb_sources_m<kTurbKOmegaSST>(b, gate);
 }
 void launch_bc(const Batch& b) {
     ATHAD_DISPATCH(k_bc_radial, dim3(cols_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
@@ -4037,6 +4179,12 @@ void launch_copy_lid(const Batch& b) {
 }
 void launch_init_rest(const Batch& b) {
     ATHAD_DISPATCH(k_init_rest, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
+}
+void launch_turb_init_fields(const Batch& b) {
+    ATHAD_DISPATCH(k_turb_init_fields, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
+}
+void launch_wall_bc(const Batch& b) {
+    ATHAD_DISPATCH(k_wall_bc, dim3(cols_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
 }
 
 }  // namespace athad

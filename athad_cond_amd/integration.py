@@ -118,6 +118,16 @@ class Integration:
     def turbulence(self):
         self._chk(L.lib().athad_turbulence(self.h))
 
+    def set_turbulence_model(self, model):
+        """The turbulence closure of every member: "k_epsilon", "k_omega" or "k_omega_SST"
+        (the default), or its TurbulenceAtm.h:24 index."""
+        m = L.TURB_MODELS.index(model) if isinstance(model, str) else int(model)
+        self._chk(L.lib().athad_set_turbulence_model(self.h, m))
+
+    def turbulence_init(self):
+        """TurbulenceAtm::init with the current closure (TurbulenceAtm.h:39-59)."""
+        self._chk(L.lib().athad_turbulence_init(self.h))
+
     def boundary(self):
         self._chk(L.lib().athad_boundary(self.h))
 

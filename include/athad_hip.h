@@ -48,6 +48,12 @@ typedef int athad_status;
                                    neighbours. Not in the reference, whose sweep order and wall
                                    distance break the mirror at rounding level and near the poles. */
 
+/* Turbulence closures, in the order of TurbulenceAtm's Model enum (TurbulenceAtm.h:24); the
+ * reference selects one from the turb_model string (cAtmosphereModel.cpp:484-489). */
+#define ATHAD_TURB_K_EPSILON    0   /* Chien 1982 low-Reynolds k-epsilon */
+#define ATHAD_TURB_K_OMEGA      1   /* Wilcox 1988 k-omega */
+#define ATHAD_TURB_K_OMEGA_SST  2   /* Menter 1994 k-omega SST, the configuration's choice (default) */
+
 /* Field ids: the cAtmosphereModel Array members of the scoped step
  * (atmosphere/cAtmosphereModel.h:535-662). */
 enum athad_field {
@@ -154,6 +160,18 @@ athad_status athad_densities(athad_ctx* ctx, int set_water_profile, int with_eva
 /* TurbulenceAtm(*this).run() and the damp_wiggles of its six fields
  * (cAtmosphereModel.cpp:1426-1433; TurbulenceAtm.h:62-79). */
 athad_status athad_turbulence(athad_ctx* ctx);
+/* The turbulence closure of every member (ATHAD_TURB_*), used from then on by the turbulence
+ * update (compute_sources with compute_k_epsilon, compute_k_omega or compute_k_omega_SST,
+ * TurbulenceAtm.h:323-773), the RK4 right-hand side (RHS_Atm_Turb.cpp:528-721) and the wall
+ * value of dis (apply_wall_bc, TurbulenceAtm.h:109-147). Replaces setting turb_model before
+ * TurbulenceAtm is constructed (cAtmosphereModel.cpp:484-489). The laminar mode is not
+ * offered: the reference never sets re_turb without TurbulenceAtm::init. */
+athad_status athad_set_turbulence_model(athad_ctx* ctx, int model);
+/* TurbulenceAtm(*this).init() with the current closure (cAtmosphereModel.cpp:613;
+ * TurbulenceAtm.h:39-59): the boundary-layer profile of tke and dis with their stored copies
+ * (init_fields, :193-255), the wall values, compute_sources and clamp_nue. athad_init_state
+ * seeds tke and dis with the benchmark's uniform k = 1e-4 m2/s2, omega = 1/s instead. */
+athad_status athad_turbulence_init(athad_ctx* ctx);
 /* bcRadius, apply_wall_bc, bcTheta, bcPhi, bcScalarSurfSur, bcSolidGround
  * (cAtmosphereModel.cpp:1440-1446). */
 athad_status athad_boundary(athad_ctx* ctx);
