@@ -140,6 +140,33 @@ def run_ice(A, precision, steps, warmup, seed=1234, profile_steps=8, graph=True)
             "ice_ms_per_call": ice_ms / max(ice_n, 1), "calls_profiled": ice_n}
 
 
+def run_closures(A, steps, warmup, seed=1234, profile_steps=8, graph=True):
+    """The FP64 step with each turbulence closure (TurbulenceAtm.h:24), each from the closure's
+    own TurbulenceAtm::init profile: the iteration, then the turbulence update's launches
+    (vel_star and compute_sources, every second iteration) alone."""
+    n = GRID[0] * GRID[1] * GRID[2]
+    out = {}
+    for model in ("k_epsilon", "k_omega", "k_omega_SST"):
+        with A.Integration(GRID, precision="fp64") as integ:
+            integ.init_state(seed)
+            integ.set_turbulence_model(model)
+            integ.turbulence_init()
+            integ.run(warmup, graph=graph)
+            integ.sync()
+            t0 = time.perf_counter()
+            integ.run(steps, graph=graph)
+            integ.sync()
+            dt = time.perf_counter() - t0
+            integ.profile("turb", True)
+            integ.run(profile_steps)
+            integ.sync()
+            t_ms, t_n = integ.profile_result("turb")
+            integ.profile("turb", False)
+        out[model] = {"ms_per_step": dt / steps * 1e3, "cells_per_s": n * steps / dt,
+                      "turbulence_update_ms": t_ms / max(t_n, 1)}
+    return out
+
+
 def run_next_rows(A, reps=10, seed=1234, warmup=8):
     """Wall time per call of the operators built beside the step, on the FP64 state at this
     grid after `warmup` iterations: mode-2 radiation (with the host skin-temperature step),
@@ -287,6 +314,8 @@ def main():
                     help="only the lex order with 1 and 32 members sharing the plane launches")
     ap.add_argument("--orders-only", action="store_true",
                     help="only the lex / jacobi / symmetric spread over 100 iterations")
+    ap.add_argument("--closures-only", action="store_true",
+                    help="only the FP64 step with each turbulence closure")
     ap.add_argument("--mixed-only", action="store_true",
                     help="only time the mixed form (for profiling it alone); prints its own JSON line")
     args = ap.parse_args()
@@ -322,6 +351,10 @@ def main():
         r = run_ice(A, "fp64", args.steps, args.warmup, graph=graph)
         print(json.dumps({"form": "fp64 with the ice scheme", "grid": list(GRID), "steps": args.steps,
                           "warmup": args.warmup, **r}))
+        return
+    if args.closures_only:
+        print(json.dumps({"grid": list(GRID), "steps": args.steps, "warmup": args.warmup,
+                          **run_closures(A, args.steps, args.warmup, graph=graph)}))
         return
     if args.mixed_only:
         mixed = run_form(A, "mixed", "jacobi", args.steps, args.warmup, profile_steps=20, graph=graph)
@@ -400,6 +433,11 @@ def main():
             "FP64 Jacobi iteration at 41x181x361 with the ice scheme and moist convection in the moist block "
             "and radiation after the store (run_3D_loop without output), hipGraph")
         note("configured loop done")
+        out["turbulence_models"] = run_closures(A, max(8, args.steps // 4), 4, graph=graph)
+        out["turbulence_models"]["workload"] = (
+            "FP64 Jacobi step at 41x181x361 with each turbulence closure from its TurbulenceAtm::init "
+            "profile; turbulence_update_ms: vel_star and compute_sources per call")
+        note("turbulence closures done")
         out["next_rows"] = run_next_rows(A)
         note("next rows done")
         out["ensemble"] = {prec: run_ensemble(32, (41, 91, 181), steps=40, precision=prec)
