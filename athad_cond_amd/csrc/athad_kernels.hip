@@ -2712,13 +2712,16 @@ __global__ __launch_bounds__(256) void k_damp_pass(const MemberDesc* __restrict_
 // it upward through levels [i0, i0 + ich). For each level it stages the tile plus a one-cell
 // halo of every field in LDS (j clamped, k periodic, exactly the neighbours the separate
 // passes read), forms the k pass and then the j pass at its own cell, and keeps the j-pass
-// results of levels i-1, i, i+1 in registers for the i pass. Reads src, writes dst (distinct
-// buffers: neighbouring tiles still read src). Results equal the separate passes bit for bit
-// for every tile shape, so the shape is free for an autotuner to choose.
+// results of levels i-1, i, i+1 in registers for the i pass. Each thread's staging cells are
+// fixed per level, so their offsets are formed once; the next level's values are loaded into
+// registers as soon as this level's are in LDS, so the loads overlap the level's passes and
+// stores. Reads src, writes dst (distinct buffers: neighbouring tiles still read src). Results
+// equal the separate passes bit for bit for every tile shape, so the shape is free for an
+// autotuner to choose.
 template <typename S, int TJ, int TK, int NF>
 __global__ __launch_bounds__(TJ * TK) void k_damp_tile(const MemberDesc* __restrict__ md, DampJob job, int ich, int gate) {
     using C = S;
-    constexpr int W = TK + 2, SL = (TJ + 2) * W, NT = TJ * TK;
+    constexpr int W = TK + 2, SL = (TJ + 2) * W, NT = TJ * TK, NE = (SL + NT - 1) / NT;
     __shared__ C slab[NF * SL];
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
@@ -2739,18 +2742,38 @@ __global__ __launch_bounds__(TJ * TK) void k_damp_tile(const MemberDesc* __restr
     S* dst[NF];
 #pragma unroll
     for (int q = 0; q < NF; q++) { src[q] = job_ptr<S>(M, job.src[q]); dst[q] = job_ptr<S>(M, job.dst[q]); }
-    // j-pass result at level il (clamped to the grid) for this thread's cell, every field
-    auto level = [&](int il, C* out) {
-        const int ii = min(max(il, 0), K.im - 1);
-        __syncthreads();
-        for (int e = threadIdx.x; e < SL; e += NT) {
-            const int r = e / W, qk = e - r * W;
-            const int jj = min(max(j0 - 1 + r, 0), K.jm - 1);
-            const int kk = (k0 - 1 + qk + K.km) % K.km;
-            const long long g = ((long long)ii * K.jm + jj) * K.km + kk;
+    // this thread's staging cells: slab position threadIdx.x + u NT, offset inside a level
+    int eoff[NE];
 #pragma unroll
-            for (int q = 0; q < NF; q++) slab[q * SL + e] = C(src[q][g]);
-        }
+    for (int u = 0; u < NE; u++) {
+        const int e = threadIdx.x + u * NT;
+        const int r = e / W, qk = e - r * W;
+        const int jj = min(max(j0 - 1 + r, 0), K.jm - 1);
+        const int kk = (k0 - 1 + qk + K.km) % K.km;
+        eoff[u] = (e < SL) ? jj * K.km + kk : 0;
+    }
+    const long long si = (long long)K.jm * K.km;
+    C st[NE][NF];   // one level's staging values, loaded a level ahead
+    auto fetch = [&](int il) {
+        const long long lo = (long long)min(max(il, 0), K.im - 1) * si;
+#pragma unroll
+        for (int u = 0; u < NE; u++)
+            if (u < NE - 1 || threadIdx.x + u * NT < SL) {
+#pragma unroll
+                for (int q = 0; q < NF; q++) st[u][q] = C(src[q][lo + eoff[u]]);
+            }
+    };
+    // j-pass result at level il (clamped to the grid) for this thread's cell, every field; the
+    // staging values of il are in st, and those of il + 1 are requested before the passes
+    auto level = [&](int il, C* out) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NE; u++)
+            if (u < NE - 1 || threadIdx.x + u * NT < SL) {
+#pragma unroll
+                for (int q = 0; q < NF; q++) slab[q * SL + threadIdx.x + u * NT] = st[u][q];
+            }
+        fetch(il + 1);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < NF; q++) {
@@ -2767,6 +2790,7 @@ __global__ __launch_bounds__(TJ * TK) void k_damp_tile(const MemberDesc* __restr
         }
     };
     C jm_[NF], jc[NF], jp[NF];
+    fetch(i0 - 1);
     level(i0 - 1, jm_);
     level(i0, jc);
     for (int i = i0; i < i1; i++) {
