@@ -38,6 +38,10 @@ def test_errors_without_context():
     assert L.athad_get_mp_field(None, 0, 0, buf) == 1
     assert L.athad_set_mp_field(None, 0, 0, buf) == 1
     assert L.athad_radiation(None) == 1
+    assert L.athad_set_turbulence_model(None, 0) == 1
+    assert L.athad_turbulence_init(None) == 1
+    out = (ctypes.c_double * 64)()
+    assert L.athad_get_conv_cape(None, 0, out) == 1
     g = A._lib.Grid(41, 180, 361)           # even jm is rejected before any device call
     h = ctypes.c_void_p()
     assert L.athad_create(1, ctypes.byref(g), None, 0, 1, ctypes.byref(h)) == 4
