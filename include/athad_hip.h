@@ -107,7 +107,7 @@ athad_status athad_member_from_surface(double p_surf_bar, double t_sea, double x
  * buffers, scratch, the tables and reductions. Not counted, because they are allocated only
  * once their operator is first used: the ice-scheme arrays (10 fields in the storage
  * precision), the radiation outputs (2 FP64 fields and jm*km) and the moist-convection arrays
- * (36 FP64 fields, jm*km and jm doubles). */
+ * (36 FP64 fields, jm*km and jm doubles, and the CAPE profile: im doubles and 2 jm*km ints). */
 int64_t athad_footprint_bytes(const athad_grid* grid, int precision);
 
 /* --- state transfer ---------------------------------------------------------------- */
