@@ -83,6 +83,9 @@ def test_members_are_independent(oracle):
             e = errors(integ, ref, member=m)
             bad = {k: v for k, v in e.items() if v > 1e-11}
             assert not bad, (m, bad)
+            cape, cref = integ.conv_cape(member=m), ref.cape()
+            assert np.array_equal(cape == 0.0, cref == 0.0), m
+            assert rel(cape, cref) <= 1e-11, m
 
 
 def test_mixed_precision_fields(oracle):
@@ -102,6 +105,9 @@ def test_mixed_precision_fields(oracle):
         print({k: f"{v:.1e}" for k, v in e.items() if v > 0})
         bad = {k: v for k, v in e.items() if v > 1e-11}
         assert not bad, bad
+        cape, cref = integ.conv_cape(), ref.cape()
+        assert np.array_equal(cape == 0.0, cref == 0.0)
+        assert rel(cape, cref) <= 1e-11
 
 
 def test_coarse_latitude_grid_matches_oracle(oracle):
