@@ -1761,38 +1761,30 @@ ATHAD_DEV void cell_geo_row(const MemberDesc& M, int i, int j, double ps_c, doub
     g.cor = C(q[G_COR]);
 }
 
-// Rows of 64-cell segments: a wave owns longitudes 64 s .. 64 s + 63 of one (i, j) row.
-// Wave-linear index L of a member -> (i, j, s); `order` picks which coordinate runs fastest
-// (0: s, then j, then i, the storage order; 1: i, then s, then j; 2: i, then j, then s).
-struct RowSeg { int i, j, s; };
-ATHAD_DEV RowSeg row_of_wave(const Consts& K, long long L, int nseg, int order) {
-    const unsigned u = (unsigned)L;
-    const unsigned d0 = (order == 0) ? (unsigned)nseg : (unsigned)K.im;
-    const unsigned d1 = (order == 1) ? (unsigned)nseg : (unsigned)K.jm;
-    const unsigned r = u / d0;
-    const int a = (int)(u - r * d0), b = (int)(r % d1), c = (int)(r / d1);
-    return RowSeg{(order == 0) ? c : a, (order == 1) ? c : b, (order == 0) ? a : (order == 1) ? b : c};
-}
-
 // Jacobi-order RK4 with one wave per row segment (same arithmetic and outputs as
-// k_rk4_jacobi): i and j are wave-uniform, so the metric coefficients sit in scalar registers,
-// the interior test of i and j is a uniform branch, and the cell index needs no division.
+// k_rk4_jacobi): a wave owns longitudes 64 s .. 64 s + 63 of one (i, j) row, so i and j are
+// wave-uniform, the metric coefficients sit in scalar registers, the interior test of i and j
+// is a uniform branch, and the cell index needs no division. A block's four waves take four
+// neighbouring latitude rows j = 4 jb .. 4 jb + 3 at one level and segment, so each row's j - 1
+// and j + 1 neighbours are the rows its sibling waves load at the same time; blocks run i
+// fastest (the i +- 1 rows are the neighbouring blocks'), then the segment, then jb.
 template <typename S, bool FUSED, int SRC>
-__global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_row_waves<S>()))) void k_rk4_rows(const MemberDesc* __restrict__ md, int order) {
+__global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_row_waves<S>()))) void k_rk4_rows(const MemberDesc* __restrict__ md) {
     using C = S;
     // 20 slots per thread (40 KB per block in FP64): four blocks of four waves fit a CU's LDS
     __shared__ C s_ba[8 * LB], s_rec[12 * LB];
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const int nseg = (K.km + 63) >> 6;
-    const long long nw = (long long)K.im * K.jm * nseg;
-    const long long b = xcd_block(blockIdx.x, gridDim.x, (nw + 3) >> 2);
+    const long long nblk = (long long)K.im * ((K.jm + 3) >> 2) * nseg;
+    const long long b = xcd_block(blockIdx.x, gridDim.x, nblk);
     if (b < 0) return;
-    const long long L = b * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (L >= nw) return;
-    const RowSeg rs = row_of_wave(K, L, nseg, order);
-    const int i = rs.i, j = rs.j;
-    const int k = rs.s * 64 + (int)(threadIdx.x & 63);
+    const int i = (int)(b % K.im);
+    const long long rb = b / K.im;
+    const int seg = (int)(rb % nseg);
+    const int j = (int)(rb / nseg) * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (j >= K.jm) return;   // the last group of four rows where jm is not a multiple of 4
+    const int k = seg * 64 + (int)(threadIdx.x & 63);
     if (k >= K.km) return;
     const long long row = ((long long)i * K.jm + j) * K.km;
     const long long id = row + k;
@@ -3985,16 +3977,16 @@ template <typename S> void rk4_jacobi_s(const Batch& b, const dim3& g, bool fuse
         default: rk4_jacobi_t<S, 3>(b, g, fused); break;
     }
 }
-template <typename S, int SRC> void rk4_rows_t(const Batch& b, const dim3& g, bool fused, int order) {
-    if (fused) hipLaunchKernelGGL((k_rk4_rows<S, true, SRC>), g, dim3(256), 0, b.stream, b.d_md, order);
-    else hipLaunchKernelGGL((k_rk4_rows<S, false, SRC>), g, dim3(256), 0, b.stream, b.d_md, order);
+template <typename S, int SRC> void rk4_rows_t(const Batch& b, const dim3& g, bool fused) {
+    if (fused) hipLaunchKernelGGL((k_rk4_rows<S, true, SRC>), g, dim3(256), 0, b.stream, b.d_md);
+    else hipLaunchKernelGGL((k_rk4_rows<S, false, SRC>), g, dim3(256), 0, b.stream, b.d_md);
 }
-template <typename S> void rk4_rows_s(const Batch& b, const dim3& g, bool fused, int src, int order) {
+template <typename S> void rk4_rows_s(const Batch& b, const dim3& g, bool fused, int src) {
     switch (src & 3) {
-        case 0: rk4_rows_t<S, 0>(b, g, fused, order); break;
-        case 1: rk4_rows_t<S, 1>(b, g, fused, order); break;
-        case 2: rk4_rows_t<S, 2>(b, g, fused, order); break;
-        default: rk4_rows_t<S, 3>(b, g, fused, order); break;
+        case 0: rk4_rows_t<S, 0>(b, g, fused); break;
+        case 1: rk4_rows_t<S, 1>(b, g, fused); break;
+        case 2: rk4_rows_t<S, 2>(b, g, fused); break;
+        default: rk4_rows_t<S, 3>(b, g, fused); break;
     }
 }
 template <typename S, int LBX> void rk4_lex_t(const Batch& b, int plane, int src) {
@@ -4015,17 +4007,17 @@ template <typename S> void rk4_lex_s(const Batch& b, int plane, int src) {
     else rk4_lex_t<S, 256>(b, plane, src);
 }
 }  // namespace
-// FP64: the row-segment kernel, waves taken i fastest, then the longitude segment, then j
-// (measured at 41 x 181 x 361: 0.445 ms against 0.476 with the storage order and 0.498 with
-// the segment slowest, 0.453 with the longitudes in two halves each swept i fastest, then j;
-// the cell-linear FP64 kernel 0.530). FP32: the cell-linear kernel at five
-// waves per SIMD (0.214 ms; the row kernel 0.245).
+// FP64: the row-segment kernel, blocks of four latitude rows taken i fastest, then the
+// longitude segment, then the row group (measured at 41 x 181 x 361: 0.447 ms against 0.450
+// with single waves taken i fastest, then segment, then j; 0.476 with the storage order, 0.498
+// with the segment slowest; the cell-linear FP64 kernel 0.530). FP32: the cell-linear kernel
+// at five waves per SIMD (0.214 ms; the row kernel 0.245).
 void launch_rk4_jacobi(const Batch& b, bool fused, int src) {
     if (!b.fp32) {
         const int nseg = (b.max_km + 63) / 64;
-        const long long nb = ((long long)b.max_im * b.max_jm * nseg + 3) / 4;
+        const long long nb = (long long)b.max_im * ((b.max_jm + 3) / 4) * nseg;
         const dim3 g((unsigned)(((nb + 7) / 8) * 8), 1, b.nmem);
-        rk4_rows_s<double>(b, g, fused, src, 1);
+        rk4_rows_s<double>(b, g, fused, src);
         return;
     }
     const dim3 g(cells_grid(b, 256), 1, b.nmem);
