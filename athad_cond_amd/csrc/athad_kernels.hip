@@ -1836,7 +1836,6 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_row_wave
 template <typename S, int SRC, int LBX>
 __global__ __launch_bounds__(LBX) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_rk4_lex_plane(const MemberDesc* __restrict__ md, int d) {
     using C = S;
-    __shared__ C s_ba[10 * LBX], s_rec[12 * LBX];
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const int nj = K.jm - 2;
@@ -1851,7 +1850,11 @@ __global__ __launch_bounds__(LBX) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     for (int f = 0; f < NPROG; f++) X[f] = fp<S>(M, kProgF[f]);
     const NbRegs<S, C, SRC> nb(M, (const S* const*)X, id, (long long)K.jm * K.km, (long long)K.km,
                                (long long)ix3(K, 0, j, k));
-    rk4_cell<S, C, false, SRC, true, true, false, LBX>(M, nb, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
+    // one wave per SIMD leaves registers to spare: the RK4 base / accumulator slots and the
+    // recorded stage velocities live in registers (stride 1, every index fixed once the stages
+    // are unrolled) instead of LDS, off the latency of the serial chain
+    C r_ba[10], r_rec[12];
+    rk4_cell<S, C, false, SRC, true, true, false, 1>(M, nb, i, j, k, r_ba, r_rec,
                           [&](int f, C v) { gst(X[f], id, S(v)); },
                           [&](int fid, C v) { gst(fp<S>(M, fid), id, S(v)); });
 }
