@@ -314,6 +314,8 @@ def main():
                     help="only the lex order with 1 and 32 members sharing the plane launches")
     ap.add_argument("--orders-only", action="store_true",
                     help="only the lex / jacobi / symmetric spread over 100 iterations")
+    ap.add_argument("--lex-only", action="store_true",
+                    help="only the FP64 step in the reference's lex order (the 1e-10 drop-in)")
     ap.add_argument("--closures-only", action="store_true",
                     help="only the FP64 step with each turbulence closure")
     ap.add_argument("--mixed-only", action="store_true",
@@ -351,6 +353,11 @@ def main():
         r = run_ice(A, "fp64", args.steps, args.warmup, graph=graph)
         print(json.dumps({"form": "fp64 with the ice scheme", "grid": list(GRID), "steps": args.steps,
                           "warmup": args.warmup, **r}))
+        return
+    if args.lex_only:
+        r = run_form(A, "fp64", "lex", args.steps, args.warmup, graph=graph)
+        r.pop("fields")
+        print(json.dumps({"form": "fp64 lex", "grid": list(GRID), "steps": args.steps, "warmup": args.warmup, **r}))
         return
     if args.closures_only:
         print(json.dumps({"grid": list(GRID), "steps": args.steps, "warmup": args.warmup,
