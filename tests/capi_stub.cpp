@@ -44,6 +44,8 @@ int null_calls() {
     const int gi = athad_get_iter(nullptr, 0, &it, &tot);
     const int st = athad_step(nullptr);
     const int rk = athad_rk4(nullptr);
+    const int tm = athad_set_turbulence_model(nullptr, ATHAD_TURB_K_EPSILON);
+    const int ti = athad_turbulence_init(nullptr);
     const int ds = athad_destroy(nullptr);
     const char* msg = athad_last_error(nullptr);
     const long long fp = (long long)athad_footprint_bytes(&g, ATHAD_FP64);
@@ -52,10 +54,11 @@ int null_calls() {
     athad_member_params p{};
     const int mfs = athad_member_from_surface(60.0, 513.15, 0.01, 0.92, &p);
     std::printf("{\"create_bad\": %d, \"upload_rows\": %d, \"download_rows\": %d, \"set_iter\": %d, "
-                "\"get_iter\": %d, \"step\": %d, \"rk4\": %d, \"destroy\": %d, \"last_error\": \"%s\", "
+                "\"get_iter\": %d, \"step\": %d, \"rk4\": %d, \"turb_model\": %d, \"turb_init\": %d, "
+                "\"destroy\": %d, \"last_error\": \"%s\", "
                 "\"footprint\": %lld, \"restart_bytes\": %lld, \"dew_point_60bar\": %.6f, "
                 "\"member_from_surface\": %d, \"x_H2O\": %.6f, \"ctx_null\": %d, \"version\": \"%s\"}\n",
-                bad_create, up, down, si, gi, st, rk, ds, msg ? msg : "", fp, rb, dew, mfs, p.x_H2O,
+                bad_create, up, down, si, gi, st, rk, tm, ti, ds, msg ? msg : "", fp, rb, dew, mfs, p.x_H2O,
                 ctx == nullptr, athad_version());
     return 0;
 }

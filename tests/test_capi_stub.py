@@ -22,7 +22,8 @@ def test_stub_compiles_links_and_refuses_null_context(tmp_path):
     out = subprocess.run([exe, "null"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stderr
     r = json.loads(out.stdout)
-    for key in ("create_bad", "upload_rows", "download_rows", "set_iter", "get_iter", "step", "rk4", "destroy"):
+    for key in ("create_bad", "upload_rows", "download_rows", "set_iter", "get_iter", "step", "rk4", "turb_model",
+                "turb_init", "destroy"):
         assert r[key] == 1, key            # ATHAD_ERR_ARG
     assert r["ctx_null"] == 1
     assert r["last_error"] == "bad arguments"
