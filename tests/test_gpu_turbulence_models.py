@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import athad_cond_amd as A
+from athad_cond_amd import _lib as L
 
 pytestmark = pytest.mark.gpu
 
@@ -115,6 +116,60 @@ def test_mixed_precision_closure(oracle, model):
         print({k: f"{v:.1e}" for k, v in e.items()})
         bad = {k: v for k, v in e.items() if v > 1e-4}
         assert not bad, bad
+
+
+@pytest.mark.parametrize("model", ["k_epsilon", "k_omega"])
+def test_closure_in_the_configured_moist_block(oracle, model):
+    """A closure other than SST with the ice scheme and moist convection live in the moist block:
+    from a spun-up state with planted condensate, a seeded Evaporation input and seeded MC_*
+    tendencies, the closure's own init, then eight Jacobi iterations, four of them moist, against
+    the oracle configured the same way (state, ice-scheme fluxes and convection arrays within
+    1e-10 relative L2)."""
+    print(f"case turbulence {model} configured loop {GRID}", flush=True)
+    ref = oracle.OracleState(*GRID, seed=31)
+    ref.run(2, "jacobi")
+    snap = ref.snapshot()
+    mp = {k: np.zeros(GRID) for k in oracle.MP_FIELDS}
+    oracle.plant_microphysics(snap, mp, 32)
+    evap = np.random.default_rng(31).uniform(-1.0, 6.0, GRID[1:])
+    conv = L.CONV_FIELDS[:-1]
+    rng = np.random.default_rng(33)
+    mcp = {n: np.zeros(GRID) for n in conv}
+    for name, lo, hi in (("MC_t", -1e-3, 1e-3), ("MC_q", -1e-6, 1e-6), ("MC_v", -1e-4, 1e-4),
+                         ("MC_w", -1e-4, 1e-4), ("P_conv", 0.0, 1e-3)):
+        mcp[name] = rng.uniform(lo, hi, GRID)
+    ref = oracle.OracleState(*GRID, blank=True)
+    ref.load(snap)
+    for k, v in mp.items():
+        ref.mp(k)[...] = v
+    ref.evap()[...] = evap
+    for n, v in mcp.items():
+        ref.mc(n)[...] = v
+    ref.set_turb_model(model)
+    ref.turbulence_init()
+    ref.set_iter(2, 2)
+    ref.set_moist_convection(True)
+    ref.run(8, "jacobi")
+    with A.Integration(GRID) as integ:
+        integ.load(snap)
+        for k, v in mp.items():
+            integ.set_mp(k, v)
+        integ.set_conv("Evaporation", evap)
+        for n, v in mcp.items():
+            integ.set_conv(n, v)
+        integ.set_turbulence_model(model)
+        integ.turbulence_init()
+        integ.set_iter(2, 2)
+        integ.set_moist_convection(True)
+        integ.run(8, graph=True)
+        e = errors(integ, ref, STATE)
+        e.update({"mp/" + n: rel(integ.get_mp(n), ref.mp(n)) for n in ("P_rain", "P_snow", "S_c", "S_v")})
+        e.update({"mc/" + n: rel(integ.get_conv(n), ref.mc(n)) for n in conv})
+        print({k: f"{v:.1e}" for k, v in e.items()})
+        bad = {k: v for k, v in e.items() if not v <= 1e-10}
+        assert not bad, bad
+        # convection was live: the scheme rewrote the tendencies it was given
+        assert rel(ref.mc("MC_t"), mcp["MC_t"]) > 1e-3 and np.abs(ref.mc("MC_t")).max() > 0
 
 
 def test_unknown_closure_is_refused():
