@@ -60,11 +60,20 @@ ATHAD_DEV void decode(const Consts& K, long long id, int& i, int& j, int& k) {
     i = (int)ii;
 }
 
+// A load through the constant address space: at a wave-uniform address it is a scalar load,
+// waited for by lgkmcnt, so it does not drain the vector loads in flight the way a late vector
+// load waited for by vmcnt(0) does; at a per-lane address it stays a vector load. Only for
+// memory that no thread of the running kernel writes: the iteration counters (k_advance, a
+// launch of its own, is their only writer) and the per-level tables.
+template <typename T> ATHAD_DEV T ro_load(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+
 // Cadence gate: a kernel launched with gate g > 0 does nothing for members whose iteration
 // counter is not a multiple of g (Poisson every 4th, moist block every 2nd iteration,
 // cAtmosphereModel.cpp:1156-1185), so members at different phases can share launches.
 ATHAD_DEV bool gated_off(const MemberDesc& M, int gate) {
-    return gate > 0 && (M.counters[0] % gate) != 0;
+    return gate > 0 && (ro_load(M.counters) % gate) != 0;
 }
 
 // ------------------------------------------------------------------------------------
@@ -998,6 +1007,16 @@ template <typename T> ATHAD_DEV void gst(T* p, long long i, T v) {
 // only adds the centre-value terms, the minmod slopes and the SST closure. The RK4 base and
 // accumulator live in LDS.
 // ------------------------------------------------------------------------------------
+// sigma = p_stat / p_surf of the Held-Suarez rate, zero where p_surf is not positive. The
+// division runs for every lane and the zero is selected after it (the same values): written as
+// `p_surf > 0 ? ps / p_surf : 0` it compiles to a branch that waits for p_surf before p_stat and
+// the cell's later loads are issued.
+ATHAD_DEV double hs_sigma(double ps_c, double p_surf) {
+    double sigma = ps_c / p_surf;
+    if (!(p_surf > 0.0)) sigma = 0.0;
+    return sigma;
+}
+
 template <typename C> struct CellGeo {
     C aR, aT, aP;          // advection metric / grid step, for the minmod slopes
     C cexp, cirm, cirs;    // advection metric exp_rm, 1/rm, 1/(rm sin theta)
@@ -1028,7 +1047,7 @@ ATHAD_DEV void cell_geo(const MemberDesc& M, int i, int j, double ps_c, double p
     g.cu = C(q[G_CU]);
     g.cvw = C(q[G_CVW]);
     g.cirm2 = C(q[G_CIRM2]); g.cirm2s = C(q[G_CIRM2S]); g.c2cos = C(q[G_C2COS]);
-    const double sigma = (p_surf > 0.0) ? ps_c / p_surf : 0.0;
+    const double sigma = hs_sigma(ps_c, p_surf);
     double bl = (sigma - 0.7) / (1.0 - 0.7);
     if (bl < 0.0) bl = 0.0;
     const double kT = K.hs_k_a + (K.hs_k_s - K.hs_k_a) * bl * q[G_S4];
@@ -1067,7 +1086,7 @@ ATHAD_DEV void cell_geo_alu(const MemberDesc& M, int i, int j, double ps_c, doub
     g.cu = C(2.0 * inv_rm2 - Lc2);
     g.cvw = C(-v_metric - Lc2);
     g.cirm2 = C(inv_rm2); g.cirm2s = C(inv_rm2sinthe); g.c2cos = C(2.0 * costhe * inv_rm2s2);
-    const double sigma = (p_surf > 0.0) ? ps_c / p_surf : 0.0;
+    const double sigma = hs_sigma(ps_c, p_surf);
     double bl = (sigma - 0.7) / (1.0 - 0.7);
     if (bl < 0.0) bl = 0.0;
     const double kT = K.hs_k_a + (K.hs_k_s - K.hs_k_a) * bl * (sinthe2 * sinthe2);
@@ -1179,11 +1198,14 @@ template <typename C> ATHAD_DEV C solid_clamp_mm(int f, C v, C tke_max, C vel_ma
 // one cell before the field (f - si - 1), so the displacements of all seven points are
 // non-negative scalar offsets: centre c = (si + 1) E, i -+ 1 at c -+ si E, j -+ 1 at c -+ sj E,
 // k -+ 1 at c -+ E (E = bytes per value). off = byte offset of the cell in a field,
-// soff = of its surface cell in p_stat (FP64).
+// soff = of its surface cell in p_stat (FP64). tkm, vm: the clamp bounds tke_max_nd and
+// vel_max_nd, read once by the kernel; the clamps then need no load of their own, which would
+// otherwise make each clamp a branch that waits for its value before the next load issues.
 template <typename S, typename C, bool CLAMP> struct NbRow {
     static constexpr unsigned E = sizeof(S);
     const MemberDesc& M;
     unsigned off, soff, dsi, dsj, nst, nfld;   // nst: bytes of a stencil descriptor, nfld: of a field
+    C tkm, vm;
     ATHAD_DEV Rsrc stencil(int fid) const { return make_rsrc((const char*)M.f[fid] - dsi - E, nst); }
     ATHAD_DEV void six_of(int fid, C* v) const {
         const Rsrc r = stencil(fid);
@@ -1195,7 +1217,6 @@ template <typename S, typename C, bool CLAMP> struct NbRow {
     ATHAD_DEV void six(int f, C* v) const {
         six_of(kProgF[f], v);
         if (CLAMP) {
-            const C tkm = C(M.K.tke_max_nd), vm = C(M.K.vel_max_nd);
 #pragma unroll
             for (int d = 0; d < 6; d++) v[d] = solid_clamp_mm(f, v[d], tkm, vm);
         }
@@ -1209,12 +1230,12 @@ template <typename S, typename C, bool CLAMP> struct NbRow {
     }
     ATHAD_DEV C start(int f) const {
         C c0 = C(bld<S>(stencil(kProgF[f]), off, dsi + E));
-        if (CLAMP) c0 = solid_clamp_mm(f, c0, C(M.K.tke_max_nd), C(M.K.vel_max_nd));
+        if (CLAMP) c0 = solid_clamp_mm(f, c0, tkm, vm);
         return c0;
     }
     ATHAD_DEV C base(int f) const {   // the *n clamps of tken, disn, un, vn, wn
         C b0 = once(M.f[kProgN[f]], nfld);
-        if (CLAMP && f != FC && f != FCL && f != FI) b0 = solid_clamp_mm(f, b0, C(M.K.tke_max_nd), C(M.K.vel_max_nd));
+        if (CLAMP && f != FC && f != FCL && f != FI) b0 = solid_clamp_mm(f, b0, tkm, vm);
         return b0;
     }
     ATHAD_DEV double ps() const { return bld<double, kNT>(make_rsrc(pstat(M), nfld / E * 8u), off / E * 8u, 0); }
@@ -1275,7 +1296,7 @@ template <typename C> ATHAD_DEV C minmod3(const C* nb, C c0, C ua, C va, C wa) {
 }
 
 template <typename C> ATHAD_DEV C buoyancy(const MemberDesc& M) {
-    const int total = M.counters[1];
+    const int total = ro_load(M.counters + 1);
     const double ramp = fmin(1.0, (double)total / 300.0);
     return C(ramp * 1.0 * M.K.g * M.K.dt / M.K.u_0);
 }
@@ -1375,7 +1396,7 @@ ATHAD_DEV void rk4_dyn(const MemberDesc& M, const Nb& src_nb, int i, const CellG
         dif_t = (inv_re + nv) * C(K.inv_pr_turb);
     }
     const C t_eq = src_nb.field(F_T_EQ);
-    const C t_ref = C(M.t_ref[i]);
+    const C t_ref = C(ro_load(M.t_ref + i));
     const C buoy = buoyancy<C>(M);
     const C ce = C(K.coeff_energy_p);
     // latent heating of the microphysics sources (RHS_Atm_Turb.cpp:965-966), stage-invariant
@@ -1725,7 +1746,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
     constexpr unsigned E = sizeof(S);
     const unsigned si = (unsigned)(K.jm * K.km), nfld = (unsigned)K.n * E;
     const NbRow<S, C, FUSED> nb{M, (unsigned)id * E, (unsigned)(j * K.km + k) * 8u, si * E, (unsigned)K.km * E,
-                                nfld + (2u * si + 2u) * E, nfld};
+                                nfld + (2u * si + 2u) * E, nfld, tkm, vm};
     rk4_cell<S, C, FUSED, SRC, false, sizeof(S) == 4, true, LB>(M, nb, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x,
         [&](int f, C v) {
             nb.put(M.alt[f], v);
@@ -1752,7 +1773,7 @@ ATHAD_DEV void cell_geo_row(const MemberDesc& M, int i, int j, double ps_c, doub
     g.cu = C(q[G_CU]);
     g.cvw = C(q[G_CVW]);
     g.cirm2 = C(q[G_CIRM2]); g.cirm2s = C(q[G_CIRM2S]); g.c2cos = C(q[G_C2COS]);
-    const double sigma = (p_surf > 0.0) ? ps_c / p_surf : 0.0;
+    const double sigma = hs_sigma(ps_c, p_surf);
     double bl = (sigma - 0.7) / (1.0 - 0.7);
     if (bl < 0.0) bl = 0.0;
     const double kT = K.hs_k_a + (K.hs_k_s - K.hs_k_a) * bl * q[G_S4];
@@ -1820,7 +1841,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_row_wave
     constexpr unsigned E = sizeof(S);
     const unsigned si = (unsigned)(K.jm * K.km), nfld = (unsigned)K.n * E;
     const NbRow<S, C, FUSED> nb{M, (unsigned)id * E, (unsigned)(j * K.km + k) * 8u, si * E, (unsigned)K.km * E,
-                                nfld + (2u * si + 2u) * E, nfld};
+                                nfld + (2u * si + 2u) * E, nfld, tkm, vm};
     rk4_cell<S, C, FUSED, SRC, false, 2, true, LB>(M, nb, i, j, k, s_ba + threadIdx.x, s_rec + threadIdx.x, [&](int f, C v) {
         nb.put(M.alt[f], v);
         if (FUSED && f != FU && f != FV && f != FW) nb.put(M.f[kProgN[f]], v);
@@ -1957,7 +1978,7 @@ __global__ void k_p_ceiling(const MemberDesc* __restrict__ md, int gate) {
     if (gated_off(M, gate)) return;
     const long long id = cell_id(M.K);
     if (id < 0) return;
-    const S c = S((M.counters[1] > 300) ? 3.0 : 10.0);
+    const S c = S((ro_load(M.counters + 1) > 300) ? 3.0 : 10.0);
     S* p = fp<S>(M, F_P_DYN);
     const S v = p[id];
     if (!finite_bits(v)) p[id] = S(0);
