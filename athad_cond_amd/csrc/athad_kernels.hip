@@ -1697,6 +1697,37 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const Nb& nb, int i, int j, int k, 
     }
 }
 
+// A boundary cell of the Jacobi-order RK4: its prognostic values copied to alt[] (clamped and
+// also stored when FUSED), and with FUSED the clamps of nue, prod and the two sources. Every
+// value is read before any is written: the compiler cannot tell that the fields do not overlap,
+// so a read placed after a write waits for that write, one memory round trip per field.
+template <typename S, bool FUSED>
+ATHAD_DEV void rk4_boundary_copy(const MemberDesc& M, const S* const* X, long long id, S tkm, S vm) {
+    using C = S;
+    const Consts& K = M.K;
+    C v[NPROG], x[4];
+#pragma unroll
+    for (int f = 0; f < NPROG; f++) v[f] = C(gld(X[f], id));
+    if (FUSED) {
+        x[0] = C(gld(fp<S>(M, F_NUE), id));
+        x[1] = C(gld(fp<S>(M, F_PROD), id));
+        x[2] = C(gld(fp<S>(M, F_TKE_SRC), id));
+        x[3] = C(gld(fp<S>(M, F_DIS_SRC), id));
+    }
+#pragma unroll
+    for (int f = 0; f < NPROG; f++) {
+        if (FUSED) v[f] = solid_clamp(f, v[f], tkm, vm);
+        gst((S*)M.alt[f], id, S(v[f]));
+        if (FUSED && f != FU && f != FV && f != FW) gst(fp<S>(M, kProgN[f]), id, S(v[f]));
+    }
+    if (FUSED) {
+        gst(fp<S>(M, F_NUE), id, S(finite_bits(x[0]) ? tclamp(x[0], C(0), C(K.nue_max)) : C(0)));
+        gst(fp<S>(M, F_PROD), id, S(finite_bits(x[1]) ? tclamp(x[1], C(0), C(1.0e4)) : C(0)));
+        gst(fp<S>(M, F_TKE_SRC), id, S(finite_bits(x[2]) ? tclamp(x[2], C(-1.0e4), C(1.0e4)) : C(-1.0e4)));
+        gst(fp<S>(M, F_DIS_SRC), id, S(finite_bits(x[3]) ? tclamp(x[3], C(-1.0e4), C(1.0e4)) : C(-1.0e4)));
+    }
+}
+
 // Jacobi-order RK4 over the whole grid, one thread per cell in storage order (the FP32 form;
 // FP64 runs k_rk4_rows). Interior cells integrate four stages against the start-of-RK4
 // neighbourhood read from f[] and write alt[]; boundary cells are copied. The solid-ground
@@ -1718,27 +1749,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_waves<S>
     const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
     const bool interior = (i > 0 && i < K.im - 1 && j > 0 && j < K.jm - 1 && k > 0 && k < K.km - 1);
     if (!interior) {
-#pragma unroll
-        for (int f = 0; f < NPROG; f++) {
-            C v = C(gld(X[f], id));
-            if (FUSED) v = solid_clamp(f, v, tkm, vm);
-            gst((S*)M.alt[f], id, S(v));
-            if (FUSED && f != FU && f != FV && f != FW) gst(fp<S>(M, kProgN[f]), id, S(v));
-        }
-        if (FUSED) {
-            S* nue = fp<S>(M, F_NUE);
-            const C nv = C(gld(nue, id));
-            gst(nue, id, S(finite_bits(nv) ? tclamp(nv, C(0), C(K.nue_max)) : C(0)));
-            S* pr = fp<S>(M, F_PROD);
-            const C pv = C(gld(pr, id));
-            gst(pr, id, S(finite_bits(pv) ? tclamp(pv, C(0), C(1.0e4)) : C(0)));
-            S* ts = fp<S>(M, F_TKE_SRC);
-            const C tv = C(gld(ts, id));
-            gst(ts, id, S(finite_bits(tv) ? tclamp(tv, C(-1.0e4), C(1.0e4)) : C(-1.0e4)));
-            S* dsr = fp<S>(M, F_DIS_SRC);
-            const C dv = C(gld(dsr, id));
-            gst(dsr, id, S(finite_bits(dv) ? tclamp(dv, C(-1.0e4), C(1.0e4)) : C(-1.0e4)));
-        }
+        rk4_boundary_copy<S, FUSED>(M, X, id, tkm, vm);
         return;
     }
     // buffer-resource addressing as in the row kernel; the metric table read per lane in FP32
@@ -1815,27 +1826,7 @@ __global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(rk4_row_wave
     const C tkm = C(K.tke_max_nd), vm = C(K.vel_max_nd);
     const bool interior = (i > 0 && i < K.im - 1 && j > 0 && j < K.jm - 1 && k > 0 && k < K.km - 1);
     if (!interior) {
-#pragma unroll
-        for (int f = 0; f < NPROG; f++) {
-            C v = C(gld(X[f], id));
-            if (FUSED) v = solid_clamp(f, v, tkm, vm);
-            gst((S*)M.alt[f], id, S(v));
-            if (FUSED && f != FU && f != FV && f != FW) gst(fp<S>(M, kProgN[f]), id, S(v));
-        }
-        if (FUSED) {
-            S* nue = fp<S>(M, F_NUE);
-            const C nv = C(gld(nue, id));
-            gst(nue, id, S(finite_bits(nv) ? tclamp(nv, C(0), C(K.nue_max)) : C(0)));
-            S* pr = fp<S>(M, F_PROD);
-            const C pv = C(gld(pr, id));
-            gst(pr, id, S(finite_bits(pv) ? tclamp(pv, C(0), C(1.0e4)) : C(0)));
-            S* ts = fp<S>(M, F_TKE_SRC);
-            const C tv = C(gld(ts, id));
-            gst(ts, id, S(finite_bits(tv) ? tclamp(tv, C(-1.0e4), C(1.0e4)) : C(-1.0e4)));
-            S* dsr = fp<S>(M, F_DIS_SRC);
-            const C dv = C(gld(dsr, id));
-            gst(dsr, id, S(finite_bits(dv) ? tclamp(dv, C(-1.0e4), C(1.0e4)) : C(-1.0e4)));
-        }
+        rk4_boundary_copy<S, FUSED>(M, X, id, tkm, vm);
         return;
     }
     constexpr unsigned E = sizeof(S);
