@@ -27,11 +27,16 @@ ATHAD_DEV double* pstat(const MemberDesc& M) { return (double*)M.f[F_P_STAT]; }
 
 // 1-D cell index of this thread for a kernel launched over a member's n cells with the
 // XCD-aware block remap; -1 when out of range.
+// Threads per block of every kernel that indexes cells through cell_id / cell_id_top_down.
+// A constant rather than blockDim.x, which is read from the dispatch packet by a 16-bit vector
+// load that every wave waits for before its first cell load can issue.
+constexpr int kCellBlock = 256;
+
 ATHAD_DEV long long cell_id(const Consts& K) {
-    const long long nb = (K.n + blockDim.x - 1) / blockDim.x;
+    const long long nb = (K.n + kCellBlock - 1) / kCellBlock;
     const long long b = xcd_block(blockIdx.x, gridDim.x, nb);
     if (b < 0) return -1;
-    const long long id = b * blockDim.x + threadIdx.x;
+    const long long id = b * kCellBlock + threadIdx.x;
     return (id < K.n) ? id : -1;
 }
 
@@ -43,9 +48,9 @@ ATHAD_DEV long long cell_id(const Consts& K) {
 // blocks run from the top level down, so the slow cells start first and the cheap lower
 // levels fill in behind them. -1 when out of range.
 ATHAD_DEV long long cell_id_top_down(const Consts& K) {
-    const long long nb = (K.n + blockDim.x - 1) / blockDim.x;
+    const long long nb = (K.n + kCellBlock - 1) / kCellBlock;
     if ((long long)blockIdx.x >= nb) return -1;
-    const long long id = (nb - 1 - (long long)blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long id = (nb - 1 - (long long)blockIdx.x) * kCellBlock + threadIdx.x;
     return (id < K.n) ? id : -1;
 }
 
@@ -1851,7 +1856,7 @@ __global__ __launch_bounds__(LBX) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const int nj = K.jm - 2;
-    const long long qd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long qd = (long long)blockIdx.x * LBX + threadIdx.x;
     if (qd >= (long long)(K.im - 2) * nj) return;
     const int i = 1 + (int)(qd / nj), j = 1 + (int)(qd % nj);
     const int k = d - i - j;
@@ -1875,11 +1880,11 @@ __global__ __launch_bounds__(LBX) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 // Poisson red-black sweep (PressureSolverAtm.h:106-121, :225-486, :504-594)
 // ------------------------------------------------------------------------------------
 template <typename S>
-__global__ void k_aux_bc(const MemberDesc* __restrict__ md, int gate) {
+__global__ __launch_bounds__(256) void k_aux_bc(const MemberDesc* __restrict__ md, int gate) {
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const Consts& K = M.K;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
     if (q >= (long long)(K.jm - 2) * (K.km - 2)) return;
     const int j = 1 + (int)(q / (K.km - 2)), k = 1 + (int)(q % (K.km - 2));
     for (int f = F_AUX_U; f <= F_AUX_W; f++) {
@@ -2352,7 +2357,7 @@ __global__ __launch_bounds__(256) void k_ice_columns(const MemberDesc* __restric
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const Consts& K = M.K;
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = blockIdx.x * 256 + threadIdx.x;
     if (col >= K.jm * K.km) return;
     const int j = col / K.km;
     ice_column_seq<S>(M, j, col - j * K.km);
@@ -2850,7 +2855,7 @@ __global__ __launch_bounds__(256) void k_damp_level0(const MemberDesc* __restric
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const Consts& K = M.K;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
     if (q >= (long long)K.jm * K.km) return;
     const int j = (int)(q / K.km), k = (int)(q % K.km);
     const S* x = job_ptr<S>(M, src);
@@ -3008,11 +3013,11 @@ __global__ __launch_bounds__(128) void k_densities(const MemberDesc* __restrict_
 // without ordering. Outputs go to scratch q0..q5 = tke, dis, nue, prod, tke_src, dis_src.
 // ------------------------------------------------------------------------------------
 template <typename S>
-__global__ void k_vel_star(const MemberDesc* __restrict__ md, int gate) {
+__global__ __launch_bounds__(256) void k_vel_star(const MemberDesc* __restrict__ md, int gate) {
     const MemberDesc& M = md[blockIdx.z];
     if (gated_off(M, gate)) return;
     const Consts& K = M.K;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
     if (q >= (long long)K.jm * K.km) return;
     const int isurf = min(1, K.im - 1);
     const size_t id = (size_t)isurf * K.jm * K.km + q;
@@ -3035,42 +3040,57 @@ __global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* __restri
     S *otk = (S*)M.tmp[0], *ods = (S*)M.tmp[1], *onu = (S*)M.tmp[2], *opr = (S*)M.tmp[3], *ots = (S*)M.tmp[4],
       *odss = (S*)M.tmp[5];
     const C nue_max = C(K.nue_max);
-    const C f_abl = C(M.nue_f[i]);
     const bool interior = (i > 0 && i < K.im - 1 && j > 0 && j < K.jm - 1 && k > 0 && k < K.km - 1);
+    // Every value is read before any is written, and the floors below are applied to values
+    // read unconditionally: a read inside a conditional, or after a write to a field the
+    // compiler cannot tell apart from it, waits for a memory round trip of its own.
     if (!interior) {
-        otk[id] = ptk[id]; ods[id] = pds[id];
+        const S tv = ptk[id], dv = pds[id];
         const C nv = C(fp<S>(M, F_NUE)[id]);
+        const S pv = fp<S>(M, F_PROD)[id], tsv = fp<S>(M, F_TKE_SRC)[id], dsv = fp<S>(M, F_DIS_SRC)[id];
+        const C f_abl = C(M.nue_f[i]);
+        otk[id] = tv; ods[id] = dv;
         onu[id] = S(tclamp(nv, C(0), nue_max) * f_abl);
-        opr[id] = fp<S>(M, F_PROD)[id];
-        ots[id] = fp<S>(M, F_TKE_SRC)[id];
-        odss[id] = fp<S>(M, F_DIS_SRC)[id];
+        opr[id] = pv;
+        ots[id] = tsv;
+        odss[id] = dsv;
         return;
     }
     const size_t si = (size_t)K.jm * K.km, sj = K.km;
     const C u0 = C(K.u_0);
-    auto kbg = [&](int jj, int kk) {
-        const C vs = C(M.vel_star[(size_t)jj * K.km + kk]);
+    auto kbg_of = [&](double vsd) {
+        const C vs = C(vsd);
         return C(0.01) * vs * vs / (C(K.sqrt_cnue) * u0 * u0);
     };
+    const double* vel_star = M.vel_star;
+    const size_t q0 = (size_t)j * K.km + k;
+    const double vs_c = vel_star[q0], vs_jm = vel_star[q0 - K.km], vs_km = vel_star[q0 - 1];
+    const double vs_jp = vel_star[q0 + K.km];
     const C dis_min = C(1.0e-10);
-    // visited neighbours: floors of the visit (max(0,.) then max(k_bg,.); dis max(dis_min,.))
-    auto tke_vis = [&](size_t q, int jj, int kk) { return tmax(kbg(jj, kk), tmax(C(0), C(ptk[q]))); };
-    auto dis_vis = [&](size_t q) { return tmax(dis_min, C(pds[q])); };
-    const C tk = tmax(C(0), C(ptk[id]));
-    const C ds = tmax(dis_min, C(pds[id]));
-    const C tk_im1 = (i - 1 >= 1) ? tke_vis(id - si, j, k) : C(ptk[id - si]);
-    const C ds_im1 = (i - 1 >= 1) ? dis_vis(id - si) : C(pds[id - si]);
-    const C tk_jm1 = (j - 1 >= 1) ? tke_vis(id - sj, j - 1, k) : C(ptk[id - sj]);
-    const C ds_jm1 = (j - 1 >= 1) ? dis_vis(id - sj) : C(pds[id - sj]);
-    const C tk_km1 = (k - 1 >= 1) ? tke_vis(id - 1, j, k - 1) : C(ptk[id - 1]);
-    const C ds_km1 = (k - 1 >= 1) ? dis_vis(id - 1) : C(pds[id - 1]);
+    const C r_tk = C(ptk[id]), r_ds = C(pds[id]);
+    const C r_tk_im1 = C(ptk[id - si]), r_ds_im1 = C(pds[id - si]);
+    const C r_tk_jm1 = C(ptk[id - sj]), r_ds_jm1 = C(pds[id - sj]);
+    const C r_tk_km1 = C(ptk[id - 1]), r_ds_km1 = C(pds[id - 1]);
     const C tk_ip1 = C(ptk[id + si]), ds_ip1 = C(pds[id + si]);
+    const C r_tk_jp1 = C(ptk[id + sj]), r_ds_jp1 = C(pds[id + sj]);
+    const C tk_kp1 = C(ptk[id + 1]), ds_kp1 = C(pds[id + 1]);
+    const C f_abl = C(M.nue_f[i]);
+    // visited neighbours: floors of the visit (max(0,.) then max(k_bg,.); dis max(dis_min,.))
+    auto tke_vis = [&](C raw, double vs) { return tmax(kbg_of(vs), tmax(C(0), raw)); };
+    auto dis_vis = [&](C raw) { return tmax(dis_min, raw); };
+    const C tk = tmax(C(0), r_tk);
+    const C ds = tmax(dis_min, r_ds);
+    const C tk_im1 = (i - 1 >= 1) ? tke_vis(r_tk_im1, vs_c) : r_tk_im1;
+    const C ds_im1 = (i - 1 >= 1) ? dis_vis(r_ds_im1) : r_ds_im1;
+    const C tk_jm1 = (j - 1 >= 1) ? tke_vis(r_tk_jm1, vs_jm) : r_tk_jm1;
+    const C ds_jm1 = (j - 1 >= 1) ? dis_vis(r_ds_jm1) : r_ds_jm1;
+    const C tk_km1 = (k - 1 >= 1) ? tke_vis(r_tk_km1, vs_km) : r_tk_km1;
+    const C ds_km1 = (k - 1 >= 1) ? dis_vis(r_ds_km1) : r_ds_km1;
     // the symmetric form floors both latitude neighbours (a serial sweep's visit order is
     // not mirror symmetric)
     const bool fl_jp = K.sym && (j + 1 <= K.jm - 2);
-    const C tk_jp1 = fl_jp ? tke_vis(id + sj, j + 1, k) : C(ptk[id + sj]);
-    const C ds_jp1 = fl_jp ? dis_vis(id + sj) : C(pds[id + sj]);
-    const C tk_kp1 = C(ptk[id + 1]), ds_kp1 = C(pds[id + 1]);
+    const C tk_jp1 = fl_jp ? tke_vis(r_tk_jp1, vs_jp) : r_tk_jp1;
+    const C ds_jp1 = fl_jp ? dis_vis(r_ds_jp1) : r_ds_jp1;
 
     const double rm = M.rad[i];
     const double exp_rm_d = 1.0 / (rm + 1.0);
@@ -3179,7 +3199,7 @@ __global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* __restri
     tsrc = P_k - Y_k;
     dsrc = P_w - Y_w + D_w;
     }
-    const C tk_out = tmax(kbg(j, k), tk);
+    const C tk_out = tmax(kbg_of(vs_c), tk);
     const C ds_out = tmax(dis_min, ds);
     nue = tclamp(nue, C(0), nue_max);
     const C dsm = C(20) * ds_out * ds_out;
@@ -3204,11 +3224,11 @@ __global__ __launch_bounds__(256) void k_turb_sources(const MemberDesc* __restri
 // on memory round trips); the final values equal those of the reference's sequence of
 // assignments.
 template <typename S>
-__global__ void k_bc_radial(const MemberDesc* __restrict__ md) {
+__global__ __launch_bounds__(256) void k_bc_radial(const MemberDesc* __restrict__ md) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
     if (q >= (long long)K.jm * K.km) return;
     const long long si = (long long)K.jm * K.km;
     const int iml = K.im - 1;
@@ -3280,11 +3300,11 @@ constexpr int kThetaF[18] = {F_T, F_P_STAT, F_R_HUMID, F_R_DRY, F_U, F_V, F_W, F
 // bcTheta (BC_Atm.h:746-917): the pole rows copy their neighbours. One thread per
 // (field, level, longitude).
 template <typename S>
-__global__ void k_bc_theta(const MemberDesc* __restrict__ md) {
+__global__ __launch_bounds__(256) void k_bc_theta(const MemberDesc* __restrict__ md) {
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const long long nik = (long long)K.im * K.km;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
     if (q >= 18 * nik) return;
     const int n = (int)(q / nik);
     const long long r = q - n * nik;
@@ -3320,12 +3340,12 @@ template <typename C> ATHAD_DEV void seam_passes(C e, C& a, C& b, C& c, C f) {
 }
 
 template <typename S>
-__global__ void k_bc_phi(const MemberDesc* __restrict__ md) {
+__global__ __launch_bounds__(256) void k_bc_phi(const MemberDesc* __restrict__ md) {
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const long long nrow = (long long)K.im * K.jm;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
     if (q >= 21 * nrow) return;
     const int n = (int)(q / nrow);
     const long long r = (q - n * nrow) * K.km;
@@ -3451,7 +3471,7 @@ __global__ __launch_bounds__(256) void k_polar_wave(const MemberDesc* __restrict
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const int lane = threadIdx.x & 63;
-    const long long pair = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long long pair = (long long)blockIdx.x * (256 >> 6) + (threadIdx.x >> 6);
     if (pair >= 3LL * K.im * K.jm) return;   // uniform per wave
     const int row = (int)(pair / 3), f = (int)(pair % 3);
     const int np = M.passes[row % K.jm];
@@ -3981,8 +4001,8 @@ void launch_level_mean(const Batch& b) {
 }
 namespace {
 template <typename S, int SRC> void rk4_jacobi_t(const Batch& b, const dim3& g, bool fused) {
-    if (fused) hipLaunchKernelGGL((k_rk4_jacobi<S, true, SRC>), g, dim3(256), 0, b.stream, b.d_md);
-    else hipLaunchKernelGGL((k_rk4_jacobi<S, false, SRC>), g, dim3(256), 0, b.stream, b.d_md);
+    if (fused) hipLaunchKernelGGL((k_rk4_jacobi<S, true, SRC>), g, dim3(kCellBlock), 0, b.stream, b.d_md);
+    else hipLaunchKernelGGL((k_rk4_jacobi<S, false, SRC>), g, dim3(kCellBlock), 0, b.stream, b.d_md);
 }
 template <typename S> void rk4_jacobi_s(const Batch& b, const dim3& g, bool fused, int src) {
     switch (src & 3) {
@@ -4045,13 +4065,13 @@ void launch_rk4_lex(const Batch& b, int plane, int src) {
 void launch_poisson(const Batch& b, int gate) {
     const unsigned na = (unsigned)(((long long)(b.max_jm - 2) * (b.max_km - 2) + 255) / 256);
     ATHAD_DISPATCH(k_aux_bc, dim3(na, 1, b.nmem), dim3(256), 0, b.d_md, gate);
-    ATHAD_DISPATCH(k_poisson_colour, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, 0, gate);
-    ATHAD_DISPATCH(k_poisson_colour, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, 1, gate);
-    ATHAD_DISPATCH(k_p_bc, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate);
-    ATHAD_DISPATCH(k_p_ceiling, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate);
+    ATHAD_DISPATCH(k_poisson_colour, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md, 0, gate);
+    ATHAD_DISPATCH(k_poisson_colour, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md, 1, gate);
+    ATHAD_DISPATCH(k_p_bc, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md, gate);
+    ATHAD_DISPATCH(k_p_ceiling, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md, gate);
 }
 void launch_satadj(const Batch& b, int gate, int to_tmp) {
-    ATHAD_DISPATCH(k_satadj, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, gate, to_tmp);
+    ATHAD_DISPATCH(k_satadj, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md, gate, to_tmp);
 }
 void launch_damp(const Batch& b, int axis, const int* src, const int* dst, int nf, int moist_clamp, int gate) {
     DampJob job{};
@@ -4060,13 +4080,13 @@ void launch_damp(const Batch& b, int axis, const int* src, const int* dst, int n
     job.moist_clamp = moist_clamp;
     const dim3 g(cells_grid(b, 256), 1, b.nmem);
     if (b.fp32) {
-        if (axis == 0) hipLaunchKernelGGL((k_damp_pass<float, 0>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
-        else if (axis == 1) hipLaunchKernelGGL((k_damp_pass<float, 1>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
-        else hipLaunchKernelGGL((k_damp_pass<float, 2>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
+        if (axis == 0) hipLaunchKernelGGL((k_damp_pass<float, 0>), g, dim3(kCellBlock), 0, b.stream, b.d_md, job, gate);
+        else if (axis == 1) hipLaunchKernelGGL((k_damp_pass<float, 1>), g, dim3(kCellBlock), 0, b.stream, b.d_md, job, gate);
+        else hipLaunchKernelGGL((k_damp_pass<float, 2>), g, dim3(kCellBlock), 0, b.stream, b.d_md, job, gate);
     } else {
-        if (axis == 0) hipLaunchKernelGGL((k_damp_pass<double, 0>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
-        else if (axis == 1) hipLaunchKernelGGL((k_damp_pass<double, 1>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
-        else hipLaunchKernelGGL((k_damp_pass<double, 2>), g, dim3(256), 0, b.stream, b.d_md, job, gate);
+        if (axis == 0) hipLaunchKernelGGL((k_damp_pass<double, 0>), g, dim3(kCellBlock), 0, b.stream, b.d_md, job, gate);
+        else if (axis == 1) hipLaunchKernelGGL((k_damp_pass<double, 1>), g, dim3(kCellBlock), 0, b.stream, b.d_md, job, gate);
+        else hipLaunchKernelGGL((k_damp_pass<double, 2>), g, dim3(kCellBlock), 0, b.stream, b.d_md, job, gate);
     }
 }
 namespace {
@@ -4120,7 +4140,7 @@ void launch_copy_job(const Batch& b, const int* src, const int* dst, int nf, int
     DampJob job{};
     job.nf = nf;
     for (int q = 0; q < nf; q++) { job.src[q] = src[q]; job.dst[q] = dst[q]; }
-    ATHAD_DISPATCH(k_copy_job, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, job, gate);
+    ATHAD_DISPATCH(k_copy_job, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md, job, gate);
 }
 void launch_densities(const Batch& b, int set_water, int evaporate, int gate) {
     ATHAD_DISPATCH(k_densities, dim3(cols_grid(b, 128), 1, b.nmem), dim3(128), 0, b.d_md, set_water, evaporate, gate);
@@ -4130,8 +4150,8 @@ void launch_vel_star(const Batch& b, int gate) {
 }
 template <int MODEL> void launch_turb_sources_m(const Batch& b, int gate) {
     const dim3 grid(cells_grid(b, 256), 1, b.nmem);
-    if (b.fp32) hipLaunchKernelGGL((k_turb_sources<float, MODEL>), grid, dim3(256), 0, b.stream, b.d_md, gate);
-    else hipLaunchKernelGGL((k_turb_sources<double, MODEL>), grid, dim3(256), 0, b.stream, b.d_md, gate);
+    if (b.fp32) hipLaunchKernelGGL((k_turb_sources<float, MODEL>), grid, dim3(kCellBlock), 0, b.stream, b.d_md, gate);
+    else hipLaunchKernelGGL((k_turb_sources<double, MODEL>), grid, dim3(kCellBlock), 0, b.stream, b.d_md, gate);
 }
 void launch_turb_sources(const Batch& b, int gate) {
     if (b.turb == kTurbKEpsilon) launch_turb_sources_m<kTurbKEpsilon>(b, gate);
@@ -4146,7 +4166,7 @@ void launch_bc(const Batch& b) {
     ATHAD_DISPATCH(k_bc_phi, dim3(np, 1, b.nmem), dim3(256), 0, b.d_md);
 }
 void launch_solid_clamp(const Batch& b) {
-    ATHAD_DISPATCH(k_solid_clamp, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
+    ATHAD_DISPATCH(k_solid_clamp, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md);
 }
 namespace {
 // Rows of up to 768 longitudes run the register kernel (one wave per row and field), longer
@@ -4198,13 +4218,13 @@ void launch_radial(const Batch& b, int store) {
     ATHAD_DISPATCH(k_radial_store, dim3(cols_grid(b, 64), 1, b.nmem), dim3(64), shm, b.d_md, store);
 }
 void launch_store(const Batch& b, int mask) {
-    ATHAD_DISPATCH(k_store, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, mask);
+    ATHAD_DISPATCH(k_store, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md, mask);
 }
 void launch_project(const Batch& b, int phase) {
-    ATHAD_DISPATCH(k_project, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, phase);
+    ATHAD_DISPATCH(k_project, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md, phase);
 }
 void launch_copy_field(const Batch& b, int dst, int src) {
-    ATHAD_DISPATCH(k_copy_field, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md, dst, src);
+    ATHAD_DISPATCH(k_copy_field, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md, dst, src);
 }
 void launch_init_column(const Batch& b) {
     ATHAD_DISPATCH(k_init_column, dim3(cols_grid(b, 128), 1, b.nmem), dim3(128), 0, b.d_md);
@@ -4213,16 +4233,16 @@ void launch_init_water(const Batch& b) {
     ATHAD_DISPATCH(k_init_water, dim3(cols_grid(b, 128), 1, b.nmem), dim3(128), 0, b.d_md);
 }
 void launch_init_co2(const Batch& b) {
-    ATHAD_DISPATCH(k_init_co2, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
+    ATHAD_DISPATCH(k_init_co2, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md);
 }
 void launch_copy_lid(const Batch& b) {
     ATHAD_DISPATCH(k_copy_lid, dim3(cols_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
 }
 void launch_init_rest(const Batch& b) {
-    ATHAD_DISPATCH(k_init_rest, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
+    ATHAD_DISPATCH(k_init_rest, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md);
 }
 void launch_turb_init_fields(const Batch& b) {
-    ATHAD_DISPATCH(k_turb_init_fields, dim3(cells_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
+    ATHAD_DISPATCH(k_turb_init_fields, dim3(cells_grid(b, kCellBlock), 1, b.nmem), dim3(kCellBlock), 0, b.d_md);
 }
 void launch_wall_bc(const Batch& b) {
     ATHAD_DISPATCH(k_wall_bc, dim3(cols_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
