@@ -1887,10 +1887,19 @@ __global__ __launch_bounds__(256) void k_aux_bc(const MemberDesc* __restrict__ m
     const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
     if (q >= (long long)(K.jm - 2) * (K.km - 2)) return;
     const int j = 1 + (int)(q / (K.km - 2)), k = 1 + (int)(q % (K.km - 2));
-    for (int f = F_AUX_U; f <= F_AUX_W; f++) {
-        S* a = fp<S>(M, f);
-        a[ix3(K, 0, j, k)] = a[ix3(K, 1, j, k)];
-        a[ix3(K, K.im - 1, j, k)] = a[ix3(K, K.im - 2, j, k)];
+    // all six values read before any is written (one memory round trip, not one per copy)
+    S lo[3], hi[3];
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        const S* a = fp<S>(M, F_AUX_U + f);
+        lo[f] = a[ix3(K, 1, j, k)];
+        hi[f] = a[ix3(K, K.im - 2, j, k)];
+    }
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        S* a = fp<S>(M, F_AUX_U + f);
+        a[ix3(K, 0, j, k)] = lo[f];
+        a[ix3(K, K.im - 1, j, k)] = hi[f];
     }
 }
 
@@ -3303,11 +3312,14 @@ template <typename S>
 __global__ __launch_bounds__(256) void k_bc_theta(const MemberDesc* __restrict__ md) {
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
-    const long long nik = (long long)K.im * K.km;
+    // each field's (i, k) range padded to whole waves: the field index is then wave-uniform,
+    // so the field table entry and the field pointer are scalar loads
+    const long long nik = (long long)K.im * K.km, nikp = (nik + 63) & ~63LL;
     const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (q >= 18 * nik) return;
-    const int n = (int)(q / nik);
-    const long long r = q - n * nik;
+    if (q >= 18 * nikp) return;
+    const int n = (int)__builtin_amdgcn_readfirstlane((int)(q / nikp));
+    const long long r = q - n * nikp;
+    if (r >= nik) return;
     const int i = (int)(r / K.km), k = (int)(r - (long long)i * K.km);
     const long long a0 = ix3(K, i, 0, k), a1 = ix3(K, i, 1, k), b0 = ix3(K, i, K.jm - 1, k), b1 = ix3(K, i, K.jm - 2, k);
     if (kThetaF[n] == F_P_STAT) {
@@ -3344,11 +3356,13 @@ __global__ __launch_bounds__(256) void k_bc_phi(const MemberDesc* __restrict__ m
     using C = S;
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
-    const long long nrow = (long long)K.im * K.jm;
+    // each field's row range padded to whole waves, so the field index is wave-uniform
+    const long long nrow = (long long)K.im * K.jm, nrowp = (nrow + 63) & ~63LL;
     const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (q >= 21 * nrow) return;
-    const int n = (int)(q / nrow);
-    const long long r = (q - n * nrow) * K.km;
+    if (q >= 21 * nrowp) return;
+    const int n = (int)__builtin_amdgcn_readfirstlane((int)(q / nrowp));
+    if (q - n * nrowp >= nrow) return;
+    const long long r = (q - n * nrowp) * K.km;
     const int km = K.km;
     const int f = kPhiF[n];
     if (f == F_P_STAT) {
@@ -3426,7 +3440,7 @@ __global__ __launch_bounds__(256) void k_polar(const MemberDesc* __restrict__ md
     const int row = blockIdx.x;
     if (row >= K.im * K.jm) return;
     const int j = row % K.jm;
-    const int np = M.passes[j];
+    const int np = ro_load(M.passes + j);
     if (np == 0) return;
     // the selected fields of u, v, w pass together: 2 buffers x 3 fields x km in LDS
     extern __shared__ unsigned char smem_raw[];
@@ -3471,19 +3485,23 @@ __global__ __launch_bounds__(256) void k_polar_wave(const MemberDesc* __restrict
     const MemberDesc& M = md[blockIdx.z];
     const Consts& K = M.K;
     const int lane = threadIdx.x & 63;
-    const long long pair = (long long)blockIdx.x * (256 >> 6) + (threadIdx.x >> 6);
+    // the wave's index in its block, made wave-uniform for the compiler: the row, the field and
+    // its pointer then sit in scalar registers and the pass count is a scalar load
+    const int wv = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long long pair = (long long)blockIdx.x * (256 >> 6) + wv;
     if (pair >= 3LL * K.im * K.jm) return;   // uniform per wave
     const int row = (int)(pair / 3), f = (int)(pair % 3);
-    const int np = M.passes[row % K.jm];
+    const int np = ro_load(M.passes + row % K.jm);
     if (np == 0) return;
     const int km = K.km, last = km - 1, l_last = last / E, s_last = last % E;
     S* x = fp<S>(M, F_U + f) + (size_t)row * km;
+    // the row is read and written through a descriptor sized to it: the buffer range check
+    // gives zero for the lanes' slots past longitude km - 1 and drops their stores, so no
+    // element needs a branch (a branch per store made each store wait for the previous one)
+    const Rsrc xr = make_rsrc(x, (unsigned)km * (unsigned)sizeof(S));
     C v[E];
 #pragma unroll
-    for (int s = 0; s < E; s++) {
-        const int e = E * lane + s;
-        v[s] = (e < km) ? C(gld(x, (long long)e)) : C(0);
-    }
+    for (int s = 0; s < E; s++) v[s] = C(bld<S>(xr, (unsigned)(E * lane + s) * (unsigned)sizeof(S), 0));
     for (int p = 0; p < np; p++) {
         const C up = dpp_move<kDppWaveShr1>(v[E - 1], C(0));   // lane l-1's last longitude
         const C dn = dpp_move<kDppWaveShl1>(v[0], C(0));       // lane l+1's first longitude
@@ -3505,10 +3523,7 @@ __global__ __launch_bounds__(256) void k_polar_wave(const MemberDesc* __restrict
         for (int s = 0; s < E; s++) v[s] = nv[s];
     }
 #pragma unroll
-    for (int s = 0; s < E; s++) {
-        const int e = E * lane + s;
-        if (e < km) gst(x, (long long)e, S(v[s]));
-    }
+    for (int s = 0; s < E; s++) bst<S>(xr, (unsigned)(E * lane + s) * (unsigned)sizeof(S), S(v[s]));
 }
 
 // Radial Shapiro filters (lib/Utils.cpp:831-933): two passes along i of u (1-2-1) and of
@@ -4160,9 +4175,12 @@ void launch_turb_sources(const Batch& b, int gate) {
 }
 void launch_bc(const Batch& b) {
     ATHAD_DISPATCH(k_bc_radial, dim3(cols_grid(b, 256), 1, b.nmem), dim3(256), 0, b.d_md);
-    const unsigned nt = (unsigned)((18LL * b.max_im * b.max_km + 255) / 256);
+    // each field's range padded to a multiple of 64 threads, as the kernels index it
+    const long long nikp = ((long long)b.max_im * b.max_km + 63) & ~63LL;
+    const unsigned nt = (unsigned)((18LL * nikp + 255) / 256);
     ATHAD_DISPATCH(k_bc_theta, dim3(nt, 1, b.nmem), dim3(256), 0, b.d_md);
-    const unsigned np = (unsigned)((21LL * b.max_im * b.max_jm + 255) / 256);
+    const long long nrowp = ((long long)b.max_im * b.max_jm + 63) & ~63LL;
+    const unsigned np = (unsigned)((21LL * nrowp + 255) / 256);
     ATHAD_DISPATCH(k_bc_phi, dim3(np, 1, b.nmem), dim3(256), 0, b.d_md);
 }
 void launch_solid_clamp(const Batch& b) {
