@@ -1448,9 +1448,11 @@ for_stages<U>([&](const int st) {
 // field's tendency involves only itself and the velocities, so the group can run whole or in
 // parts. cen[0..NQ-1] = start values in, final values out; ba holds NQ bases, then NQ
 // accumulators.
+// rk4_moist_prep reads the group's inputs and forms its stage-invariant terms, and returns the
+// stage loop (taking cen[]), so that a caller can read the inputs of two groups before running
+// either loop: their reads then go out as one batch.
 template <typename S, typename C, bool CLAMP, int SRC, bool U, int SB, int Q0, int NQ, typename Nb, typename Vel>
-ATHAD_DEV void rk4_moist(const MemberDesc& M, const Nb& src_nb, const CellGeo<C>& g, C* ba,
-                         Vel vel, C* cen) {
+ATHAD_DEV auto rk4_moist_prep(const MemberDesc& M, const Nb& src_nb, const CellGeo<C>& g, C* ba, Vel vel) {
     const Consts& K = M.K;
     const C i2r = C(K.inv_2dr), i2t = C(K.inv_2dthe), i2p = C(K.inv_2dphi);
     constexpr int NM = (Q0 < 3) ? ((Q0 + NQ < 3 ? Q0 + NQ : 3) - Q0) : 0;   // minmod fields
@@ -1483,6 +1485,9 @@ ATHAD_DEV void rk4_moist(const MemberDesc& M, const Nb& src_nb, const CellGeo<C>
     // moist convection's vapour tendency (RHS_Atm_Turb.cpp:1061), stage-invariant
     const C mcQ = ((SRC & SRC_CONV) && Q0 == 0) ? C(K.coeff_mc_q * src_nb.mc(MC_MC_Q)) : C(0);
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt_sixth);
+    const CellGeo<C> gg = g;
+    return [=](C* cen) {
+    const CellGeo<C>& g = gg;
 for_stages<U>([&](const int st) {
         C u, v, w;
         vel(st, u, v, w);
@@ -1506,6 +1511,13 @@ for_stages<U>([&](const int st) {
             cen[p] = (Q0 + p < 4) ? tmax(C(0), val) : val;
         }
     });
+    };
+}
+
+template <typename S, typename C, bool CLAMP, int SRC, bool U, int SB, int Q0, int NQ, typename Nb, typename Vel>
+ATHAD_DEV void rk4_moist(const MemberDesc& M, const Nb& src_nb, const CellGeo<C>& g, C* ba,
+                         Vel vel, C* cen) {
+    rk4_moist_prep<S, C, CLAMP, SRC, U, SB, Q0, NQ>(M, src_nb, g, ba, vel)(cen);
 }
 
 // RK4 of the turbulence group tke, dis, advected by the recorded stage velocities, with the
@@ -1513,9 +1525,11 @@ for_stages<U>([&](const int st) {
 // (:601-631); the production refresh (:528-561) takes nu_t = C_mu k^2 / eps in the k-epsilon
 // closure, k / omega otherwise. cen[0..1] = start values in, final values out; prod and the
 // sources are those of the fourth stage.
+// rk4_turb_prep: the reads and stage-invariant terms; returns the stage loop (taking cen[],
+// prod, tsrc, dsrc), as rk4_moist_prep does.
 template <typename S, typename C, bool CLAMP, bool U, int SB, typename Nb, typename Vel>
-ATHAD_DEV void rk4_turb(const MemberDesc& M, const Nb& src_nb, int i, int j, int k,
-                        const CellGeo<C>& g, C* ba, Vel vel, C A, C B, C* cen, C& prod, C& tsrc, C& dsrc) {
+ATHAD_DEV auto rk4_turb_prep(const MemberDesc& M, const Nb& src_nb, int i, int j, int k,
+                             const CellGeo<C>& g, C* ba, Vel vel, C A, C B) {
     const Consts& K = M.K;
     const C tkm = C(K.tke_max_nd);
     const C i2r = C(K.inv_2dr), i2t = C(K.inv_2dthe), i2p = C(K.inv_2dphi);
@@ -1557,6 +1571,8 @@ ATHAD_DEV void rk4_turb(const MemberDesc& M, const Nb& src_nb, int i, int j, int
     const C dif_tke_2 = eps ? inv_re + nue / C(1.0) : inv_re + C(0.6) * nue;
     const C dif_dis_2 = eps ? inv_re + nue / C(1.3) : inv_re + C(0.5) * nue;
     const C sig_cross = (eps || cross <= C(0)) ? C(0) : C(0.125) * cross;
+    const C Lc2 = g.Lc2, nue_max_c = C(K.nue_max);
+    return [=](C* cen, C& prod, C& tsrc, C& dsrc) {
 for_stages<U>([&](const int st) {
         C u, v, w;
         vel(st, u, v, w);
@@ -1569,7 +1585,7 @@ for_stages<U>([&](const int st) {
         const C tke_s = tmax(tke, C(1.0e-10));
         C dif_tke, dif_dis;
         if (model == kTurbKOmegaSST) {
-            const C cnue = tmin(tmax(tke, C(0)) * inv_dis, C(K.nue_max));
+            const C cnue = tmin(tmax(tke, C(0)) * inv_dis, nue_max_c);
             prod = tmax(C(0), cnue * A - C(0.66667) * tke * B);
             const C CD_kw = tmax(C(2) * sig_w2 * inv_dis * cross, C(1.0e-20));
             const C arg1 = tmin(tmax(C(sqrt(tke)) * inv_dis * K2, K1 * inv_dis), tke * K3 * rcp_nr(CD_kw));
@@ -1587,7 +1603,7 @@ for_stages<U>([&](const int st) {
             dsrc = tclamp(P_w - Y_w + D_w, -dsm, dsm);
         } else {
             const C tke_h = tmax(tke, C(0));
-            const C cnue = tmin(eps ? C(0.028) * tke_h * tke_h * inv_dis : tke_h * inv_dis, C(K.nue_max));
+            const C cnue = tmin(eps ? C(0.028) * tke_h * tke_h * inv_dis : tke_h * inv_dis, nue_max_c);
             prod = tmax(C(0), cnue * A - C(0.66667) * tke * B);
             dif_tke = dif_tke_2;
             dif_dis = dif_dis_2;
@@ -1608,14 +1624,21 @@ for_stages<U>([&](const int st) {
             tsrc = tclamp(P_k - Y_k, -tsm, tsm);
             dsrc = tclamp(P_w - Y_w + sig_cross * inv_dis, -dsm, dsm);
         }
-        const C rk = -(u * gc[0][0] + v * gc[0][1] + w * gc[0][2]) + (L0[0] - g.Lc2 * tke) * dif_tke + tsrc;
-        const C rd = -(u * gc[1][0] + v * gc[1][1] + w * gc[1][2]) + (L0[1] - g.Lc2 * dis) * dif_dis + dsrc;
+        const C rk = -(u * gc[0][0] + v * gc[0][1] + w * gc[0][2]) + (L0[0] - Lc2 * tke) * dif_tke + tsrc;
+        const C rd = -(u * gc[1][0] + v * gc[1][1] + w * gc[1][2]) + (L0[1] - Lc2 * dis) * dif_dis + dsrc;
         C vk = stage_update(st, rk, ba + 0 * SB, ba + 2 * SB, dt, half_dt, dt_sixth);
         vk = finite_bits(vk) ? ((vk < C(0)) ? C(0) : (vk > tkm) ? tkm : vk) : C(0);
         const C vd = stage_update(st, rd, ba + 1 * SB, ba + 3 * SB, dt, half_dt, dt_sixth);
         cen[0] = vk;
         cen[1] = tmax(C(1.0e-10), vd);
     });
+    };
+}
+
+template <typename S, typename C, bool CLAMP, bool U, int SB, typename Nb, typename Vel>
+ATHAD_DEV void rk4_turb(const MemberDesc& M, const Nb& src_nb, int i, int j, int k,
+                        const CellGeo<C>& g, C* ba, Vel vel, C A, C B, C* cen, C& prod, C& tsrc, C& dsrc) {
+    rk4_turb_prep<S, C, CLAMP, U, SB>(M, src_nb, i, j, k, g, ba, vel, A, B)(cen, prod, tsrc, dsrc);
 }
 
 // Waves per SIMD the RK4 kernels are compiled for (register budget 512 / waves).
@@ -1668,7 +1691,10 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const Nb& nb, int i, int j, int k, 
     auto vel = [&](int st, C& u, C& v, C& w) {
         u = rec[(3 * st + 0) * SB]; v = rec[(3 * st + 1) * SB]; w = rec[(3 * st + 2) * SB];
     };
-    if constexpr (SPLIT) {   // c, cloud, ice, then gr, co2: six LDS slots instead of ten
+    if constexpr (SPLIT) {
+        // c, cloud, ice (six LDS slots), then gr, co2 (slots 0-3) together with tke, dis
+        // (slots 4-7): the inputs of both of the last two groups are read before either stage
+        // loop runs, one batch of reads instead of two round trips
         {
             C cen[3];
             group_start<SB>(nb, FC, 3, cen, ba);
@@ -1676,26 +1702,31 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const Nb& nb, int i, int j, int k, 
 #pragma unroll
             for (int q = 0; q < 3; q++) out(FC + q, cen[q]);
         }
-        {
-            C cen[2];
-            group_start<SB>(nb, FC + 3, 2, cen, ba);
-            rk4_moist<S, C, CLAMP, SRC, U, SB, 3, 2>(M, nb, g, ba, vel, cen);
+        C cen_g[2], cen_t[2], prod, ts, ds;
+        group_start<SB>(nb, FC + 3, 2, cen_g, ba);
+        group_start<SB>(nb, FK, 2, cen_t, ba + 4 * SB);
+        auto run_g = rk4_moist_prep<S, C, CLAMP, SRC, U, SB, 3, 2>(M, nb, g, ba, vel);
+        auto run_t = rk4_turb_prep<S, C, CLAMP, U, SB>(M, nb, i, j, k, g, ba + 4 * SB, vel, A, B);
+        run_g(cen_g);
 #pragma unroll
-            for (int q = 0; q < 2; q++) out(FC + 3 + q, cen[q]);
-        }
+        for (int q = 0; q < 2; q++) out(FC + 3 + q, cen_g[q]);
+        run_t(cen_t, prod, ts, ds);
+        out(FK, cen_t[0]);
+        out(FD, cen_t[1]);
+        outx(F_PROD, prod);
+        outx(F_TKE_SRC, ts);
+        outx(F_DIS_SRC, ds);
     } else {
         C cen[5];
         group_start<SB>(nb, FC, 5, cen, ba);
         rk4_moist<S, C, CLAMP, SRC, U, SB, 0, 5>(M, nb, g, ba, vel, cen);
 #pragma unroll
         for (int q = 0; q < 5; q++) out(FC + q, cen[q]);
-    }
-    {
-        C cen[2], prod, ts, ds;
-        group_start<SB>(nb, FK, 2, cen, ba);
-        rk4_turb<S, C, CLAMP, U, SB>(M, nb, i, j, k, g, ba, vel, A, B, cen, prod, ts, ds);
-        out(FK, cen[0]);
-        out(FD, cen[1]);
+        C cen_t[2], prod, ts, ds;
+        group_start<SB>(nb, FK, 2, cen_t, ba);
+        rk4_turb<S, C, CLAMP, U, SB>(M, nb, i, j, k, g, ba, vel, A, B, cen_t, prod, ts, ds);
+        out(FK, cen_t[0]);
+        out(FD, cen_t[1]);
         outx(F_PROD, prod);
         outx(F_TKE_SRC, ts);
         outx(F_DIS_SRC, ds);
