@@ -1351,9 +1351,16 @@ template <bool U, typename F> ATHAD_DEV void for_stages(F&& f) {
 // tendencies involve no other prognostic field, so the group integrates first and records
 // the velocity entering each stage in rec[] for the scalar groups. cen[] = start values in,
 // final values out. A, B: velocity-gradient invariants of the production term.
-template <typename S, typename C, bool CLAMP, int SRC, bool U, int SB, typename Nb>
+// A per-stage hook of the stage loops (called at the start of stage st, after the dynamics
+// group has recorded the stage's velocity): nothing, except where the groups of one cell run
+// on different waves and meet at a barrier per stage.
+struct NoHook {
+    ATHAD_DEV void operator()(int) const {}
+};
+
+template <typename S, typename C, bool CLAMP, int SRC, bool U, int SB, typename Nb, typename Hook = NoHook>
 ATHAD_DEV void rk4_dyn(const MemberDesc& M, const Nb& src_nb, int i, const CellGeo<C>& g,
-                       C* ba, C* rec, C* cen, C* aux, C& A, C& B) {
+                       C* ba, C* rec, C* cen, C* aux, C& A, C& B, Hook hook = Hook{}) {
     const Consts& K = M.K;
     const C i2r = C(K.inv_2dr), i2t = C(K.inv_2dthe), i2p = C(K.inv_2dphi);
     C nb[4][6], g1r[4], g1t[4], g1p[4], Sr[4], St[4], Sp[4];
@@ -1424,6 +1431,7 @@ ATHAD_DEV void rk4_dyn(const MemberDesc& M, const Nb& src_nb, int i, const CellG
 for_stages<U>([&](const int st) {
         const C T = cen[FT], u = cen[FU], v = cen[FV], w = cen[FW];
         rec[(3 * st + 0) * SB] = u; rec[(3 * st + 1) * SB] = v; rec[(3 * st + 2) * SB] = w;
+        hook(st);
         const C ua = u * g.aR, va = v * g.aT, wa = w * g.aP;
         C rT = ce * (u * PR + v * PT + w * PP) - minmod3(nb[FT], T, ua, va, wa) + (L0T - g.Lc2 * T) * dif_t;
         if (SRC & SRC_CONV) rT = rT + mcT;
@@ -1449,8 +1457,8 @@ for_stages<U>([&](const int st) {
 // parts. cen[0..NQ-1] = start values in, final values out; ba holds NQ bases, then NQ
 // accumulators.
 // rk4_moist_prep reads the group's inputs and forms its stage-invariant terms, and returns the
-// stage loop (taking cen[]), so that a caller can read the inputs of two groups before running
-// either loop: their reads then go out as one batch.
+// stage loop (taking cen[] and a per-stage hook), so that a caller can read the inputs of two
+// groups before running either loop: their reads then go out as one batch.
 template <typename S, typename C, bool CLAMP, int SRC, bool U, int SB, int Q0, int NQ, typename Nb, typename Vel>
 ATHAD_DEV auto rk4_moist_prep(const MemberDesc& M, const Nb& src_nb, const CellGeo<C>& g, C* ba, Vel vel) {
     const Consts& K = M.K;
@@ -1486,9 +1494,10 @@ ATHAD_DEV auto rk4_moist_prep(const MemberDesc& M, const Nb& src_nb, const CellG
     const C mcQ = ((SRC & SRC_CONV) && Q0 == 0) ? C(K.coeff_mc_q * src_nb.mc(MC_MC_Q)) : C(0);
     const C dt = C(K.dt), half_dt = C(0.5 * K.dt), dt_sixth = C(K.dt_sixth);
     const CellGeo<C> gg = g;
-    return [=](C* cen) {
+    return [=](C* cen, auto hook) {
     const CellGeo<C>& g = gg;
 for_stages<U>([&](const int st) {
+        hook(st);
         C u, v, w;
         vel(st, u, v, w);
         const C ua = u * g.aR, va = v * g.aT, wa = w * g.aP;
@@ -1517,7 +1526,7 @@ for_stages<U>([&](const int st) {
 template <typename S, typename C, bool CLAMP, int SRC, bool U, int SB, int Q0, int NQ, typename Nb, typename Vel>
 ATHAD_DEV void rk4_moist(const MemberDesc& M, const Nb& src_nb, const CellGeo<C>& g, C* ba,
                          Vel vel, C* cen) {
-    rk4_moist_prep<S, C, CLAMP, SRC, U, SB, Q0, NQ>(M, src_nb, g, ba, vel)(cen);
+    rk4_moist_prep<S, C, CLAMP, SRC, U, SB, Q0, NQ>(M, src_nb, g, ba, vel)(cen, NoHook{});
 }
 
 // RK4 of the turbulence group tke, dis, advected by the recorded stage velocities, with the
@@ -1525,11 +1534,12 @@ ATHAD_DEV void rk4_moist(const MemberDesc& M, const Nb& src_nb, const CellGeo<C>
 // (:601-631); the production refresh (:528-561) takes nu_t = C_mu k^2 / eps in the k-epsilon
 // closure, k / omega otherwise. cen[0..1] = start values in, final values out; prod and the
 // sources are those of the fourth stage.
-// rk4_turb_prep: the reads and stage-invariant terms; returns the stage loop (taking cen[],
-// prod, tsrc, dsrc), as rk4_moist_prep does.
+// rk4_turb_prep: the reads and stage-invariant terms; returns the stage loop (taking cen[], the
+// production invariants A, B of the dynamics group, prod, tsrc, dsrc and a per-stage hook), as
+// rk4_moist_prep does.
 template <typename S, typename C, bool CLAMP, bool U, int SB, typename Nb, typename Vel>
 ATHAD_DEV auto rk4_turb_prep(const MemberDesc& M, const Nb& src_nb, int i, int j, int k,
-                             const CellGeo<C>& g, C* ba, Vel vel, C A, C B) {
+                             const CellGeo<C>& g, C* ba, Vel vel) {
     const Consts& K = M.K;
     const C tkm = C(K.tke_max_nd);
     const C i2r = C(K.inv_2dr), i2t = C(K.inv_2dthe), i2p = C(K.inv_2dphi);
@@ -1572,8 +1582,9 @@ ATHAD_DEV auto rk4_turb_prep(const MemberDesc& M, const Nb& src_nb, int i, int j
     const C dif_dis_2 = eps ? inv_re + nue / C(1.3) : inv_re + C(0.5) * nue;
     const C sig_cross = (eps || cross <= C(0)) ? C(0) : C(0.125) * cross;
     const C Lc2 = g.Lc2, nue_max_c = C(K.nue_max);
-    return [=](C* cen, C& prod, C& tsrc, C& dsrc) {
+    return [=](C* cen, C A, C B, C& prod, C& tsrc, C& dsrc, auto hook) {
 for_stages<U>([&](const int st) {
+        hook(st);
         C u, v, w;
         vel(st, u, v, w);
         const C tke = cen[0], dis = cen[1];
@@ -1638,7 +1649,7 @@ for_stages<U>([&](const int st) {
 template <typename S, typename C, bool CLAMP, bool U, int SB, typename Nb, typename Vel>
 ATHAD_DEV void rk4_turb(const MemberDesc& M, const Nb& src_nb, int i, int j, int k,
                         const CellGeo<C>& g, C* ba, Vel vel, C A, C B, C* cen, C& prod, C& tsrc, C& dsrc) {
-    rk4_turb_prep<S, C, CLAMP, U, SB>(M, src_nb, i, j, k, g, ba, vel, A, B)(cen, prod, tsrc, dsrc);
+    rk4_turb_prep<S, C, CLAMP, U, SB>(M, src_nb, i, j, k, g, ba, vel)(cen, A, B, prod, tsrc, dsrc, NoHook{});
 }
 
 // Waves per SIMD the RK4 kernels are compiled for (register budget 512 / waves).
@@ -1706,11 +1717,11 @@ ATHAD_DEV void rk4_cell(const MemberDesc& M, const Nb& nb, int i, int j, int k, 
         group_start<SB>(nb, FC + 3, 2, cen_g, ba);
         group_start<SB>(nb, FK, 2, cen_t, ba + 4 * SB);
         auto run_g = rk4_moist_prep<S, C, CLAMP, SRC, U, SB, 3, 2>(M, nb, g, ba, vel);
-        auto run_t = rk4_turb_prep<S, C, CLAMP, U, SB>(M, nb, i, j, k, g, ba + 4 * SB, vel, A, B);
-        run_g(cen_g);
+        auto run_t = rk4_turb_prep<S, C, CLAMP, U, SB>(M, nb, i, j, k, g, ba + 4 * SB, vel);
+        run_g(cen_g, NoHook{});
 #pragma unroll
         for (int q = 0; q < 2; q++) out(FC + 3 + q, cen_g[q]);
-        run_t(cen_t, prod, ts, ds);
+        run_t(cen_t, A, B, prod, ts, ds, NoHook{});
         out(FK, cen_t[0]);
         out(FD, cen_t[1]);
         outx(F_PROD, prod);
@@ -1905,6 +1916,78 @@ __global__ __launch_bounds__(LBX) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     rk4_cell<S, C, false, SRC, true, true, false, 1>(M, nb, i, j, k, r_ba, r_rec,
                           [&](int f, C v) { gst(X[f], id, S(v)); },
                           [&](int fid, C v) { gst(fp<S>(M, fid), id, S(v)); });
+}
+
+// Lexicographic-order RK4 on plane d with the groups of each cell on three waves of one block:
+// wave 0 the dynamics group, wave 1 the moisture group, wave 2 the turbulence group, all for the
+// same 64 cells. The scalar groups need from the dynamics group only the velocity entering each
+// stage and the production invariants A, B; the dynamics wave leaves them in LDS before a
+// barrier at the start of every stage, so the turbulence group (half of a cell's arithmetic
+// with the SST closure) runs each stage as soon as its velocity exists and the plane's chain is
+// about that group's instead of the three groups' one after another. The same arithmetic as
+// k_rk4_lex_plane. All three waves hold the same cells, so a lane off the plane is idle in each
+// and a wave with no cell on the plane exits with its two siblings: every wave that runs meets
+// the same four barriers.
+template <typename S, int SRC>
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_rk4_lex_roles(const MemberDesc* __restrict__ md, int d) {
+    using C = S;
+    constexpr int W = 64;
+    __shared__ C s_bd[8 * W], s_rec[12 * W], s_ab[2 * W];
+    const MemberDesc& M = md[blockIdx.z];
+    const Consts& K = M.K;
+    const int role = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = (int)(threadIdx.x & 63);
+    const int nj = K.jm - 2;
+    const long long qd = (long long)blockIdx.x * W + lane;
+    if (qd >= (long long)(K.im - 2) * nj) return;
+    const int i = 1 + (int)(qd / nj), j = 1 + (int)(qd % nj);
+    const int k = d - i - j;
+    if (k < 1 || k > K.km - 2) return;
+    const long long id = (long long)ix3(K, i, j, k);
+    S* X[NPROG];
+#pragma unroll
+    for (int f = 0; f < NPROG; f++) X[f] = fp<S>(M, kProgF[f]);
+    // each wave keeps only the reads of its own group (the compiler drops the others)
+    const NbRegs<S, C, SRC> nb(M, (const S* const*)X, id, (long long)K.jm * K.km, (long long)K.km,
+                               (long long)ix3(K, 0, j, k));
+    CellGeo<C> g;
+    cell_geo<C>(M, i, j, nb.ps(), nb.psurf(), g);
+    auto vel = [&](int st, C& u, C& v, C& w) {
+        u = s_rec[(3 * st + 0) * W + lane]; v = s_rec[(3 * st + 1) * W + lane]; w = s_rec[(3 * st + 2) * W + lane];
+    };
+    if (role == 0) {
+        C cen[4], aux[3], A, B;
+        group_start<W>(nb, FT, 4, cen, s_bd + lane);
+        rk4_dyn<S, C, false, SRC, true, W>(M, nb, i, g, s_bd + lane, s_rec + lane, cen, aux, A, B, [&](int st) {
+            if (st == 0) { s_ab[lane] = A; s_ab[W + lane] = B; }
+            __syncthreads();
+        });
+#pragma unroll
+        for (int q = 0; q < 4; q++) gst(X[FT + q], id, S(cen[q]));
+        gst(fp<S>(M, F_AUX_U), id, S(aux[0]));
+        gst(fp<S>(M, F_AUX_V), id, S(aux[1]));
+        gst(fp<S>(M, F_AUX_W), id, S(aux[2]));
+    } else if (role == 1) {
+        C cen[5], ba[10];
+        group_start<1>(nb, FC, 5, cen, ba);
+        rk4_moist_prep<S, C, false, SRC, true, 1, 0, 5>(M, nb, g, ba, vel)(cen, [&](int) { __syncthreads(); });
+#pragma unroll
+        for (int q = 0; q < 5; q++) gst(X[FC + q], id, S(cen[q]));
+    } else {
+        C cen[2], ba[4], prod, ts, ds;
+        group_start<1>(nb, FK, 2, cen, ba);
+        auto run = rk4_turb_prep<S, C, false, true, 1>(M, nb, i, j, k, g, ba, vel);
+        __syncthreads();   // stage 0: A, B and the start velocities are in LDS
+        const C A = s_ab[lane], B = s_ab[W + lane];
+        run(cen, A, B, prod, ts, ds, [&](int st) {
+            if (st > 0) __syncthreads();
+        });
+        gst(X[FK], id, S(cen[0]));
+        gst(X[FD], id, S(cen[1]));
+        gst(fp<S>(M, F_PROD), id, S(prod));
+        gst(fp<S>(M, F_TKE_SRC), id, S(ts));
+        gst(fp<S>(M, F_DIS_SRC), id, S(ds));
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -4080,11 +4163,24 @@ template <typename S, int LBX> void rk4_lex_t(const Batch& b, int plane, int src
         default: hipLaunchKernelGGL((k_rk4_lex_plane<S, 3, LBX>), g, dim3(LBX), 0, b.stream, b.d_md, plane); break;
     }
 }
+template <typename S> void rk4_lex_roles_t(const Batch& b, int plane, int src) {
+    const long long nq = (long long)(b.max_im - 2) * (b.max_jm - 2);
+    const dim3 g((unsigned)((nq + 63) / 64), 1, b.nmem);
+    switch (src & 3) {
+        case 0: hipLaunchKernelGGL((k_rk4_lex_roles<S, 0>), g, dim3(192), 0, b.stream, b.d_md, plane); break;
+        case 1: hipLaunchKernelGGL((k_rk4_lex_roles<S, 1>), g, dim3(192), 0, b.stream, b.d_md, plane); break;
+        case 2: hipLaunchKernelGGL((k_rk4_lex_roles<S, 2>), g, dim3(192), 0, b.stream, b.d_md, plane); break;
+        default: hipLaunchKernelGGL((k_rk4_lex_roles<S, 3>), g, dim3(192), 0, b.stream, b.d_md, plane); break;
+    }
+}
 template <typename S> void rk4_lex_s(const Batch& b, int plane, int src) {
-    // one 64-lane wave per block while the waves of a plane launch fit the device a few times
-    // over (one member: ~110 waves); measured 256-thread blocks 5 % faster at 32 members
+    // While the waves of a plane fit the device a few times over (one member: ~110 blocks of
+    // 64 cells), each cell's groups run on three waves (k_rk4_lex_roles: 9.12 -> 8.15 ms per
+    // iteration at 41 x 181 x 361 FP64, against one wave per cell); with many members the device
+    // is full anyway and 256-thread blocks of the one-wave kernel run (5 % faster than 64-thread
+    // blocks at 32 members).
     const long long waves = (long long)b.nmem * (((long long)(b.max_im - 2) * (b.max_jm - 2) + 63) / 64);
-    if (waves <= 1024) rk4_lex_t<S, 64>(b, plane, src);
+    if (waves <= 1024) rk4_lex_roles_t<S>(b, plane, src);
     else rk4_lex_t<S, 256>(b, plane, src);
 }
 }  // namespace
