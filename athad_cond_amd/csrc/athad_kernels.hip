@@ -4513,10 +4513,12 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
     ArrP Wu = (ArrP)d.mc[MC_W_U]; ArrP Ed = (ArrP)d.mc[MC_E_D];
     ArrP Dd = (ArrP)d.mc[MC_D_D]; ArrP Ud = (ArrP)d.mc[MC_U_D];
     ArrP Vd = (ArrP)d.mc[MC_V_D]; ArrP Wd = (ArrP)d.mc[MC_W_D];
+    // layer heights by ro_load: a scalar load at the wave-uniform level of a level loop, a
+    // global (not flat) load at a column's own level
     const double* __restrict__ h = d.hgt;
     const double t0 = K.t_0, Mb = K.M_bg, Rb = K.R_bg, lv = K.lv, ls = K.ls;
     auto id = [&](int i) { return (size_t)i * pl + col; };
-    auto step = [&](int i) { return (i + 1 < im) ? h[i + 1] - h[i] : h[im - 1] - h[im - 2]; };   // :306-309
+    auto step = [&](int i) { return (i + 1 < im) ? ro_load(h + (i + 1)) - ro_load(h + (i)) : ro_load(h + (im - 1)) - ro_load(h + (im - 2)); };   // :306-309
     auto qs_env = [&](double E, double p, double qv) {                // :181-184
         return th_q_sat(E, p, th_M_nonwater(qv, K.co2_0, Mb));
     };
@@ -4548,13 +4550,13 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             for (int l = 0; l < 4 && i0 + l < im; l++)
                 if (pcol[l] <= f_beg * p_sfc) { beg = i0 + l; break; }
         }
-        if (beg >= 0) d.mc[MC_DEEP_BEG][id(beg)] = h[beg];
+        if (beg >= 0) d.mc[MC_DEEP_BEG][id(beg)] = ro_load(h + (beg));
         for (int i0 = im - 1; i0 >= 0 && end < 0; i0 -= 4) {
             for (int l = 0; l < 4; l++) pcol[l] = (i0 - l >= 0) ? PS[id(i0 - l)] : 0.0;
             for (int l = 0; l < 4 && i0 - l >= 0; l++)
                 if (pcol[l] >= f_end * p_sfc) { end = i0 - l; break; }
         }
-        if (end >= 0) d.mc[MC_DEEP_END][id(end)] = h[end];
+        if (end >= 0) d.mc[MC_DEEP_END][id(end)] = ro_load(h + (end));
     }
     const G1<double>* __restrict__ QSL = (const G1<double>*)d.conv_qs[0];
     const G1<double>* __restrict__ QSR = (const G1<double>*)d.conv_qs[1];
@@ -4573,7 +4575,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
             Qvu[q] = qv; Qcu[q] = qc;
             if (qv >= scale * qs && w > 0.01 && psv <= f_cb * p_sfc) {
                 base = i;
-                d.mc[MC_CLOUD_BASE][q] = h[i] * w;
+                d.mc[MC_CLOUD_BASE][q] = ro_load(h + (i)) * w;
                 break;
             }
         }
@@ -4596,14 +4598,14 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
                 if (thp - the > 0.0) {
                     buoy = true;
                     top = i;
-                    cape += K.g * (thp - the) / the * (h[i] - h[i - 1]);
+                    cape += K.g * (thp - the) / the * (ro_load(h + (i)) - ro_load(h + (i - 1)));
                 } else if (buoy) {
                     break;
                 }
             }
             lfs = tmin(tmax(top, base + 1), im - 2);
             cape_c = tmax(0.0, cape);
-            d.mc[MC_LFS][id(lfs)] = h[lfs];
+            d.mc[MC_LFS][id(lfs)] = ro_load(h + (lfs));
         }
     }
     // surfaceFluxPerturbations, switch 1 (:402-485); Q_sensible_2D is zero in the reference
@@ -4619,7 +4621,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
         } else if (!(PS[id(base)] - PS[id(lfs)] >= f_diff * p_sfc)) {
             const double Hs = 0.0;
             const double E = tmax(0.0, d.evap[col]) / 86400.0;
-            const double z = h[base];
+            const double z = ro_load(h + (base));
             if (!(E <= 0.0 && Hs <= 0.0) && z > 0.0) {
                 const double B = (K.g / Ts) * (Hs / (rho * cp_l) + 0.61 * Ts * E / rho);
                 const double ws = cbrt(tmax(0.0, B * z));
@@ -4647,7 +4649,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
     auto entrain = [&](int i, const EnvL& v, double mu, bool with_m, bool deep) {
         double e = 0.0;
         if (deep) {
-            const double rth = (K.r_earth * 1.0e3 + h[i]) * d.conv_cos[j];
+            const double rth = (K.r_earth * 1.0e3 + ro_load(h + (i))) * d.conv_cos[j];
             const double Ue = rth * 2.0 * M_PI;
             const double dr = (v.cip - v.cim) * (0.5 / step(i));
             const double dth = (v.cjp - v.cjm) * (0.5 / (Ue / (jm - 1)));
@@ -4749,7 +4751,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
         if (hit < 0 && i + 1 <= lfs && cum + 0.5 * step(i) * (fa + fb) > 0.0) hit = i + 1;
         if (hit >= 0 && hit < im && PS[id(hit)] <= f_cb * p_sfc && (base <= 0 || hit <= base)) {
             base = hit;
-            d.mc[MC_CLOUD_BASE][id(hit)] = h[hit];
+            d.mc[MC_CLOUD_BASE][id(hit)] = ro_load(h + (hit));
         }
     };
     cloud_base_update();
@@ -4757,7 +4759,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
         // updraftEntrainment (:780-908)
         if (inner && lfs > base) {
             const bool shallow = PS[id(base)] - PS[id(lfs)] < f_diff * p_sfc;   // convection_mode 1
-            const double dic = shallow ? 1.0e9 : 500.0, hb = h[base];
+            const double dic = shallow ? 1.0e9 : 500.0, hb = ro_load(h + (base));
             UpL nx = load_up(base + 1);
             for (int i = 0; i < base; i++) Mu[id(i)] = 0.0;
             double mprev = cap((double)RH[id(base)] * seed_w, M_max);
@@ -4786,7 +4788,7 @@ __global__ __launch_bounds__(64) void k_moist_convection(const MemberDesc* __res
                     if (el <= 0.0) el = 0.0;
                 }
                 El[q] = el;
-                const double Kp = (h[i] > hb + dic) ? bet_p : 0.0;
+                const double Kp = (ro_load(h + (i)) > hb + dic) ? bet_p : 0.0;
                 Gp[q] = (t_u >= t0 && ip != 1) ? Kp * L_.qc : 0.0;
             }
             Mu[id(lfs)] = 0.0;
