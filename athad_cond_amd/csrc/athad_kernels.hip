@@ -74,6 +74,8 @@ template <typename T> ATHAD_DEV T ro_load(const T* p) {
     return *(const __attribute__((address_space(4))) T*)p;
 }
 
+template <typename T> using G1 = __attribute__((address_space(1))) T;   // global-memory object
+
 // Cadence gate: a kernel launched with gate g > 0 does nothing for members whose iteration
 // counter is not a multiple of g (Poisson every 4th, moist block every 2nd iteration,
 // cAtmosphereModel.cpp:1156-1185), so members at different phases can share launches.
@@ -564,9 +566,12 @@ __global__ __launch_bounds__(256) void k_radiation(const MemberDesc* __restrict_
     const int im = K.im, jm = K.jm, km = K.km, top = im - 1;
     const int j = blockIdx.y, k0 = blockIdx.x * kRadCols;
     if (j >= jm || k0 >= km) return;
-    const S *ft = fp<S>(M, F_T), *fc = fp<S>(M, F_C), *fco2 = fp<S>(M, F_CO2);
-    const S *fcl = fp<S>(M, F_CLOUD), *fice = fp<S>(M, F_ICE);
-    const double* fpst = (const double*)M.f[F_P_STAT];
+    // global-memory pointers: through plain pointers the reads are flat loads, which the LDS
+    // stores of the staging below would wait for one at a time (both count on lgkmcnt)
+    typedef const G1<S>* __restrict__ FieldP;
+    const FieldP ft = (FieldP)fp<S>(M, F_T), fc = (FieldP)fp<S>(M, F_C), fco2 = (FieldP)fp<S>(M, F_CO2);
+    const FieldP fcl = (FieldP)fp<S>(M, F_CLOUD), fice = (FieldP)fp<S>(M, F_ICE);
+    const G1<double>* __restrict__ fpst = (const G1<double>*)M.f[F_P_STAT];
     const long long si = (long long)jm * km, row = (long long)j * km;
     auto slot = [&](int f, int i, int kk) -> double& { return s_col[(f * im + i) * kRadCols + kk]; };
     // stage t, c, co2, p_stat, cloud, ice of the tile's columns: thread (i mod 16, kk) of the
@@ -577,12 +582,10 @@ __global__ __launch_bounds__(256) void k_radiation(const MemberDesc* __restrict_
     for (int i = i_s; i < im; i += 16) {
         if (!col_in) break;
         const long long g = i * si + row + k0 + kk_s;
-        slot(0, i, kk_s) = (double)ft[g];
-        slot(1, i, kk_s) = (double)fc[g];
-        slot(2, i, kk_s) = (double)fco2[g];
-        slot(3, i, kk_s) = fpst[g];
-        slot(4, i, kk_s) = (double)fcl[g];
-        slot(5, i, kk_s) = (double)fice[g];
+        const double v0 = (double)ft[g], v1 = (double)fc[g], v2 = (double)fco2[g], v3 = fpst[g];
+        const double v4 = (double)fcl[g], v5 = (double)fice[g];
+        slot(0, i, kk_s) = v0; slot(1, i, kk_s) = v1; slot(2, i, kk_s) = v2;
+        slot(3, i, kk_s) = v3; slot(4, i, kk_s) = v4; slot(5, i, kk_s) = v5;
     }
     __syncthreads();
     // incoming short wave of this latitude (MultiLayerRadiation.h:98-104)
@@ -612,7 +615,7 @@ __global__ __launch_bounds__(256) void k_radiation(const MemberDesc* __restrict_
                 const double broad = p * 100.0 / 1.0e5;
                 const double tau_gas = (K.kappa_h2o * qv + K.kappa_co2 * qc + K.kappa_bg * qb) * mass * broad;
                 const int ia = (i < top) ? i : i - 1;
-                const double dz = (double)((float)M.hgt[ia + 1] - (float)M.hgt[ia]);
+                const double dz = (double)((float)ro_load(M.hgt + ia + 1) - (float)ro_load(M.hgt + ia));
                 const double rho = (T[e] > 0.0) ? (p * 100.0) / (th_R_of(qv, qc, K.R_bg) * T[e]) : 0.0;
                 lw[e] = tmax(0.0, slot(4, i, kk)) * rho * dz * 1000.0;
                 iw[e] = tmax(0.0, slot(5, i, kk)) * rho * dz * 1000.0;
@@ -989,7 +992,6 @@ template <typename C, bool BRANCHLESS = false> ATHAD_DEV C tanh_pos(C x) {
 
 // Global-memory access through address space 1 (pointers read from a MemberDesc are
 // otherwise generic and compile to flat instructions).
-template <typename T> using G1 = __attribute__((address_space(1))) T;   // global-memory object
 template <typename T> ATHAD_DEV T gld(const T* p, long long i) {
     return ((const __attribute__((address_space(1))) T*)p)[i];
 }
